@@ -1,0 +1,90 @@
+// Shared device/host helpers for the gfx950 evaluation kernels.
+//
+// Status convention of every extern "C" entry point:
+//   0            success
+//   1 | 2 | 4    input-validation bitmask (only from entry points that validate)
+//   < 0          argument or HIP runtime error; pe_last_error() has the text.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdio>
+#include <cstring>
+
+namespace pe {
+
+using u64 = unsigned long long;
+using i64 = int64_t;
+
+constexpr int kWave = 64;  // CDNA wavefront width
+
+enum Status : int {
+  kOk = 0,
+  kBadArgument = -1,
+  kWorkspaceTooSmall = -2,
+  kUnsupportedShape = -3,
+  kHipError = -4,
+};
+
+// Per-thread message buffer behind pe_last_error().
+char* error_buffer();
+int fail(int code, const char* fmt, ...);
+
+#define PE_HIP(expr)                                                               \
+  do {                                                                             \
+    hipError_t pe_status_ = (expr);                                                \
+    if (pe_status_ != hipSuccess)                                                  \
+      return ::pe::fail(::pe::kHipError, "%s failed: %s", #expr,                   \
+                        hipGetErrorString(pe_status_));                            \
+  } while (0)
+
+#define PE_LAUNCH_CHECK()                                                          \
+  do {                                                                             \
+    hipError_t pe_status_ = hipGetLastError();                                     \
+    if (pe_status_ != hipSuccess)                                                  \
+      return ::pe::fail(::pe::kHipError, "kernel launch failed: %s",               \
+                        hipGetErrorString(pe_status_));                            \
+  } while (0)
+
+#define PE_REQUIRE(cond, ...)                                                      \
+  do {                                                                             \
+    if (!(cond)) return ::pe::fail(::pe::kBadArgument, __VA_ARGS__);              \
+  } while (0)
+
+// Compute units of the current device (cached per device ordinal).
+int compute_units();
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Bump allocator over a caller-provided workspace.  Every slice is 256-byte
+// aligned so vector loads/stores on any slice stay aligned.
+struct Carve {
+  char* base;
+  size_t used = 0;
+  explicit Carve(void* p) : base(static_cast<char*>(p)) {}
+  template <typename T>
+  T* take(size_t count) {
+    used = align_up(used, 256);
+    T* out = reinterpret_cast<T*>(base ? base + used : nullptr);
+    used += count * sizeof(T);
+    return out;
+  }
+};
+
+// ---------------------------------------------------------------- device side
+
+__device__ inline i64 wave_sum(i64 v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+__device__ inline int lane_id() { return __lane_id(); }
+
+// Zero-fill with 16-byte stores (graph-capture safe replacement for memset).
+__global__ void zero_words(uint4* data, i64 count16, uint32_t* tail, i64 tail_words);
+
+int launch_zero(void* ptr, size_t bytes, hipStream_t stream);
+
+}  // namespace pe

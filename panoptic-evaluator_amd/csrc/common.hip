@@ -1,0 +1,73 @@
+// Host helpers shared by every translation unit: error text, device queries
+// and the zero-fill kernel used instead of hipMemsetAsync (so that callers may
+// capture any entry point in a HIP graph).
+#include <cstdarg>
+#include <mutex>
+
+#include "common.h"
+#include "../../include/panoptic_hip.h"
+
+namespace pe {
+
+char* error_buffer() {
+  static thread_local char buffer[512];
+  return buffer;
+}
+
+int fail(int code, const char* fmt, ...) {
+  va_list args;
+  va_start(args, fmt);
+  vsnprintf(error_buffer(), 512, fmt, args);
+  va_end(args);
+  return code;
+}
+
+int compute_units() {
+  static int cached[64] = {0};
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= 64) return 256;
+  if (!cached[device]) {
+    int count = 0;
+    if (hipDeviceGetAttribute(&count, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        count <= 0)
+      count = 256;
+    cached[device] = count;
+  }
+  return cached[device];
+}
+
+__global__ void zero_words(uint4* data, i64 count16, uint32_t* tail, i64 tail_words) {
+  i64 stride = static_cast<i64>(gridDim.x) * blockDim.x;
+  for (i64 i = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; i < count16; i += stride)
+    data[i] = make_uint4(0, 0, 0, 0);
+  if (blockIdx.x == 0)
+    for (i64 i = threadIdx.x; i < tail_words; i += blockDim.x) tail[i] = 0;
+}
+
+int launch_zero(void* ptr, size_t bytes, hipStream_t stream) {
+  if (!bytes) return kOk;
+  // Callers pass 4-byte multiples; the body uses 16-byte stores when aligned.
+  char* p = static_cast<char*>(ptr);
+  size_t lead = (16 - (reinterpret_cast<uintptr_t>(p) & 15)) & 15;
+  if (lead > bytes) lead = bytes;
+  if (lead % 4) return fail(kBadArgument, "zero fill needs 4-byte alignment");
+  if (lead) {
+    zero_words<<<1, 64, 0, stream>>>(nullptr, 0, reinterpret_cast<uint32_t*>(p), lead / 4);
+    p += lead;
+    bytes -= lead;
+  }
+  i64 count16 = bytes / 16;
+  i64 tail_words = (bytes % 16) / 4;
+  int blocks = static_cast<int>(std::min<i64>((count16 + 255) / 256, 4 * compute_units()));
+  if (blocks < 1) blocks = 1;
+  zero_words<<<blocks, 256, 0, stream>>>(reinterpret_cast<uint4*>(p), count16,
+                                         reinterpret_cast<uint32_t*>(p + count16 * 16), tail_words);
+  PE_LAUNCH_CHECK();
+  return kOk;
+}
+
+}  // namespace pe
+
+extern "C" const char* pe_last_error(void) { return pe::error_buffer(); }
+
+extern "C" int pe_abi_version(void) { return PE_ABI_VERSION; }

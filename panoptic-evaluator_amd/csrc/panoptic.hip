@@ -1,0 +1,715 @@
+// Panoptic path on gfx950: joint slot histogram, areas/validation, PQ
+// matching with semantic confusion, and the FP64 finaliser.
+//
+// Behaviour follows the reference's CPU path (evaluator.py:228-243 histogram,
+// :381-417 matching, :448-468 compute, :471-544 per-image PQ) and its native
+// boundary (histogram_kernel.cu:257-341 update_cuda, compute_kernel.cu:9-95).
+//
+// Design (MI355X-first, not a translation of the CUDA kernels):
+//  * histogram: one workgroup owns a <= 65532-pixel chunk of one image and
+//    privatises the G x P histogram in LDS as packed 16-bit counters (two per
+//    32-bit word; a chunk can never carry a count past 0xFFFF), so G x P up to
+//    64K bins fits in 128 KiB of the 160 KiB LDS.  Maps are streamed with
+//    16-byte loads; each lane folds repeated keys in a two-entry register
+//    cache (Voronoi-like maps repeat a key along rows and down columns), so
+//    LDS atomics are issued per key change rather than per pixel.  Only
+//    nonzero bins are flushed with 64-bit global atomics.
+//  * score: one workgroup per image computes areas, matches every prediction
+//    slot against the GT slots (integer strict-half test, IEEE FP64 IoU),
+//    adds confusion directly from the pair counts, and writes per-image,
+//    class-sorted matched IoUs.  The last workgroup to finish folds iou_sum in
+//    (image, slot) order, so iou_sum is deterministic and bit-identical to the
+//    reference CPU path's in-order scatter_add_.
+#include <climits>
+
+#include "common.h"
+#include "panoptic_kernels.h"
+#include "../../include/panoptic_hip.h"
+
+namespace pe {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------ slot lookup
+
+template <typename T>
+__device__ __forceinline__ int compact_slot(T v, int S) {
+  return (v >= 0 && v < static_cast<T>(S)) ? static_cast<int>(v) : -1;
+}
+
+// Original ID -> metadata slot: binary search of the sorted per-image ID
+// table (evaluator.py:147-157 semantics: ID 0 is void; padding IDs are
+// INT64_MAX with slot 0 and never resolve).
+__device__ __forceinline__ int lookup_slot(i64 key, const i64* ids, const int* slots, int S) {
+  if (key == 0) return 0;
+  if (key < 0 || key == LLONG_MAX) return -1;
+  int lo = 1, hi = S;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (ids[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  if (lo == S || ids[lo] != key) return -1;
+  int s = slots[lo];
+  return (s > 0 && s < S) ? s : -1;
+}
+
+template <typename T, bool LOOKUP>
+struct SlotResolver {
+  const i64* ids;
+  const int* slots;
+  int S;
+  i64 last_key = LLONG_MIN;
+  int last_slot = -1;
+  __device__ __forceinline__ int operator()(T v) {
+    if (!LOOKUP || ids == nullptr) return compact_slot(v, S);
+    i64 key = static_cast<i64>(v);
+    if (key != last_key) {
+      last_key = key;
+      last_slot = lookup_slot(key, ids, slots, S);
+    }
+    return last_slot;
+  }
+};
+
+// ------------------------------------------------------- joint histogram
+
+constexpr uint32_t kNoKey = 0xFFFFFFFFu;
+
+// Two-entry per-lane key cache in front of the LDS (or global) histogram.
+template <bool GLOBAL>
+struct KeyCache {
+  uint32_t k0 = kNoKey, c0 = 0, k1 = kNoKey, c1 = 0;
+  __device__ __forceinline__ static void flush(uint32_t* bins, u64* pairs, uint32_t key, uint32_t n) {
+    if (GLOBAL) atomicAdd(pairs + key, static_cast<u64>(n));
+    else atomicAdd(bins + (key >> 1), n << ((key & 1u) * 16u));
+  }
+  __device__ __forceinline__ void add(uint32_t key, uint32_t* bins, u64* pairs) {
+    if (key == k0) {
+      ++c0;
+    } else if (key == k1) {
+      ++c1;
+    } else {
+      if (c1) flush(bins, pairs, k1, c1);
+      k1 = k0;
+      c1 = c0;
+      k0 = key;
+      c0 = 1;
+    }
+  }
+  __device__ __forceinline__ void drain(uint32_t* bins, u64* pairs) {
+    if (c0) flush(bins, pairs, k0, c0);
+    if (c1) flush(bins, pairs, k1, c1);
+  }
+};
+
+template <typename GT, typename PR, bool LOOKUP, bool VEC, bool GLOBAL>
+__global__ __launch_bounds__(kHistThreads) void joint_histogram(HistArgs a) {
+  extern __shared__ uint4 lds4[];
+  uint32_t* bins = reinterpret_cast<uint32_t*>(lds4);
+  const int b = blockIdx.x / a.chunks_per_image;
+  const int chunk = blockIdx.x - b * a.chunks_per_image;
+  const int tid = threadIdx.x;
+  if (!GLOBAL) {
+    for (int w = tid; w < a.words4; w += kHistThreads) lds4[w] = make_uint4(0, 0, 0, 0);
+    __syncthreads();
+  }
+  const GT* gt = static_cast<const GT*>(a.gt);
+  const PR* pred = static_cast<const PR*>(a.pred);
+  const i64 image = static_cast<i64>(b) * a.HW;
+  const i64 begin = image + static_cast<i64>(chunk) * a.chunk;
+  const i64 end = image + min(static_cast<i64>(chunk + 1) * a.chunk, a.HW);
+  u64* pairs = a.pairs + static_cast<i64>(b) * a.G * a.P;
+  SlotResolver<GT, LOOKUP> gslot{LOOKUP && a.gt_ids ? a.gt_ids + static_cast<i64>(b) * a.G : nullptr,
+                                 LOOKUP && a.gt_ids ? a.gt_slots + static_cast<i64>(b) * a.G : nullptr, a.G};
+  SlotResolver<PR, LOOKUP> pslot{LOOKUP && a.pred_ids ? a.pred_ids + static_cast<i64>(b) * a.P : nullptr,
+                                 LOOKUP && a.pred_ids ? a.pred_slots + static_cast<i64>(b) * a.P : nullptr, a.P};
+  KeyCache<GLOBAL> cache;
+  bool bad = false;
+  const uint32_t P = static_cast<uint32_t>(a.P);
+  auto visit = [&](GT gv, PR pv) {
+    int g = gslot(gv);
+    int p = pslot(pv);
+    if ((g | p) < 0) {
+      bad = true;
+      return;
+    }
+    cache.add(static_cast<uint32_t>(g) * P + static_cast<uint32_t>(p), bins, pairs);
+  };
+  if (VEC) {
+    // 16-byte vectors of four int32 pixels; scalar head/tail of <= 3 pixels.
+    const i64 va = (begin + 3) & ~static_cast<i64>(3);
+    const i64 vb = end & ~static_cast<i64>(3);
+    const i64 head_end = va < end ? va : end;
+    for (i64 i = begin + tid; i < head_end; i += kHistThreads) visit(gt[i], pred[i]);
+    if (va < vb) {
+      const v4i* g4 = reinterpret_cast<const v4i*>(gt + va);
+      const v4i* p4 = reinterpret_cast<const v4i*>(pred + va);
+      const i64 n4 = (vb - va) >> 2;
+      constexpr int U = 4;
+      for (i64 base = tid; base < n4; base += static_cast<i64>(kHistThreads) * U) {
+        v4i gv[U], pv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          i64 k = base + static_cast<i64>(u) * kHistThreads;
+          if (k < n4) {
+            gv[u] = __builtin_nontemporal_load(g4 + k);
+            pv[u] = __builtin_nontemporal_load(p4 + k);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          i64 k = base + static_cast<i64>(u) * kHistThreads;
+          if (k < n4) {
+            visit(static_cast<GT>(gv[u].x), static_cast<PR>(pv[u].x));
+            visit(static_cast<GT>(gv[u].y), static_cast<PR>(pv[u].y));
+            visit(static_cast<GT>(gv[u].z), static_cast<PR>(pv[u].z));
+            visit(static_cast<GT>(gv[u].w), static_cast<PR>(pv[u].w));
+          }
+        }
+      }
+      for (i64 i = (vb > head_end ? vb : head_end) + tid; i < end; i += kHistThreads) visit(gt[i], pred[i]);
+    } else {
+      for (i64 i = head_end + tid; i < end; i += kHistThreads) visit(gt[i], pred[i]);
+    }
+  } else {
+    for (i64 i = begin + tid; i < end; i += kHistThreads) visit(gt[i], pred[i]);
+  }
+  cache.drain(bins, pairs);
+  if (bad && a.error) atomicOr(a.error, 1);
+  if (GLOBAL) return;
+  __syncthreads();
+  for (int w = tid; w < a.words4; w += kHistThreads) {
+    uint4 v = lds4[w];
+    uint32_t words[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (!words[j]) continue;
+      uint32_t key = (static_cast<uint32_t>(w) * 4u + j) * 2u;
+      uint32_t lo = words[j] & 0xFFFFu, hi = words[j] >> 16;
+      if (lo) atomicAdd(pairs + key, static_cast<u64>(lo));
+      if (hi) atomicAdd(pairs + key + 1, static_cast<u64>(hi));
+    }
+  }
+}
+
+template <typename GT, typename PR, bool LOOKUP>
+static int launch_hist_typed(const HistArgs& a, int blocks, size_t lds, bool vec, bool global,
+                             hipStream_t stream) {
+  if (global) {
+    joint_histogram<GT, PR, LOOKUP, false, true><<<blocks, kHistThreads, 0, stream>>>(a);
+  } else if (vec) {
+    joint_histogram<GT, PR, LOOKUP, true, false><<<blocks, kHistThreads, lds, stream>>>(a);
+  } else {
+    joint_histogram<GT, PR, LOOKUP, false, false><<<blocks, kHistThreads, lds, stream>>>(a);
+  }
+  PE_LAUNCH_CHECK();
+  return kOk;
+}
+
+int hist_default_chunk(i64 total_pixels) {
+  // Aim for >= 8 workgroups per compute unit; chunks are multiples of 4096
+  // pixels and never exceed kMaxChunk (packed 16-bit LDS counters).
+  i64 target = total_pixels / (8LL * compute_units());
+  i64 chunk = (target + 4095) / 4096 * 4096;
+  if (chunk < 4096) chunk = 4096;
+  if (chunk > kMaxChunk) chunk = kMaxChunk;
+  return static_cast<int>(chunk);
+}
+
+int launch_joint_histogram(HistArgs a, int gt_bytes, int pred_bytes, int chunk, hipStream_t stream) {
+  if (a.HW == 0 || a.B == 0) return kOk;
+  if (chunk <= 0) chunk = hist_default_chunk(a.HW * a.B);
+  chunk = std::min<int>(std::max<int>(chunk & ~3, 4), kMaxChunk);
+  a.chunk = chunk;
+  a.chunks_per_image = static_cast<int>((a.HW + chunk - 1) / chunk);
+  i64 blocks = static_cast<i64>(a.B) * a.chunks_per_image;
+  if (blocks > INT_MAX) return fail(kUnsupportedShape, "too many histogram workgroups");
+  i64 bins = static_cast<i64>(a.G) * a.P;
+  i64 words = (bins + 1) / 2;
+  a.words4 = static_cast<int>((words + 3) / 4);
+  size_t lds = static_cast<size_t>(a.words4) * 16;
+  bool global = lds > kMaxHistLds;
+  if (global) lds = 0;
+  bool lookup = a.gt_ids || a.pred_ids;
+  bool vec = gt_bytes == 4 && pred_bytes == 4 && (reinterpret_cast<uintptr_t>(a.gt) % 16) == 0 &&
+             (reinterpret_cast<uintptr_t>(a.pred) % 16) == 0;
+  int nb = static_cast<int>(blocks);
+#define PE_HIST(GT, PR)                                                          \
+  return lookup ? launch_hist_typed<GT, PR, true>(a, nb, lds, vec, global, stream) \
+                : launch_hist_typed<GT, PR, false>(a, nb, lds, vec, global, stream)
+  if (gt_bytes == 4 && pred_bytes == 4) PE_HIST(int32_t, int32_t);
+  if (gt_bytes == 4 && pred_bytes == 8) PE_HIST(int32_t, int64_t);
+  if (gt_bytes == 8 && pred_bytes == 4) PE_HIST(int64_t, int32_t);
+  PE_HIST(int64_t, int64_t);
+#undef PE_HIST
+}
+
+// ------------------------------------------------------ areas / validation
+
+__device__ void image_areas(const u64* pairs, int G, int P, i64* areas, int* matched) {
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const int waves = blockDim.x / kWave;
+  for (int g = wave; g < G; g += waves) {
+    i64 s = 0;
+    for (int p = lane; p < P; p += kWave) s += static_cast<i64>(pairs[static_cast<i64>(g) * P + p]);
+    s = wave_sum(s);
+    if (lane == 0) {
+      areas[g] = s;
+      matched[g] = 0;
+    }
+  }
+  for (int p = tid; p < P; p += blockDim.x) {
+    i64 s = 0;
+    for (int g = 0; g < G; ++g) s += static_cast<i64>(pairs[static_cast<i64>(g) * P + p]);
+    areas[G + p] = s;
+  }
+}
+
+// Error bits 2 and 4 of the reference's validation (evaluator.py:208-224).
+__device__ void image_checks(const i64* areas, const int* gc, const int* pc, const uint8_t* crowd,
+                             int G, int P, int C, int reward, int* error) {
+  int bits = 0;
+  for (int g = threadIdx.x; g < G; g += blockDim.x) {
+    int c = gc[g];
+    if (c < -1 || c >= C || (g == 0 && (c != -1 || crowd[0]))) bits |= 2;
+    if (g > 0 && ((areas[g] > 0) != (c >= 0))) bits |= 4;
+  }
+  for (int p = threadIdx.x; p < P; p += blockDim.x) {
+    int c = pc[p];
+    i64 pa = areas[G + p];
+    if (c < -1 || c >= C || (p == 0 && c != -1)) bits |= 2;
+    if (p > 0 && (reward ? (pa > 0 && c < 0) : ((pa > 0) != (c >= 0)))) bits |= 4;
+  }
+  if (bits) atomicOr(error, bits);
+}
+
+__global__ __launch_bounds__(kScoreThreads) void prepare_images(ScoreArgs a, int* error) {
+  const int b = blockIdx.x;
+  const i64 GP = static_cast<i64>(a.G) * a.P;
+  i64* areas = a.areas + static_cast<i64>(b) * (a.G + a.P);
+  image_areas(a.pairs + b * GP, a.G, a.P, areas, a.matched + static_cast<i64>(b) * a.G);
+  __syncthreads();
+  image_checks(areas, a.gc + static_cast<i64>(b) * a.G, a.pc + static_cast<i64>(b) * a.P,
+               a.crowd + static_cast<i64>(b) * a.G, a.G, a.P, a.C, a.reward, error);
+}
+
+// ------------------------------------------------------------ PQ scoring
+
+__device__ void score_one_image(const ScoreArgs& a, int b) {
+  const int G = a.G, P = a.P, C = a.C;
+  const i64 GP = static_cast<i64>(G) * P;
+  const u64* pairs = a.pairs + b * GP;
+  const int* gc = a.gc + static_cast<i64>(b) * G;
+  const int* pc = a.pc + static_cast<i64>(b) * P;
+  const uint8_t* crowd = a.crowd + static_cast<i64>(b) * G;
+  i64* areas = a.areas + static_cast<i64>(b) * (G + P);
+  int* matched = a.matched + static_cast<i64>(b) * G;
+  const i64 row = a.reward ? static_cast<i64>(b) * C : 0;
+  u64* tp = a.tp + row;
+  u64* fp = a.fp + row;
+  u64* fn = a.fn + row;
+  int* rec_lab = a.rec_lab + static_cast<i64>(b) * P;
+  double* rec_val = a.rec_val + static_cast<i64>(b) * P;
+  if (!a.areas_ready) {
+    image_areas(pairs, G, P, areas, matched);
+    __syncthreads();
+  }
+  for (int p = threadIdx.x; p < P; p += blockDim.x) {
+    const int pcls = pc[p];
+    const bool valid_pred = pcls >= 0 && pcls < C;
+    const i64 pa = areas[G + p];
+    const i64 void_overlap = static_cast<i64>(pairs[p]);
+    i64 ignored = void_overlap;
+    bool pm = false;
+    double sum_iou = 0.0;
+    for (int g = 0; g < G; ++g) {
+      const int tcls = gc[g];
+      const i64 inter = static_cast<i64>(pairs[static_cast<i64>(g) * P + p]);
+      const bool valid_target = tcls >= 0 && tcls < C;
+      if (!a.reward && valid_target && inter && !crowd[g])
+        atomicAdd(a.confusion + static_cast<i64>(tcls) * (C + 1) + (valid_pred ? pcls : C),
+                  static_cast<u64>(inter));
+      if (!valid_target || !valid_pred || tcls != pcls) continue;
+      if (crowd[g]) {
+        ignored = void_overlap + inter;  // the last same-class crowd slot wins
+      } else if (inter) {
+        const i64 uni = areas[g] + pa - inter - void_overlap;
+        if (inter > uni / 2) {  // IoU > 0.5 without rounding
+          pm = true;
+          sum_iou += static_cast<double>(inter) / static_cast<double>(uni);
+          matched[g] = 1;
+        }
+      }
+    }
+    int label = -1;
+    if (valid_pred && (!a.reward || pa > 0)) {
+      if (pm) {
+        atomicAdd(tp + pcls, 1ULL);
+        label = pcls;
+      } else if (static_cast<double>(ignored) <= 0.5 * static_cast<double>(pa)) {
+        atomicAdd(fp + pcls, 1ULL);
+      }
+    }
+    rec_lab[p] = label;
+    rec_val[p] = sum_iou;
+  }
+  __syncthreads();
+  for (int g = threadIdx.x; g < G; g += blockDim.x) {
+    const int c = gc[g];
+    if (c >= 0 && c < C && !crowd[g] && !matched[g]) atomicAdd(fn + c, 1ULL);
+  }
+  if (a.reward) {
+    // Per-image sums: in-order fold over prediction slots.
+    double* sums = a.iou_sum + row;
+    for (int c = threadIdx.x; c < C; c += blockDim.x) {
+      double acc = sums[c];
+      for (int p = 0; p < P; ++p)
+        if (rec_lab[p] == c) acc += rec_val[p];
+      sums[c] = acc;
+    }
+    return;
+  }
+  // Dataset mode: class-sorted matched IoUs of this image, slot order kept.
+  int* count = a.cls_count + static_cast<i64>(b) * C;
+  int* offset = a.cls_off + static_cast<i64>(b) * C;
+  double* comp = a.comp_val + static_cast<i64>(b) * P;
+  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    int n = 0;
+    for (int p = 0; p < P; ++p) n += rec_lab[p] == c;
+    count[c] = n;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int c = 0; c < C; ++c) {
+      offset[c] = run;
+      run += count[c];
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    if (!count[c]) continue;
+    int k = offset[c];
+    for (int p = 0; p < P; ++p)
+      if (rec_lab[p] == c) comp[k++] = rec_val[p];
+  }
+}
+
+// iou_sum[c] += matched IoUs of class c in (image, prediction slot) order.
+__device__ void fold_iou_sum(const ScoreArgs& a, int images) {
+  for (int c = threadIdx.x; c < a.C; c += blockDim.x) {
+    double acc = a.iou_sum[c];
+    for (int b = 0; b < images; ++b) {
+      const i64 idx = static_cast<i64>(b) * a.C + c;
+      const int n = a.cls_count[idx];
+      if (!n) continue;
+      const double* v = a.comp_val + static_cast<i64>(b) * a.P + a.cls_off[idx];
+      for (int e = 0; e < n; ++e) acc += v[e];
+    }
+    a.iou_sum[c] = acc;
+  }
+}
+
+__global__ __launch_bounds__(kScoreThreads) void score_images(ScoreArgs a) {
+  if (a.guard && *a.guard) return;  // a failed validation never mutates totals
+  score_one_image(a, blockIdx.x);
+  if (a.reward) return;
+  __shared__ int last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(a.counter, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  fold_iou_sum(a, a.B);
+  if (threadIdx.x == 0) *a.counter = 0;
+}
+
+int launch_score(const ScoreArgs& a, hipStream_t stream) {
+  if (a.B == 0) return kOk;
+  score_images<<<a.B, kScoreThreads, 0, stream>>>(a);
+  PE_LAUNCH_CHECK();
+  return kOk;
+}
+
+int launch_prepare(const ScoreArgs& a, int* error, hipStream_t stream) {
+  if (a.B == 0) return kOk;
+  prepare_images<<<a.B, kScoreThreads, 0, stream>>>(a, error);
+  PE_LAUNCH_CHECK();
+  return kOk;
+}
+
+// --------------------------------------------------------------- finaliser
+
+__global__ __launch_bounds__(kScoreThreads) void finalize(FinalArgs a) {
+  const i64 k = blockIdx.x;
+  const int C = a.C, tid = threadIdx.x;
+  const i64 CC = static_cast<i64>(C) * (C + 1);
+  const i64* tp = a.tp + k * C;
+  const i64* fp = a.fp + k * C;
+  const i64* fn = a.fn + k * C;
+  const double* sums = a.sums + k * C;
+  const i64* conf = a.confusion + k * CC;
+  double* per_class = a.per_class + k * 4 * C;
+  __shared__ double red[14][kScoreThreads];
+  double part[14];
+#pragma unroll
+  for (int j = 0; j < 14; ++j) part[j] = 0.0;
+  for (int c = tid; c < C; c += kScoreThreads) {
+    const i64 t = tp[c], f = fp[c], n = fn[c];
+    const double s = sums[c];
+    if (a.out_tp) {
+      a.out_tp[k * C + c] = t;
+      a.out_fp[k * C + c] = f;
+      a.out_fn[k * C + c] = n;
+      a.out_sums[k * C + c] = s;
+    }
+    const double denom = static_cast<double>(t) + 0.5 * static_cast<double>(f + n);
+    const double dd = denom < 1.0 ? 1.0 : denom;
+    const double pq = s / dd;
+    const double sq = s / static_cast<double>(t < 1 ? 1 : t);
+    const double rq = static_cast<double>(t) / dd;
+    per_class[c] = pq;
+    per_class[C + c] = sq;
+    per_class[2 * C + c] = rq;
+    i64 rowsum = 0, colsum = 0;
+    const i64 diag = conf[static_cast<i64>(c) * (C + 1) + c];
+    for (int j = 0; j <= C; ++j) rowsum += conf[static_cast<i64>(c) * (C + 1) + j];
+    for (int j = 0; j < C; ++j) colsum += conf[static_cast<i64>(j) * (C + 1) + c];
+    const i64 uni = rowsum + colsum - diag;
+    const double iou = uni > 0 ? static_cast<double>(diag) / static_cast<double>(uni) : __builtin_nan("");
+    per_class[3 * C + c] = iou;
+    if (uni > 0) {
+      part[12] += iou;
+      part[13] += 1.0;
+    }
+    if (denom > 0) {
+      const int group = a.things[c] ? 1 : 2;
+      part[0] += pq; part[1] += sq; part[2] += rq; part[3] += 1.0;
+      part[group * 4] += pq; part[group * 4 + 1] += sq; part[group * 4 + 2] += rq; part[group * 4 + 3] += 1.0;
+    }
+  }
+  if (a.out_confusion)
+    for (i64 i = tid; i < CC; i += kScoreThreads) a.out_confusion[k * CC + i] = conf[i];
+#pragma unroll
+  for (int j = 0; j < 14; ++j) red[j][tid] = part[j];
+  __syncthreads();
+  for (int stride = kScoreThreads / 2; stride > 0; stride >>= 1) {
+    if (tid < stride)
+#pragma unroll
+      for (int j = 0; j < 14; ++j) red[j][tid] += red[j][tid + stride];
+    __syncthreads();
+  }
+  if (tid < 3) {
+    const double n = red[tid * 4 + 3][0];
+    a.counts[k * 3 + tid] = static_cast<i64>(n);
+    for (int j = 0; j < 3; ++j) a.summaries[k * 9 + tid * 3 + j] = n > 0 ? red[tid * 4 + j][0] / n : 0.0;
+  }
+  if (tid == 0) a.miou[k] = red[13][0] > 0 ? red[12][0] / red[13][0] : 0.0;
+}
+
+int launch_finalize(const FinalArgs& a, i64 K, hipStream_t stream) {
+  if (K <= 0) return kOk;
+  finalize<<<static_cast<unsigned>(K), kScoreThreads, 0, stream>>>(a);
+  PE_LAUNCH_CHECK();
+  return kOk;
+}
+
+// Per-image mean-class PQ from (B, C) statistics (evaluator.py:539-544):
+// one wave per image, classes folded in order then combined across lanes in
+// a fixed tree, so the result does not depend on the launch.
+__global__ __launch_bounds__(256) void image_pq(const i64* tp, const i64* fp, const i64* fn,
+                                                const double* sums, int B, int C, double* out,
+                                                double* stats) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (wave >= B) return;
+  const i64 row = static_cast<i64>(wave) * C;
+  double pq_sum = 0.0, active = 0.0;
+  for (int c = lane; c < C; c += kWave) {
+    const i64 t = tp[row + c], f = fp[row + c], n = fn[row + c];
+    const double s = sums[row + c];
+    const double denom = static_cast<double>(t) + 0.5 * static_cast<double>(f + n);
+    pq_sum += s / (denom < 1.0 ? 1.0 : denom);
+    active += denom > 0 ? 1.0 : 0.0;
+    if (stats) {
+      double* o = stats + (row + c) * 4;
+      o[0] = s;
+      o[1] = static_cast<double>(t);
+      o[2] = static_cast<double>(f);
+      o[3] = static_cast<double>(n);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    pq_sum += __shfl_xor(pq_sum, off, kWave);
+    active += __shfl_xor(active, off, kWave);
+  }
+  if (lane == 0 && out) out[wave] = pq_sum / (active < 1.0 ? 1.0 : active);
+}
+
+// ---------------------------------------------------------------- packing
+
+template <typename T>
+__global__ __launch_bounds__(256) void pack_slots(const T* maps, i64 HW, const i64* ids, const int* slots,
+                                                  int S, int blocks_per_image, int32_t* out, int* error) {
+  const int b = blockIdx.x / blocks_per_image;
+  const int part = blockIdx.x - b * blocks_per_image;
+  const i64 span = (HW + blocks_per_image - 1) / blocks_per_image;
+  const i64 begin = static_cast<i64>(b) * HW + part * span;
+  const i64 end = static_cast<i64>(b) * HW + min(HW, (part + 1) * span);
+  SlotResolver<T, true> resolve{ids + static_cast<i64>(b) * S, slots + static_cast<i64>(b) * S, S};
+  bool bad = false;
+  for (i64 i = begin + threadIdx.x; i < end; i += blockDim.x) {
+    int s = resolve(maps[i]);
+    out[i] = s >= 0 ? s : 0;
+    bad |= s < 0;
+  }
+  if (bad) atomicOr(error, 1);
+}
+
+}  // namespace pe
+
+using namespace pe;
+
+extern "C" int64_t pe_update_workspace_bytes(int64_t B, int64_t G, int64_t P, int64_t C) {
+  Carve carve(nullptr);
+  carve.take<int>(B * P);        // rec_lab
+  carve.take<double>(B * P);     // rec_val
+  carve.take<int>(B * C);        // cls_count
+  carve.take<int>(B * C);        // cls_off
+  carve.take<double>(B * P);     // comp_val
+  carve.take<unsigned>(64);      // counter
+  return static_cast<int64_t>(align_up(carve.used, 256));
+}
+
+extern "C" int pe_update(const void* gt, int gt_bytes, const void* pred, int pred_bytes, int64_t B,
+                         int64_t H, int64_t W, const int32_t* gc, const int32_t* pc, const uint8_t* crowd,
+                         int64_t G, int64_t P, int64_t C, int64_t* pairs, int64_t* areas, int32_t* matched,
+                         int64_t* tp, int64_t* fp, int64_t* fn, double* iou_sum, int64_t* confusion,
+                         const int64_t* gt_ids, const int32_t* gt_slots, const int64_t* pred_ids,
+                         const int32_t* pred_slots, int32_t* error, void* workspace, int64_t workspace_bytes,
+                         int validate, int reward, void* stream_ptr) {
+  PE_REQUIRE(B >= 0 && H >= 0 && W >= 0, "negative shape");
+  PE_REQUIRE(G > 0 && G < INT_MAX && P > 0 && P < INT_MAX && C > 0 && C < INT_MAX, "invalid capacities");
+  PE_REQUIRE(B < INT_MAX && G * P < (1LL << 32), "pair histogram too large");
+  PE_REQUIRE((gt_bytes == 4 || gt_bytes == 8) && (pred_bytes == 4 || pred_bytes == 8), "maps must be int32/int64");
+  PE_REQUIRE((gt_ids == nullptr) == (gt_slots == nullptr) && (pred_ids == nullptr) == (pred_slots == nullptr),
+             "lookup IDs and slots must be supplied together");
+  PE_REQUIRE(workspace_bytes >= pe_update_workspace_bytes(B, G, P, C), "workspace too small");
+  PE_REQUIRE(!validate || error, "validation needs the error word");
+  hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+  if (B == 0) return kOk;
+  Carve carve(workspace);
+  ScoreArgs s{};
+  s.pairs = reinterpret_cast<const u64*>(pairs);
+  s.gc = gc;
+  s.pc = pc;
+  s.crowd = crowd;
+  s.areas = reinterpret_cast<i64*>(areas);
+  s.matched = matched;
+  s.tp = reinterpret_cast<u64*>(tp);
+  s.fp = reinterpret_cast<u64*>(fp);
+  s.fn = reinterpret_cast<u64*>(fn);
+  s.iou_sum = iou_sum;
+  s.confusion = reinterpret_cast<u64*>(confusion);
+  s.rec_lab = carve.take<int>(B * P);
+  s.rec_val = carve.take<double>(B * P);
+  s.cls_count = carve.take<int>(B * C);
+  s.cls_off = carve.take<int>(B * C);
+  s.comp_val = carve.take<double>(B * P);
+  s.counter = carve.take<unsigned>(64);
+  s.B = static_cast<int>(B);
+  s.G = static_cast<int>(G);
+  s.P = static_cast<int>(P);
+  s.C = static_cast<int>(C);
+  s.reward = reward;
+  int rc = launch_zero(pairs, static_cast<size_t>(B * G * P) * 8, stream);
+  if (rc) return rc;
+  if ((rc = launch_zero(s.counter, 256, stream))) return rc;
+  if (validate && (rc = launch_zero(error, 4, stream))) return rc;
+  HistArgs h{};
+  h.gt = gt;
+  h.pred = pred;
+  h.B = static_cast<int>(B);
+  h.HW = H * W;
+  h.G = s.G;
+  h.P = s.P;
+  h.pairs = reinterpret_cast<u64*>(pairs);
+  h.gt_ids = gt_ids;
+  h.gt_slots = gt_slots;
+  h.pred_ids = pred_ids;
+  h.pred_slots = pred_slots;
+  h.error = validate ? error : nullptr;
+  if ((rc = launch_joint_histogram(h, gt_bytes, pred_bytes, 0, stream))) return rc;
+  if (validate) {
+    if ((rc = launch_prepare(s, error, stream))) return rc;
+    int32_t host_error = 0;
+    PE_HIP(hipMemcpyAsync(&host_error, error, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    PE_HIP(hipStreamSynchronize(stream));
+    if (host_error) return host_error;  // before any accumulator changes
+    s.areas_ready = 1;
+  }
+  return launch_score(s, stream);
+}
+
+extern "C" int pe_compute(const int64_t* tp, const int64_t* fp, const int64_t* fn, const double* sums,
+                          const int64_t* confusion, const uint8_t* things, int64_t K, int64_t C,
+                          double* per_class, double* summaries, int64_t* counts, double* miou,
+                          int64_t* out_tp, int64_t* out_fp, int64_t* out_fn, double* out_sums,
+                          int64_t* out_confusion, void* stream) {
+  PE_REQUIRE(C > 0 && C < INT_MAX && K >= 0 && K < INT_MAX, "invalid compute shape");
+  PE_REQUIRE((out_tp == nullptr) == (out_fp == nullptr) && (out_tp == nullptr) == (out_fn == nullptr) &&
+                 (out_tp == nullptr) == (out_sums == nullptr),
+             "state copies must be requested together");
+  FinalArgs a{};
+  a.tp = reinterpret_cast<const i64*>(tp);
+  a.fp = reinterpret_cast<const i64*>(fp);
+  a.fn = reinterpret_cast<const i64*>(fn);
+  a.sums = sums;
+  a.confusion = reinterpret_cast<const i64*>(confusion);
+  a.things = things;
+  a.C = static_cast<int>(C);
+  a.per_class = per_class;
+  a.summaries = summaries;
+  a.counts = reinterpret_cast<i64*>(counts);
+  a.miou = miou;
+  a.out_tp = reinterpret_cast<i64*>(out_tp);
+  a.out_fp = reinterpret_cast<i64*>(out_fp);
+  a.out_fn = reinterpret_cast<i64*>(out_fn);
+  a.out_sums = out_sums;
+  a.out_confusion = reinterpret_cast<i64*>(out_confusion);
+  return launch_finalize(a, K, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int pe_pack(const void* maps, int map_bytes, int64_t B, int64_t HW, const int64_t* ids,
+                       const int32_t* slots, int64_t S, int32_t* out, int32_t* error, void* stream_ptr) {
+  PE_REQUIRE(map_bytes == 4 || map_bytes == 8, "maps must be int32/int64");
+  PE_REQUIRE(S > 0 && S < INT_MAX && B >= 0 && HW >= 0, "invalid lookup shapes");
+  hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+  int rc = launch_zero(error, 4, stream);
+  if (rc || B == 0 || HW == 0) return rc;
+  i64 per_image = std::max<i64>(1, std::min<i64>((HW + 8191) / 8192, (4LL * compute_units() + B - 1) / B));
+  PE_REQUIRE(B * per_image < INT_MAX, "too many images");
+  int blocks = static_cast<int>(B * per_image);
+  if (map_bytes == 4)
+    pack_slots<int32_t><<<blocks, 256, 0, stream>>>(static_cast<const int32_t*>(maps), HW, ids, slots,
+                                                    static_cast<int>(S), static_cast<int>(per_image), out, error);
+  else
+    pack_slots<int64_t><<<blocks, 256, 0, stream>>>(static_cast<const int64_t*>(maps), HW, ids, slots,
+                                                    static_cast<int>(S), static_cast<int>(per_image), out, error);
+  PE_LAUNCH_CHECK();
+  return kOk;
+}
+
+extern "C" int pe_image_pq(const int64_t* tp, const int64_t* fp, const int64_t* fn, const double* sums,
+                           int64_t B, int64_t C, double* out, double* stats, void* stream_ptr) {
+  PE_REQUIRE(B >= 0 && B < INT_MAX / 64 && C > 0 && C < INT_MAX, "invalid reward shape");
+  if (B == 0) return kOk;
+  int blocks = static_cast<int>((B * kWave + 255) / 256);
+  image_pq<<<blocks, 256, 0, static_cast<hipStream_t>(stream_ptr)>>>(
+      reinterpret_cast<const i64*>(tp), reinterpret_cast<const i64*>(fp), reinterpret_cast<const i64*>(fn),
+      sums, static_cast<int>(B), static_cast<int>(C), out, stats);
+  PE_LAUNCH_CHECK();
+  return kOk;
+}

@@ -1,0 +1,80 @@
+// Kernel argument blocks and launchers of the panoptic path, shared by the
+// plain entry points (panoptic.hip) and the batched / serving forms.
+#pragma once
+
+#include "common.h"
+
+namespace pe {
+
+constexpr int kHistThreads = 512;
+constexpr int kScoreThreads = 256;
+constexpr int kMaxChunk = 65532;                 // packed 16-bit LDS counters stay exact
+constexpr size_t kMaxHistLds = 128 * 1024;       // beyond this, global atomics
+
+struct HistArgs {
+  const void* gt;
+  const void* pred;
+  int B;
+  i64 HW;
+  int G, P;
+  int chunk;             // pixels per workgroup
+  int chunks_per_image;
+  int words4;            // LDS size in 16-byte units
+  u64* pairs;            // B x G x P
+  const i64* gt_ids;
+  const int* gt_slots;
+  const i64* pred_ids;
+  const int* pred_slots;
+  int* error;            // nullptr: no validation
+};
+
+struct ScoreArgs {
+  const u64* pairs;
+  const int* gc;
+  const int* pc;
+  const uint8_t* crowd;
+  i64* areas;
+  int* matched;
+  u64* tp;
+  u64* fp;
+  u64* fn;
+  double* iou_sum;
+  u64* confusion;
+  int* rec_lab;
+  double* rec_val;
+  int* cls_count;
+  int* cls_off;
+  double* comp_val;
+  unsigned* counter;
+  const int* guard;
+  int B, G, P, C;
+  int reward;
+  int areas_ready;
+};
+
+struct FinalArgs {
+  const i64* tp;
+  const i64* fp;
+  const i64* fn;
+  const double* sums;
+  const i64* confusion;
+  const uint8_t* things;
+  int C;
+  double* per_class;
+  double* summaries;
+  i64* counts;
+  double* miou;
+  i64* out_tp;
+  i64* out_fp;
+  i64* out_fn;
+  double* out_sums;
+  i64* out_confusion;
+};
+
+int hist_default_chunk(i64 total_pixels);
+int launch_joint_histogram(HistArgs a, int gt_bytes, int pred_bytes, int chunk, hipStream_t stream);
+int launch_prepare(const ScoreArgs& a, int* error, hipStream_t stream);
+int launch_score(const ScoreArgs& a, hipStream_t stream);
+int launch_finalize(const FinalArgs& a, i64 K, hipStream_t stream);
+
+}  // namespace pe

@@ -1,0 +1,14 @@
+"""panoptic_evaluator for AMD Instinct MI355X (gfx950).
+
+Drop-in for the reference package's public API (its __init__.py:1-14): PQ/SQ/RQ,
+semantic mIoU, COCO mask AP/AR and generation-prefix rewards, computed by
+hand-written HIP kernels behind a C ABI (libpanoptic_hip.so).
+"""
+
+from .evaluator import PanopticBatch, PanopticEvaluator, panoptic_quality
+
+__all__ = [
+    "PanopticBatch",
+    "PanopticEvaluator",
+    "panoptic_quality",
+]

@@ -1,0 +1,98 @@
+"""ctypes binding of libpanoptic_hip.so (the gfx950 kernels' C ABI).
+
+The library is built in-tree by ``panoptic-evaluator_amd/build.py``.  There is
+no fallback: if the library is missing or the tensors are not on a HIP device,
+the call fails loudly.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import pathlib
+import threading
+
+import torch
+
+LIBRARY_PATH = pathlib.Path(__file__).with_name("libpanoptic_hip.so")
+ABI_VERSION = 1
+
+_i64, _int, _ptr, _dbl = ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_double
+
+# name: (restype, argtypes)
+_SIGNATURES = {
+    "pe_last_error": (ctypes.c_char_p, []),
+    "pe_abi_version": (_int, []),
+    "pe_update_workspace_bytes": (_i64, [_i64] * 4),
+    "pe_update": (_int, [_ptr, _int, _ptr, _int, _i64, _i64, _i64, _ptr, _ptr, _ptr, _i64, _i64, _i64,
+                         _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr,
+                         _ptr, _ptr, _i64, _int, _int, _ptr]),
+    "pe_compute": (_int, [_ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _ptr, _ptr, _ptr, _ptr,
+                          _ptr, _ptr, _ptr, _ptr, _ptr, _ptr]),
+    "pe_image_pq": (_int, [_ptr, _ptr, _ptr, _ptr, _i64, _i64, _ptr, _ptr, _ptr]),
+    "pe_pack": (_int, [_ptr, _int, _i64, _i64, _ptr, _ptr, _i64, _ptr, _ptr, _ptr]),
+}
+
+_lock = threading.Lock()
+_library = None
+
+
+def register(name, restype, argtypes):
+    """Declare an additional entry point (used by the other modules)."""
+    _SIGNATURES[name] = (restype, argtypes)
+    if _library is not None:
+        _bind(_library, name)
+
+
+def _bind(lib, name):
+    fn = getattr(lib, name)
+    fn.restype, fn.argtypes = _SIGNATURES[name]
+
+
+def library():
+    """Load and bind the HIP library (once per process)."""
+    global _library
+    if _library is None:
+        with _lock:
+            if _library is None:
+                if not LIBRARY_PATH.exists():
+                    raise RuntimeError(
+                        f"{LIBRARY_PATH.name} is not built; run python panoptic-evaluator_amd/build.py")
+                lib = ctypes.CDLL(str(LIBRARY_PATH))
+                for name in _SIGNATURES:
+                    _bind(lib, name)
+                if lib.pe_abi_version() != ABI_VERSION:
+                    raise RuntimeError("libpanoptic_hip.so ABI version mismatch; rebuild it")
+                _library = lib
+    return _library
+
+
+def ptr(tensor):
+    return None if tensor is None else tensor.data_ptr()
+
+
+def stream(device=None) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def check(code: int) -> int:
+    """Raise on negative status; return validation bits (>= 0) unchanged."""
+    if code < 0:
+        message = library().pe_last_error().decode(errors="replace")
+        raise RuntimeError(f"panoptic HIP kernels: {message}")
+    return code
+
+
+def call(name, *args) -> int:
+    return check(getattr(library(), name)(*args))
+
+
+def require_gpu(device: torch.device) -> torch.device:
+    """The product path evaluates HIP device tensors only; no CPU fallback."""
+    device = torch.device(device)
+    if device.type != "cuda":
+        raise ValueError(
+            "this panoptic_evaluator build evaluates on AMD GPUs (device='cuda'); "
+            f"got device {device}")
+    if not torch.cuda.is_available():
+        raise RuntimeError("no HIP device is available")
+    return device

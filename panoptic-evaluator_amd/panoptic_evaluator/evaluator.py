@@ -1,0 +1,328 @@
+"""Panoptic and semantic evaluation on AMD GPUs.
+
+Public surface identical to the reference's panoptic_evaluator/evaluator.py
+(PanopticBatch :23-158, PanopticEvaluator :246-468, panoptic_quality
+:471-544).  Every pixel- and segment-proportional step runs in the gfx950
+kernels of libpanoptic_hip.so (joint histogram, areas/validation, PQ matching,
+confusion, FP64 finaliser); this module only checks shapes, owns the
+accumulator tensors and maps error codes to the reference's exceptions.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, replace
+from numbers import Integral
+from typing import Sequence
+
+import torch
+
+from . import _native
+
+INT64_MAX = torch.iinfo(torch.int64).max
+
+MSG_UNKNOWN = "map contains an unknown segment ID or a slot outside metadata capacity"
+MSG_CLASSES = "classes must be -1 or valid category indices; slot 0 must be void"
+MSG_ACTIVE = "nonvoid map slots and active metadata must correspond exactly"
+
+
+def raise_validation(code: int):
+    """Map the kernels' validation bitmask to the reference's messages
+    (evaluator.py:365-376)."""
+    if code & 1:
+        raise ValueError(MSG_UNKNOWN)
+    if code & 2:
+        raise ValueError(MSG_CLASSES)
+    raise ValueError(MSG_ACTIVE)
+
+
+@dataclass(frozen=True)
+class PanopticBatch:
+    """Segment maps and metadata; optional lookup tables allow original IDs.
+
+    segment_map  B x H x W; compact int32 slots, or original int32/int64 IDs
+    classes      B x S int32; slot 0 is void, class -1 is padding
+    crowd        optional B x S bool (ground truth only)
+    lookup_ids   optional B x S sorted int64 original IDs
+    lookup_slots optional B x S int32 original metadata slots
+    """
+
+    segment_map: torch.Tensor
+    classes: torch.Tensor
+    crowd: torch.Tensor | None = None
+    lookup_ids: torch.Tensor | None = None
+    lookup_slots: torch.Tensor | None = None
+
+    @classmethod
+    def from_coco(cls, maps: torch.Tensor, segments_info: Sequence[Sequence[dict]],
+                  category_ids: Sequence[int], *, compact: bool = True) -> PanopticBatch:
+        """Build slot metadata from COCO ``segments_info`` with one host-to-device
+        upload (evaluator.py:33-106).  Metadata keeps input order, so the
+        official "last crowd segment per category" rule applies.  With
+        compact=False the maps keep their original IDs and the lookup is fused
+        into the histogram kernel; pixel validation then happens in update().
+        """
+        if maps.ndim != 3 or maps.dtype not in (torch.int32, torch.int64):
+            raise ValueError("maps must be integer B x H x W tensors")
+        if len(segments_info) != maps.shape[0] or len(set(category_ids)) != len(category_ids):
+            raise ValueError("batch length or category IDs are invalid")
+        index_of = {category: index for index, category in enumerate(category_ids)}
+        capacity = 1 + max((len(info) for info in segments_info), default=0)
+        ids, slots, classes, crowd = [], [], [], []
+        for info in segments_info:
+            segment_ids = [entry["id"] for entry in info]
+            if len(set(segment_ids)) != len(segment_ids) or any(
+                    not isinstance(i, Integral) or not 0 < i < INT64_MAX for i in segment_ids):
+                raise ValueError(
+                    "segment IDs must be unique positive int64 values below INT64_MAX; 0 is void")
+            try:
+                labels = [index_of[entry["category_id"]] for entry in info]
+            except KeyError as error:
+                raise ValueError("unknown category_id") from error
+            # Sorted (id, metadata slot) table for the binary search; slot
+            # numbers keep metadata order.
+            table = sorted(zip(segment_ids, range(1, len(info) + 1)))
+            padding = capacity - 1 - len(info)
+            ids += [0, *(i for i, _ in table)] + [INT64_MAX] * padding
+            slots += [0, *(s for _, s in table)] + [0] * padding
+            classes += [-1, *labels] + [-1] * padding
+            crowd += [False, *(bool(entry.get("iscrowd", 0)) for entry in info)] + [False] * padding
+        host = [torch.tensor(ids, dtype=torch.int64), torch.tensor(slots, dtype=torch.int32),
+                torch.tensor(classes, dtype=torch.int32), torch.tensor(crowd, dtype=torch.bool)]
+        # One upload: the typed fields share a single byte buffer (bool last
+        # keeps the 8- and 4-byte fields aligned).
+        blob = torch.cat([t.view(torch.uint8) for t in host]).to(maps.device)
+        views, offset = [], 0
+        for t in host:
+            size = t.numel() * t.element_size()
+            views.append(blob[offset:offset + size].view(t.dtype).reshape(maps.shape[0], capacity))
+            offset += size
+        batch = cls(maps, views[2], views[3], views[0], views[1])
+        return batch.compact() if compact else batch
+
+    def with_maps(self, maps: torch.Tensor) -> PanopticBatch:
+        """Reuse prepared metadata for another map batch (evaluator.py:108-118)."""
+        if maps.ndim != 3 or maps.shape[0] != self.classes.shape[0] or maps.device != self.classes.device:
+            raise ValueError("new maps must have the same batch size and metadata device")
+        return replace(self, segment_map=maps)
+
+    def compact(self) -> PanopticBatch:
+        """Materialize compact slots with one pixel kernel (evaluator.py:120-158)."""
+        if self.lookup_ids is None:
+            return self
+        maps = self.segment_map
+        if maps.ndim != 3 or maps.dtype not in (torch.int32, torch.int64):
+            raise ValueError("maps must be integer B x H x W tensors")
+        if (self.lookup_slots is None or self.lookup_ids.shape != self.classes.shape
+                or self.lookup_slots.shape != self.classes.shape):
+            raise ValueError("lookup IDs and slots must match classes shape")
+        if (self.lookup_ids.device != maps.device or self.lookup_slots.device != maps.device
+                or maps.shape[0] != self.classes.shape[0]):
+            raise ValueError("lookup tables and maps must share device and batch size")
+        _native.require_gpu(maps.device)
+        maps = maps.contiguous()
+        ids = self.lookup_ids.contiguous()
+        slots = self.lookup_slots.contiguous()
+        packed = torch.empty(maps.shape, dtype=torch.int32, device=maps.device)
+        error = torch.empty(1, dtype=torch.int32, device=maps.device)
+        B, H, W = maps.shape
+        _native.call("pe_pack", maps.data_ptr(), maps.element_size(), B, H * W, ids.data_ptr(),
+                     slots.data_ptr(), ids.shape[1], packed.data_ptr(), error.data_ptr(),
+                     _native.stream(maps.device))
+        if error.item():
+            raise ValueError("map contains a segment absent from segments_info")
+        return PanopticBatch(packed, self.classes, self.crowd)
+
+
+def _prepare(batch: PanopticBatch, num_classes: int, validate: bool, *, ignore_crowd: bool = False,
+             allow_empty_slots: bool = False):
+    """Shape, dtype and device checks (evaluator.py:161-207).  Value checks run
+    inside the kernels (``validate``) and never on the host."""
+    maps, classes = batch.segment_map, batch.classes
+    if maps.ndim != 3 or classes.ndim != 2 or classes.shape[0] != maps.shape[0] or classes.shape[1] < 1:
+        raise ValueError("expected maps B x H x W and classes B x S with S >= 1")
+    original_ids = batch.lookup_ids is not None
+    if (batch.lookup_slots is not None) != original_ids:
+        raise ValueError("lookup IDs and slots must be supplied together")
+    map_dtypes = (torch.int32, torch.int64) if original_ids else (torch.int32,)
+    if maps.dtype not in map_dtypes or classes.dtype != torch.int32:
+        raise ValueError("classes and compact maps must be int32; original ID maps may also be int64")
+    if original_ids:
+        for lookup, dtype in ((batch.lookup_ids, torch.int64), (batch.lookup_slots, torch.int32)):
+            if lookup.shape != classes.shape or lookup.dtype != dtype or lookup.device != maps.device:
+                raise ValueError("lookup tables must match metadata shape, device and expected dtypes")
+    crowd = None
+    if not ignore_crowd:
+        crowd = batch.crowd if batch.crowd is not None else torch.zeros_like(classes, dtype=torch.bool)
+        if crowd.shape != classes.shape or crowd.dtype != torch.bool:
+            raise ValueError("crowd must be bool with the same shape as classes")
+    if classes.device != maps.device or (crowd is not None and crowd.device != maps.device):
+        raise ValueError("batch tensors must share a device")
+    return maps.contiguous(), classes.contiguous(), None if crowd is None else crowd.contiguous()
+
+
+def _lookup(batch: PanopticBatch):
+    if batch.lookup_ids is None:
+        return None, None
+    return batch.lookup_ids.contiguous(), batch.lookup_slots.contiguous()
+
+
+class PanopticEvaluator:
+    """Accumulate dataset PQ/SQ/RQ and semantic IoU without copying pixels to
+    the host (evaluator.py:246-468)."""
+
+    def __init__(self, num_classes: int, *, isthing: Sequence[bool] | None = None,
+                 device: str | torch.device = "cuda", validate: bool = True):
+        if not isinstance(num_classes, int) or isinstance(num_classes, bool) or num_classes <= 0:
+            raise ValueError("num_classes must be a positive integer")
+        if isthing is not None and len(isthing) != num_classes:
+            raise ValueError("isthing must contain one flag per category")
+        self.num_classes = num_classes
+        self.device = _native.require_gpu(device)
+        self.validate = validate
+        self.isthing = torch.tensor(list(isthing) if isthing is not None else [True] * num_classes,
+                                    dtype=torch.bool, device=self.device)
+        self._scratch = None
+        self._workspace = None
+        self._default_crowd = None
+        self._validation_error = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self.reset()
+
+    def reset(self):
+        """Zero the accumulators in place (their storage is stable)."""
+        if hasattr(self, "tp"):
+            for tensor in (self.tp, self.fp, self.fn, self.iou_sum, self.confusion):
+                tensor.zero_()
+            return
+        C = self.num_classes
+        self.iou_sum = torch.zeros(C, dtype=torch.float64, device=self.device)
+        self.tp = torch.zeros(C, dtype=torch.int64, device=self.device)
+        self.fp = torch.zeros_like(self.tp)
+        self.fn = torch.zeros_like(self.tp)
+        # Column C counts void (or inactive) predictions on valid GT pixels.
+        self.confusion = torch.zeros((C, C + 1), dtype=torch.int64, device=self.device)
+
+    @torch.no_grad()
+    def update(self, prediction: PanopticBatch, target: PanopticBatch):
+        return self._update(prediction, target)
+
+    def _scratch_for(self, B, G, P):
+        shape = (B, G, P)
+        if self._scratch is None or self._scratch[0].shape != shape:
+            device = self.device
+            self._scratch = (torch.empty(shape, dtype=torch.int64, device=device),
+                             torch.empty((B, G + P), dtype=torch.int64, device=device),
+                             torch.empty((B, G), dtype=torch.int32, device=device))
+            size = _native.library().pe_update_workspace_bytes(B, G, P, self.num_classes)
+            self._workspace = torch.zeros(max(size, 256), dtype=torch.uint8, device=device)
+        return self._scratch
+
+    @torch.no_grad()
+    def _update(self, prediction: PanopticBatch, target: PanopticBatch, *, return_areas=False):
+        if target.crowd is None:
+            if (self._default_crowd is None or self._default_crowd.shape != target.classes.shape
+                    or self._default_crowd.device != target.classes.device):
+                self._default_crowd = torch.zeros_like(target.classes, dtype=torch.bool)
+            target = replace(target, crowd=self._default_crowd)
+        gt, gc, crowd = _prepare(target, self.num_classes, False)
+        pred, pc, _ = _prepare(prediction, self.num_classes, False, ignore_crowd=True)
+        if gt.shape != pred.shape or gt.device != pred.device or gt.device != self.tp.device:
+            raise ValueError("prediction and target must share shape and evaluator device")
+        B, H, W = gt.shape
+        G, P = gc.shape[1], pc.shape[1]
+        pairs, areas, matched = self._scratch_for(B, G, P)
+        gt_ids, gt_slots = _lookup(target)
+        pred_ids, pred_slots = _lookup(prediction)
+        code = _native.call(
+            "pe_update", gt.data_ptr(), gt.element_size(), pred.data_ptr(), pred.element_size(), B, H, W,
+            gc.data_ptr(), pc.data_ptr(), crowd.data_ptr(), G, P, self.num_classes,
+            pairs.data_ptr(), areas.data_ptr(), matched.data_ptr(), self.tp.data_ptr(),
+            self.fp.data_ptr(), self.fn.data_ptr(), self.iou_sum.data_ptr(), self.confusion.data_ptr(),
+            _native.ptr(gt_ids), _native.ptr(gt_slots), _native.ptr(pred_ids), _native.ptr(pred_slots),
+            self._validation_error.data_ptr(), self._workspace.data_ptr(), self._workspace.numel(),
+            int(self.validate), 0, _native.stream(self.device))
+        if code:
+            raise_validation(code)
+        return areas[:, :G] if return_areas else self
+
+    @torch.no_grad()
+    def compute(self) -> dict[str, torch.Tensor | dict]:
+        """Device tensors in [0, 1]; absent classes have zero PQ and NaN IoU.
+        The returned state tensors are copies (valid after reset/update)."""
+        return finalize(self.tp[None], self.fp[None], self.fn[None], self.iou_sum[None],
+                        self.confusion[None], self.isthing)[0]
+
+
+def finalize(tp, fp, fn, sums, confusion, things):
+    """Run the FP64 finaliser over K accumulator rows; return K result dicts."""
+    K, C = tp.shape
+    device = tp.device
+    doubles = dict(dtype=torch.float64, device=device)
+    per_class = torch.empty((K, 4, C), **doubles)
+    summaries = torch.empty((K, 3, 3), **doubles)
+    counts = torch.empty((K, 3), dtype=torch.int64, device=device)
+    miou = torch.empty(K, **doubles)
+    copies = [torch.empty_like(t) for t in (tp, fp, fn, sums, confusion)]
+    tensors = [t.contiguous() for t in (tp, fp, fn, sums, confusion)]
+    _native.call("pe_compute", *(t.data_ptr() for t in tensors), things.data_ptr(), K, C,
+                 per_class.data_ptr(), summaries.data_ptr(), counts.data_ptr(), miou.data_ptr(),
+                 *(t.data_ptr() for t in copies), _native.stream(device))
+    out_tp, out_fp, out_fn, out_sums, out_confusion = copies
+    results = []
+    for k in range(K):
+        result = {name: {"pq": summaries[k, i, 0], "sq": summaries[k, i, 1], "rq": summaries[k, i, 2],
+                         "n": counts[k, i]}
+                  for i, name in enumerate(("All", "Things", "Stuff"))}
+        result.update(per_class={name: per_class[k, i] for i, name in enumerate(("pq", "sq", "rq", "iou"))},
+                      miou=miou[k], tp=out_tp[k], fp=out_fp[k], fn=out_fn[k], iou_sum=out_sums[k],
+                      confusion=out_confusion[k])
+        results.append(result)
+    return results
+
+
+@torch.no_grad()
+def panoptic_quality(prediction: PanopticBatch, target: PanopticBatch, num_classes: int,
+                     reduction: str = "none", *, validate: bool = True,
+                     return_stats: bool = False) -> torch.Tensor:
+    """Stateless mean-class PQ per image, float64 on the input device
+    (evaluator.py:471-544).  Prediction slots without pixels are ignored.
+    With return_stats=True return [image, class, (IoU sum, TP, FP, FN)]."""
+    if not isinstance(num_classes, int) or isinstance(num_classes, bool) or num_classes <= 0:
+        raise ValueError("num_classes must be a positive integer")
+    if reduction != "none":
+        raise ValueError('reduction must be "none"')
+    gt, gc, crowd = _prepare(target, num_classes, False)
+    pred, pc, _ = _prepare(prediction, num_classes, False, ignore_crowd=True, allow_empty_slots=True)
+    if gt.shape != pred.shape or gt.device != pred.device:
+        raise ValueError("prediction and target must share shape and device")
+    device = _native.require_gpu(gt.device)
+    B, H, W = gt.shape
+    G, P, C = gc.shape[1], pc.shape[1], num_classes
+    tp = torch.zeros((B, C), dtype=torch.int64, device=device)
+    fp, fn = torch.zeros_like(tp), torch.zeros_like(tp)
+    sums = torch.zeros((B, C), dtype=torch.float64, device=device)
+    pairs = torch.empty((B, G, P), dtype=torch.int64, device=device)
+    areas = torch.empty((B, G + P), dtype=torch.int64, device=device)
+    matched = torch.empty((B, G), dtype=torch.int32, device=device)
+    error = torch.empty(1, dtype=torch.int32, device=device)
+    size = max(_native.library().pe_update_workspace_bytes(B, G, P, C), 256)
+    workspace = torch.zeros(size, dtype=torch.uint8, device=device)
+    gt_ids, gt_slots = _lookup(target)
+    pred_ids, pred_slots = _lookup(prediction)
+    stream = _native.stream(device)
+    code = _native.call(
+        "pe_update", gt.data_ptr(), gt.element_size(), pred.data_ptr(), pred.element_size(), B, H, W,
+        gc.data_ptr(), pc.data_ptr(), crowd.data_ptr(), G, P, C, pairs.data_ptr(), areas.data_ptr(),
+        matched.data_ptr(), tp.data_ptr(), fp.data_ptr(), fn.data_ptr(), sums.data_ptr(), None,
+        _native.ptr(gt_ids), _native.ptr(gt_slots), _native.ptr(pred_ids), _native.ptr(pred_slots),
+        error.data_ptr(), workspace.data_ptr(), workspace.numel(), int(validate), 1, stream)
+    if code:
+        raise_validation(code)
+    if return_stats:
+        stats = torch.empty((B, C, 4), dtype=torch.float64, device=device)
+        _native.call("pe_image_pq", tp.data_ptr(), fp.data_ptr(), fn.data_ptr(), sums.data_ptr(), B, C,
+                     None, stats.data_ptr(), stream)
+        return stats
+    out = torch.empty(B, dtype=torch.float64, device=device)
+    _native.call("pe_image_pq", tp.data_ptr(), fp.data_ptr(), fn.data_ptr(), sums.data_ptr(), B, C,
+                 out.data_ptr(), None, stream)
+    return out

@@ -1,0 +1,49 @@
+"""CPU checks of the C ABI: the built library loads and exports every entry
+point declared in include/panoptic_hip.h (no compute calls without a GPU)."""
+
+import ctypes
+import pathlib
+import re
+import subprocess
+
+import pytest
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "panoptic_hip.h"
+LIB = ROOT / "panoptic-evaluator_amd" / "panoptic_evaluator" / "libpanoptic_hip.so"
+
+
+def declared_functions():
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(pe_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    assert "pe_update" in names and "pe_compute" in names and len(names) >= 6
+
+
+@pytest.mark.skipif(not LIB.exists(), reason="library not built")
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(str(LIB))
+    missing = [name for name in declared_functions() if not hasattr(lib, name)]
+    assert not missing, missing
+    lib.pe_abi_version.restype = ctypes.c_int
+    assert lib.pe_abi_version() == 1
+
+
+@pytest.mark.skipif(not LIB.exists(), reason="library not built")
+def test_library_has_no_torch_dependency():
+    needed = subprocess.run(["readelf", "-d", str(LIB)], capture_output=True, text=True).stdout
+    assert "torch" not in needed and "c10" not in needed
+    assert "libamdhip64" in needed
+
+
+@pytest.mark.skipif(not LIB.exists(), reason="library not built")
+def test_workspace_queries_are_host_only():
+    lib = ctypes.CDLL(str(LIB))
+    fn = lib.pe_update_workspace_bytes
+    fn.restype = ctypes.c_int64
+    fn.argtypes = [ctypes.c_int64] * 4
+    small, large = fn(1, 2, 3, 4), fn(8, 41, 47, 133)
+    assert 0 < small <= large and large % 256 == 0
