@@ -1,0 +1,212 @@
+"""Benchmark of the MI355X evaluation path (one JSON line on stdout).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1]
+
+A step is one reset -> update -> compute pass of the public API over one batch
+of synthetic input already resident in HBM (the reference's timing protocol:
+DEVELOPMENT.md:47-49).  ``value`` is images/sec over exactly K timed steps.
+The dominant kernel's average duration is measured with HIP events recorded by
+the library on the kernel's own launch stream during the timed steps and
+reported against the MI355X HBM roofline; the CPU oracle is timed on a bounded
+sample of the same workload on the host cores as ``cpu_baseline``.
+
+Other modes: --copy-calibration runs a plain copy of known size (the reference
+point that scales rocprofv3 FETCH_SIZE/WRITE_SIZE into bytes).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+for _path in (ROOT, ROOT / "panoptic-evaluator_amd"):
+    if str(_path) not in sys.path:
+        sys.path.insert(0, str(_path))
+
+METRIC = "images/sec PQ+mIoU+mask AP, batched vs plain on 1 GPU; p50/p99 reward latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*args):
+    print(*args, file=sys.stderr, flush=True)
+
+
+# ------------------------------------------------------------------ configs
+
+class PanopticConfig:
+    """Plain PanopticEvaluator reset/update/compute over one batch."""
+
+    dominant = "joint_histogram"
+    dtype = "int32"
+
+    def __init__(self, name, maker, classes, things, description):
+        self.name, self.maker, self.classes, self.things = name, maker, classes, things
+        self.description = description
+
+    def setup(self, device):
+        import workloads
+        import panoptic_evaluator as pe
+        self.pair = self.maker(device=device)
+        self.isthing = workloads.isthing_vector(self.classes, self.things)
+        self.evaluator = pe.PanopticEvaluator(self.classes, isthing=self.isthing, validate=False)
+        self.target = pe.PanopticBatch(self.pair.gt, self.pair.gc, self.pair.crowd)
+        self.prediction = pe.PanopticBatch(self.pair.pred, self.pair.pc)
+        B, H, W = self.pair.gt.shape
+        self.images = B
+        self.bytes_per_launch = 8 * B * H * W  # int32 GT + prediction maps
+        self.config = {"workload": self.description, "images": B, "height": H, "width": W,
+                       "classes": self.classes, "gt_slots": int(self.pair.gc.shape[1]),
+                       "pred_slots": int(self.pair.pc.shape[1]), "validate": False}
+
+    def step(self):
+        self.evaluator.reset()
+        self.evaluator.update(self.prediction, self.target)
+        return self.evaluator.compute()
+
+    def cpu_sample(self, budget_s):
+        from oracle import panoptic as opan
+        arrays = [t.cpu().numpy() for t in (self.pair.gt, self.pair.pred, self.pair.gc, self.pair.pc,
+                                            self.pair.crowd)]
+        done, start = 0, time.perf_counter()
+        state = opan.PanopticState(self.classes, self.isthing)
+        while done < self.images:
+            sl = slice(done, done + 1)
+            state.update(*(a[sl] for a in arrays), validate=False)
+            done += 1
+            if time.perf_counter() - start > budget_s:
+                break
+        state.compute()
+        elapsed = time.perf_counter() - start
+        return done / elapsed, f"{done} of {self.images} images, oracle update+compute"
+
+
+def make_configs():
+    import workloads
+    return {
+        "c0": PanopticConfig("c0", workloads.config0, 133, 80,
+                             "c0: PanopticEvaluator, 8 x 512x512, 133 classes, 40 Voronoi seeds"),
+        "c3": PanopticConfig("c3", workloads.config3, 19, 8,
+                             "c3: PanopticEvaluator, 64 x 1024x2048, 19 classes, 150 Voronoi seeds"),
+    }
+
+
+DEFAULT_CONFIG = "c3"
+
+
+# ------------------------------------------------------------------ main
+
+def copy_calibration():
+    """Plain device copy of a known size: the reference point for rocprofv3
+    FETCH_SIZE / WRITE_SIZE scaling (1 GiB read + 1 GiB written per launch)."""
+    import torch
+    src = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4,
+                      "launches": 3}))
+
+
+def traffic_for(config_name):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 counter summary (profiles/*traffic*.json), or None."""
+    path = ROOT / "profiles" / f"traffic_{config_name}.json"
+    if not path.exists():
+        return None
+    try:
+        return json.loads(path.read_text()).get("traffic_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main(argv=None):
+    parser = argparse.ArgumentParser(description=__doc__)
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=20)
+    parser.add_argument("--warmup", type=int, default=3)
+    parser.add_argument("--config", default=DEFAULT_CONFIG)
+    parser.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU oracle sample")
+    parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--copy-calibration", action="store_true")
+    args = parser.parse_args(argv)
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the evaluation path shards by image with one "
+              "exchange at compute(), and no multi-GPU figure is produced here.", file=sys.stderr)
+        return 2
+    import torch
+    if args.copy_calibration:
+        copy_calibration()
+        return 0
+    import panoptic_evaluator as pe  # noqa: F401
+    from panoptic_evaluator import _native
+
+    configs = make_configs()
+    cfg = configs[args.config]
+    log(f"[bench] setting up {cfg.name} on cuda:0")
+    cfg.setup("cuda")
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        cfg.step()
+    torch.cuda.synchronize()
+    lib = _native.library()
+    lib.pe_profile_reset()
+    lib.pe_profile_enable(1)
+    log(f"[bench] timing {args.steps} steps")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        cfg.step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    lib.pe_profile_enable(0)
+    kernel_ms, launches = _native.profile_read(cfg.dominant)
+    avg_ms = kernel_ms / max(launches, 1)
+    achieved = cfg.bytes_per_launch / (avg_ms * 1e-3) / 1e9 if launches else None
+    images_per_s = cfg.images * args.steps / elapsed
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] timing the CPU oracle sample")
+        rate, sample = cfg.cpu_sample(args.cpu_budget)
+        cpu = {"value": round(rate, 4), "unit": "images/sec", "cores": 1, "kind": "port",
+               "sample": sample, "host_threads_available": os.cpu_count()}
+    traffic = traffic_for(cfg.name)
+    line = {
+        "metric": METRIC,
+        "value": round(images_per_s, 3),
+        "unit": "images/sec",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": cfg.dtype,
+        "data": "synthetic",
+        "config": cfg.config,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": cfg.dominant,
+            "achieved": None if achieved is None else round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": cfg.bytes_per_launch,
+            "avg_kernel_ms": round(avg_ms, 5),
+            "launches": launches,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

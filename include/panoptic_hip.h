@@ -27,6 +27,15 @@ extern "C" {
 const char* pe_last_error(void);
 int pe_abi_version(void);
 
+/* Benchmark timing hooks (no counterpart in the reference): while enabled,
+ * HIP events are recorded on the launch stream around every launch of the
+ * library's dominant kernels (by kernel name, e.g. "joint_histogram");
+ * pe_profile_read synchronizes on them and returns the summed milliseconds and
+ * the launch count.  For eager runs only (not inside HIP graph capture). */
+int pe_profile_enable(int on);
+int pe_profile_reset(void);
+int pe_profile_read(const char* kernel, double* total_ms, int64_t* launches);
+
 /* ------------------------------------------------------------ panoptic ---- */
 
 /* Scratch bytes pe_update needs beyond the reference's pairs/areas/matched. */
@@ -46,10 +55,16 @@ int64_t pe_update_workspace_bytes(int64_t B, int64_t G, int64_t P, int64_t C);
  *   tp, fp, fn, iou_sum   C (or B x C when reward != 0) accumulators.
  *   confusion       C x (C + 1) int64 accumulator (ignored when reward != 0).
  *   error           1 int32 scratch.
+ *   iou_exact       C x 2 uint64 accumulator (dataset mode; NULL for reward):
+ *                   the exact sum of matched IoUs in units of 2^-53 (every
+ *                   matched IoU is an exact multiple), as low-32-bit and
+ *                   high-bit partial sums; iou_sum is rewritten as its
+ *                   correctly rounded FP64 value after every update.
  *   reward != 0 selects per-image statistics (panoptic_quality): empty
  *   prediction slots are ignored and no confusion is accumulated.
  * validate != 0 synchronizes the stream once to read the error bitmask.
- * iou_sum is accumulated in (image, prediction slot) order, deterministic. */
+ * Dataset iou_sum is thus independent of batch composition and launch
+ * order; per-image (reward) sums fold in prediction-slot order. */
 int pe_update(const void* gt, int gt_bytes, const void* pred, int pred_bytes,
               int64_t B, int64_t H, int64_t W,
               const int32_t* gc, const int32_t* pc, const uint8_t* crowd,
@@ -58,7 +73,7 @@ int pe_update(const void* gt, int gt_bytes, const void* pred, int pred_bytes,
               int64_t* tp, int64_t* fp, int64_t* fn, double* iou_sum, int64_t* confusion,
               const int64_t* gt_ids, const int32_t* gt_slots,
               const int64_t* pred_ids, const int32_t* pred_slots,
-              int32_t* error, void* workspace, int64_t workspace_bytes,
+              int32_t* error, uint64_t* iou_exact, void* workspace, int64_t workspace_bytes,
               int validate, int reward, void* stream);
 
 /* FP64 finaliser over K independent accumulator rows.  Replaces ``compute``
@@ -88,6 +103,109 @@ int pe_image_pq(const int64_t* tp, const int64_t* fp, const int64_t* fn, const d
 int pe_pack(const void* maps, int map_bytes, int64_t B, int64_t HW,
             const int64_t* ids, const int32_t* slots, int64_t S,
             int32_t* out, int32_t* error, void* stream);
+
+/* ------------------------------------------------------------ instance ---- */
+
+/* Mask x mask intersections with int8 MFMA (bool bytes are 0/1 int8):
+ * inter[d, g] = #pixels set in both pm[d] and gm[g] (D x G int32), plus the
+ * mask areas pa (D int32) and ga (G int32) from the same pass (either may be
+ * NULL).  pm: D x HW bool, gm: G x HW bool.  Replaces pack_masks + overlap of
+ * instance_update_cuda (instance_kernel.cu:10-39, :282-309; CPU _match_cpu,
+ * instance.py:255-264). */
+int pe_mask_overlap(const uint8_t* pm, int64_t D, const uint8_t* gm, int64_t G, int64_t HW,
+                    int32_t* inter, int32_t* pa, int32_t* ga, void* stream);
+
+/* Detection x target-slot intersections for a batch of images sharing one
+ * size: pairs[row, s] = #pixels of mask order[row] (within its image) whose
+ * target slot is s; rows are grouped per image by det_off (B + 1, device).
+ * masks: device array of B pointers to D_b x HW bool tensors; target: B x HW
+ * int32 slots; pairs: (sum D_b) x S int64.  aligned != 0 promises HW % 16 == 0
+ * and 16-byte aligned mask pointers.  Replaces ``instance_histogram``
+ * (bindings.cpp:18-19; instance_histogram_cuda, instance_kernel.cu:335-363). */
+int pe_detection_histogram(int64_t B, const uint8_t* const* masks, const int64_t* det_off,
+                           const int64_t* order, const int32_t* target, int64_t HW, int64_t S,
+                           int64_t max_dets, int64_t total_dets, int64_t* pairs, int aligned,
+                           void* stream);
+
+/* COCO greedy matching of a batch of images.  Per image b, detections
+ * det_off[b]..det_off[b+1] are in score order (det_cls int32, det_area f64
+ * pixel areas) and ground truth gt_off[b]..gt_off[b+1] in metadata order
+ * (gt_cls int32, -1 = not a candidate; gt_area f64 pixel areas; gt_crowd;
+ * gt_ann f64 annotation areas for the area ranges).  Intersections of image b
+ * start at inter_off[b] as a D_b x G_b block (int32 or int64, inter_bytes);
+ * det_row (may be NULL) gives the block row of each detection.  Outputs:
+ * status (sum D) x 40 int8 [area*10 + threshold], ranks (sum D) int32 (1..100,
+ * else 0), counts C x 4 int64 (accumulated).  guard (may be NULL): nothing is
+ * written when *guard != 0.  Replaces ``instance_update`` /
+ * ``instance_match_histogram`` matching (bindings.cpp:13-15, :20-22;
+ * match_instances, instance_kernel.cu:41-174; CPU _match_ious_cpu,
+ * instance.py:267-297). */
+int pe_coco_match(int64_t B, int64_t C, const int64_t* det_off, const int64_t* gt_off,
+                  const int64_t* inter_off, const int32_t* det_cls, const double* det_area,
+                  const int64_t* det_row, const int32_t* gt_cls, const double* gt_area,
+                  const uint8_t* gt_crowd, const double* gt_ann, const void* inter, int inter_bytes,
+                  int64_t total_dets, int64_t max_gt, int8_t* status, int32_t* ranks, int64_t* counts,
+                  const int32_t* guard, void* stream);
+
+/* Scratch bytes of pe_ap_compute for N detection records and C classes. */
+int64_t pe_ap_workspace_bytes(int64_t N, int64_t C);
+
+/* COCO AP/AR accumulation and summaries.  Records (scores f64, classes int32,
+ * ranks int32, status N x 40 int8) in any order; they are stably sorted by
+ * descending score on the device.  counts: C x 4 positives.  Outputs:
+ * precision 10 x 101 x C x 4 x 3 f64, recall 10 x C x 4 x 3 f64, and summary
+ * (12 f64: ap, ap50, ap75, ap_small, ap_medium, ap_large, ar1, ar10, ar100,
+ * ar_small, ar_medium, ar_large; -1 when undefined).  Replaces
+ * ``instance_compute`` (bindings.cpp:16-17; instance_compute_cuda,
+ * instance_kernel.cu:176-209, :311-333; CPU instance.py:205-248, :300-331). */
+int pe_ap_compute(int64_t N, const double* scores, const int32_t* classes, const int32_t* ranks,
+                  const int8_t* status, const int64_t* counts, int64_t C, double* precision,
+                  double* recall, double* summary, void* workspace, int64_t workspace_bytes,
+                  void* stream);
+
+/* out[r] = sum of int64 row r as f64 (detection areas of a shared target
+ * histogram, combined.py:107-108). */
+int pe_row_sums(const int64_t* pairs, int64_t rows, int64_t cols, double* out, void* stream);
+
+/* Instance input validation into an int32 error bitmask (zeroed first):
+ * 1 non-finite score / annotation area or negative annotation area; 2 class
+ * outside 0..C-1; 4 mask without pixels; 8 detection class not a thing
+ * (things may be NULL); 16 non-finite or negative instance_areas.  Any array
+ * may be NULL to skip its check (instance.py:104-126; combined.py:47-61,
+ * :92-93, :109-110). */
+int pe_instance_checks(int64_t n_det, const int64_t* det_cls, const double* scores, const double* det_area,
+                       int64_t n_gt, const int64_t* gt_cls, const double* gt_area, const double* ann,
+                       int64_t C, const uint8_t* things, int64_t n_inst, const double* inst_areas,
+                       int32_t* error, void* stream);
+
+/* -------------------------------------------------------------- prefix ---- */
+
+/* Scratch bytes of pe_prefix: sum_n = total masks, sum_tri = sum over images
+ * of N_b (N_b + 1) / 2. */
+int64_t pe_prefix_workspace_bytes(int64_t B, int64_t G, int64_t C, int64_t T, int64_t sum_n, int64_t sum_tri);
+
+/* Generation-order prefix rewards for B images sharing one target size.
+ * target: B x HW int32 slots; masks: device array of B pointers to N_b x HW
+ * bool masks in generation order (n_off: B + 1 device offsets; cls: sum N
+ * int32 classes); gc/crowd: B x G target metadata; things: C uint8;
+ * thresholds: T f64 IoU thresholds; k_off (B + 1) / points (sum K int32):
+ * strictly increasing checkpoints per image, rows of every output.
+ * Outputs per checkpoint row r < total_rows: tp/fp/fn (C int64), sums (C
+ * f64), confusion (C x (C + 1) int64) of the visible canvas (feed pe_compute
+ * with K = total_rows), and out_itp/out_ifp/out_ifn (T x C int64) and
+ * precision/recall (T x C f64) of greedy instance matching.  error bits: 1
+ * target slot outside 0..G-1, 2 bad target classes / slot 0, 4 active target
+ * metadata and pixels disagree, 8 generated class outside 0..C-1, 16 empty
+ * mask.  Replaces ``prefix`` (bindings.cpp:24-26; prefix_cuda,
+ * prefix_kernel.cu:142-199; CPU segmentation_prefixes, prefix.py:11-122). */
+int pe_prefix(int64_t B, const int32_t* target, int64_t HW, int64_t G, int64_t C,
+              const uint8_t* const* masks, const int64_t* n_off, int64_t sum_n, int64_t sum_tri,
+              const int32_t* cls, const int32_t* gc, const uint8_t* crowd, const uint8_t* things,
+              const double* thresholds, int64_t T, const int64_t* k_off, const int32_t* points,
+              int64_t total_rows, int64_t* tp, int64_t* fp, int64_t* fn, double* sums,
+              int64_t* confusion, int64_t* out_itp, int64_t* out_ifp, int64_t* out_ifn,
+              double* precision, double* recall, int32_t* error, void* workspace,
+              int64_t workspace_bytes, int aligned, void* stream);
 
 #ifdef __cplusplus
 }

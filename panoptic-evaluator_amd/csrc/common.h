@@ -86,5 +86,23 @@ __device__ inline int lane_id() { return __lane_id(); }
 __global__ void zero_words(uint4* data, i64 count16, uint32_t* tail, i64 tail_words);
 
 int launch_zero(void* ptr, size_t bytes, hipStream_t stream);
+// Byte fill kernel (graph-capture safe replacement for hipMemsetD8Async).
+int launch_fill_u8(void* ptr, uint8_t value, size_t bytes, hipStream_t stream);
+
+}  // namespace pe
+
+namespace pe {
+
+// Benchmark timing: when enabled (pe_profile_enable), a pair of HIP events is
+// recorded on the launch stream around every launch of a named kernel.
+// Intended for eager runs; leave disabled while capturing HIP graphs.
+bool profiling_enabled();
+struct ProfileScope {
+  const char* name;
+  hipStream_t stream;
+  hipEvent_t start = nullptr;
+  ProfileScope(const char* kernel, hipStream_t s);
+  ~ProfileScope();
+};
 
 }  // namespace pe

@@ -66,8 +66,111 @@ int launch_zero(void* ptr, size_t bytes, hipStream_t stream) {
   return kOk;
 }
 
+__global__ void fill_bytes(uint8_t* data, uint8_t value, i64 count) {
+  const i64 stride = static_cast<i64>(gridDim.x) * blockDim.x;
+  for (i64 i = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) data[i] = value;
+}
+
+int launch_fill_u8(void* ptr, uint8_t value, size_t bytes, hipStream_t stream) {
+  if (!bytes) return kOk;
+  int blocks = static_cast<int>(std::min<i64>((static_cast<i64>(bytes) + 255) / 256, 4 * compute_units()));
+  fill_bytes<<<blocks, 256, 0, stream>>>(static_cast<uint8_t*>(ptr), value, static_cast<i64>(bytes));
+  PE_LAUNCH_CHECK();
+  return kOk;
+}
+
 }  // namespace pe
 
 extern "C" const char* pe_last_error(void) { return pe::error_buffer(); }
 
 extern "C" int pe_abi_version(void) { return PE_ABI_VERSION; }
+
+// ------------------------------------------------------------ timing hooks
+
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace pe {
+namespace {
+struct Profile {
+  std::mutex lock;
+  bool enabled = false;
+  std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> spans;
+  std::vector<hipEvent_t> pool;
+  hipEvent_t take() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+};
+Profile& profile() {
+  static Profile p;
+  return p;
+}
+}  // namespace
+
+bool profiling_enabled() { return profile().enabled; }
+
+ProfileScope::ProfileScope(const char* kernel, hipStream_t s) : name(kernel), stream(s) {
+  Profile& p = profile();
+  if (!p.enabled) return;
+  std::lock_guard<std::mutex> guard(p.lock);
+  start = p.take();
+  if (start && hipEventRecord(start, stream) != hipSuccess) start = nullptr;
+}
+
+ProfileScope::~ProfileScope() {
+  if (!start) return;
+  Profile& p = profile();
+  std::lock_guard<std::mutex> guard(p.lock);
+  hipEvent_t stop = p.take();
+  if (stop && hipEventRecord(stop, stream) == hipSuccess) p.spans[name].emplace_back(start, stop);
+}
+
+}  // namespace pe
+
+extern "C" int pe_profile_enable(int on) {
+  pe::profile().enabled = on != 0;
+  return 0;
+}
+
+extern "C" int pe_profile_reset(void) {
+  pe::Profile& p = pe::profile();
+  std::lock_guard<std::mutex> guard(p.lock);
+  for (auto& item : p.spans)
+    for (auto& span : item.second) {
+      p.pool.push_back(span.first);
+      p.pool.push_back(span.second);
+    }
+  p.spans.clear();
+  return 0;
+}
+
+extern "C" int pe_profile_read(const char* kernel, double* total_ms, int64_t* launches) {
+  pe::Profile& p = pe::profile();
+  std::lock_guard<std::mutex> guard(p.lock);
+  double total = 0.0;
+  int64_t count = 0;
+  auto it = p.spans.find(kernel);
+  if (it != p.spans.end()) {
+    for (auto& span : it->second) {
+      if (hipEventSynchronize(span.second) != hipSuccess)
+        return pe::fail(pe::kHipError, "event synchronize failed");
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, span.first, span.second) != hipSuccess)
+        return pe::fail(pe::kHipError, "event elapsed time failed");
+      total += ms;
+      ++count;
+    }
+  }
+  if (total_ms) *total_ms = total;
+  if (launches) *launches = count;
+  return 0;
+}

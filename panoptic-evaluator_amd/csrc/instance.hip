@@ -1,0 +1,802 @@
+// Instance path on gfx950: mask intersections (int8 MFMA), detection x
+// target-slot histograms, COCO greedy matching and AP/AR accumulation.
+//
+// Behaviour follows the reference's CPU path (instance.py:255-331 matching and
+// accumulation, :230-248 summaries, combined.py:63-161 shared-target rules)
+// and its native boundary (instance_kernel.cu:282-381).
+//
+// Design (MI355X-first):
+//  * mask x mask intersections: bool masks are bytes 0/1, i.e. valid int8
+//    MFMA operands as they lie in HBM.  One wave owns a 16-detection tile and
+//    up to 8 GT tiles for a K (pixel) chunk; each K step is one 16x16x64 int8
+//    MFMA per GT tile with exact int32 accumulation.  Partial tiles are added
+//    atomically (counts never exceed the pixel count).  Mask areas come from
+//    the same loads (byte sums), so every mask byte is read from HBM once.
+//  * detection x slot histogram: one workgroup owns a pixel tile of one image,
+//    keeps the target slots of the tile in registers and streams every
+//    detection mask over it with 16-byte loads (all-zero chunks skip at once),
+//    counting (detection, slot) in packed 16-bit LDS bins.
+//  * matching: one wave per (image, class); lane L < 40 owns the chain
+//    (area range L / 10, IoU threshold L % 10); detections are processed in
+//    score order with a per-chain GT bitmask in LDS.
+//  * AP: two stable radix sorts (score descending, then class) group each
+//    class's detections in global score order; one thread per
+//    (class, area, maxDets, threshold) runs the cumulative TP/FP scan forward
+//    and the precision envelope backward, matching the reference's
+//    searchsorted/cummax result exactly.
+#include <hipcub/hipcub.hpp>
+
+#include <cfloat>
+#include <climits>
+#include <cmath>
+
+#include "common.h"
+#include "../../include/panoptic_hip.h"
+
+namespace pe {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// --------------------------------------------------------- MFMA overlaps
+
+constexpr int kOverlapGTiles = 8;     // GT tiles (of 16) per wave
+constexpr int kOverlapChunk = 16384;  // pixels per wave (multiple of 64)
+
+__device__ __forceinline__ int byte_sum4(int w) {
+  // four bytes, each 0 or 1
+  return static_cast<int>((static_cast<unsigned>(w) * 0x01010101u) >> 24);
+}
+
+__device__ __forceinline__ v4i load16(const uint8_t* row, i64 k, i64 HW, bool valid_row, bool aligned) {
+  v4i v = {0, 0, 0, 0};
+  if (!valid_row || k >= HW) return v;
+  if (aligned && k + 16 <= HW) return *reinterpret_cast<const v4i*>(row + k);
+  uint8_t bytes[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) bytes[j] = k + j < HW ? row[k + j] : 0;
+  __builtin_memcpy(&v, bytes, 16);
+  return v;
+}
+
+// inter[d, g] += popcount(pm[d] & gm[g]) over this wave's K chunk.
+__global__ __launch_bounds__(64) void mask_overlap_mfma(const uint8_t* pm, int D, const uint8_t* gm, int G,
+                                                        i64 HW, int32_t* inter, int32_t* pa, int32_t* ga,
+                                                        int aligned) {
+  const int lane = threadIdx.x;
+  const int chunk = blockIdx.x, dtile = blockIdx.y, ggroup = blockIdx.z;
+  const int row = lane & 15, kgroup = lane >> 4;
+  const int d = dtile * 16 + row;
+  const bool dvalid = d < D;
+  const uint8_t* arow = pm + static_cast<i64>(dvalid ? d : 0) * HW;
+  const uint8_t* brow[kOverlapGTiles];
+  bool bvalid[kOverlapGTiles];
+#pragma unroll
+  for (int t = 0; t < kOverlapGTiles; ++t) {
+    const int g = (ggroup * kOverlapGTiles + t) * 16 + row;
+    bvalid[t] = g < G;
+    brow[t] = gm + static_cast<i64>(bvalid[t] ? g : 0) * HW;
+  }
+  int tiles = G - ggroup * kOverlapGTiles * 16;
+  tiles = tiles <= 0 ? 0 : min(kOverlapGTiles, (tiles + 15) / 16);
+  v4i acc[kOverlapGTiles];
+#pragma unroll
+  for (int t = 0; t < kOverlapGTiles; ++t) acc[t] = v4i{0, 0, 0, 0};
+  int asum = 0, bsum[kOverlapGTiles];
+#pragma unroll
+  for (int t = 0; t < kOverlapGTiles; ++t) bsum[t] = 0;
+  const bool count_a = pa != nullptr && ggroup == 0;
+  const bool count_b = ga != nullptr && dtile == 0;
+  const i64 k_begin = static_cast<i64>(chunk) * kOverlapChunk;
+  const i64 k_end = min(k_begin + kOverlapChunk, HW);
+  const bool al = aligned != 0;
+  for (i64 k0 = k_begin; k0 < k_end; k0 += 64) {
+    const i64 k = k0 + kgroup * 16;
+    v4i a = load16(arow, k, k_end, dvalid, al);
+    if (count_a) asum += byte_sum4(a.x) + byte_sum4(a.y) + byte_sum4(a.z) + byte_sum4(a.w);
+#pragma unroll
+    for (int t = 0; t < kOverlapGTiles; ++t) {
+      if (t >= tiles) break;
+      v4i b = load16(brow[t], k, k_end, bvalid[t], al);
+      if (count_b) bsum[t] += byte_sum4(b.x) + byte_sum4(b.y) + byte_sum4(b.z) + byte_sum4(b.w);
+      acc[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc[t], 0, 0, 0);
+    }
+  }
+  // D fragment: lane holds rows 4*(lane/16) + r, column lane % 16.
+#pragma unroll
+  for (int t = 0; t < kOverlapGTiles; ++t) {
+    if (t >= tiles) break;
+    const int g = (ggroup * kOverlapGTiles + t) * 16 + (lane & 15);
+    if (g >= G) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int dd = dtile * 16 + 4 * (lane >> 4) + r;
+      const int v = acc[t][r];
+      if (dd < D && v) atomicAdd(inter + static_cast<i64>(dd) * G + g, v);
+    }
+  }
+  // Areas: the four lanes of a row (kgroups) hold disjoint pixels.
+  if (count_a) {
+    asum += __shfl_xor(asum, 16, kWave);
+    asum += __shfl_xor(asum, 32, kWave);
+    if (kgroup == 0 && dvalid && asum) atomicAdd(pa + d, asum);
+  }
+  if (count_b) {
+#pragma unroll
+    for (int t = 0; t < kOverlapGTiles; ++t) {
+      if (t >= tiles) break;
+      int s = bsum[t];
+      s += __shfl_xor(s, 16, kWave);
+      s += __shfl_xor(s, 32, kWave);
+      const int g = (ggroup * kOverlapGTiles + t) * 16 + row;
+      if (kgroup == 0 && g < G && s) atomicAdd(ga + g, s);
+    }
+  }
+}
+
+// ------------------------------------------------- detection x slot counts
+
+constexpr int kDetThreads = 256;
+constexpr int kDetTile = 16 * kDetThreads * 2;  // pixels per workgroup (2 x 16 B per lane)
+constexpr size_t kDetMaxLds = 96 * 1024;
+
+struct DetHistArgs {
+  const uint8_t* const* masks;   // B device pointers (D_b x HW bools)
+  const i64* det_off;            // B + 1
+  const i64* order;              // per detection: mask index within its image
+  const int32_t* target;         // B x HW slots
+  i64 HW;
+  int S;
+  int tiles;                     // pixel tiles per image
+  int dgroups;                   // detection groups per image
+  int group;                     // detections per group
+  u64* pairs;                    // (sum D) x S
+  int aligned;
+};
+
+// Slot keys of one 16-pixel chunk: counts runs of equal slots per lane.
+__device__ __forceinline__ void count_chunk(v4i m, const int* slots, int S, int rowbase, uint32_t* bins) {
+  const uint32_t words[4] = {static_cast<uint32_t>(m.x), static_cast<uint32_t>(m.y),
+                             static_cast<uint32_t>(m.z), static_cast<uint32_t>(m.w)};
+  int run_key = -1, run = 0;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const bool on = (words[j >> 2] >> ((j & 3) * 8)) & 0xFFu;
+    const int s = slots[j];
+    const int key = (on && s >= 0 && s < S) ? rowbase + s : -1;
+    if (key != run_key) {
+      if (run_key >= 0) atomicAdd(bins + (run_key >> 1), static_cast<uint32_t>(run) << ((run_key & 1) * 16));
+      run_key = key;
+      run = 0;
+    }
+    run += key >= 0;
+  }
+  if (run_key >= 0) atomicAdd(bins + (run_key >> 1), static_cast<uint32_t>(run) << ((run_key & 1) * 16));
+}
+
+__global__ __launch_bounds__(kDetThreads) void detection_histogram(DetHistArgs a) {
+  extern __shared__ uint4 lds4[];
+  uint32_t* bins = reinterpret_cast<uint32_t*>(lds4);
+  const int per_image = a.tiles * a.dgroups;
+  const int b = blockIdx.x / per_image;
+  const int rest = blockIdx.x - b * per_image;
+  const int tile = rest / a.dgroups, dgroup = rest - tile * a.dgroups;
+  const i64 first = a.det_off[b];
+  const int D = static_cast<int>(a.det_off[b + 1] - first);
+  const int d0 = dgroup * a.group;
+  const int nd = min(a.group, D - d0);
+  if (nd <= 0) return;
+  const int words = (nd * a.S + 1) / 2;
+  const int words4 = (words + 3) / 4;
+  for (int w = threadIdx.x; w < words4; w += kDetThreads) lds4[w] = make_uint4(0, 0, 0, 0);
+  __syncthreads();
+  const i64 p0 = static_cast<i64>(tile) * kDetTile;
+  const int32_t* target = a.target + static_cast<i64>(b) * a.HW;
+  const uint8_t* masks = a.masks[b];
+  // Each lane owns two 16-pixel chunks of the tile.
+  int slots[2][16];
+  i64 base[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    base[h] = p0 + (static_cast<i64>(h) * kDetThreads + threadIdx.x) * 16;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) slots[h][j] = base[h] + j < a.HW ? target[base[h] + j] : -1;
+  }
+  for (int r = 0; r < nd; ++r) {
+    const uint8_t* mask = masks + a.order[first + d0 + r] * a.HW;
+    v4i m[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) m[h] = load16(mask, base[h], a.HW, true, a.aligned != 0);
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+      if ((m[h].x | m[h].y | m[h].z | m[h].w) != 0) count_chunk(m[h], slots[h], a.S, r * a.S, bins);
+  }
+  __syncthreads();
+  u64* out = a.pairs + (first + d0) * a.S;
+  for (int w = threadIdx.x; w < words; w += kDetThreads) {
+    const uint32_t v = bins[w];
+    if (!v) continue;
+    const uint32_t lo = v & 0xFFFFu, hi = v >> 16;
+    if (lo) atomicAdd(out + 2 * w, static_cast<u64>(lo));
+    if (hi) atomicAdd(out + 2 * w + 1, static_cast<u64>(hi));
+  }
+}
+
+// ------------------------------------------------------------ COCO matching
+
+struct MatchArgs {
+  int B, C;
+  const i64* det_off;
+  const i64* gt_off;
+  const i64* inter_off;
+  const int32_t* det_cls;     // score order
+  const double* det_area;     // pixel areas, score order
+  const i64* det_row;         // row of the intersection block for a detection (NULL: identity)
+  const int32_t* gt_cls;      // -1: not a candidate
+  const double* gt_area;      // pixel areas (IoU)
+  const uint8_t* gt_crowd;
+  const double* gt_ann;       // annotation areas (area ranges)
+  const void* inter;          // per image D_b x G_b intersections
+  int inter_bytes;            // 4 or 8
+  int8_t* status;             // (sum D) x 40
+  int32_t* ranks;             // sum D
+  i64* counts;                // C x 4 (accumulated)
+  int max_gt;
+  int words;                  // used-bitmask words per chain
+  const int* guard;           // skip everything when *guard != 0
+};
+
+__device__ __forceinline__ void area_range(int a, double& lo, double& hi) {
+  lo = a == 2 ? 1024.0 : a == 3 ? 9216.0 : 0.0;
+  hi = a == 1 ? 1024.0 : a == 2 ? 9216.0 : 1e10;
+}
+
+__device__ __forceinline__ double iou_threshold(int t) {
+  // COCO's np.linspace(.5, .95, 10) values (instance.py:252), incl. .8999999999999999
+  const double table[10] = {0.5, 0.55, 0.6, 0.65, 0.7, 0.75, 0.8, 0.85, 0.8999999999999999, 0.95};
+  return table[t];
+}
+
+__global__ __launch_bounds__(64) void coco_match(MatchArgs a) {
+  extern __shared__ uint4 lds4[];
+  if (a.guard && *a.guard) return;
+  const int b = blockIdx.x / a.C, c = blockIdx.x - (blockIdx.x / a.C) * a.C;
+  const int lane = threadIdx.x;
+  int* dets = reinterpret_cast<int*>(lds4);            // 100
+  int* gts = dets + 100;                               // max_gt
+  uint32_t* used = reinterpret_cast<uint32_t*>(gts + a.max_gt);  // 40 x words
+  const i64 d0 = a.det_off[b], g0 = a.gt_off[b];
+  const int D = static_cast<int>(a.det_off[b + 1] - d0);
+  const int G = static_cast<int>(a.gt_off[b + 1] - g0);
+  // Detections of class c in score order; the first 100 get ranks 1..100.
+  int nd = 0;
+  for (int base = 0; base < D; base += kWave) {
+    const int i = base + lane;
+    const bool sel = i < D && a.det_cls[d0 + i] == c;
+    const u64 mask = __ballot(sel);
+    if (sel) {
+      const int rank = nd + __popcll(mask & ((1ULL << lane) - 1));
+      if (rank < 100) {
+        dets[rank] = i;
+        a.ranks[d0 + i] = rank + 1;
+      }
+    }
+    nd += __popcll(mask);
+  }
+  nd = min(nd, 100);
+  // Ground truth of class c (metadata order) and positives per area range.
+  int ng = 0;
+  int positives[4] = {0, 0, 0, 0};
+  for (int base = 0; base < G; base += kWave) {
+    const int j = base + lane;
+    const bool sel = j < G && a.gt_cls[g0 + j] == c;
+    const u64 mask = __ballot(sel);
+    if (sel) gts[ng + __popcll(mask & ((1ULL << lane) - 1))] = j;
+    ng += __popcll(mask);
+    const bool eligible = sel && !a.gt_crowd[g0 + j];
+    const double area = eligible ? a.gt_ann[g0 + j] : 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      double lo, hi;
+      area_range(r, lo, hi);
+      positives[r] += __popcll(__ballot(eligible && area >= lo && area <= hi));
+    }
+  }
+  if (lane == 0 && ng)
+    for (int r = 0; r < 4; ++r)
+      if (positives[r]) atomicAdd(reinterpret_cast<u64*>(a.counts) + c * 4 + r, static_cast<u64>(positives[r]));
+  if (nd == 0) return;
+  for (int w = lane; w < 40 * a.words; w += kWave) used[w] = 0;
+  __syncthreads();
+  if (lane >= 40) return;
+  const int range = lane / 10, t = lane - (lane / 10) * 10;
+  double lo, hi;
+  area_range(range, lo, hi);
+  const double thr = iou_threshold(t);
+  uint32_t* taken = used + lane * a.words;
+  const i64 ib = a.inter_off[b];
+  for (int r = 0; r < nd; ++r) {
+    const int i = dets[r];
+    const i64 row = a.det_row ? a.det_row[d0 + i] : i;
+    const double pa = a.det_area[d0 + i];
+    int best = -1, best_ignore = 2;
+    double best_iou = thr;
+    for (int q = 0; q < ng; ++q) {
+      const int j = gts[q];
+      const bool crowd = a.gt_crowd[g0 + j] != 0;
+      if (!crowd && ((taken[q >> 5] >> (q & 31)) & 1u)) continue;
+      const i64 idx = ib + row * G + j;
+      const double inter = a.inter_bytes == 8 ? static_cast<double>(static_cast<const i64*>(a.inter)[idx])
+                                              : static_cast<double>(static_cast<const int32_t*>(a.inter)[idx]);
+      double uni = crowd ? pa : pa + a.gt_area[g0 + j] - inter;
+      if (uni < 1.0) uni = 1.0;
+      const double iou = inter / uni;
+      const double ann = a.gt_ann[g0 + j];
+      const int ignore = crowd || ann < lo || ann > hi;
+      // Non-ignored GT first, then larger IoU (>= running best), later
+      // metadata entry on equal IoU: COCO's scan order.
+      if (iou >= thr && (ignore < best_ignore || (ignore == best_ignore && iou >= best_iou))) {
+        best = q;
+        best_ignore = ignore;
+        best_iou = iou;
+      }
+    }
+    int8_t flag = -1;
+    if (best >= 0) {
+      taken[best >> 5] |= 1u << (best & 31);
+      flag = best_ignore ? -1 : 1;
+    } else if (lo <= pa && pa <= hi) {
+      flag = 0;
+    }
+    a.status[(d0 + i) * 40 + lane] = flag;
+  }
+}
+
+// ------------------------------------------------------------ AP / AR
+
+__device__ __forceinline__ u64 score_key_desc(double s) {
+  if (s == 0.0) s = 0.0;  // -0.0 and +0.0 compare equal in the reference's sort
+  u64 bits = __double_as_longlong(s);
+  u64 asc = (bits >> 63) ? ~bits : (bits | 0x8000000000000000ULL);
+  return ~asc;  // ascending radix order == descending score
+}
+
+__global__ void score_keys(const double* scores, i64 N, u64* keys, int32_t* idx) {
+  for (i64 i = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += static_cast<i64>(gridDim.x) * blockDim.x) {
+    keys[i] = score_key_desc(scores[i]);
+    idx[i] = static_cast<int32_t>(i);
+  }
+}
+
+// Class key of each score-sorted record; ranks outside 1..100 sort last (C).
+__global__ void class_keys(const int32_t* perm, const int32_t* classes, const int32_t* ranks, i64 N, int C,
+                           uint32_t* keys) {
+  for (i64 i = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += static_cast<i64>(gridDim.x) * blockDim.x) {
+    const int r = perm[i];
+    const int c = classes[r];
+    keys[i] = (ranks[r] > 0 && ranks[r] <= 100 && c >= 0 && c < C) ? static_cast<uint32_t>(c) : static_cast<uint32_t>(C);
+  }
+}
+
+__global__ void class_bounds(const uint32_t* keys, i64 N, int C, int32_t* start, int32_t* end) {
+  for (i64 i = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += static_cast<i64>(gridDim.x) * blockDim.x) {
+    const uint32_t k = keys[i];
+    if (k >= static_cast<uint32_t>(C)) continue;
+    if (i == 0 || keys[i - 1] != k) start[k] = static_cast<int32_t>(i);
+    if (i == N - 1 || keys[i + 1] != k) end[k] = static_cast<int32_t>(i + 1);
+  }
+}
+
+struct ApArgs {
+  int C;
+  const int32_t* order;      // record index per sorted position (class-grouped, score order)
+  const int32_t* start;      // C
+  const int32_t* end;        // C
+  const int32_t* ranks;
+  const int8_t* status;      // N x 40
+  const i64* counts;         // C x 4
+  double* precision;         // 10 x 101 x C x 4 x 3
+  double* recall;            // 10 x C x 4 x 3
+  double* partial;           // (C*120) x 2: valid-precision sum, count
+};
+
+// One thread per (t, c, a, m): instance.py:300-331 for one threshold row.
+__global__ __launch_bounds__(128) void ap_accumulate(ApArgs a) {
+  const int job = blockIdx.x * blockDim.x + threadIdx.x;
+  if (job >= a.C * 120) return;
+  const int m = job % 3, ar = (job / 3) % 4, c = (job / 12) % a.C, t = job / (12 * a.C);
+  const int limit = m == 0 ? 1 : m == 1 ? 10 : 100;
+  const i64 positives = a.counts[c * 4 + ar];
+  const i64 pstride = static_cast<i64>(a.C) * 12;
+  double* prec = a.precision + static_cast<i64>(t) * 101 * pstride + (c * 4 + ar) * 3 + m;
+  double* rec = a.recall + (static_cast<i64>(t) * a.C + c) * 12 + ar * 3 + m;
+  double* part = a.partial + 2 * static_cast<i64>(job);
+  if (!positives) {
+    for (int r = 0; r <= 100; ++r) prec[r * pstride] = -1.0;
+    *rec = -1.0;
+    part[0] = 0.0;
+    part[1] = 0.0;
+    return;
+  }
+  const int lane_flag = ar * 10 + t;
+  const int s0 = a.start[c], s1 = a.end[c];
+  // forward: totals
+  i64 tp = 0, fp = 0, n = 0;
+  for (int i = s0; i < s1; ++i) {
+    const int idx = a.order[i];
+    if (a.ranks[idx] > limit) continue;
+    const int f = a.status[static_cast<i64>(idx) * 40 + lane_flag];
+    tp += f == 1;
+    fp += f == 0;
+    ++n;
+  }
+  if (n == 0) {
+    for (int r = 0; r <= 100; ++r) prec[r * pstride] = 0.0;
+    *rec = 0.0;
+    part[0] = 0.0;
+    part[1] = 101.0;
+    return;
+  }
+  const double npos = static_cast<double>(positives);
+  // precision[r] = max(pr[pos_r:]) where pos_r is the first record whose
+  // recall reaches r/100 (0 when none does), filled backwards with the
+  // running maximum of precision.
+  double env = 0.0, sum = 0.0;
+  int r = 100;
+  i64 ctp = tp, cfp = fp, remaining = n;
+  while (r >= 0 && static_cast<double>(r) * 0.01 > static_cast<double>(ctp) / npos) {
+    prec[r * pstride] = 0.0;
+    --r;
+  }
+  for (int i = s1 - 1; i >= s0 && r >= 0; --i) {
+    const int idx = a.order[i];
+    if (a.ranks[idx] > limit) continue;
+    const double pr = static_cast<double>(ctp) / (static_cast<double>(ctp + cfp) + DBL_EPSILON);
+    env = fmax(env, pr);
+    const int f = a.status[static_cast<i64>(idx) * 40 + lane_flag];
+    ctp -= f == 1;
+    cfp -= f == 0;
+    --remaining;
+    // recall of the previous record; none before the first one
+    const double prev = remaining > 0 ? static_cast<double>(ctp) / npos : -1.0;
+    while (r >= 0 && static_cast<double>(r) * 0.01 > prev) {
+      prec[r * pstride] = env;
+      sum += env;
+      --r;
+    }
+  }
+  *rec = static_cast<double>(tp) / npos;
+  part[0] = sum;
+  part[1] = 101.0;
+}
+
+// instance.py:236-248: the 12 summaries as means over valid entries (>= 0).
+__global__ __launch_bounds__(256) void ap_summaries(int C, const double* precision_partial, const double* recall,
+                                                    double* out) {
+  __shared__ double red[24][256];
+  const int tid = threadIdx.x;
+  double acc[24];
+#pragma unroll
+  for (int j = 0; j < 24; ++j) acc[j] = 0.0;
+  const int jobs = C * 120;
+  for (int job = tid; job < jobs; job += 256) {
+    const int m = job % 3, ar = (job / 3) % 4, t = job / (12 * C);
+    const double s = precision_partial[2 * job], n = precision_partial[2 * job + 1];
+    if (m == 2) {
+      if (ar == 0) {
+        acc[0] += s; acc[1] += n;                       // ap
+        if (t == 0) { acc[2] += s; acc[3] += n; }       // ap50
+        if (t == 5) { acc[4] += s; acc[5] += n; }       // ap75
+      } else {
+        acc[4 + 2 * ar] += s; acc[5 + 2 * ar] += n;     // ap_small/medium/large: 6..11
+      }
+    }
+    const double rv = recall[(static_cast<i64>(t) * C + job / 12 % C) * 12 + ar * 3 + m];
+    if (rv >= 0) {
+      if (ar == 0) { acc[12 + 2 * m] += rv; acc[13 + 2 * m] += 1.0; }          // ar1/ar10/ar100: 12..17
+      else if (m == 2) { acc[16 + 2 * ar] += rv; acc[17 + 2 * ar] += 1.0; }   // ar_small..: 18..23
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 24; ++j) red[j][tid] = acc[j];
+  __syncthreads();
+  for (int stride = 128; stride > 0; stride >>= 1) {
+    if (tid < stride)
+#pragma unroll
+      for (int j = 0; j < 24; ++j) red[j][tid] += red[j][tid + stride];
+    __syncthreads();
+  }
+  if (tid < 12) out[tid] = red[2 * tid + 1][0] > 0 ? red[2 * tid][0] / red[2 * tid + 1][0] : -1.0;
+}
+
+}  // namespace pe
+
+using namespace pe;
+
+extern "C" int pe_mask_overlap(const uint8_t* pm, int64_t D, const uint8_t* gm, int64_t G, int64_t HW,
+                               int32_t* inter, int32_t* pa, int32_t* ga, void* stream_ptr) {
+  PE_REQUIRE(D >= 0 && G >= 0 && HW >= 0 && D < INT_MAX && G < INT_MAX && HW < INT_MAX, "invalid mask shapes");
+  hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+  int rc;
+  if (D * G && (rc = launch_zero(inter, 4 * static_cast<size_t>(D * G), stream))) return rc;
+  if (pa && D && (rc = launch_zero(pa, 4 * static_cast<size_t>(D), stream))) return rc;
+  if (ga && G && (rc = launch_zero(ga, 4 * static_cast<size_t>(G), stream))) return rc;
+  if (HW == 0 || (D == 0 && G == 0)) return kOk;
+  const bool aligned = HW % 16 == 0 && reinterpret_cast<uintptr_t>(pm) % 16 == 0 &&
+                       reinterpret_cast<uintptr_t>(gm) % 16 == 0;
+  const i64 chunks = (HW + kOverlapChunk - 1) / kOverlapChunk;
+  PE_REQUIRE(chunks < INT_MAX, "image too large");
+  // With D == 0 or G == 0 still count the other side's areas.
+  const i64 dtiles = std::max<i64>(1, (D + 15) / 16);
+  const i64 ggroups = std::max<i64>(1, (G + 16 * kOverlapGTiles - 1) / (16 * kOverlapGTiles));
+  PE_REQUIRE(dtiles < 65536 && ggroups < 65536, "too many masks");
+  dim3 grid(static_cast<unsigned>(chunks), static_cast<unsigned>(dtiles), static_cast<unsigned>(ggroups));
+  ProfileScope timing("mask_overlap_mfma", stream);
+  mask_overlap_mfma<<<grid, 64, 0, stream>>>(pm, static_cast<int>(D), gm, static_cast<int>(G), HW,
+                                             inter, pa, ga, aligned ? 1 : 0);
+  PE_LAUNCH_CHECK();
+  return kOk;
+}
+
+extern "C" int pe_detection_histogram(int64_t B, const uint8_t* const* masks, const int64_t* det_off,
+                                      const int64_t* order, const int32_t* target, int64_t HW, int64_t S,
+                                      int64_t max_dets, int64_t total_dets, int64_t* pairs, int aligned,
+                                      void* stream_ptr) {
+  PE_REQUIRE(B >= 0 && HW > 0 && S > 0 && S < INT_MAX && max_dets >= 0 && total_dets >= 0, "invalid shapes");
+  hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+  int rc;
+  if (total_dets && (rc = launch_zero(pairs, 8 * static_cast<size_t>(total_dets * S), stream))) return rc;
+  if (B == 0 || max_dets == 0) return kOk;
+  DetHistArgs a{};
+  a.masks = masks;
+  a.det_off = reinterpret_cast<const i64*>(det_off);
+  a.order = reinterpret_cast<const i64*>(order);
+  a.target = target;
+  a.HW = HW;
+  a.S = static_cast<int>(S);
+  a.tiles = static_cast<int>((HW + kDetTile - 1) / kDetTile);
+  // Detections per group: packed 16-bit counters of group x S within the LDS budget.
+  i64 group = static_cast<i64>(kDetMaxLds * 2 / 4) / S;  // bins per group / S
+  PE_REQUIRE(group >= 1, "too many target slots for the detection histogram");
+  group = std::min<i64>(group, max_dets);
+  a.group = static_cast<int>(group);
+  a.dgroups = static_cast<int>((max_dets + group - 1) / group);
+  a.pairs = reinterpret_cast<u64*>(pairs);
+  a.aligned = aligned;
+  const i64 blocks = B * a.tiles * a.dgroups;
+  PE_REQUIRE(blocks < INT_MAX, "too many workgroups");
+  const size_t lds = static_cast<size_t>((group * S + 1) / 2 + 3) / 4 * 16;
+  if (lds > 64 * 1024)
+    PE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(detection_histogram),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+  ProfileScope timing("detection_histogram", stream);
+  detection_histogram<<<static_cast<unsigned>(blocks), kDetThreads, lds, stream>>>(a);
+  PE_LAUNCH_CHECK();
+  return kOk;
+}
+
+extern "C" int pe_coco_match(int64_t B, int64_t C, const int64_t* det_off, const int64_t* gt_off,
+                             const int64_t* inter_off, const int32_t* det_cls, const double* det_area,
+                             const int64_t* det_row, const int32_t* gt_cls, const double* gt_area,
+                             const uint8_t* gt_crowd, const double* gt_ann, const void* inter, int inter_bytes,
+                             int64_t total_dets, int64_t max_gt, int8_t* status, int32_t* ranks, int64_t* counts,
+                             const int32_t* guard, void* stream_ptr) {
+  PE_REQUIRE(B >= 0 && C > 0 && C < INT_MAX && B * C < INT_MAX && max_gt >= 0, "invalid matching shapes");
+  PE_REQUIRE(inter_bytes == 4 || inter_bytes == 8, "intersections must be int32 or int64");
+  hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+  int rc;
+  // status = -1, ranks = 0 for every detection (ranks > 100 stay 0 as on the CPU path)
+  if (total_dets) {
+    if ((rc = launch_zero(ranks, 4 * static_cast<size_t>(total_dets), stream))) return rc;
+    if ((rc = launch_fill_u8(status, 0xFF, static_cast<size_t>(total_dets) * 40, stream))) return rc;
+  }
+  if (B == 0) return kOk;
+  MatchArgs a{};
+  a.B = static_cast<int>(B);
+  a.C = static_cast<int>(C);
+  a.det_off = reinterpret_cast<const i64*>(det_off);
+  a.gt_off = reinterpret_cast<const i64*>(gt_off);
+  a.inter_off = reinterpret_cast<const i64*>(inter_off);
+  a.det_cls = det_cls;
+  a.det_area = det_area;
+  a.det_row = reinterpret_cast<const i64*>(det_row);
+  a.gt_cls = gt_cls;
+  a.gt_area = gt_area;
+  a.gt_crowd = gt_crowd;
+  a.gt_ann = gt_ann;
+  a.inter = inter;
+  a.inter_bytes = inter_bytes;
+  a.status = status;
+  a.ranks = ranks;
+  a.counts = reinterpret_cast<i64*>(counts);
+  a.max_gt = static_cast<int>(max_gt);
+  a.words = static_cast<int>((max_gt + 31) / 32);
+  a.guard = guard;
+  const size_t lds = 4 * (100 + static_cast<size_t>(max_gt)) + 4 * 40 * static_cast<size_t>(a.words) + 16;
+  PE_REQUIRE(lds <= 150 * 1024, "too many ground-truth instances per image for the matcher");
+  if (lds > 64 * 1024)
+    PE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(coco_match), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               static_cast<int>(lds)));
+  ProfileScope timing("coco_match", stream);
+  coco_match<<<static_cast<unsigned>(B * C), 64, lds, stream>>>(a);
+  PE_LAUNCH_CHECK();
+  return kOk;
+}
+
+namespace {
+struct ApBuffers {
+  u64* keys_in;
+  u64* keys_out;
+  int32_t* idx_in;
+  int32_t* perm;
+  uint32_t* ckeys_in;
+  uint32_t* ckeys_out;
+  int32_t* order;
+  int32_t* start;
+  int32_t* end;
+  double* partial;
+  void* temp;
+  size_t temp_bytes;
+};
+
+size_t sort_temp_bytes(i64 N) {
+  size_t a = 0, b = 0;
+  hipcub::DeviceRadixSort::SortPairs(nullptr, a, static_cast<u64*>(nullptr), static_cast<u64*>(nullptr),
+                                     static_cast<int32_t*>(nullptr), static_cast<int32_t*>(nullptr),
+                                     static_cast<int>(N));
+  hipcub::DeviceRadixSort::SortPairs(nullptr, b, static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr),
+                                     static_cast<int32_t*>(nullptr), static_cast<int32_t*>(nullptr),
+                                     static_cast<int>(N));
+  return std::max(a, b);
+}
+
+ApBuffers carve_ap(void* workspace, i64 N, i64 C, size_t temp_bytes, size_t* used = nullptr) {
+  Carve carve(workspace);
+  ApBuffers w{};
+  w.keys_in = carve.take<u64>(N);
+  w.keys_out = carve.take<u64>(N);
+  w.idx_in = carve.take<int32_t>(N);
+  w.perm = carve.take<int32_t>(N);
+  w.ckeys_in = carve.take<uint32_t>(N);
+  w.ckeys_out = carve.take<uint32_t>(N);
+  w.order = carve.take<int32_t>(N);
+  w.start = carve.take<int32_t>(C);
+  w.end = carve.take<int32_t>(C);
+  w.partial = carve.take<double>(C * 240);
+  w.temp = carve.take<char>(temp_bytes);
+  w.temp_bytes = temp_bytes;
+  if (used) *used = carve.used;
+  return w;
+}
+}  // namespace
+
+extern "C" int64_t pe_ap_workspace_bytes(int64_t N, int64_t C) {
+  size_t used = 0;
+  carve_ap(nullptr, N, C, N > 0 ? sort_temp_bytes(N) : 0, &used);
+  return static_cast<int64_t>(align_up(used, 256));
+}
+
+extern "C" int pe_ap_compute(int64_t N, const double* scores, const int32_t* classes, const int32_t* ranks,
+                             const int8_t* status, const int64_t* counts, int64_t C, double* precision,
+                             double* recall, double* summary, void* workspace, int64_t workspace_bytes,
+                             void* stream_ptr) {
+  PE_REQUIRE(N >= 0 && N < INT_MAX && C > 0 && C <= INT_MAX / 240, "invalid AP shapes");
+  PE_REQUIRE(workspace_bytes >= pe_ap_workspace_bytes(N, C), "workspace too small");
+  hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+  const size_t temp = N > 0 ? sort_temp_bytes(N) : 0;
+  ApBuffers w = carve_ap(workspace, N, C, temp);
+  int rc;
+  if ((rc = launch_zero(w.start, 4 * static_cast<size_t>(C), stream))) return rc;
+  if ((rc = launch_zero(w.end, 4 * static_cast<size_t>(C), stream))) return rc;
+  if (N > 0) {
+    const int blocks = static_cast<int>(std::min<i64>((N + 255) / 256, 4096));
+    score_keys<<<blocks, 256, 0, stream>>>(scores, N, w.keys_in, w.idx_in);
+    PE_LAUNCH_CHECK();
+    size_t bytes = w.temp_bytes;
+    PE_HIP(hipcub::DeviceRadixSort::SortPairs(w.temp, bytes, w.keys_in, w.keys_out, w.idx_in, w.perm,
+                                              static_cast<int>(N), 0, 64, stream));
+    class_keys<<<blocks, 256, 0, stream>>>(w.perm, classes, ranks, N, static_cast<int>(C), w.ckeys_in);
+    PE_LAUNCH_CHECK();
+    bytes = w.temp_bytes;
+    const int class_bits = 32 - __builtin_clz(static_cast<unsigned>(C) | 1u);
+    PE_HIP(hipcub::DeviceRadixSort::SortPairs(w.temp, bytes, w.ckeys_in, w.ckeys_out, w.perm, w.order,
+                                              static_cast<int>(N), 0, class_bits, stream));
+    class_bounds<<<blocks, 256, 0, stream>>>(w.ckeys_out, N, static_cast<int>(C), w.start, w.end);
+    PE_LAUNCH_CHECK();
+  }
+  ApArgs a{};
+  a.C = static_cast<int>(C);
+  a.order = w.order;
+  a.start = w.start;
+  a.end = w.end;
+  a.ranks = ranks;
+  a.status = status;
+  a.counts = reinterpret_cast<const i64*>(counts);
+  a.precision = precision;
+  a.recall = recall;
+  a.partial = w.partial;
+  {
+    ProfileScope timing("ap_accumulate", stream);
+    ap_accumulate<<<static_cast<unsigned>((C * 120 + 127) / 128), 128, 0, stream>>>(a);
+    PE_LAUNCH_CHECK();
+  }
+  ap_summaries<<<1, 256, 0, stream>>>(static_cast<int>(C), w.partial, recall, summary);
+  PE_LAUNCH_CHECK();
+  return kOk;
+}
+
+// ------------------------------------------------------- small helpers
+
+namespace pe {
+
+// out[r] = sum of row r (int64 counts) as FP64: detection areas of the shared
+// target histogram (combined.py:107-108).
+__global__ __launch_bounds__(256) void row_sums(const i64* pairs, i64 rows, int cols, double* out) {
+  const i64 r = (static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (r >= rows) return;
+  i64 s = 0;
+  for (int c = lane; c < cols; c += kWave) s += pairs[r * cols + c];
+  s = wave_sum(s);
+  if (lane == 0) out[r] = static_cast<double>(s);
+}
+
+// Instance input validation as an error bitmask (instance.py:104-126,
+// combined.py:47-61, :92-93, :109-110):
+//   1 non-finite score or annotation area, or negative annotation area
+//   2 class outside 0..C-1        4 mask without pixels
+//   8 detection class not a thing 16 instance_areas not finite / negative
+__global__ __launch_bounds__(256) void instance_checks(i64 n_det, const i64* det_cls, const double* scores,
+                                                       const double* det_area, i64 n_gt, const i64* gt_cls,
+                                                       const double* gt_area, const double* ann, int C,
+                                                       const uint8_t* things, i64 n_inst, const double* inst_areas,
+                                                       int* error) {
+  int bits = 0;
+  const i64 stride = static_cast<i64>(gridDim.x) * blockDim.x;
+  for (i64 i = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; i < n_det; i += stride) {
+    const i64 c = det_cls[i];
+    if (c < 0 || c >= C) bits |= 2;
+    else if (things && !things[c]) bits |= 8;
+    if (scores && !isfinite(scores[i])) bits |= 1;
+    if (det_area && det_area[i] == 0.0) bits |= 4;
+  }
+  for (i64 j = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; j < n_gt; j += stride) {
+    const i64 c = gt_cls[j];
+    if (c < 0 || c >= C) bits |= 2;
+    if (gt_area && gt_area[j] == 0.0) bits |= 4;
+    if (ann && (!isfinite(ann[j]) || ann[j] < 0.0)) bits |= 1;
+  }
+  for (i64 j = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; j < n_inst; j += stride)
+    if (!isfinite(inst_areas[j]) || inst_areas[j] < 0.0) bits |= 16;
+  if (bits) atomicOr(error, bits);
+}
+
+}  // namespace pe
+
+extern "C" int pe_row_sums(const int64_t* pairs, int64_t rows, int64_t cols, double* out, void* stream_ptr) {
+  PE_REQUIRE(rows >= 0 && cols > 0 && cols < INT_MAX && rows < (1LL << 40), "invalid row-sum shape");
+  if (rows == 0) return kOk;
+  const i64 blocks = (rows * kWave + 255) / 256;
+  PE_REQUIRE(blocks < INT_MAX, "too many rows");
+  row_sums<<<static_cast<unsigned>(blocks), 256, 0, static_cast<hipStream_t>(stream_ptr)>>>(
+      reinterpret_cast<const i64*>(pairs), rows, static_cast<int>(cols), out);
+  PE_LAUNCH_CHECK();
+  return kOk;
+}
+
+extern "C" int pe_instance_checks(int64_t n_det, const int64_t* det_cls, const double* scores, const double* det_area,
+                                  int64_t n_gt, const int64_t* gt_cls, const double* gt_area, const double* ann,
+                                  int64_t C, const uint8_t* things, int64_t n_inst, const double* inst_areas,
+                                  int32_t* error, void* stream_ptr) {
+  PE_REQUIRE(n_det >= 0 && n_gt >= 0 && n_inst >= 0 && C > 0 && C < INT_MAX, "invalid check shapes");
+  hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+  int rc = launch_zero(error, 4, stream);
+  if (rc) return rc;
+  const i64 n = std::max(n_det, std::max(n_gt, n_inst));
+  if (n == 0) return kOk;
+  const int blocks = static_cast<int>(std::min<i64>((n + 255) / 256, 1024));
+  instance_checks<<<blocks, 256, 0, stream>>>(n_det, reinterpret_cast<const i64*>(det_cls), scores, det_area, n_gt,
+                                              reinterpret_cast<const i64*>(gt_cls), gt_area, ann, static_cast<int>(C),
+                                              things, n_inst, inst_areas, error);
+  PE_LAUNCH_CHECK();
+  return kOk;
+}

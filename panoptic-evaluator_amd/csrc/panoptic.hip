@@ -76,6 +76,8 @@ struct SlotResolver {
 
 constexpr uint32_t kNoKey = 0xFFFFFFFFu;
 
+__host__ __device__ inline size_t stage_bytes(int G, int P);
+
 // Two-entry per-lane key cache in front of the LDS (or global) histogram.
 template <bool GLOBAL>
 struct KeyCache {
@@ -103,8 +105,14 @@ struct KeyCache {
   }
 };
 
-template <typename GT, typename PR, bool LOOKUP, bool VEC, bool GLOBAL>
-__global__ __launch_bounds__(kHistThreads) void joint_histogram(HistArgs a) {
+__device__ void score_image(const ScoreArgs& a, int b, char* lds, size_t lds_bytes);
+__device__ void finish_images(const ScoreArgs& a);
+
+// FUSE: the last workgroup to finish an image's chunks scores that image (and
+// the last image's scorer folds iou_sum), so update() is a single launch
+// after the zero fill when it does not validate.
+template <typename GT, typename PR, bool LOOKUP, bool VEC, bool GLOBAL, bool FUSE>
+__global__ __launch_bounds__(kHistThreads) void joint_histogram(HistArgs a, ScoreArgs s) {
   extern __shared__ uint4 lds4[];
   uint32_t* bins = reinterpret_cast<uint32_t*>(lds4);
   const int b = blockIdx.x / a.chunks_per_image;
@@ -177,34 +185,60 @@ __global__ __launch_bounds__(kHistThreads) void joint_histogram(HistArgs a) {
   }
   cache.drain(bins, pairs);
   if (bad && a.error) atomicOr(a.error, 1);
-  if (GLOBAL) return;
-  __syncthreads();
-  for (int w = tid; w < a.words4; w += kHistThreads) {
-    uint4 v = lds4[w];
-    uint32_t words[4] = {v.x, v.y, v.z, v.w};
+  if (!GLOBAL) {
+    __syncthreads();
+    for (int w = tid; w < a.words4; w += kHistThreads) {
+      uint4 v = lds4[w];
+      uint32_t words[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      if (!words[j]) continue;
-      uint32_t key = (static_cast<uint32_t>(w) * 4u + j) * 2u;
-      uint32_t lo = words[j] & 0xFFFFu, hi = words[j] >> 16;
-      if (lo) atomicAdd(pairs + key, static_cast<u64>(lo));
-      if (hi) atomicAdd(pairs + key + 1, static_cast<u64>(hi));
+      for (int j = 0; j < 4; ++j) {
+        if (!words[j]) continue;
+        uint32_t key = (static_cast<uint32_t>(w) * 4u + j) * 2u;
+        uint32_t lo = words[j] & 0xFFFFu, hi = words[j] >> 16;
+        if (lo) atomicAdd(pairs + key, static_cast<u64>(lo));
+        if (hi) atomicAdd(pairs + key + 1, static_cast<u64>(hi));
+      }
     }
+  }
+  if (FUSE) {
+    __shared__ int last_of_image;
+    __threadfence();
+    __syncthreads();  // also retires every LDS read of the flush above
+    if (tid == 0)
+      last_of_image = atomicAdd(a.image_done + b, 1u) == static_cast<unsigned>(a.chunks_per_image) - 1;
+    __syncthreads();
+    if (!last_of_image) return;
+    __threadfence();
+    if (tid == 0) a.image_done[b] = 0;
+    score_image(s, b, reinterpret_cast<char*>(lds4), a.lds_bytes);
+    if (!s.reward) finish_images(s);
   }
 }
 
-template <typename GT, typename PR, bool LOOKUP>
-static int launch_hist_typed(const HistArgs& a, int blocks, size_t lds, bool vec, bool global,
-                             hipStream_t stream) {
-  if (global) {
-    joint_histogram<GT, PR, LOOKUP, false, true><<<blocks, kHistThreads, 0, stream>>>(a);
-  } else if (vec) {
-    joint_histogram<GT, PR, LOOKUP, true, false><<<blocks, kHistThreads, lds, stream>>>(a);
-  } else {
-    joint_histogram<GT, PR, LOOKUP, false, false><<<blocks, kHistThreads, lds, stream>>>(a);
-  }
+template <typename K>
+static bool allow_lds(K kernel, size_t bytes) {
+  if (bytes <= 64 * 1024) return true;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             static_cast<int>(bytes)) == hipSuccess;
+}
+
+template <typename GT, typename PR, bool LOOKUP, bool VEC, bool GLOBAL, bool FUSE>
+static int launch_hist_kernel(const HistArgs& a, const ScoreArgs& s, int blocks, size_t lds,
+                              hipStream_t stream) {
+  auto kernel = joint_histogram<GT, PR, LOOKUP, VEC, GLOBAL, FUSE>;
+  if (!allow_lds(kernel, lds)) return fail(kUnsupportedShape, "LDS request of %zu bytes refused", lds);
+  ProfileScope timing("joint_histogram", stream);
+  kernel<<<blocks, kHistThreads, lds, stream>>>(a, s);
   PE_LAUNCH_CHECK();
   return kOk;
+}
+
+template <typename GT, typename PR, bool LOOKUP, bool FUSE>
+static int launch_hist_typed(const HistArgs& a, const ScoreArgs& s, int blocks, size_t lds, bool vec,
+                             bool global, hipStream_t stream) {
+  if (global) return launch_hist_kernel<GT, PR, LOOKUP, false, true, FUSE>(a, s, blocks, lds, stream);
+  if (vec) return launch_hist_kernel<GT, PR, LOOKUP, true, false, FUSE>(a, s, blocks, lds, stream);
+  return launch_hist_kernel<GT, PR, LOOKUP, false, false, FUSE>(a, s, blocks, lds, stream);
 }
 
 int hist_default_chunk(i64 total_pixels) {
@@ -217,8 +251,10 @@ int hist_default_chunk(i64 total_pixels) {
   return static_cast<int>(chunk);
 }
 
-int launch_joint_histogram(HistArgs a, int gt_bytes, int pred_bytes, int chunk, hipStream_t stream) {
-  if (a.HW == 0 || a.B == 0) return kOk;
+int launch_joint_histogram(HistArgs a, int gt_bytes, int pred_bytes, int chunk, hipStream_t stream,
+                           const ScoreArgs* fuse) {
+  if (a.B == 0) return kOk;
+  if (a.HW == 0) return fuse ? launch_score(*fuse, stream) : kOk;
   if (chunk <= 0) chunk = hist_default_chunk(a.HW * a.B);
   chunk = std::min<int>(std::max<int>(chunk & ~3, 4), kMaxChunk);
   a.chunk = chunk;
@@ -231,13 +267,26 @@ int launch_joint_histogram(HistArgs a, int gt_bytes, int pred_bytes, int chunk, 
   size_t lds = static_cast<size_t>(a.words4) * 16;
   bool global = lds > kMaxHistLds;
   if (global) lds = 0;
+  ScoreArgs s{};
+  if (fuse) {
+    s = *fuse;
+    size_t stage = stage_bytes(a.G, a.P);
+    if (stage <= kMaxStageLds && stage > lds) lds = stage;
+    s.lds_bytes = lds;
+  }
+  a.lds_bytes = lds;
   bool lookup = a.gt_ids || a.pred_ids;
   bool vec = gt_bytes == 4 && pred_bytes == 4 && (reinterpret_cast<uintptr_t>(a.gt) % 16) == 0 &&
              (reinterpret_cast<uintptr_t>(a.pred) % 16) == 0;
   int nb = static_cast<int>(blocks);
-#define PE_HIST(GT, PR)                                                          \
-  return lookup ? launch_hist_typed<GT, PR, true>(a, nb, lds, vec, global, stream) \
-                : launch_hist_typed<GT, PR, false>(a, nb, lds, vec, global, stream)
+#define PE_HIST(GT, PR)                                                                               \
+  do {                                                                                                \
+    if (fuse)                                                                                         \
+      return lookup ? launch_hist_typed<GT, PR, true, true>(a, s, nb, lds, vec, global, stream)       \
+                    : launch_hist_typed<GT, PR, false, true>(a, s, nb, lds, vec, global, stream);     \
+    return lookup ? launch_hist_typed<GT, PR, true, false>(a, s, nb, lds, vec, global, stream)        \
+                  : launch_hist_typed<GT, PR, false, false>(a, s, nb, lds, vec, global, stream);      \
+  } while (0)
   if (gt_bytes == 4 && pred_bytes == 4) PE_HIST(int32_t, int32_t);
   if (gt_bytes == 4 && pred_bytes == 8) PE_HIST(int32_t, int64_t);
   if (gt_bytes == 8 && pred_bytes == 4) PE_HIST(int64_t, int32_t);
@@ -247,7 +296,60 @@ int launch_joint_histogram(HistArgs a, int gt_bytes, int pred_bytes, int chunk, 
 
 // ------------------------------------------------------ areas / validation
 
-__device__ void image_areas(const u64* pairs, int G, int P, i64* areas, int* matched) {
+// Per-image metadata staged in LDS for the scoring loops (falls back to the
+// global arrays when G and P are too large for the workgroup's LDS).
+struct Stage {
+  i64* ga;
+  i64* pa;
+  double* val;
+  int* gc;
+  int* matched;
+  int* lab;
+  uint8_t* crowd;
+  bool staged;
+};
+
+__host__ __device__ inline size_t stage_bytes(int G, int P) {
+  // ga, pa, val (8 bytes each), gc, matched, lab (4 bytes), crowd (1 byte)
+  return 8 * static_cast<size_t>(G) + 16 * static_cast<size_t>(P) + 8 * static_cast<size_t>(G) +
+         4 * static_cast<size_t>(P) + static_cast<size_t>(G) + 64;
+}
+
+__device__ Stage make_stage(const ScoreArgs& a, int b, char* lds, size_t lds_bytes) {
+  const int G = a.G, P = a.P;
+  Stage st;
+  st.staged = lds != nullptr && stage_bytes(G, P) <= lds_bytes;
+  if (st.staged) {
+    char* p = lds;
+    st.ga = reinterpret_cast<i64*>(p);
+    p += 8 * static_cast<size_t>(G);
+    st.pa = reinterpret_cast<i64*>(p);
+    p += 8 * static_cast<size_t>(P);
+    st.val = reinterpret_cast<double*>(p);
+    p += 8 * static_cast<size_t>(P);
+    st.gc = reinterpret_cast<int*>(p);
+    p += 4 * static_cast<size_t>(G);
+    st.matched = reinterpret_cast<int*>(p);
+    p += 4 * static_cast<size_t>(G);
+    st.lab = reinterpret_cast<int*>(p);
+    p += 4 * static_cast<size_t>(P);
+    st.crowd = reinterpret_cast<uint8_t*>(p);
+  } else {
+    i64* areas = a.areas + static_cast<i64>(b) * (G + P);
+    st.ga = areas;
+    st.pa = areas + G;
+    st.val = a.rec_val + static_cast<i64>(b) * P;
+    st.gc = const_cast<int*>(a.gc + static_cast<i64>(b) * G);
+    st.matched = a.matched + static_cast<i64>(b) * G;
+    st.lab = a.rec_lab + static_cast<i64>(b) * P;
+    st.crowd = const_cast<uint8_t*>(a.crowd + static_cast<i64>(b) * G);
+  }
+  return st;
+}
+
+// Row sums (GT areas) with one wave per row; column sums (prediction areas)
+// with one lane per column and eight loads in flight.
+__device__ void image_areas(const u64* pairs, int G, int P, i64* areas, i64* ga, i64* pa) {
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const int waves = blockDim.x / kWave;
   for (int g = wave; g < G; g += waves) {
@@ -256,13 +358,22 @@ __device__ void image_areas(const u64* pairs, int G, int P, i64* areas, int* mat
     s = wave_sum(s);
     if (lane == 0) {
       areas[g] = s;
-      matched[g] = 0;
+      ga[g] = s;
     }
   }
   for (int p = tid; p < P; p += blockDim.x) {
     i64 s = 0;
-    for (int g = 0; g < G; ++g) s += static_cast<i64>(pairs[static_cast<i64>(g) * P + p]);
+    int g = 0;
+    for (; g + 8 <= G; g += 8) {
+      u64 v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = pairs[static_cast<i64>(g + j) * P + p];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) s += static_cast<i64>(v[j]);
+    }
+    for (; g < G; ++g) s += static_cast<i64>(pairs[static_cast<i64>(g) * P + p]);
     areas[G + p] = s;
+    pa[p] = s;
   }
 }
 
@@ -286,59 +397,102 @@ __device__ void image_checks(const i64* areas, const int* gc, const int* pc, con
 
 __global__ __launch_bounds__(kScoreThreads) void prepare_images(ScoreArgs a, int* error) {
   const int b = blockIdx.x;
-  const i64 GP = static_cast<i64>(a.G) * a.P;
-  i64* areas = a.areas + static_cast<i64>(b) * (a.G + a.P);
-  image_areas(a.pairs + b * GP, a.G, a.P, areas, a.matched + static_cast<i64>(b) * a.G);
+  const int G = a.G, P = a.P;
+  const i64 GP = static_cast<i64>(G) * P;
+  i64* areas = a.areas + static_cast<i64>(b) * (G + P);
+  // Areas go to the global scratch only (the staging copies are discarded).
+  const u64* pairs = a.pairs + b * GP;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  for (int g = wave; g < G; g += blockDim.x / kWave) {
+    i64 s = 0;
+    for (int p = lane; p < P; p += kWave) s += static_cast<i64>(pairs[static_cast<i64>(g) * P + p]);
+    s = wave_sum(s);
+    if (lane == 0) areas[g] = s;
+  }
+  for (int p = tid; p < P; p += blockDim.x) {
+    i64 s = 0;
+    int g = 0;
+    for (; g + 8 <= G; g += 8) {
+      u64 v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = pairs[static_cast<i64>(g + j) * P + p];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) s += static_cast<i64>(v[j]);
+    }
+    for (; g < G; ++g) s += static_cast<i64>(pairs[static_cast<i64>(g) * P + p]);
+    areas[G + p] = s;
+  }
   __syncthreads();
-  image_checks(areas, a.gc + static_cast<i64>(b) * a.G, a.pc + static_cast<i64>(b) * a.P,
-               a.crowd + static_cast<i64>(b) * a.G, a.G, a.P, a.C, a.reward, error);
+  image_checks(areas, a.gc + static_cast<i64>(b) * G, a.pc + static_cast<i64>(b) * P,
+               a.crowd + static_cast<i64>(b) * G, G, P, a.C, a.reward, error);
 }
 
 // ------------------------------------------------------------ PQ scoring
 
-__device__ void score_one_image(const ScoreArgs& a, int b) {
+// Score one image with the whole workgroup: areas, PQ matching, confusion,
+// TP/FP/FN and the per-image matched-IoU records.
+__device__ void score_image(const ScoreArgs& a, int b, char* lds, size_t lds_bytes) {
   const int G = a.G, P = a.P, C = a.C;
+  const int tid = threadIdx.x, nthreads = blockDim.x;
   const i64 GP = static_cast<i64>(G) * P;
   const u64* pairs = a.pairs + b * GP;
-  const int* gc = a.gc + static_cast<i64>(b) * G;
   const int* pc = a.pc + static_cast<i64>(b) * P;
-  const uint8_t* crowd = a.crowd + static_cast<i64>(b) * G;
   i64* areas = a.areas + static_cast<i64>(b) * (G + P);
-  int* matched = a.matched + static_cast<i64>(b) * G;
   const i64 row = a.reward ? static_cast<i64>(b) * C : 0;
   u64* tp = a.tp + row;
   u64* fp = a.fp + row;
   u64* fn = a.fn + row;
-  int* rec_lab = a.rec_lab + static_cast<i64>(b) * P;
-  double* rec_val = a.rec_val + static_cast<i64>(b) * P;
-  if (!a.areas_ready) {
-    image_areas(pairs, G, P, areas, matched);
-    __syncthreads();
+  Stage st = make_stage(a, b, lds, lds_bytes);
+  if (st.staged) {
+    const int* gc = a.gc + static_cast<i64>(b) * G;
+    const uint8_t* crowd = a.crowd + static_cast<i64>(b) * G;
+    for (int g = tid; g < G; g += nthreads) {
+      st.gc[g] = gc[g];
+      st.crowd[g] = crowd[g];
+    }
   }
-  for (int p = threadIdx.x; p < P; p += blockDim.x) {
+  if (a.areas_ready) {
+    for (int g = tid; g < G; g += nthreads) st.ga[g] = areas[g];
+    for (int p = tid; p < P; p += nthreads) st.pa[p] = areas[G + p];
+  } else {
+    image_areas(pairs, G, P, areas, st.ga, st.pa);
+  }
+  for (int g = tid; g < G; g += nthreads) st.matched[g] = 0;
+  __syncthreads();
+  for (int p = tid; p < P; p += nthreads) {
     const int pcls = pc[p];
     const bool valid_pred = pcls >= 0 && pcls < C;
-    const i64 pa = areas[G + p];
+    const i64 pa = st.pa[p];
     const i64 void_overlap = static_cast<i64>(pairs[p]);
     i64 ignored = void_overlap;
     bool pm = false;
     double sum_iou = 0.0;
-    for (int g = 0; g < G; ++g) {
-      const int tcls = gc[g];
-      const i64 inter = static_cast<i64>(pairs[static_cast<i64>(g) * P + p]);
-      const bool valid_target = tcls >= 0 && tcls < C;
-      if (!a.reward && valid_target && inter && !crowd[g])
-        atomicAdd(a.confusion + static_cast<i64>(tcls) * (C + 1) + (valid_pred ? pcls : C),
-                  static_cast<u64>(inter));
-      if (!valid_target || !valid_pred || tcls != pcls) continue;
-      if (crowd[g]) {
-        ignored = void_overlap + inter;  // the last same-class crowd slot wins
-      } else if (inter) {
-        const i64 uni = areas[g] + pa - inter - void_overlap;
-        if (inter > uni / 2) {  // IoU > 0.5 without rounding
-          pm = true;
-          sum_iou += static_cast<double>(inter) / static_cast<double>(uni);
-          matched[g] = 1;
+    for (int g0 = 0; g0 < G; g0 += 8) {
+      u64 v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = g0 + j < G ? pairs[static_cast<i64>(g0 + j) * P + p] : 0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int g = g0 + j;
+        if (g >= G) break;
+        const i64 inter = static_cast<i64>(v[j]);
+        const int tcls = st.gc[g];
+        const bool valid_target = tcls >= 0 && tcls < C;
+        if (!valid_target) continue;
+        const bool is_crowd = st.crowd[g] != 0;
+        if (!a.reward && inter && !is_crowd)
+          atomicAdd(a.confusion + static_cast<i64>(tcls) * (C + 1) + (valid_pred ? pcls : C),
+                    static_cast<u64>(inter));
+        if (!valid_pred || tcls != pcls) continue;
+        if (is_crowd) {
+          ignored = void_overlap + inter;  // the last same-class crowd slot wins
+        } else if (inter) {
+          const i64 uni = st.ga[g] + pa - inter - void_overlap;
+          if (inter > uni / 2) {  // IoU > 0.5 without rounding
+            pm = true;
+            sum_iou += static_cast<double>(inter) / static_cast<double>(uni);
+            st.matched[g] = 1;
+          }
         }
       }
     }
@@ -351,84 +505,75 @@ __device__ void score_one_image(const ScoreArgs& a, int b) {
         atomicAdd(fp + pcls, 1ULL);
       }
     }
-    rec_lab[p] = label;
-    rec_val[p] = sum_iou;
+    st.lab[p] = label;
+    st.val[p] = sum_iou;
   }
   __syncthreads();
-  for (int g = threadIdx.x; g < G; g += blockDim.x) {
-    const int c = gc[g];
-    if (c >= 0 && c < C && !crowd[g] && !matched[g]) atomicAdd(fn + c, 1ULL);
+  for (int g = tid; g < G; g += nthreads) {
+    const int c = st.gc[g];
+    if (c >= 0 && c < C && !st.crowd[g] && !st.matched[g]) atomicAdd(fn + c, 1ULL);
   }
   if (a.reward) {
     // Per-image sums: in-order fold over prediction slots.
     double* sums = a.iou_sum + row;
-    for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    for (int c = tid; c < C; c += nthreads) {
       double acc = sums[c];
       for (int p = 0; p < P; ++p)
-        if (rec_lab[p] == c) acc += rec_val[p];
+        if (st.lab[p] == c) acc += st.val[p];
       sums[c] = acc;
     }
     return;
   }
-  // Dataset mode: class-sorted matched IoUs of this image, slot order kept.
-  int* count = a.cls_count + static_cast<i64>(b) * C;
-  int* offset = a.cls_off + static_cast<i64>(b) * C;
-  double* comp = a.comp_val + static_cast<i64>(b) * P;
-  for (int c = threadIdx.x; c < C; c += blockDim.x) {
-    int n = 0;
-    for (int p = 0; p < P; ++p) n += rec_lab[p] == c;
-    count[c] = n;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int run = 0;
-    for (int c = 0; c < C; ++c) {
-      offset[c] = run;
-      run += count[c];
-    }
-  }
-  __syncthreads();
-  for (int c = threadIdx.x; c < C; c += blockDim.x) {
-    if (!count[c]) continue;
-    int k = offset[c];
-    for (int p = 0; p < P; ++p)
-      if (rec_lab[p] == c) comp[k++] = rec_val[p];
+  // Dataset mode: every matched IoU lies in (0.5, 1], an exact multiple of
+  // 2^-53, so the class sums accumulate exactly as 2^-53 fixed point in two
+  // 64-bit words (low 32 bits / high bits of each value): order-independent.
+  for (int p = tid; p < P; p += nthreads) {
+    const int c = st.lab[p];
+    if (c < 0) continue;
+    const u64 m = static_cast<u64>(st.val[p] * 9007199254740992.0);  // * 2^53, exact
+    atomicAdd(a.iou_exact + 2 * c, m & 0xFFFFFFFFULL);
+    atomicAdd(a.iou_exact + 2 * c + 1, m >> 32);
   }
 }
 
-// iou_sum[c] += matched IoUs of class c in (image, prediction slot) order.
-__device__ void fold_iou_sum(const ScoreArgs& a, int images) {
-  for (int c = threadIdx.x; c < a.C; c += blockDim.x) {
-    double acc = a.iou_sum[c];
-    for (int b = 0; b < images; ++b) {
-      const i64 idx = static_cast<i64>(b) * a.C + c;
-      const int n = a.cls_count[idx];
-      if (!n) continue;
-      const double* v = a.comp_val + static_cast<i64>(b) * a.P + a.cls_off[idx];
-      for (int e = 0; e < n; ++e) acc += v[e];
-    }
-    a.iou_sum[c] = acc;
-  }
+// iou_sum[c] = the exact accumulated sum rounded once to FP64.
+__device__ __forceinline__ double exact_to_double(u64 lo, u64 hi) {
+  return (static_cast<double>(hi) * 4294967296.0 + static_cast<double>(lo)) * 0x1p-53;
 }
 
-__global__ __launch_bounds__(kScoreThreads) void score_images(ScoreArgs a) {
-  if (a.guard && *a.guard) return;  // a failed validation never mutates totals
-  score_one_image(a, blockIdx.x);
-  if (a.reward) return;
+__device__ void convert_iou_sum(const ScoreArgs& a) {
+  for (int c = threadIdx.x; c < a.C; c += blockDim.x)
+    a.iou_sum[c] = exact_to_double(a.iou_exact[2 * c], a.iou_exact[2 * c + 1]);
+}
+
+// Called by every workgroup after it scored an image: the last one (over the
+// whole launch) folds iou_sum in image order.  Dataset mode only.
+__device__ void finish_images(const ScoreArgs& a) {
   __shared__ int last;
   __threadfence();
   __syncthreads();
-  if (threadIdx.x == 0) last = atomicAdd(a.counter, 1u) == gridDim.x - 1;
+  if (threadIdx.x == 0) last = atomicAdd(a.counter, 1u) == static_cast<unsigned>(a.B) - 1;
   __syncthreads();
   if (!last) return;
   __threadfence();
-  fold_iou_sum(a, a.B);
+  convert_iou_sum(a);
   if (threadIdx.x == 0) *a.counter = 0;
 }
 
-int launch_score(const ScoreArgs& a, hipStream_t stream) {
+__global__ __launch_bounds__(kScoreThreads) void score_images(ScoreArgs a) {
+  extern __shared__ uint4 lds4[];
+  if (a.guard && *a.guard) return;  // a failed validation never mutates totals
+  score_image(a, blockIdx.x, reinterpret_cast<char*>(lds4), a.lds_bytes);
+  if (!a.reward) finish_images(a);
+}
+
+int launch_score(ScoreArgs a, hipStream_t stream) {
   if (a.B == 0) return kOk;
-  score_images<<<a.B, kScoreThreads, 0, stream>>>(a);
+  size_t lds = stage_bytes(a.G, a.P);
+  if (lds > kMaxStageLds) lds = 0;
+  a.lds_bytes = lds;
+  ProfileScope timing("score_images", stream);
+  score_images<<<a.B, kScoreThreads, lds, stream>>>(a);
   PE_LAUNCH_CHECK();
   return kOk;
 }
@@ -442,7 +587,13 @@ int launch_prepare(const ScoreArgs& a, int* error, hipStream_t stream) {
 
 // --------------------------------------------------------------- finaliser
 
+// One workgroup per accumulator row.  Confusion row and column sums come from
+// one coalesced sweep into LDS (64-bit LDS atomics); per-class ratios follow
+// the reference CPU formulas exactly; group means use a fixed-order tree.
 __global__ __launch_bounds__(kScoreThreads) void finalize(FinalArgs a) {
+  extern __shared__ uint4 lds4[];
+  u64* rowsum = reinterpret_cast<u64*>(lds4);
+  u64* colsum = rowsum + a.C;
   const i64 k = blockIdx.x;
   const int C = a.C, tid = threadIdx.x;
   const i64 CC = static_cast<i64>(C) * (C + 1);
@@ -453,6 +604,17 @@ __global__ __launch_bounds__(kScoreThreads) void finalize(FinalArgs a) {
   const i64* conf = a.confusion + k * CC;
   double* per_class = a.per_class + k * 4 * C;
   __shared__ double red[14][kScoreThreads];
+  for (int c = tid; c < 2 * C; c += kScoreThreads) rowsum[c] = 0;
+  __syncthreads();
+  for (i64 i = tid; i < CC; i += kScoreThreads) {
+    const i64 v = conf[i];
+    if (a.out_confusion) a.out_confusion[k * CC + i] = v;
+    if (!v) continue;
+    const int r = static_cast<int>(i / (C + 1)), col = static_cast<int>(i - static_cast<i64>(r) * (C + 1));
+    atomicAdd(rowsum + r, static_cast<u64>(v));
+    if (col < C) atomicAdd(colsum + col, static_cast<u64>(v));
+  }
+  __syncthreads();
   double part[14];
 #pragma unroll
   for (int j = 0; j < 14; ++j) part[j] = 0.0;
@@ -473,11 +635,8 @@ __global__ __launch_bounds__(kScoreThreads) void finalize(FinalArgs a) {
     per_class[c] = pq;
     per_class[C + c] = sq;
     per_class[2 * C + c] = rq;
-    i64 rowsum = 0, colsum = 0;
     const i64 diag = conf[static_cast<i64>(c) * (C + 1) + c];
-    for (int j = 0; j <= C; ++j) rowsum += conf[static_cast<i64>(c) * (C + 1) + j];
-    for (int j = 0; j < C; ++j) colsum += conf[static_cast<i64>(j) * (C + 1) + c];
-    const i64 uni = rowsum + colsum - diag;
+    const i64 uni = static_cast<i64>(rowsum[c]) + static_cast<i64>(colsum[c]) - diag;
     const double iou = uni > 0 ? static_cast<double>(diag) / static_cast<double>(uni) : __builtin_nan("");
     per_class[3 * C + c] = iou;
     if (uni > 0) {
@@ -490,8 +649,6 @@ __global__ __launch_bounds__(kScoreThreads) void finalize(FinalArgs a) {
       part[group * 4] += pq; part[group * 4 + 1] += sq; part[group * 4 + 2] += rq; part[group * 4 + 3] += 1.0;
     }
   }
-  if (a.out_confusion)
-    for (i64 i = tid; i < CC; i += kScoreThreads) a.out_confusion[k * CC + i] = conf[i];
 #pragma unroll
   for (int j = 0; j < 14; ++j) red[j][tid] = part[j];
   __syncthreads();
@@ -511,7 +668,9 @@ __global__ __launch_bounds__(kScoreThreads) void finalize(FinalArgs a) {
 
 int launch_finalize(const FinalArgs& a, i64 K, hipStream_t stream) {
   if (K <= 0) return kOk;
-  finalize<<<static_cast<unsigned>(K), kScoreThreads, 0, stream>>>(a);
+  size_t lds = 16 * static_cast<size_t>(a.C);
+  if (lds > kMaxStageLds) return fail(kUnsupportedShape, "too many categories for the finaliser");
+  finalize<<<static_cast<unsigned>(K), kScoreThreads, lds, stream>>>(a);
   PE_LAUNCH_CHECK();
   return kOk;
 }
@@ -574,12 +733,10 @@ using namespace pe;
 
 extern "C" int64_t pe_update_workspace_bytes(int64_t B, int64_t G, int64_t P, int64_t C) {
   Carve carve(nullptr);
-  carve.take<int>(B * P);        // rec_lab
+  carve.take<int>(B * P);        // rec_lab (prediction labels, large G/P only)
   carve.take<double>(B * P);     // rec_val
-  carve.take<int>(B * C);        // cls_count
-  carve.take<int>(B * C);        // cls_off
-  carve.take<double>(B * P);     // comp_val
   carve.take<unsigned>(64);      // counter
+  carve.take<unsigned>(B);       // image_done
   return static_cast<int64_t>(align_up(carve.used, 256));
 }
 
@@ -588,8 +745,8 @@ extern "C" int pe_update(const void* gt, int gt_bytes, const void* pred, int pre
                          int64_t G, int64_t P, int64_t C, int64_t* pairs, int64_t* areas, int32_t* matched,
                          int64_t* tp, int64_t* fp, int64_t* fn, double* iou_sum, int64_t* confusion,
                          const int64_t* gt_ids, const int32_t* gt_slots, const int64_t* pred_ids,
-                         const int32_t* pred_slots, int32_t* error, void* workspace, int64_t workspace_bytes,
-                         int validate, int reward, void* stream_ptr) {
+                         const int32_t* pred_slots, int32_t* error, uint64_t* iou_exact, void* workspace,
+                         int64_t workspace_bytes, int validate, int reward, void* stream_ptr) {
   PE_REQUIRE(B >= 0 && H >= 0 && W >= 0, "negative shape");
   PE_REQUIRE(G > 0 && G < INT_MAX && P > 0 && P < INT_MAX && C > 0 && C < INT_MAX, "invalid capacities");
   PE_REQUIRE(B < INT_MAX && G * P < (1LL << 32), "pair histogram too large");
@@ -598,6 +755,7 @@ extern "C" int pe_update(const void* gt, int gt_bytes, const void* pred, int pre
              "lookup IDs and slots must be supplied together");
   PE_REQUIRE(workspace_bytes >= pe_update_workspace_bytes(B, G, P, C), "workspace too small");
   PE_REQUIRE(!validate || error, "validation needs the error word");
+  PE_REQUIRE(reward || (iou_exact && confusion), "dataset mode needs the confusion and exact IoU accumulators");
   hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
   if (B == 0) return kOk;
   Carve carve(workspace);
@@ -615,10 +773,9 @@ extern "C" int pe_update(const void* gt, int gt_bytes, const void* pred, int pre
   s.confusion = reinterpret_cast<u64*>(confusion);
   s.rec_lab = carve.take<int>(B * P);
   s.rec_val = carve.take<double>(B * P);
-  s.cls_count = carve.take<int>(B * C);
-  s.cls_off = carve.take<int>(B * C);
-  s.comp_val = carve.take<double>(B * P);
   s.counter = carve.take<unsigned>(64);
+  s.iou_exact = reinterpret_cast<u64*>(iou_exact);
+  unsigned* image_done = carve.take<unsigned>(B);
   s.B = static_cast<int>(B);
   s.G = static_cast<int>(G);
   s.P = static_cast<int>(P);
@@ -627,6 +784,7 @@ extern "C" int pe_update(const void* gt, int gt_bytes, const void* pred, int pre
   int rc = launch_zero(pairs, static_cast<size_t>(B * G * P) * 8, stream);
   if (rc) return rc;
   if ((rc = launch_zero(s.counter, 256, stream))) return rc;
+  if ((rc = launch_zero(image_done, 4 * static_cast<size_t>(B), stream))) return rc;
   if (validate && (rc = launch_zero(error, 4, stream))) return rc;
   HistArgs h{};
   h.gt = gt;
@@ -641,8 +799,10 @@ extern "C" int pe_update(const void* gt, int gt_bytes, const void* pred, int pre
   h.pred_ids = pred_ids;
   h.pred_slots = pred_slots;
   h.error = validate ? error : nullptr;
-  if ((rc = launch_joint_histogram(h, gt_bytes, pred_bytes, 0, stream))) return rc;
-  if (validate) {
+  h.image_done = image_done;
+  if (!validate) return launch_joint_histogram(h, gt_bytes, pred_bytes, 0, stream, &s);
+  if ((rc = launch_joint_histogram(h, gt_bytes, pred_bytes, 0, stream, nullptr))) return rc;
+  {
     if ((rc = launch_prepare(s, error, stream))) return rc;
     int32_t host_error = 0;
     PE_HIP(hipMemcpyAsync(&host_error, error, sizeof(int32_t), hipMemcpyDeviceToHost, stream));

@@ -10,6 +10,7 @@ constexpr int kHistThreads = 512;
 constexpr int kScoreThreads = 256;
 constexpr int kMaxChunk = 65532;                 // packed 16-bit LDS counters stay exact
 constexpr size_t kMaxHistLds = 128 * 1024;       // beyond this, global atomics
+constexpr size_t kMaxStageLds = 120 * 1024;      // per-image scoring metadata in LDS
 
 struct HistArgs {
   const void* gt;
@@ -26,6 +27,8 @@ struct HistArgs {
   const i64* pred_ids;
   const int* pred_slots;
   int* error;            // nullptr: no validation
+  unsigned* image_done;  // B counters, zero between launches (fused scoring)
+  size_t lds_bytes;
 };
 
 struct ScoreArgs {
@@ -42,14 +45,13 @@ struct ScoreArgs {
   u64* confusion;
   int* rec_lab;
   double* rec_val;
-  int* cls_count;
-  int* cls_off;
-  double* comp_val;
+  u64* iou_exact;        // C x 2: exact 2^-53 fixed-point IoU sums (dataset mode)
   unsigned* counter;
   const int* guard;
   int B, G, P, C;
   int reward;
   int areas_ready;
+  size_t lds_bytes;
 };
 
 struct FinalArgs {
@@ -72,9 +74,11 @@ struct FinalArgs {
 };
 
 int hist_default_chunk(i64 total_pixels);
-int launch_joint_histogram(HistArgs a, int gt_bytes, int pred_bytes, int chunk, hipStream_t stream);
+// fuse != nullptr scores every image inside the same launch (no validation).
+int launch_joint_histogram(HistArgs a, int gt_bytes, int pred_bytes, int chunk, hipStream_t stream,
+                           const ScoreArgs* fuse);
 int launch_prepare(const ScoreArgs& a, int* error, hipStream_t stream);
-int launch_score(const ScoreArgs& a, hipStream_t stream);
+int launch_score(ScoreArgs a, hipStream_t stream);
 int launch_finalize(const FinalArgs& a, i64 K, hipStream_t stream);
 
 }  // namespace pe

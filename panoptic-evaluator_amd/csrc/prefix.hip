@@ -1,0 +1,494 @@
+// Generation-order prefix rewards on gfx950 (segmentation_prefixes).
+//
+// Behaviour follows the reference's CPU path (prefix.py:11-122: later masks
+// overwrite earlier ones on the canvas; visible-segment PQ/semantic stats at
+// checkpoints; independent-mask greedy instance matching per IoU threshold)
+// and its native boundary (prefix_kernel.cu:142-199).
+//
+// Design (MI355X-first; replaces 2 launches per mask with 2 launches per
+// batch of images):
+//  * events: one pixel pass over every mask of every image.  Each lane owns
+//    32 pixels, keeps their GT slots and current top mask in registers and
+//    streams the masks in generation order with 16-byte loads.  Every covered
+//    pixel is a transfer (mask i, GT slot g, previous top p), counted in
+//    raw[i][g] (intersection of the original mask) and in the triangular
+//    per-step table dec[i][g][p] (p <= i).  GT areas come from the same pass.
+//  * scoring: one workgroup per image replays the steps: column i+1 of the
+//    G x (N+1) pair table becomes raw[i], dec[i] is subtracted, the greedy
+//    instance match of mask i runs with one lane per IoU threshold, and at
+//    each checkpoint the full PQ/confusion of the visible canvas is computed
+//    (iou_sum folded in slot order, as the reference's scatter_add_).
+#include <climits>
+
+#include "common.h"
+#include "../../include/panoptic_hip.h"
+
+namespace pe {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr int kPrefixThreads = 256;
+constexpr int kPrefixTile = 16 * kPrefixThreads * 2;
+
+struct PrefixArgs {
+  int B, G, C, T;
+  i64 HW;
+  const int32_t* target;        // B x HW
+  const uint8_t* const* masks;  // B pointers (N_b x HW)
+  const i64* n_off;             // B + 1: mask offsets
+  const int32_t* cls;           // sum N: classes in generation order
+  const int32_t* gc;            // B x G
+  const uint8_t* crowd;         // B x G
+  const uint8_t* things;        // C
+  const double* thresholds;     // T
+  const i64* k_off;             // B + 1: checkpoint offsets
+  const int32_t* points;        // sum K: checkpoints (strictly increasing per image)
+  // Per-image scratch; image b's sections start at (in elements)
+  //   raw: n_off[b] * G, dec: G * sum_{b' < b} N_b'(N_b' + 1) / 2,
+  //   pairs: (n_off[b] + b) * G, pa/lab/val: n_off[b] + b, used: b * T * G.
+  int32_t* raw;                         // N_b x G
+  int32_t* dec;                         // step-major triangles: step i holds G x (i + 1)
+  int32_t* pairs;                       // G x (N_b + 1)
+  i64* pa;                              // N_b + 1
+  int32_t* ga;                          // B x G
+  uint8_t* used;                        // T x G
+  int32_t* matched;                     // B x G
+  i64* itp;                             // B x T x C
+  i64* ifp;                             // B x T x C
+  i64* positives;                       // B x C
+  int32_t* lab;                         // N_b + 1
+  double* val;                          // N_b + 1
+  // outputs per snapshot row
+  i64* tp; i64* fp; i64* fn; double* sums; i64* confusion;
+  i64* out_itp; i64* out_ifp; i64* out_ifn; double* precision; double* recall;
+  int32_t* error;
+  int tiles;
+  int aligned;
+};
+
+__device__ __forceinline__ v4i load16(const uint8_t* row, i64 k, i64 HW, bool aligned) {
+  v4i v = {0, 0, 0, 0};
+  if (k >= HW) return v;
+  if (aligned && k + 16 <= HW) return *reinterpret_cast<const v4i*>(row + k);
+  uint8_t bytes[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) bytes[j] = k + j < HW ? row[k + j] : 0;
+  __builtin_memcpy(&v, bytes, 16);
+  return v;
+}
+
+__device__ __forceinline__ i64 tri(i64 i) { return i * (i + 1) / 2; }
+
+__device__ __forceinline__ i64 dec_base(const PrefixArgs& a, int b) {
+  i64 total = 0;
+  for (int q = 0; q < b; ++q) total += tri(a.n_off[q + 1] - a.n_off[q]);
+  return total * a.G;
+}
+
+__global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
+  extern __shared__ uint4 lds4[];
+  uint32_t* ga = reinterpret_cast<uint32_t*>(lds4);
+  const int b = blockIdx.x / a.tiles, tile = blockIdx.x - (blockIdx.x / a.tiles) * a.tiles;
+  const int G = a.G;
+  for (int g = threadIdx.x; g < G; g += kPrefixThreads) ga[g] = 0;
+  __syncthreads();
+  const i64 n0 = a.n_off[b];
+  const int N = static_cast<int>(a.n_off[b + 1] - n0);
+  const int32_t* target = a.target + static_cast<i64>(b) * a.HW;
+  int slots[2][16], top[2][16];
+  i64 base[2];
+  bool bad = false;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    base[h] = static_cast<i64>(tile) * kPrefixTile + (static_cast<i64>(h) * kPrefixThreads + threadIdx.x) * 16;
+    int run_g = -1, run = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      int s = -1;
+      if (base[h] + j < a.HW) {
+        s = target[base[h] + j];
+        if (s < 0 || s >= G) {
+          bad = true;
+          s = -1;
+        }
+      }
+      slots[h][j] = s;
+      top[h][j] = 0;
+      if (s != run_g) {
+        if (run_g >= 0) atomicAdd(ga + run_g, static_cast<uint32_t>(run));
+        run_g = s;
+        run = 0;
+      }
+      run += s >= 0;
+    }
+    if (run_g >= 0) atomicAdd(ga + run_g, static_cast<uint32_t>(run));
+  }
+  if (bad) atomicOr(a.error, 1);
+  const uint8_t* masks = a.masks[b];
+  int32_t* raw = a.raw + n0 * G;
+  int32_t* dec = a.dec + dec_base(a, b);
+  const int P1 = N + 1;
+  for (int i = 0; i < N; ++i) {
+    const uint8_t* mask = masks + static_cast<i64>(i) * a.HW;
+    v4i m[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) m[h] = load16(mask, base[h], a.HW, a.aligned != 0);
+    int32_t* dec_i = dec + static_cast<i64>(G) * tri(i);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if ((m[h].x | m[h].y | m[h].z | m[h].w) == 0) continue;
+      const uint32_t words[4] = {static_cast<uint32_t>(m[h].x), static_cast<uint32_t>(m[h].y),
+                                 static_cast<uint32_t>(m[h].z), static_cast<uint32_t>(m[h].w)};
+      int run_key = -1, run = 0;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const bool on = (words[j >> 2] >> ((j & 3) * 8)) & 0xFFu;
+        const int g = slots[h][j];
+        int key = -1;
+        if (on && g >= 0) {
+          key = g * P1 + top[h][j];
+          top[h][j] = i + 1;
+        }
+        if (key != run_key) {
+          if (run_key >= 0) {
+            const int rg = run_key / P1, prev = run_key - rg * P1;
+            atomicAdd(raw + static_cast<i64>(i) * G + rg, run);
+            atomicAdd(dec_i + static_cast<i64>(rg) * (i + 1) + prev, run);
+          }
+          run_key = key;
+          run = 0;
+        }
+        run += key >= 0;
+      }
+      if (run_key >= 0) {
+        const int rg = run_key / P1, prev = run_key - rg * P1;
+        atomicAdd(raw + static_cast<i64>(i) * G + rg, run);
+        atomicAdd(dec_i + static_cast<i64>(rg) * (i + 1) + prev, run);
+      }
+    }
+  }
+  __syncthreads();
+  for (int g = threadIdx.x; g < G; g += kPrefixThreads)
+    if (ga[g]) atomicAdd(a.ga + static_cast<i64>(b) * G + g, static_cast<int32_t>(ga[g]));
+}
+
+// Panoptic and semantic statistics of the visible canvas (snapshot row r).
+__device__ void prefix_snapshot(const PrefixArgs& a, int b, int k, i64 r, int P1, const int32_t* pairs, i64* pa,
+                                const int32_t* cls, const int32_t* gc, const uint8_t* crowd, const int32_t* ga,
+                                int32_t* matched, int32_t* lab, double* val, const i64* itp, const i64* ifp,
+                                const i64* positives) {
+  const int G = a.G, C = a.C, tid = threadIdx.x;
+  for (int j = tid; j <= k; j += blockDim.x) {
+    i64 s = 0;
+    for (int g = 0; g < G; ++g) s += pairs[static_cast<i64>(g) * P1 + j];
+    pa[j] = s;
+  }
+  for (int g = tid; g < G; g += blockDim.x) matched[g] = 0;
+  __syncthreads();
+  u64* tp = reinterpret_cast<u64*>(a.tp + r * C);
+  u64* fp = reinterpret_cast<u64*>(a.fp + r * C);
+  u64* fn = reinterpret_cast<u64*>(a.fn + r * C);
+  u64* conf = reinterpret_cast<u64*>(a.confusion + r * C * (C + 1));
+  for (int j = tid; j <= k; j += blockDim.x) {
+    const int label = (j > 0 && pa[j] > 0) ? cls[j - 1] : -1;
+    const bool valid_pred = label >= 0 && label < C;
+    const i64 void_overlap = pairs[j];
+    i64 ignored = void_overlap;
+    bool pm = false;
+    double sum = 0.0;
+    for (int g = 0; g < G; ++g) {
+      const int tcls = gc[g];
+      if (tcls < 0 || tcls >= C) continue;
+      const i64 inter = pairs[static_cast<i64>(g) * P1 + j];
+      const bool is_crowd = crowd[g] != 0;
+      if (inter && !is_crowd) atomicAdd(conf + static_cast<i64>(tcls) * (C + 1) + (valid_pred ? label : C),
+                                        static_cast<u64>(inter));
+      if (!valid_pred || tcls != label) continue;
+      if (is_crowd) {
+        ignored = void_overlap + inter;
+      } else if (inter) {
+        const i64 uni = static_cast<i64>(ga[g]) + pa[j] - inter - void_overlap;
+        if (inter > uni / 2) {
+          pm = true;
+          sum += static_cast<double>(inter) / static_cast<double>(uni);
+          matched[g] = 1;
+        }
+      }
+    }
+    int out = -1;
+    if (valid_pred) {
+      if (pm) {
+        atomicAdd(tp + label, 1ULL);
+        out = label;
+      } else if (static_cast<double>(ignored) <= 0.5 * static_cast<double>(pa[j])) {
+        atomicAdd(fp + label, 1ULL);
+      }
+    }
+    lab[j] = out;
+    val[j] = sum;
+  }
+  __syncthreads();
+  for (int g = tid; g < G; g += blockDim.x) {
+    const int c = gc[g];
+    if (c >= 0 && c < C && !crowd[g] && !matched[g]) atomicAdd(fn + c, 1ULL);
+  }
+  double* sums = a.sums + r * C;
+  for (int c = tid; c < C; c += blockDim.x) {
+    double acc = 0.0;
+    for (int j = 0; j <= k; ++j)
+      if (lab[j] == c) acc += val[j];
+    sums[c] = acc;
+  }
+  const int TC = a.T * C;
+  for (int e = tid; e < TC; e += blockDim.x) {
+    const i64 t_ = itp[e], f_ = ifp[e], n_ = positives[e % C];
+    a.out_itp[r * TC + e] = t_;
+    a.out_ifp[r * TC + e] = f_;
+    a.out_ifn[r * TC + e] = n_ - t_;
+    a.precision[r * TC + e] = static_cast<double>(t_) / static_cast<double>(t_ + f_ < 1 ? 1 : t_ + f_);
+    a.recall[r * TC + e] = static_cast<double>(t_) / static_cast<double>(n_ < 1 ? 1 : n_);
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kPrefixThreads) void prefix_score(PrefixArgs a) {
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int G = a.G, C = a.C, T = a.T;
+  const i64 n0 = a.n_off[b];
+  const int N = static_cast<int>(a.n_off[b + 1] - n0);
+  const int P1 = N + 1;
+  int32_t* pairs = a.pairs + (n0 + b) * G;
+  const int32_t* raw = a.raw + n0 * G;
+  const int32_t* dec = a.dec + dec_base(a, b);
+  const int32_t* ga = a.ga + static_cast<i64>(b) * G;
+  const int32_t* gc = a.gc + static_cast<i64>(b) * G;
+  const uint8_t* crowd = a.crowd + static_cast<i64>(b) * G;
+  const int32_t* cls = a.cls + n0;
+  uint8_t* used = a.used + static_cast<i64>(b) * T * G;
+  int32_t* matched = a.matched + static_cast<i64>(b) * G;
+  i64* itp = a.itp + static_cast<i64>(b) * T * C;
+  i64* ifp = a.ifp + static_cast<i64>(b) * T * C;
+  i64* positives = a.positives + static_cast<i64>(b) * C;
+  i64* pa = a.pa + n0 + b;
+  int32_t* lab = a.lab + n0 + b;
+  double* val = a.val + n0 + b;
+  const i64 kfirst = a.k_off[b], klast = a.k_off[b + 1];
+  // Validation of the target metadata and the generated classes.
+  int bits = 0;
+  for (int g = tid; g < G; g += blockDim.x) {
+    const int c = gc[g];
+    if (c < -1 || c >= C || (g == 0 && (c != -1 || crowd[0]))) bits |= 2;
+    if (g > 0 && ((ga[g] > 0) != (c >= 0))) bits |= 4;
+  }
+  for (int i = tid; i < N; i += blockDim.x) {
+    if (cls[i] < 0 || cls[i] >= C) bits |= 8;
+    i64 area = 0;
+    for (int g = 0; g < G; ++g) area += raw[static_cast<i64>(i) * G + g];
+    if (area == 0) bits |= 16;
+  }
+  if (bits) atomicOr(a.error, bits);
+  for (i64 e = tid; e < static_cast<i64>(G) * P1; e += blockDim.x) {
+    const i64 g = e / P1;
+    pairs[e] = (e - g * P1) == 0 ? ga[g] : 0;
+  }
+  for (i64 e = tid; e < static_cast<i64>(T) * G; e += blockDim.x) used[e] = 0;
+  for (i64 e = tid; e < static_cast<i64>(T) * C; e += blockDim.x) {
+    itp[e] = 0;
+    ifp[e] = 0;
+  }
+  for (int c = tid; c < C; c += blockDim.x) {
+    i64 n = 0;
+    if (a.things[c])
+      for (int g = 0; g < G; ++g) n += gc[g] == c && !crowd[g];
+    positives[c] = n;
+  }
+  __syncthreads();
+  i64 q = kfirst;
+  if (q < klast && a.points[q] == 0) {
+    prefix_snapshot(a, b, 0, q, P1, pairs, pa, cls, gc, crowd, ga, matched, lab, val, itp, ifp, positives);
+    ++q;
+  }
+  for (int i = 0; i < N; ++i) {
+    const int32_t* dec_i = dec + static_cast<i64>(G) * tri(i);
+    const i64 width = i + 1;
+    for (i64 e = tid; e < static_cast<i64>(G) * width; e += blockDim.x) {
+      const int32_t v = dec_i[e];
+      if (v) {
+        const i64 g = e / width;
+        pairs[g * P1 + (e - g * width)] -= v;
+      }
+    }
+    for (int g = tid; g < G; g += blockDim.x) pairs[static_cast<i64>(g) * P1 + i + 1] = raw[static_cast<i64>(i) * G + g];
+    __syncthreads();
+    const int c = cls[i];
+    if (c >= 0 && c < C && a.things[c]) {
+      double area = 0.0;
+      {
+        i64 s = 0;
+        for (int g = 0; g < G; ++g) s += raw[static_cast<i64>(i) * G + g];
+        area = static_cast<double>(s);
+      }
+      for (int t = tid; t < T; t += blockDim.x) {
+        const double thr = a.thresholds[t];
+        int best = -1;
+        double best_iou = -1.0;
+        bool ignored = false;
+        for (int g = 0; g < G; ++g) {
+          if (gc[g] != c) continue;
+          const i64 inter = raw[static_cast<i64>(i) * G + g];
+          double uni = crowd[g] ? area : area + static_cast<double>(ga[g]) - static_cast<double>(inter);
+          if (uni < 1.0) uni = 1.0;
+          const double iou = static_cast<double>(inter) / uni;
+          if (iou < thr) continue;
+          if (crowd[g]) ignored = true;
+          else if (!used[static_cast<i64>(t) * G + g] && iou > best_iou) {
+            best = g;
+            best_iou = iou;
+          }
+        }
+        if (best >= 0) {
+          used[static_cast<i64>(t) * G + best] = 1;
+          ++itp[static_cast<i64>(t) * C + c];
+        } else if (!ignored) {
+          ++ifp[static_cast<i64>(t) * C + c];
+        }
+      }
+    }
+    __syncthreads();
+    if (q < klast && a.points[q] == i + 1) {
+      prefix_snapshot(a, b, i + 1, q, P1, pairs, pa, cls, gc, crowd, ga, matched, lab, val, itp, ifp, positives);
+      ++q;
+    }
+  }
+}
+
+}  // namespace pe
+
+using namespace pe;
+
+namespace {
+struct PrefixScratch {
+  int32_t* raw;
+  int32_t* dec;
+  int32_t* pairs;
+  i64* pa;
+  int32_t* ga;
+  uint8_t* used;
+  int32_t* matched;
+  i64* itp;
+  i64* ifp;
+  i64* positives;
+  int32_t* lab;
+  double* val;
+  size_t zero_bytes;  // raw + dec + ga, contiguous at the start
+  size_t used_bytes;
+};
+
+PrefixScratch carve_prefix(void* base, i64 B, i64 G, i64 C, i64 T, i64 sum_n, i64 sum_tri) {
+  Carve carve(base);
+  PrefixScratch s{};
+  s.raw = carve.take<int32_t>(sum_n * G);
+  s.dec = carve.take<int32_t>(sum_tri * G);
+  s.ga = carve.take<int32_t>(B * G);
+  s.zero_bytes = carve.used;
+  s.pairs = carve.take<int32_t>((sum_n + B) * G);
+  s.pa = carve.take<i64>(sum_n + B);
+  s.lab = carve.take<int32_t>(sum_n + B);
+  s.val = carve.take<double>(sum_n + B);
+  s.used = carve.take<uint8_t>(B * T * G);
+  s.matched = carve.take<int32_t>(B * G);
+  s.itp = carve.take<i64>(B * T * C);
+  s.ifp = carve.take<i64>(B * T * C);
+  s.positives = carve.take<i64>(B * C);
+  s.used_bytes = carve.used;
+  return s;
+}
+}  // namespace
+
+extern "C" int64_t pe_prefix_workspace_bytes(int64_t B, int64_t G, int64_t C, int64_t T, int64_t sum_n,
+                                             int64_t sum_tri) {
+  return static_cast<int64_t>(align_up(carve_prefix(nullptr, B, G, C, T, sum_n, sum_tri).used_bytes, 256));
+}
+
+extern "C" int pe_prefix(int64_t B, const int32_t* target, int64_t HW, int64_t G, int64_t C,
+                         const uint8_t* const* masks, const int64_t* n_off, int64_t sum_n, int64_t sum_tri,
+                         const int32_t* cls, const int32_t* gc, const uint8_t* crowd, const uint8_t* things,
+                         const double* thresholds, int64_t T, const int64_t* k_off, const int32_t* points,
+                         int64_t total_rows, int64_t* tp, int64_t* fp, int64_t* fn, double* sums,
+                         int64_t* confusion, int64_t* out_itp, int64_t* out_ifp, int64_t* out_ifn,
+                         double* precision, double* recall, int32_t* error, void* workspace,
+                         int64_t workspace_bytes, int aligned, void* stream_ptr) {
+  PE_REQUIRE(B >= 0 && HW > 0 && G > 0 && G < INT_MAX && C > 0 && C < INT_MAX && T > 0 && T * C < INT_MAX,
+             "prefix dimensions outside supported range");
+  PE_REQUIRE(sum_n >= 0 && sum_tri >= 0 && total_rows >= 0, "invalid prefix sizes");
+  PE_REQUIRE(workspace_bytes >= pe_prefix_workspace_bytes(B, G, C, T, sum_n, sum_tri), "workspace too small");
+  hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+  int rc;
+  if ((rc = launch_zero(error, 4, stream))) return rc;
+  if (total_rows) {
+    const size_t rows = static_cast<size_t>(total_rows);
+    if ((rc = launch_zero(tp, 8 * rows * C, stream))) return rc;
+    if ((rc = launch_zero(fp, 8 * rows * C, stream))) return rc;
+    if ((rc = launch_zero(fn, 8 * rows * C, stream))) return rc;
+    if ((rc = launch_zero(confusion, 8 * rows * C * (C + 1), stream))) return rc;
+  }
+  if (B == 0) return kOk;
+  PrefixScratch s = carve_prefix(workspace, B, G, C, T, sum_n, sum_tri);
+  if ((rc = launch_zero(workspace, s.zero_bytes, stream))) return rc;
+  PrefixArgs a{};
+  a.B = static_cast<int>(B);
+  a.G = static_cast<int>(G);
+  a.C = static_cast<int>(C);
+  a.T = static_cast<int>(T);
+  a.HW = HW;
+  a.target = target;
+  a.masks = masks;
+  a.n_off = reinterpret_cast<const i64*>(n_off);
+  a.cls = cls;
+  a.gc = gc;
+  a.crowd = crowd;
+  a.things = things;
+  a.thresholds = thresholds;
+  a.k_off = reinterpret_cast<const i64*>(k_off);
+  a.points = points;
+  a.raw = s.raw;
+  a.dec = s.dec;
+  a.pairs = s.pairs;
+  a.pa = s.pa;
+  a.ga = s.ga;
+  a.used = s.used;
+  a.matched = s.matched;
+  a.itp = s.itp;
+  a.ifp = s.ifp;
+  a.positives = s.positives;
+  a.lab = s.lab;
+  a.val = s.val;
+  a.tp = reinterpret_cast<i64*>(tp);
+  a.fp = reinterpret_cast<i64*>(fp);
+  a.fn = reinterpret_cast<i64*>(fn);
+  a.sums = sums;
+  a.confusion = reinterpret_cast<i64*>(confusion);
+  a.out_itp = reinterpret_cast<i64*>(out_itp);
+  a.out_ifp = reinterpret_cast<i64*>(out_ifp);
+  a.out_ifn = reinterpret_cast<i64*>(out_ifn);
+  a.precision = precision;
+  a.recall = recall;
+  a.error = error;
+  a.tiles = static_cast<int>((HW + kPrefixTile - 1) / kPrefixTile);
+  a.aligned = aligned;
+  const i64 blocks = B * a.tiles;
+  PE_REQUIRE(blocks < INT_MAX, "too many workgroups");
+  const size_t lds = 4 * static_cast<size_t>(G) + 16;
+  PE_REQUIRE(lds <= 64 * 1024, "too many target slots for the prefix kernel");
+  {
+    ProfileScope timing("prefix_events", stream);
+    prefix_events<<<static_cast<unsigned>(blocks), kPrefixThreads, lds, stream>>>(a);
+    PE_LAUNCH_CHECK();
+  }
+  {
+    ProfileScope timing("prefix_score", stream);
+    prefix_score<<<static_cast<unsigned>(B), kPrefixThreads, 0, stream>>>(a);
+    PE_LAUNCH_CHECK();
+  }
+  return kOk;
+}

@@ -6,9 +6,16 @@ hand-written HIP kernels behind a C ABI (libpanoptic_hip.so).
 """
 
 from .evaluator import PanopticBatch, PanopticEvaluator, panoptic_quality
+from .instance import InstanceBatch, InstanceEvaluator
+from .combined import SegmentationEvaluator
+from .prefix import segmentation_prefixes
 
 __all__ = [
     "PanopticBatch",
     "PanopticEvaluator",
     "panoptic_quality",
+    "InstanceBatch",
+    "InstanceEvaluator",
+    "SegmentationEvaluator",
+    "segmentation_prefixes",
 ]

@@ -22,10 +22,13 @@ _i64, _int, _ptr, _dbl = ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c
 _SIGNATURES = {
     "pe_last_error": (ctypes.c_char_p, []),
     "pe_abi_version": (_int, []),
+    "pe_profile_enable": (_int, [_int]),
+    "pe_profile_reset": (_int, []),
+    "pe_profile_read": (_int, [ctypes.c_char_p, ctypes.POINTER(_dbl), ctypes.POINTER(_i64)]),
     "pe_update_workspace_bytes": (_i64, [_i64] * 4),
     "pe_update": (_int, [_ptr, _int, _ptr, _int, _i64, _i64, _i64, _ptr, _ptr, _ptr, _i64, _i64, _i64,
                          _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr,
-                         _ptr, _ptr, _i64, _int, _int, _ptr]),
+                         _ptr, _ptr, _ptr, _i64, _int, _int, _ptr]),
     "pe_compute": (_int, [_ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _ptr, _ptr, _ptr, _ptr,
                           _ptr, _ptr, _ptr, _ptr, _ptr, _ptr]),
     "pe_image_pq": (_int, [_ptr, _ptr, _ptr, _ptr, _i64, _i64, _ptr, _ptr, _ptr]),
@@ -84,6 +87,13 @@ def check(code: int) -> int:
 
 def call(name, *args) -> int:
     return check(getattr(library(), name)(*args))
+
+
+def profile_read(kernel: str):
+    """(total milliseconds, launches) of a kernel since pe_profile_reset()."""
+    total, count = _dbl(), _i64()
+    call("pe_profile_read", kernel.encode(), ctypes.byref(total), ctypes.byref(count))
+    return total.value, count.value
 
 
 def require_gpu(device: torch.device) -> torch.device:

@@ -1,0 +1,106 @@
+"""Joint panoptic, semantic and instance evaluation with a shared target map.
+
+Public surface of the reference's panoptic_evaluator/combined.py
+(SegmentationEvaluator :18-167).  One update() runs, for the whole batch:
+the detection x target-slot histogram (one launch), the device-side input
+checks (one host read when validating), the panoptic update, and one COCO
+matching launch; AP ground truth is exactly the target's thing segments.
+"""
+
+from __future__ import annotations
+
+from typing import Sequence
+
+import torch
+
+from . import _native
+from .evaluator import PanopticBatch, PanopticEvaluator, _prepare
+from .instance import InstanceBatch, InstanceEvaluator, _cat, offsets, raise_instance_error
+
+
+class SegmentationEvaluator:
+    """PQ/mIoU and mask AP/AR with separate predictions and one shared target."""
+
+    def __init__(self, num_classes: int, *, isthing: Sequence[bool] | None = None,
+                 device: str | torch.device = "cuda", validate: bool = True):
+        self.panoptic = PanopticEvaluator(num_classes, isthing=isthing, device=device, validate=validate)
+        self.device = self.panoptic.tp.device
+        self.instance = InstanceEvaluator(num_classes, device=self.device, validate=validate)
+        self.num_classes = num_classes
+        self.validate = validate
+        self._error = torch.zeros(1, dtype=torch.int32, device=self.device)
+
+    def reset(self):
+        self.panoptic.reset()
+        self.instance.reset()
+
+    def _validate_areas(self, areas, classes):
+        if areas is None:
+            return
+        if areas.shape != classes.shape or areas.device != self.device or not areas.is_floating_point():
+            raise ValueError("instance_areas must be floating point B x S on evaluator device")
+
+    @torch.no_grad()
+    def update(self, prediction: PanopticBatch, detections: InstanceBatch, target: PanopticBatch, *,
+               instance_areas: torch.Tensor | None = None):
+        """Accumulate all metrics (combined.py:63-161); optional annotation areas
+        have shape B x S in target slot order and only drive AP area ranges."""
+        target = target.compact()
+        maps, classes, crowd = _prepare(target, self.num_classes, False)
+        if maps.device != self.device:
+            raise ValueError("target must share evaluator device")
+        prepared = self.instance._prepare(detections, prediction=True)
+        if len(prepared) != len(maps) or any(image.masks.shape[1:] != maps.shape[1:] for image in prepared):
+            raise ValueError("detections and target must have matching image counts and dimensions")
+        self._validate_areas(instance_areas, classes)
+        device, C = self.device, self.num_classes
+        stream = _native.stream(device)
+        B, H, W = maps.shape
+        S = classes.shape[1]
+        HW = H * W
+        det_counts = [image.masks.shape[0] for image in prepared]
+        N = sum(det_counts)
+        det_off = offsets(det_counts, device)
+        order = _cat([image.order for image in prepared], torch.int64, device)
+        pointers = torch.tensor([image.masks.data_ptr() for image in prepared], dtype=torch.int64).to(device)
+        aligned = int(HW % 16 == 0 and all(image.masks.data_ptr() % 16 == 0 for image in prepared))
+        pairs = torch.empty((N, S), dtype=torch.int64, device=device)
+        _native.call("pe_detection_histogram", B, pointers.data_ptr(), det_off.data_ptr(), order.data_ptr(),
+                     maps.data_ptr(), HW, S, max(det_counts, default=0), N, pairs.data_ptr(), aligned, stream)
+        # Full detection areas, including void and stuff pixels.
+        det_area = torch.empty(N, dtype=torch.float64, device=device)
+        _native.call("pe_row_sums", pairs.data_ptr(), N, S, det_area.data_ptr(), stream)
+        det_cls = _cat([image.classes for image in prepared], torch.int64, device)
+        scores = _cat([image.scores for image in prepared], torch.float64, device)
+        if self.validate:
+            inst = instance_areas.double().contiguous() if instance_areas is not None else None
+            _native.call("pe_instance_checks", N, det_cls.data_ptr(), scores.data_ptr(), det_area.data_ptr(), 0,
+                         None, None, None, C, self.panoptic.isthing.data_ptr(),
+                         0 if inst is None else inst.numel(), _native.ptr(inst), self._error.data_ptr(), stream)
+            code = int(self._error.item())
+            if code:
+                raise_instance_error(code, combined=True)
+        # Panoptic totals change only after the instance inputs passed.
+        target_areas = self.panoptic._update(prediction, target, return_areas=True)
+        isthing = self.panoptic.isthing
+        valid = classes.clamp(0, C - 1).long()
+        gt_cls = torch.where((classes >= 0) & isthing[valid], classes, -1).int().contiguous()
+        gt_area = target_areas.double().contiguous()
+        ann = gt_area if instance_areas is None else instance_areas.double().contiguous()
+        status = torch.empty((N, 40), dtype=torch.int8, device=device)
+        ranks = torch.empty(N, dtype=torch.int32, device=device)
+        det_cls32 = det_cls.int()
+        gt_off = torch.arange(0, (B + 1) * S, S, dtype=torch.int64, device=device)
+        inter_off = offsets([d * S for d in det_counts], device)
+        _native.call("pe_coco_match", B, C, det_off.data_ptr(), gt_off.data_ptr(), inter_off.data_ptr(),
+                     det_cls32.data_ptr(), det_area.data_ptr(), None, gt_cls.data_ptr(), gt_area.data_ptr(),
+                     crowd.data_ptr(), ann.data_ptr(), pairs.data_ptr(), 8, N, S, status.data_ptr(),
+                     ranks.data_ptr(), self.instance._counts.data_ptr(), None, stream)
+        self.instance._commit((scores, det_cls32, ranks, status))
+        return self
+
+    def compute(self):
+        """Panoptic/semantic results plus an 'instance' AP/AR dictionary."""
+        result = self.panoptic.compute()
+        result["instance"] = self.instance.compute()
+        return result

@@ -190,16 +190,19 @@ class PanopticEvaluator:
     def reset(self):
         """Zero the accumulators in place (their storage is stable)."""
         if hasattr(self, "tp"):
-            for tensor in (self.tp, self.fp, self.fn, self.iou_sum, self.confusion):
-                tensor.zero_()
+            self._state.zero_()  # one fill for all accumulators
             return
         C = self.num_classes
-        self.iou_sum = torch.zeros(C, dtype=torch.float64, device=self.device)
-        self.tp = torch.zeros(C, dtype=torch.int64, device=self.device)
-        self.fp = torch.zeros_like(self.tp)
-        self.fn = torch.zeros_like(self.tp)
+        # One 8-byte-element buffer holds every accumulator, including the
+        # exact 2^-53 fixed-point IoU sums behind iou_sum (2 words per class).
+        self._state = torch.zeros(6 * C + C * (C + 1), dtype=torch.int64, device=self.device)
+        self.tp = self._state[0:C]
+        self.fp = self._state[C:2 * C]
+        self.fn = self._state[2 * C:3 * C]
+        self.iou_sum = self._state[3 * C:4 * C].view(torch.float64)
         # Column C counts void (or inactive) predictions on valid GT pixels.
-        self.confusion = torch.zeros((C, C + 1), dtype=torch.int64, device=self.device)
+        self.confusion = self._state[4 * C:4 * C + C * (C + 1)].view(C, C + 1)
+        self._iou_exact = self._state[4 * C + C * (C + 1):]
 
     @torch.no_grad()
     def update(self, prediction: PanopticBatch, target: PanopticBatch):
@@ -238,8 +241,8 @@ class PanopticEvaluator:
             pairs.data_ptr(), areas.data_ptr(), matched.data_ptr(), self.tp.data_ptr(),
             self.fp.data_ptr(), self.fn.data_ptr(), self.iou_sum.data_ptr(), self.confusion.data_ptr(),
             _native.ptr(gt_ids), _native.ptr(gt_slots), _native.ptr(pred_ids), _native.ptr(pred_slots),
-            self._validation_error.data_ptr(), self._workspace.data_ptr(), self._workspace.numel(),
-            int(self.validate), 0, _native.stream(self.device))
+            self._validation_error.data_ptr(), self._iou_exact.data_ptr(), self._workspace.data_ptr(),
+            self._workspace.numel(), int(self.validate), 0, _native.stream(self.device))
         if code:
             raise_validation(code)
         return areas[:, :G] if return_areas else self
@@ -314,7 +317,7 @@ def panoptic_quality(prediction: PanopticBatch, target: PanopticBatch, num_class
         gc.data_ptr(), pc.data_ptr(), crowd.data_ptr(), G, P, C, pairs.data_ptr(), areas.data_ptr(),
         matched.data_ptr(), tp.data_ptr(), fp.data_ptr(), fn.data_ptr(), sums.data_ptr(), None,
         _native.ptr(gt_ids), _native.ptr(gt_slots), _native.ptr(pred_ids), _native.ptr(pred_slots),
-        error.data_ptr(), workspace.data_ptr(), workspace.numel(), int(validate), 1, stream)
+        error.data_ptr(), None, workspace.data_ptr(), workspace.numel(), int(validate), 1, stream)
     if code:
         raise_validation(code)
     if return_stats:

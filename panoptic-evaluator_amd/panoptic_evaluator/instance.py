@@ -1,0 +1,237 @@
+"""Individual mask inputs and COCO instance segmentation evaluation on AMD GPUs.
+
+Public surface of the reference's panoptic_evaluator/instance.py
+(InstanceBatch :13-26, InstanceEvaluator :46-227).  Mask intersections run as
+int8 MFMA contractions, matching and AP/AR accumulation in the gfx950 kernels;
+only scores, classes, ranks and match flags are kept between updates.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import NamedTuple, Sequence
+
+import torch
+
+from . import _native
+
+_i64, _int, _ptr = ctypes.c_int64, ctypes.c_int, ctypes.c_void_p
+for _name, _sig in {
+    "pe_mask_overlap": (_int, [_ptr, _i64, _ptr, _i64, _i64, _ptr, _ptr, _ptr, _ptr]),
+    "pe_detection_histogram": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _i64, _ptr, _int, _ptr]),
+    "pe_coco_match": (_int, [_i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _int,
+                             _i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr]),
+    "pe_ap_workspace_bytes": (_i64, [_i64, _i64]),
+    "pe_ap_compute": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _ptr, _ptr, _ptr, _ptr, _i64, _ptr]),
+    "pe_row_sums": (_int, [_ptr, _i64, _i64, _ptr, _ptr]),
+    "pe_instance_checks": (_int, [_i64, _ptr, _ptr, _ptr, _i64, _ptr, _ptr, _ptr, _i64, _ptr, _i64, _ptr, _ptr,
+                                  _ptr]),
+}.items():
+    _native.register(_name, *_sig)
+
+SUMMARY_KEYS = ("ap", "ap50", "ap75", "ap_small", "ap_medium", "ap_large", "ar1", "ar10", "ar100", "ar_small",
+                "ar_medium", "ar_large")
+
+
+@dataclass(frozen=True)
+class InstanceBatch:
+    """One N x H x W bool tensor per image, with N category indices.
+
+    Predictions require floating point scores.  Targets accept crowd flags and
+    annotation areas (COCO's area filtering uses annotation area for targets).
+    Masks may overlap.  Empty images use tensors with N = 0.
+    """
+
+    masks: Sequence[torch.Tensor]
+    classes: Sequence[torch.Tensor]
+    scores: Sequence[torch.Tensor] | None = None
+    crowd: Sequence[torch.Tensor] | None = None
+    areas: Sequence[torch.Tensor] | None = None
+
+
+class _PreparedImage(NamedTuple):
+    masks: torch.Tensor
+    classes: torch.Tensor         # int64, score order for predictions
+    crowd: torch.Tensor
+    annotation_areas: torch.Tensor | None
+    scores: torch.Tensor | None   # float64, score order
+    order: torch.Tensor | None    # int64 stable descending score order
+
+
+def raise_instance_error(code: int, *, combined: bool = False):
+    """Map the instance validation bitmask to the reference's messages."""
+    if code & 1:
+        raise ValueError("scores must be finite; areas must be nonnegative")
+    if code & 2:
+        raise ValueError("classes must be valid category indices")
+    if code & 16:
+        raise ValueError("instance_areas must be finite and nonnegative")
+    if code & 8:
+        raise ValueError("detection categories must be thing classes")
+    raise ValueError("each instance mask must have pixels")
+
+
+def _cat(tensors, dtype, device):
+    tensors = [t.reshape(-1).to(dtype) for t in tensors]
+    return torch.cat(tensors) if tensors else torch.empty(0, dtype=dtype, device=device)
+
+
+def offsets(counts, device):
+    out = torch.zeros(len(counts) + 1, dtype=torch.int64)
+    if counts:
+        out[1:] = torch.tensor(counts, dtype=torch.int64).cumsum(0)
+    return out.to(device)
+
+
+class InstanceEvaluator:
+    """COCO mask AP/AR with MFMA intersections and on-device greedy matching
+    (instance.py:46-227).  Only scores, matches and counts are retained."""
+
+    def __init__(self, num_classes: int, *, device: str | torch.device = "cuda", validate: bool = True):
+        if not isinstance(num_classes, int) or isinstance(num_classes, bool) or num_classes <= 0:
+            raise ValueError("num_classes must be a positive integer")
+        self.num_classes = num_classes
+        self.device = _native.require_gpu(device)
+        self.validate = validate
+        self._error = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self.reset()
+        self.device = self._counts.device  # e.g. "cuda" -> cuda:0, as instance.py:70
+
+    def reset(self):
+        self._records = []
+        self._counts = torch.zeros((self.num_classes, 4), dtype=torch.int64, device=self.device)
+
+    def _prepare(self, batch: InstanceBatch, *, prediction: bool):
+        """Shape/dtype/device checks and the stable per-image score order
+        (instance.py:78-159).  Value checks run on the device."""
+        count = len(batch.masks)
+        for field in (batch.classes, batch.scores, batch.crowd, batch.areas):
+            if field is not None and len(field) != count:
+                raise ValueError("all fields must contain one tensor per image")
+        if prediction and batch.scores is None:
+            raise ValueError("predictions require scores")
+        prepared = []
+        for b, (masks, classes) in enumerate(zip(batch.masks, batch.classes)):
+            if masks.ndim != 3 or masks.dtype != torch.bool or min(masks.shape[1:]) <= 0:
+                raise ValueError("masks must be bool N x H x W with positive image dimensions")
+            n = masks.shape[0]
+            if classes.shape != (n,) or classes.dtype not in (torch.int32, torch.int64):
+                raise ValueError("classes must be int32/int64 N tensors")
+            for name in ("scores", "crowd", "areas"):
+                field = getattr(batch, name)
+                if field is None:
+                    continue
+                value = field[b]
+                if value.shape != (n,) or value.device != self.device:
+                    raise ValueError(f"{name} must have shape N and share evaluator device")
+                if name == "crowd":
+                    if value.dtype != torch.bool:
+                        raise ValueError("crowd must be bool")
+                elif not value.is_floating_point():
+                    raise ValueError(f"{name} must be floating point")
+            if masks.device != self.device or classes.device != self.device:
+                raise ValueError("inputs must share evaluator device")
+            crowd = (batch.crowd[b] if not prediction and batch.crowd is not None
+                     else torch.zeros(n, dtype=torch.bool, device=self.device))
+            annotation = batch.areas[b].double() if not prediction and batch.areas is not None else None
+            scores, order = None, None
+            classes = classes.long()
+            if prediction:
+                scores = batch.scores[b].double()
+                order = torch.argsort(scores, descending=True, stable=True)
+                classes, scores = classes[order], scores[order]
+            prepared.append(_PreparedImage(masks.contiguous(), classes, crowd.contiguous(), annotation,
+                                           scores, order))
+        return prepared
+
+    def _commit(self, record):
+        self._records.append(record)
+
+    @torch.no_grad()
+    def update(self, prediction: InstanceBatch, target: InstanceBatch):
+        predictions = self._prepare(prediction, prediction=True)
+        targets = self._prepare(target, prediction=False)
+        if len(predictions) != len(targets) or any(
+                p.masks.shape[1:] != g.masks.shape[1:] for p, g in zip(predictions, targets)):
+            raise ValueError("prediction and target must have matching image counts and dimensions")
+        device, C = self.device, self.num_classes
+        stream = _native.stream(device)
+        # Intersections and pixel areas per image (one MFMA launch each).
+        inters, det_areas, gt_areas = [], [], []
+        for p, g in zip(predictions, targets):
+            D, G = p.masks.shape[0], g.masks.shape[0]
+            HW = p.masks.shape[1] * p.masks.shape[2]
+            inter = torch.empty(D * G, dtype=torch.int32, device=device)
+            pa = torch.empty(D, dtype=torch.int32, device=device)
+            ga = torch.empty(G, dtype=torch.int32, device=device)
+            _native.call("pe_mask_overlap", p.masks.data_ptr(), D, g.masks.data_ptr(), G, HW, inter.data_ptr(),
+                         pa.data_ptr(), ga.data_ptr(), stream)
+            inters.append(inter)
+            det_areas.append(pa[p.order].double())
+            gt_areas.append(ga.double())
+        det_cls = _cat([p.classes for p in predictions], torch.int64, device)
+        scores = _cat([p.scores for p in predictions], torch.float64, device)
+        det_area = _cat(det_areas, torch.float64, device)
+        gt_cls = _cat([g.classes for g in targets], torch.int64, device)
+        gt_area = _cat(gt_areas, torch.float64, device)
+        ann = _cat([g.annotation_areas if g.annotation_areas is not None else a
+                    for g, a in zip(targets, gt_areas)], torch.float64, device)
+        if self.validate:
+            _native.call("pe_instance_checks", det_cls.numel(), det_cls.data_ptr(), scores.data_ptr(),
+                         det_area.data_ptr(), gt_cls.numel(), gt_cls.data_ptr(), gt_area.data_ptr(), ann.data_ptr(),
+                         C, None, 0, None, self._error.data_ptr(), stream)
+            code = int(self._error.item())
+            if code:
+                raise_instance_error(code)
+        det_counts = [p.masks.shape[0] for p in predictions]
+        gt_counts = [g.masks.shape[0] for g in targets]
+        det_off, gt_off = offsets(det_counts, device), offsets(gt_counts, device)
+        inter_off = offsets([d * g for d, g in zip(det_counts, gt_counts)], device)
+        inter = _cat(inters, torch.int32, device)
+        det_row = _cat([p.order for p in predictions], torch.int64, device)
+        N = det_cls.numel()
+        status = torch.empty((N, 40), dtype=torch.int8, device=device)
+        ranks = torch.empty(N, dtype=torch.int32, device=device)
+        det_cls32, gt_cls32 = det_cls.int(), gt_cls.int()
+        crowd = _cat([g.crowd for g in targets], torch.bool, device)
+        _native.call("pe_coco_match", len(predictions), C, det_off.data_ptr(), gt_off.data_ptr(),
+                     inter_off.data_ptr(), det_cls32.data_ptr(), det_area.data_ptr(), det_row.data_ptr(),
+                     gt_cls32.data_ptr(), gt_area.data_ptr(), crowd.data_ptr(), ann.data_ptr(), inter.data_ptr(), 4,
+                     N, max(gt_counts, default=0), status.data_ptr(), ranks.data_ptr(), self._counts.data_ptr(),
+                     None, stream)
+        self._commit((scores, det_cls32, ranks, status))
+        return self
+
+    @torch.no_grad()
+    def compute(self):
+        """COCO AP/AR summaries and the full precision/recall arrays."""
+        return accumulate(self._records, self._counts, self.num_classes, self.device)
+
+
+def accumulate(records, counts, num_classes, device):
+    """Sort all records by score and accumulate AP/AR on the device
+    (instance.py:205-248)."""
+    C = num_classes
+    if records:
+        scores = torch.cat([r[0] for r in records])
+        classes = torch.cat([r[1] for r in records])
+        ranks = torch.cat([r[2] for r in records])
+        status = torch.cat([r[3] for r in records])
+    else:
+        scores = torch.empty(0, dtype=torch.float64, device=device)
+        classes = torch.empty(0, dtype=torch.int32, device=device)
+        ranks = torch.empty(0, dtype=torch.int32, device=device)
+        status = torch.empty((0, 40), dtype=torch.int8, device=device)
+    N = scores.numel()
+    precision = torch.empty((10, 101, C, 4, 3), dtype=torch.float64, device=device)
+    recall = torch.empty((10, C, 4, 3), dtype=torch.float64, device=device)
+    summary = torch.empty(12, dtype=torch.float64, device=device)
+    size = max(_native.library().pe_ap_workspace_bytes(N, C), 256)
+    workspace = torch.empty(size, dtype=torch.uint8, device=device)
+    _native.call("pe_ap_compute", N, scores.data_ptr(), classes.data_ptr(), ranks.data_ptr(), status.data_ptr(),
+                 counts.data_ptr(), C, precision.data_ptr(), recall.data_ptr(), summary.data_ptr(),
+                 workspace.data_ptr(), workspace.numel(), _native.stream(device))
+    result = {key: summary[i] for i, key in enumerate(SUMMARY_KEYS)}
+    result.update(precision=precision, recall=recall)
+    return result

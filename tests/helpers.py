@@ -29,9 +29,10 @@ def assert_close(actual, expected, rtol=1e-12, what=""):
     np.testing.assert_allclose(actual, expected, rtol=rtol, atol=1e-15, equal_nan=True, err_msg=what)
 
 
-def compare_panoptic(actual, expected, *, exact_iou_sum=True):
-    """GPU compute() dict vs oracle compute() dict: integers bit-exact,
-    iou_sum bit-exact (deterministic in-order fold), the rest <= 1e-12 rel."""
+def compare_panoptic(actual, expected, *, exact_iou_sum=False):
+    """GPU compute() dict vs oracle compute() dict: integers bit-exact; iou_sum
+    and the ratios <= 1e-12 relative (the GPU's dataset iou_sum is the exact
+    sum rounded once, the oracle's an in-order FP64 fold)."""
     for key in INT_KEYS:
         assert_exact(actual[key], expected[key], key)
     if exact_iou_sum:
