@@ -85,9 +85,66 @@ class PanopticConfig:
         return done / elapsed, f"{done} of {self.images} images, oracle update+compute"
 
 
+class CombinedConfig:
+    """Plain SegmentationEvaluator reset/update/compute over one batch: PQ,
+    mIoU and mask AP with the panoptic target's thing segments as AP ground
+    truth (the metric's own workload)."""
+
+    dominant = "detection_histogram"
+    dtype = "u8 masks / int32 maps (int64 counts, f64 ratios)"
+
+    def __init__(self, name, description, **sizes):
+        self.name, self.description, self.sizes = name, description, sizes
+
+    def setup(self, device):
+        import workloads
+        import panoptic_evaluator as pe
+        self.pair, self.masks, self.classes, self.scores = workloads.config1(device=device, **self.sizes)
+        self.isthing = workloads.isthing_vector(133, 80)
+        self.evaluator = pe.SegmentationEvaluator(133, isthing=self.isthing, validate=False)
+        self.target = pe.PanopticBatch(self.pair.gt, self.pair.gc, self.pair.crowd)
+        self.prediction = pe.PanopticBatch(self.pair.pred, self.pair.pc)
+        self.detections = pe.InstanceBatch(self.masks, self.classes, self.scores)
+        B, H, W = self.pair.gt.shape
+        self.images = B
+        dets = sum(m.shape[0] for m in self.masks)
+        # detection masks (1 B/px each) + the int32 target map, per image
+        self.bytes_per_launch = H * W * (dets + 4 * B)
+        self.config = {"workload": self.description, "images": B, "height": H, "width": W, "classes": 133,
+                       "detections_per_image": dets // B, "gt_slots": int(self.pair.gc.shape[1]),
+                       "pred_slots": int(self.pair.pc.shape[1]), "validate": False,
+                       "update_bytes_per_image": H * W * (8 + dets // B)}
+
+    def step(self):
+        self.evaluator.reset()
+        self.evaluator.update(self.prediction, self.detections, self.target)
+        return self.evaluator.compute()
+
+    def cpu_sample(self, budget_s):
+        from oracle import combined as ocomb
+        state = ocomb.SegmentationState(133, self.isthing, validate=False)
+        done, elapsed = 0, 0.0
+        while done < self.images and elapsed < budget_s:
+            b = done  # the device-to-host copy of the inputs is not timed
+            pair = [t[b:b + 1].cpu().numpy() for t in (self.pair.gt, self.pair.pred, self.pair.gc, self.pair.pc,
+                                                       self.pair.crowd)]
+            det = [dict(masks=self.masks[b].cpu().numpy(), classes=self.classes[b].cpu().numpy(),
+                        scores=self.scores[b].cpu().numpy())]
+            t0 = time.perf_counter()
+            state.update(pair[1], pair[3], det, pair[0], pair[2], pair[4])
+            elapsed += time.perf_counter() - t0
+            done += 1
+        t0 = time.perf_counter()
+        state.compute()
+        elapsed += time.perf_counter() - t0
+        return done / elapsed, f"{done} of {self.images} images, oracle SegmentationEvaluator update+compute"
+
+
 def make_configs():
     import workloads
     return {
+        "c1": CombinedConfig("c1", "c1: SegmentationEvaluator, 16 x 800x1088, 133 classes, 80 Voronoi seeds, "
+                                   "100 ellipse detections per image"),
         "c0": PanopticConfig("c0", workloads.config0, 133, 80,
                              "c0: PanopticEvaluator, 8 x 512x512, 133 classes, 40 Voronoi seeds"),
         "c3": PanopticConfig("c3", workloads.config3, 19, 8,
@@ -95,7 +152,7 @@ def make_configs():
     }
 
 
-DEFAULT_CONFIG = "c3"
+DEFAULT_CONFIG = "c1"
 
 
 # ------------------------------------------------------------------ main

@@ -49,7 +49,9 @@ int64_t pe_update_workspace_bytes(int64_t B, int64_t G, int64_t P, int64_t C);
  *                   (int64).  Compact slots, or original IDs when the
  *                   matching ID/slot lookup tables (B x G / B x P) are given.
  *   gc, pc          B x G / B x P int32 classes; crowd B x G uint8 (bool).
- *   pairs           B x G x P int64 scratch (fully rewritten).
+ *   pairs           B x G x P int64 scratch.  pairs_zeroed != 0 promises it
+ *                   is all zero on entry (as every call leaves it), which
+ *                   saves a fill; otherwise it is zeroed first.
  *   areas           B x (G + P) int64 scratch: GT then predicted slot areas.
  *   matched         B x G int32 scratch.
  *   tp, fp, fn, iou_sum   C (or B x C when reward != 0) accumulators.
@@ -74,7 +76,7 @@ int pe_update(const void* gt, int gt_bytes, const void* pred, int pred_bytes,
               const int64_t* gt_ids, const int32_t* gt_slots,
               const int64_t* pred_ids, const int32_t* pred_slots,
               int32_t* error, uint64_t* iou_exact, void* workspace, int64_t workspace_bytes,
-              int validate, int reward, void* stream);
+              int validate, int reward, int pairs_zeroed, void* stream);
 
 /* FP64 finaliser over K independent accumulator rows.  Replaces ``compute``
  * (bindings.cpp:10-11; compute_cuda at compute_kernel.cu:66-95; K > 1 is
@@ -136,7 +138,10 @@ int pe_detection_histogram(int64_t B, const uint8_t* const* masks, const int64_t
  * det_row (may be NULL) gives the block row of each detection.  Outputs:
  * status (sum D) x 40 int8 [area*10 + threshold], ranks (sum D) int32 (1..100,
  * else 0), counts C x 4 int64 (accumulated).  guard (may be NULL): nothing is
- * written when *guard != 0.  Replaces ``instance_update`` /
+ * written when *guard != 0.  gt_things (C uint8, may be NULL) keeps only GT
+ * of thing classes as candidates; gt_area_i64 (may be NULL, then gt_area is
+ * used) reads int64 pixel areas at b * gt_area_stride + j; gt_ann may be NULL
+ * (pixel areas drive the area ranges).  Replaces ``instance_update`` /
  * ``instance_match_histogram`` matching (bindings.cpp:13-15, :20-22;
  * match_instances, instance_kernel.cu:41-174; CPU _match_ious_cpu,
  * instance.py:267-297). */
@@ -145,7 +150,21 @@ int pe_coco_match(int64_t B, int64_t C, const int64_t* det_off, const int64_t* g
                   const int64_t* det_row, const int32_t* gt_cls, const double* gt_area,
                   const uint8_t* gt_crowd, const double* gt_ann, const void* inter, int inter_bytes,
                   int64_t total_dets, int64_t max_gt, int8_t* status, int32_t* ranks, int64_t* counts,
-                  const int32_t* guard, void* stream);
+                  const int32_t* guard, const uint8_t* gt_things, const int64_t* gt_area_i64,
+                  int64_t gt_area_stride, void* stream);
+
+/* Scratch bytes of pe_score_order for N detections in B images. */
+int64_t pe_score_order_workspace_bytes(int64_t N, int64_t B);
+
+/* Per-image stable descending score order (ties keep input order) of N
+ * detections grouped by det_off (B + 1, device): order[i] = within-image
+ * index of sorted position i; scores/classes gathered into that order
+ * (classes also as int32).  Replaces the per-image
+ * torch.argsort(scores, descending=True, stable=True) of instance.py:140. */
+int pe_score_order(int64_t B, int64_t N, const int64_t* det_off, const double* scores,
+                   const int64_t* classes, int64_t* order, double* sorted_scores,
+                   int64_t* sorted_classes, int32_t* sorted_classes32, void* workspace,
+                   int64_t workspace_bytes, void* stream);
 
 /* Scratch bytes of pe_ap_compute for N detection records and C classes. */
 int64_t pe_ap_workspace_bytes(int64_t N, int64_t C);

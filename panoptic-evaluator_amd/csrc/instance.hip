@@ -243,7 +243,25 @@ struct MatchArgs {
   int max_gt;
   int words;                  // used-bitmask words per chain
   const int* guard;           // skip everything when *guard != 0
+  const uint8_t* gt_things;   // optional: only GT of thing classes are candidates
+  const i64* gt_area_i64;     // optional: int64 pixel areas at b * gt_area_stride + j
+  i64 gt_area_stride;
 };
+
+// GT candidate class (or -1), pixel area and annotation area of GT j of image b.
+__device__ __forceinline__ int gt_class(const MatchArgs& a, i64 g) {
+  const int c = a.gt_cls[g];
+  if (a.gt_things && (c < 0 || c >= a.C || !a.gt_things[c])) return -1;
+  return c;
+}
+
+__device__ __forceinline__ double gt_pixel_area(const MatchArgs& a, int b, i64 g0, int j) {
+  return a.gt_area_i64 ? static_cast<double>(a.gt_area_i64[b * a.gt_area_stride + j]) : a.gt_area[g0 + j];
+}
+
+__device__ __forceinline__ double gt_ann_area(const MatchArgs& a, int b, i64 g0, int j) {
+  return a.gt_ann ? a.gt_ann[g0 + j] : gt_pixel_area(a, b, g0, j);
+}
 
 __device__ __forceinline__ void area_range(int a, double& lo, double& hi) {
   lo = a == 2 ? 1024.0 : a == 3 ? 9216.0 : 0.0;
@@ -288,12 +306,12 @@ __global__ __launch_bounds__(64) void coco_match(MatchArgs a) {
   int positives[4] = {0, 0, 0, 0};
   for (int base = 0; base < G; base += kWave) {
     const int j = base + lane;
-    const bool sel = j < G && a.gt_cls[g0 + j] == c;
+    const bool sel = j < G && gt_class(a, g0 + j) == c;
     const u64 mask = __ballot(sel);
     if (sel) gts[ng + __popcll(mask & ((1ULL << lane) - 1))] = j;
     ng += __popcll(mask);
-    const bool eligible = sel && !a.gt_crowd[g0 + j];
-    const double area = eligible ? a.gt_ann[g0 + j] : 0.0;
+    const bool eligible = sel && !(a.gt_crowd && a.gt_crowd[g0 + j]);
+    const double area = eligible ? gt_ann_area(a, b, g0, j) : 0.0;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       double lo, hi;
@@ -322,15 +340,15 @@ __global__ __launch_bounds__(64) void coco_match(MatchArgs a) {
     double best_iou = thr;
     for (int q = 0; q < ng; ++q) {
       const int j = gts[q];
-      const bool crowd = a.gt_crowd[g0 + j] != 0;
+      const bool crowd = a.gt_crowd && a.gt_crowd[g0 + j] != 0;
       if (!crowd && ((taken[q >> 5] >> (q & 31)) & 1u)) continue;
       const i64 idx = ib + row * G + j;
       const double inter = a.inter_bytes == 8 ? static_cast<double>(static_cast<const i64*>(a.inter)[idx])
                                               : static_cast<double>(static_cast<const int32_t*>(a.inter)[idx]);
-      double uni = crowd ? pa : pa + a.gt_area[g0 + j] - inter;
+      double uni = crowd ? pa : pa + gt_pixel_area(a, b, g0, j) - inter;
       if (uni < 1.0) uni = 1.0;
       const double iou = inter / uni;
-      const double ann = a.gt_ann[g0 + j];
+      const double ann = gt_ann_area(a, b, g0, j);
       const int ignore = crowd || ann < lo || ann > hi;
       // Non-ignored GT first, then larger IoU (>= running best), later
       // metadata entry on equal IoU: COCO's scan order.
@@ -399,70 +417,106 @@ struct ApArgs {
   double* partial;           // (C*120) x 2: valid-precision sum, count
 };
 
-// One thread per (t, c, a, m): instance.py:300-331 for one threshold row.
+// One workgroup per class; thread j < 120 owns (t, area, maxDets) =
+// (j / 12, (j / 3) % 4, j % 3).  The class's records (score order) are staged
+// through LDS in chunks: rank and the 40 match flags per record.  Forward pass:
+// totals; backward pass: precision envelope at the 101 recall thresholds
+// (instance.py:300-331).
+constexpr int kApChunk = 1024;
+
 __global__ __launch_bounds__(128) void ap_accumulate(ApArgs a) {
-  const int job = blockIdx.x * blockDim.x + threadIdx.x;
-  if (job >= a.C * 120) return;
-  const int m = job % 3, ar = (job / 3) % 4, c = (job / 12) % a.C, t = job / (12 * a.C);
+  __shared__ uint8_t s_rank[kApChunk];
+  __shared__ int8_t s_flag[kApChunk * 40];
+  const int c = blockIdx.x, job = threadIdx.x;
+  const bool active = job < 120;
+  const int m = job % 3, ar = (job / 3) % 4, t = job / 12;
   const int limit = m == 0 ? 1 : m == 1 ? 10 : 100;
-  const i64 positives = a.counts[c * 4 + ar];
+  const i64 positives = active ? a.counts[c * 4 + ar] : 0;
   const i64 pstride = static_cast<i64>(a.C) * 12;
+  const i64 global_job = static_cast<i64>(t) * pstride + c * 12 + ar * 3 + m;
   double* prec = a.precision + static_cast<i64>(t) * 101 * pstride + (c * 4 + ar) * 3 + m;
-  double* rec = a.recall + (static_cast<i64>(t) * a.C + c) * 12 + ar * 3 + m;
-  double* part = a.partial + 2 * static_cast<i64>(job);
+  double* rec = a.recall + global_job;
+  double* part = a.partial + 2 * global_job;
+  const int s0 = a.start[c], s1 = a.end[c];
+  const int lane_flag = ar * 10 + t;
+  auto stage = [&](int first, int count) {
+    for (int i = threadIdx.x; i < count; i += blockDim.x) {
+      const int idx = a.order[first + i];
+      s_rank[i] = static_cast<uint8_t>(a.ranks[idx]);
+    }
+    const int bytes = count * 40;
+    for (int e = threadIdx.x; e < bytes; e += blockDim.x) {
+      const int i = e / 40;
+      s_flag[e] = a.status[static_cast<i64>(a.order[first + i]) * 40 + (e - i * 40)];
+    }
+  };
+  // forward: totals
+  i64 tp = 0, fp = 0, n = 0;
+  for (int first = s0; first < s1; first += kApChunk) {
+    const int count = min(kApChunk, s1 - first);
+    __syncthreads();
+    stage(first, count);
+    __syncthreads();
+    if (active && positives)
+      for (int i = 0; i < count; ++i) {
+        if (s_rank[i] > limit) continue;
+        const int f = s_flag[i * 40 + lane_flag];
+        tp += f == 1;
+        fp += f == 0;
+        ++n;
+      }
+  }
+  const bool scan = active && positives && n > 0;
+  const double npos = static_cast<double>(positives);
+  double env = 0.0, sum = 0.0;
+  int r = 100;
+  i64 ctp = tp, cfp = fp, remaining = n;
+  if (scan)
+    while (r >= 0 && static_cast<double>(r) * 0.01 > static_cast<double>(ctp) / npos) {
+      prec[r * pstride] = 0.0;
+      --r;
+    }
+  // backward: precision[r] = max(pr[pos_r:]), pos_r = first record with
+  // recall >= r / 100 (reusing the staged chunk when the class fits in one).
+  const int last_first = s1 > s0 ? s0 + ((s1 - s0 - 1) / kApChunk) * kApChunk : s0;
+  for (int first = last_first; first >= s0 && s1 > s0; first -= kApChunk) {
+    const int count = min(kApChunk, s1 - first);
+    if (s1 - s0 > kApChunk) {
+      __syncthreads();
+      stage(first, count);
+      __syncthreads();
+    }
+    if (scan)
+      for (int i = count - 1; i >= 0 && r >= 0; --i) {
+        if (s_rank[i] > limit) continue;
+        const double pr = static_cast<double>(ctp) / (static_cast<double>(ctp + cfp) + DBL_EPSILON);
+        env = fmax(env, pr);
+        const int f = s_flag[i * 40 + lane_flag];
+        ctp -= f == 1;
+        cfp -= f == 0;
+        --remaining;
+        const double prev = remaining > 0 ? static_cast<double>(ctp) / npos : -1.0;
+        while (r >= 0 && static_cast<double>(r) * 0.01 > prev) {
+          prec[r * pstride] = env;
+          sum += env;
+          --r;
+        }
+      }
+  }
+  if (!active) return;
   if (!positives) {
-    for (int r = 0; r <= 100; ++r) prec[r * pstride] = -1.0;
+    for (int q = 0; q <= 100; ++q) prec[q * pstride] = -1.0;
     *rec = -1.0;
     part[0] = 0.0;
     part[1] = 0.0;
     return;
   }
-  const int lane_flag = ar * 10 + t;
-  const int s0 = a.start[c], s1 = a.end[c];
-  // forward: totals
-  i64 tp = 0, fp = 0, n = 0;
-  for (int i = s0; i < s1; ++i) {
-    const int idx = a.order[i];
-    if (a.ranks[idx] > limit) continue;
-    const int f = a.status[static_cast<i64>(idx) * 40 + lane_flag];
-    tp += f == 1;
-    fp += f == 0;
-    ++n;
-  }
   if (n == 0) {
-    for (int r = 0; r <= 100; ++r) prec[r * pstride] = 0.0;
+    for (int q = 0; q <= 100; ++q) prec[q * pstride] = 0.0;
     *rec = 0.0;
     part[0] = 0.0;
     part[1] = 101.0;
     return;
-  }
-  const double npos = static_cast<double>(positives);
-  // precision[r] = max(pr[pos_r:]) where pos_r is the first record whose
-  // recall reaches r/100 (0 when none does), filled backwards with the
-  // running maximum of precision.
-  double env = 0.0, sum = 0.0;
-  int r = 100;
-  i64 ctp = tp, cfp = fp, remaining = n;
-  while (r >= 0 && static_cast<double>(r) * 0.01 > static_cast<double>(ctp) / npos) {
-    prec[r * pstride] = 0.0;
-    --r;
-  }
-  for (int i = s1 - 1; i >= s0 && r >= 0; --i) {
-    const int idx = a.order[i];
-    if (a.ranks[idx] > limit) continue;
-    const double pr = static_cast<double>(ctp) / (static_cast<double>(ctp + cfp) + DBL_EPSILON);
-    env = fmax(env, pr);
-    const int f = a.status[static_cast<i64>(idx) * 40 + lane_flag];
-    ctp -= f == 1;
-    cfp -= f == 0;
-    --remaining;
-    // recall of the previous record; none before the first one
-    const double prev = remaining > 0 ? static_cast<double>(ctp) / npos : -1.0;
-    while (r >= 0 && static_cast<double>(r) * 0.01 > prev) {
-      prec[r * pstride] = env;
-      sum += env;
-      --r;
-    }
   }
   *rec = static_cast<double>(tp) / npos;
   part[0] = sum;
@@ -470,36 +524,49 @@ __global__ __launch_bounds__(128) void ap_accumulate(ApArgs a) {
 }
 
 // instance.py:236-248: the 12 summaries as means over valid entries (>= 0).
-__global__ __launch_bounds__(256) void ap_summaries(int C, const double* precision_partial, const double* recall,
-                                                    double* out) {
-  __shared__ double red[24][256];
+__global__ __launch_bounds__(512) void ap_summaries(int C, const double* precision_partial, const double* recall,
+                                                     double* out) {
+  __shared__ double red[24][512];
   const int tid = threadIdx.x;
   double acc[24];
 #pragma unroll
   for (int j = 0; j < 24; ++j) acc[j] = 0.0;
   const int jobs = C * 120;
-  for (int job = tid; job < jobs; job += 256) {
-    const int m = job % 3, ar = (job / 3) % 4, t = job / (12 * C);
-    const double s = precision_partial[2 * job], n = precision_partial[2 * job + 1];
-    if (m == 2) {
-      if (ar == 0) {
-        acc[0] += s; acc[1] += n;                       // ap
-        if (t == 0) { acc[2] += s; acc[3] += n; }       // ap50
-        if (t == 5) { acc[4] += s; acc[5] += n; }       // ap75
-      } else {
-        acc[4 + 2 * ar] += s; acc[5 + 2 * ar] += n;     // ap_small/medium/large: 6..11
-      }
+  constexpr int U = 4;
+  for (int j0 = tid; j0 < jobs; j0 += 512 * U) {
+    double s[U], n[U], rv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int job = j0 + u * 512;
+      const bool ok = job < jobs;
+      s[u] = ok ? precision_partial[2 * job] : 0.0;
+      n[u] = ok ? precision_partial[2 * job + 1] : 0.0;
+      rv[u] = ok ? recall[job] : -1.0;
     }
-    const double rv = recall[(static_cast<i64>(t) * C + job / 12 % C) * 12 + ar * 3 + m];
-    if (rv >= 0) {
-      if (ar == 0) { acc[12 + 2 * m] += rv; acc[13 + 2 * m] += 1.0; }          // ar1/ar10/ar100: 12..17
-      else if (m == 2) { acc[16 + 2 * ar] += rv; acc[17 + 2 * ar] += 1.0; }   // ar_small..: 18..23
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int job = j0 + u * 512;
+      if (job >= jobs) break;
+      const int m = job % 3, ar = (job / 3) % 4, t = job / (12 * C);
+      if (m == 2) {
+        if (ar == 0) {
+          acc[0] += s[u]; acc[1] += n[u];                       // ap
+          if (t == 0) { acc[2] += s[u]; acc[3] += n[u]; }       // ap50
+          if (t == 5) { acc[4] += s[u]; acc[5] += n[u]; }       // ap75
+        } else {
+          acc[4 + 2 * ar] += s[u]; acc[5 + 2 * ar] += n[u];     // ap_small/medium/large: 6..11
+        }
+      }
+      if (rv[u] >= 0) {
+        if (ar == 0) { acc[12 + 2 * m] += rv[u]; acc[13 + 2 * m] += 1.0; }          // ar1/ar10/ar100
+        else if (m == 2) { acc[16 + 2 * ar] += rv[u]; acc[17 + 2 * ar] += 1.0; }   // ar_small..large
+      }
     }
   }
 #pragma unroll
   for (int j = 0; j < 24; ++j) red[j][tid] = acc[j];
   __syncthreads();
-  for (int stride = 128; stride > 0; stride >>= 1) {
+  for (int stride = 256; stride > 0; stride >>= 1) {
     if (tid < stride)
 #pragma unroll
       for (int j = 0; j < 24; ++j) red[j][tid] += red[j][tid + stride];
@@ -579,9 +646,11 @@ extern "C" int pe_coco_match(int64_t B, int64_t C, const int64_t* det_off, const
                              const int64_t* det_row, const int32_t* gt_cls, const double* gt_area,
                              const uint8_t* gt_crowd, const double* gt_ann, const void* inter, int inter_bytes,
                              int64_t total_dets, int64_t max_gt, int8_t* status, int32_t* ranks, int64_t* counts,
-                             const int32_t* guard, void* stream_ptr) {
+                             const int32_t* guard, const uint8_t* gt_things, const int64_t* gt_area_i64,
+                             int64_t gt_area_stride, void* stream_ptr) {
   PE_REQUIRE(B >= 0 && C > 0 && C < INT_MAX && B * C < INT_MAX && max_gt >= 0, "invalid matching shapes");
   PE_REQUIRE(inter_bytes == 4 || inter_bytes == 8, "intersections must be int32 or int64");
+  PE_REQUIRE(max_gt == 0 || gt_area || gt_area_i64, "GT pixel areas are required");
   hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
   int rc;
   // status = -1, ranks = 0 for every detection (ranks > 100 stay 0 as on the CPU path)
@@ -611,6 +680,9 @@ extern "C" int pe_coco_match(int64_t B, int64_t C, const int64_t* det_off, const
   a.max_gt = static_cast<int>(max_gt);
   a.words = static_cast<int>((max_gt + 31) / 32);
   a.guard = guard;
+  a.gt_things = gt_things;
+  a.gt_area_i64 = reinterpret_cast<const i64*>(gt_area_i64);
+  a.gt_area_stride = gt_area_stride;
   const size_t lds = 4 * (100 + static_cast<size_t>(max_gt)) + 4 * 40 * static_cast<size_t>(a.words) + 16;
   PE_REQUIRE(lds <= 150 * 1024, "too many ground-truth instances per image for the matcher");
   if (lds > 64 * 1024)
@@ -640,10 +712,10 @@ struct ApBuffers {
 
 size_t sort_temp_bytes(i64 N) {
   size_t a = 0, b = 0;
-  hipcub::DeviceRadixSort::SortPairs(nullptr, a, static_cast<u64*>(nullptr), static_cast<u64*>(nullptr),
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, a, static_cast<u64*>(nullptr), static_cast<u64*>(nullptr),
                                      static_cast<int32_t*>(nullptr), static_cast<int32_t*>(nullptr),
                                      static_cast<int>(N));
-  hipcub::DeviceRadixSort::SortPairs(nullptr, b, static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr),
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, b, static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr),
                                      static_cast<int32_t*>(nullptr), static_cast<int32_t*>(nullptr),
                                      static_cast<int>(N));
   return std::max(a, b);
@@ -716,10 +788,10 @@ extern "C" int pe_ap_compute(int64_t N, const double* scores, const int32_t* cla
   a.partial = w.partial;
   {
     ProfileScope timing("ap_accumulate", stream);
-    ap_accumulate<<<static_cast<unsigned>((C * 120 + 127) / 128), 128, 0, stream>>>(a);
+    ap_accumulate<<<static_cast<unsigned>(C), 128, 0, stream>>>(a);
     PE_LAUNCH_CHECK();
   }
-  ap_summaries<<<1, 256, 0, stream>>>(static_cast<int>(C), w.partial, recall, summary);
+  ap_summaries<<<1, 512, 0, stream>>>(static_cast<int>(C), w.partial, recall, summary);
   PE_LAUNCH_CHECK();
   return kOk;
 }
@@ -797,6 +869,89 @@ extern "C" int pe_instance_checks(int64_t n_det, const int64_t* det_cls, const d
   instance_checks<<<blocks, 256, 0, stream>>>(n_det, reinterpret_cast<const i64*>(det_cls), scores, det_area, n_gt,
                                               reinterpret_cast<const i64*>(gt_cls), gt_area, ann, static_cast<int>(C),
                                               things, n_inst, inst_areas, error);
+  PE_LAUNCH_CHECK();
+  return kOk;
+}
+
+// ------------------------------------------------------- per-image score order
+
+namespace pe {
+
+__global__ void score_sort_keys(const double* scores, i64 N, u64* keys, int32_t* idx) {
+  for (i64 i = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += static_cast<i64>(gridDim.x) * blockDim.x) {
+    keys[i] = score_key_desc(scores[i]);
+    idx[i] = static_cast<int32_t>(i);
+  }
+}
+
+// One workgroup per image: gather classes/scores into score order and record
+// the within-image mask index of every sorted position.
+__global__ __launch_bounds__(256) void gather_score_order(const i64* det_off, const int32_t* sorted_idx,
+                                                          const double* scores, const i64* classes, i64* order,
+                                                          double* sorted_scores, i64* sorted_classes,
+                                                          int32_t* sorted_classes32) {
+  const int b = blockIdx.x;
+  const i64 first = det_off[b], last = det_off[b + 1];
+  for (i64 i = first + threadIdx.x; i < last; i += blockDim.x) {
+    const int32_t src = sorted_idx[i];
+    order[i] = src - first;
+    sorted_scores[i] = scores[src];
+    const i64 c = classes[src];
+    sorted_classes[i] = c;
+    sorted_classes32[i] = static_cast<int32_t>(c);
+  }
+}
+
+namespace {
+size_t segmented_sort_bytes(i64 N, i64 B) {
+  size_t bytes = 0;
+  (void)hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, bytes, static_cast<u64*>(nullptr), static_cast<u64*>(nullptr),
+                                              static_cast<int32_t*>(nullptr), static_cast<int32_t*>(nullptr),
+                                              static_cast<int>(N), static_cast<int>(B),
+                                              static_cast<const i64*>(nullptr), static_cast<const i64*>(nullptr));
+  return bytes;
+}
+}  // namespace
+
+}  // namespace pe
+
+extern "C" int64_t pe_score_order_workspace_bytes(int64_t N, int64_t B) {
+  Carve carve(nullptr);
+  carve.take<u64>(N);
+  carve.take<u64>(N);
+  carve.take<int32_t>(N);
+  carve.take<int32_t>(N);
+  carve.take<char>(N > 0 && B > 0 ? segmented_sort_bytes(N, B) : 0);
+  return static_cast<int64_t>(align_up(carve.used, 256));
+}
+
+extern "C" int pe_score_order(int64_t B, int64_t N, const int64_t* det_off, const double* scores,
+                              const int64_t* classes, int64_t* order, double* sorted_scores,
+                              int64_t* sorted_classes, int32_t* sorted_classes32, void* workspace,
+                              int64_t workspace_bytes, void* stream_ptr) {
+  PE_REQUIRE(B >= 0 && B < INT_MAX && N >= 0 && N < INT_MAX, "invalid score-order shapes");
+  PE_REQUIRE(workspace_bytes >= pe_score_order_workspace_bytes(N, B), "workspace too small");
+  if (N == 0 || B == 0) return kOk;
+  hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+  Carve carve(workspace);
+  u64* keys_in = carve.take<u64>(N);
+  u64* keys_out = carve.take<u64>(N);
+  int32_t* idx_in = carve.take<int32_t>(N);
+  int32_t* idx_out = carve.take<int32_t>(N);
+  size_t temp_bytes = segmented_sort_bytes(N, B);
+  void* temp = carve.take<char>(temp_bytes);
+  const int blocks = static_cast<int>(std::min<i64>((N + 255) / 256, 4096));
+  score_sort_keys<<<blocks, 256, 0, stream>>>(scores, N, keys_in, idx_in);
+  PE_LAUNCH_CHECK();
+  // Stable LSD radix sort per image: descending score, ties in input order
+  // (torch.argsort(descending=True, stable=True), instance.py:140).
+  const i64* off = reinterpret_cast<const i64*>(det_off);
+  PE_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(temp, temp_bytes, keys_in, keys_out, idx_in, idx_out,
+                                                     static_cast<int>(N), static_cast<int>(B), off, off + 1, 0, 64,
+                                                     stream));
+  gather_score_order<<<static_cast<unsigned>(B), 256, 0, stream>>>(
+      off, idx_out, scores, reinterpret_cast<const i64*>(classes), reinterpret_cast<i64*>(order), sorted_scores,
+      reinterpret_cast<i64*>(sorted_classes), sorted_classes32);
   PE_LAUNCH_CHECK();
   return kOk;
 }

@@ -105,14 +105,11 @@ struct KeyCache {
   }
 };
 
-__device__ void score_image(const ScoreArgs& a, int b, char* lds, size_t lds_bytes);
-__device__ void finish_images(const ScoreArgs& a);
-
-// FUSE: the last workgroup to finish an image's chunks scores that image (and
-// the last image's scorer folds iou_sum), so update() is a single launch
-// after the zero fill when it does not validate.
-template <typename GT, typename PR, bool LOOKUP, bool VEC, bool GLOBAL, bool FUSE>
-__global__ __launch_bounds__(kHistThreads) void joint_histogram(HistArgs a, ScoreArgs s) {
+// The pair table must be zero on entry (the scorer re-zeroes what it read).
+// Completion is published by the kernel boundary: a per-workgroup
+// device-scope fence would write back the XCD's L2 for every chunk.
+template <typename GT, typename PR, bool LOOKUP, bool VEC, bool GLOBAL>
+__global__ __launch_bounds__(kHistThreads) void joint_histogram(HistArgs a) {
   extern __shared__ uint4 lds4[];
   uint32_t* bins = reinterpret_cast<uint32_t*>(lds4);
   const int b = blockIdx.x / a.chunks_per_image;
@@ -200,19 +197,6 @@ __global__ __launch_bounds__(kHistThreads) void joint_histogram(HistArgs a, Scor
       }
     }
   }
-  if (FUSE) {
-    __shared__ int last_of_image;
-    __threadfence();
-    __syncthreads();  // also retires every LDS read of the flush above
-    if (tid == 0)
-      last_of_image = atomicAdd(a.image_done + b, 1u) == static_cast<unsigned>(a.chunks_per_image) - 1;
-    __syncthreads();
-    if (!last_of_image) return;
-    __threadfence();
-    if (tid == 0) a.image_done[b] = 0;
-    score_image(s, b, reinterpret_cast<char*>(lds4), a.lds_bytes);
-    if (!s.reward) finish_images(s);
-  }
 }
 
 template <typename K>
@@ -222,23 +206,22 @@ static bool allow_lds(K kernel, size_t bytes) {
                              static_cast<int>(bytes)) == hipSuccess;
 }
 
-template <typename GT, typename PR, bool LOOKUP, bool VEC, bool GLOBAL, bool FUSE>
-static int launch_hist_kernel(const HistArgs& a, const ScoreArgs& s, int blocks, size_t lds,
-                              hipStream_t stream) {
-  auto kernel = joint_histogram<GT, PR, LOOKUP, VEC, GLOBAL, FUSE>;
+template <typename GT, typename PR, bool LOOKUP, bool VEC, bool GLOBAL>
+static int launch_hist_kernel(const HistArgs& a, int blocks, size_t lds, hipStream_t stream) {
+  auto kernel = joint_histogram<GT, PR, LOOKUP, VEC, GLOBAL>;
   if (!allow_lds(kernel, lds)) return fail(kUnsupportedShape, "LDS request of %zu bytes refused", lds);
   ProfileScope timing("joint_histogram", stream);
-  kernel<<<blocks, kHistThreads, lds, stream>>>(a, s);
+  kernel<<<blocks, kHistThreads, lds, stream>>>(a);
   PE_LAUNCH_CHECK();
   return kOk;
 }
 
-template <typename GT, typename PR, bool LOOKUP, bool FUSE>
-static int launch_hist_typed(const HistArgs& a, const ScoreArgs& s, int blocks, size_t lds, bool vec,
-                             bool global, hipStream_t stream) {
-  if (global) return launch_hist_kernel<GT, PR, LOOKUP, false, true, FUSE>(a, s, blocks, lds, stream);
-  if (vec) return launch_hist_kernel<GT, PR, LOOKUP, true, false, FUSE>(a, s, blocks, lds, stream);
-  return launch_hist_kernel<GT, PR, LOOKUP, false, false, FUSE>(a, s, blocks, lds, stream);
+template <typename GT, typename PR, bool LOOKUP>
+static int launch_hist_typed(const HistArgs& a, int blocks, size_t lds, bool vec, bool global,
+                             hipStream_t stream) {
+  if (global) return launch_hist_kernel<GT, PR, LOOKUP, false, true>(a, blocks, lds, stream);
+  if (vec) return launch_hist_kernel<GT, PR, LOOKUP, true, false>(a, blocks, lds, stream);
+  return launch_hist_kernel<GT, PR, LOOKUP, false, false>(a, blocks, lds, stream);
 }
 
 int hist_default_chunk(i64 total_pixels) {
@@ -251,10 +234,8 @@ int hist_default_chunk(i64 total_pixels) {
   return static_cast<int>(chunk);
 }
 
-int launch_joint_histogram(HistArgs a, int gt_bytes, int pred_bytes, int chunk, hipStream_t stream,
-                           const ScoreArgs* fuse) {
-  if (a.B == 0) return kOk;
-  if (a.HW == 0) return fuse ? launch_score(*fuse, stream) : kOk;
+int launch_joint_histogram(HistArgs a, int gt_bytes, int pred_bytes, int chunk, hipStream_t stream) {
+  if (a.B == 0 || a.HW == 0) return kOk;
   if (chunk <= 0) chunk = hist_default_chunk(a.HW * a.B);
   chunk = std::min<int>(std::max<int>(chunk & ~3, 4), kMaxChunk);
   a.chunk = chunk;
@@ -267,25 +248,15 @@ int launch_joint_histogram(HistArgs a, int gt_bytes, int pred_bytes, int chunk, 
   size_t lds = static_cast<size_t>(a.words4) * 16;
   bool global = lds > kMaxHistLds;
   if (global) lds = 0;
-  ScoreArgs s{};
-  if (fuse) {
-    s = *fuse;
-    size_t stage = stage_bytes(a.G, a.P);
-    if (stage <= kMaxStageLds && stage > lds) lds = stage;
-    s.lds_bytes = lds;
-  }
   a.lds_bytes = lds;
   bool lookup = a.gt_ids || a.pred_ids;
   bool vec = gt_bytes == 4 && pred_bytes == 4 && (reinterpret_cast<uintptr_t>(a.gt) % 16) == 0 &&
              (reinterpret_cast<uintptr_t>(a.pred) % 16) == 0;
   int nb = static_cast<int>(blocks);
-#define PE_HIST(GT, PR)                                                                               \
-  do {                                                                                                \
-    if (fuse)                                                                                         \
-      return lookup ? launch_hist_typed<GT, PR, true, true>(a, s, nb, lds, vec, global, stream)       \
-                    : launch_hist_typed<GT, PR, false, true>(a, s, nb, lds, vec, global, stream);     \
-    return lookup ? launch_hist_typed<GT, PR, true, false>(a, s, nb, lds, vec, global, stream)        \
-                  : launch_hist_typed<GT, PR, false, false>(a, s, nb, lds, vec, global, stream);      \
+#define PE_HIST(GT, PR)                                                                   \
+  do {                                                                                    \
+    return lookup ? launch_hist_typed<GT, PR, true>(a, nb, lds, vec, global, stream)     \
+                  : launch_hist_typed<GT, PR, false>(a, nb, lds, vec, global, stream);   \
   } while (0)
   if (gt_bytes == 4 && pred_bytes == 4) PE_HIST(int32_t, int32_t);
   if (gt_bytes == 4 && pred_bytes == 8) PE_HIST(int32_t, int64_t);
@@ -306,7 +277,9 @@ struct Stage {
   int* matched;
   int* lab;
   uint8_t* crowd;
+  uint32_t* cnt;      // G x P pair counts in LDS (32-bit: per-image counts < 2^32)
   bool staged;
+  bool staged_pairs;
 };
 
 __host__ __device__ inline size_t stage_bytes(int G, int P) {
@@ -334,7 +307,13 @@ __device__ Stage make_stage(const ScoreArgs& a, int b, char* lds, size_t lds_byt
     st.lab = reinterpret_cast<int*>(p);
     p += 4 * static_cast<size_t>(P);
     st.crowd = reinterpret_cast<uint8_t*>(p);
+    p += static_cast<size_t>(G);
+    const size_t used = (static_cast<size_t>(p - lds) + 15) & ~static_cast<size_t>(15);
+    st.staged_pairs = used + 4 * static_cast<size_t>(G) * P <= lds_bytes;
+    st.cnt = st.staged_pairs ? reinterpret_cast<uint32_t*>(lds + used) : nullptr;
   } else {
+    st.staged_pairs = false;
+    st.cnt = nullptr;
     i64* areas = a.areas + static_cast<i64>(b) * (G + P);
     st.ga = areas;
     st.pa = areas + G;
@@ -349,7 +328,8 @@ __device__ Stage make_stage(const ScoreArgs& a, int b, char* lds, size_t lds_byt
 
 // Row sums (GT areas) with one wave per row; column sums (prediction areas)
 // with one lane per column and eight loads in flight.
-__device__ void image_areas(const u64* pairs, int G, int P, i64* areas, i64* ga, i64* pa) {
+template <typename T>
+__device__ void image_areas(const T* pairs, int G, int P, i64* areas, i64* ga, i64* pa) {
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const int waves = blockDim.x / kWave;
   for (int g = wave; g < G; g += waves) {
@@ -365,7 +345,7 @@ __device__ void image_areas(const u64* pairs, int G, int P, i64* areas, i64* ga,
     i64 s = 0;
     int g = 0;
     for (; g + 8 <= G; g += 8) {
-      u64 v[8];
+      T v[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j) v[j] = pairs[static_cast<i64>(g + j) * P + p];
 #pragma unroll
@@ -431,26 +411,16 @@ __global__ __launch_bounds__(kScoreThreads) void prepare_images(ScoreArgs a, int
 
 // Score one image with the whole workgroup: areas, PQ matching, confusion,
 // TP/FP/FN and the per-image matched-IoU records.
-__device__ void score_image(const ScoreArgs& a, int b, char* lds, size_t lds_bytes) {
+template <typename T>
+__device__ void score_core(const ScoreArgs& a, int b, const Stage& st, const T* pairs) {
   const int G = a.G, P = a.P, C = a.C;
   const int tid = threadIdx.x, nthreads = blockDim.x;
-  const i64 GP = static_cast<i64>(G) * P;
-  const u64* pairs = a.pairs + b * GP;
   const int* pc = a.pc + static_cast<i64>(b) * P;
   i64* areas = a.areas + static_cast<i64>(b) * (G + P);
   const i64 row = a.reward ? static_cast<i64>(b) * C : 0;
   u64* tp = a.tp + row;
   u64* fp = a.fp + row;
   u64* fn = a.fn + row;
-  Stage st = make_stage(a, b, lds, lds_bytes);
-  if (st.staged) {
-    const int* gc = a.gc + static_cast<i64>(b) * G;
-    const uint8_t* crowd = a.crowd + static_cast<i64>(b) * G;
-    for (int g = tid; g < G; g += nthreads) {
-      st.gc[g] = gc[g];
-      st.crowd[g] = crowd[g];
-    }
-  }
   if (a.areas_ready) {
     for (int g = tid; g < G; g += nthreads) st.ga[g] = areas[g];
     for (int p = tid; p < P; p += nthreads) st.pa[p] = areas[G + p];
@@ -468,7 +438,7 @@ __device__ void score_image(const ScoreArgs& a, int b, char* lds, size_t lds_byt
     bool pm = false;
     double sum_iou = 0.0;
     for (int g0 = 0; g0 < G; g0 += 8) {
-      u64 v[8];
+      T v[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j) v[j] = g0 + j < G ? pairs[static_cast<i64>(g0 + j) * P + p] : 0;
 #pragma unroll
@@ -536,6 +506,31 @@ __device__ void score_image(const ScoreArgs& a, int b, char* lds, size_t lds_byt
   }
 }
 
+__device__ void score_image(const ScoreArgs& a, int b, char* lds, size_t lds_bytes) {
+  const int G = a.G, P = a.P;
+  const int tid = threadIdx.x, nthreads = blockDim.x;
+  const i64 GP = static_cast<i64>(G) * P;
+  const u64* pairs = a.pairs + b * GP;
+  Stage st = make_stage(a, b, lds, lds_bytes);
+  if (st.staged) {
+    const int* gc = a.gc + static_cast<i64>(b) * G;
+    const uint8_t* crowd = a.crowd + static_cast<i64>(b) * G;
+    for (int g = tid; g < G; g += nthreads) {
+      st.gc[g] = gc[g];
+      st.crowd[g] = crowd[g];
+    }
+  }
+  if (st.staged_pairs) {
+    // One coalesced copy of the pair table into LDS; every loop below then
+    // reads LDS instead of waiting on global memory.
+    for (i64 e = tid; e < GP; e += nthreads) st.cnt[e] = static_cast<uint32_t>(pairs[e]);
+    __syncthreads();
+    score_core<uint32_t>(a, b, st, st.cnt);
+  } else {
+    score_core<u64>(a, b, st, pairs);
+  }
+}
+
 // iou_sum[c] = the exact accumulated sum rounded once to FP64.
 __device__ __forceinline__ double exact_to_double(u64 lo, u64 hi) {
   return (static_cast<double>(hi) * 4294967296.0 + static_cast<double>(lo)) * 0x1p-53;
@@ -546,35 +541,39 @@ __device__ void convert_iou_sum(const ScoreArgs& a) {
     a.iou_sum[c] = exact_to_double(a.iou_exact[2 * c], a.iou_exact[2 * c + 1]);
 }
 
-// Called by every workgroup after it scored an image: the last one (over the
-// whole launch) folds iou_sum in image order.  Dataset mode only.
-__device__ void finish_images(const ScoreArgs& a) {
-  __shared__ int last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) last = atomicAdd(a.counter, 1u) == static_cast<unsigned>(a.B) - 1;
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  convert_iou_sum(a);
-  if (threadIdx.x == 0) *a.counter = 0;
-}
-
 __global__ __launch_bounds__(kScoreThreads) void score_images(ScoreArgs a) {
   extern __shared__ uint4 lds4[];
   if (a.guard && *a.guard) return;  // a failed validation never mutates totals
   score_image(a, blockIdx.x, reinterpret_cast<char*>(lds4), a.lds_bytes);
-  if (!a.reward) finish_images(a);
+  // Leave this image's pair table zeroed for the next update.
+  __syncthreads();
+  const i64 GP = static_cast<i64>(a.G) * a.P;
+  u64* pairs = const_cast<u64*>(a.pairs) + blockIdx.x * GP;
+  for (i64 e = threadIdx.x; e < GP; e += blockDim.x) pairs[e] = 0;
+}
+
+__global__ __launch_bounds__(kScoreThreads) void convert_iou(ScoreArgs a) {
+  if (a.guard && *a.guard) return;
+  convert_iou_sum(a);
 }
 
 int launch_score(ScoreArgs a, hipStream_t stream) {
   if (a.B == 0) return kOk;
-  size_t lds = stage_bytes(a.G, a.P);
+  // Metadata plus the 32-bit pair table when they fit, else metadata only.
+  size_t lds = stage_bytes(a.G, a.P) + 4 * static_cast<size_t>(a.G) * a.P;
+  if (lds > kMaxStageLds) lds = stage_bytes(a.G, a.P);
   if (lds > kMaxStageLds) lds = 0;
+  if (!allow_lds(score_images, lds)) lds = 0;
   a.lds_bytes = lds;
-  ProfileScope timing("score_images", stream);
-  score_images<<<a.B, kScoreThreads, lds, stream>>>(a);
-  PE_LAUNCH_CHECK();
+  {
+    ProfileScope timing("score_images", stream);
+    score_images<<<a.B, kScoreThreads, lds, stream>>>(a);
+    PE_LAUNCH_CHECK();
+  }
+  if (!a.reward) {
+    convert_iou<<<1, kScoreThreads, 0, stream>>>(a);
+    PE_LAUNCH_CHECK();
+  }
   return kOk;
 }
 
@@ -606,13 +605,25 @@ __global__ __launch_bounds__(kScoreThreads) void finalize(FinalArgs a) {
   __shared__ double red[14][kScoreThreads];
   for (int c = tid; c < 2 * C; c += kScoreThreads) rowsum[c] = 0;
   __syncthreads();
-  for (i64 i = tid; i < CC; i += kScoreThreads) {
-    const i64 v = conf[i];
-    if (a.out_confusion) a.out_confusion[k * CC + i] = v;
-    if (!v) continue;
-    const int r = static_cast<int>(i / (C + 1)), col = static_cast<int>(i - static_cast<i64>(r) * (C + 1));
-    atomicAdd(rowsum + r, static_cast<u64>(v));
-    if (col < C) atomicAdd(colsum + col, static_cast<u64>(v));
+  // Eight loads in flight per lane: the sweep is latency-bound otherwise.
+  constexpr int U = 8;
+  for (i64 i0 = tid; i0 < CC; i0 += static_cast<i64>(kScoreThreads) * U) {
+    i64 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const i64 i = i0 + static_cast<i64>(u) * kScoreThreads;
+      v[u] = i < CC ? conf[i] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const i64 i = i0 + static_cast<i64>(u) * kScoreThreads;
+      if (i >= CC) break;
+      if (a.out_confusion) a.out_confusion[k * CC + i] = v[u];
+      if (!v[u]) continue;
+      const int r = static_cast<int>(i / (C + 1)), col = static_cast<int>(i - static_cast<i64>(r) * (C + 1));
+      atomicAdd(rowsum + r, static_cast<u64>(v[u]));
+      if (col < C) atomicAdd(colsum + col, static_cast<u64>(v[u]));
+    }
   }
   __syncthreads();
   double part[14];
@@ -735,8 +746,6 @@ extern "C" int64_t pe_update_workspace_bytes(int64_t B, int64_t G, int64_t P, in
   Carve carve(nullptr);
   carve.take<int>(B * P);        // rec_lab (prediction labels, large G/P only)
   carve.take<double>(B * P);     // rec_val
-  carve.take<unsigned>(64);      // counter
-  carve.take<unsigned>(B);       // image_done
   return static_cast<int64_t>(align_up(carve.used, 256));
 }
 
@@ -746,7 +755,8 @@ extern "C" int pe_update(const void* gt, int gt_bytes, const void* pred, int pre
                          int64_t* tp, int64_t* fp, int64_t* fn, double* iou_sum, int64_t* confusion,
                          const int64_t* gt_ids, const int32_t* gt_slots, const int64_t* pred_ids,
                          const int32_t* pred_slots, int32_t* error, uint64_t* iou_exact, void* workspace,
-                         int64_t workspace_bytes, int validate, int reward, void* stream_ptr) {
+                         int64_t workspace_bytes, int validate, int reward, int pairs_zeroed,
+                         void* stream_ptr) {
   PE_REQUIRE(B >= 0 && H >= 0 && W >= 0, "negative shape");
   PE_REQUIRE(G > 0 && G < INT_MAX && P > 0 && P < INT_MAX && C > 0 && C < INT_MAX, "invalid capacities");
   PE_REQUIRE(B < INT_MAX && G * P < (1LL << 32), "pair histogram too large");
@@ -773,18 +783,15 @@ extern "C" int pe_update(const void* gt, int gt_bytes, const void* pred, int pre
   s.confusion = reinterpret_cast<u64*>(confusion);
   s.rec_lab = carve.take<int>(B * P);
   s.rec_val = carve.take<double>(B * P);
-  s.counter = carve.take<unsigned>(64);
   s.iou_exact = reinterpret_cast<u64*>(iou_exact);
-  unsigned* image_done = carve.take<unsigned>(B);
   s.B = static_cast<int>(B);
   s.G = static_cast<int>(G);
   s.P = static_cast<int>(P);
   s.C = static_cast<int>(C);
   s.reward = reward;
-  int rc = launch_zero(pairs, static_cast<size_t>(B * G * P) * 8, stream);
-  if (rc) return rc;
-  if ((rc = launch_zero(s.counter, 256, stream))) return rc;
-  if ((rc = launch_zero(image_done, 4 * static_cast<size_t>(B), stream))) return rc;
+  const size_t pair_bytes = static_cast<size_t>(B * G * P) * 8;
+  int rc = kOk;
+  if (!pairs_zeroed && (rc = launch_zero(pairs, pair_bytes, stream))) return rc;
   if (validate && (rc = launch_zero(error, 4, stream))) return rc;
   HistArgs h{};
   h.gt = gt;
@@ -799,15 +806,17 @@ extern "C" int pe_update(const void* gt, int gt_bytes, const void* pred, int pre
   h.pred_ids = pred_ids;
   h.pred_slots = pred_slots;
   h.error = validate ? error : nullptr;
-  h.image_done = image_done;
-  if (!validate) return launch_joint_histogram(h, gt_bytes, pred_bytes, 0, stream, &s);
-  if ((rc = launch_joint_histogram(h, gt_bytes, pred_bytes, 0, stream, nullptr))) return rc;
-  {
+  if ((rc = launch_joint_histogram(h, gt_bytes, pred_bytes, 0, stream))) return rc;
+  if (validate) {
     if ((rc = launch_prepare(s, error, stream))) return rc;
     int32_t host_error = 0;
     PE_HIP(hipMemcpyAsync(&host_error, error, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
     PE_HIP(hipStreamSynchronize(stream));
-    if (host_error) return host_error;  // before any accumulator changes
+    if (host_error) {
+      // Nothing accumulated; restore the zeroed pair table for the next call.
+      if ((rc = launch_zero(pairs, pair_bytes, stream))) return rc;
+      return host_error;
+    }
     s.areas_ready = 1;
   }
   return launch_score(s, stream);

@@ -27,7 +27,6 @@ struct HistArgs {
   const i64* pred_ids;
   const int* pred_slots;
   int* error;            // nullptr: no validation
-  unsigned* image_done;  // B counters, zero between launches (fused scoring)
   size_t lds_bytes;
 };
 
@@ -46,7 +45,6 @@ struct ScoreArgs {
   int* rec_lab;
   double* rec_val;
   u64* iou_exact;        // C x 2: exact 2^-53 fixed-point IoU sums (dataset mode)
-  unsigned* counter;
   const int* guard;
   int B, G, P, C;
   int reward;
@@ -74,9 +72,8 @@ struct FinalArgs {
 };
 
 int hist_default_chunk(i64 total_pixels);
-// fuse != nullptr scores every image inside the same launch (no validation).
-int launch_joint_histogram(HistArgs a, int gt_bytes, int pred_bytes, int chunk, hipStream_t stream,
-                           const ScoreArgs* fuse);
+// The pair table must be zero on entry; launch_score leaves it zeroed again.
+int launch_joint_histogram(HistArgs a, int gt_bytes, int pred_bytes, int chunk, hipStream_t stream);
 int launch_prepare(const ScoreArgs& a, int* error, hipStream_t stream);
 int launch_score(ScoreArgs a, hipStream_t stream);
 int launch_finalize(const FinalArgs& a, i64 K, hipStream_t stream);

@@ -28,7 +28,7 @@ _SIGNATURES = {
     "pe_update_workspace_bytes": (_i64, [_i64] * 4),
     "pe_update": (_int, [_ptr, _int, _ptr, _int, _i64, _i64, _i64, _ptr, _ptr, _ptr, _i64, _i64, _i64,
                          _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr,
-                         _ptr, _ptr, _ptr, _i64, _int, _int, _ptr]),
+                         _ptr, _ptr, _ptr, _i64, _int, _int, _int, _ptr]),
     "pe_compute": (_int, [_ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _ptr, _ptr, _ptr, _ptr,
                           _ptr, _ptr, _ptr, _ptr, _ptr, _ptr]),
     "pe_image_pq": (_int, [_ptr, _ptr, _ptr, _ptr, _i64, _i64, _ptr, _ptr, _ptr]),

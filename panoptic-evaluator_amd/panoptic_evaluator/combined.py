@@ -15,7 +15,7 @@ import torch
 
 from . import _native
 from .evaluator import PanopticBatch, PanopticEvaluator, _prepare
-from .instance import InstanceBatch, InstanceEvaluator, _cat, offsets, raise_instance_error
+from .instance import InstanceBatch, InstanceEvaluator, raise_instance_error, score_order
 
 
 class SegmentationEvaluator:
@@ -58,44 +58,38 @@ class SegmentationEvaluator:
         B, H, W = maps.shape
         S = classes.shape[1]
         HW = H * W
-        det_counts = [image.masks.shape[0] for image in prepared]
-        N = sum(det_counts)
-        det_off = offsets(det_counts, device)
-        order = _cat([image.order for image in prepared], torch.int64, device)
+        ranked = score_order(prepared, device)
+        det_counts, N, det_off = ranked.counts, sum(ranked.counts), ranked.det_off
         pointers = torch.tensor([image.masks.data_ptr() for image in prepared], dtype=torch.int64).to(device)
         aligned = int(HW % 16 == 0 and all(image.masks.data_ptr() % 16 == 0 for image in prepared))
         pairs = torch.empty((N, S), dtype=torch.int64, device=device)
-        _native.call("pe_detection_histogram", B, pointers.data_ptr(), det_off.data_ptr(), order.data_ptr(),
+        _native.call("pe_detection_histogram", B, pointers.data_ptr(), det_off.data_ptr(), ranked.order.data_ptr(),
                      maps.data_ptr(), HW, S, max(det_counts, default=0), N, pairs.data_ptr(), aligned, stream)
         # Full detection areas, including void and stuff pixels.
         det_area = torch.empty(N, dtype=torch.float64, device=device)
         _native.call("pe_row_sums", pairs.data_ptr(), N, S, det_area.data_ptr(), stream)
-        det_cls = _cat([image.classes for image in prepared], torch.int64, device)
-        scores = _cat([image.scores for image in prepared], torch.float64, device)
+        det_cls, scores, det_cls32 = ranked.classes, ranked.scores, ranked.classes32
+        inst = None if instance_areas is None else instance_areas.double().contiguous()
         if self.validate:
-            inst = instance_areas.double().contiguous() if instance_areas is not None else None
             _native.call("pe_instance_checks", N, det_cls.data_ptr(), scores.data_ptr(), det_area.data_ptr(), 0,
                          None, None, None, C, self.panoptic.isthing.data_ptr(),
                          0 if inst is None else inst.numel(), _native.ptr(inst), self._error.data_ptr(), stream)
             code = int(self._error.item())
             if code:
                 raise_instance_error(code, combined=True)
-        # Panoptic totals change only after the instance inputs passed.
-        target_areas = self.panoptic._update(prediction, target, return_areas=True)
-        isthing = self.panoptic.isthing
-        valid = classes.clamp(0, C - 1).long()
-        gt_cls = torch.where((classes >= 0) & isthing[valid], classes, -1).int().contiguous()
-        gt_area = target_areas.double().contiguous()
-        ann = gt_area if instance_areas is None else instance_areas.double().contiguous()
+        # Panoptic totals change only after the instance inputs passed.  The
+        # matcher reads the target slot areas straight from the panoptic
+        # update's int64 area table and keeps only thing-class slots.
+        areas = self.panoptic._update(prediction, target, return_areas=True)
         status = torch.empty((N, 40), dtype=torch.int8, device=device)
         ranks = torch.empty(N, dtype=torch.int32, device=device)
-        det_cls32 = det_cls.int()
         gt_off = torch.arange(0, (B + 1) * S, S, dtype=torch.int64, device=device)
-        inter_off = offsets([d * S for d in det_counts], device)
+        inter_off = det_off * S
         _native.call("pe_coco_match", B, C, det_off.data_ptr(), gt_off.data_ptr(), inter_off.data_ptr(),
-                     det_cls32.data_ptr(), det_area.data_ptr(), None, gt_cls.data_ptr(), gt_area.data_ptr(),
-                     crowd.data_ptr(), ann.data_ptr(), pairs.data_ptr(), 8, N, S, status.data_ptr(),
-                     ranks.data_ptr(), self.instance._counts.data_ptr(), None, stream)
+                     det_cls32.data_ptr(), det_area.data_ptr(), None, classes.data_ptr(), None,
+                     _native.ptr(crowd), _native.ptr(inst), pairs.data_ptr(), 8, N, S, status.data_ptr(),
+                     ranks.data_ptr(), self.instance._counts.data_ptr(), None, self.panoptic.isthing.data_ptr(),
+                     areas.data_ptr(), areas.stride(0), stream)
         self.instance._commit((scores, det_cls32, ranks, status))
         return self
 

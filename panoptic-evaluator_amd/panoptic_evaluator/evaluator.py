@@ -212,7 +212,8 @@ class PanopticEvaluator:
         shape = (B, G, P)
         if self._scratch is None or self._scratch[0].shape != shape:
             device = self.device
-            self._scratch = (torch.empty(shape, dtype=torch.int64, device=device),
+            # The pair table starts zeroed and every update leaves it zeroed.
+            self._scratch = (torch.zeros(shape, dtype=torch.int64, device=device),
                              torch.empty((B, G + P), dtype=torch.int64, device=device),
                              torch.empty((B, G), dtype=torch.int32, device=device))
             size = _native.library().pe_update_workspace_bytes(B, G, P, self.num_classes)
@@ -242,10 +243,10 @@ class PanopticEvaluator:
             self.fp.data_ptr(), self.fn.data_ptr(), self.iou_sum.data_ptr(), self.confusion.data_ptr(),
             _native.ptr(gt_ids), _native.ptr(gt_slots), _native.ptr(pred_ids), _native.ptr(pred_slots),
             self._validation_error.data_ptr(), self._iou_exact.data_ptr(), self._workspace.data_ptr(),
-            self._workspace.numel(), int(self.validate), 0, _native.stream(self.device))
+            self._workspace.numel(), int(self.validate), 0, 1, _native.stream(self.device))
         if code:
             raise_validation(code)
-        return areas[:, :G] if return_areas else self
+        return areas if return_areas else self
 
     @torch.no_grad()
     def compute(self) -> dict[str, torch.Tensor | dict]:
@@ -317,7 +318,7 @@ def panoptic_quality(prediction: PanopticBatch, target: PanopticBatch, num_class
         gc.data_ptr(), pc.data_ptr(), crowd.data_ptr(), G, P, C, pairs.data_ptr(), areas.data_ptr(),
         matched.data_ptr(), tp.data_ptr(), fp.data_ptr(), fn.data_ptr(), sums.data_ptr(), None,
         _native.ptr(gt_ids), _native.ptr(gt_slots), _native.ptr(pred_ids), _native.ptr(pred_slots),
-        error.data_ptr(), None, workspace.data_ptr(), workspace.numel(), int(validate), 1, stream)
+        error.data_ptr(), None, workspace.data_ptr(), workspace.numel(), int(validate), 1, 0, stream)
     if code:
         raise_validation(code)
     if return_stats:

@@ -21,7 +21,9 @@ for _name, _sig in {
     "pe_mask_overlap": (_int, [_ptr, _i64, _ptr, _i64, _i64, _ptr, _ptr, _ptr, _ptr]),
     "pe_detection_histogram": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _i64, _ptr, _int, _ptr]),
     "pe_coco_match": (_int, [_i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _int,
-                             _i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr]),
+                             _i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _ptr]),
+    "pe_score_order_workspace_bytes": (_i64, [_i64, _i64]),
+    "pe_score_order": (_int, [_i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _ptr]),
     "pe_ap_workspace_bytes": (_i64, [_i64, _i64]),
     "pe_ap_compute": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _ptr, _ptr, _ptr, _ptr, _i64, _ptr]),
     "pe_row_sums": (_int, [_ptr, _i64, _i64, _ptr, _ptr]),
@@ -52,11 +54,42 @@ class InstanceBatch:
 
 class _PreparedImage(NamedTuple):
     masks: torch.Tensor
-    classes: torch.Tensor         # int64, score order for predictions
+    classes: torch.Tensor         # input order
     crowd: torch.Tensor
     annotation_areas: torch.Tensor | None
-    scores: torch.Tensor | None   # float64, score order
-    order: torch.Tensor | None    # int64 stable descending score order
+    scores: torch.Tensor | None   # input order
+    order: torch.Tensor | None    # unused (kept for the record layout)
+
+
+class ScoreOrder(NamedTuple):
+    """Stable descending score order of every detection of a batch."""
+
+    counts: list                  # detections per image
+    det_off: torch.Tensor         # B + 1 (device)
+    order: torch.Tensor           # within-image mask index per sorted position (int64)
+    scores: torch.Tensor          # float64, sorted
+    classes: torch.Tensor         # int64, sorted (validation)
+    classes32: torch.Tensor       # int32, sorted (matching, records)
+
+
+def score_order(images, device) -> ScoreOrder:
+    """One segmented radix sort over all images (instance.py:140 per image)."""
+    counts = [image.masks.shape[0] for image in images]
+    B, N = len(images), sum(counts)
+    det_off = offsets(counts, device)
+    scores = _cat([image.scores for image in images], torch.float64, device)
+    classes = _cat([image.classes for image in images], torch.int64, device)
+    order = torch.empty(N, dtype=torch.int64, device=device)
+    sorted_scores = torch.empty(N, dtype=torch.float64, device=device)
+    sorted_classes = torch.empty(N, dtype=torch.int64, device=device)
+    sorted_classes32 = torch.empty(N, dtype=torch.int32, device=device)
+    if N:
+        size = max(_native.library().pe_score_order_workspace_bytes(N, B), 256)
+        workspace = torch.empty(size, dtype=torch.uint8, device=device)
+        _native.call("pe_score_order", B, N, det_off.data_ptr(), scores.data_ptr(), classes.data_ptr(),
+                     order.data_ptr(), sorted_scores.data_ptr(), sorted_classes.data_ptr(),
+                     sorted_classes32.data_ptr(), workspace.data_ptr(), workspace.numel(), _native.stream(device))
+    return ScoreOrder(counts, det_off, order, sorted_scores, sorted_classes, sorted_classes32)
 
 
 def raise_instance_error(code: int, *, combined: bool = False):
@@ -135,14 +168,11 @@ class InstanceEvaluator:
             crowd = (batch.crowd[b] if not prediction and batch.crowd is not None
                      else torch.zeros(n, dtype=torch.bool, device=self.device))
             annotation = batch.areas[b].double() if not prediction and batch.areas is not None else None
-            scores, order = None, None
-            classes = classes.long()
-            if prediction:
-                scores = batch.scores[b].double()
-                order = torch.argsort(scores, descending=True, stable=True)
-                classes, scores = classes[order], scores[order]
+            # Predictions stay in input order here; ScoreOrder sorts every
+            # image of the batch in one segmented radix sort.
+            scores = batch.scores[b] if prediction else None
             prepared.append(_PreparedImage(masks.contiguous(), classes, crowd.contiguous(), annotation,
-                                           scores, order))
+                                           scores, None))
         return prepared
 
     def _commit(self, record):
@@ -157,7 +187,10 @@ class InstanceEvaluator:
             raise ValueError("prediction and target must have matching image counts and dimensions")
         device, C = self.device, self.num_classes
         stream = _native.stream(device)
-        # Intersections and pixel areas per image (one MFMA launch each).
+        ranked = score_order(predictions, device)
+        det_cls, scores, det_cls32 = ranked.classes, ranked.scores, ranked.classes32
+        # Intersections and pixel areas per image (one MFMA launch each),
+        # in input (mask) order.
         inters, det_areas, gt_areas = [], [], []
         for p, g in zip(predictions, targets):
             D, G = p.masks.shape[0], g.masks.shape[0]
@@ -168,11 +201,11 @@ class InstanceEvaluator:
             _native.call("pe_mask_overlap", p.masks.data_ptr(), D, g.masks.data_ptr(), G, HW, inter.data_ptr(),
                          pa.data_ptr(), ga.data_ptr(), stream)
             inters.append(inter)
-            det_areas.append(pa[p.order].double())
+            det_areas.append(pa)
             gt_areas.append(ga.double())
-        det_cls = _cat([p.classes for p in predictions], torch.int64, device)
-        scores = _cat([p.scores for p in predictions], torch.float64, device)
-        det_area = _cat(det_areas, torch.float64, device)
+        # Detection pixel areas in score order.
+        base = torch.repeat_interleave(ranked.det_off[:-1], torch.tensor(ranked.counts, device=device))
+        det_area = _cat(det_areas, torch.float64, device)[ranked.order + base]
         gt_cls = _cat([g.classes for g in targets], torch.int64, device)
         gt_area = _cat(gt_areas, torch.float64, device)
         ann = _cat([g.annotation_areas if g.annotation_areas is not None else a
@@ -184,22 +217,22 @@ class InstanceEvaluator:
             code = int(self._error.item())
             if code:
                 raise_instance_error(code)
-        det_counts = [p.masks.shape[0] for p in predictions]
+        det_counts = ranked.counts
         gt_counts = [g.masks.shape[0] for g in targets]
-        det_off, gt_off = offsets(det_counts, device), offsets(gt_counts, device)
+        det_off, gt_off = ranked.det_off, offsets(gt_counts, device)
         inter_off = offsets([d * g for d, g in zip(det_counts, gt_counts)], device)
         inter = _cat(inters, torch.int32, device)
-        det_row = _cat([p.order for p in predictions], torch.int64, device)
+        det_row = ranked.order  # intersection rows are in mask order
         N = det_cls.numel()
         status = torch.empty((N, 40), dtype=torch.int8, device=device)
         ranks = torch.empty(N, dtype=torch.int32, device=device)
-        det_cls32, gt_cls32 = det_cls.int(), gt_cls.int()
+        gt_cls32 = gt_cls.int()
         crowd = _cat([g.crowd for g in targets], torch.bool, device)
         _native.call("pe_coco_match", len(predictions), C, det_off.data_ptr(), gt_off.data_ptr(),
                      inter_off.data_ptr(), det_cls32.data_ptr(), det_area.data_ptr(), det_row.data_ptr(),
                      gt_cls32.data_ptr(), gt_area.data_ptr(), crowd.data_ptr(), ann.data_ptr(), inter.data_ptr(), 4,
                      N, max(gt_counts, default=0), status.data_ptr(), ranks.data_ptr(), self._counts.data_ptr(),
-                     None, stream)
+                     None, None, None, 0, stream)
         self._commit((scores, det_cls32, ranks, status))
         return self
 

@@ -407,11 +407,16 @@ def test_config3_histogram_invariants(pe):
     areas = ev._update(pe.PanopticBatch(pair.pred, pair.pc), pe.PanopticBatch(pair.gt, pair.gc, pair.crowd),
                        return_areas=True)
     B, H, W = pair.gt.shape
-    pairs = ev._scratch[0]
-    assert pairs.sum().item() == B * H * W
-    G = pair.gc.shape[1]
+    G, P = pair.gc.shape[1], pair.pc.shape[1]
+    # Slot areas of both maps; each image's areas add up to its pixel count.
+    assert torch.equal(areas[:, :G].sum(1), torch.full((B,), H * W, device=areas.device))
+    assert torch.equal(areas[:, G:].sum(1), torch.full((B,), H * W, device=areas.device))
     want = torch.stack([torch.bincount(pair.gt[b].flatten().long(), minlength=G) for b in range(B)])
-    assert torch.equal(areas, want)
+    assert torch.equal(areas[:, :G], want)
+    want_pred = torch.stack([torch.bincount(pair.pred[b].flatten().long(), minlength=P) for b in range(B)])
+    assert torch.equal(areas[:, G:], want_pred)
+    # The pair table is handed back zeroed for the next update.
+    assert not ev._scratch[0].any().item()
     conf = ev.compute()["confusion"]
     valid = (pair.gc >= 0) & ~pair.crowd
     assert conf.sum().item() == int((want * valid).sum().item())
