@@ -42,6 +42,7 @@ class PanopticConfig:
     """Plain PanopticEvaluator reset/update/compute over one batch."""
 
     dominant = "joint_histogram"
+    tuned = ("joint_histogram",)
     dtype = "int32"
 
     def __init__(self, name, maker, classes, things, description):
@@ -91,6 +92,7 @@ class CombinedConfig:
     truth (the metric's own workload)."""
 
     dominant = "detection_histogram"
+    tuned = ("detection_histogram", "joint_histogram")
     dtype = "u8 masks / int32 maps (int64 counts, f64 ratios)"
 
     def __init__(self, name, description, **sizes):
@@ -191,6 +193,8 @@ def main(argv=None):
     parser.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU oracle sample")
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--copy-calibration", action="store_true")
+    parser.add_argument("--no-autotune", action="store_true",
+                        help="keep the default launch shapes instead of tuning them on this workload")
     args = parser.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the evaluation path shards by image with one "
@@ -211,6 +215,19 @@ def main(argv=None):
     for _ in range(args.warmup):
         cfg.step()
     torch.cuda.synchronize()
+    tuning_report = None
+    if not args.no_autotune:
+        from panoptic_evaluator import tuning
+        log("[bench] autotuning launch shapes (untimed)")
+        report = tuning.autotune(cfg.step, cfg.tuned, repeats=3)
+        tuning_report = {kernel: {"variant": report["selection"][kernel],
+                                  "describe": report["describe"][kernel][report["selection"][kernel]],
+                                  "ms_per_variant": [None if ms is None else round(ms, 5)
+                                                     for ms in report["timings"][kernel]]}
+                         for kernel in cfg.tuned}
+        for _ in range(args.warmup):
+            cfg.step()
+        torch.cuda.synchronize()
     lib = _native.library()
     lib.pe_profile_reset()
     lib.pe_profile_enable(1)
@@ -260,6 +277,7 @@ def main(argv=None):
             "launches": launches,
         },
         "cpu_baseline": cpu,
+        "tuning": tuning_report,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -105,4 +105,19 @@ struct ProfileScope {
   ~ProfileScope();
 };
 
+// Launch-shape tuning.  Each tunable kernel has a fixed list of variants
+// (template instantiations and launch shapes, all computing identical
+// results); the selected variant is used by every later launch in the
+// process.  Selection is process-wide and meant to be set once, before
+// launches are issued (the autotuner times each variant and keeps the best).
+enum TuneKernel { kTuneDetectionHistogram = 0, kTuneJointHistogram, kTuneMaskOverlap, kTuneKernels };
+struct TuneSpec {
+  const char* name;
+  int count;
+  int fallback;                   // default variant
+  const char* const* describe;    // count entries
+};
+const TuneSpec& tune_spec(TuneKernel kernel);
+int tuned_variant(TuneKernel kernel);
+
 }  // namespace pe

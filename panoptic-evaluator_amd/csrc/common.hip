@@ -87,6 +87,7 @@ extern "C" int pe_abi_version(void) { return PE_ABI_VERSION; }
 
 // ------------------------------------------------------------ timing hooks
 
+#include <atomic>
 #include <map>
 #include <string>
 #include <utility>
@@ -173,4 +174,61 @@ extern "C" int pe_profile_read(const char* kernel, double* total_ms, int64_t* la
   if (total_ms) *total_ms = total;
   if (launches) *launches = count;
   return 0;
+}
+
+// ------------------------------------------------------------------ tuning
+
+namespace pe {
+namespace {
+const char* const kDetVariants[] = {"1 chunk x 4 masks", "1 chunk x 8 masks", "2 chunks x 2 masks",
+                                    "2 chunks x 4 masks", "4 chunks x 2 masks"};
+const char* const kJointVariants[] = {"auto chunk", "8192 px", "16384 px", "32768 px", "65532 px"};
+const char* const kOverlapVariants[] = {"8 GT tiles x 16384 px", "4 GT tiles x 16384 px", "8 GT tiles x 8192 px",
+                                        "8 GT tiles x 32768 px"};
+const TuneSpec kSpecs[kTuneKernels] = {
+    {"detection_histogram", 5, 0, kDetVariants},
+    {"joint_histogram", 5, 0, kJointVariants},
+    {"mask_overlap_mfma", 4, 0, kOverlapVariants},
+};
+std::atomic<int> g_selected[kTuneKernels] = {{-1}, {-1}, {-1}};
+
+int tune_index(const char* name) {
+  if (!name) return -1;
+  for (int k = 0; k < kTuneKernels; ++k)
+    if (std::string(name) == kSpecs[k].name) return k;
+  return -1;
+}
+}  // namespace
+
+const TuneSpec& tune_spec(TuneKernel kernel) { return kSpecs[kernel]; }
+
+int tuned_variant(TuneKernel kernel) {
+  const int v = g_selected[kernel].load(std::memory_order_relaxed);
+  return v < 0 ? kSpecs[kernel].fallback : v;
+}
+
+}  // namespace pe
+
+extern "C" int pe_tune_count(const char* kernel) {
+  const int k = pe::tune_index(kernel);
+  return k < 0 ? 0 : pe::kSpecs[k].count;
+}
+
+extern "C" int pe_tune_set(const char* kernel, int variant) {
+  const int k = pe::tune_index(kernel);
+  PE_REQUIRE(k >= 0, "unknown tunable kernel");
+  PE_REQUIRE(variant >= -1 && variant < pe::kSpecs[k].count, "variant out of range");
+  pe::g_selected[k].store(variant, std::memory_order_relaxed);
+  return 0;
+}
+
+extern "C" int pe_tune_get(const char* kernel) {
+  const int k = pe::tune_index(kernel);
+  return k < 0 ? -1 : pe::tuned_variant(static_cast<pe::TuneKernel>(k));
+}
+
+extern "C" const char* pe_tune_describe(const char* kernel, int variant) {
+  const int k = pe::tune_index(kernel);
+  if (k < 0 || variant < 0 || variant >= pe::kSpecs[k].count) return "";
+  return pe::kSpecs[k].describe[variant];
 }

@@ -39,8 +39,6 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 
 // --------------------------------------------------------- MFMA overlaps
 
-constexpr int kOverlapGTiles = 8;     // GT tiles (of 16) per wave
-constexpr int kOverlapChunk = 16384;  // pixels per wave (multiple of 64)
 
 __device__ __forceinline__ int byte_sum4(int w) {
   // four bytes, each 0 or 1
@@ -59,6 +57,9 @@ __device__ __forceinline__ v4i load16(const uint8_t* row, i64 k, i64 HW, bool va
 }
 
 // inter[d, g] += popcount(pm[d] & gm[g]) over this wave's K chunk.
+// kOverlapGTiles GT tiles (of 16) per wave; kOverlapChunk pixels per wave
+// (a multiple of 64).
+template <int kOverlapGTiles, int kOverlapChunk>
 __global__ __launch_bounds__(64) void mask_overlap_mfma(const uint8_t* pm, int D, const uint8_t* gm, int G,
                                                         i64 HW, int32_t* inter, int32_t* pa, int32_t* ga,
                                                         int aligned) {
@@ -133,10 +134,22 @@ __global__ __launch_bounds__(64) void mask_overlap_mfma(const uint8_t* pm, int D
   }
 }
 
+struct OverlapVariant {
+  int gtiles;
+  int chunk;
+  void (*kernel)(const uint8_t*, int, const uint8_t*, int, i64, int32_t*, int32_t*, int32_t*, int);
+};
+// Order matches the tuning registry's descriptions.
+const OverlapVariant kOverlap[] = {
+    {8, 16384, mask_overlap_mfma<8, 16384>},
+    {4, 16384, mask_overlap_mfma<4, 16384>},
+    {8, 8192, mask_overlap_mfma<8, 8192>},
+    {8, 32768, mask_overlap_mfma<8, 32768>},
+};
+
 // ------------------------------------------------- detection x slot counts
 
 constexpr int kDetThreads = 256;
-constexpr int kDetTile = 16 * kDetThreads * 2;  // pixels per workgroup (2 x 16 B per lane)
 constexpr size_t kDetMaxLds = 96 * 1024;
 
 struct DetHistArgs {
@@ -173,7 +186,17 @@ __device__ __forceinline__ void count_chunk(v4i m, const int* slots, int S, int 
   if (run_key >= 0) atomicAdd(bins + (run_key >> 1), static_cast<uint32_t>(run) << ((run_key & 1) * 16));
 }
 
+// Masks are read exactly once: stream them past the caches.
+__device__ __forceinline__ v4i load16_stream(const uint8_t* row, i64 k, i64 HW, bool aligned) {
+  if (aligned && k + 16 <= HW) return __builtin_nontemporal_load(reinterpret_cast<const v4i*>(row + k));
+  return load16(row, k, HW, true, false);
+}
+
+// CH = 16-pixel chunks per lane (tile = 16 * 256 * CH pixels); U = masks
+// whose loads are in flight together.
+template <int CH, int U>
 __global__ __launch_bounds__(kDetThreads) void detection_histogram(DetHistArgs a) {
+  constexpr int kTile = 16 * kDetThreads * CH;
   extern __shared__ uint4 lds4[];
   uint32_t* bins = reinterpret_cast<uint32_t*>(lds4);
   const int per_image = a.tiles * a.dgroups;
@@ -189,26 +212,38 @@ __global__ __launch_bounds__(kDetThreads) void detection_histogram(DetHistArgs a
   const int words4 = (words + 3) / 4;
   for (int w = threadIdx.x; w < words4; w += kDetThreads) lds4[w] = make_uint4(0, 0, 0, 0);
   __syncthreads();
-  const i64 p0 = static_cast<i64>(tile) * kDetTile;
+  const i64 p0 = static_cast<i64>(tile) * kTile;
   const int32_t* target = a.target + static_cast<i64>(b) * a.HW;
   const uint8_t* masks = a.masks[b];
-  // Each lane owns two 16-pixel chunks of the tile.
-  int slots[2][16];
-  i64 base[2];
+  const i64* order = a.order + first + d0;
+  const bool aligned = a.aligned != 0;
+  int slots[CH][16];
+  i64 base[CH];
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < CH; ++h) {
     base[h] = p0 + (static_cast<i64>(h) * kDetThreads + threadIdx.x) * 16;
 #pragma unroll
     for (int j = 0; j < 16; ++j) slots[h][j] = base[h] + j < a.HW ? target[base[h] + j] : -1;
   }
-  for (int r = 0; r < nd; ++r) {
-    const uint8_t* mask = masks + a.order[first + d0 + r] * a.HW;
-    v4i m[2];
+  for (int r0 = 0; r0 < nd; r0 += U) {
+    v4i m[U][CH];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) m[h] = load16(mask, base[h], a.HW, true, a.aligned != 0);
+    for (int u = 0; u < U; ++u) {
+      if (r0 + u < nd) {
+        const uint8_t* mask = masks + order[r0 + u] * a.HW;
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
-      if ((m[h].x | m[h].y | m[h].z | m[h].w) != 0) count_chunk(m[h], slots[h], a.S, r * a.S, bins);
+        for (int h = 0; h < CH; ++h) m[u][h] = load16_stream(mask, base[h], a.HW, aligned);
+      } else {
+#pragma unroll
+        for (int h = 0; h < CH; ++h) m[u][h] = v4i{0, 0, 0, 0};
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int h = 0; h < CH; ++h)
+        if ((m[u][h].x | m[u][h].y | m[u][h].z | m[u][h].w) != 0)
+          count_chunk(m[u][h], slots[h], a.S, (r0 + u) * a.S, bins);
   }
   __syncthreads();
   u64* out = a.pairs + (first + d0) * a.S;
@@ -220,6 +255,16 @@ __global__ __launch_bounds__(kDetThreads) void detection_histogram(DetHistArgs a
     if (hi) atomicAdd(out + 2 * w + 1, static_cast<u64>(hi));
   }
 }
+
+struct DetVariant {
+  int chunks;
+  void (*kernel)(DetHistArgs);
+};
+// Order matches the tuning registry's descriptions.
+const DetVariant kDetHist[] = {
+    {1, detection_histogram<1, 4>}, {1, detection_histogram<1, 8>}, {2, detection_histogram<2, 2>},
+    {2, detection_histogram<2, 4>}, {4, detection_histogram<4, 2>},
+};
 
 // ------------------------------------------------------------ COCO matching
 
@@ -590,16 +635,17 @@ extern "C" int pe_mask_overlap(const uint8_t* pm, int64_t D, const uint8_t* gm, 
   if (HW == 0 || (D == 0 && G == 0)) return kOk;
   const bool aligned = HW % 16 == 0 && reinterpret_cast<uintptr_t>(pm) % 16 == 0 &&
                        reinterpret_cast<uintptr_t>(gm) % 16 == 0;
-  const i64 chunks = (HW + kOverlapChunk - 1) / kOverlapChunk;
+  const OverlapVariant& v = kOverlap[tuned_variant(kTuneMaskOverlap)];
+  const i64 chunks = (HW + v.chunk - 1) / v.chunk;
   PE_REQUIRE(chunks < INT_MAX, "image too large");
   // With D == 0 or G == 0 still count the other side's areas.
   const i64 dtiles = std::max<i64>(1, (D + 15) / 16);
-  const i64 ggroups = std::max<i64>(1, (G + 16 * kOverlapGTiles - 1) / (16 * kOverlapGTiles));
+  const i64 ggroups = std::max<i64>(1, (G + 16 * v.gtiles - 1) / (16 * v.gtiles));
   PE_REQUIRE(dtiles < 65536 && ggroups < 65536, "too many masks");
   dim3 grid(static_cast<unsigned>(chunks), static_cast<unsigned>(dtiles), static_cast<unsigned>(ggroups));
   ProfileScope timing("mask_overlap_mfma", stream);
-  mask_overlap_mfma<<<grid, 64, 0, stream>>>(pm, static_cast<int>(D), gm, static_cast<int>(G), HW,
-                                             inter, pa, ga, aligned ? 1 : 0);
+  v.kernel<<<grid, 64, 0, stream>>>(pm, static_cast<int>(D), gm, static_cast<int>(G), HW, inter, pa, ga,
+                                    aligned ? 1 : 0);
   PE_LAUNCH_CHECK();
   return kOk;
 }
@@ -620,8 +666,11 @@ extern "C" int pe_detection_histogram(int64_t B, const uint8_t* const* masks, co
   a.target = target;
   a.HW = HW;
   a.S = static_cast<int>(S);
-  a.tiles = static_cast<int>((HW + kDetTile - 1) / kDetTile);
-  // Detections per group: packed 16-bit counters of group x S within the LDS budget.
+  const DetVariant& v = kDetHist[tuned_variant(kTuneDetectionHistogram)];
+  const i64 tile = 16 * kDetThreads * v.chunks;
+  a.tiles = static_cast<int>((HW + tile - 1) / tile);
+  // Detections per group: packed 16-bit counters of group x S within the LDS
+  // budget (a tile holds at most 16384 pixels, so no counter exceeds 0xFFFF).
   i64 group = static_cast<i64>(kDetMaxLds * 2 / 4) / S;  // bins per group / S
   PE_REQUIRE(group >= 1, "too many target slots for the detection histogram");
   group = std::min<i64>(group, max_dets);
@@ -633,10 +682,10 @@ extern "C" int pe_detection_histogram(int64_t B, const uint8_t* const* masks, co
   PE_REQUIRE(blocks < INT_MAX, "too many workgroups");
   const size_t lds = static_cast<size_t>((group * S + 1) / 2 + 3) / 4 * 16;
   if (lds > 64 * 1024)
-    PE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(detection_histogram),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    PE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(v.kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               static_cast<int>(lds)));
   ProfileScope timing("detection_histogram", stream);
-  detection_histogram<<<static_cast<unsigned>(blocks), kDetThreads, lds, stream>>>(a);
+  v.kernel<<<static_cast<unsigned>(blocks), kDetThreads, lds, stream>>>(a);
   PE_LAUNCH_CHECK();
   return kOk;
 }

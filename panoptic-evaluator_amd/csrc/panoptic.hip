@@ -236,7 +236,12 @@ int hist_default_chunk(i64 total_pixels) {
 
 int launch_joint_histogram(HistArgs a, int gt_bytes, int pred_bytes, int chunk, hipStream_t stream) {
   if (a.B == 0 || a.HW == 0) return kOk;
-  if (chunk <= 0) chunk = hist_default_chunk(a.HW * a.B);
+  if (chunk <= 0) {
+    // Tuned pixel chunk per workgroup (variant 0 sizes it from the batch).
+    static const int kChunks[] = {0, 8192, 16384, 32768, kMaxChunk};
+    chunk = kChunks[tuned_variant(kTuneJointHistogram)];
+    if (chunk <= 0) chunk = hist_default_chunk(a.HW * a.B);
+  }
   chunk = std::min<int>(std::max<int>(chunk & ~3, 4), kMaxChunk);
   a.chunk = chunk;
   a.chunks_per_image = static_cast<int>((a.HW + chunk - 1) / chunk);

@@ -25,6 +25,10 @@ _SIGNATURES = {
     "pe_profile_enable": (_int, [_int]),
     "pe_profile_reset": (_int, []),
     "pe_profile_read": (_int, [ctypes.c_char_p, ctypes.POINTER(_dbl), ctypes.POINTER(_i64)]),
+    "pe_tune_count": (_int, [ctypes.c_char_p]),
+    "pe_tune_set": (_int, [ctypes.c_char_p, _int]),
+    "pe_tune_get": (_int, [ctypes.c_char_p]),
+    "pe_tune_describe": (ctypes.c_char_p, [ctypes.c_char_p, _int]),
     "pe_update_workspace_bytes": (_i64, [_i64] * 4),
     "pe_update": (_int, [_ptr, _int, _ptr, _int, _i64, _i64, _i64, _ptr, _ptr, _ptr, _i64, _i64, _i64,
                          _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr,

@@ -55,7 +55,7 @@ class InstanceBatch:
 class _PreparedImage(NamedTuple):
     masks: torch.Tensor
     classes: torch.Tensor         # input order
-    crowd: torch.Tensor
+    crowd: torch.Tensor | None    # targets only
     annotation_areas: torch.Tensor | None
     scores: torch.Tensor | None   # input order
     order: torch.Tensor | None    # unused (kept for the record layout)
@@ -165,14 +165,15 @@ class InstanceEvaluator:
                     raise ValueError(f"{name} must be floating point")
             if masks.device != self.device or classes.device != self.device:
                 raise ValueError("inputs must share evaluator device")
-            crowd = (batch.crowd[b] if not prediction and batch.crowd is not None
-                     else torch.zeros(n, dtype=torch.bool, device=self.device))
+            crowd = None  # detections carry no crowd flags
+            if not prediction:
+                crowd = (batch.crowd[b].contiguous() if batch.crowd is not None
+                         else torch.zeros(n, dtype=torch.bool, device=self.device))
             annotation = batch.areas[b].double() if not prediction and batch.areas is not None else None
             # Predictions stay in input order here; ScoreOrder sorts every
             # image of the batch in one segmented radix sort.
             scores = batch.scores[b] if prediction else None
-            prepared.append(_PreparedImage(masks.contiguous(), classes, crowd.contiguous(), annotation,
-                                           scores, None))
+            prepared.append(_PreparedImage(masks.contiguous(), classes, crowd, annotation, scores, None))
         return prepared
 
     def _commit(self, record):
