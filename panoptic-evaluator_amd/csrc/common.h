@@ -80,6 +80,14 @@ __device__ inline i64 wave_sum(i64 v) {
   return v;
 }
 
+// Butterfly sum: every lane ends with the same value (a + b == b + a), and the
+// pairing is fixed, so the result does not depend on scheduling.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
 __device__ inline int lane_id() { return __lane_id(); }
 
 // Zero-fill with 16-byte stores (graph-capture safe replacement for memset).

@@ -449,6 +449,32 @@ __global__ void class_bounds(const uint32_t* keys, i64 N, int C, int32_t* start,
   }
 }
 
+// Contribution of job j = (t, area, maxDets) = (j / 12, (j / 3) % 4, j % 3),
+// with valid-precision sum s, count n and recall rv (< 0: none), to summary
+// accumulator q; even q are sums, odd q the matching counts:
+// 0-5 ap/ap50/ap75, 6-11 ap small/medium/large (maxDets 100),
+// 12-17 ar1/ar10/ar100 (all areas), 18-23 ar small/medium/large.
+__device__ __forceinline__ double summary_term(int q, int j, double s, double n, double rv) {
+  const int m = j % 3, ar = (j / 3) % 4, t = j / 12;
+  const bool count = q & 1;
+  if (q < 12) {
+    if (m != 2) return 0.0;
+    const double v = count ? n : s;
+    if (ar == 0) {
+      if (q < 2) return v;
+      if (q < 4) return t == 0 ? v : 0.0;
+      if (q < 6) return t == 5 ? v : 0.0;
+      return 0.0;
+    }
+    return (q >> 1) == 2 + ar ? v : 0.0;
+  }
+  if (rv < 0) return 0.0;
+  const double v = count ? 1.0 : rv;
+  if (ar == 0) return (q >> 1) == 6 + m ? v : 0.0;
+  if (m == 2) return (q >> 1) == 8 + ar ? v : 0.0;
+  return 0.0;
+}
+
 struct ApArgs {
   int C;
   const int32_t* order;      // record index per sorted position (class-grouped, score order)
@@ -459,7 +485,7 @@ struct ApArgs {
   const i64* counts;         // C x 4
   double* precision;         // 10 x 101 x C x 4 x 3
   double* recall;            // 10 x C x 4 x 3
-  double* partial;           // (C*120) x 2: valid-precision sum, count
+  double* partial;           // C x 24 summary accumulators
 };
 
 // One workgroup per class; thread j < 120 owns (t, area, maxDets) =
@@ -481,7 +507,6 @@ __global__ __launch_bounds__(128) void ap_accumulate(ApArgs a) {
   const i64 global_job = static_cast<i64>(t) * pstride + c * 12 + ar * 3 + m;
   double* prec = a.precision + static_cast<i64>(t) * 101 * pstride + (c * 4 + ar) * 3 + m;
   double* rec = a.recall + global_job;
-  double* part = a.partial + 2 * global_job;
   const int s0 = a.start[c], s1 = a.end[c];
   const int lane_flag = ar * 10 + t;
   auto stage = [&](int first, int count) {
@@ -548,76 +573,68 @@ __global__ __launch_bounds__(128) void ap_accumulate(ApArgs a) {
         }
       }
   }
-  if (!active) return;
-  if (!positives) {
-    for (int q = 0; q <= 100; ++q) prec[q * pstride] = -1.0;
-    *rec = -1.0;
-    part[0] = 0.0;
-    part[1] = 0.0;
-    return;
+  double part_sum = 0.0, part_n = 0.0, rv = -1.0;
+  if (active) {
+    if (!positives) {
+      for (int q = 0; q <= 100; ++q) prec[q * pstride] = -1.0;
+    } else if (n == 0) {
+      for (int q = 0; q <= 100; ++q) prec[q * pstride] = 0.0;
+      rv = 0.0;
+      part_n = 101.0;
+    } else {
+      rv = static_cast<double>(tp) / npos;
+      part_sum = sum;
+      part_n = 101.0;
+    }
+    *rec = rv;
   }
-  if (n == 0) {
-    for (int q = 0; q <= 100; ++q) prec[q * pstride] = 0.0;
-    *rec = 0.0;
-    part[0] = 0.0;
-    part[1] = 101.0;
-    return;
+  // This class's share of the 12 summaries: 24 (sum, count) accumulators,
+  // each folded over the 120 jobs in job order.
+  __syncthreads();
+  double* sh = reinterpret_cast<double*>(s_flag);
+  if (active) {
+    sh[3 * job] = part_sum;
+    sh[3 * job + 1] = part_n;
+    sh[3 * job + 2] = rv;
   }
-  *rec = static_cast<double>(tp) / npos;
-  part[0] = sum;
-  part[1] = 101.0;
+  __syncthreads();
+  if (threadIdx.x < 24) {
+    double acc = 0.0;
+    for (int j = 0; j < 120; ++j) acc += summary_term(threadIdx.x, j, sh[3 * j], sh[3 * j + 1], sh[3 * j + 2]);
+    a.partial[c * 24 + threadIdx.x] = acc;
+  }
 }
 
-// instance.py:236-248: the 12 summaries as means over valid entries (>= 0).
-__global__ __launch_bounds__(512) void ap_summaries(int C, const double* precision_partial, const double* recall,
-                                                     double* out) {
-  __shared__ double red[24][512];
+// instance.py:236-248: the 12 summaries as means over valid entries (>= 0),
+// from the per-class accumulators; classes are folded in a fixed order.
+__global__ __launch_bounds__(256) void ap_summaries(int C, const double* class_part, double* out) {
+  __shared__ double red[24][10];
   const int tid = threadIdx.x;
-  double acc[24];
+  if (tid < 240) {
+    const int q = tid / 10, k = tid - q * 10;
+    double acc = 0.0;
+    constexpr int U = 8;
+    for (int c0 = k; c0 < C; c0 += 10 * U) {
+      double v[U];
 #pragma unroll
-  for (int j = 0; j < 24; ++j) acc[j] = 0.0;
-  const int jobs = C * 120;
-  constexpr int U = 4;
-  for (int j0 = tid; j0 < jobs; j0 += 512 * U) {
-    double s[U], n[U], rv[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int job = j0 + u * 512;
-      const bool ok = job < jobs;
-      s[u] = ok ? precision_partial[2 * job] : 0.0;
-      n[u] = ok ? precision_partial[2 * job + 1] : 0.0;
-      rv[u] = ok ? recall[job] : -1.0;
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int job = j0 + u * 512;
-      if (job >= jobs) break;
-      const int m = job % 3, ar = (job / 3) % 4, t = job / (12 * C);
-      if (m == 2) {
-        if (ar == 0) {
-          acc[0] += s[u]; acc[1] += n[u];                       // ap
-          if (t == 0) { acc[2] += s[u]; acc[3] += n[u]; }       // ap50
-          if (t == 5) { acc[4] += s[u]; acc[5] += n[u]; }       // ap75
-        } else {
-          acc[4 + 2 * ar] += s[u]; acc[5 + 2 * ar] += n[u];     // ap_small/medium/large: 6..11
-        }
+      for (int u = 0; u < U; ++u) {
+        const int c = c0 + 10 * u;
+        v[u] = c < C ? class_part[static_cast<i64>(c) * 24 + q] : 0.0;
       }
-      if (rv[u] >= 0) {
-        if (ar == 0) { acc[12 + 2 * m] += rv[u]; acc[13 + 2 * m] += 1.0; }          // ar1/ar10/ar100
-        else if (m == 2) { acc[16 + 2 * ar] += rv[u]; acc[17 + 2 * ar] += 1.0; }   // ar_small..large
-      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc += v[u];
     }
+    red[q][k] = acc;
   }
-#pragma unroll
-  for (int j = 0; j < 24; ++j) red[j][tid] = acc[j];
   __syncthreads();
-  for (int stride = 256; stride > 0; stride >>= 1) {
-    if (tid < stride)
-#pragma unroll
-      for (int j = 0; j < 24; ++j) red[j][tid] += red[j][tid + stride];
-    __syncthreads();
+  if (tid < 12) {
+    double s = 0.0, n = 0.0;
+    for (int k = 0; k < 10; ++k) {
+      s += red[2 * tid][k];
+      n += red[2 * tid + 1][k];
+    }
+    out[tid] = n > 0 ? s / n : -1.0;
   }
-  if (tid < 12) out[tid] = red[2 * tid + 1][0] > 0 ? red[2 * tid][0] / red[2 * tid + 1][0] : -1.0;
 }
 
 }  // namespace pe
@@ -782,7 +799,7 @@ ApBuffers carve_ap(void* workspace, i64 N, i64 C, size_t temp_bytes, size_t* use
   w.order = carve.take<int32_t>(N);
   w.start = carve.take<int32_t>(C);
   w.end = carve.take<int32_t>(C);
-  w.partial = carve.take<double>(C * 240);
+  w.partial = carve.take<double>(C * 24);
   w.temp = carve.take<char>(temp_bytes);
   w.temp_bytes = temp_bytes;
   if (used) *used = carve.used;
@@ -840,7 +857,7 @@ extern "C" int pe_ap_compute(int64_t N, const double* scores, const int32_t* cla
     ap_accumulate<<<static_cast<unsigned>(C), 128, 0, stream>>>(a);
     PE_LAUNCH_CHECK();
   }
-  ap_summaries<<<1, 512, 0, stream>>>(static_cast<int>(C), w.partial, recall, summary);
+  ap_summaries<<<1, 256, 0, stream>>>(static_cast<int>(C), w.partial, summary);
   PE_LAUNCH_CHECK();
   return kOk;
 }

@@ -511,7 +511,8 @@ __device__ void score_core(const ScoreArgs& a, int b, const Stage& st, const T* 
   }
 }
 
-__device__ void score_image(const ScoreArgs& a, int b, char* lds, size_t lds_bytes) {
+// Returns true when the image's global pair table was already zeroed.
+__device__ bool score_image(const ScoreArgs& a, int b, char* lds, size_t lds_bytes) {
   const int G = a.G, P = a.P;
   const int tid = threadIdx.x, nthreads = blockDim.x;
   const i64 GP = static_cast<i64>(G) * P;
@@ -526,14 +527,33 @@ __device__ void score_image(const ScoreArgs& a, int b, char* lds, size_t lds_byt
     }
   }
   if (st.staged_pairs) {
-    // One coalesced copy of the pair table into LDS; every loop below then
-    // reads LDS instead of waiting on global memory.
-    for (i64 e = tid; e < GP; e += nthreads) st.cnt[e] = static_cast<uint32_t>(pairs[e]);
+    // One coalesced copy of the pair table into LDS (eight loads in flight
+    // per lane); every loop below then reads LDS.  The global table is
+    // zeroed in the same pass, ready for the next update.
+    u64* table = const_cast<u64*>(pairs);
+    constexpr int U = 8;
+    for (i64 e0 = tid; e0 < GP; e0 += static_cast<i64>(nthreads) * U) {
+      u64 v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const i64 e = e0 + static_cast<i64>(u) * nthreads;
+        v[u] = e < GP ? table[e] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const i64 e = e0 + static_cast<i64>(u) * nthreads;
+        if (e < GP) {
+          st.cnt[e] = static_cast<uint32_t>(v[u]);
+          if (v[u]) table[e] = 0;
+        }
+      }
+    }
     __syncthreads();
     score_core<uint32_t>(a, b, st, st.cnt);
-  } else {
-    score_core<u64>(a, b, st, pairs);
+    return true;
   }
+  score_core<u64>(a, b, st, pairs);
+  return false;
 }
 
 // iou_sum[c] = the exact accumulated sum rounded once to FP64.
@@ -549,7 +569,7 @@ __device__ void convert_iou_sum(const ScoreArgs& a) {
 __global__ __launch_bounds__(kScoreThreads) void score_images(ScoreArgs a) {
   extern __shared__ uint4 lds4[];
   if (a.guard && *a.guard) return;  // a failed validation never mutates totals
-  score_image(a, blockIdx.x, reinterpret_cast<char*>(lds4), a.lds_bytes);
+  if (score_image(a, blockIdx.x, reinterpret_cast<char*>(lds4), a.lds_bytes)) return;
   // Leave this image's pair table zeroed for the next update.
   __syncthreads();
   const i64 GP = static_cast<i64>(a.G) * a.P;
@@ -593,39 +613,45 @@ int launch_prepare(const ScoreArgs& a, int* error, hipStream_t stream) {
 
 // One workgroup per accumulator row.  Confusion row and column sums come from
 // one coalesced sweep into LDS (64-bit LDS atomics); per-class ratios follow
-// the reference CPU formulas exactly; group means use a fixed-order tree.
-__global__ __launch_bounds__(kScoreThreads) void finalize(FinalArgs a) {
+// the reference CPU formulas exactly; group means reduce in a fixed order
+// (butterfly within each wave, then waves in index order).
+constexpr int kFinalThreads = 1024;
+
+__global__ __launch_bounds__(kFinalThreads) void finalize(FinalArgs a) {
   extern __shared__ uint4 lds4[];
   u64* rowsum = reinterpret_cast<u64*>(lds4);
   u64* colsum = rowsum + a.C;
+  constexpr int kWaves = kFinalThreads / kWave;
+  __shared__ double red[14][kWaves];
+  __shared__ double total[14];
   const i64 k = blockIdx.x;
-  const int C = a.C, tid = threadIdx.x;
-  const i64 CC = static_cast<i64>(C) * (C + 1);
+  const int C = a.C, tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const int C1 = C + 1;
+  const int CC = C * C1;  // the launcher caps C at 7680 (LDS), so C * (C + 1) fits an int
   const i64* tp = a.tp + k * C;
   const i64* fp = a.fp + k * C;
   const i64* fn = a.fn + k * C;
   const double* sums = a.sums + k * C;
   const i64* conf = a.confusion + k * CC;
   double* per_class = a.per_class + k * 4 * C;
-  __shared__ double red[14][kScoreThreads];
-  for (int c = tid; c < 2 * C; c += kScoreThreads) rowsum[c] = 0;
+  for (int c = tid; c < 2 * C; c += kFinalThreads) rowsum[c] = 0;
   __syncthreads();
   // Eight loads in flight per lane: the sweep is latency-bound otherwise.
   constexpr int U = 8;
-  for (i64 i0 = tid; i0 < CC; i0 += static_cast<i64>(kScoreThreads) * U) {
+  for (int i0 = tid; i0 < CC; i0 += kFinalThreads * U) {
     i64 v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const i64 i = i0 + static_cast<i64>(u) * kScoreThreads;
+      const int i = i0 + u * kFinalThreads;
       v[u] = i < CC ? conf[i] : 0;
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const i64 i = i0 + static_cast<i64>(u) * kScoreThreads;
+      const int i = i0 + u * kFinalThreads;
       if (i >= CC) break;
       if (a.out_confusion) a.out_confusion[k * CC + i] = v[u];
       if (!v[u]) continue;
-      const int r = static_cast<int>(i / (C + 1)), col = static_cast<int>(i - static_cast<i64>(r) * (C + 1));
+      const int r = i / C1, col = i - r * C1;
       atomicAdd(rowsum + r, static_cast<u64>(v[u]));
       if (col < C) atomicAdd(colsum + col, static_cast<u64>(v[u]));
     }
@@ -634,7 +660,7 @@ __global__ __launch_bounds__(kScoreThreads) void finalize(FinalArgs a) {
   double part[14];
 #pragma unroll
   for (int j = 0; j < 14; ++j) part[j] = 0.0;
-  for (int c = tid; c < C; c += kScoreThreads) {
+  for (int c = tid; c < C; c += kFinalThreads) {
     const i64 t = tp[c], f = fp[c], n = fn[c];
     const double s = sums[c];
     if (a.out_tp) {
@@ -651,7 +677,7 @@ __global__ __launch_bounds__(kScoreThreads) void finalize(FinalArgs a) {
     per_class[c] = pq;
     per_class[C + c] = sq;
     per_class[2 * C + c] = rq;
-    const i64 diag = conf[static_cast<i64>(c) * (C + 1) + c];
+    const i64 diag = conf[c * C1 + c];
     const i64 uni = static_cast<i64>(rowsum[c]) + static_cast<i64>(colsum[c]) - diag;
     const double iou = uni > 0 ? static_cast<double>(diag) / static_cast<double>(uni) : __builtin_nan("");
     per_class[3 * C + c] = iou;
@@ -666,27 +692,31 @@ __global__ __launch_bounds__(kScoreThreads) void finalize(FinalArgs a) {
     }
   }
 #pragma unroll
-  for (int j = 0; j < 14; ++j) red[j][tid] = part[j];
-  __syncthreads();
-  for (int stride = kScoreThreads / 2; stride > 0; stride >>= 1) {
-    if (tid < stride)
+  for (int j = 0; j < 14; ++j) part[j] = wave_sum(part[j]);
+  if (lane == 0)
 #pragma unroll
-      for (int j = 0; j < 14; ++j) red[j][tid] += red[j][tid + stride];
-    __syncthreads();
+    for (int j = 0; j < 14; ++j) red[j][wave] = part[j];
+  __syncthreads();
+  if (tid < 14) {
+    double s = 0.0;
+    for (int w = 0; w < kWaves; ++w) s += red[tid][w];
+    total[tid] = s;
   }
+  __syncthreads();
   if (tid < 3) {
-    const double n = red[tid * 4 + 3][0];
+    const double n = total[tid * 4 + 3];
     a.counts[k * 3 + tid] = static_cast<i64>(n);
-    for (int j = 0; j < 3; ++j) a.summaries[k * 9 + tid * 3 + j] = n > 0 ? red[tid * 4 + j][0] / n : 0.0;
+    for (int j = 0; j < 3; ++j) a.summaries[k * 9 + tid * 3 + j] = n > 0 ? total[tid * 4 + j] / n : 0.0;
   }
-  if (tid == 0) a.miou[k] = red[13][0] > 0 ? red[12][0] / red[13][0] : 0.0;
+  if (tid == 0) a.miou[k] = total[13] > 0 ? total[12] / total[13] : 0.0;
 }
 
 int launch_finalize(const FinalArgs& a, i64 K, hipStream_t stream) {
   if (K <= 0) return kOk;
   size_t lds = 16 * static_cast<size_t>(a.C);
   if (lds > kMaxStageLds) return fail(kUnsupportedShape, "too many categories for the finaliser");
-  finalize<<<static_cast<unsigned>(K), kScoreThreads, lds, stream>>>(a);
+  if (!allow_lds(finalize, lds)) return fail(kUnsupportedShape, "finaliser LDS request refused");
+  finalize<<<static_cast<unsigned>(K), kFinalThreads, lds, stream>>>(a);
   PE_LAUNCH_CHECK();
   return kOk;
 }

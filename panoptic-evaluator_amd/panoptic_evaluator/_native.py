@@ -7,6 +7,7 @@ the call fails loudly.
 
 from __future__ import annotations
 
+import collections
 import ctypes
 import pathlib
 import threading
@@ -71,6 +72,32 @@ def library():
                     raise RuntimeError("libpanoptic_hip.so ABI version mismatch; rebuild it")
                 _library = lib
     return _library
+
+
+_tables: "collections.OrderedDict[tuple, torch.Tensor]" = collections.OrderedDict()
+_TABLE_CACHE = 256
+
+
+def device_table(values, device) -> torch.Tensor:
+    """Read-only int64 device copy of a short host list (offsets, pointer
+    tables), cached by content: repeated batch layouts upload nothing, so an
+    update issued after a warm-up call performs no host-to-device copy (as HIP
+    graph capture requires).  Callers must never write to the result."""
+    device = torch.device(device)
+    if device.index is None:
+        device = torch.device(device.type, torch.cuda.current_device())
+    key = (device.index, tuple(int(v) for v in values))
+    with _lock:
+        table = _tables.get(key)
+        if table is not None:
+            _tables.move_to_end(key)
+            return table
+    table = torch.tensor(key[1], dtype=torch.int64).to(device)
+    with _lock:
+        _tables[key] = table
+        while len(_tables) > _TABLE_CACHE:
+            _tables.popitem(last=False)
+    return table
 
 
 def ptr(tensor):

@@ -15,7 +15,7 @@ import torch
 
 from . import _native
 from .evaluator import PanopticBatch, PanopticEvaluator, _prepare
-from .instance import InstanceBatch, InstanceEvaluator, raise_instance_error, score_order
+from .instance import InstanceBatch, InstanceEvaluator, offsets, raise_instance_error, score_order
 
 
 class SegmentationEvaluator:
@@ -60,7 +60,7 @@ class SegmentationEvaluator:
         HW = H * W
         ranked = score_order(prepared, device)
         det_counts, N, det_off = ranked.counts, sum(ranked.counts), ranked.det_off
-        pointers = torch.tensor([image.masks.data_ptr() for image in prepared], dtype=torch.int64).to(device)
+        pointers = _native.device_table([image.masks.data_ptr() for image in prepared], device)
         aligned = int(HW % 16 == 0 and all(image.masks.data_ptr() % 16 == 0 for image in prepared))
         pairs = torch.empty((N, S), dtype=torch.int64, device=device)
         _native.call("pe_detection_histogram", B, pointers.data_ptr(), det_off.data_ptr(), ranked.order.data_ptr(),
@@ -83,8 +83,8 @@ class SegmentationEvaluator:
         areas = self.panoptic._update(prediction, target, return_areas=True)
         status = torch.empty((N, 40), dtype=torch.int8, device=device)
         ranks = torch.empty(N, dtype=torch.int32, device=device)
-        gt_off = torch.arange(0, (B + 1) * S, S, dtype=torch.int64, device=device)
-        inter_off = det_off * S
+        gt_off = _native.device_table(range(0, (B + 1) * S, S), device)
+        inter_off = offsets([d * S for d in det_counts], device)
         _native.call("pe_coco_match", B, C, det_off.data_ptr(), gt_off.data_ptr(), inter_off.data_ptr(),
                      det_cls32.data_ptr(), det_area.data_ptr(), None, classes.data_ptr(), None,
                      _native.ptr(crowd), _native.ptr(inst), pairs.data_ptr(), 8, N, S, status.data_ptr(),

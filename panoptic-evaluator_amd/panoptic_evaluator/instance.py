@@ -111,10 +111,12 @@ def _cat(tensors, dtype, device):
 
 
 def offsets(counts, device):
-    out = torch.zeros(len(counts) + 1, dtype=torch.int64)
-    if counts:
-        out[1:] = torch.tensor(counts, dtype=torch.int64).cumsum(0)
-    return out.to(device)
+    """Device int64 exclusive prefix sums (len(counts) + 1) of host counts."""
+    total, values = 0, [0]
+    for count in counts:
+        total += int(count)
+        values.append(total)
+    return _native.device_table(values, device)
 
 
 class InstanceEvaluator:
@@ -205,8 +207,11 @@ class InstanceEvaluator:
             det_areas.append(pa)
             gt_areas.append(ga.double())
         # Detection pixel areas in score order.
-        base = torch.repeat_interleave(ranked.det_off[:-1], torch.tensor(ranked.counts, device=device))
-        det_area = _cat(det_areas, torch.float64, device)[ranked.order + base]
+        first, base = 0, []
+        for count in ranked.counts:
+            base.extend([first] * count)
+            first += count
+        det_area = _cat(det_areas, torch.float64, device)[ranked.order + _native.device_table(base, device)]
         gt_cls = _cat([g.classes for g in targets], torch.int64, device)
         gt_area = _cat(gt_areas, torch.float64, device)
         ann = _cat([g.annotation_areas if g.annotation_areas is not None else a
@@ -247,7 +252,9 @@ def accumulate(records, counts, num_classes, device):
     """Sort all records by score and accumulate AP/AR on the device
     (instance.py:205-248)."""
     C = num_classes
-    if records:
+    if len(records) == 1:
+        scores, classes, ranks, status = records[0]
+    elif records:
         scores = torch.cat([r[0] for r in records])
         classes = torch.cat([r[1] for r in records])
         ranks = torch.cat([r[2] for r in records])
