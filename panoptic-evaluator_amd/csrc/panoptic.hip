@@ -293,48 +293,56 @@ __host__ __device__ inline size_t stage_bytes(int G, int P) {
          4 * static_cast<size_t>(P) + static_cast<size_t>(G) + 64;
 }
 
-__device__ Stage make_stage(const ScoreArgs& a, int b, char* lds, size_t lds_bytes) {
+// The staged and global layouts are built in separate functions and used in
+// separate inlined call paths, so every pointer of one path has a single
+// address space: the compiler then emits LDS (ds_*) instructions for the
+// staged path instead of generic flat accesses, which would also make each
+// wait cover the in-flight global atomics.
+__device__ __forceinline__ Stage lds_stage(int G, int P, char* lds, size_t lds_bytes) {
+  Stage st;
+  char* p = lds;
+  st.ga = reinterpret_cast<i64*>(p);
+  p += 8 * static_cast<size_t>(G);
+  st.pa = reinterpret_cast<i64*>(p);
+  p += 8 * static_cast<size_t>(P);
+  st.val = reinterpret_cast<double*>(p);
+  p += 8 * static_cast<size_t>(P);
+  st.gc = reinterpret_cast<int*>(p);
+  p += 4 * static_cast<size_t>(G);
+  st.matched = reinterpret_cast<int*>(p);
+  p += 4 * static_cast<size_t>(G);
+  st.lab = reinterpret_cast<int*>(p);
+  p += 4 * static_cast<size_t>(P);
+  st.crowd = reinterpret_cast<uint8_t*>(p);
+  p += static_cast<size_t>(G);
+  const size_t used = (static_cast<size_t>(p - lds) + 15) & ~static_cast<size_t>(15);
+  st.staged = true;
+  st.staged_pairs = used + 4 * static_cast<size_t>(G) * P <= lds_bytes;
+  st.cnt = reinterpret_cast<uint32_t*>(lds + used);  // valid only when staged_pairs
+  return st;
+}
+
+__device__ __forceinline__ Stage global_stage(const ScoreArgs& a, int b) {
   const int G = a.G, P = a.P;
   Stage st;
-  st.staged = lds != nullptr && stage_bytes(G, P) <= lds_bytes;
-  if (st.staged) {
-    char* p = lds;
-    st.ga = reinterpret_cast<i64*>(p);
-    p += 8 * static_cast<size_t>(G);
-    st.pa = reinterpret_cast<i64*>(p);
-    p += 8 * static_cast<size_t>(P);
-    st.val = reinterpret_cast<double*>(p);
-    p += 8 * static_cast<size_t>(P);
-    st.gc = reinterpret_cast<int*>(p);
-    p += 4 * static_cast<size_t>(G);
-    st.matched = reinterpret_cast<int*>(p);
-    p += 4 * static_cast<size_t>(G);
-    st.lab = reinterpret_cast<int*>(p);
-    p += 4 * static_cast<size_t>(P);
-    st.crowd = reinterpret_cast<uint8_t*>(p);
-    p += static_cast<size_t>(G);
-    const size_t used = (static_cast<size_t>(p - lds) + 15) & ~static_cast<size_t>(15);
-    st.staged_pairs = used + 4 * static_cast<size_t>(G) * P <= lds_bytes;
-    st.cnt = st.staged_pairs ? reinterpret_cast<uint32_t*>(lds + used) : nullptr;
-  } else {
-    st.staged_pairs = false;
-    st.cnt = nullptr;
-    i64* areas = a.areas + static_cast<i64>(b) * (G + P);
-    st.ga = areas;
-    st.pa = areas + G;
-    st.val = a.rec_val + static_cast<i64>(b) * P;
-    st.gc = const_cast<int*>(a.gc + static_cast<i64>(b) * G);
-    st.matched = a.matched + static_cast<i64>(b) * G;
-    st.lab = a.rec_lab + static_cast<i64>(b) * P;
-    st.crowd = const_cast<uint8_t*>(a.crowd + static_cast<i64>(b) * G);
-  }
+  st.staged = false;
+  st.staged_pairs = false;
+  st.cnt = nullptr;
+  i64* areas = a.areas + static_cast<i64>(b) * (G + P);
+  st.ga = areas;
+  st.pa = areas + G;
+  st.val = a.rec_val + static_cast<i64>(b) * P;
+  st.gc = const_cast<int*>(a.gc + static_cast<i64>(b) * G);
+  st.matched = a.matched + static_cast<i64>(b) * G;
+  st.lab = a.rec_lab + static_cast<i64>(b) * P;
+  st.crowd = const_cast<uint8_t*>(a.crowd + static_cast<i64>(b) * G);
   return st;
 }
 
 // Row sums (GT areas) with one wave per row; column sums (prediction areas)
 // with one lane per column and eight loads in flight.
 template <typename T>
-__device__ void image_areas(const T* pairs, int G, int P, i64* areas, i64* ga, i64* pa) {
+__device__ __forceinline__ void image_areas(const T* pairs, int G, int P, i64* areas, i64* ga, i64* pa) {
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const int waves = blockDim.x / kWave;
   for (int g = wave; g < G; g += waves) {
@@ -417,7 +425,7 @@ __global__ __launch_bounds__(kScoreThreads) void prepare_images(ScoreArgs a, int
 // Score one image with the whole workgroup: areas, PQ matching, confusion,
 // TP/FP/FN and the per-image matched-IoU records.
 template <typename T>
-__device__ void score_core(const ScoreArgs& a, int b, const Stage& st, const T* pairs) {
+__device__ __forceinline__ void score_core(const ScoreArgs& a, int b, const Stage& st, const T* pairs) {
   const int G = a.G, P = a.P, C = a.C;
   const int tid = threadIdx.x, nthreads = blockDim.x;
   const int* pc = a.pc + static_cast<i64>(b) * P;
@@ -511,49 +519,238 @@ __device__ void score_core(const ScoreArgs& a, int b, const Stage& st, const T* 
   }
 }
 
+// ---- parallel scoring (pair table, metadata and a per-class crowd table in LDS)
+//
+// Every (g, p) pair is independent once the slot areas are known: IoU > 0.5
+// matches are unique per GT and per prediction slot (segments are disjoint),
+// so each prediction slot has at most one matched IoU, and the crowd rule
+// needs only the last same-class crowd slot, a per-class maximum.  The work
+// is a sweep over the G x P table by the whole workgroup instead of one
+// G-long loop per prediction slot.
+
+struct Par {
+  i64* ga;
+  i64* pa;
+  double* val;
+  int* gc;
+  int* pc;
+  int* matched;
+  int* hit;
+  int* last;      // C: last crowd GT slot of each class, -1 if none
+  uint8_t* crowd;
+  uint32_t* cnt;  // G x P
+};
+
+__host__ __device__ inline size_t par_bytes(int G, int P, int C) {
+  const size_t g = static_cast<size_t>(G), p = static_cast<size_t>(P), c = static_cast<size_t>(C);
+  const size_t meta = 8 * g + 16 * p + 4 * g + 4 * p + 4 * g + 4 * p + 4 * c + g;
+  return ((meta + 15) & ~static_cast<size_t>(15)) + 4 * g * p;
+}
+
+__device__ __forceinline__ Par par_layout(int G, int P, int C, char* lds) {
+  Par s;
+  char* q = lds;
+  s.ga = reinterpret_cast<i64*>(q);
+  q += 8 * static_cast<size_t>(G);
+  s.pa = reinterpret_cast<i64*>(q);
+  q += 8 * static_cast<size_t>(P);
+  s.val = reinterpret_cast<double*>(q);
+  q += 8 * static_cast<size_t>(P);
+  s.gc = reinterpret_cast<int*>(q);
+  q += 4 * static_cast<size_t>(G);
+  s.pc = reinterpret_cast<int*>(q);
+  q += 4 * static_cast<size_t>(P);
+  s.matched = reinterpret_cast<int*>(q);
+  q += 4 * static_cast<size_t>(G);
+  s.hit = reinterpret_cast<int*>(q);
+  q += 4 * static_cast<size_t>(P);
+  s.last = reinterpret_cast<int*>(q);
+  q += 4 * static_cast<size_t>(C);
+  s.crowd = reinterpret_cast<uint8_t*>(q);
+  q += static_cast<size_t>(G);
+  const size_t used = (static_cast<size_t>(q - lds) + 15) & ~static_cast<size_t>(15);
+  s.cnt = reinterpret_cast<uint32_t*>(lds + used);
+  return s;
+}
+
+__device__ void score_parallel(const ScoreArgs& a, int b, char* lds) {
+  const int G = a.G, P = a.P, C = a.C;
+  const int tid = threadIdx.x, nthreads = blockDim.x;
+  const int GP = G * P;  // fits the LDS, so far below 2^31
+  const Par s = par_layout(G, P, C, lds);
+  const int* gc = a.gc + static_cast<i64>(b) * G;
+  const int* pc = a.pc + static_cast<i64>(b) * P;
+  const uint8_t* crowd = a.crowd + static_cast<i64>(b) * G;
+  i64* areas = a.areas + static_cast<i64>(b) * (G + P);
+  u64* table = const_cast<u64*>(a.pairs) + static_cast<i64>(b) * GP;
+  const i64 row = a.reward ? static_cast<i64>(b) * C : 0;
+  const bool sum_areas = !a.areas_ready;
+  for (int g = tid; g < G; g += nthreads) {
+    s.gc[g] = gc[g];
+    s.crowd[g] = crowd[g];
+    s.matched[g] = 0;
+    s.ga[g] = sum_areas ? 0 : areas[g];
+  }
+  for (int p = tid; p < P; p += nthreads) {
+    s.pc[p] = pc[p];
+    s.hit[p] = 0;
+    s.val[p] = 0.0;
+    s.pa[p] = sum_areas ? 0 : areas[G + p];
+  }
+  for (int c = tid; c < C; c += nthreads) s.last[c] = -1;
+  __syncthreads();
+  for (int g = tid; g < G; g += nthreads) {
+    const int c = s.gc[g];
+    if (s.crowd[g] && c >= 0 && c < C) atomicMax(s.last + c, g);
+  }
+  // Pair table -> LDS (sixteen loads in flight per lane), zeroing the global
+  // table for the next update; slot areas as row/column sums of the nonzeros.
+  constexpr int U = 16;
+  for (int e0 = tid; e0 < GP; e0 += nthreads * U) {
+    u64 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = e0 + u * nthreads;
+      v[u] = e < GP ? table[e] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = e0 + u * nthreads;
+      if (e >= GP) break;
+      s.cnt[e] = static_cast<uint32_t>(v[u]);
+      if (!v[u]) continue;
+      table[e] = 0;
+      if (sum_areas) {
+        const int g = e / P, p = e - g * P;
+        atomicAdd(reinterpret_cast<u64*>(s.ga + g), v[u]);
+        atomicAdd(reinterpret_cast<u64*>(s.pa + p), v[u]);
+      }
+    }
+  }
+  __syncthreads();
+  if (sum_areas) {
+    for (int g = tid; g < G; g += nthreads) areas[g] = s.ga[g];
+    for (int p = tid; p < P; p += nthreads) areas[G + p] = s.pa[p];
+  }
+  // Pair sweep: confusion, and the unique IoU > 0.5 match of each slot.
+  for (int e = tid; e < GP; e += nthreads) {
+    const uint32_t inter = s.cnt[e];
+    if (!inter) continue;
+    const int g = e / P, p = e - g * P;
+    const int tcls = s.gc[g];
+    if (tcls < 0 || tcls >= C) continue;
+    const int pcls = s.pc[p];
+    const bool valid_pred = pcls >= 0 && pcls < C;
+    const bool is_crowd = s.crowd[g] != 0;
+    if (!a.reward && !is_crowd)
+      atomicAdd(a.confusion + static_cast<i64>(tcls) * (C + 1) + (valid_pred ? pcls : C),
+                static_cast<u64>(inter));
+    if (!valid_pred || tcls != pcls || is_crowd) continue;
+    const i64 void_overlap = s.cnt[p];
+    const i64 uni = s.ga[g] + s.pa[p] - static_cast<i64>(inter) - void_overlap;
+    if (static_cast<i64>(inter) > uni / 2) {  // IoU > 0.5 without rounding
+      s.matched[g] = 1;
+      s.hit[p] = 1;
+      s.val[p] = static_cast<double>(inter) / static_cast<double>(uni);
+    }
+  }
+  __syncthreads();
+  u64* tp = a.tp + row;
+  u64* fp = a.fp + row;
+  u64* fn = a.fn + row;
+  for (int p = tid; p < P; p += nthreads) {
+    const int pcls = s.pc[p];
+    int label = -1;
+    if (pcls >= 0 && pcls < C) {
+      const i64 pa = s.pa[p];
+      const int lg = s.last[pcls];
+      const i64 void_overlap = G > 0 ? static_cast<i64>(s.cnt[p]) : 0;
+      const i64 ignored = void_overlap + (lg >= 0 ? static_cast<i64>(s.cnt[lg * P + p]) : 0);
+      if (!a.reward || pa > 0) {
+        if (s.hit[p]) {
+          atomicAdd(tp + pcls, 1ULL);
+          label = pcls;
+        } else if (static_cast<double>(ignored) <= 0.5 * static_cast<double>(pa)) {
+          atomicAdd(fp + pcls, 1ULL);
+        }
+      }
+    }
+    s.hit[p] = label;  // reused as the slot's label
+  }
+  for (int g = tid; g < G; g += nthreads) {
+    const int c = s.gc[g];
+    if (c >= 0 && c < C && !s.crowd[g] && !s.matched[g]) atomicAdd(fn + c, 1ULL);
+  }
+  __syncthreads();
+  if (a.reward) {
+    // Per-image sums: in-order fold over prediction slots.
+    double* sums = a.iou_sum + row;
+    for (int c = tid; c < C; c += nthreads) {
+      double acc = sums[c];
+      for (int p = 0; p < P; ++p)
+        if (s.hit[p] == c) acc += s.val[p];
+      sums[c] = acc;
+    }
+    return;
+  }
+  for (int p = tid; p < P; p += nthreads) {
+    const int c = s.hit[p];
+    if (c < 0) continue;
+    const u64 m = static_cast<u64>(s.val[p] * 9007199254740992.0);  // * 2^53, exact
+    atomicAdd(a.iou_exact + 2 * c, m & 0xFFFFFFFFULL);
+    atomicAdd(a.iou_exact + 2 * c + 1, m >> 32);
+  }
+}
+
 // Returns true when the image's global pair table was already zeroed.
 __device__ bool score_image(const ScoreArgs& a, int b, char* lds, size_t lds_bytes) {
   const int G = a.G, P = a.P;
   const int tid = threadIdx.x, nthreads = blockDim.x;
   const i64 GP = static_cast<i64>(G) * P;
   const u64* pairs = a.pairs + b * GP;
-  Stage st = make_stage(a, b, lds, lds_bytes);
-  if (st.staged) {
-    const int* gc = a.gc + static_cast<i64>(b) * G;
-    const uint8_t* crowd = a.crowd + static_cast<i64>(b) * G;
-    for (int g = tid; g < G; g += nthreads) {
-      st.gc[g] = gc[g];
-      st.crowd[g] = crowd[g];
-    }
-  }
-  if (st.staged_pairs) {
-    // One coalesced copy of the pair table into LDS (eight loads in flight
-    // per lane); every loop below then reads LDS.  The global table is
-    // zeroed in the same pass, ready for the next update.
-    u64* table = const_cast<u64*>(pairs);
-    constexpr int U = 8;
-    for (i64 e0 = tid; e0 < GP; e0 += static_cast<i64>(nthreads) * U) {
-      u64 v[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const i64 e = e0 + static_cast<i64>(u) * nthreads;
-        v[u] = e < GP ? table[e] : 0;
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const i64 e = e0 + static_cast<i64>(u) * nthreads;
-        if (e < GP) {
-          st.cnt[e] = static_cast<uint32_t>(v[u]);
-          if (v[u]) table[e] = 0;
-        }
-      }
-    }
-    __syncthreads();
-    score_core<uint32_t>(a, b, st, st.cnt);
+  if (lds != nullptr && par_bytes(G, P, a.C) <= lds_bytes) {
+    score_parallel(a, b, lds);
     return true;
   }
-  score_core<u64>(a, b, st, pairs);
-  return false;
+  if (lds == nullptr || stage_bytes(G, P) > lds_bytes) {
+    score_core<u64>(a, b, global_stage(a, b), pairs);
+    return false;
+  }
+  const Stage st = lds_stage(G, P, lds, lds_bytes);
+  const int* gc = a.gc + static_cast<i64>(b) * G;
+  const uint8_t* crowd = a.crowd + static_cast<i64>(b) * G;
+  for (int g = tid; g < G; g += nthreads) {
+    st.gc[g] = gc[g];
+    st.crowd[g] = crowd[g];
+  }
+  if (!st.staged_pairs) {
+    score_core<u64>(a, b, st, pairs);
+    return false;
+  }
+  // One coalesced copy of the pair table into LDS (eight loads in flight per
+  // lane); every loop below then reads LDS.  The global table is zeroed in
+  // the same pass, ready for the next update.
+  u64* table = const_cast<u64*>(pairs);
+  constexpr int U = 8;
+  for (i64 e0 = tid; e0 < GP; e0 += static_cast<i64>(nthreads) * U) {
+    u64 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const i64 e = e0 + static_cast<i64>(u) * nthreads;
+      v[u] = e < GP ? table[e] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const i64 e = e0 + static_cast<i64>(u) * nthreads;
+      if (e < GP) {
+        st.cnt[e] = static_cast<uint32_t>(v[u]);
+        if (v[u]) table[e] = 0;
+      }
+    }
+  }
+  __syncthreads();
+  score_core<uint32_t>(a, b, st, st.cnt);
+  return true;
 }
 
 // iou_sum[c] = the exact accumulated sum rounded once to FP64.
@@ -584,8 +781,10 @@ __global__ __launch_bounds__(kScoreThreads) void convert_iou(ScoreArgs a) {
 
 int launch_score(ScoreArgs a, hipStream_t stream) {
   if (a.B == 0) return kOk;
-  // Metadata plus the 32-bit pair table when they fit, else metadata only.
-  size_t lds = stage_bytes(a.G, a.P) + 4 * static_cast<size_t>(a.G) * a.P;
+  // The parallel layout when it fits; else metadata plus the 32-bit pair
+  // table; else metadata only.
+  size_t lds = par_bytes(a.G, a.P, a.C);
+  if (lds > kMaxStageLds) lds = stage_bytes(a.G, a.P) + 4 * static_cast<size_t>(a.G) * a.P;
   if (lds > kMaxStageLds) lds = stage_bytes(a.G, a.P);
   if (lds > kMaxStageLds) lds = 0;
   if (!allow_lds(score_images, lds)) lds = 0;
