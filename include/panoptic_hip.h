@@ -185,7 +185,8 @@ int64_t pe_ap_workspace_bytes(int64_t N, int64_t C);
  * descending score on the device.  counts: C x 4 positives.  Outputs:
  * precision 10 x 101 x C x 4 x 3 f64, recall 10 x C x 4 x 3 f64, and summary
  * (12 f64: ap, ap50, ap75, ap_small, ap_medium, ap_large, ar1, ar10, ar100,
- * ar_small, ar_medium, ar_large; -1 when undefined).  Replaces
+ * ar_small, ar_medium, ar_large; -1 when undefined).  status must be 8-byte
+ * aligned.  Replaces
  * ``instance_compute`` (bindings.cpp:16-17; instance_compute_cuda,
  * instance_kernel.cu:176-209, :311-333; CPU instance.py:205-248, :300-331). */
 int pe_ap_compute(int64_t N, const double* scores, const int32_t* classes, const int32_t* ranks,

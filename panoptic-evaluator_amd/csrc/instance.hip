@@ -449,32 +449,6 @@ __global__ void class_bounds(const uint32_t* keys, i64 N, int C, int32_t* start,
   }
 }
 
-// Contribution of job j = (t, area, maxDets) = (j / 12, (j / 3) % 4, j % 3),
-// with valid-precision sum s, count n and recall rv (< 0: none), to summary
-// accumulator q; even q are sums, odd q the matching counts:
-// 0-5 ap/ap50/ap75, 6-11 ap small/medium/large (maxDets 100),
-// 12-17 ar1/ar10/ar100 (all areas), 18-23 ar small/medium/large.
-__device__ __forceinline__ double summary_term(int q, int j, double s, double n, double rv) {
-  const int m = j % 3, ar = (j / 3) % 4, t = j / 12;
-  const bool count = q & 1;
-  if (q < 12) {
-    if (m != 2) return 0.0;
-    const double v = count ? n : s;
-    if (ar == 0) {
-      if (q < 2) return v;
-      if (q < 4) return t == 0 ? v : 0.0;
-      if (q < 6) return t == 5 ? v : 0.0;
-      return 0.0;
-    }
-    return (q >> 1) == 2 + ar ? v : 0.0;
-  }
-  if (rv < 0) return 0.0;
-  const double v = count ? 1.0 : rv;
-  if (ar == 0) return (q >> 1) == 6 + m ? v : 0.0;
-  if (m == 2) return (q >> 1) == 8 + ar ? v : 0.0;
-  return 0.0;
-}
-
 struct ApArgs {
   int C;
   const int32_t* order;      // record index per sorted position (class-grouped, score order)
@@ -494,10 +468,11 @@ struct ApArgs {
 // totals; backward pass: precision envelope at the 101 recall thresholds
 // (instance.py:300-331).
 constexpr int kApChunk = 1024;
+constexpr int kApThreads = 256;
 
-__global__ __launch_bounds__(128) void ap_accumulate(ApArgs a) {
+__global__ __launch_bounds__(kApThreads) void ap_accumulate(ApArgs a) {
   __shared__ uint8_t s_rank[kApChunk];
-  __shared__ int8_t s_flag[kApChunk * 40];
+  __shared__ __align__(16) int8_t s_flag[kApChunk * 40];
   const int c = blockIdx.x, job = threadIdx.x;
   const bool active = job < 120;
   const int m = job % 3, ar = (job / 3) % 4, t = job / 12;
@@ -509,15 +484,35 @@ __global__ __launch_bounds__(128) void ap_accumulate(ApArgs a) {
   double* rec = a.recall + global_job;
   const int s0 = a.start[c], s1 = a.end[c];
   const int lane_flag = ar * 10 + t;
+  // Records of one chunk -> LDS: the rank and the 40 flags (five 8-byte words,
+  // rows are 8-byte aligned) of each record, four gathers in flight per lane.
   auto stage = [&](int first, int count) {
-    for (int i = threadIdx.x; i < count; i += blockDim.x) {
-      const int idx = a.order[first + i];
-      s_rank[i] = static_cast<uint8_t>(a.ranks[idx]);
-    }
-    const int bytes = count * 40;
-    for (int e = threadIdx.x; e < bytes; e += blockDim.x) {
-      const int i = e / 40;
-      s_flag[e] = a.status[static_cast<i64>(a.order[first + i]) * 40 + (e - i * 40)];
+    uint64_t* flag_words = reinterpret_cast<uint64_t*>(s_flag);
+    const int words = count * 5;
+    constexpr int U = 4;
+    for (int e0 = threadIdx.x; e0 < words; e0 += kApThreads * U) {
+      uint64_t v[U];
+      int rank[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = e0 + u * kApThreads;
+        v[u] = 0;
+        rank[u] = 0;
+        if (e < words) {
+          const int i = e / 5, w = e - i * 5;
+          const int idx = a.order[first + i];
+          v[u] = reinterpret_cast<const uint64_t*>(a.status + static_cast<i64>(idx) * 40)[w];
+          if (w == 0) rank[u] = a.ranks[idx];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = e0 + u * kApThreads;
+        if (e >= words) break;
+        const int i = e / 5, w = e - i * 5;
+        flag_words[e] = v[u];
+        if (w == 0) s_rank[i] = static_cast<uint8_t>(rank[u]);
+      }
     }
   };
   // forward: totals
@@ -598,10 +593,31 @@ __global__ __launch_bounds__(128) void ap_accumulate(ApArgs a) {
     sh[3 * job + 2] = rv;
   }
   __syncthreads();
+  // Accumulator q: even q sums, odd q the matching counts, of summary q / 2:
+  // 0 ap, 1 ap50, 2 ap75 (all areas, maxDets 100), 3-5 ap small/medium/large,
+  // 6-8 ar1/ar10/ar100 (all areas), 9-11 ar small/medium/large (maxDets 100).
+  // Each folds its jobs over thresholds t in increasing order.
   if (threadIdx.x < 24) {
+    const int q = threadIdx.x, k = q >> 1;
+    const bool count = q & 1;
+    const bool precision_side = k < 6;
+    int ar = 0, m = 2, t0 = 0, t1 = 10;
+    if (k == 1) t1 = 1;
+    else if (k == 2) t0 = 5, t1 = 6;
+    else if (k >= 3 && k < 6) ar = k - 2;
+    else if (k >= 6 && k < 9) m = k - 6;
+    else if (k >= 9) ar = k - 8;
     double acc = 0.0;
-    for (int j = 0; j < 120; ++j) acc += summary_term(threadIdx.x, j, sh[3 * j], sh[3 * j + 1], sh[3 * j + 2]);
-    a.partial[c * 24 + threadIdx.x] = acc;
+    for (int t = t0; t < t1; ++t) {
+      const int j = t * 12 + ar * 3 + m;
+      if (precision_side) {
+        acc += count ? sh[3 * j + 1] : sh[3 * j];
+      } else {
+        const double rv = sh[3 * j + 2];
+        if (rv >= 0) acc += count ? 1.0 : rv;
+      }
+    }
+    a.partial[c * 24 + q] = acc;
   }
 }
 
@@ -819,6 +835,7 @@ extern "C" int pe_ap_compute(int64_t N, const double* scores, const int32_t* cla
                              void* stream_ptr) {
   PE_REQUIRE(N >= 0 && N < INT_MAX && C > 0 && C <= INT_MAX / 240, "invalid AP shapes");
   PE_REQUIRE(workspace_bytes >= pe_ap_workspace_bytes(N, C), "workspace too small");
+  PE_REQUIRE(reinterpret_cast<uintptr_t>(status) % 8 == 0, "match flags must be 8-byte aligned");
   hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
   const size_t temp = N > 0 ? sort_temp_bytes(N) : 0;
   ApBuffers w = carve_ap(workspace, N, C, temp);
@@ -854,7 +871,7 @@ extern "C" int pe_ap_compute(int64_t N, const double* scores, const int32_t* cla
   a.partial = w.partial;
   {
     ProfileScope timing("ap_accumulate", stream);
-    ap_accumulate<<<static_cast<unsigned>(C), 128, 0, stream>>>(a);
+    ap_accumulate<<<static_cast<unsigned>(C), kApThreads, 0, stream>>>(a);
     PE_LAUNCH_CHECK();
   }
   ap_summaries<<<1, 256, 0, stream>>>(static_cast<int>(C), w.partial, summary);
