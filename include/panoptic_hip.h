@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define PE_ABI_VERSION 1
+#define PE_ABI_VERSION 2
 
 const char* pe_last_error(void);
 int pe_abi_version(void);
@@ -49,8 +49,9 @@ const char* pe_tune_describe(const char* kernel, int variant);
 
 /* ------------------------------------------------------------ panoptic ---- */
 
-/* Scratch bytes pe_update needs beyond the reference's pairs/areas/matched. */
-int64_t pe_update_workspace_bytes(int64_t B, int64_t G, int64_t P, int64_t C);
+/* Scratch bytes pe_update needs beyond the reference's pairs/areas/matched;
+ * packed_pixels = the pixel total of a packed batch (0 for B x H x W maps). */
+int64_t pe_update_workspace_bytes(int64_t B, int64_t G, int64_t P, int64_t C, int64_t packed_pixels);
 
 /* Joint slot histogram, areas, validation, PQ matching and accumulation.
  * Replaces the reference's ``update`` binding
@@ -59,6 +60,11 @@ int64_t pe_update_workspace_bytes(int64_t B, int64_t G, int64_t P, int64_t C);
  *   gt, pred        B x H x W maps; element size map_bytes 4 (int32) or 8
  *                   (int64).  Compact slots, or original IDs when the
  *                   matching ID/slot lookup tables (B x G / B x P) are given.
+ *   pix_off         NULL, or B + 1 device pixel offsets of a packed batch of
+ *                   uneven images stored one after another (image b covers
+ *                   pix_off[b] .. pix_off[b + 1]; then H * W is the pixel
+ *                   total).  No counterpart in the reference, whose batches
+ *                   share one size (README.md:57-58).
  *   gc, pc          B x G / B x P int32 classes; crowd B x G uint8 (bool).
  *   pairs           B x G x P int64 scratch.  pairs_zeroed != 0 promises it
  *                   is all zero on entry (as every call leaves it), which
@@ -79,7 +85,7 @@ int64_t pe_update_workspace_bytes(int64_t B, int64_t G, int64_t P, int64_t C);
  * Dataset iou_sum is thus independent of batch composition and launch
  * order; per-image (reward) sums fold in prediction-slot order. */
 int pe_update(const void* gt, int gt_bytes, const void* pred, int pred_bytes,
-              int64_t B, int64_t H, int64_t W,
+              int64_t B, int64_t H, int64_t W, const int64_t* pix_off,
               const int32_t* gc, const int32_t* pc, const uint8_t* crowd,
               int64_t G, int64_t P, int64_t C,
               int64_t* pairs, int64_t* areas, int32_t* matched,
@@ -128,17 +134,25 @@ int pe_pack(const void* maps, int map_bytes, int64_t B, int64_t HW,
 int pe_mask_overlap(const uint8_t* pm, int64_t D, const uint8_t* gm, int64_t G, int64_t HW,
                     int32_t* inter, int32_t* pa, int32_t* ga, void* stream);
 
-/* Detection x target-slot intersections for a batch of images sharing one
- * size: pairs[row, s] = #pixels of mask order[row] (within its image) whose
- * target slot is s; rows are grouped per image by det_off (B + 1, device).
- * masks: device array of B pointers to D_b x HW bool tensors; target: B x HW
- * int32 slots; pairs: (sum D_b) x S int64.  aligned != 0 promises HW % 16 == 0
- * and 16-byte aligned mask pointers.  Replaces ``instance_histogram``
- * (bindings.cpp:18-19; instance_histogram_cuda, instance_kernel.cu:335-363). */
+/* Scratch bytes of pe_detection_histogram for a packed batch of
+ * packed_pixels pixels (0 for a batch of one image size: no scratch). */
+int64_t pe_detection_histogram_workspace_bytes(int64_t B, int64_t packed_pixels);
+
+/* Detection x target-slot intersections: pairs[row, s] = #pixels of mask
+ * order[row] (within its image) whose target slot is s; rows are grouped per
+ * image by det_off (B + 1, device).  masks: device array of B pointers to
+ * D_b x pixels_b bool tensors; target: the images' int32 slots one after
+ * another.  pix_off NULL: every image has HW pixels (target is B x HW);
+ * otherwise B + 1 device pixel offsets of a packed batch of uneven images
+ * and HW is the pixel total (workspace from the query above).  pairs:
+ * (sum D_b) x S int64.  aligned != 0 promises that every image's pixel count
+ * is a multiple of 16 and the mask pointers are 16-byte aligned.  Replaces
+ * ``instance_histogram`` (bindings.cpp:18-19; instance_histogram_cuda,
+ * instance_kernel.cu:335-363), which takes one image at a time. */
 int pe_detection_histogram(int64_t B, const uint8_t* const* masks, const int64_t* det_off,
-                           const int64_t* order, const int32_t* target, int64_t HW, int64_t S,
-                           int64_t max_dets, int64_t total_dets, int64_t* pairs, int aligned,
-                           void* stream);
+                           const int64_t* order, const int32_t* target, const int64_t* pix_off,
+                           int64_t HW, int64_t S, int64_t max_dets, int64_t total_dets, int64_t* pairs,
+                           int aligned, void* workspace, int64_t workspace_bytes, void* stream);
 
 /* COCO greedy matching of a batch of images.  Per image b, detections
  * det_off[b]..det_off[b+1] are in score order (det_cls int32, det_area f64

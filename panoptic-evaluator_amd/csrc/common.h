@@ -94,6 +94,37 @@ __device__ inline int lane_id() { return __lane_id(); }
 __global__ void zero_words(uint4* data, i64 count16, uint32_t* tail, i64 tail_words);
 
 int launch_zero(void* ptr, size_t bytes, hipStream_t stream);
+
+// ---- work lists over packed batches of uneven images
+//
+// A packed batch stores its images one after another; image b covers pixels
+// pix_off[b] .. pix_off[b + 1] of the flat maps.  Kernels walk it as a list of
+// tiles, each inside one image: image b has ceil(pixels_b / tile) tiles,
+// tile_off[b] is their exclusive prefix and tile_image[t] the image of tile t.
+// Uniform batches (pix_off == nullptr) use arithmetic instead of the tables.
+struct TileList {
+  const i64* pix_off;     // B + 1, or nullptr: every image has HW pixels
+  i64 HW;
+  int tile;               // pixels per tile
+  int tiles_per_image;    // uniform batches
+  const int* tile_off;    // B + 1 (packed)
+  const int* tile_image;  // tile_off[B] entries (packed)
+
+  __device__ __forceinline__ int count(int B) const { return pix_off ? tile_off[B] : B * tiles_per_image; }
+  __device__ __forceinline__ int image_of(int t) const { return pix_off ? tile_image[t] : t / tiles_per_image; }
+  __device__ __forceinline__ int first_tile(int b) const { return pix_off ? tile_off[b] : b * tiles_per_image; }
+  __device__ __forceinline__ i64 begin(int b) const { return pix_off ? pix_off[b] : static_cast<i64>(b) * HW; }
+  __device__ __forceinline__ i64 pixels(int b) const { return pix_off ? pix_off[b + 1] - pix_off[b] : HW; }
+};
+
+// Upper bound of the tile count of a packed batch (tiles of >= min_tile pixels).
+inline i64 tile_list_bound(i64 B, i64 total_pixels, i64 min_tile) { return total_pixels / min_tile + B; }
+// Ints of workspace a packed tile list needs (tile_off then tile_image).
+inline i64 tile_list_ints(i64 B, i64 total_pixels, i64 min_tile) {
+  return B + 1 + tile_list_bound(B, total_pixels, min_tile);
+}
+// Builds tile_off / tile_image on the device (one workgroup); graph-capture safe.
+int launch_tile_list(const i64* pix_off, int B, int tile, int* tile_off, int* tile_image, hipStream_t stream);
 // Byte fill kernel (graph-capture safe replacement for hipMemsetD8Async).
 int launch_fill_u8(void* ptr, uint8_t value, size_t bytes, hipStream_t stream);
 

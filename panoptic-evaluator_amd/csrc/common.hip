@@ -66,6 +66,48 @@ int launch_zero(void* ptr, size_t bytes, hipStream_t stream) {
   return kOk;
 }
 
+// One workgroup: per-thread runs of images are summed, the run totals scanned
+// in LDS, then every image writes its tiles' image index.
+constexpr int kTileListThreads = 1024;
+
+__global__ __launch_bounds__(kTileListThreads) void build_tile_list(const i64* pix_off, int B, int tile,
+                                                                      int* tile_off, int* tile_image) {
+  __shared__ int run_total[kTileListThreads];
+  const int tid = threadIdx.x;
+  const int per = (B + kTileListThreads - 1) / kTileListThreads;
+  const int b0 = min(tid * per, B), b1 = min(b0 + per, B);
+  int sum = 0;
+  for (int b = b0; b < b1; ++b) sum += static_cast<int>((pix_off[b + 1] - pix_off[b] + tile - 1) / tile);
+  run_total[tid] = sum;
+  __syncthreads();
+  // Inclusive Hillis-Steele scan of the run totals.
+  for (int d = 1; d < kTileListThreads; d <<= 1) {
+    const int v = tid >= d ? run_total[tid - d] : 0;
+    __syncthreads();
+    run_total[tid] += v;
+    __syncthreads();
+  }
+  int off = run_total[tid] - sum;
+  for (int b = b0; b < b1; ++b) {
+    tile_off[b] = off;
+    off += static_cast<int>((pix_off[b + 1] - pix_off[b] + tile - 1) / tile);
+  }
+  if (tid == kTileListThreads - 1) tile_off[B] = run_total[tid];
+  __syncthreads();
+  for (int b = b0; b < b1; ++b) {
+    const int first = tile_off[b], last = tile_off[b + 1];
+    for (int t = first; t < last; ++t) tile_image[t] = b;
+  }
+}
+
+int launch_tile_list(const i64* pix_off, int B, int tile, int* tile_off, int* tile_image, hipStream_t stream) {
+  if (B <= 0) return kOk;
+  if (tile <= 0) return fail(kBadArgument, "tile size must be positive");
+  build_tile_list<<<1, kTileListThreads, 0, stream>>>(pix_off, B, tile, tile_off, tile_image);
+  PE_LAUNCH_CHECK();
+  return kOk;
+}
+
 __global__ void fill_bytes(uint8_t* data, uint8_t value, i64 count) {
   const i64 stride = static_cast<i64>(gridDim.x) * blockDim.x;
   for (i64 i = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) data[i] = value;

@@ -153,13 +153,13 @@ constexpr int kDetThreads = 256;
 constexpr size_t kDetMaxLds = 96 * 1024;
 
 struct DetHistArgs {
-  const uint8_t* const* masks;   // B device pointers (D_b x HW bools)
+  const uint8_t* const* masks;   // B device pointers (D_b x pixels_b bools)
   const i64* det_off;            // B + 1
   const i64* order;              // per detection: mask index within its image
-  const int32_t* target;         // B x HW slots
-  i64 HW;
+  const int32_t* target;         // slots of every image, one after another
+  int B;
   int S;
-  int tiles;                     // pixel tiles per image
+  TileList tiles;                // pixel tiles, each inside one image
   int dgroups;                   // detection groups per image
   int group;                     // detections per group
   u64* pairs;                    // (sum D) x S
@@ -199,60 +199,64 @@ __global__ __launch_bounds__(kDetThreads) void detection_histogram(DetHistArgs a
   constexpr int kTile = 16 * kDetThreads * CH;
   extern __shared__ uint4 lds4[];
   uint32_t* bins = reinterpret_cast<uint32_t*>(lds4);
-  const int per_image = a.tiles * a.dgroups;
-  const int b = blockIdx.x / per_image;
-  const int rest = blockIdx.x - b * per_image;
-  const int tile = rest / a.dgroups, dgroup = rest - tile * a.dgroups;
-  const i64 first = a.det_off[b];
-  const int D = static_cast<int>(a.det_off[b + 1] - first);
-  const int d0 = dgroup * a.group;
-  const int nd = min(a.group, D - d0);
-  if (nd <= 0) return;
-  const int words = (nd * a.S + 1) / 2;
-  const int words4 = (words + 3) / 4;
-  for (int w = threadIdx.x; w < words4; w += kDetThreads) lds4[w] = make_uint4(0, 0, 0, 0);
-  __syncthreads();
-  const i64 p0 = static_cast<i64>(tile) * kTile;
-  const int32_t* target = a.target + static_cast<i64>(b) * a.HW;
-  const uint8_t* masks = a.masks[b];
-  const i64* order = a.order + first + d0;
+  const int total = a.tiles.count(a.B) * a.dgroups;
   const bool aligned = a.aligned != 0;
-  int slots[CH][16];
-  i64 base[CH];
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    const int tile_item = item / a.dgroups, dgroup = item - tile_item * a.dgroups;
+    const int b = a.tiles.image_of(tile_item);
+    const int tile = tile_item - a.tiles.first_tile(b);
+    const i64 first = a.det_off[b];
+    const int D = static_cast<int>(a.det_off[b + 1] - first);
+    const int d0 = dgroup * a.group;
+    const int nd = min(a.group, D - d0);
+    if (nd <= 0) continue;  // uniform across the workgroup
+    const int words = (nd * a.S + 1) / 2;
+    const int words4 = (words + 3) / 4;
+    for (int w = threadIdx.x; w < words4; w += kDetThreads) lds4[w] = make_uint4(0, 0, 0, 0);
+    __syncthreads();
+    const i64 HW = a.tiles.pixels(b);
+    const i64 p0 = static_cast<i64>(tile) * kTile;
+    const int32_t* target = a.target + a.tiles.begin(b);
+    const uint8_t* masks = a.masks[b];
+    const i64* order = a.order + first + d0;
+    int slots[CH][16];
+    i64 base[CH];
 #pragma unroll
-  for (int h = 0; h < CH; ++h) {
-    base[h] = p0 + (static_cast<i64>(h) * kDetThreads + threadIdx.x) * 16;
+    for (int h = 0; h < CH; ++h) {
+      base[h] = p0 + (static_cast<i64>(h) * kDetThreads + threadIdx.x) * 16;
 #pragma unroll
-    for (int j = 0; j < 16; ++j) slots[h][j] = base[h] + j < a.HW ? target[base[h] + j] : -1;
-  }
-  for (int r0 = 0; r0 < nd; r0 += U) {
-    v4i m[U][CH];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (r0 + u < nd) {
-        const uint8_t* mask = masks + order[r0 + u] * a.HW;
-#pragma unroll
-        for (int h = 0; h < CH; ++h) m[u][h] = load16_stream(mask, base[h], a.HW, aligned);
-      } else {
-#pragma unroll
-        for (int h = 0; h < CH; ++h) m[u][h] = v4i{0, 0, 0, 0};
-      }
+      for (int j = 0; j < 16; ++j) slots[h][j] = base[h] + j < HW ? target[base[h] + j] : -1;
     }
+    for (int r0 = 0; r0 < nd; r0 += U) {
+      v4i m[U][CH];
 #pragma unroll
-    for (int u = 0; u < U; ++u)
+      for (int u = 0; u < U; ++u) {
+        if (r0 + u < nd) {
+          const uint8_t* mask = masks + order[r0 + u] * HW;
 #pragma unroll
-      for (int h = 0; h < CH; ++h)
-        if ((m[u][h].x | m[u][h].y | m[u][h].z | m[u][h].w) != 0)
-          count_chunk(m[u][h], slots[h], a.S, (r0 + u) * a.S, bins);
-  }
-  __syncthreads();
-  u64* out = a.pairs + (first + d0) * a.S;
-  for (int w = threadIdx.x; w < words; w += kDetThreads) {
-    const uint32_t v = bins[w];
-    if (!v) continue;
-    const uint32_t lo = v & 0xFFFFu, hi = v >> 16;
-    if (lo) atomicAdd(out + 2 * w, static_cast<u64>(lo));
-    if (hi) atomicAdd(out + 2 * w + 1, static_cast<u64>(hi));
+          for (int h = 0; h < CH; ++h) m[u][h] = load16_stream(mask, base[h], HW, aligned);
+        } else {
+#pragma unroll
+          for (int h = 0; h < CH; ++h) m[u][h] = v4i{0, 0, 0, 0};
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int h = 0; h < CH; ++h)
+          if ((m[u][h].x | m[u][h].y | m[u][h].z | m[u][h].w) != 0)
+            count_chunk(m[u][h], slots[h], a.S, (r0 + u) * a.S, bins);
+    }
+    __syncthreads();
+    u64* out = a.pairs + (first + d0) * a.S;
+    for (int w = threadIdx.x; w < words; w += kDetThreads) {
+      const uint32_t v = bins[w];
+      if (!v) continue;
+      const uint32_t lo = v & 0xFFFFu, hi = v >> 16;
+      if (lo) atomicAdd(out + 2 * w, static_cast<u64>(lo));
+      if (hi) atomicAdd(out + 2 * w + 1, static_cast<u64>(hi));
+    }
+    __syncthreads();  // the next item re-zeroes the bins
   }
 }
 
@@ -683,11 +687,21 @@ extern "C" int pe_mask_overlap(const uint8_t* pm, int64_t D, const uint8_t* gm, 
   return kOk;
 }
 
+extern "C" int64_t pe_detection_histogram_workspace_bytes(int64_t B, int64_t packed_pixels) {
+  if (packed_pixels <= 0) return 0;
+  return static_cast<int64_t>(align_up(4 * static_cast<size_t>(tile_list_ints(B, packed_pixels, 16 * kDetThreads)),
+                                       256));
+}
+
 extern "C" int pe_detection_histogram(int64_t B, const uint8_t* const* masks, const int64_t* det_off,
-                                      const int64_t* order, const int32_t* target, int64_t HW, int64_t S,
-                                      int64_t max_dets, int64_t total_dets, int64_t* pairs, int aligned,
+                                      const int64_t* order, const int32_t* target, const int64_t* pix_off,
+                                      int64_t HW, int64_t S, int64_t max_dets, int64_t total_dets,
+                                      int64_t* pairs, int aligned, void* workspace, int64_t workspace_bytes,
                                       void* stream_ptr) {
-  PE_REQUIRE(B >= 0 && HW > 0 && S > 0 && S < INT_MAX && max_dets >= 0 && total_dets >= 0, "invalid shapes");
+  PE_REQUIRE(B >= 0 && B < INT_MAX && HW > 0 && S > 0 && S < INT_MAX && max_dets >= 0 && total_dets >= 0,
+             "invalid shapes");
+  PE_REQUIRE(!pix_off || workspace_bytes >= pe_detection_histogram_workspace_bytes(B, HW),
+             "workspace too small for a packed batch");
   hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
   int rc;
   if (total_dets && (rc = launch_zero(pairs, 8 * static_cast<size_t>(total_dets * S), stream))) return rc;
@@ -697,11 +711,10 @@ extern "C" int pe_detection_histogram(int64_t B, const uint8_t* const* masks, co
   a.det_off = reinterpret_cast<const i64*>(det_off);
   a.order = reinterpret_cast<const i64*>(order);
   a.target = target;
-  a.HW = HW;
+  a.B = static_cast<int>(B);
   a.S = static_cast<int>(S);
   const DetVariant& v = kDetHist[tuned_variant(kTuneDetectionHistogram)];
   const i64 tile = 16 * kDetThreads * v.chunks;
-  a.tiles = static_cast<int>((HW + tile - 1) / tile);
   // Detections per group: packed 16-bit counters of group x S within the LDS
   // budget (a tile holds at most 16384 pixels, so no counter exceeds 0xFFFF).
   i64 group = static_cast<i64>(kDetMaxLds * 2 / 4) / S;  // bins per group / S
@@ -711,7 +724,26 @@ extern "C" int pe_detection_histogram(int64_t B, const uint8_t* const* masks, co
   a.dgroups = static_cast<int>((max_dets + group - 1) / group);
   a.pairs = reinterpret_cast<u64*>(pairs);
   a.aligned = aligned;
-  const i64 blocks = B * a.tiles * a.dgroups;
+  TileList& t = a.tiles;
+  t.tile = static_cast<int>(tile);
+  i64 blocks;
+  if (pix_off) {
+    // HW is the pixel total of the packed batch.
+    const i64 bound = tile_list_bound(B, HW, tile);
+    PE_REQUIRE(bound * a.dgroups < INT_MAX, "too many detection tiles");
+    int* ws = static_cast<int*>(workspace);
+    t.pix_off = reinterpret_cast<const i64*>(pix_off);
+    t.tile_off = ws;
+    t.tile_image = ws + B + 1;
+    if ((rc = launch_tile_list(t.pix_off, a.B, t.tile, ws, ws + B + 1, stream))) return rc;
+    // Persistent grid over (tile, detection group) items: eight workgroups
+    // per compute unit walk the list with a grid stride.
+    blocks = std::min<i64>(bound * a.dgroups, 8LL * compute_units());
+  } else {
+    t.HW = HW;
+    t.tiles_per_image = static_cast<int>((HW + tile - 1) / tile);
+    blocks = B * t.tiles_per_image * a.dgroups;
+  }
   PE_REQUIRE(blocks < INT_MAX, "too many workgroups");
   const size_t lds = static_cast<size_t>((group * S + 1) / 2 + 3) / 4 * 16;
   if (lds > 64 * 1024)

@@ -109,21 +109,10 @@ struct KeyCache {
 // Completion is published by the kernel boundary: a per-workgroup
 // device-scope fence would write back the XCD's L2 for every chunk.
 template <typename GT, typename PR, bool LOOKUP, bool VEC, bool GLOBAL>
-__global__ __launch_bounds__(kHistThreads) void joint_histogram(HistArgs a) {
-  extern __shared__ uint4 lds4[];
-  uint32_t* bins = reinterpret_cast<uint32_t*>(lds4);
-  const int b = blockIdx.x / a.chunks_per_image;
-  const int chunk = blockIdx.x - b * a.chunks_per_image;
-  const int tid = threadIdx.x;
-  if (!GLOBAL) {
-    for (int w = tid; w < a.words4; w += kHistThreads) lds4[w] = make_uint4(0, 0, 0, 0);
-    __syncthreads();
-  }
+__device__ __forceinline__ void histogram_chunk(const HistArgs& a, int b, i64 begin, i64 end, uint32_t* bins) {
   const GT* gt = static_cast<const GT*>(a.gt);
   const PR* pred = static_cast<const PR*>(a.pred);
-  const i64 image = static_cast<i64>(b) * a.HW;
-  const i64 begin = image + static_cast<i64>(chunk) * a.chunk;
-  const i64 end = image + min(static_cast<i64>(chunk + 1) * a.chunk, a.HW);
+  const int tid = threadIdx.x;
   u64* pairs = a.pairs + static_cast<i64>(b) * a.G * a.P;
   SlotResolver<GT, LOOKUP> gslot{LOOKUP && a.gt_ids ? a.gt_ids + static_cast<i64>(b) * a.G : nullptr,
                                  LOOKUP && a.gt_ids ? a.gt_slots + static_cast<i64>(b) * a.G : nullptr, a.G};
@@ -182,19 +171,48 @@ __global__ __launch_bounds__(kHistThreads) void joint_histogram(HistArgs a) {
   }
   cache.drain(bins, pairs);
   if (bad && a.error) atomicOr(a.error, 1);
-  if (!GLOBAL) {
-    __syncthreads();
-    for (int w = tid; w < a.words4; w += kHistThreads) {
-      uint4 v = lds4[w];
-      uint32_t words[4] = {v.x, v.y, v.z, v.w};
+}
+
+// The pair table must be zero on entry (the scorer re-zeroes what it read).
+// Completion is published by the kernel boundary: a per-workgroup
+// device-scope fence would write back the XCD's L2 for every chunk.
+// Workgroups walk the chunk list with a grid stride: one chunk each for
+// uniform batches (grid = chunk count), several for packed batches (a
+// persistent grid of a few workgroups per compute unit).
+template <typename GT, typename PR, bool LOOKUP, bool VEC, bool GLOBAL>
+__global__ __launch_bounds__(kHistThreads) void joint_histogram(HistArgs a) {
+  extern __shared__ uint4 lds4[];
+  uint32_t* bins = reinterpret_cast<uint32_t*>(lds4);
+  const int tid = threadIdx.x;
+  const int total = a.tiles.count(a.B);
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    const int b = a.tiles.image_of(item);
+    const int chunk = item - a.tiles.first_tile(b);
+    const i64 image = a.tiles.begin(b);
+    const i64 pixels = a.tiles.pixels(b);
+    const i64 begin = image + static_cast<i64>(chunk) * a.tiles.tile;
+    const i64 end = image + min(static_cast<i64>(chunk + 1) * a.tiles.tile, pixels);
+    if (!GLOBAL) {
+      for (int w = tid; w < a.words4; w += kHistThreads) lds4[w] = make_uint4(0, 0, 0, 0);
+      __syncthreads();
+    }
+    histogram_chunk<GT, PR, LOOKUP, VEC, GLOBAL>(a, b, begin, end, bins);
+    if (!GLOBAL) {
+      __syncthreads();
+      u64* pairs = a.pairs + static_cast<i64>(b) * a.G * a.P;
+      for (int w = tid; w < a.words4; w += kHistThreads) {
+        uint4 v = lds4[w];
+        uint32_t words[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        if (!words[j]) continue;
-        uint32_t key = (static_cast<uint32_t>(w) * 4u + j) * 2u;
-        uint32_t lo = words[j] & 0xFFFFu, hi = words[j] >> 16;
-        if (lo) atomicAdd(pairs + key, static_cast<u64>(lo));
-        if (hi) atomicAdd(pairs + key + 1, static_cast<u64>(hi));
+        for (int j = 0; j < 4; ++j) {
+          if (!words[j]) continue;
+          uint32_t key = (static_cast<uint32_t>(w) * 4u + j) * 2u;
+          uint32_t lo = words[j] & 0xFFFFu, hi = words[j] >> 16;
+          if (lo) atomicAdd(pairs + key, static_cast<u64>(lo));
+          if (hi) atomicAdd(pairs + key + 1, static_cast<u64>(hi));
+        }
       }
+      __syncthreads();  // the next chunk re-zeroes the bins
     }
   }
 }
@@ -234,18 +252,38 @@ int hist_default_chunk(i64 total_pixels) {
   return static_cast<int>(chunk);
 }
 
-int launch_joint_histogram(HistArgs a, int gt_bytes, int pred_bytes, int chunk, hipStream_t stream) {
-  if (a.B == 0 || a.HW == 0) return kOk;
+// Packed batches need tile_ints() ints of workspace for the chunk list.
+int launch_joint_histogram(HistArgs a, int gt_bytes, int pred_bytes, int chunk, int* tile_ws, hipStream_t stream) {
+  if (a.B == 0) return kOk;
+  const i64 total_pixels = a.pix_off ? a.HW : a.HW * a.B;  // a packed batch passes its pixel total
+  if (total_pixels == 0) return kOk;
   if (chunk <= 0) {
     // Tuned pixel chunk per workgroup (variant 0 sizes it from the batch).
     static const int kChunks[] = {0, 8192, 16384, 32768, kMaxChunk};
     chunk = kChunks[tuned_variant(kTuneJointHistogram)];
-    if (chunk <= 0) chunk = hist_default_chunk(a.HW * a.B);
+    if (chunk <= 0) chunk = hist_default_chunk(total_pixels);
   }
-  chunk = std::min<int>(std::max<int>(chunk & ~3, 4), kMaxChunk);
-  a.chunk = chunk;
-  a.chunks_per_image = static_cast<int>((a.HW + chunk - 1) / chunk);
-  i64 blocks = static_cast<i64>(a.B) * a.chunks_per_image;
+  chunk = std::min<int>(std::max<int>(chunk & ~3, 4096), kMaxChunk);
+  TileList& t = a.tiles;
+  t = TileList{};
+  t.tile = chunk;
+  i64 blocks;
+  if (a.pix_off) {
+    if (!tile_ws) return fail(kBadArgument, "a packed batch needs the chunk-list workspace");
+    const i64 bound = tile_list_bound(a.B, total_pixels, 4096);
+    if (bound >= INT_MAX) return fail(kUnsupportedShape, "too many histogram chunks");
+    t.pix_off = a.pix_off;
+    t.tile_off = tile_ws;
+    t.tile_image = tile_ws + a.B + 1;
+    int rc = launch_tile_list(a.pix_off, a.B, chunk, tile_ws, tile_ws + a.B + 1, stream);
+    if (rc) return rc;
+    // Persistent grid: the exact chunk count lives on the device.
+    blocks = std::min<i64>(tile_list_bound(a.B, total_pixels, chunk), 8LL * compute_units());
+  } else {
+    t.HW = a.HW;
+    t.tiles_per_image = static_cast<int>((a.HW + chunk - 1) / chunk);
+    blocks = static_cast<i64>(a.B) * t.tiles_per_image;
+  }
   if (blocks > INT_MAX) return fail(kUnsupportedShape, "too many histogram workgroups");
   i64 bins = static_cast<i64>(a.G) * a.P;
   i64 words = (bins + 1) / 2;
@@ -976,15 +1014,17 @@ __global__ __launch_bounds__(256) void pack_slots(const T* maps, i64 HW, const i
 
 using namespace pe;
 
-extern "C" int64_t pe_update_workspace_bytes(int64_t B, int64_t G, int64_t P, int64_t C) {
+extern "C" int64_t pe_update_workspace_bytes(int64_t B, int64_t G, int64_t P, int64_t C, int64_t packed_pixels) {
   Carve carve(nullptr);
   carve.take<int>(B * P);        // rec_lab (prediction labels, large G/P only)
   carve.take<double>(B * P);     // rec_val
+  if (packed_pixels > 0) carve.take<int>(tile_list_ints(B, packed_pixels, 4096));  // chunk list
   return static_cast<int64_t>(align_up(carve.used, 256));
 }
 
 extern "C" int pe_update(const void* gt, int gt_bytes, const void* pred, int pred_bytes, int64_t B,
-                         int64_t H, int64_t W, const int32_t* gc, const int32_t* pc, const uint8_t* crowd,
+                         int64_t H, int64_t W, const int64_t* pix_off, const int32_t* gc, const int32_t* pc,
+                         const uint8_t* crowd,
                          int64_t G, int64_t P, int64_t C, int64_t* pairs, int64_t* areas, int32_t* matched,
                          int64_t* tp, int64_t* fp, int64_t* fn, double* iou_sum, int64_t* confusion,
                          const int64_t* gt_ids, const int32_t* gt_slots, const int64_t* pred_ids,
@@ -997,7 +1037,8 @@ extern "C" int pe_update(const void* gt, int gt_bytes, const void* pred, int pre
   PE_REQUIRE((gt_bytes == 4 || gt_bytes == 8) && (pred_bytes == 4 || pred_bytes == 8), "maps must be int32/int64");
   PE_REQUIRE((gt_ids == nullptr) == (gt_slots == nullptr) && (pred_ids == nullptr) == (pred_slots == nullptr),
              "lookup IDs and slots must be supplied together");
-  PE_REQUIRE(workspace_bytes >= pe_update_workspace_bytes(B, G, P, C), "workspace too small");
+  const i64 packed_pixels = pix_off ? H * W : 0;
+  PE_REQUIRE(workspace_bytes >= pe_update_workspace_bytes(B, G, P, C, packed_pixels), "workspace too small");
   PE_REQUIRE(!validate || error, "validation needs the error word");
   PE_REQUIRE(reward || (iou_exact && confusion), "dataset mode needs the confusion and exact IoU accumulators");
   hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
@@ -1031,7 +1072,8 @@ extern "C" int pe_update(const void* gt, int gt_bytes, const void* pred, int pre
   h.gt = gt;
   h.pred = pred;
   h.B = static_cast<int>(B);
-  h.HW = H * W;
+  h.HW = H * W;  // a packed batch: the pixel total
+  h.pix_off = reinterpret_cast<const i64*>(pix_off);
   h.G = s.G;
   h.P = s.P;
   h.pairs = reinterpret_cast<u64*>(pairs);
@@ -1040,7 +1082,8 @@ extern "C" int pe_update(const void* gt, int gt_bytes, const void* pred, int pre
   h.pred_ids = pred_ids;
   h.pred_slots = pred_slots;
   h.error = validate ? error : nullptr;
-  if ((rc = launch_joint_histogram(h, gt_bytes, pred_bytes, 0, stream))) return rc;
+  int* tile_ws = pix_off ? carve.take<int>(tile_list_ints(B, packed_pixels, 4096)) : nullptr;
+  if ((rc = launch_joint_histogram(h, gt_bytes, pred_bytes, 0, tile_ws, stream))) return rc;
   if (validate) {
     if ((rc = launch_prepare(s, error, stream))) return rc;
     int32_t host_error = 0;

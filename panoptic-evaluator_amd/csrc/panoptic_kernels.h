@@ -16,10 +16,10 @@ struct HistArgs {
   const void* gt;
   const void* pred;
   int B;
-  i64 HW;
+  i64 HW;                // pixels per image (uniform batches)
+  const i64* pix_off;    // B + 1 pixel offsets of a packed batch, or nullptr
   int G, P;
-  int chunk;             // pixels per workgroup
-  int chunks_per_image;
+  TileList tiles;        // chunks of <= kMaxChunk pixels, each inside one image
   int words4;            // LDS size in 16-byte units
   u64* pairs;            // B x G x P
   const i64* gt_ids;
@@ -73,7 +73,7 @@ struct FinalArgs {
 
 int hist_default_chunk(i64 total_pixels);
 // The pair table must be zero on entry; launch_score leaves it zeroed again.
-int launch_joint_histogram(HistArgs a, int gt_bytes, int pred_bytes, int chunk, hipStream_t stream);
+int launch_joint_histogram(HistArgs a, int gt_bytes, int pred_bytes, int chunk, int* tile_ws, hipStream_t stream);
 int launch_prepare(const ScoreArgs& a, int* error, hipStream_t stream);
 int launch_score(ScoreArgs a, hipStream_t stream);
 int launch_finalize(const FinalArgs& a, i64 K, hipStream_t stream);

@@ -15,7 +15,7 @@ import threading
 import torch
 
 LIBRARY_PATH = pathlib.Path(__file__).with_name("libpanoptic_hip.so")
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 _i64, _int, _ptr, _dbl = ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_double
 
@@ -30,8 +30,8 @@ _SIGNATURES = {
     "pe_tune_set": (_int, [ctypes.c_char_p, _int]),
     "pe_tune_get": (_int, [ctypes.c_char_p]),
     "pe_tune_describe": (ctypes.c_char_p, [ctypes.c_char_p, _int]),
-    "pe_update_workspace_bytes": (_i64, [_i64] * 4),
-    "pe_update": (_int, [_ptr, _int, _ptr, _int, _i64, _i64, _i64, _ptr, _ptr, _ptr, _i64, _i64, _i64,
+    "pe_update_workspace_bytes": (_i64, [_i64] * 5),
+    "pe_update": (_int, [_ptr, _int, _ptr, _int, _i64, _i64, _i64, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64,
                          _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr,
                          _ptr, _ptr, _ptr, _i64, _int, _int, _int, _ptr]),
     "pe_compute": (_int, [_ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _ptr, _ptr, _ptr, _ptr,

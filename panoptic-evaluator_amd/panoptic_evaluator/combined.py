@@ -50,21 +50,32 @@ class SegmentationEvaluator:
         if maps.device != self.device:
             raise ValueError("target must share evaluator device")
         prepared = self.instance._prepare(detections, prediction=True)
-        if len(prepared) != len(maps) or any(image.masks.shape[1:] != maps.shape[1:] for image in prepared):
+        sizes = target.image_sizes if target.packed else [tuple(maps.shape[1:])] * maps.shape[0]
+        if len(prepared) != len(sizes) or any(tuple(image.masks.shape[1:]) != tuple(size)
+                                              for image, size in zip(prepared, sizes)):
             raise ValueError("detections and target must have matching image counts and dimensions")
         self._validate_areas(instance_areas, classes)
         device, C = self.device, self.num_classes
         stream = _native.stream(device)
-        B, H, W = maps.shape
-        S = classes.shape[1]
-        HW = H * W
+        B, S = len(sizes), classes.shape[1]
         ranked = score_order(prepared, device)
         det_counts, N, det_off = ranked.counts, sum(ranked.counts), ranked.det_off
         pointers = _native.device_table([image.masks.data_ptr() for image in prepared], device)
-        aligned = int(HW % 16 == 0 and all(image.masks.data_ptr() % 16 == 0 for image in prepared))
+        aligned = int(all((h * w) % 16 == 0 for h, w in sizes)
+                      and all(image.masks.data_ptr() % 16 == 0 for image in prepared))
         pairs = torch.empty((N, S), dtype=torch.int64, device=device)
+        if target.packed:
+            # Packed images: pixel offsets, the pixel total and a tile-list workspace.
+            pix_off = offsets([h * w for h, w in sizes], device)
+            HW = maps.numel()
+            size = _native.library().pe_detection_histogram_workspace_bytes(B, HW)
+            workspace = torch.empty(max(size, 256), dtype=torch.uint8, device=device)
+        else:
+            pix_off, HW, workspace = None, sizes[0][0] * sizes[0][1], None
         _native.call("pe_detection_histogram", B, pointers.data_ptr(), det_off.data_ptr(), ranked.order.data_ptr(),
-                     maps.data_ptr(), HW, S, max(det_counts, default=0), N, pairs.data_ptr(), aligned, stream)
+                     maps.data_ptr(), _native.ptr(pix_off), HW, S, max(det_counts, default=0), N,
+                     pairs.data_ptr(), aligned, _native.ptr(workspace),
+                     0 if workspace is None else workspace.numel(), stream)
         # Full detection areas, including void and stuff pixels.
         det_area = torch.empty(N, dtype=torch.float64, device=device)
         _native.call("pe_row_sums", pairs.data_ptr(), N, S, det_area.data_ptr(), stream)

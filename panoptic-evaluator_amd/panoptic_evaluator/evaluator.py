@@ -51,6 +51,43 @@ class PanopticBatch:
     crowd: torch.Tensor | None = None
     lookup_ids: torch.Tensor | None = None
     lookup_slots: torch.Tensor | None = None
+    image_sizes: tuple | None = None
+
+    @classmethod
+    def pack(cls, maps: Sequence[torch.Tensor], classes: Sequence[torch.Tensor],
+             crowd: Sequence[torch.Tensor] | None = None) -> PanopticBatch:
+        """A batch of images of uneven size (no counterpart in the reference,
+        whose batches share one size, README.md:57-58): compact int32 H_i x W_i
+        maps are stored one after another in a flat ``segment_map`` and the
+        per-image slot metadata (S_i int32 classes, S_i bool crowd) is padded
+        with class -1 to the largest S.  ``image_sizes`` keeps (H_i, W_i)."""
+        if len(maps) == 0 or len(maps) != len(classes) or (crowd is not None and len(crowd) != len(maps)):
+            raise ValueError("pack needs one map, one classes tensor (and one crowd tensor) per image")
+        device = maps[0].device
+        for m, c in zip(maps, classes):
+            if m.ndim != 2 or m.dtype != torch.int32 or c.ndim != 1 or c.dtype != torch.int32 or c.numel() < 1:
+                raise ValueError("packed maps must be int32 H x W and classes int32 S tensors with S >= 1")
+            if m.device != device or c.device != device:
+                raise ValueError("batch tensors must share a device")
+        S = max(c.numel() for c in classes)
+        B = len(maps)
+        flat = torch.cat([m.reshape(-1) for m in maps])
+        padded = torch.full((B, S), -1, dtype=torch.int32, device=device)
+        for b, c in enumerate(classes):
+            padded[b, :c.numel()] = c
+        padded_crowd = None
+        if crowd is not None:
+            padded_crowd = torch.zeros((B, S), dtype=torch.bool, device=device)
+            for b, (c, k) in enumerate(zip(classes, crowd)):
+                if k.shape != c.shape or k.dtype != torch.bool or k.device != device:
+                    raise ValueError("crowd must be bool with the same shape as classes")
+                padded_crowd[b, :k.numel()] = k
+        sizes = tuple((int(m.shape[0]), int(m.shape[1])) for m in maps)
+        return cls(flat, padded, padded_crowd, image_sizes=sizes)
+
+    @property
+    def packed(self) -> bool:
+        return self.image_sizes is not None
 
     @classmethod
     def from_coco(cls, maps: torch.Tensor, segments_info: Sequence[Sequence[dict]],
@@ -101,6 +138,10 @@ class PanopticBatch:
 
     def with_maps(self, maps: torch.Tensor) -> PanopticBatch:
         """Reuse prepared metadata for another map batch (evaluator.py:108-118)."""
+        if self.packed:
+            if maps.shape != self.segment_map.shape or maps.device != self.classes.device:
+                raise ValueError("new packed maps must have the same pixel count and metadata device")
+            return replace(self, segment_map=maps)
         if maps.ndim != 3 or maps.shape[0] != self.classes.shape[0] or maps.device != self.classes.device:
             raise ValueError("new maps must have the same batch size and metadata device")
         return replace(self, segment_map=maps)
@@ -138,7 +179,13 @@ def _prepare(batch: PanopticBatch, num_classes: int, validate: bool, *, ignore_c
     """Shape, dtype and device checks (evaluator.py:161-207).  Value checks run
     inside the kernels (``validate``) and never on the host."""
     maps, classes = batch.segment_map, batch.classes
-    if maps.ndim != 3 or classes.ndim != 2 or classes.shape[0] != maps.shape[0] or classes.shape[1] < 1:
+    if batch.packed:
+        if (maps.ndim != 1 or classes.ndim != 2 or classes.shape[0] != len(batch.image_sizes)
+                or classes.shape[1] < 1 or maps.numel() != sum(h * w for h, w in batch.image_sizes)):
+            raise ValueError("a packed batch needs flat maps of sum(H_i * W_i) pixels and classes B x S")
+        if batch.lookup_ids is not None:
+            raise ValueError("packed batches take compact slots (compact() them image by image first)")
+    elif maps.ndim != 3 or classes.ndim != 2 or classes.shape[0] != maps.shape[0] or classes.shape[1] < 1:
         raise ValueError("expected maps B x H x W and classes B x S with S >= 1")
     original_ids = batch.lookup_ids is not None
     if (batch.lookup_slots is not None) != original_ids:
@@ -164,6 +211,25 @@ def _lookup(batch: PanopticBatch):
     if batch.lookup_ids is None:
         return None, None
     return batch.lookup_ids.contiguous(), batch.lookup_slots.contiguous()
+
+
+def _geometry(prediction: PanopticBatch, target: PanopticBatch, gt: torch.Tensor, pred: torch.Tensor):
+    """(B, H, W, pix_off, packed_pixels) of a prediction/target pair: B x H x W
+    maps, or packed images (H = 1, W = pixel total, device pixel offsets)."""
+    if target.packed or prediction.packed:
+        if target.image_sizes != prediction.image_sizes:
+            raise ValueError("prediction and target must pack the same image sizes")
+        if gt.shape != pred.shape or gt.device != pred.device:
+            raise ValueError("prediction and target must share shape and device")
+        offsets, total = [0], 0
+        for h, w in target.image_sizes:
+            total += h * w
+            offsets.append(total)
+        return len(target.image_sizes), 1, total, _native.device_table(offsets, gt.device), total
+    if gt.shape != pred.shape or gt.device != pred.device:
+        raise ValueError("prediction and target must share shape and device")
+    B, H, W = gt.shape
+    return B, H, W, None, 0
 
 
 class PanopticEvaluator:
@@ -208,15 +274,17 @@ class PanopticEvaluator:
     def update(self, prediction: PanopticBatch, target: PanopticBatch):
         return self._update(prediction, target)
 
-    def _scratch_for(self, B, G, P):
+    def _scratch_for(self, B, G, P, packed_pixels=0):
         shape = (B, G, P)
+        size = _native.library().pe_update_workspace_bytes(B, G, P, self.num_classes, packed_pixels)
+        if self._workspace is not None and self._workspace.numel() < size:
+            self._scratch = None
         if self._scratch is None or self._scratch[0].shape != shape:
             device = self.device
             # The pair table starts zeroed and every update leaves it zeroed.
             self._scratch = (torch.zeros(shape, dtype=torch.int64, device=device),
                              torch.empty((B, G + P), dtype=torch.int64, device=device),
                              torch.empty((B, G), dtype=torch.int32, device=device))
-            size = _native.library().pe_update_workspace_bytes(B, G, P, self.num_classes)
             self._workspace = torch.zeros(max(size, 256), dtype=torch.uint8, device=device)
         return self._scratch
 
@@ -229,16 +297,16 @@ class PanopticEvaluator:
             target = replace(target, crowd=self._default_crowd)
         gt, gc, crowd = _prepare(target, self.num_classes, False)
         pred, pc, _ = _prepare(prediction, self.num_classes, False, ignore_crowd=True)
-        if gt.shape != pred.shape or gt.device != pred.device or gt.device != self.tp.device:
+        if gt.device != self.tp.device:
             raise ValueError("prediction and target must share shape and evaluator device")
-        B, H, W = gt.shape
+        B, H, W, pix_off, packed_pixels = _geometry(prediction, target, gt, pred)
         G, P = gc.shape[1], pc.shape[1]
-        pairs, areas, matched = self._scratch_for(B, G, P)
+        pairs, areas, matched = self._scratch_for(B, G, P, packed_pixels)
         gt_ids, gt_slots = _lookup(target)
         pred_ids, pred_slots = _lookup(prediction)
         code = _native.call(
             "pe_update", gt.data_ptr(), gt.element_size(), pred.data_ptr(), pred.element_size(), B, H, W,
-            gc.data_ptr(), pc.data_ptr(), crowd.data_ptr(), G, P, self.num_classes,
+            _native.ptr(pix_off), gc.data_ptr(), pc.data_ptr(), crowd.data_ptr(), G, P, self.num_classes,
             pairs.data_ptr(), areas.data_ptr(), matched.data_ptr(), self.tp.data_ptr(),
             self.fp.data_ptr(), self.fn.data_ptr(), self.iou_sum.data_ptr(), self.confusion.data_ptr(),
             _native.ptr(gt_ids), _native.ptr(gt_slots), _native.ptr(pred_ids), _native.ptr(pred_slots),
@@ -296,10 +364,8 @@ def panoptic_quality(prediction: PanopticBatch, target: PanopticBatch, num_class
         raise ValueError('reduction must be "none"')
     gt, gc, crowd = _prepare(target, num_classes, False)
     pred, pc, _ = _prepare(prediction, num_classes, False, ignore_crowd=True, allow_empty_slots=True)
-    if gt.shape != pred.shape or gt.device != pred.device:
-        raise ValueError("prediction and target must share shape and device")
     device = _native.require_gpu(gt.device)
-    B, H, W = gt.shape
+    B, H, W, pix_off, packed_pixels = _geometry(prediction, target, gt, pred)
     G, P, C = gc.shape[1], pc.shape[1], num_classes
     tp = torch.zeros((B, C), dtype=torch.int64, device=device)
     fp, fn = torch.zeros_like(tp), torch.zeros_like(tp)
@@ -308,14 +374,15 @@ def panoptic_quality(prediction: PanopticBatch, target: PanopticBatch, num_class
     areas = torch.empty((B, G + P), dtype=torch.int64, device=device)
     matched = torch.empty((B, G), dtype=torch.int32, device=device)
     error = torch.empty(1, dtype=torch.int32, device=device)
-    size = max(_native.library().pe_update_workspace_bytes(B, G, P, C), 256)
+    size = max(_native.library().pe_update_workspace_bytes(B, G, P, C, packed_pixels), 256)
     workspace = torch.zeros(size, dtype=torch.uint8, device=device)
     gt_ids, gt_slots = _lookup(target)
     pred_ids, pred_slots = _lookup(prediction)
     stream = _native.stream(device)
     code = _native.call(
         "pe_update", gt.data_ptr(), gt.element_size(), pred.data_ptr(), pred.element_size(), B, H, W,
-        gc.data_ptr(), pc.data_ptr(), crowd.data_ptr(), G, P, C, pairs.data_ptr(), areas.data_ptr(),
+        _native.ptr(pix_off), gc.data_ptr(), pc.data_ptr(), crowd.data_ptr(), G, P, C, pairs.data_ptr(),
+        areas.data_ptr(),
         matched.data_ptr(), tp.data_ptr(), fp.data_ptr(), fn.data_ptr(), sums.data_ptr(), None,
         _native.ptr(gt_ids), _native.ptr(gt_slots), _native.ptr(pred_ids), _native.ptr(pred_slots),
         error.data_ptr(), None, workspace.data_ptr(), workspace.numel(), int(validate), 1, 0, stream)

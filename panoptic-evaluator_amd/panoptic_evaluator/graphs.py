@@ -1,0 +1,96 @@
+"""HIP-graph capture of an evaluator's update().
+
+``CapturedUpdate(evaluator, *update_args)`` records one ``evaluator.update``
+(PanopticEvaluator, InstanceEvaluator or SegmentationEvaluator, built with
+``validate=False``) as a HIP graph.  ``replay()`` re-runs every kernel of that
+update on the same input buffers -- refill them in place between replays --
+and accumulates into the evaluator's state exactly as an eager update would;
+``compute()`` and ``reset()`` work as usual.  The batch layout (sizes, slot
+capacities, detection counts, buffer addresses) is fixed at capture.
+
+Capture needs no host work inside the graph: the update path allocates its
+scratch on the first call, keeps small metadata tables in a device cache, and
+clears memory with fill kernels (no memset nodes).  One untimed eager update
+runs first to populate those; the evaluator's state is restored afterwards.
+"""
+
+from __future__ import annotations
+
+import torch
+
+
+def _parts(evaluator):
+    """(panoptic evaluator or None, instance evaluator or None)."""
+    from .combined import SegmentationEvaluator
+    from .evaluator import PanopticEvaluator
+    from .instance import InstanceEvaluator
+    if isinstance(evaluator, SegmentationEvaluator):
+        return evaluator.panoptic, evaluator.instance
+    if isinstance(evaluator, PanopticEvaluator):
+        return evaluator, None
+    if isinstance(evaluator, InstanceEvaluator):
+        return None, evaluator
+    raise TypeError("expected a PanopticEvaluator, InstanceEvaluator or SegmentationEvaluator")
+
+
+class CapturedUpdate:
+    """One update() of ``evaluator`` recorded as a HIP graph."""
+
+    def __init__(self, evaluator, *args, **kwargs):
+        panoptic, instance = _parts(evaluator)
+        if any(part is not None and part.validate for part in (panoptic, instance)):
+            raise ValueError("capture needs validate=False: validation reads its result on the host")
+        self.evaluator = evaluator
+        self._instance = instance
+        device = (panoptic or instance).device
+        saved = self._save(panoptic, instance)
+        # Warm-up on a side stream (default priority): scratch allocation and
+        # the device metadata cache are filled outside the capture.
+        side = torch.cuda.Stream(device)
+        side.wait_stream(torch.cuda.current_stream(device))
+        with torch.cuda.stream(side):
+            evaluator.update(*args, **kwargs)
+        torch.cuda.current_stream(device).wait_stream(side)
+        torch.cuda.synchronize(device)
+        self._restore(panoptic, instance, saved)
+        self.graph = torch.cuda.CUDAGraph()
+        self._record = None
+        if instance is not None:
+            # The captured update's detection records live in graph memory;
+            # every replay appends a copy of them.
+            instance._commit = self._keep
+        try:
+            with torch.cuda.graph(self.graph):
+                evaluator.update(*args, **kwargs)
+        finally:
+            if instance is not None:
+                del instance._commit  # back to the class's method
+        torch.cuda.synchronize(device)
+
+    def _keep(self, record):
+        self._record = record
+
+    @staticmethod
+    def _save(panoptic, instance):
+        state = {}
+        if panoptic is not None:
+            state["panoptic"] = panoptic._state.clone()
+        if instance is not None:
+            state["counts"] = instance._counts.clone()
+            state["records"] = list(instance._records)
+        return state
+
+    @staticmethod
+    def _restore(panoptic, instance, state):
+        if panoptic is not None:
+            panoptic._state.copy_(state["panoptic"])
+        if instance is not None:
+            instance._counts.copy_(state["counts"])
+            instance._records = state["records"]
+
+    def replay(self):
+        """Run the recorded update once more (on the current stream's device)."""
+        self.graph.replay()
+        if self._record is not None:
+            self._instance._records.append(tuple(t.clone() for t in self._record))
+        return self.evaluator

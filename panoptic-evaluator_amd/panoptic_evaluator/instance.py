@@ -19,7 +19,9 @@ from . import _native
 _i64, _int, _ptr = ctypes.c_int64, ctypes.c_int, ctypes.c_void_p
 for _name, _sig in {
     "pe_mask_overlap": (_int, [_ptr, _i64, _ptr, _i64, _i64, _ptr, _ptr, _ptr, _ptr]),
-    "pe_detection_histogram": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _i64, _ptr, _int, _ptr]),
+    "pe_detection_histogram_workspace_bytes": (_i64, [_i64, _i64]),
+    "pe_detection_histogram": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _i64, _ptr, _int,
+                                      _ptr, _i64, _ptr]),
     "pe_coco_match": (_int, [_i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _int,
                              _i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _ptr]),
     "pe_score_order_workspace_bytes": (_i64, [_i64, _i64]),
@@ -134,8 +136,13 @@ class InstanceEvaluator:
         self.device = self._counts.device  # e.g. "cuda" -> cuda:0, as instance.py:70
 
     def reset(self):
+        """Drop the records and zero the positive counts in place (their
+        storage is stable, so a captured update keeps accumulating into it)."""
         self._records = []
-        self._counts = torch.zeros((self.num_classes, 4), dtype=torch.int64, device=self.device)
+        if getattr(self, "_counts", None) is None:
+            self._counts = torch.zeros((self.num_classes, 4), dtype=torch.int64, device=self.device)
+        else:
+            self._counts.zero_()
 
     def _prepare(self, batch: InstanceBatch, *, prediction: bool):
         """Shape/dtype/device checks and the stable per-image score order

@@ -100,24 +100,29 @@ def panoptic_batch(seed, B, H, W, seeds, num_classes, num_things, device="cpu", 
     return PanopticPair(gt.contiguous(), pred.contiguous(), gc, pc, crowd)
 
 
-def ellipse_masks(gen, N, H, W, device="cpu", min_axis=8.0, max_axis=200.0) -> torch.Tensor:
+def ellipse_masks(gen, N, H, W, device="cpu", min_axis=8.0, max_axis=200.0, block=16) -> torch.Tensor:
     """N rotated ellipses (semi-axes U[min, max], centre uniform, angle
-    U[0, pi)) as an N x H x W bool tensor; every mask has its centre pixel."""
+    U[0, pi)) as an N x H x W bool tensor; every mask has its centre pixel.
+    Masks are evaluated ``block`` at a time (same float32 formula per pixel)."""
     centre = _rand(gen, N, 2, device=device) * torch.tensor([H, W], device=device, dtype=torch.float64)
     axes = min_axis + _rand(gen, N, 2, device=device) * (max_axis - min_axis)
     angle = _rand(gen, N, device=device) * math.pi
-    ys = torch.arange(H, device=device, dtype=torch.float32)[:, None]
-    xs = torch.arange(W, device=device, dtype=torch.float32)[None, :]
+    ys = torch.arange(H, device=device, dtype=torch.float32)[None, :, None]
+    xs = torch.arange(W, device=device, dtype=torch.float32)[None, None, :]
     masks = torch.empty((N, H, W), dtype=torch.bool, device=device)
     cy, cx = centre[:, 0].float(), centre[:, 1].float()
     a, b = axes[:, 0].float(), axes[:, 1].float()
     cos, sin = torch.cos(angle).float(), torch.sin(angle).float()
-    for n in range(N):
-        dy, dx = ys - cy[n], xs - cx[n]
-        u = (dx * cos[n] + dy * sin[n]) / a[n]
-        v = (-dx * sin[n] + dy * cos[n]) / b[n]
-        masks[n] = u * u + v * v <= 1.0
-        masks[n, int(cy[n].clamp(0, H - 1)), int(cx[n].clamp(0, W - 1))] = True
+    for n0 in range(0, N, block):
+        sl = slice(n0, min(N, n0 + block))
+        col = lambda t: t[sl][:, None, None]  # noqa: E731
+        dy, dx = ys - col(cy), xs - col(cx)
+        u = (dx * col(cos) + dy * col(sin)) / col(a)
+        v = (-dx * col(sin) + dy * col(cos)) / col(b)
+        masks[sl] = u * u + v * v <= 1.0
+    rows = cy.clamp(0, H - 1).long()
+    cols = cx.clamp(0, W - 1).long()
+    masks[torch.arange(N, device=device), rows, cols] = True
     return masks
 
 
@@ -132,6 +137,45 @@ def config1(device="cpu", seed=1, B=16, H=800, W=1088, detections=100):
     classes = [_randint(gen, 0, 80, (detections,), device) for _ in range(B)]
     scores = [_rand(gen, detections, device=device) for _ in range(B)]
     return pair, masks, classes, scores
+
+
+@dataclass
+class UnevenBatch:
+    """Per-image lists of a batch of uneven images (config 2)."""
+
+    sizes: list          # (H, W)
+    gt: list             # H x W int32
+    pred: list
+    gc: list             # S_i int32
+    pc: list
+    crowd: list          # S_i bool
+    masks: list          # D_i x H x W bool
+    classes: list        # D_i int64
+    scores: list         # D_i float64
+
+
+def config2(device="cpu", seed=2, B=256, sizes=(10, 42), segments=(5, 120), detections=(0, 300),
+            num_classes=133, num_things=80):
+    """B images with H and W drawn from multiples of 32 in [32 * sizes[0],
+    32 * sizes[1]], U[segments] Voronoi seeds and U[detections] ellipses."""
+    gen = torch.Generator(device=device).manual_seed(seed)
+    out = UnevenBatch([], [], [], [], [], [], [], [], [])
+    for _ in range(B):
+        hw = _randint(gen, sizes[0], sizes[1] + 1, (2,), device)
+        H, W = 32 * int(hw[0]), 32 * int(hw[1])
+        seeds = int(_randint(gen, segments[0], segments[1] + 1, (1,), device))
+        D = int(_randint(gen, detections[0], detections[1] + 1, (1,), device))
+        gt, pred, gc, pc, crowd = voronoi_image(gen, H, W, seeds, num_classes, num_things, device=device)
+        out.sizes.append((H, W))
+        out.gt.append(gt)
+        out.pred.append(pred)
+        out.gc.append(gc)
+        out.pc.append(pc)
+        out.crowd.append(crowd)
+        out.masks.append(ellipse_masks(gen, D, H, W, device=device))
+        out.classes.append(_randint(gen, 0, num_things, (D,), device))
+        out.scores.append(_rand(gen, D, device=device))
+    return out
 
 
 def config3(device="cpu", seed=3, B=64, H=1024, W=2048):

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
     missing = [name for name in declared_functions() if not hasattr(lib, name)]
     assert not missing, missing
     lib.pe_abi_version.restype = ctypes.c_int
-    assert lib.pe_abi_version() == 1
+    assert lib.pe_abi_version() == 2
 
 
 @pytest.mark.skipif(not LIB.exists(), reason="library not built")
@@ -44,6 +44,13 @@ def test_workspace_queries_are_host_only():
     lib = ctypes.CDLL(str(LIB))
     fn = lib.pe_update_workspace_bytes
     fn.restype = ctypes.c_int64
-    fn.argtypes = [ctypes.c_int64] * 4
-    small, large = fn(1, 2, 3, 4), fn(8, 41, 47, 133)
+    fn.argtypes = [ctypes.c_int64] * 5
+    small, large = fn(1, 2, 3, 4, 0), fn(8, 41, 47, 133, 0)
     assert 0 < small <= large and large % 256 == 0
+    # A packed batch adds its tile list (one int per 4096 pixels plus two per image).
+    packed = fn(8, 41, 47, 133, 8 * 512 * 512)
+    assert packed >= large + 4 * (8 * 512 * 512 // 4096)
+    det = lib.pe_detection_histogram_workspace_bytes
+    det.restype = ctypes.c_int64
+    det.argtypes = [ctypes.c_int64] * 2
+    assert det(16, 0) == 0 and det(16, 16 * 800 * 1088) >= 4 * (16 * 800 * 1088 // 4096)

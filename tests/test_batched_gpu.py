@@ -1,0 +1,203 @@
+"""Batched form: packed batches of uneven images and HIP-graph-captured updates.
+
+A packed batch must give exactly the statistics of evaluating its images one
+at a time (integers bit-exact, FP64 within 1e-12 relative; the dataset iou_sum
+is exact, so it is identical), and every replay of a captured update must equal
+the eager update it recorded.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import assert_exact, compare_panoptic, to_numpy
+from oracle import combined as ocomb
+from oracle import panoptic as opan
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pe():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import panoptic_evaluator
+    return panoptic_evaluator
+
+
+def uneven(seed, B, sizes=(2, 5), segments=(3, 20), detections=(0, 12), odd=False):
+    import workloads
+    batch = workloads.config2(seed=seed, B=B, sizes=sizes, segments=segments, detections=detections)
+    if odd:
+        # Sizes that are not multiples of 4 or 16 (scalar head/tail paths).
+        for i, (h, w) in enumerate(batch.sizes):
+            h2, w2 = h - 3 - i % 5, w - 1 - i % 3
+            batch.sizes[i] = (h2, w2)
+            for name in ("gt", "pred"):
+                getattr(batch, name)[i] = getattr(batch, name)[i][:h2, :w2].contiguous()
+            batch.masks[i] = batch.masks[i][:, :h2, :w2].contiguous()
+            # Slots that lost every pixel become inactive metadata.
+            for maps, meta in ((batch.gt, batch.gc), (batch.pred, batch.pc)):
+                area = torch.bincount(maps[i].flatten().long(), minlength=meta[i].numel())
+                meta[i] = torch.where(area[:meta[i].numel()] > 0, meta[i], -1).int()
+                meta[i][0] = -1
+            batch.crowd[i] = batch.crowd[i] & (batch.gc[i] >= 0)
+    return batch
+
+
+def oracle_panoptic(batch, C, isthing):
+    state = opan.PanopticState(C, isthing)
+    for i in range(len(batch.sizes)):
+        state.update(batch.gt[i][None].numpy(), batch.pred[i][None].numpy(), batch.gc[i][None].numpy(),
+                     batch.pc[i][None].numpy(), batch.crowd[i][None].numpy(), validate=True)
+    return state.compute()
+
+
+def packed(pe, batch, device="cuda"):
+    cuda = lambda xs: [x.to(device) for x in xs]  # noqa: E731
+    target = pe.PanopticBatch.pack(cuda(batch.gt), cuda(batch.gc), cuda(batch.crowd))
+    prediction = pe.PanopticBatch.pack(cuda(batch.pred), cuda(batch.pc))
+    return prediction, target
+
+
+@pytest.mark.parametrize("odd", [False, True])
+def test_packed_panoptic_equals_per_image_oracle(pe, odd):
+    batch = uneven(21, 7, odd=odd)
+    isthing = [c < 80 for c in range(133)]
+    prediction, target = packed(pe, batch)
+    for validate in (True, False):
+        ev = pe.PanopticEvaluator(133, isthing=isthing, validate=validate)
+        ev.update(prediction, target)
+        compare_panoptic(to_numpy(ev.compute()), oracle_panoptic(batch, 133, isthing))
+
+
+def test_packed_panoptic_matches_plain_loop_exactly(pe):
+    batch = uneven(5, 9, sizes=(3, 9))
+    isthing = [c < 80 for c in range(133)]
+    prediction, target = packed(pe, batch)
+    batched = pe.PanopticEvaluator(133, isthing=isthing, validate=False)
+    batched.update(prediction, target)
+    plain = pe.PanopticEvaluator(133, isthing=isthing, validate=False)
+    for i in range(len(batch.sizes)):
+        plain.update(pe.PanopticBatch(batch.pred[i][None].cuda(), batch.pc[i][None].cuda()),
+                     pe.PanopticBatch(batch.gt[i][None].cuda(), batch.gc[i][None].cuda(),
+                                      batch.crowd[i][None].cuda()))
+    compare_panoptic(to_numpy(batched.compute()), to_numpy(plain.compute()), exact_iou_sum=True)
+
+
+def test_packed_panoptic_quality(pe):
+    batch = uneven(8, 5, odd=True)
+    prediction, target = packed(pe, batch)
+    got = pe.panoptic_quality(prediction, target, 133).cpu().numpy()
+    want = np.concatenate([opan.panoptic_quality(batch.gt[i][None].numpy(), batch.pred[i][None].numpy(),
+                                                 batch.gc[i][None].numpy(), batch.pc[i][None].numpy(),
+                                                 batch.crowd[i][None].numpy(), 133)
+                           for i in range(len(batch.sizes))])
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-15)
+
+
+def test_packed_validation_errors(pe):
+    batch = uneven(3, 4)
+    batch.gc[2] = batch.gc[2].clone()
+    batch.gc[2][1] = 500  # class outside 0..C-1
+    prediction, target = packed(pe, batch)
+    ev = pe.PanopticEvaluator(133)
+    with pytest.raises(ValueError, match="classes"):
+        ev.update(prediction, target)
+    assert ev.tp.sum().item() == 0
+
+
+@pytest.mark.parametrize("odd", [False, True])
+def test_packed_combined_equals_per_image_oracle(pe, odd):
+    batch = uneven(13, 6, detections=(0, 25), odd=odd)
+    isthing = [c < 80 for c in range(133)]
+    prediction, target = packed(pe, batch)
+    dets = pe.InstanceBatch([m.cuda() for m in batch.masks], [c.cuda() for c in batch.classes],
+                            [s.cuda() for s in batch.scores])
+    joint = pe.SegmentationEvaluator(133, isthing=isthing, validate=True)
+    joint.update(prediction, dets, target)
+    state = ocomb.SegmentationState(133, isthing)
+    for i in range(len(batch.sizes)):
+        state.update(batch.pred[i][None].numpy(), batch.pc[i][None].numpy(),
+                     [dict(masks=batch.masks[i].numpy(), classes=batch.classes[i].numpy(),
+                           scores=batch.scores[i].numpy())],
+                     batch.gt[i][None].numpy(), batch.gc[i][None].numpy(), batch.crowd[i][None].numpy())
+    result, expected = to_numpy(joint.compute()), state.compute()
+    compare_panoptic(result, expected)
+    for key in ("precision", "recall"):
+        assert_exact(result["instance"][key], expected["instance"][key], key)
+
+
+def _replay_matches_eager(pe, make, args, replays=3):
+    from panoptic_evaluator.graphs import CapturedUpdate
+    eager = make()
+    eager.update(*args)
+    want = to_numpy(eager.compute())
+    captured_ev = make()
+    captured = CapturedUpdate(captured_ev, *args)
+    # Capture leaves the state untouched.
+    assert to_numpy(captured_ev.compute())["tp"].sum() == 0
+    for _ in range(replays):
+        captured_ev.reset()
+        # eager work and a synchronisation between replays
+        scratch = torch.randn(1 << 20, device="cuda").sum()
+        torch.cuda.synchronize()
+        captured.replay()
+        got = to_numpy(captured_ev.compute())
+        assert scratch.isfinite()
+        yield got, want
+
+
+def test_graph_replays_equal_eager_panoptic(pe):
+    import workloads
+    pair = workloads.config0(B=3)
+    isthing = workloads.isthing_vector(133, 80)
+    args = (pe.PanopticBatch(pair.pred.cuda(), pair.pc.cuda()),
+            pe.PanopticBatch(pair.gt.cuda(), pair.gc.cuda(), pair.crowd.cuda()))
+    make = lambda: pe.PanopticEvaluator(133, isthing=isthing, validate=False)  # noqa: E731
+    for got, want in _replay_matches_eager(pe, make, args):
+        compare_panoptic(got, want, exact_iou_sum=True)
+
+
+def test_graph_replays_equal_eager_packed_combined(pe):
+    batch = uneven(17, 5, detections=(0, 30))
+    isthing = [c < 80 for c in range(133)]
+    prediction, target = packed(pe, batch)
+    dets = pe.InstanceBatch([m.cuda() for m in batch.masks], [c.cuda() for c in batch.classes],
+                            [s.cuda() for s in batch.scores])
+    make = lambda: pe.SegmentationEvaluator(133, isthing=isthing, validate=False)  # noqa: E731
+    for got, want in _replay_matches_eager(pe, make, (prediction, dets, target)):
+        compare_panoptic(got, want, exact_iou_sum=True)
+        for key in ("precision", "recall"):
+            assert_exact(got["instance"][key], want["instance"][key], key)
+
+
+def test_graph_replay_accumulates_like_repeated_updates(pe):
+    """Two replays without reset == two eager updates (records appended twice)."""
+    from panoptic_evaluator.graphs import CapturedUpdate
+    batch = uneven(19, 4, detections=(1, 20))
+    isthing = [c < 80 for c in range(133)]
+    prediction, target = packed(pe, batch)
+    dets = pe.InstanceBatch([m.cuda() for m in batch.masks], [c.cuda() for c in batch.classes],
+                            [s.cuda() for s in batch.scores])
+    eager = pe.SegmentationEvaluator(133, isthing=isthing, validate=False)
+    eager.update(prediction, dets, target)
+    eager.update(prediction, dets, target)
+    ev = pe.SegmentationEvaluator(133, isthing=isthing, validate=False)
+    captured = CapturedUpdate(ev, prediction, dets, target)
+    captured.replay()
+    captured.replay()
+    got, want = to_numpy(ev.compute()), to_numpy(eager.compute())
+    compare_panoptic(got, want, exact_iou_sum=True)
+    for key in ("precision", "recall"):
+        assert_exact(got["instance"][key], want["instance"][key], key)
+
+
+def test_capture_rejects_validation(pe):
+    from panoptic_evaluator.graphs import CapturedUpdate
+    import workloads
+    pair = workloads.config0(B=1)
+    ev = pe.PanopticEvaluator(133, validate=True)
+    with pytest.raises(ValueError, match="validate=False"):
+        CapturedUpdate(ev, pe.PanopticBatch(pair.pred.cuda(), pair.pc.cuda()),
+                       pe.PanopticBatch(pair.gt.cuda(), pair.gc.cuda()))
