@@ -4,11 +4,20 @@
 
 A step is one reset -> update -> compute pass of the public API over one batch
 of synthetic input already resident in HBM (the reference's timing protocol:
-DEVELOPMENT.md:47-49).  ``value`` is images/sec over exactly K timed steps.
-The dominant kernel's average duration is measured with HIP events recorded by
-the library on the kernel's own launch stream during the timed steps and
-reported against the MI355X HBM roofline; the CPU oracle is timed on a bounded
-sample of the same workload on the host cores as ``cpu_baseline``.
+DEVELOPMENT.md:47-49).  Two forms are timed over exactly K steps each:
+
+* plain   -- the drop-in API called eagerly: one update() per batch for the
+             uniform configs, one update() per image for the uneven batch
+             (the reference cannot batch uneven sizes, README.md:57-58);
+* batched -- the whole batch (packed when uneven) in one update() recorded as
+             a HIP graph and replayed, then compute().
+
+``value`` is the batched form's images/sec.  The dominant kernel's average
+duration is measured with HIP events recorded by the library on the kernel's
+own launch stream during K eager steps of the batched update (the same
+kernel and launch shape as in the graph) and reported against the MI355X HBM
+roofline; the CPU oracle is timed on a bounded sample of the same workload on
+the host as ``cpu_baseline``.
 
 Other modes: --copy-calibration runs a plain copy of known size (the reference
 point that scales rocprofv3 FETCH_SIZE/WRITE_SIZE into bytes).
@@ -39,7 +48,7 @@ def log(*args):
 # ------------------------------------------------------------------ configs
 
 class PanopticConfig:
-    """Plain PanopticEvaluator reset/update/compute over one batch."""
+    """PanopticEvaluator reset/update/compute over one uniform batch."""
 
     dominant = "joint_histogram"
     tuned = ("joint_histogram",)
@@ -64,9 +73,12 @@ class PanopticConfig:
                        "classes": self.classes, "gt_slots": int(self.pair.gc.shape[1]),
                        "pred_slots": int(self.pair.pc.shape[1]), "validate": False}
 
+    def update_args(self):
+        return self.prediction, self.target
+
     def step(self):
         self.evaluator.reset()
-        self.evaluator.update(self.prediction, self.target)
+        self.evaluator.update(*self.update_args())
         return self.evaluator.compute()
 
     def cpu_sample(self, budget_s):
@@ -87,7 +99,7 @@ class PanopticConfig:
 
 
 class CombinedConfig:
-    """Plain SegmentationEvaluator reset/update/compute over one batch: PQ,
+    """SegmentationEvaluator reset/update/compute over one uniform batch: PQ,
     mIoU and mask AP with the panoptic target's thing segments as AP ground
     truth (the metric's own workload)."""
 
@@ -117,9 +129,12 @@ class CombinedConfig:
                        "pred_slots": int(self.pair.pc.shape[1]), "validate": False,
                        "update_bytes_per_image": H * W * (8 + dets // B)}
 
+    def update_args(self):
+        return self.prediction, self.detections, self.target
+
     def step(self):
         self.evaluator.reset()
-        self.evaluator.update(self.prediction, self.detections, self.target)
+        self.evaluator.update(*self.update_args())
         return self.evaluator.compute()
 
     def cpu_sample(self, budget_s):
@@ -142,11 +157,84 @@ class CombinedConfig:
         return done / elapsed, f"{done} of {self.images} images, oracle SegmentationEvaluator update+compute"
 
 
+class UnevenConfig:
+    """SegmentationEvaluator over a batch of uneven images: plain = one
+    update() per image; batched = the packed batch in one captured update()."""
+
+    dominant = "detection_histogram"
+    tuned = ("detection_histogram", "joint_histogram", "packed_grid")
+    dtype = "u8 masks / int32 maps (int64 counts, f64 ratios)"
+
+    def __init__(self, name, description, **sizes):
+        self.name, self.description, self.sizes = name, description, sizes
+
+    def setup(self, device):
+        import workloads
+        import panoptic_evaluator as pe
+        u = self.batch = workloads.config2(device=device, **self.sizes)
+        self.isthing = workloads.isthing_vector(133, 80)
+        self.evaluator = pe.SegmentationEvaluator(133, isthing=self.isthing, validate=False)
+        self.target = pe.PanopticBatch.pack(u.gt, u.gc, u.crowd)
+        self.prediction = pe.PanopticBatch.pack(u.pred, u.pc)
+        self.detections = pe.InstanceBatch(u.masks, u.classes, u.scores)
+        self.per_image = [(pe.PanopticBatch(u.pred[i][None], u.pc[i][None]),
+                           pe.InstanceBatch([u.masks[i]], [u.classes[i]], [u.scores[i]]),
+                           pe.PanopticBatch(u.gt[i][None], u.gc[i][None], u.crowd[i][None]))
+                          for i in range(len(u.sizes))]
+        self.plain_evaluator = pe.SegmentationEvaluator(133, isthing=self.isthing, validate=False)
+        self.images = len(u.sizes)
+        pixels = [h * w for h, w in u.sizes]
+        dets = [m.shape[0] for m in u.masks]
+        self.bytes_per_launch = sum(p * (d + 4) for p, d in zip(pixels, dets))
+        self.config = {"workload": self.description, "images": self.images, "pixels": sum(pixels),
+                       "detections": sum(dets), "classes": 133,
+                       "max_gt_slots": int(self.target.classes.shape[1]),
+                       "max_pred_slots": int(self.prediction.classes.shape[1]), "validate": False,
+                       "update_bytes": sum(p * (d + 8) for p, d in zip(pixels, dets))}
+
+    def update_args(self):
+        return self.prediction, self.detections, self.target
+
+    def step(self):
+        # the drop-in API, one image at a time
+        ev = self.plain_evaluator
+        ev.reset()
+        for args in self.per_image:
+            ev.update(*args)
+        return ev.compute()
+
+    def eager_batched_step(self):
+        self.evaluator.reset()
+        self.evaluator.update(*self.update_args())
+        return self.evaluator.compute()
+
+    def cpu_sample(self, budget_s):
+        from oracle import combined as ocomb
+        u = self.batch
+        state = ocomb.SegmentationState(133, self.isthing, validate=False)
+        done, elapsed = 0, 0.0
+        while done < self.images and elapsed < budget_s:
+            i = done
+            arrays = [t.cpu().numpy()[None] for t in (u.gt[i], u.pred[i], u.gc[i], u.pc[i], u.crowd[i])]
+            det = [dict(masks=u.masks[i].cpu().numpy(), classes=u.classes[i].cpu().numpy(),
+                        scores=u.scores[i].cpu().numpy())]
+            t0 = time.perf_counter()
+            state.update(arrays[1], arrays[3], det, arrays[0], arrays[2], arrays[4])
+            elapsed += time.perf_counter() - t0
+            done += 1
+        t0 = time.perf_counter()
+        state.compute()
+        elapsed += time.perf_counter() - t0
+        return done / elapsed, f"{done} of {self.images} images, oracle SegmentationEvaluator update+compute"
+
+
 def make_configs():
     import workloads
     return {
         "c1": CombinedConfig("c1", "c1: SegmentationEvaluator, 16 x 800x1088, 133 classes, 80 Voronoi seeds, "
                                    "100 ellipse detections per image"),
+        "c2": UnevenConfig("c2", "c2: SegmentationEvaluator, 256 uneven images (H, W multiples of 32 in "
+                                 "[320, 1344]), 5..120 segments, 0..300 ellipse detections, 133 classes"),
         "c0": PanopticConfig("c0", workloads.config0, 133, 80,
                              "c0: PanopticEvaluator, 8 x 512x512, 133 classes, 40 Voronoi seeds"),
         "c3": PanopticConfig("c3", workloads.config3, 19, 8,
@@ -174,7 +262,7 @@ def copy_calibration():
 
 def traffic_for(config_name):
     """HBM bytes per launch of the dominant kernel from the committed
-    rocprofv3 counter summary (profiles/*traffic*.json), or None."""
+    rocprofv3 counter summary (profiles/traffic_<config>.json), or None."""
     path = ROOT / "profiles" / f"traffic_{config_name}.json"
     if not path.exists():
         return None
@@ -182,6 +270,16 @@ def traffic_for(config_name):
         return json.loads(path.read_text()).get("traffic_bytes_per_launch")
     except (OSError, ValueError):
         return None
+
+
+def timed(fn, steps):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
 
 
 def main(argv=None):
@@ -195,6 +293,7 @@ def main(argv=None):
     parser.add_argument("--copy-calibration", action="store_true")
     parser.add_argument("--no-autotune", action="store_true",
                         help="keep the default launch shapes instead of tuning them on this workload")
+    parser.add_argument("--plain-only", action="store_true", help="time only the plain eager form")
     args = parser.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the evaluation path shards by image with one "
@@ -212,14 +311,16 @@ def main(argv=None):
     log(f"[bench] setting up {cfg.name} on cuda:0")
     cfg.setup("cuda")
     torch.cuda.synchronize()
+    eager_batched = getattr(cfg, "eager_batched_step", cfg.step)
     for _ in range(args.warmup):
         cfg.step()
+        eager_batched()
     torch.cuda.synchronize()
     tuning_report = None
     if not args.no_autotune:
         from panoptic_evaluator import tuning
         log("[bench] autotuning launch shapes (untimed)")
-        report = tuning.autotune(cfg.step, cfg.tuned, repeats=3)
+        report = tuning.autotune(eager_batched, cfg.tuned, repeats=3)
         tuning_report = {kernel: {"variant": report["selection"][kernel],
                                   "describe": report["describe"][kernel][report["selection"][kernel]],
                                   "ms_per_variant": [None if ms is None else round(ms, 5)
@@ -227,22 +328,41 @@ def main(argv=None):
                          for kernel in cfg.tuned}
         for _ in range(args.warmup):
             cfg.step()
+            eager_batched()
         torch.cuda.synchronize()
     lib = _native.library()
+
+    log(f"[bench] timing {args.steps} plain steps")
+    plain_elapsed = timed(cfg.step, args.steps)
+    # The dominant kernel under the library's HIP events, in eager steps of
+    # the batched update (the kernels and launch shapes the graph replays).
     lib.pe_profile_reset()
     lib.pe_profile_enable(1)
-    log(f"[bench] timing {args.steps} steps")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        cfg.step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    profile_elapsed = timed(eager_batched, args.steps)
     lib.pe_profile_enable(0)
     kernel_ms, launches = _native.profile_read(cfg.dominant)
     avg_ms = kernel_ms / max(launches, 1)
     achieved = cfg.bytes_per_launch / (avg_ms * 1e-3) / 1e9 if launches else None
-    images_per_s = cfg.images * args.steps / elapsed
+
+    batched_elapsed = None
+    if not args.plain_only:
+        from panoptic_evaluator.graphs import CapturedUpdate
+        log("[bench] capturing the batched update as a HIP graph")
+        captured = CapturedUpdate(cfg.evaluator, *cfg.update_args())
+
+        def batched_step():
+            cfg.evaluator.reset()
+            captured.replay()
+            return cfg.evaluator.compute()
+
+        for _ in range(args.warmup):
+            batched_step()
+        log(f"[bench] timing {args.steps} batched steps")
+        batched_elapsed = timed(batched_step, args.steps)
+
+    plain_rate = cfg.images * args.steps / plain_elapsed
+    value_elapsed = batched_elapsed if batched_elapsed is not None else plain_elapsed
+    images_per_s = cfg.images * args.steps / value_elapsed
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU oracle sample")
@@ -250,6 +370,7 @@ def main(argv=None):
         cpu = {"value": round(rate, 4), "unit": "images/sec", "cores": 1, "kind": "port",
                "sample": sample, "host_threads_available": os.cpu_count()}
     traffic = traffic_for(cfg.name)
+    uneven = isinstance(cfg, UnevenConfig)
     line = {
         "metric": METRIC,
         "value": round(images_per_s, 3),
@@ -257,13 +378,17 @@ def main(argv=None):
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "ms_per_step": round(value_elapsed / args.steps * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": cfg.dtype,
         "data": "synthetic",
-        "config": cfg.config,
+        "config": dict(cfg.config, form="batched: one update() replayed as a HIP graph"
+                       if batched_elapsed is not None else "plain"),
+        "plain": {"value": round(plain_rate, 3), "ms_per_step": round(plain_elapsed / args.steps * 1e3, 4),
+                  "form": "drop-in API, eager, " + ("one update() per image" if uneven else "one update() per batch")},
+        "speedup_vs_plain": None if batched_elapsed is None else round(plain_elapsed / batched_elapsed, 3),
         "roofline": {
             "bound": "hbm",
             "kernel": cfg.dominant,
@@ -275,6 +400,8 @@ def main(argv=None):
             "algorithmic_bytes_per_launch": cfg.bytes_per_launch,
             "avg_kernel_ms": round(avg_ms, 5),
             "launches": launches,
+            "measured_over": f"{args.steps} eager steps of the batched update "
+                             f"({profile_elapsed * 1e3 / args.steps:.3f} ms/step)",
         },
         "cpu_baseline": cpu,
         "tuning": tuning_report,

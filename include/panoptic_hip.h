@@ -37,9 +37,10 @@ int pe_profile_reset(void);
 int pe_profile_read(const char* kernel, double* total_ms, int64_t* launches);
 
 /* Launch-shape tuning (no counterpart in the reference; used by the
- * autotuner of the batched form): each tunable kernel ("joint_histogram",
- * "detection_histogram", "mask_overlap_mfma") has pe_tune_count variants that
- * all compute identical results.  pe_tune_set selects one for every later
+ * autotuner of the batched form): each tuning knob ("joint_histogram",
+ * "detection_histogram", "mask_overlap_mfma": launch shapes of those kernels;
+ * "packed_grid": one workgroup per work item or a persistent grid for packed
+ * batches) has pe_tune_count variants that all compute identical results.  pe_tune_set selects one for every later
  * launch in the process (-1 restores the default); pe_tune_get returns the
  * variant in use and pe_tune_describe a short description of one. */
 int pe_tune_count(const char* kernel);

@@ -90,6 +90,14 @@ __device__ inline double wave_sum(double v) {
 
 __device__ inline int lane_id() { return __lane_id(); }
 
+// Global-address-space view of a pointer.  Pointers read from memory inside
+// a loop (per-image mask tables, offsets) are otherwise generic, and generic
+// (flat) loads also count against the LDS wait counter.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* as_global(const T* p) {
+  return (const __attribute__((address_space(1))) T*)(p);
+}
+
 // Zero-fill with 16-byte stores (graph-capture safe replacement for memset).
 __global__ void zero_words(uint4* data, i64 count16, uint32_t* tail, i64 tail_words);
 
@@ -149,7 +157,8 @@ struct ProfileScope {
 // results); the selected variant is used by every later launch in the
 // process.  Selection is process-wide and meant to be set once, before
 // launches are issued (the autotuner times each variant and keeps the best).
-enum TuneKernel { kTuneDetectionHistogram = 0, kTuneJointHistogram, kTuneMaskOverlap, kTuneKernels };
+enum TuneKernel { kTuneDetectionHistogram = 0, kTuneJointHistogram, kTuneMaskOverlap, kTunePackedGrid,
+                  kTuneKernels };
 struct TuneSpec {
   const char* name;
   int count;
@@ -158,5 +167,8 @@ struct TuneSpec {
 };
 const TuneSpec& tune_spec(TuneKernel kernel);
 int tuned_variant(TuneKernel kernel);
+// Workgroups per compute unit of the persistent grids over packed batches
+// (0: one workgroup per work item).
+int packed_grid_per_cu();
 
 }  // namespace pe

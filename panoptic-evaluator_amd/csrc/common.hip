@@ -227,12 +227,15 @@ const char* const kDetVariants[] = {"1 chunk x 4 masks", "1 chunk x 8 masks", "2
 const char* const kJointVariants[] = {"auto chunk", "8192 px", "16384 px", "32768 px", "65532 px"};
 const char* const kOverlapVariants[] = {"8 GT tiles x 16384 px", "4 GT tiles x 16384 px", "8 GT tiles x 8192 px",
                                         "8 GT tiles x 32768 px"};
+const char* const kGridVariants[] = {"one workgroup per work item", "persistent: 8 workgroups per CU",
+                                     "persistent: 4 workgroups per CU"};
 const TuneSpec kSpecs[kTuneKernels] = {
     {"detection_histogram", 5, 0, kDetVariants},
     {"joint_histogram", 5, 0, kJointVariants},
     {"mask_overlap_mfma", 4, 0, kOverlapVariants},
+    {"packed_grid", 3, 0, kGridVariants},
 };
-std::atomic<int> g_selected[kTuneKernels] = {{-1}, {-1}, {-1}};
+std::atomic<int> g_selected[kTuneKernels] = {{-1}, {-1}, {-1}, {-1}};
 
 int tune_index(const char* name) {
   if (!name) return -1;
@@ -243,6 +246,11 @@ int tune_index(const char* name) {
 }  // namespace
 
 const TuneSpec& tune_spec(TuneKernel kernel) { return kSpecs[kernel]; }
+
+int packed_grid_per_cu() {
+  static const int kPerCu[] = {0, 8, 4};
+  return kPerCu[tuned_variant(kTunePackedGrid)];
+}
 
 int tuned_variant(TuneKernel kernel) {
   const int v = g_selected[kernel].load(std::memory_order_relaxed);

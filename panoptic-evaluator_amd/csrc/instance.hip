@@ -188,74 +188,93 @@ __device__ __forceinline__ void count_chunk(v4i m, const int* slots, int S, int 
 
 // Masks are read exactly once: stream them past the caches.
 __device__ __forceinline__ v4i load16_stream(const uint8_t* row, i64 k, i64 HW, bool aligned) {
-  if (aligned && k + 16 <= HW) return __builtin_nontemporal_load(reinterpret_cast<const v4i*>(row + k));
-  return load16(row, k, HW, true, false);
+  const auto* g = as_global(row);
+  if (aligned && k + 16 <= HW) return __builtin_nontemporal_load(as_global(reinterpret_cast<const v4i*>(row + k)));
+  v4i v = {0, 0, 0, 0};
+  if (k >= HW) return v;
+  uint8_t bytes[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) bytes[j] = k + j < HW ? g[k + j] : 0;
+  __builtin_memcpy(&v, bytes, 16);
+  return v;
 }
 
 // CH = 16-pixel chunks per lane (tile = 16 * 256 * CH pixels); U = masks
-// whose loads are in flight together.
+// whose loads are in flight together.  One (tile, detection group) item.
 template <int CH, int U>
-__global__ __launch_bounds__(kDetThreads) void detection_histogram(DetHistArgs a) {
+__device__ __forceinline__ void detection_item(const DetHistArgs& a, int item, uint4* lds4) {
   constexpr int kTile = 16 * kDetThreads * CH;
-  extern __shared__ uint4 lds4[];
   uint32_t* bins = reinterpret_cast<uint32_t*>(lds4);
-  const int total = a.tiles.count(a.B) * a.dgroups;
+  const int tile_item = item / a.dgroups, dgroup = item - tile_item * a.dgroups;
+  const int b = __builtin_amdgcn_readfirstlane(a.tiles.image_of(tile_item));
+  const int tile = tile_item - a.tiles.first_tile(b);
+  const i64 first = a.det_off[b];
+  const int D = static_cast<int>(a.det_off[b + 1] - first);
+  const int d0 = dgroup * a.group;
+  const int nd = min(a.group, D - d0);
+  if (nd <= 0) return;  // uniform across the workgroup
   const bool aligned = a.aligned != 0;
-  for (int item = blockIdx.x; item < total; item += gridDim.x) {
-    const int tile_item = item / a.dgroups, dgroup = item - tile_item * a.dgroups;
-    const int b = a.tiles.image_of(tile_item);
-    const int tile = tile_item - a.tiles.first_tile(b);
-    const i64 first = a.det_off[b];
-    const int D = static_cast<int>(a.det_off[b + 1] - first);
-    const int d0 = dgroup * a.group;
-    const int nd = min(a.group, D - d0);
-    if (nd <= 0) continue;  // uniform across the workgroup
-    const int words = (nd * a.S + 1) / 2;
-    const int words4 = (words + 3) / 4;
-    for (int w = threadIdx.x; w < words4; w += kDetThreads) lds4[w] = make_uint4(0, 0, 0, 0);
-    __syncthreads();
-    const i64 HW = a.tiles.pixels(b);
-    const i64 p0 = static_cast<i64>(tile) * kTile;
-    const int32_t* target = a.target + a.tiles.begin(b);
-    const uint8_t* masks = a.masks[b];
-    const i64* order = a.order + first + d0;
-    int slots[CH][16];
-    i64 base[CH];
+  const int words = (nd * a.S + 1) / 2;
+  const int words4 = (words + 3) / 4;
+  for (int w = threadIdx.x; w < words4; w += kDetThreads) lds4[w] = make_uint4(0, 0, 0, 0);
+  __syncthreads();
+  const i64 HW = a.tiles.pixels(b);
+  const i64 p0 = static_cast<i64>(tile) * kTile;
+  const auto* target = as_global(a.target + a.tiles.begin(b));
+  const uint8_t* masks = a.masks[b];
+  const i64* order = a.order + first + d0;
+  int slots[CH][16];
+  i64 base[CH];
 #pragma unroll
-    for (int h = 0; h < CH; ++h) {
-      base[h] = p0 + (static_cast<i64>(h) * kDetThreads + threadIdx.x) * 16;
+  for (int h = 0; h < CH; ++h) {
+    base[h] = p0 + (static_cast<i64>(h) * kDetThreads + threadIdx.x) * 16;
 #pragma unroll
-      for (int j = 0; j < 16; ++j) slots[h][j] = base[h] + j < HW ? target[base[h] + j] : -1;
-    }
-    for (int r0 = 0; r0 < nd; r0 += U) {
-      v4i m[U][CH];
+    for (int j = 0; j < 16; ++j) slots[h][j] = base[h] + j < HW ? target[base[h] + j] : -1;
+  }
+  for (int r0 = 0; r0 < nd; r0 += U) {
+    v4i m[U][CH];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        if (r0 + u < nd) {
-          const uint8_t* mask = masks + order[r0 + u] * HW;
+    for (int u = 0; u < U; ++u) {
+      if (r0 + u < nd) {
+        const uint8_t* mask = masks + order[r0 + u] * HW;
 #pragma unroll
-          for (int h = 0; h < CH; ++h) m[u][h] = load16_stream(mask, base[h], HW, aligned);
-        } else {
+        for (int h = 0; h < CH; ++h) m[u][h] = load16_stream(mask, base[h], HW, aligned);
+      } else {
 #pragma unroll
-          for (int h = 0; h < CH; ++h) m[u][h] = v4i{0, 0, 0, 0};
-        }
+        for (int h = 0; h < CH; ++h) m[u][h] = v4i{0, 0, 0, 0};
       }
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-#pragma unroll
-        for (int h = 0; h < CH; ++h)
-          if ((m[u][h].x | m[u][h].y | m[u][h].z | m[u][h].w) != 0)
-            count_chunk(m[u][h], slots[h], a.S, (r0 + u) * a.S, bins);
     }
-    __syncthreads();
-    u64* out = a.pairs + (first + d0) * a.S;
-    for (int w = threadIdx.x; w < words; w += kDetThreads) {
-      const uint32_t v = bins[w];
-      if (!v) continue;
-      const uint32_t lo = v & 0xFFFFu, hi = v >> 16;
-      if (lo) atomicAdd(out + 2 * w, static_cast<u64>(lo));
-      if (hi) atomicAdd(out + 2 * w + 1, static_cast<u64>(hi));
-    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int h = 0; h < CH; ++h)
+        if ((m[u][h].x | m[u][h].y | m[u][h].z | m[u][h].w) != 0)
+          count_chunk(m[u][h], slots[h], a.S, (r0 + u) * a.S, bins);
+  }
+  __syncthreads();
+  u64* out = a.pairs + (first + d0) * a.S;
+  for (int w = threadIdx.x; w < words; w += kDetThreads) {
+    const uint32_t v = bins[w];
+    if (!v) continue;
+    const uint32_t lo = v & 0xFFFFu, hi = v >> 16;
+    if (lo) atomicAdd(out + 2 * w, static_cast<u64>(lo));
+    if (hi) atomicAdd(out + 2 * w + 1, static_cast<u64>(hi));
+  }
+}
+
+// One workgroup per item (grid >= item count; surplus workgroups of a packed
+// batch, whose count is only known on the device, exit at once), or a
+// persistent grid that walks the items with a grid stride.
+template <int CH, int U, bool PERSIST>
+__global__ __launch_bounds__(kDetThreads) void detection_histogram(DetHistArgs a) {
+  extern __shared__ uint4 lds4[];
+  const int total = a.tiles.count(a.B) * a.dgroups;
+  if (!PERSIST) {
+    if (static_cast<int>(blockIdx.x) < total) detection_item<CH, U>(a, blockIdx.x, lds4);
+    return;
+  }
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    detection_item<CH, U>(a, item, lds4);
     __syncthreads();  // the next item re-zeroes the bins
   }
 }
@@ -263,11 +282,15 @@ __global__ __launch_bounds__(kDetThreads) void detection_histogram(DetHistArgs a
 struct DetVariant {
   int chunks;
   void (*kernel)(DetHistArgs);
+  void (*persistent)(DetHistArgs);
 };
 // Order matches the tuning registry's descriptions.
 const DetVariant kDetHist[] = {
-    {1, detection_histogram<1, 4>}, {1, detection_histogram<1, 8>}, {2, detection_histogram<2, 2>},
-    {2, detection_histogram<2, 4>}, {4, detection_histogram<4, 2>},
+    {1, detection_histogram<1, 4, false>, detection_histogram<1, 4, true>},
+    {1, detection_histogram<1, 8, false>, detection_histogram<1, 8, true>},
+    {2, detection_histogram<2, 2, false>, detection_histogram<2, 2, true>},
+    {2, detection_histogram<2, 4, false>, detection_histogram<2, 4, true>},
+    {4, detection_histogram<4, 2, false>, detection_histogram<4, 2, true>},
 };
 
 // ------------------------------------------------------------ COCO matching
@@ -727,6 +750,7 @@ extern "C" int pe_detection_histogram(int64_t B, const uint8_t* const* masks, co
   TileList& t = a.tiles;
   t.tile = static_cast<int>(tile);
   i64 blocks;
+  bool persist = false;
   if (pix_off) {
     // HW is the pixel total of the packed batch.
     const i64 bound = tile_list_bound(B, HW, tile);
@@ -736,9 +760,14 @@ extern "C" int pe_detection_histogram(int64_t B, const uint8_t* const* masks, co
     t.tile_off = ws;
     t.tile_image = ws + B + 1;
     if ((rc = launch_tile_list(t.pix_off, a.B, t.tile, ws, ws + B + 1, stream))) return rc;
-    // Persistent grid over (tile, detection group) items: eight workgroups
-    // per compute unit walk the list with a grid stride.
-    blocks = std::min<i64>(bound * a.dgroups, 8LL * compute_units());
+    // The exact item count lives on the device: launch the bound, or a
+    // persistent grid of a few workgroups per compute unit.
+    blocks = bound * a.dgroups;
+    const int per_cu = packed_grid_per_cu();
+    if (per_cu > 0) {
+      blocks = std::min<i64>(blocks, static_cast<i64>(per_cu) * compute_units());
+      persist = true;
+    }
   } else {
     t.HW = HW;
     t.tiles_per_image = static_cast<int>((HW + tile - 1) / tile);
@@ -746,11 +775,12 @@ extern "C" int pe_detection_histogram(int64_t B, const uint8_t* const* masks, co
   }
   PE_REQUIRE(blocks < INT_MAX, "too many workgroups");
   const size_t lds = static_cast<size_t>((group * S + 1) / 2 + 3) / 4 * 16;
+  auto kernel = persist ? v.persistent : v.kernel;
   if (lds > 64 * 1024)
-    PE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(v.kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+    PE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
                                static_cast<int>(lds)));
   ProfileScope timing("detection_histogram", stream);
-  v.kernel<<<static_cast<unsigned>(blocks), kDetThreads, lds, stream>>>(a);
+  kernel<<<static_cast<unsigned>(blocks), kDetThreads, lds, stream>>>(a);
   PE_LAUNCH_CHECK();
   return kOk;
 }

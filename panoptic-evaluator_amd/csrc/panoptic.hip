@@ -105,9 +105,6 @@ struct KeyCache {
   }
 };
 
-// The pair table must be zero on entry (the scorer re-zeroes what it read).
-// Completion is published by the kernel boundary: a per-workgroup
-// device-scope fence would write back the XCD's L2 for every chunk.
 template <typename GT, typename PR, bool LOOKUP, bool VEC, bool GLOBAL>
 __device__ __forceinline__ void histogram_chunk(const HistArgs& a, int b, i64 begin, i64 end, uint32_t* bins) {
   const GT* gt = static_cast<const GT*>(a.gt);
@@ -173,47 +170,57 @@ __device__ __forceinline__ void histogram_chunk(const HistArgs& a, int b, i64 be
   if (bad && a.error) atomicOr(a.error, 1);
 }
 
-// The pair table must be zero on entry (the scorer re-zeroes what it read).
-// Completion is published by the kernel boundary: a per-workgroup
+// One chunk of one image: zero the LDS bins, count, flush to the image's
+// pair table.  The pair table must be zero on entry (the scorer re-zeroes what
+// it read).  Completion is published by the kernel boundary: a per-workgroup
 // device-scope fence would write back the XCD's L2 for every chunk.
-// Workgroups walk the chunk list with a grid stride: one chunk each for
-// uniform batches (grid = chunk count), several for packed batches (a
-// persistent grid of a few workgroups per compute unit).
 template <typename GT, typename PR, bool LOOKUP, bool VEC, bool GLOBAL>
-__global__ __launch_bounds__(kHistThreads) void joint_histogram(HistArgs a) {
-  extern __shared__ uint4 lds4[];
+__device__ __forceinline__ void histogram_item(const HistArgs& a, int item, uint4* lds4) {
   uint32_t* bins = reinterpret_cast<uint32_t*>(lds4);
   const int tid = threadIdx.x;
-  const int total = a.tiles.count(a.B);
-  for (int item = blockIdx.x; item < total; item += gridDim.x) {
-    const int b = a.tiles.image_of(item);
-    const int chunk = item - a.tiles.first_tile(b);
-    const i64 image = a.tiles.begin(b);
-    const i64 pixels = a.tiles.pixels(b);
-    const i64 begin = image + static_cast<i64>(chunk) * a.tiles.tile;
-    const i64 end = image + min(static_cast<i64>(chunk + 1) * a.tiles.tile, pixels);
-    if (!GLOBAL) {
-      for (int w = tid; w < a.words4; w += kHistThreads) lds4[w] = make_uint4(0, 0, 0, 0);
-      __syncthreads();
-    }
-    histogram_chunk<GT, PR, LOOKUP, VEC, GLOBAL>(a, b, begin, end, bins);
-    if (!GLOBAL) {
-      __syncthreads();
-      u64* pairs = a.pairs + static_cast<i64>(b) * a.G * a.P;
-      for (int w = tid; w < a.words4; w += kHistThreads) {
-        uint4 v = lds4[w];
-        uint32_t words[4] = {v.x, v.y, v.z, v.w};
+  const int b = __builtin_amdgcn_readfirstlane(a.tiles.image_of(item));
+  const int chunk = item - a.tiles.first_tile(b);
+  const i64 image = a.tiles.begin(b);
+  const i64 pixels = a.tiles.pixels(b);
+  const i64 begin = image + static_cast<i64>(chunk) * a.tiles.tile;
+  const i64 end = image + min(static_cast<i64>(chunk + 1) * a.tiles.tile, pixels);
+  if (!GLOBAL) {
+    for (int w = tid; w < a.words4; w += kHistThreads) lds4[w] = make_uint4(0, 0, 0, 0);
+    __syncthreads();
+  }
+  histogram_chunk<GT, PR, LOOKUP, VEC, GLOBAL>(a, b, begin, end, bins);
+  if (!GLOBAL) {
+    __syncthreads();
+    u64* pairs = a.pairs + static_cast<i64>(b) * a.G * a.P;
+    for (int w = tid; w < a.words4; w += kHistThreads) {
+      uint4 v = lds4[w];
+      uint32_t words[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          if (!words[j]) continue;
-          uint32_t key = (static_cast<uint32_t>(w) * 4u + j) * 2u;
-          uint32_t lo = words[j] & 0xFFFFu, hi = words[j] >> 16;
-          if (lo) atomicAdd(pairs + key, static_cast<u64>(lo));
-          if (hi) atomicAdd(pairs + key + 1, static_cast<u64>(hi));
-        }
+      for (int j = 0; j < 4; ++j) {
+        if (!words[j]) continue;
+        uint32_t key = (static_cast<uint32_t>(w) * 4u + j) * 2u;
+        uint32_t lo = words[j] & 0xFFFFu, hi = words[j] >> 16;
+        if (lo) atomicAdd(pairs + key, static_cast<u64>(lo));
+        if (hi) atomicAdd(pairs + key + 1, static_cast<u64>(hi));
       }
-      __syncthreads();  // the next chunk re-zeroes the bins
     }
+  }
+}
+
+// One workgroup per chunk (surplus workgroups of a packed batch, whose chunk
+// count is only known on the device, exit at once), or a persistent grid
+// that walks the chunk list with a grid stride.
+template <typename GT, typename PR, bool LOOKUP, bool VEC, bool GLOBAL, bool PERSIST>
+__global__ __launch_bounds__(kHistThreads) void joint_histogram(HistArgs a) {
+  extern __shared__ uint4 lds4[];
+  const int total = a.tiles.count(a.B);
+  if (!PERSIST) {
+    if (static_cast<int>(blockIdx.x) < total) histogram_item<GT, PR, LOOKUP, VEC, GLOBAL>(a, blockIdx.x, lds4);
+    return;
+  }
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    histogram_item<GT, PR, LOOKUP, VEC, GLOBAL>(a, item, lds4);
+    __syncthreads();  // the next chunk re-zeroes the bins
   }
 }
 
@@ -226,7 +233,8 @@ static bool allow_lds(K kernel, size_t bytes) {
 
 template <typename GT, typename PR, bool LOOKUP, bool VEC, bool GLOBAL>
 static int launch_hist_kernel(const HistArgs& a, int blocks, size_t lds, hipStream_t stream) {
-  auto kernel = joint_histogram<GT, PR, LOOKUP, VEC, GLOBAL>;
+  auto kernel = a.persist ? joint_histogram<GT, PR, LOOKUP, VEC, GLOBAL, true>
+                          : joint_histogram<GT, PR, LOOKUP, VEC, GLOBAL, false>;
   if (!allow_lds(kernel, lds)) return fail(kUnsupportedShape, "LDS request of %zu bytes refused", lds);
   ProfileScope timing("joint_histogram", stream);
   kernel<<<blocks, kHistThreads, lds, stream>>>(a);
@@ -277,8 +285,14 @@ int launch_joint_histogram(HistArgs a, int gt_bytes, int pred_bytes, int chunk, 
     t.tile_image = tile_ws + a.B + 1;
     int rc = launch_tile_list(a.pix_off, a.B, chunk, tile_ws, tile_ws + a.B + 1, stream);
     if (rc) return rc;
-    // Persistent grid: the exact chunk count lives on the device.
-    blocks = std::min<i64>(tile_list_bound(a.B, total_pixels, chunk), 8LL * compute_units());
+    // The exact chunk count lives on the device: launch the bound, or a
+    // persistent grid of a few workgroups per compute unit.
+    blocks = tile_list_bound(a.B, total_pixels, chunk);
+    const int per_cu = packed_grid_per_cu();
+    if (per_cu > 0) {
+      blocks = std::min<i64>(blocks, static_cast<i64>(per_cu) * compute_units());
+      a.persist = true;
+    }
   } else {
     t.HW = a.HW;
     t.tiles_per_image = static_cast<int>((a.HW + chunk - 1) / chunk);

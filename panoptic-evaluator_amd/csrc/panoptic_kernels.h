@@ -20,6 +20,7 @@ struct HistArgs {
   const i64* pix_off;    // B + 1 pixel offsets of a packed batch, or nullptr
   int G, P;
   TileList tiles;        // chunks of <= kMaxChunk pixels, each inside one image
+  bool persist;          // grid-stride walk of the chunk list
   int words4;            // LDS size in 16-byte units
   u64* pairs;            // B x G x P
   const i64* gt_ids;

@@ -1,9 +1,10 @@
 """Launch-shape autotuner for the streaming kernels.
 
-Each tunable kernel of libpanoptic_hip.so ("joint_histogram",
-"detection_histogram", "mask_overlap_mfma") has a fixed list of variants --
-pixel tile per workgroup, masks in flight per lane, GT tiles and pixel chunk
-per MFMA wave -- that compute identical results.  ``autotune`` times every
+Each tuning knob of libpanoptic_hip.so ("joint_histogram",
+"detection_histogram", "mask_overlap_mfma", "packed_grid") has a fixed list of
+variants -- pixel tile per workgroup, masks in flight per lane, GT tiles and
+pixel chunk per MFMA wave, persistent grid size over packed batches -- that
+compute identical results.  ``autotune`` times every
 variant on the caller's own workload with HIP events recorded around the
 kernel's launches and keeps the fastest; the selection is process-wide.
 Choices can be saved as JSON and applied again with ``apply``.
@@ -19,7 +20,11 @@ import torch
 
 from . import _native
 
-KERNELS = ("joint_histogram", "detection_histogram", "mask_overlap_mfma")
+KERNELS = ("joint_histogram", "detection_histogram", "mask_overlap_mfma", "packed_grid")
+
+# Kernels whose time decides each knob ("packed_grid" shapes the persistent
+# grids of both histograms over packed batches of uneven images).
+MEASURED = {"packed_grid": ("joint_histogram", "detection_histogram")}
 
 
 def variants(kernel: str) -> list[str]:
@@ -57,7 +62,9 @@ def save(path, report: dict) -> None:
     pathlib.Path(path).write_text(json.dumps(report, indent=1, sort_keys=True))
 
 
-def _kernel_ms(step: Callable[[], object], kernel: str, repeats: int):
+def _kernel_ms(step: Callable[[], object], knob: str, repeats: int):
+    """Milliseconds per step spent in the kernels a knob shapes, and their
+    launch count (0: the step launches none of them)."""
     lib = _native.library()
     torch.cuda.synchronize()
     lib.pe_profile_reset()
@@ -68,17 +75,20 @@ def _kernel_ms(step: Callable[[], object], kernel: str, repeats: int):
         torch.cuda.synchronize()
     finally:
         lib.pe_profile_enable(0)
-    total, launches = _native.profile_read(kernel)
-    return (total / launches if launches else None), launches
+    total, launches = 0.0, 0
+    for kernel in MEASURED.get(knob, (knob,)):
+        ms, count = _native.profile_read(kernel)
+        total, launches = total + ms, launches + count
+    return (total / repeats if launches else None), launches
 
 
 def autotune(step: Callable[[], object], kernels: Iterable[str] = KERNELS, *, repeats: int = 5,
              warmup: int = 1) -> dict:
-    """Time each variant of each kernel launched by ``step`` and keep the
-    fastest.  Kernels that ``step`` never launches keep their selection.
+    """Time each variant of each knob on ``step`` and keep the fastest.
+    Knobs whose kernels ``step`` never launches keep their selection.
 
-    Returns {"selection": {kernel: variant}, "timings": {kernel: [ms per
-    launch or None, ...]}, "describe": {kernel: [...]}}."""
+    Returns {"selection": {knob: variant}, "timings": {knob: [ms per step in
+    the knob's kernels, or None, ...]}, "describe": {knob: [...]}}."""
     report = {"selection": {}, "timings": {}, "describe": {}}
     for kernel in kernels:
         names = variants(kernel)

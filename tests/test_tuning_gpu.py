@@ -119,3 +119,31 @@ def test_autotune_keeps_results(pe, tuning):
     assert report["timings"]["detection_histogram"] == [None]
     step()
     compare_panoptic(to_numpy(ev.compute()), before, exact_iou_sum=True)
+
+
+def test_packed_grid_variants(pe, tuning):
+    """Persistent and one-workgroup-per-item grids over a packed batch."""
+    import workloads
+    batch = workloads.config2(seed=9, B=5, sizes=(2, 6), segments=(3, 25), detections=(0, 40))
+    isthing = workloads.isthing_vector(133, 80)
+    state = ocomb.SegmentationState(133, isthing)
+    for i in range(len(batch.sizes)):
+        state.update(batch.pred[i][None].numpy(), batch.pc[i][None].numpy(),
+                     [dict(masks=batch.masks[i].numpy(), classes=batch.classes[i].numpy(),
+                           scores=batch.scores[i].numpy())],
+                     batch.gt[i][None].numpy(), batch.gc[i][None].numpy(), batch.crowd[i][None].numpy())
+    expected = state.compute()
+    cuda = lambda xs: [x.cuda() for x in xs]  # noqa: E731
+    target = pe.PanopticBatch.pack(cuda(batch.gt), cuda(batch.gc), cuda(batch.crowd))
+    prediction = pe.PanopticBatch.pack(cuda(batch.pred), cuda(batch.pc))
+    dets = pe.InstanceBatch(cuda(batch.masks), cuda(batch.classes), cuda(batch.scores))
+    for grid in range(count(tuning, "packed_grid")):
+        for det in (0, count(tuning, "detection_histogram") - 1):
+            tuning.select("packed_grid", grid)
+            tuning.select("detection_histogram", det)
+            joint = pe.SegmentationEvaluator(133, isthing=isthing)
+            joint.update(prediction, dets, target)
+            result = to_numpy(joint.compute())
+            compare_panoptic(result, expected)
+            for key in ("precision", "recall"):
+                assert_exact(result["instance"][key], expected["instance"][key], f"{key} grid {grid}")
