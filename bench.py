@@ -104,7 +104,7 @@ class CombinedConfig:
     truth (the metric's own workload)."""
 
     dominant = "detection_histogram"
-    tuned = ("detection_histogram", "joint_histogram")
+    tuned = ("detection_group", "detection_histogram", "joint_histogram")
     dtype = "u8 masks / int32 maps (int64 counts, f64 ratios)"
 
     def __init__(self, name, description, **sizes):
@@ -162,7 +162,7 @@ class UnevenConfig:
     update() per image; batched = the packed batch in one captured update()."""
 
     dominant = "detection_histogram"
-    tuned = ("detection_histogram", "joint_histogram", "packed_grid")
+    tuned = ("detection_group", "detection_histogram", "joint_histogram", "packed_grid")
     dtype = "u8 masks / int32 maps (int64 counts, f64 ratios)"
 
     def __init__(self, name, description, **sizes):

@@ -40,7 +40,8 @@ int pe_profile_read(const char* kernel, double* total_ms, int64_t* launches);
  * autotuner of the batched form): each tuning knob ("joint_histogram",
  * "detection_histogram", "mask_overlap_mfma": launch shapes of those kernels;
  * "packed_grid": one workgroup per work item or a persistent grid for packed
- * batches) has pe_tune_count variants that all compute identical results.  pe_tune_set selects one for every later
+ * batches; "detection_group": detections per detection-histogram workgroup)
+ * has pe_tune_count variants that all compute identical results.  pe_tune_set selects one for every later
  * launch in the process (-1 restores the default); pe_tune_get returns the
  * variant in use and pe_tune_describe a short description of one. */
 int pe_tune_count(const char* kernel);
@@ -227,11 +228,16 @@ int pe_instance_checks(int64_t n_det, const int64_t* det_cls, const double* scor
 /* -------------------------------------------------------------- prefix ---- */
 
 /* Scratch bytes of pe_prefix: sum_n = total masks, sum_tri = sum over images
- * of N_b (N_b + 1) / 2. */
-int64_t pe_prefix_workspace_bytes(int64_t B, int64_t G, int64_t C, int64_t T, int64_t sum_n, int64_t sum_tri);
+ * of N_b (N_b + 1) / 2, packed_pixels = pixel total of a packed batch (0 for
+ * images of one size). */
+int64_t pe_prefix_workspace_bytes(int64_t B, int64_t G, int64_t C, int64_t T, int64_t sum_n, int64_t sum_tri,
+                                  int64_t packed_pixels);
 
-/* Generation-order prefix rewards for B images sharing one target size.
- * target: B x HW int32 slots; masks: device array of B pointers to N_b x HW
+/* Generation-order prefix rewards for B images.  pix_off NULL: the images
+ * share one size and target is B x HW int32 slots; otherwise B + 1 device
+ * pixel offsets of a packed batch of uneven images (target holds them one
+ * after another, HW is the pixel total, mask b is N_b x pixels_b; no
+ * counterpart in the reference).  masks: device array of B pointers to N_b x HW
  * bool masks in generation order (n_off: B + 1 device offsets; cls: sum N
  * int32 classes); gc/crowd: B x G target metadata; things: C uint8;
  * thresholds: T f64 IoU thresholds; k_off (B + 1) / points (sum K int32):
@@ -244,7 +250,7 @@ int64_t pe_prefix_workspace_bytes(int64_t B, int64_t G, int64_t C, int64_t T, in
  * metadata and pixels disagree, 8 generated class outside 0..C-1, 16 empty
  * mask.  Replaces ``prefix`` (bindings.cpp:24-26; prefix_cuda,
  * prefix_kernel.cu:142-199; CPU segmentation_prefixes, prefix.py:11-122). */
-int pe_prefix(int64_t B, const int32_t* target, int64_t HW, int64_t G, int64_t C,
+int pe_prefix(int64_t B, const int32_t* target, const int64_t* pix_off, int64_t HW, int64_t G, int64_t C,
               const uint8_t* const* masks, const int64_t* n_off, int64_t sum_n, int64_t sum_tri,
               const int32_t* cls, const int32_t* gc, const uint8_t* crowd, const uint8_t* things,
               const double* thresholds, int64_t T, const int64_t* k_off, const int32_t* points,

@@ -158,7 +158,7 @@ struct ProfileScope {
 // process.  Selection is process-wide and meant to be set once, before
 // launches are issued (the autotuner times each variant and keeps the best).
 enum TuneKernel { kTuneDetectionHistogram = 0, kTuneJointHistogram, kTuneMaskOverlap, kTunePackedGrid,
-                  kTuneKernels };
+                  kTuneDetectionGroup, kTuneKernels };
 struct TuneSpec {
   const char* name;
   int count;
@@ -170,5 +170,8 @@ int tuned_variant(TuneKernel kernel);
 // Workgroups per compute unit of the persistent grids over packed batches
 // (0: one workgroup per work item).
 int packed_grid_per_cu();
+// Detections per workgroup of the detection histogram (0: as many as the LDS
+// budget holds); more groups re-read the target tile but need less LDS.
+int detection_group_limit();
 
 }  // namespace pe

@@ -229,13 +229,15 @@ const char* const kOverlapVariants[] = {"8 GT tiles x 16384 px", "4 GT tiles x 1
                                         "8 GT tiles x 32768 px"};
 const char* const kGridVariants[] = {"one workgroup per work item", "persistent: 8 workgroups per CU",
                                      "persistent: 4 workgroups per CU"};
+const char* const kGroupVariants[] = {"LDS budget (96 KB)", "128 detections", "64 detections", "32 detections"};
 const TuneSpec kSpecs[kTuneKernels] = {
     {"detection_histogram", 5, 0, kDetVariants},
     {"joint_histogram", 5, 0, kJointVariants},
     {"mask_overlap_mfma", 4, 0, kOverlapVariants},
     {"packed_grid", 3, 0, kGridVariants},
+    {"detection_group", 4, 0, kGroupVariants},
 };
-std::atomic<int> g_selected[kTuneKernels] = {{-1}, {-1}, {-1}, {-1}};
+std::atomic<int> g_selected[kTuneKernels] = {{-1}, {-1}, {-1}, {-1}, {-1}};
 
 int tune_index(const char* name) {
   if (!name) return -1;
@@ -250,6 +252,11 @@ const TuneSpec& tune_spec(TuneKernel kernel) { return kSpecs[kernel]; }
 int packed_grid_per_cu() {
   static const int kPerCu[] = {0, 8, 4};
   return kPerCu[tuned_variant(kTunePackedGrid)];
+}
+
+int detection_group_limit() {
+  static const int kGroup[] = {0, 128, 64, 32};
+  return kGroup[tuned_variant(kTuneDetectionGroup)];
 }
 
 int tuned_variant(TuneKernel kernel) {

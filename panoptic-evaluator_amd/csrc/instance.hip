@@ -742,6 +742,7 @@ extern "C" int pe_detection_histogram(int64_t B, const uint8_t* const* masks, co
   // budget (a tile holds at most 16384 pixels, so no counter exceeds 0xFFFF).
   i64 group = static_cast<i64>(kDetMaxLds * 2 / 4) / S;  // bins per group / S
   PE_REQUIRE(group >= 1, "too many target slots for the detection histogram");
+  if (detection_group_limit() > 0) group = std::min<i64>(group, detection_group_limit());
   group = std::min<i64>(group, max_dets);
   a.group = static_cast<int>(group);
   a.dgroups = static_cast<int>((max_dets + group - 1) / group);

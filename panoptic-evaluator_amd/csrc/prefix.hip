@@ -32,8 +32,8 @@ constexpr int kPrefixTile = 16 * kPrefixThreads * 2;
 
 struct PrefixArgs {
   int B, G, C, T;
-  i64 HW;
-  const int32_t* target;        // B x HW
+  TileList tiles;               // pixel tiles, each inside one image
+  const int32_t* target;        // every image's slots, one after another
   const uint8_t* const* masks;  // B pointers (N_b x HW)
   const i64* n_off;             // B + 1: mask offsets
   const int32_t* cls;           // sum N: classes in generation order
@@ -62,17 +62,17 @@ struct PrefixArgs {
   i64* tp; i64* fp; i64* fn; double* sums; i64* confusion;
   i64* out_itp; i64* out_ifp; i64* out_ifn; double* precision; double* recall;
   int32_t* error;
-  int tiles;
   int aligned;
 };
 
 __device__ __forceinline__ v4i load16(const uint8_t* row, i64 k, i64 HW, bool aligned) {
   v4i v = {0, 0, 0, 0};
   if (k >= HW) return v;
-  if (aligned && k + 16 <= HW) return *reinterpret_cast<const v4i*>(row + k);
+  const auto* g = as_global(row);
+  if (aligned && k + 16 <= HW) return __builtin_nontemporal_load(as_global(reinterpret_cast<const v4i*>(row + k)));
   uint8_t bytes[16];
 #pragma unroll
-  for (int j = 0; j < 16; ++j) bytes[j] = k + j < HW ? row[k + j] : 0;
+  for (int j = 0; j < 16; ++j) bytes[j] = k + j < HW ? g[k + j] : 0;
   __builtin_memcpy(&v, bytes, 16);
   return v;
 }
@@ -85,16 +85,26 @@ __device__ __forceinline__ i64 dec_base(const PrefixArgs& a, int b) {
   return total * a.G;
 }
 
+// Masks whose loads are in flight together (processing stays in generation
+// order: only the loads run ahead).
+constexpr int kPrefixAhead = 4;
+
+// One workgroup per pixel tile of one image (grid >= tile count; surplus
+// workgroups of a packed batch, whose count is only known on the device, exit
+// at once).
 __global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
   extern __shared__ uint4 lds4[];
   uint32_t* ga = reinterpret_cast<uint32_t*>(lds4);
-  const int b = blockIdx.x / a.tiles, tile = blockIdx.x - (blockIdx.x / a.tiles) * a.tiles;
+  if (static_cast<int>(blockIdx.x) >= a.tiles.count(a.B)) return;
+  const int b = __builtin_amdgcn_readfirstlane(a.tiles.image_of(blockIdx.x));
+  const int tile = blockIdx.x - a.tiles.first_tile(b);
+  const i64 HW = a.tiles.pixels(b);
   const int G = a.G;
   for (int g = threadIdx.x; g < G; g += kPrefixThreads) ga[g] = 0;
   __syncthreads();
   const i64 n0 = a.n_off[b];
   const int N = static_cast<int>(a.n_off[b + 1] - n0);
-  const int32_t* target = a.target + static_cast<i64>(b) * a.HW;
+  const auto* target = as_global(a.target + a.tiles.begin(b));
   int slots[2][16], top[2][16];
   i64 base[2];
   bool bad = false;
@@ -105,7 +115,7 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       int s = -1;
-      if (base[h] + j < a.HW) {
+      if (base[h] + j < HW) {
         s = target[base[h] + j];
         if (s < 0 || s >= G) {
           bad = true;
@@ -128,11 +138,25 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
   int32_t* raw = a.raw + n0 * G;
   int32_t* dec = a.dec + dec_base(a, b);
   const int P1 = N + 1;
-  for (int i = 0; i < N; ++i) {
-    const uint8_t* mask = masks + static_cast<i64>(i) * a.HW;
-    v4i m[2];
+  const bool aligned = a.aligned != 0;
+  for (int i0 = 0; i0 < N; i0 += kPrefixAhead) {
+    v4i ahead[kPrefixAhead][2];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) m[h] = load16(mask, base[h], a.HW, a.aligned != 0);
+    for (int u = 0; u < kPrefixAhead; ++u) {
+      if (i0 + u < N) {
+        const uint8_t* mask = masks + static_cast<i64>(i0 + u) * HW;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) ahead[u][h] = load16(mask, base[h], HW, aligned);
+      } else {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) ahead[u][h] = v4i{0, 0, 0, 0};
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kPrefixAhead; ++u) {
+    const int i = i0 + u;
+    if (i >= N) break;
+    const v4i* m = ahead[u];
     int32_t* dec_i = dec + static_cast<i64>(G) * tri(i);
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -165,6 +189,7 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
         atomicAdd(raw + static_cast<i64>(i) * G + rg, run);
         atomicAdd(dec_i + static_cast<i64>(rg) * (i + 1) + prev, run);
       }
+    }
     }
   }
   __syncthreads();
@@ -380,11 +405,12 @@ struct PrefixScratch {
   i64* positives;
   int32_t* lab;
   double* val;
+  int* tile_list;     // packed batches: tile_off then tile_image
   size_t zero_bytes;  // raw + dec + ga, contiguous at the start
   size_t used_bytes;
 };
 
-PrefixScratch carve_prefix(void* base, i64 B, i64 G, i64 C, i64 T, i64 sum_n, i64 sum_tri) {
+PrefixScratch carve_prefix(void* base, i64 B, i64 G, i64 C, i64 T, i64 sum_n, i64 sum_tri, i64 packed_pixels) {
   Carve carve(base);
   PrefixScratch s{};
   s.raw = carve.take<int32_t>(sum_n * G);
@@ -400,17 +426,19 @@ PrefixScratch carve_prefix(void* base, i64 B, i64 G, i64 C, i64 T, i64 sum_n, i6
   s.itp = carve.take<i64>(B * T * C);
   s.ifp = carve.take<i64>(B * T * C);
   s.positives = carve.take<i64>(B * C);
+  s.tile_list = packed_pixels > 0 ? carve.take<int>(tile_list_ints(B, packed_pixels, kPrefixTile)) : nullptr;
   s.used_bytes = carve.used;
   return s;
 }
 }  // namespace
 
 extern "C" int64_t pe_prefix_workspace_bytes(int64_t B, int64_t G, int64_t C, int64_t T, int64_t sum_n,
-                                             int64_t sum_tri) {
-  return static_cast<int64_t>(align_up(carve_prefix(nullptr, B, G, C, T, sum_n, sum_tri).used_bytes, 256));
+                                             int64_t sum_tri, int64_t packed_pixels) {
+  return static_cast<int64_t>(
+      align_up(carve_prefix(nullptr, B, G, C, T, sum_n, sum_tri, packed_pixels).used_bytes, 256));
 }
 
-extern "C" int pe_prefix(int64_t B, const int32_t* target, int64_t HW, int64_t G, int64_t C,
+extern "C" int pe_prefix(int64_t B, const int32_t* target, const int64_t* pix_off, int64_t HW, int64_t G, int64_t C,
                          const uint8_t* const* masks, const int64_t* n_off, int64_t sum_n, int64_t sum_tri,
                          const int32_t* cls, const int32_t* gc, const uint8_t* crowd, const uint8_t* things,
                          const double* thresholds, int64_t T, const int64_t* k_off, const int32_t* points,
@@ -421,7 +449,9 @@ extern "C" int pe_prefix(int64_t B, const int32_t* target, int64_t HW, int64_t G
   PE_REQUIRE(B >= 0 && HW > 0 && G > 0 && G < INT_MAX && C > 0 && C < INT_MAX && T > 0 && T * C < INT_MAX,
              "prefix dimensions outside supported range");
   PE_REQUIRE(sum_n >= 0 && sum_tri >= 0 && total_rows >= 0, "invalid prefix sizes");
-  PE_REQUIRE(workspace_bytes >= pe_prefix_workspace_bytes(B, G, C, T, sum_n, sum_tri), "workspace too small");
+  const i64 packed_pixels = pix_off ? HW : 0;
+  PE_REQUIRE(workspace_bytes >= pe_prefix_workspace_bytes(B, G, C, T, sum_n, sum_tri, packed_pixels),
+             "workspace too small");
   hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
   int rc;
   if ((rc = launch_zero(error, 4, stream))) return rc;
@@ -433,14 +463,13 @@ extern "C" int pe_prefix(int64_t B, const int32_t* target, int64_t HW, int64_t G
     if ((rc = launch_zero(confusion, 8 * rows * C * (C + 1), stream))) return rc;
   }
   if (B == 0) return kOk;
-  PrefixScratch s = carve_prefix(workspace, B, G, C, T, sum_n, sum_tri);
+  PrefixScratch s = carve_prefix(workspace, B, G, C, T, sum_n, sum_tri, packed_pixels);
   if ((rc = launch_zero(workspace, s.zero_bytes, stream))) return rc;
   PrefixArgs a{};
   a.B = static_cast<int>(B);
   a.G = static_cast<int>(G);
   a.C = static_cast<int>(C);
   a.T = static_cast<int>(T);
-  a.HW = HW;
   a.target = target;
   a.masks = masks;
   a.n_off = reinterpret_cast<const i64*>(n_off);
@@ -474,9 +503,22 @@ extern "C" int pe_prefix(int64_t B, const int32_t* target, int64_t HW, int64_t G
   a.precision = precision;
   a.recall = recall;
   a.error = error;
-  a.tiles = static_cast<int>((HW + kPrefixTile - 1) / kPrefixTile);
   a.aligned = aligned;
-  const i64 blocks = B * a.tiles;
+  TileList& t = a.tiles;
+  t.tile = kPrefixTile;
+  i64 blocks;
+  if (pix_off) {
+    // HW is the pixel total of the packed batch.
+    t.pix_off = reinterpret_cast<const i64*>(pix_off);
+    t.tile_off = s.tile_list;
+    t.tile_image = s.tile_list + B + 1;
+    if ((rc = launch_tile_list(t.pix_off, a.B, kPrefixTile, s.tile_list, s.tile_list + B + 1, stream))) return rc;
+    blocks = tile_list_bound(B, HW, kPrefixTile);
+  } else {
+    t.HW = HW;
+    t.tiles_per_image = static_cast<int>((HW + kPrefixTile - 1) / kPrefixTile);
+    blocks = B * t.tiles_per_image;
+  }
   PE_REQUIRE(blocks < INT_MAX, "too many workgroups");
   const size_t lds = 4 * static_cast<size_t>(G) + 16;
   PE_REQUIRE(lds <= 64 * 1024, "too many target slots for the prefix kernel");

@@ -18,8 +18,8 @@ from .evaluator import MSG_ACTIVE, MSG_CLASSES, _prepare, finalize
 from .instance import InstanceBatch, _cat, offsets
 
 _i64, _int, _ptr = ctypes.c_int64, ctypes.c_int, ctypes.c_void_p
-_native.register("pe_prefix_workspace_bytes", _i64, [_i64] * 6)
-_native.register("pe_prefix", _int, [_i64, _ptr, _i64, _i64, _i64, _ptr, _ptr, _i64, _i64, _ptr, _ptr, _ptr, _ptr,
+_native.register("pe_prefix_workspace_bytes", _i64, [_i64] * 7)
+_native.register("pe_prefix", _int, [_i64, _ptr, _ptr, _i64, _i64, _i64, _ptr, _ptr, _i64, _i64, _ptr, _ptr, _ptr, _ptr,
                                      _ptr, _i64, _ptr, _ptr, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr,
                                      _ptr, _ptr, _ptr, _ptr, _i64, _int, _ptr])
 
@@ -81,12 +81,13 @@ def segmentation_prefixes(prediction: InstanceBatch, target, num_classes: int, *
             raise ValueError("classes must be int32/int64 N tensors")
         if masks.device != device or classes.device != device:
             raise ValueError("inputs must share evaluator device")
-    if len(masks_list) != len(maps) or any(m.shape[1:] != maps.shape[1:] for m in masks_list):
+    sizes = target.image_sizes if target.packed else [tuple(maps.shape[1:])] * maps.shape[0]
+    if len(masks_list) != len(sizes) or any(tuple(m.shape[1:]) != tuple(size) for m, size in zip(masks_list, sizes)):
         raise ValueError("prediction and target must have matching batch and image dimensions")
     counts = [m.shape[0] for m in masks_list]
     if requested and any(requested[-1] > n for n in counts):
         raise ValueError("checkpoint exceeds prediction count")
-    B, H, W = maps.shape
+    B = len(sizes)
     G, C, T = gc.shape[1], num_classes, len(thresholds)
     points = [sorted(set(requested)) if requested is not None else (list(range(1, n + 1)) if n else [0])
               for n in counts]
@@ -94,9 +95,9 @@ def segmentation_prefixes(prediction: InstanceBatch, target, num_classes: int, *
     masks_list = [m.contiguous() for m in masks_list]
     n_off = offsets(counts, device)
     k_off = offsets([len(p) for p in points], device)
-    flat_points = torch.tensor([k for p in points for k in p], dtype=torch.int32).to(device)
+    flat_points = _native.device_table([k for p in points for k in p], device).int()
     cls = _cat(list(classes_list), torch.int32, device)
-    pointers = torch.tensor([m.data_ptr() for m in masks_list], dtype=torch.int64).to(device)
+    pointers = _native.device_table([m.data_ptr() for m in masks_list], device)
     threshold_tensor = torch.tensor(thresholds, dtype=torch.float64, device=device)
     sum_n, sum_tri = sum(counts), sum(n * (n + 1) // 2 for n in counts)
     longs = dict(dtype=torch.int64, device=device)
@@ -107,12 +108,17 @@ def segmentation_prefixes(prediction: InstanceBatch, target, num_classes: int, *
     itp, ifp, ifn = (torch.empty((rows, T, C), **longs) for _ in range(3))
     precision, recall = torch.empty((rows, T, C), **doubles), torch.empty((rows, T, C), **doubles)
     error = torch.empty(1, dtype=torch.int32, device=device)
-    size = max(_native.library().pe_prefix_workspace_bytes(B, G, C, T, sum_n, sum_tri), 256)
+    if target.packed:
+        pix_off, HW = offsets([h * w for h, w in sizes], device), maps.numel()
+    else:
+        pix_off, HW = None, sizes[0][0] * sizes[0][1] if sizes else 1
+    size = max(_native.library().pe_prefix_workspace_bytes(B, G, C, T, sum_n, sum_tri,
+                                                           HW if target.packed else 0), 256)
     workspace = torch.empty(size, dtype=torch.uint8, device=device)
-    HW = H * W
-    aligned = int(HW % 16 == 0 and all(m.data_ptr() % 16 == 0 for m in masks_list))
+    aligned = int(all((h * w) % 16 == 0 for h, w in sizes) and all(m.data_ptr() % 16 == 0 for m in masks_list))
     stream = _native.stream(device)
-    _native.call("pe_prefix", B, maps.data_ptr(), HW, G, C, pointers.data_ptr(), n_off.data_ptr(), sum_n, sum_tri,
+    _native.call("pe_prefix", B, maps.data_ptr(), _native.ptr(pix_off), HW, G, C, pointers.data_ptr(),
+                 n_off.data_ptr(), sum_n, sum_tri,
                  cls.data_ptr(), gc.data_ptr(), crowd.data_ptr(), things.data_ptr(), threshold_tensor.data_ptr(), T,
                  k_off.data_ptr(), flat_points.data_ptr(), rows, tp.data_ptr(), fp.data_ptr(), fn.data_ptr(),
                  sums.data_ptr(), confusion.data_ptr(), itp.data_ptr(), ifp.data_ptr(), ifn.data_ptr(),

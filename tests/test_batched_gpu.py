@@ -201,3 +201,26 @@ def test_capture_rejects_validation(pe):
     with pytest.raises(ValueError, match="validate=False"):
         CapturedUpdate(ev, pe.PanopticBatch(pair.pred.cuda(), pair.pc.cuda()),
                        pe.PanopticBatch(pair.gt.cuda(), pair.gc.cuda()))
+
+
+def test_packed_prefix_equals_per_image_oracle(pe):
+    """Prefix rewards over a packed batch of uneven images (odd sizes too)."""
+    import workloads
+    from oracle import prefix as opre
+    from test_prefix_gpu import compare_snapshots
+    batch = uneven(23, 4, sizes=(2, 4), segments=(3, 10), odd=True)
+    gen = torch.Generator().manual_seed(3)
+    masks = [workloads.ellipse_masks(gen, n, h, w, max_axis=40.0) for n, (h, w) in zip((7, 0, 12, 5), batch.sizes)]
+    classes = [torch.randint(0, 133, (len(m),), generator=gen).int() for m in masks]
+    isthing = [c < 80 for c in range(133)]
+    target = pe.PanopticBatch.pack([g.cuda() for g in batch.gt], [c.cuda() for c in batch.gc],
+                                   [c.cuda() for c in batch.crowd])
+    for points in (None, [0]):
+        actual = to_numpy(pe.segmentation_prefixes(pe.InstanceBatch([m.cuda() for m in masks],
+                                                                    [c.cuda() for c in classes]),
+                                                   target, 133, isthing=isthing, checkpoints=points))
+        for i in range(len(masks)):
+            expected = opre.prefixes([masks[i].numpy()], [classes[i].numpy()], batch.gt[i][None].numpy(),
+                                     batch.gc[i][None].numpy(), batch.crowd[i][None].numpy(), 133,
+                                     isthing=isthing, checkpoints=points)
+            compare_snapshots(actual[i], expected[0])

@@ -138,9 +138,10 @@ def test_packed_grid_variants(pe, tuning):
     prediction = pe.PanopticBatch.pack(cuda(batch.pred), cuda(batch.pc))
     dets = pe.InstanceBatch(cuda(batch.masks), cuda(batch.classes), cuda(batch.scores))
     for grid in range(count(tuning, "packed_grid")):
-        for det in (0, count(tuning, "detection_histogram") - 1):
+        for det, group in ((0, 0), (count(tuning, "detection_histogram") - 1, 3), (2, 2)):
             tuning.select("packed_grid", grid)
             tuning.select("detection_histogram", det)
+            tuning.select("detection_group", group)
             joint = pe.SegmentationEvaluator(133, isthing=isthing)
             joint.update(prediction, dets, target)
             result = to_numpy(joint.compute())
