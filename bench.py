@@ -228,6 +228,98 @@ class UnevenConfig:
         return done / elapsed, f"{done} of {self.images} images, oracle SegmentationEvaluator update+compute"
 
 
+class ServingConfig:
+    """RewardServer under Poisson arrivals at a fixed offered load: half
+    per-image PQ requests, half prefix-reward requests (uneven sizes)."""
+
+    dominant = "prefix_events"
+    dtype = "int32 maps / u8 masks (int64 counts, f64 rewards)"
+
+    def __init__(self, name, description, **sizes):
+        self.name, self.description, self.sizes = name, description, sizes
+
+    def setup(self, device):
+        import workloads
+        import panoptic_evaluator as pe
+        from panoptic_evaluator.serving import PQRequest, PrefixRequest
+        self.isthing = workloads.isthing_vector(133, 80)
+        pq, prefix = workloads.config4(device=device, **self.sizes)
+        self.raw = (pq, prefix)
+        self.pool = []
+        for images in pq:
+            target = pe.PanopticBatch.pack([i.gt for i in images], [i.gc for i in images], [i.crowd for i in images])
+            prediction = pe.PanopticBatch.pack([i.pred for i in images], [i.pc for i in images])
+            self.pool.append(PQRequest(prediction, target))
+        for images, points in prefix:
+            target = pe.PanopticBatch.pack([i.gt for i in images], [i.gc for i in images], [i.crowd for i in images])
+            masks = pe.InstanceBatch([i.masks for i in images], [i.classes for i in images])
+            self.pool.append(PrefixRequest(masks, target, points))
+        # interleave the two kinds
+        half = len(self.pool) // 2
+        self.pool = [r for pair in zip(self.pool[:half], self.pool[half:]) for r in pair]
+        self.config = {"workload": self.description, "pool_requests": len(self.pool),
+                       "pool_images": sum(len(images) for images in pq) + sum(len(images) for images, _ in prefix),
+                       "classes": 133, "validate": False}
+
+    def run(self, rate, requests, budget, streams, seed=7):
+        import numpy as np
+        import torch
+        from panoptic_evaluator import _native
+        from panoptic_evaluator.serving import RewardServer
+        rng = np.random.default_rng(seed)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, requests))
+        order = [self.pool[i % len(self.pool)] for i in range(requests)]
+        server = RewardServer(133, isthing=self.isthing, hbm_budget=budget, streams=streams)
+        lib = _native.library()
+        torch.cuda.synchronize()
+        lib.pe_profile_reset()
+        lib.pe_profile_enable(1)
+        tickets, i = [], 0
+        start = time.perf_counter()
+        while i < requests or not server.idle:
+            now = time.perf_counter() - start
+            while i < requests and arrivals[i] <= now:
+                tickets.append(server.submit(order[i], arrival=start + arrivals[i]))
+                i += 1
+            server.poll()
+            if server.idle and i < requests:
+                time.sleep(max(0.0, min(0.0005, arrivals[i] - (time.perf_counter() - start))))
+        elapsed = max(t.completed for t in tickets) - start
+        torch.cuda.synchronize()
+        lib.pe_profile_enable(0)
+        kernel_ms, launches = _native.profile_read(self.dominant)
+        latencies = np.array([t.latency for t in tickets]) * 1e3
+        images = sum(t.images for t in tickets)
+        return dict(images=images, elapsed=elapsed, latencies=latencies, kernel_ms=kernel_ms, launches=launches,
+                    streamed=server.prefix_pixels_streamed, batches=dict(server.launches), tickets=tickets)
+
+    def cpu_sample(self, budget_s):
+        from oracle import panoptic as opan
+        from oracle import prefix as opre
+        pq, prefix = self.raw
+        done, elapsed, r = 0, 0.0, 0
+        while elapsed < budget_s and r < len(pq):
+            images = pq[r]
+            for im in images:
+                a = [t.cpu().numpy()[None] for t in (im.gt, im.pred, im.gc, im.pc, im.crowd)]
+                t0 = time.perf_counter()
+                opan.panoptic_quality(a[0], a[1], a[2], a[3], a[4], 133, validate=False)
+                elapsed += time.perf_counter() - t0
+                done += 1
+            images, points = prefix[r]
+            for im in images[:2]:
+                args = ([im.masks.cpu().numpy()], [im.classes.cpu().numpy()], im.gt.cpu().numpy()[None],
+                        im.gc.cpu().numpy()[None], im.crowd.cpu().numpy()[None], 133)
+                t0 = time.perf_counter()
+                opre.prefixes(*args, isthing=self.isthing, checkpoints=points)
+                elapsed += time.perf_counter() - t0
+                done += 1
+                if elapsed > budget_s:
+                    break
+            r += 1
+        return done / elapsed, f"{done} images of PQ and prefix requests, oracle panoptic_quality/prefixes"
+
+
 def make_configs():
     import workloads
     return {
@@ -239,6 +331,9 @@ def make_configs():
                              "c0: PanopticEvaluator, 8 x 512x512, 133 classes, 40 Voronoi seeds"),
         "c3": PanopticConfig("c3", workloads.config3, 19, 8,
                              "c3: PanopticEvaluator, 64 x 1024x2048, 19 classes, 150 Voronoi seeds"),
+        "c4": ServingConfig("c4", "c4: RewardServer, Poisson arrivals; requests of 1..64 images of "
+                                  "256^2..1024^2, 10..100 segments; half per-image PQ, half prefix rewards "
+                                  "over 8..128 masks (every prefix or every 8th)"),
     }
 
 
@@ -282,6 +377,60 @@ def timed(fn, steps):
     return time.perf_counter() - t0
 
 
+def serve(cfg, args):
+    """c4: one JSON line with throughput and p50/p99 request latency."""
+    import numpy as np
+    log(f"[bench] warm-up: {min(len(cfg.pool), 8)} requests")
+    cfg.run(rate=1e6, requests=min(len(cfg.pool), 8), budget=args.hbm_budget, streams=args.streams)
+    log(f"[bench] serving {args.requests} requests at {args.rate} req/s offered")
+    out = cfg.run(rate=args.rate, requests=args.requests, budget=args.hbm_budget, streams=args.streams)
+    lat = out["latencies"]
+    achieved = out["streamed"] / (out["kernel_ms"] * 1e-3) / 1e9 if out["launches"] else None
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] timing the CPU oracle sample")
+        rate, sample = cfg.cpu_sample(args.cpu_budget)
+        cpu = {"value": round(rate, 4), "unit": "images/sec", "cores": 1, "kind": "port",
+               "sample": sample, "host_threads_available": os.cpu_count()}
+    line = {
+        "metric": METRIC,
+        "value": round(out["images"] / out["elapsed"], 3),
+        "unit": "images/sec",
+        "n_gpus": 1,
+        "steps": args.requests,
+        "warmup": min(len(cfg.pool), 8),
+        "ms_per_step": round(out["elapsed"] * 1e3 / args.requests, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": cfg.dtype,
+        "data": "synthetic",
+        "config": dict(cfg.config, offered_load_rps=args.rate, requests=args.requests,
+                       hbm_budget_bytes=args.hbm_budget, streams=args.streams,
+                       form="serving: continuous batching"),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                       "p99": round(float(np.percentile(lat, 99)), 3),
+                       "mean": round(float(lat.mean()), 3), "max": round(float(lat.max()), 3)},
+        "throughput_rps": round(args.requests / out["elapsed"], 3),
+        "batches": out["batches"],
+        "roofline": {
+            "bound": "hbm",
+            "kernel": cfg.dominant,
+            "achieved": None if achieved is None else round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "algorithmic_bytes_per_launch": round(out["streamed"] / max(out["launches"], 1)),
+            "avg_kernel_ms": round(out["kernel_ms"] / max(out["launches"], 1), 5),
+            "launches": out["launches"],
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main(argv=None):
     parser = argparse.ArgumentParser(description=__doc__)
     parser.add_argument("--gpus", type=int, default=1)
@@ -294,6 +443,10 @@ def main(argv=None):
     parser.add_argument("--no-autotune", action="store_true",
                         help="keep the default launch shapes instead of tuning them on this workload")
     parser.add_argument("--plain-only", action="store_true", help="time only the plain eager form")
+    parser.add_argument("--rate", type=float, default=200.0, help="c4: offered load, requests/sec")
+    parser.add_argument("--requests", type=int, default=400, help="c4: requests served")
+    parser.add_argument("--hbm-budget", type=int, default=16 << 30, help="c4: admission budget, bytes")
+    parser.add_argument("--streams", type=int, default=3, help="c4: streams for overlapping batches")
     args = parser.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the evaluation path shards by image with one "
@@ -311,6 +464,8 @@ def main(argv=None):
     log(f"[bench] setting up {cfg.name} on cuda:0")
     cfg.setup("cuda")
     torch.cuda.synchronize()
+    if isinstance(cfg, ServingConfig):
+        return serve(cfg, args)
     eager_batched = getattr(cfg, "eager_batched_step", cfg.step)
     for _ in range(args.warmup):
         cfg.step()

@@ -324,8 +324,10 @@ class PanopticEvaluator:
                         self.confusion[None], self.isthing)[0]
 
 
-def finalize(tp, fp, fn, sums, confusion, things):
-    """Run the FP64 finaliser over K accumulator rows; return K result dicts."""
+def finalize_arrays(tp, fp, fn, sums, confusion, things, *, copies=True):
+    """FP64 finaliser over K accumulator rows as device arrays: per_class
+    (K x 4 x C), summaries (K x 3 x 3: All/Things/Stuff x pq/sq/rq), counts
+    (K x 3), miou (K), and copies of the five state tensors (or None)."""
     K, C = tp.shape
     device = tp.device
     doubles = dict(dtype=torch.float64, device=device)
@@ -333,11 +335,18 @@ def finalize(tp, fp, fn, sums, confusion, things):
     summaries = torch.empty((K, 3, 3), **doubles)
     counts = torch.empty((K, 3), dtype=torch.int64, device=device)
     miou = torch.empty(K, **doubles)
-    copies = [torch.empty_like(t) for t in (tp, fp, fn, sums, confusion)]
+    state = [torch.empty_like(t) for t in (tp, fp, fn, sums, confusion)] if copies else [None] * 5
     tensors = [t.contiguous() for t in (tp, fp, fn, sums, confusion)]
     _native.call("pe_compute", *(t.data_ptr() for t in tensors), things.data_ptr(), K, C,
                  per_class.data_ptr(), summaries.data_ptr(), counts.data_ptr(), miou.data_ptr(),
-                 *(t.data_ptr() for t in copies), _native.stream(device))
+                 *(_native.ptr(t) for t in state), _native.stream(device))
+    return per_class, summaries, counts, miou, state
+
+
+def finalize(tp, fp, fn, sums, confusion, things):
+    """Run the FP64 finaliser over K accumulator rows; return K result dicts."""
+    K = tp.shape[0]
+    per_class, summaries, counts, miou, copies = finalize_arrays(tp, fp, fn, sums, confusion, things)
     out_tp, out_fp, out_fn, out_sums, out_confusion = copies
     results = []
     for k in range(K):

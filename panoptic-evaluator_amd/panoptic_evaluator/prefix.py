@@ -48,6 +48,53 @@ def _raise_prefix_error(code):
     raise ValueError("unknown target slot or empty prediction mask")
 
 
+def launch_prefix(masks_list, classes_list, maps, gc, crowd, sizes, packed, points, things, thresholds,
+                  num_classes, device):
+    """Launch the prefix kernels for images with per-image checkpoints
+    (``points[b]``: strictly increasing prefix lengths of image b) on the
+    current stream; returns the per-row device outputs (rows in image order)
+    and the error word.  ``keep`` holds the launch's scratch alive."""
+    B = len(sizes)
+    counts = [m.shape[0] for m in masks_list]
+    G, C, T = gc.shape[1], num_classes, len(thresholds)
+    rows = sum(len(p) for p in points)
+    masks_list = [m.contiguous() for m in masks_list]
+    n_off = offsets(counts, device)
+    k_off = offsets([len(p) for p in points], device)
+    flat_points = _native.device_table([k for p in points for k in p], device).int()
+    cls = _cat(list(classes_list), torch.int32, device)
+    pointers = _native.device_table([m.data_ptr() for m in masks_list], device)
+    threshold_tensor = torch.tensor(thresholds, dtype=torch.float64, device=device)
+    sum_n, sum_tri = sum(counts), sum(n * (n + 1) // 2 for n in counts)
+    longs = dict(dtype=torch.int64, device=device)
+    doubles = dict(dtype=torch.float64, device=device)
+    tp, fp, fn = (torch.empty((rows, C), **longs) for _ in range(3))
+    sums = torch.empty((rows, C), **doubles)
+    confusion = torch.empty((rows, C, C + 1), **longs)
+    itp, ifp, ifn = (torch.empty((rows, T, C), **longs) for _ in range(3))
+    precision, recall = torch.empty((rows, T, C), **doubles), torch.empty((rows, T, C), **doubles)
+    error = torch.empty(1, dtype=torch.int32, device=device)
+    if packed:
+        pix_off, HW = offsets([h * w for h, w in sizes], device), maps.numel()
+    else:
+        pix_off, HW = None, sizes[0][0] * sizes[0][1] if sizes else 1
+    size = max(_native.library().pe_prefix_workspace_bytes(B, G, C, T, sum_n, sum_tri,
+                                                           HW if packed else 0), 256)
+    workspace = torch.empty(size, dtype=torch.uint8, device=device)
+    aligned = int(all((h * w) % 16 == 0 for h, w in sizes) and all(m.data_ptr() % 16 == 0 for m in masks_list))
+    stream = _native.stream(device)
+    _native.call("pe_prefix", B, maps.data_ptr(), _native.ptr(pix_off), HW, G, C, pointers.data_ptr(),
+                 n_off.data_ptr(), sum_n, sum_tri,
+                 cls.data_ptr(), gc.data_ptr(), crowd.data_ptr(), things.data_ptr(), threshold_tensor.data_ptr(), T,
+                 k_off.data_ptr(), flat_points.data_ptr(), rows, tp.data_ptr(), fp.data_ptr(), fn.data_ptr(),
+                 sums.data_ptr(), confusion.data_ptr(), itp.data_ptr(), ifp.data_ptr(), ifn.data_ptr(),
+                 precision.data_ptr(), recall.data_ptr(), error.data_ptr(), workspace.data_ptr(), workspace.numel(),
+                 aligned, stream)
+    return dict(rows=rows, tp=tp, fp=fp, fn=fn, sums=sums, confusion=confusion, itp=itp, ifp=ifp, ifn=ifn,
+                precision=precision, recall=recall, error=error, thresholds=threshold_tensor,
+                keep=(workspace, pointers, n_off, k_off, flat_points, cls, pix_off))
+
+
 @torch.no_grad()
 def segmentation_prefixes(prediction: InstanceBatch, target, num_classes: int, *, isthing=None, checkpoints=None,
                           iou_thresholds=(0.5, 0.75), validate=True):
@@ -87,47 +134,18 @@ def segmentation_prefixes(prediction: InstanceBatch, target, num_classes: int, *
     counts = [m.shape[0] for m in masks_list]
     if requested and any(requested[-1] > n for n in counts):
         raise ValueError("checkpoint exceeds prediction count")
-    B = len(sizes)
-    G, C, T = gc.shape[1], num_classes, len(thresholds)
     points = [sorted(set(requested)) if requested is not None else (list(range(1, n + 1)) if n else [0])
               for n in counts]
-    rows = sum(len(p) for p in points)
-    masks_list = [m.contiguous() for m in masks_list]
-    n_off = offsets(counts, device)
-    k_off = offsets([len(p) for p in points], device)
-    flat_points = _native.device_table([k for p in points for k in p], device).int()
-    cls = _cat(list(classes_list), torch.int32, device)
-    pointers = _native.device_table([m.data_ptr() for m in masks_list], device)
-    threshold_tensor = torch.tensor(thresholds, dtype=torch.float64, device=device)
-    sum_n, sum_tri = sum(counts), sum(n * (n + 1) // 2 for n in counts)
-    longs = dict(dtype=torch.int64, device=device)
-    doubles = dict(dtype=torch.float64, device=device)
-    tp, fp, fn = (torch.empty((rows, C), **longs) for _ in range(3))
-    sums = torch.empty((rows, C), **doubles)
-    confusion = torch.empty((rows, C, C + 1), **longs)
-    itp, ifp, ifn = (torch.empty((rows, T, C), **longs) for _ in range(3))
-    precision, recall = torch.empty((rows, T, C), **doubles), torch.empty((rows, T, C), **doubles)
-    error = torch.empty(1, dtype=torch.int32, device=device)
-    if target.packed:
-        pix_off, HW = offsets([h * w for h, w in sizes], device), maps.numel()
-    else:
-        pix_off, HW = None, sizes[0][0] * sizes[0][1] if sizes else 1
-    size = max(_native.library().pe_prefix_workspace_bytes(B, G, C, T, sum_n, sum_tri,
-                                                           HW if target.packed else 0), 256)
-    workspace = torch.empty(size, dtype=torch.uint8, device=device)
-    aligned = int(all((h * w) % 16 == 0 for h, w in sizes) and all(m.data_ptr() % 16 == 0 for m in masks_list))
-    stream = _native.stream(device)
-    _native.call("pe_prefix", B, maps.data_ptr(), _native.ptr(pix_off), HW, G, C, pointers.data_ptr(),
-                 n_off.data_ptr(), sum_n, sum_tri,
-                 cls.data_ptr(), gc.data_ptr(), crowd.data_ptr(), things.data_ptr(), threshold_tensor.data_ptr(), T,
-                 k_off.data_ptr(), flat_points.data_ptr(), rows, tp.data_ptr(), fp.data_ptr(), fn.data_ptr(),
-                 sums.data_ptr(), confusion.data_ptr(), itp.data_ptr(), ifp.data_ptr(), ifn.data_ptr(),
-                 precision.data_ptr(), recall.data_ptr(), error.data_ptr(), workspace.data_ptr(), workspace.numel(),
-                 aligned, stream)
+    out = launch_prefix(masks_list, classes_list, maps, gc, crowd, sizes, target.packed, points, things,
+                        thresholds, num_classes, device)
     if validate:
-        code = int(error.item())
+        code = int(out["error"].item())
         if code:
             _raise_prefix_error(code)
+    rows = out["rows"]
+    tp, fp, fn, sums, confusion = out["tp"], out["fp"], out["fn"], out["sums"], out["confusion"]
+    itp, ifp, ifn, precision, recall = out["itp"], out["ifp"], out["ifn"], out["precision"], out["recall"]
+    threshold_tensor = out["thresholds"]
     snapshots = finalize(tp, fp, fn, sums, confusion, things) if rows else []
     results, row = [], 0
     for image_points in points:

@@ -10,7 +10,8 @@ Every input comes from one seed; nothing is read from disk.
             0..300 detections
   config 3: 64 x 1024 x 2048, 19 classes (8 thing), 150 seeds
   config 4: serving requests of 1..64 images (256..1024 square), 10..100
-            segments; per-image PQ or prefix rewards over 8..128 masks
+            segments; per-image PQ or prefix rewards over 8..128 masks with
+            checkpoints every prefix or every 8th
 """
 
 from __future__ import annotations
@@ -176,6 +177,50 @@ def config2(device="cpu", seed=2, B=256, sizes=(10, 42), segments=(5, 120), dete
         out.classes.append(_randint(gen, 0, num_things, (D,), device))
         out.scores.append(_rand(gen, D, device=device))
     return out
+
+
+@dataclass
+class RewardImage:
+    size: tuple
+    gt: torch.Tensor
+    pred: torch.Tensor
+    gc: torch.Tensor
+    pc: torch.Tensor
+    crowd: torch.Tensor
+    masks: torch.Tensor | None = None     # prefix requests: K x H x W generated masks
+    classes: torch.Tensor | None = None
+
+
+def config4(device="cpu", seed=4, pool=16, images=(1, 64), sides=(256, 512, 768, 1024), segments=(10, 100),
+            masks=(8, 128), strides=(1, 8), num_classes=133, num_things=80):
+    """Serving requests: ``pool`` per-image-PQ requests and ``pool`` prefix
+    requests, each with U[images] images of a side drawn from ``sides`` and
+    U[segments] Voronoi seeds; prefix requests generate U[masks] ellipse masks
+    per image and evaluate every prefix or every 8th (the last always)."""
+    gen = torch.Generator(device=device).manual_seed(seed)
+    pq, prefix = [], []
+    for kind in ("pq", "prefix"):
+        for _ in range(pool):
+            count = int(_randint(gen, images[0], images[1] + 1, (1,), device))
+            K = int(_randint(gen, masks[0], masks[1] + 1, (1,), device))
+            stride = strides[int(_randint(gen, 0, len(strides), (1,), device))]
+            request = []
+            for _ in range(count):
+                side = sides[int(_randint(gen, 0, len(sides), (1,), device))]
+                seeds = int(_randint(gen, segments[0], segments[1] + 1, (1,), device))
+                gt, pred, gc, pc, crowd = voronoi_image(gen, side, side, seeds, num_classes, num_things,
+                                                        device=device)
+                image = RewardImage((side, side), gt, pred, gc, pc, crowd)
+                if kind == "prefix":
+                    image.masks = ellipse_masks(gen, K, side, side, device=device)
+                    image.classes = _randint(gen, 0, num_classes, (K,), device).int()
+                request.append(image)
+            if kind == "pq":
+                pq.append(request)
+            else:
+                points = None if stride == 1 else sorted(set(list(range(stride, K + 1, stride)) + [K]))
+                prefix.append((request, points))
+    return pq, prefix
 
 
 def config3(device="cpu", seed=3, B=64, H=1024, W=2048):
