@@ -237,7 +237,8 @@ int64_t pe_prefix_workspace_bytes(int64_t B, int64_t G, int64_t C, int64_t T, in
  * share one size and target is B x HW int32 slots; otherwise B + 1 device
  * pixel offsets of a packed batch of uneven images (target holds them one
  * after another, HW is the pixel total, mask b is N_b x pixels_b; no
- * counterpart in the reference).  masks: device array of B pointers to N_b x HW
+ * counterpart in the reference).  max_n = the largest N_b.  masks: device
+ * array of B pointers to N_b x HW
  * bool masks in generation order (n_off: B + 1 device offsets; cls: sum N
  * int32 classes); gc/crowd: B x G target metadata; things: C uint8;
  * thresholds: T f64 IoU thresholds; k_off (B + 1) / points (sum K int32):
@@ -251,7 +252,7 @@ int64_t pe_prefix_workspace_bytes(int64_t B, int64_t G, int64_t C, int64_t T, in
  * mask.  Replaces ``prefix`` (bindings.cpp:24-26; prefix_cuda,
  * prefix_kernel.cu:142-199; CPU segmentation_prefixes, prefix.py:11-122). */
 int pe_prefix(int64_t B, const int32_t* target, const int64_t* pix_off, int64_t HW, int64_t G, int64_t C,
-              const uint8_t* const* masks, const int64_t* n_off, int64_t sum_n, int64_t sum_tri,
+              const uint8_t* const* masks, const int64_t* n_off, int64_t sum_n, int64_t sum_tri, int64_t max_n,
               const int32_t* cls, const int32_t* gc, const uint8_t* crowd, const uint8_t* things,
               const double* thresholds, int64_t T, const int64_t* k_off, const int32_t* points,
               int64_t total_rows, int64_t* tp, int64_t* fp, int64_t* fn, double* sums,

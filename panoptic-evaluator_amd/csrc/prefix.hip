@@ -89,9 +89,23 @@ __device__ __forceinline__ i64 dec_base(const PrefixArgs& a, int b) {
 // order: only the loads run ahead).
 constexpr int kPrefixAhead = 4;
 
+// Per-step event table in LDS (TABLE): the transitions of step i are counted
+// in a G x (N + 1) table, the keys touched first are listed, and after a
+// barrier the workgroup adds each touched entry once to the global raw/dec
+// tables.  Without it (large G x N) every pixel run goes to global atomics.
+constexpr int kPrefixList = 2048;                 // touched keys listed per step
+constexpr int kPrefixTableMax = 12288;            // G x (N + 1) entries in LDS
+
+__host__ __device__ inline size_t prefix_lds_bytes(int G, int P1, bool table) {
+  size_t bytes = 4 * static_cast<size_t>(G);
+  if (table) bytes += 4 * (static_cast<size_t>(G) * P1 + kPrefixList + 2);
+  return (bytes + 15) & ~static_cast<size_t>(15);
+}
+
 // One workgroup per pixel tile of one image (grid >= tile count; surplus
 // workgroups of a packed batch, whose count is only known on the device, exit
 // at once).
+template <bool TABLE>
 __global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
   extern __shared__ uint4 lds4[];
   uint32_t* ga = reinterpret_cast<uint32_t*>(lds4);
@@ -139,6 +153,29 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
   int32_t* dec = a.dec + dec_base(a, b);
   const int P1 = N + 1;
   const bool aligned = a.aligned != 0;
+  // LDS step table (TABLE): G x P1 counts, the touched-key list and two
+  // alternating list lengths (step i uses counter i & 1, and resets the other
+  // one for step i + 1 before its closing barrier).
+  uint32_t* table = ga + G;
+  uint32_t* list = table + static_cast<size_t>(G) * P1;
+  uint32_t* listed = list + kPrefixList;
+  if (TABLE) {
+    for (int e = threadIdx.x; e < G * P1; e += kPrefixThreads) table[e] = 0;
+    if (threadIdx.x < 2) listed[threadIdx.x] = 0;
+    __syncthreads();
+  }
+  auto add_run = [&](int i, int32_t* dec_i, int key, int run) {
+    if (TABLE) {
+      if (atomicAdd(table + key, static_cast<uint32_t>(run)) == 0) {
+        const uint32_t slot = atomicAdd(listed + (i & 1), 1u);
+        if (slot < kPrefixList) list[slot] = static_cast<uint32_t>(key);
+      }
+    } else {
+      const int rg = key / P1, prev = key - rg * P1;
+      atomicAdd(raw + static_cast<i64>(i) * G + rg, run);
+      atomicAdd(dec_i + static_cast<i64>(rg) * (i + 1) + prev, run);
+    }
+  };
   for (int i0 = 0; i0 < N; i0 += kPrefixAhead) {
     v4i ahead[kPrefixAhead][2];
 #pragma unroll
@@ -158,6 +195,10 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
     if (i >= N) break;
     const v4i* m = ahead[u];
     int32_t* dec_i = dec + static_cast<i64>(G) * tri(i);
+    // Counter (i + 1) & 1 was last read in step i - 1 before a barrier (or
+    // holds 0 already): clear it for step i + 1, whose users start after the
+    // next barrier.
+    if (TABLE && threadIdx.x == 0) listed[(i + 1) & 1] = 0;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       if ((m[h].x | m[h].y | m[h].z | m[h].w) == 0) continue;
@@ -174,20 +215,34 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
           top[h][j] = i + 1;
         }
         if (key != run_key) {
-          if (run_key >= 0) {
-            const int rg = run_key / P1, prev = run_key - rg * P1;
-            atomicAdd(raw + static_cast<i64>(i) * G + rg, run);
-            atomicAdd(dec_i + static_cast<i64>(rg) * (i + 1) + prev, run);
-          }
+          if (run_key >= 0) add_run(i, dec_i, run_key, run);
           run_key = key;
           run = 0;
         }
         run += key >= 0;
       }
-      if (run_key >= 0) {
-        const int rg = run_key / P1, prev = run_key - rg * P1;
-        atomicAdd(raw + static_cast<i64>(i) * G + rg, run);
-        atomicAdd(dec_i + static_cast<i64>(rg) * (i + 1) + prev, run);
+      if (run_key >= 0) add_run(i, dec_i, run_key, run);
+    }
+    if (TABLE) {
+      // Flush this step's touched entries once each (skipped when the tile
+      // saw no pixel of mask i: the decision is uniform after the barrier).
+      __syncthreads();
+      const uint32_t n = listed[i & 1];
+      if (n) {
+        auto flush = [&](uint32_t key) {
+          const uint32_t v = table[key];
+          if (!v) return;
+          table[key] = 0;
+          const int rg = static_cast<int>(key) / P1, prev = static_cast<int>(key) - rg * P1;
+          atomicAdd(raw + static_cast<i64>(i) * G + rg, static_cast<int32_t>(v));
+          atomicAdd(dec_i + static_cast<i64>(rg) * (i + 1) + prev, static_cast<int32_t>(v));
+        };
+        if (n <= kPrefixList) {
+          for (uint32_t e = threadIdx.x; e < n; e += kPrefixThreads) flush(list[e]);
+        } else {
+          for (int e = threadIdx.x; e < G * P1; e += kPrefixThreads) flush(static_cast<uint32_t>(e));
+        }
+        __syncthreads();
       }
     }
     }
@@ -440,6 +495,7 @@ extern "C" int64_t pe_prefix_workspace_bytes(int64_t B, int64_t G, int64_t C, in
 
 extern "C" int pe_prefix(int64_t B, const int32_t* target, const int64_t* pix_off, int64_t HW, int64_t G, int64_t C,
                          const uint8_t* const* masks, const int64_t* n_off, int64_t sum_n, int64_t sum_tri,
+                         int64_t max_n,
                          const int32_t* cls, const int32_t* gc, const uint8_t* crowd, const uint8_t* things,
                          const double* thresholds, int64_t T, const int64_t* k_off, const int32_t* points,
                          int64_t total_rows, int64_t* tp, int64_t* fp, int64_t* fn, double* sums,
@@ -448,7 +504,8 @@ extern "C" int pe_prefix(int64_t B, const int32_t* target, const int64_t* pix_of
                          int64_t workspace_bytes, int aligned, void* stream_ptr) {
   PE_REQUIRE(B >= 0 && HW > 0 && G > 0 && G < INT_MAX && C > 0 && C < INT_MAX && T > 0 && T * C < INT_MAX,
              "prefix dimensions outside supported range");
-  PE_REQUIRE(sum_n >= 0 && sum_tri >= 0 && total_rows >= 0, "invalid prefix sizes");
+  PE_REQUIRE(sum_n >= 0 && sum_tri >= 0 && total_rows >= 0 && max_n >= 0 && max_n <= sum_n,
+             "invalid prefix sizes");
   const i64 packed_pixels = pix_off ? HW : 0;
   PE_REQUIRE(workspace_bytes >= pe_prefix_workspace_bytes(B, G, C, T, sum_n, sum_tri, packed_pixels),
              "workspace too small");
@@ -520,11 +577,16 @@ extern "C" int pe_prefix(int64_t B, const int32_t* target, const int64_t* pix_of
     blocks = B * t.tiles_per_image;
   }
   PE_REQUIRE(blocks < INT_MAX, "too many workgroups");
-  const size_t lds = 4 * static_cast<size_t>(G) + 16;
+  // The LDS step table when G x (max_n + 1) fits.
+  const bool table = static_cast<i64>(G) * (max_n + 1) <= kPrefixTableMax;
+  const size_t lds = prefix_lds_bytes(static_cast<int>(G), static_cast<int>(table ? max_n + 1 : 1), table);
   PE_REQUIRE(lds <= 64 * 1024, "too many target slots for the prefix kernel");
   {
     ProfileScope timing("prefix_events", stream);
-    prefix_events<<<static_cast<unsigned>(blocks), kPrefixThreads, lds, stream>>>(a);
+    if (table)
+      prefix_events<true><<<static_cast<unsigned>(blocks), kPrefixThreads, lds, stream>>>(a);
+    else
+      prefix_events<false><<<static_cast<unsigned>(blocks), kPrefixThreads, lds, stream>>>(a);
     PE_LAUNCH_CHECK();
   }
   {

@@ -19,7 +19,8 @@ from .instance import InstanceBatch, _cat, offsets
 
 _i64, _int, _ptr = ctypes.c_int64, ctypes.c_int, ctypes.c_void_p
 _native.register("pe_prefix_workspace_bytes", _i64, [_i64] * 7)
-_native.register("pe_prefix", _int, [_i64, _ptr, _ptr, _i64, _i64, _i64, _ptr, _ptr, _i64, _i64, _ptr, _ptr, _ptr, _ptr,
+_native.register("pe_prefix", _int, [_i64, _ptr, _ptr, _i64, _i64, _i64, _ptr, _ptr, _i64, _i64, _i64, _ptr, _ptr,
+                                     _ptr, _ptr,
                                      _ptr, _i64, _ptr, _ptr, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr,
                                      _ptr, _ptr, _ptr, _ptr, _i64, _int, _ptr])
 
@@ -84,7 +85,7 @@ def launch_prefix(masks_list, classes_list, maps, gc, crowd, sizes, packed, poin
     aligned = int(all((h * w) % 16 == 0 for h, w in sizes) and all(m.data_ptr() % 16 == 0 for m in masks_list))
     stream = _native.stream(device)
     _native.call("pe_prefix", B, maps.data_ptr(), _native.ptr(pix_off), HW, G, C, pointers.data_ptr(),
-                 n_off.data_ptr(), sum_n, sum_tri,
+                 n_off.data_ptr(), sum_n, sum_tri, max(counts, default=0),
                  cls.data_ptr(), gc.data_ptr(), crowd.data_ptr(), things.data_ptr(), threshold_tensor.data_ptr(), T,
                  k_off.data_ptr(), flat_points.data_ptr(), rows, tp.data_ptr(), fp.data_ptr(), fn.data_ptr(),
                  sums.data_ptr(), confusion.data_ptr(), itp.data_ptr(), ifp.data_ptr(), ifn.data_ptr(),
