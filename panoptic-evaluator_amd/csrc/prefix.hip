@@ -94,6 +94,7 @@ constexpr int kPrefixAhead = 4;
 // barrier the workgroup adds each touched entry once to the global raw/dec
 // tables.  Without it (large G x N) every pixel run goes to global atomics.
 constexpr int kPrefixList = 2048;                 // touched keys listed per step
+constexpr size_t kPrefixStageMax = 120 * 1024;    // LDS for the staged scoring state
 constexpr int kPrefixTableMax = 12288;            // G x (N + 1) entries in LDS
 
 __host__ __device__ inline size_t prefix_lds_bytes(int G, int P1, bool table) {
@@ -252,46 +253,111 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
     if (ga[g]) atomicAdd(a.ga + static_cast<i64>(b) * G + g, static_cast<int32_t>(ga[g]));
 }
 
+// Per-image scoring state: the G x (N + 1) pair table of the visible canvas
+// and its metadata.  Built either entirely in LDS (STAGED) or entirely in the
+// global scratch, so that every access of one instantiation has a single
+// address space (LDS accesses then compile to ds_* instructions).
+struct PrefixView {
+  int32_t* pairs;      // G x P1
+  i64* pa;             // P1
+  int32_t* lab;        // P1
+  double* val;         // P1
+  int32_t* ga;         // G
+  int32_t* gc;         // G
+  uint8_t* crowd;      // G
+  int32_t* matched;    // G
+  uint8_t* used;       // T x G
+  i64* itp;            // T x C
+  i64* ifp;            // T x C
+  i64* positives;      // C
+};
+
+__host__ __device__ inline size_t prefix_view_bytes(int G, int P1, int C, int T) {
+  const size_t g = static_cast<size_t>(G), p = static_cast<size_t>(P1), c = static_cast<size_t>(C);
+  const size_t t = static_cast<size_t>(T);
+  // 8-byte arrays first, then 4-byte, then bytes
+  return 8 * (p + p + 2 * t * c + c) + 4 * (g * p + p + 3 * g) + (g + t * g) + 16;
+}
+
+__device__ __forceinline__ PrefixView lds_view(char* lds, int G, int P1, int C, int T) {
+  PrefixView v;
+  char* q = lds;
+  auto take = [&](size_t bytes) {
+    char* r = q;
+    q += bytes;
+    return r;
+  };
+  v.pa = reinterpret_cast<i64*>(take(8 * static_cast<size_t>(P1)));
+  v.val = reinterpret_cast<double*>(take(8 * static_cast<size_t>(P1)));
+  v.itp = reinterpret_cast<i64*>(take(8 * static_cast<size_t>(T) * C));
+  v.ifp = reinterpret_cast<i64*>(take(8 * static_cast<size_t>(T) * C));
+  v.positives = reinterpret_cast<i64*>(take(8 * static_cast<size_t>(C)));
+  v.pairs = reinterpret_cast<int32_t*>(take(4 * static_cast<size_t>(G) * P1));
+  v.lab = reinterpret_cast<int32_t*>(take(4 * static_cast<size_t>(P1)));
+  v.ga = reinterpret_cast<int32_t*>(take(4 * static_cast<size_t>(G)));
+  v.gc = reinterpret_cast<int32_t*>(take(4 * static_cast<size_t>(G)));
+  v.matched = reinterpret_cast<int32_t*>(take(4 * static_cast<size_t>(G)));
+  v.crowd = reinterpret_cast<uint8_t*>(take(static_cast<size_t>(G)));
+  v.used = reinterpret_cast<uint8_t*>(take(static_cast<size_t>(T) * G));
+  return v;
+}
+
+__device__ __forceinline__ PrefixView global_view(const PrefixArgs& a, int b, i64 n0) {
+  const int G = a.G, C = a.C, T = a.T;
+  PrefixView v;
+  v.pairs = a.pairs + (n0 + b) * G;
+  v.pa = a.pa + n0 + b;
+  v.lab = a.lab + n0 + b;
+  v.val = a.val + n0 + b;
+  v.ga = a.ga + static_cast<i64>(b) * G;
+  v.gc = const_cast<int32_t*>(a.gc + static_cast<i64>(b) * G);
+  v.crowd = const_cast<uint8_t*>(a.crowd + static_cast<i64>(b) * G);
+  v.matched = a.matched + static_cast<i64>(b) * G;
+  v.used = a.used + static_cast<i64>(b) * T * G;
+  v.itp = a.itp + static_cast<i64>(b) * T * C;
+  v.ifp = a.ifp + static_cast<i64>(b) * T * C;
+  v.positives = a.positives + static_cast<i64>(b) * C;
+  return v;
+}
+
 // Panoptic and semantic statistics of the visible canvas (snapshot row r).
-__device__ void prefix_snapshot(const PrefixArgs& a, int b, int k, i64 r, int P1, const int32_t* pairs, i64* pa,
-                                const int32_t* cls, const int32_t* gc, const uint8_t* crowd, const int32_t* ga,
-                                int32_t* matched, int32_t* lab, double* val, const i64* itp, const i64* ifp,
-                                const i64* positives) {
+__device__ __forceinline__ void prefix_snapshot(const PrefixArgs& a, int k, i64 r, int P1, const PrefixView& v,
+                                                const int32_t* cls) {
   const int G = a.G, C = a.C, tid = threadIdx.x;
   for (int j = tid; j <= k; j += blockDim.x) {
     i64 s = 0;
-    for (int g = 0; g < G; ++g) s += pairs[static_cast<i64>(g) * P1 + j];
-    pa[j] = s;
+    for (int g = 0; g < G; ++g) s += v.pairs[static_cast<i64>(g) * P1 + j];
+    v.pa[j] = s;
   }
-  for (int g = tid; g < G; g += blockDim.x) matched[g] = 0;
+  for (int g = tid; g < G; g += blockDim.x) v.matched[g] = 0;
   __syncthreads();
   u64* tp = reinterpret_cast<u64*>(a.tp + r * C);
   u64* fp = reinterpret_cast<u64*>(a.fp + r * C);
   u64* fn = reinterpret_cast<u64*>(a.fn + r * C);
   u64* conf = reinterpret_cast<u64*>(a.confusion + r * C * (C + 1));
   for (int j = tid; j <= k; j += blockDim.x) {
-    const int label = (j > 0 && pa[j] > 0) ? cls[j - 1] : -1;
+    const int label = (j > 0 && v.pa[j] > 0) ? cls[j - 1] : -1;
     const bool valid_pred = label >= 0 && label < C;
-    const i64 void_overlap = pairs[j];
+    const i64 void_overlap = v.pairs[j];
     i64 ignored = void_overlap;
     bool pm = false;
     double sum = 0.0;
     for (int g = 0; g < G; ++g) {
-      const int tcls = gc[g];
+      const int tcls = v.gc[g];
       if (tcls < 0 || tcls >= C) continue;
-      const i64 inter = pairs[static_cast<i64>(g) * P1 + j];
-      const bool is_crowd = crowd[g] != 0;
+      const i64 inter = v.pairs[static_cast<i64>(g) * P1 + j];
+      const bool is_crowd = v.crowd[g] != 0;
       if (inter && !is_crowd) atomicAdd(conf + static_cast<i64>(tcls) * (C + 1) + (valid_pred ? label : C),
                                         static_cast<u64>(inter));
       if (!valid_pred || tcls != label) continue;
       if (is_crowd) {
         ignored = void_overlap + inter;
       } else if (inter) {
-        const i64 uni = static_cast<i64>(ga[g]) + pa[j] - inter - void_overlap;
+        const i64 uni = static_cast<i64>(v.ga[g]) + v.pa[j] - inter - void_overlap;
         if (inter > uni / 2) {
           pm = true;
           sum += static_cast<double>(inter) / static_cast<double>(uni);
-          matched[g] = 1;
+          v.matched[g] = 1;
         }
       }
     }
@@ -300,28 +366,28 @@ __device__ void prefix_snapshot(const PrefixArgs& a, int b, int k, i64 r, int P1
       if (pm) {
         atomicAdd(tp + label, 1ULL);
         out = label;
-      } else if (static_cast<double>(ignored) <= 0.5 * static_cast<double>(pa[j])) {
+      } else if (static_cast<double>(ignored) <= 0.5 * static_cast<double>(v.pa[j])) {
         atomicAdd(fp + label, 1ULL);
       }
     }
-    lab[j] = out;
-    val[j] = sum;
+    v.lab[j] = out;
+    v.val[j] = sum;
   }
   __syncthreads();
   for (int g = tid; g < G; g += blockDim.x) {
-    const int c = gc[g];
-    if (c >= 0 && c < C && !crowd[g] && !matched[g]) atomicAdd(fn + c, 1ULL);
+    const int c = v.gc[g];
+    if (c >= 0 && c < C && !v.crowd[g] && !v.matched[g]) atomicAdd(fn + c, 1ULL);
   }
   double* sums = a.sums + r * C;
   for (int c = tid; c < C; c += blockDim.x) {
     double acc = 0.0;
     for (int j = 0; j <= k; ++j)
-      if (lab[j] == c) acc += val[j];
+      if (v.lab[j] == c) acc += v.val[j];
     sums[c] = acc;
   }
   const int TC = a.T * C;
   for (int e = tid; e < TC; e += blockDim.x) {
-    const i64 t_ = itp[e], f_ = ifp[e], n_ = positives[e % C];
+    const i64 t_ = v.itp[e], f_ = v.ifp[e], n_ = v.positives[e % C];
     a.out_itp[r * TC + e] = t_;
     a.out_ifp[r * TC + e] = f_;
     a.out_ifn[r * TC + e] = n_ - t_;
@@ -331,34 +397,34 @@ __device__ void prefix_snapshot(const PrefixArgs& a, int b, int k, i64 r, int P1
   __syncthreads();
 }
 
-__global__ __launch_bounds__(kPrefixThreads) void prefix_score(PrefixArgs a) {
-  const int b = blockIdx.x, tid = threadIdx.x;
+template <bool STAGED>
+__device__ __forceinline__ void prefix_score_image(const PrefixArgs& a, int b, const PrefixView& v) {
+  const int tid = threadIdx.x;
   const int G = a.G, C = a.C, T = a.T;
   const i64 n0 = a.n_off[b];
   const int N = static_cast<int>(a.n_off[b + 1] - n0);
   const int P1 = N + 1;
-  int32_t* pairs = a.pairs + (n0 + b) * G;
   const int32_t* raw = a.raw + n0 * G;
   const int32_t* dec = a.dec + dec_base(a, b);
-  const int32_t* ga = a.ga + static_cast<i64>(b) * G;
-  const int32_t* gc = a.gc + static_cast<i64>(b) * G;
-  const uint8_t* crowd = a.crowd + static_cast<i64>(b) * G;
   const int32_t* cls = a.cls + n0;
-  uint8_t* used = a.used + static_cast<i64>(b) * T * G;
-  int32_t* matched = a.matched + static_cast<i64>(b) * G;
-  i64* itp = a.itp + static_cast<i64>(b) * T * C;
-  i64* ifp = a.ifp + static_cast<i64>(b) * T * C;
-  i64* positives = a.positives + static_cast<i64>(b) * C;
-  i64* pa = a.pa + n0 + b;
-  int32_t* lab = a.lab + n0 + b;
-  double* val = a.val + n0 + b;
   const i64 kfirst = a.k_off[b], klast = a.k_off[b + 1];
+  if (STAGED) {
+    const int32_t* gc = a.gc + static_cast<i64>(b) * G;
+    const uint8_t* crowd = a.crowd + static_cast<i64>(b) * G;
+    const int32_t* ga = a.ga + static_cast<i64>(b) * G;
+    for (int g = tid; g < G; g += blockDim.x) {
+      v.gc[g] = gc[g];
+      v.crowd[g] = crowd[g];
+      v.ga[g] = ga[g];
+    }
+    __syncthreads();
+  }
   // Validation of the target metadata and the generated classes.
   int bits = 0;
   for (int g = tid; g < G; g += blockDim.x) {
-    const int c = gc[g];
-    if (c < -1 || c >= C || (g == 0 && (c != -1 || crowd[0]))) bits |= 2;
-    if (g > 0 && ((ga[g] > 0) != (c >= 0))) bits |= 4;
+    const int c = v.gc[g];
+    if (c < -1 || c >= C || (g == 0 && (c != -1 || v.crowd[0]))) bits |= 2;
+    if (g > 0 && ((v.ga[g] > 0) != (c >= 0))) bits |= 4;
   }
   for (int i = tid; i < N; i += blockDim.x) {
     if (cls[i] < 0 || cls[i] >= C) bits |= 8;
@@ -367,38 +433,38 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_score(PrefixArgs a) {
     if (area == 0) bits |= 16;
   }
   if (bits) atomicOr(a.error, bits);
-  for (i64 e = tid; e < static_cast<i64>(G) * P1; e += blockDim.x) {
-    const i64 g = e / P1;
-    pairs[e] = (e - g * P1) == 0 ? ga[g] : 0;
+  for (int e = tid; e < G * P1; e += blockDim.x) {
+    const int g = e / P1;
+    v.pairs[e] = (e - g * P1) == 0 ? v.ga[g] : 0;
   }
-  for (i64 e = tid; e < static_cast<i64>(T) * G; e += blockDim.x) used[e] = 0;
-  for (i64 e = tid; e < static_cast<i64>(T) * C; e += blockDim.x) {
-    itp[e] = 0;
-    ifp[e] = 0;
+  for (int e = tid; e < T * G; e += blockDim.x) v.used[e] = 0;
+  for (int e = tid; e < T * C; e += blockDim.x) {
+    v.itp[e] = 0;
+    v.ifp[e] = 0;
   }
   for (int c = tid; c < C; c += blockDim.x) {
     i64 n = 0;
     if (a.things[c])
-      for (int g = 0; g < G; ++g) n += gc[g] == c && !crowd[g];
-    positives[c] = n;
+      for (int g = 0; g < G; ++g) n += v.gc[g] == c && !v.crowd[g];
+    v.positives[c] = n;
   }
   __syncthreads();
   i64 q = kfirst;
   if (q < klast && a.points[q] == 0) {
-    prefix_snapshot(a, b, 0, q, P1, pairs, pa, cls, gc, crowd, ga, matched, lab, val, itp, ifp, positives);
+    prefix_snapshot(a, 0, q, P1, v, cls);
     ++q;
   }
   for (int i = 0; i < N; ++i) {
     const int32_t* dec_i = dec + static_cast<i64>(G) * tri(i);
-    const i64 width = i + 1;
-    for (i64 e = tid; e < static_cast<i64>(G) * width; e += blockDim.x) {
-      const int32_t v = dec_i[e];
-      if (v) {
-        const i64 g = e / width;
-        pairs[g * P1 + (e - g * width)] -= v;
+    const int width = i + 1;
+    for (int e = tid; e < G * width; e += blockDim.x) {
+      const int32_t d = dec_i[e];
+      if (d) {
+        const int g = e / width;
+        v.pairs[g * P1 + (e - g * width)] -= d;
       }
     }
-    for (int g = tid; g < G; g += blockDim.x) pairs[static_cast<i64>(g) * P1 + i + 1] = raw[static_cast<i64>(i) * G + g];
+    for (int g = tid; g < G; g += blockDim.x) v.pairs[g * P1 + i + 1] = raw[static_cast<i64>(i) * G + g];
     __syncthreads();
     const int c = cls[i];
     if (c >= 0 && c < C && a.things[c]) {
@@ -414,31 +480,47 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_score(PrefixArgs a) {
         double best_iou = -1.0;
         bool ignored = false;
         for (int g = 0; g < G; ++g) {
-          if (gc[g] != c) continue;
+          if (v.gc[g] != c) continue;
           const i64 inter = raw[static_cast<i64>(i) * G + g];
-          double uni = crowd[g] ? area : area + static_cast<double>(ga[g]) - static_cast<double>(inter);
+          double uni = v.crowd[g] ? area : area + static_cast<double>(v.ga[g]) - static_cast<double>(inter);
           if (uni < 1.0) uni = 1.0;
           const double iou = static_cast<double>(inter) / uni;
           if (iou < thr) continue;
-          if (crowd[g]) ignored = true;
-          else if (!used[static_cast<i64>(t) * G + g] && iou > best_iou) {
+          if (v.crowd[g]) ignored = true;
+          else if (!v.used[t * G + g] && iou > best_iou) {
             best = g;
             best_iou = iou;
           }
         }
         if (best >= 0) {
-          used[static_cast<i64>(t) * G + best] = 1;
-          ++itp[static_cast<i64>(t) * C + c];
+          v.used[t * G + best] = 1;
+          ++v.itp[t * C + c];
         } else if (!ignored) {
-          ++ifp[static_cast<i64>(t) * C + c];
+          ++v.ifp[t * C + c];
         }
       }
     }
     __syncthreads();
     if (q < klast && a.points[q] == i + 1) {
-      prefix_snapshot(a, b, i + 1, q, P1, pairs, pa, cls, gc, crowd, ga, matched, lab, val, itp, ifp, positives);
+      prefix_snapshot(a, i + 1, q, P1, v, cls);
       ++q;
     }
+  }
+}
+
+// One workgroup per image.  STAGED keeps the whole scoring state in LDS
+// (sized for the largest image of the launch); otherwise it lives in the
+// global scratch.
+template <bool STAGED>
+__global__ __launch_bounds__(kPrefixThreads) void prefix_score(PrefixArgs a) {
+  extern __shared__ uint4 lds4[];
+  const int b = blockIdx.x;
+  if (STAGED) {
+    const i64 n0 = a.n_off[b];
+    const int P1 = static_cast<int>(a.n_off[b + 1] - n0) + 1;
+    prefix_score_image<true>(a, b, lds_view(reinterpret_cast<char*>(lds4), a.G, P1, a.C, a.T));
+  } else {
+    prefix_score_image<false>(a, b, global_view(a, b, a.n_off[b]));
   }
 }
 
@@ -506,6 +588,7 @@ extern "C" int pe_prefix(int64_t B, const int32_t* target, const int64_t* pix_of
              "prefix dimensions outside supported range");
   PE_REQUIRE(sum_n >= 0 && sum_tri >= 0 && total_rows >= 0 && max_n >= 0 && max_n <= sum_n,
              "invalid prefix sizes");
+  PE_REQUIRE(G * (max_n + 1) < INT_MAX && G * (max_n * (max_n + 1) / 2) < INT_MAX, "prefix tables too large");
   const i64 packed_pixels = pix_off ? HW : 0;
   PE_REQUIRE(workspace_bytes >= pe_prefix_workspace_bytes(B, G, C, T, sum_n, sum_tri, packed_pixels),
              "workspace too small");
@@ -590,8 +673,18 @@ extern "C" int pe_prefix(int64_t B, const int32_t* target, const int64_t* pix_of
     PE_LAUNCH_CHECK();
   }
   {
+    // Scoring state in LDS when the largest image's fits.
+    const size_t view = prefix_view_bytes(static_cast<int>(G), static_cast<int>(max_n + 1), static_cast<int>(C),
+                                          static_cast<int>(T));
+    const bool staged = view <= kPrefixStageMax &&
+                        hipFuncSetAttribute(reinterpret_cast<const void*>(prefix_score<true>),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            static_cast<int>(kPrefixStageMax)) == hipSuccess;
     ProfileScope timing("prefix_score", stream);
-    prefix_score<<<static_cast<unsigned>(B), kPrefixThreads, 0, stream>>>(a);
+    if (staged)
+      prefix_score<true><<<static_cast<unsigned>(B), kPrefixThreads, view, stream>>>(a);
+    else
+      prefix_score<false><<<static_cast<unsigned>(B), kPrefixThreads, 0, stream>>>(a);
     PE_LAUNCH_CHECK();
   }
   return kOk;
