@@ -497,10 +497,13 @@ struct ApArgs {
 constexpr int kApChunk = 1024;
 constexpr int kApThreads = 256;
 
-__global__ __launch_bounds__(kApThreads) void ap_accumulate(ApArgs a) {
-  __shared__ uint8_t s_rank[kApChunk];
-  __shared__ __align__(16) int8_t s_flag[kApChunk * 40];
-  const int c = blockIdx.x, job = threadIdx.x;
+// The accumulation of class c over its records order[s0 .. s1) (global score
+// order, ties by record index).  ORDER is the address space of ``order``
+// (a global array or an LDS list), kept apart so accesses stay typed.
+template <typename OrderPtr>
+__device__ __forceinline__ void ap_class(const ApArgs& a, int c, OrderPtr order, int s0, int s1, uint8_t* s_rank,
+                                         int8_t* s_flag) {
+  const int job = threadIdx.x;
   const bool active = job < 120;
   const int m = job % 3, ar = (job / 3) % 4, t = job / 12;
   const int limit = m == 0 ? 1 : m == 1 ? 10 : 100;
@@ -509,7 +512,6 @@ __global__ __launch_bounds__(kApThreads) void ap_accumulate(ApArgs a) {
   const i64 global_job = static_cast<i64>(t) * pstride + c * 12 + ar * 3 + m;
   double* prec = a.precision + static_cast<i64>(t) * 101 * pstride + (c * 4 + ar) * 3 + m;
   double* rec = a.recall + global_job;
-  const int s0 = a.start[c], s1 = a.end[c];
   const int lane_flag = ar * 10 + t;
   // Records of one chunk -> LDS: the rank and the 40 flags (five 8-byte words,
   // rows are 8-byte aligned) of each record, four gathers in flight per lane.
@@ -527,7 +529,7 @@ __global__ __launch_bounds__(kApThreads) void ap_accumulate(ApArgs a) {
         rank[u] = 0;
         if (e < words) {
           const int i = e / 5, w = e - i * 5;
-          const int idx = a.order[first + i];
+          const int idx = order[first + i];
           v[u] = reinterpret_cast<const uint64_t*>(a.status + static_cast<i64>(idx) * 40)[w];
           if (w == 0) rank[u] = a.ranks[idx];
         }
@@ -647,6 +649,95 @@ __global__ __launch_bounds__(kApThreads) void ap_accumulate(ApArgs a) {
     a.partial[c * 24 + q] = acc;
   }
 }
+
+// Classes whose records were grouped by the two radix sorts.
+__global__ __launch_bounds__(kApThreads) void ap_accumulate(ApArgs a) {
+  __shared__ uint8_t s_rank[kApChunk];
+  __shared__ __align__(16) int8_t s_flag[kApChunk * 40];
+  const int c = blockIdx.x;
+  ap_class(a, c, as_global(a.order), a.start[c], a.end[c], s_rank, s_flag);
+}
+
+// Batches of at most kApSmall records: each class workgroup gathers its own
+// records (valid rank, this class) in index order, sorts them by (score
+// descending, index) with a bitonic network in LDS, and accumulates -- the
+// same order the two global radix sorts give, in one launch.
+constexpr int kApSmall = 4096;
+
+struct ApSmallArgs {
+  ApArgs ap;
+  const double* scores;
+  const int32_t* classes;
+  int N;
+  int sort_slots;        // power of two >= N
+};
+
+__global__ __launch_bounds__(kApThreads) void ap_small(ApSmallArgs s) {
+  extern __shared__ uint4 lds4[];
+  const ApArgs& a = s.ap;
+  const int c = blockIdx.x, tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  constexpr int kWaves = kApThreads / kWave;
+  u64* keys = reinterpret_cast<u64*>(lds4);
+  int32_t* idx = reinterpret_cast<int32_t*>(keys + s.sort_slots);
+  uint8_t* s_rank = reinterpret_cast<uint8_t*>(idx + s.sort_slots);
+  int8_t* s_flag = reinterpret_cast<int8_t*>(s_rank + kApChunk);
+  __shared__ int wave_count[kWaves];
+  __shared__ int total;
+  if (tid == 0) total = 0;
+  __syncthreads();
+  // Gather this class's records in index order.
+  for (int base = 0; base < s.N; base += kApThreads) {
+    const int i = base + tid;
+    bool mine = false;
+    if (i < s.N) {
+      const int r = a.ranks[i];
+      mine = s.classes[i] == c && r > 0 && r <= 100;
+    }
+    const u64 ballot = __ballot(mine);
+    if (lane == 0) wave_count[wave] = __popcll(ballot);
+    __syncthreads();
+    int offset = total;
+    for (int w = 0; w < wave; ++w) offset += wave_count[w];
+    if (mine) {
+      const int slot = offset + __popcll(ballot & ((1ULL << lane) - 1));
+      idx[slot] = i;
+      keys[slot] = score_key_desc(s.scores[i]);
+    }
+    __syncthreads();
+    if (tid == 0)
+      for (int w = 0; w < kWaves; ++w) total += wave_count[w];
+    __syncthreads();
+  }
+  const int n = total;
+  int m = 1;
+  while (m < n) m <<= 1;
+  for (int e = n + tid; e < m; e += kApThreads) {
+    keys[e] = ~0ULL;
+    idx[e] = INT_MAX;
+  }
+  __syncthreads();
+  // Bitonic sort of (key, index) ascending: descending score, then index.
+  for (int k = 2; k <= m; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int e = tid; e < m; e += kApThreads) {
+        const int f = e ^ j;
+        if (f <= e) continue;
+        const u64 ke = keys[e], kf = keys[f];
+        const int ie = idx[e], jf = idx[f];
+        const bool greater = ke > kf || (ke == kf && ie > jf);
+        if (greater == ((e & k) == 0)) {
+          keys[e] = kf;
+          keys[f] = ke;
+          idx[e] = jf;
+          idx[f] = ie;
+        }
+      }
+      __syncthreads();
+    }
+  ap_class(a, c, static_cast<const int32_t*>(idx), 0, n, s_rank, s_flag);
+}
+
+size_t ap_small_lds(int slots) { return 12 * static_cast<size_t>(slots) + kApChunk + 40 * kApChunk; }
 
 // instance.py:236-248: the 12 summaries as means over valid entries (>= 0),
 // from the per-class accumulators; classes are folded in a fixed order.
@@ -903,6 +994,38 @@ extern "C" int pe_ap_compute(int64_t N, const double* scores, const int32_t* cla
   const size_t temp = N > 0 ? sort_temp_bytes(N) : 0;
   ApBuffers w = carve_ap(workspace, N, C, temp);
   int rc;
+  ApArgs a{};
+  a.C = static_cast<int>(C);
+  a.order = w.order;
+  a.start = w.start;
+  a.end = w.end;
+  a.ranks = ranks;
+  a.status = status;
+  a.counts = reinterpret_cast<const i64*>(counts);
+  a.precision = precision;
+  a.recall = recall;
+  a.partial = w.partial;
+  if (N <= kApSmall) {
+    // One launch: every class gathers and sorts its own records in LDS.
+    ApSmallArgs sa{};
+    sa.ap = a;
+    sa.scores = scores;
+    sa.classes = classes;
+    sa.N = static_cast<int>(N);
+    sa.sort_slots = 64;  // keeps the flag staging area 16-byte aligned
+    while (sa.sort_slots < N) sa.sort_slots <<= 1;
+    const size_t lds = ap_small_lds(sa.sort_slots);
+    PE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(ap_small), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               static_cast<int>(lds)));
+    {
+      ProfileScope timing("ap_accumulate", stream);
+      ap_small<<<static_cast<unsigned>(C), kApThreads, lds, stream>>>(sa);
+      PE_LAUNCH_CHECK();
+    }
+    ap_summaries<<<1, 256, 0, stream>>>(static_cast<int>(C), w.partial, summary);
+    PE_LAUNCH_CHECK();
+    return kOk;
+  }
   if ((rc = launch_zero(w.start, 4 * static_cast<size_t>(C), stream))) return rc;
   if ((rc = launch_zero(w.end, 4 * static_cast<size_t>(C), stream))) return rc;
   if (N > 0) {
@@ -921,17 +1044,6 @@ extern "C" int pe_ap_compute(int64_t N, const double* scores, const int32_t* cla
     class_bounds<<<blocks, 256, 0, stream>>>(w.ckeys_out, N, static_cast<int>(C), w.start, w.end);
     PE_LAUNCH_CHECK();
   }
-  ApArgs a{};
-  a.C = static_cast<int>(C);
-  a.order = w.order;
-  a.start = w.start;
-  a.end = w.end;
-  a.ranks = ranks;
-  a.status = status;
-  a.counts = reinterpret_cast<const i64*>(counts);
-  a.precision = precision;
-  a.recall = recall;
-  a.partial = w.partial;
   {
     ProfileScope timing("ap_accumulate", stream);
     ap_accumulate<<<static_cast<unsigned>(C), kApThreads, 0, stream>>>(a);
