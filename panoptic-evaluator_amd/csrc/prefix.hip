@@ -29,6 +29,7 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr int kPrefixThreads = 256;
 constexpr int kPrefixTile = 16 * kPrefixThreads * 2;
+static_assert(kPrefixTile < 65536, "a step's count per pixel tile must fit the 16-bit table entries");
 
 struct PrefixArgs {
   int B, G, C, T;
@@ -95,11 +96,13 @@ constexpr int kPrefixAhead = 4;
 // tables.  Without it (large G x N) every pixel run goes to global atomics.
 constexpr int kPrefixList = 2048;                 // touched keys listed per step
 constexpr size_t kPrefixStageMax = 120 * 1024;    // LDS for the staged scoring state
-constexpr int kPrefixTableMax = 12288;            // G x (N + 1) entries in LDS
+constexpr int kPrefixTableMax = 32768;            // G x (N + 1) 16-bit entries in LDS
 
+// A step adds at most one tile's pixels (kPrefixTile < 2^16) to an entry, so
+// the table packs two 16-bit counts per 32-bit word.
 __host__ __device__ inline size_t prefix_lds_bytes(int G, int P1, bool table) {
   size_t bytes = 4 * static_cast<size_t>(G);
-  if (table) bytes += 4 * (static_cast<size_t>(G) * P1 + kPrefixList + 2);
+  if (table) bytes += 4 * ((static_cast<size_t>(G) * P1 + 1) / 2 + kPrefixList + 2);
   return (bytes + 15) & ~static_cast<size_t>(15);
 }
 
@@ -158,16 +161,17 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
   // alternating list lengths (step i uses counter i & 1, and resets the other
   // one for step i + 1 before its closing barrier).
   uint32_t* table = ga + G;
-  uint32_t* list = table + static_cast<size_t>(G) * P1;
+  uint32_t* list = table + (static_cast<size_t>(G) * P1 + 1) / 2;
   uint32_t* listed = list + kPrefixList;
   if (TABLE) {
-    for (int e = threadIdx.x; e < G * P1; e += kPrefixThreads) table[e] = 0;
+    for (int e = threadIdx.x; e < (G * P1 + 1) / 2; e += kPrefixThreads) table[e] = 0;
     if (threadIdx.x < 2) listed[threadIdx.x] = 0;
     __syncthreads();
   }
   auto add_run = [&](int i, int32_t* dec_i, int key, int run) {
     if (TABLE) {
-      if (atomicAdd(table + key, static_cast<uint32_t>(run)) == 0) {
+      const uint32_t shift = (key & 1) * 16;
+      if (((atomicAdd(table + (key >> 1), static_cast<uint32_t>(run) << shift) >> shift) & 0xFFFFu) == 0) {
         const uint32_t slot = atomicAdd(listed + (i & 1), 1u);
         if (slot < kPrefixList) list[slot] = static_cast<uint32_t>(key);
       }
@@ -231,9 +235,10 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
       const uint32_t n = listed[i & 1];
       if (n) {
         auto flush = [&](uint32_t key) {
-          const uint32_t v = table[key];
+          const uint32_t shift = (key & 1) * 16;
+          const uint32_t v = (table[key >> 1] >> shift) & 0xFFFFu;
           if (!v) return;
-          table[key] = 0;
+          atomicAnd(table + (key >> 1), ~(0xFFFFu << shift));  // the other half may be flushed concurrently
           const int rg = static_cast<int>(key) / P1, prev = static_cast<int>(key) - rg * P1;
           atomicAdd(raw + static_cast<i64>(i) * G + rg, static_cast<int32_t>(v));
           atomicAdd(dec_i + static_cast<i64>(rg) * (i + 1) + prev, static_cast<int32_t>(v));
@@ -661,9 +666,15 @@ extern "C" int pe_prefix(int64_t B, const int32_t* target, const int64_t* pix_of
   }
   PE_REQUIRE(blocks < INT_MAX, "too many workgroups");
   // The LDS step table when G x (max_n + 1) fits.
-  const bool table = static_cast<i64>(G) * (max_n + 1) <= kPrefixTableMax;
-  const size_t lds = prefix_lds_bytes(static_cast<int>(G), static_cast<int>(table ? max_n + 1 : 1), table);
-  PE_REQUIRE(lds <= 64 * 1024, "too many target slots for the prefix kernel");
+  bool table = static_cast<i64>(G) * (max_n + 1) <= kPrefixTableMax;
+  size_t lds = prefix_lds_bytes(static_cast<int>(G), static_cast<int>(table ? max_n + 1 : 1), table);
+  if (table && lds > 64 * 1024 &&
+      hipFuncSetAttribute(reinterpret_cast<const void*>(prefix_events<true>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)) != hipSuccess) {
+    table = false;
+    lds = prefix_lds_bytes(static_cast<int>(G), 1, false);
+  }
+  PE_REQUIRE(lds <= 64 * 1024 || table, "too many target slots for the prefix kernel");
   {
     ProfileScope timing("prefix_events", stream);
     if (table)
