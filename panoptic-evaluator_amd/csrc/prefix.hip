@@ -158,11 +158,15 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
   const int P1 = N + 1;
   const bool aligned = a.aligned != 0;
   // LDS step table (TABLE): G x P1 counts, the touched-key list and two
-  // alternating list lengths (step i uses counter i & 1, and resets the other
-  // one for step i + 1 before its closing barrier).
+  // alternating list lengths.  Only steps whose mask covers some pixel of the
+  // tile are processed; processed step number ps uses counter ps & 1 and
+  // clears the other one for the next processed step.
   uint32_t* table = ga + G;
   uint32_t* list = table + (static_cast<size_t>(G) * P1 + 1) / 2;
   uint32_t* listed = list + kPrefixList;
+  __shared__ uint32_t wave_touch[2][kPrefixThreads / kWave];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  int ps = 0;
   if (TABLE) {
     for (int e = threadIdx.x; e < (G * P1 + 1) / 2; e += kPrefixThreads) table[e] = 0;
     if (threadIdx.x < 2) listed[threadIdx.x] = 0;
@@ -172,7 +176,7 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
     if (TABLE) {
       const uint32_t shift = (key & 1) * 16;
       if (((atomicAdd(table + (key >> 1), static_cast<uint32_t>(run) << shift) >> shift) & 0xFFFFu) == 0) {
-        const uint32_t slot = atomicAdd(listed + (i & 1), 1u);
+        const uint32_t slot = atomicAdd(listed + (ps & 1), 1u);
         if (slot < kPrefixList) list[slot] = static_cast<uint32_t>(key);
       }
     } else {
@@ -194,16 +198,38 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
         for (int h = 0; h < 2; ++h) ahead[u][h] = v4i{0, 0, 0, 0};
       }
     }
+    // Which of these steps touch the tile at all (one barrier per group; the
+    // per-wave bits alternate between two buffers across groups).
+    uint32_t touched = (1u << kPrefixAhead) - 1;
+    if (TABLE) {
+      uint32_t mine = 0;
+#pragma unroll
+      for (int u = 0; u < kPrefixAhead; ++u)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          if ((ahead[u][h].x | ahead[u][h].y | ahead[u][h].z | ahead[u][h].w) != 0) mine |= 1u << u;
+      uint32_t bits = 0;
+#pragma unroll
+      for (int u = 0; u < kPrefixAhead; ++u)
+        if (__ballot((mine >> u) & 1u)) bits |= 1u << u;
+      const int parity = (i0 / kPrefixAhead) & 1;
+      if (lane == 0) wave_touch[parity][wave] = bits;
+      __syncthreads();
+      touched = 0;
+#pragma unroll
+      for (int w = 0; w < kPrefixThreads / kWave; ++w) touched |= wave_touch[parity][w];
+    }
 #pragma unroll
     for (int u = 0; u < kPrefixAhead; ++u) {
     const int i = i0 + u;
     if (i >= N) break;
+    if (!((touched >> u) & 1u)) continue;  // uniform: no pixel of mask i in this tile
     const v4i* m = ahead[u];
     int32_t* dec_i = dec + static_cast<i64>(G) * tri(i);
-    // Counter (i + 1) & 1 was last read in step i - 1 before a barrier (or
-    // holds 0 already): clear it for step i + 1, whose users start after the
-    // next barrier.
-    if (TABLE && threadIdx.x == 0) listed[(i + 1) & 1] = 0;
+    // Counter (ps + 1) & 1 was last read in processed step ps - 1 before a
+    // barrier (or holds 0 already): clear it for the next processed step,
+    // whose users start after the next barrier.
+    if (TABLE && threadIdx.x == 0) listed[(ps + 1) & 1] = 0;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       if ((m[h].x | m[h].y | m[h].z | m[h].w) == 0) continue;
@@ -232,7 +258,8 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
       // Flush this step's touched entries once each (skipped when the tile
       // saw no pixel of mask i: the decision is uniform after the barrier).
       __syncthreads();
-      const uint32_t n = listed[i & 1];
+      const uint32_t n = listed[ps & 1];
+      ++ps;
       if (n) {
         auto flush = [&](uint32_t key) {
           const uint32_t shift = (key & 1) * 16;
