@@ -39,6 +39,7 @@ for _path in (ROOT, ROOT / "panoptic-evaluator_amd"):
 
 METRIC = "images/sec PQ+mIoU+mask AP, batched vs plain on 1 GPU; p50/p99 reward latency"
 HBM_PEAK_GBS = 8000.0
+MFMA_INT8_PEAK_TOPS = 5000.0  # dense int8, MI355X
 
 
 def log(*args):
@@ -155,6 +156,73 @@ class CombinedConfig:
         state.compute()
         elapsed += time.perf_counter() - t0
         return done / elapsed, f"{done} of {self.images} images, oracle SegmentationEvaluator update+compute"
+
+
+class InstanceConfig:
+    """InstanceEvaluator reset/update/compute: c1's 100 ellipse detections per
+    image against GT instance masks (the target's thing segments as bool
+    masks); the mask x mask contraction runs on int8 MFMA."""
+
+    dominant = "mask_overlap_mfma"
+    tuned = ("mask_overlap_mfma",)
+    dtype = "u8 masks as int8 MFMA operands (int32 accumulate, f64 ratios)"
+
+    def __init__(self, name, description, **sizes):
+        self.name, self.description, self.sizes = name, description, sizes
+
+    def setup(self, device):
+        import torch
+        import workloads
+        import panoptic_evaluator as pe
+        pair, masks, classes, scores = workloads.config1(device=device, **self.sizes)
+        things = torch.tensor(workloads.isthing_vector(133, 80), device=device)
+        gt_masks, gt_classes, gt_crowd = [], [], []
+        for b in range(pair.gt.shape[0]):
+            gc = pair.gc[b]
+            slots = [s for s in range(gc.numel()) if gc[s] >= 0 and bool(things[gc[s]])]
+            index = torch.tensor(slots, device=device, dtype=torch.int32)
+            gt_masks.append((pair.gt[b][None] == index[:, None, None]).contiguous())
+            gt_classes.append(gc[index.long()].long())
+            gt_crowd.append(pair.crowd[b][index.long()])
+        self.masks, self.gt_masks = masks, gt_masks
+        self.evaluator = pe.InstanceEvaluator(133, validate=False)
+        self.prediction = pe.InstanceBatch(masks, classes, scores)
+        self.target = pe.InstanceBatch(gt_masks, gt_classes, crowd=gt_crowd)
+        B, H, W = pair.gt.shape
+        self.images = B
+        dets = [m.shape[0] for m in masks]
+        gts = [m.shape[0] for m in gt_masks]
+        self.bytes_per_launch = H * W * (sum(dets) + sum(gts))
+        self.mfma_ops = sum(2 * d * g * H * W for d, g in zip(dets, gts))
+        self.config = {"workload": self.description, "images": B, "height": H, "width": W, "classes": 133,
+                       "detections_per_image": sum(dets) // B, "gt_masks": sum(gts), "validate": False}
+
+    def update_args(self):
+        return self.prediction, self.target
+
+    def step(self):
+        self.evaluator.reset()
+        self.evaluator.update(*self.update_args())
+        return self.evaluator.compute()
+
+    def cpu_sample(self, budget_s):
+        from oracle import instance as oinst
+        state = oinst.InstanceState(133, validate=False)
+        done, elapsed = 0, 0.0
+        while done < self.images and elapsed < budget_s:
+            b = done
+            pred = [dict(masks=self.masks[b].cpu().numpy(), classes=self.prediction.classes[b].cpu().numpy(),
+                         scores=self.prediction.scores[b].cpu().numpy())]
+            tgt = [dict(masks=self.gt_masks[b].cpu().numpy(), classes=self.target.classes[b].cpu().numpy(),
+                        crowd=self.target.crowd[b].cpu().numpy())]
+            t0 = time.perf_counter()
+            state.update(pred, tgt)
+            elapsed += time.perf_counter() - t0
+            done += 1
+        t0 = time.perf_counter()
+        state.compute()
+        elapsed += time.perf_counter() - t0
+        return done / elapsed, f"{done} of {self.images} images, oracle InstanceEvaluator update+compute"
 
 
 class UnevenConfig:
@@ -327,6 +395,8 @@ def make_configs():
                                    "100 ellipse detections per image"),
         "c2": UnevenConfig("c2", "c2: SegmentationEvaluator, 256 uneven images (H, W multiples of 32 in "
                                  "[320, 1344]), 5..120 segments, 0..300 ellipse detections, 133 classes"),
+        "c1i": InstanceConfig("c1i", "c1i: InstanceEvaluator, 16 x 800x1088, 100 ellipse detections vs the "
+                                     "target's thing segments as GT masks, 133 classes"),
         "c0": PanopticConfig("c0", workloads.config0, 133, 80,
                              "c0: PanopticEvaluator, 8 x 512x512, 133 classes, 40 Voronoi seeds"),
         "c3": PanopticConfig("c3", workloads.config3, 19, 8,
@@ -571,6 +641,11 @@ def main(argv=None):
         "cpu_baseline": cpu,
         "tuning": tuning_report,
     }
+    if getattr(cfg, "mfma_ops", None) and launches:
+        tops = cfg.mfma_ops / (avg_ms * 1e-3) / 1e12
+        line["mfma"] = {"kernel": cfg.dominant, "dense_equivalent_ops_per_launch": cfg.mfma_ops,
+                        "achieved": round(tops, 2), "peak": MFMA_INT8_PEAK_TOPS, "unit": "TOP/s",
+                        "frac": round(tops / MFMA_INT8_PEAK_TOPS, 4)}
     print(json.dumps(line), flush=True)
     return 0
 

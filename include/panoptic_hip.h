@@ -136,6 +136,22 @@ int pe_pack(const void* maps, int map_bytes, int64_t B, int64_t HW,
 int pe_mask_overlap(const uint8_t* pm, int64_t D, const uint8_t* gm, int64_t G, int64_t HW,
                     int32_t* inter, int32_t* pa, int32_t* ga, void* stream);
 
+/* The same contraction for a batch of B images in one launch: pm/gm are
+ * device arrays of B pointers (D_b x pixels_b and G_b x pixels_b bool
+ * masks); det_off, gt_off (B + 1) give the rows of each image's areas in
+ * pa/ga and inter_off (B + 1) the start of its D_b x G_b block in inter.
+ * pixels (B, device) holds each image's pixel count, or is NULL when every
+ * image has HW (then max_hw == HW).  max_d/max_g/max_hw bound the images
+ * (they size the grid); total_* size the outputs (zeroed first).  aligned != 0
+ * promises 16-byte aligned mask pointers and pixel counts multiple of 16.
+ * Replaces the per-image instance_update_cuda contraction loop
+ * (instance.py:182-186 calls it per image). */
+int pe_mask_overlap_batch(int64_t B, const uint8_t* const* pm, const uint8_t* const* gm,
+                          const int64_t* det_off, const int64_t* gt_off, const int64_t* inter_off,
+                          const int64_t* pixels, int64_t HW, int64_t max_d, int64_t max_g, int64_t max_hw,
+                          int64_t total_d, int64_t total_g, int64_t total_inter, int32_t* inter, int32_t* pa,
+                          int32_t* ga, int aligned, void* stream);
+
 /* Scratch bytes of pe_detection_histogram for a packed batch of
  * packed_pixels pixels (0 for a batch of one image size: no scratch). */
 int64_t pe_detection_histogram_workspace_bytes(int64_t B, int64_t packed_pixels);

@@ -226,14 +226,16 @@ const char* const kDetVariants[] = {"1 chunk x 4 masks", "1 chunk x 8 masks", "2
                                     "2 chunks x 4 masks", "4 chunks x 2 masks"};
 const char* const kJointVariants[] = {"auto chunk", "8192 px", "16384 px", "32768 px", "65532 px"};
 const char* const kOverlapVariants[] = {"8 GT tiles x 16384 px", "4 GT tiles x 16384 px", "8 GT tiles x 8192 px",
-                                        "8 GT tiles x 32768 px"};
+                                        "8 GT tiles x 32768 px", "4 GT tiles x 8192 px, 2 steps ahead",
+                                        "4 GT tiles x 8192 px, 4 steps ahead", "4 GT tiles x 16384 px, 4 steps ahead",
+                                        "4 GT tiles x 32768 px, 4 steps ahead"};
 const char* const kGridVariants[] = {"one workgroup per work item", "persistent: 8 workgroups per CU",
                                      "persistent: 4 workgroups per CU"};
 const char* const kGroupVariants[] = {"LDS budget (96 KB)", "128 detections", "64 detections", "32 detections"};
 const TuneSpec kSpecs[kTuneKernels] = {
     {"detection_histogram", 5, 0, kDetVariants},
     {"joint_histogram", 5, 0, kJointVariants},
-    {"mask_overlap_mfma", 4, 0, kOverlapVariants},
+    {"mask_overlap_mfma", 8, 0, kOverlapVariants},
     {"packed_grid", 3, 0, kGridVariants},
     {"detection_group", 4, 0, kGroupVariants},
 };

@@ -48,23 +48,61 @@ __device__ __forceinline__ int byte_sum4(int w) {
 __device__ __forceinline__ v4i load16(const uint8_t* row, i64 k, i64 HW, bool valid_row, bool aligned) {
   v4i v = {0, 0, 0, 0};
   if (!valid_row || k >= HW) return v;
-  if (aligned && k + 16 <= HW) return *reinterpret_cast<const v4i*>(row + k);
+  if (aligned && k + 16 <= HW) return *as_global(reinterpret_cast<const v4i*>(row + k));
+  const auto* g = as_global(row);
   uint8_t bytes[16];
 #pragma unroll
-  for (int j = 0; j < 16; ++j) bytes[j] = k + j < HW ? row[k + j] : 0;
+  for (int j = 0; j < 16; ++j) bytes[j] = k + j < HW ? g[k + j] : 0;
   __builtin_memcpy(&v, bytes, 16);
   return v;
 }
 
-// inter[d, g] += popcount(pm[d] & gm[g]) over this wave's K chunk.
-// kOverlapGTiles GT tiles (of 16) per wave; kOverlapChunk pixels per wave
-// (a multiple of 64).
-template <int kOverlapGTiles, int kOverlapChunk>
-__global__ __launch_bounds__(64) void mask_overlap_mfma(const uint8_t* pm, int D, const uint8_t* gm, int G,
-                                                        i64 HW, int32_t* inter, int32_t* pa, int32_t* ga,
-                                                        int aligned) {
+// Images of a mask-overlap batch: detection masks pm[b] (D_b x pixels_b) and
+// GT masks gm[b] (G_b x pixels_b); inter[b] is a D_b x G_b int32 block at
+// inter_off[b]; areas at det_off[b] / gt_off[b].
+// One image (pm_table == nullptr) takes its pointers and counts from the
+// arguments themselves.
+struct OverlapArgs {
+  const uint8_t* const* pm_table;
+  const uint8_t* const* gm_table;
+  const i64* det_off;     // B + 1
+  const i64* gt_off;      // B + 1
+  const i64* inter_off;   // B + 1
+  const i64* pixels;      // B pixel counts, or nullptr: every image has HW
+  const uint8_t* pm;      // one image
+  const uint8_t* gm;
+  int D, G;
+  i64 HW;
+  int chunks;             // pixel chunks of the largest image
+  int32_t* inter;
+  int32_t* pa;            // may be null
+  int32_t* ga;            // may be null
+  int aligned;
+};
+
+// inter[d, g] += popcount(pm[d] & gm[g]) over one wave's pixel chunk, for a
+// 16-detection tile and up to kOverlapGTiles 16-GT tiles (kOverlapChunk
+// pixels per wave, a multiple of 64).  Grid: x = detection tile (fastest, so
+// the waves reading one chunk's GT tiles run together and share them in L2),
+// y = GT tile group, z = image * chunks + chunk.
+template <int kOverlapGTiles, int kOverlapChunk, int kOverlapAhead>
+__global__ __launch_bounds__(64) void mask_overlap_mfma(OverlapArgs o) {
   const int lane = threadIdx.x;
-  const int chunk = blockIdx.x, dtile = blockIdx.y, ggroup = blockIdx.z;
+  const int dtile = blockIdx.x, ggroup = blockIdx.y;
+  const int b = blockIdx.z / o.chunks, chunk = blockIdx.z - b * o.chunks;
+  const bool table = o.pm_table != nullptr;
+  const i64 d_first = table ? o.det_off[b] : 0, g_first = table ? o.gt_off[b] : 0;
+  const int D = table ? static_cast<int>(o.det_off[b + 1] - d_first) : o.D;
+  const int G = table ? static_cast<int>(o.gt_off[b + 1] - g_first) : o.G;
+  const i64 HW = o.pixels ? o.pixels[b] : o.HW;
+  const i64 k_begin = static_cast<i64>(chunk) * kOverlapChunk;
+  // With D == 0 or G == 0 the first tile of the other side still counts areas.
+  if (k_begin >= HW || dtile * 16 >= max(D, 1) || ggroup * kOverlapGTiles * 16 >= max(G, 1)) return;
+  const uint8_t* pm = table ? o.pm_table[b] : o.pm;
+  const uint8_t* gm = table ? o.gm_table[b] : o.gm;
+  int32_t* inter = o.inter + (table ? o.inter_off[b] : 0);
+  int32_t* pa = o.pa ? o.pa + d_first : nullptr;
+  int32_t* ga = o.ga ? o.ga + g_first : nullptr;
   const int row = lane & 15, kgroup = lane >> 4;
   const int d = dtile * 16 + row;
   const bool dvalid = d < D;
@@ -87,19 +125,29 @@ __global__ __launch_bounds__(64) void mask_overlap_mfma(const uint8_t* pm, int D
   for (int t = 0; t < kOverlapGTiles; ++t) bsum[t] = 0;
   const bool count_a = pa != nullptr && ggroup == 0;
   const bool count_b = ga != nullptr && dtile == 0;
-  const i64 k_begin = static_cast<i64>(chunk) * kOverlapChunk;
   const i64 k_end = min(k_begin + kOverlapChunk, HW);
-  const bool al = aligned != 0;
-  for (i64 k0 = k_begin; k0 < k_end; k0 += 64) {
-    const i64 k = k0 + kgroup * 16;
-    v4i a = load16(arow, k, k_end, dvalid, al);
-    if (count_a) asum += byte_sum4(a.x) + byte_sum4(a.y) + byte_sum4(a.z) + byte_sum4(a.w);
+  const bool al = o.aligned != 0;
+  // kOverlapAhead K steps of loads are issued before their MFMAs.
+  for (i64 k0 = k_begin; k0 < k_end; k0 += 64 * kOverlapAhead) {
+    v4i a[kOverlapAhead], bv[kOverlapAhead][kOverlapGTiles];
 #pragma unroll
-    for (int t = 0; t < kOverlapGTiles; ++t) {
-      if (t >= tiles) break;
-      v4i b = load16(brow[t], k, k_end, bvalid[t], al);
-      if (count_b) bsum[t] += byte_sum4(b.x) + byte_sum4(b.y) + byte_sum4(b.z) + byte_sum4(b.w);
-      acc[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc[t], 0, 0, 0);
+    for (int u = 0; u < kOverlapAhead; ++u) {
+      const i64 k = k0 + u * 64 + kgroup * 16;
+      a[u] = load16(arow, k, k_end, dvalid, al);
+#pragma unroll
+      for (int t = 0; t < kOverlapGTiles; ++t) bv[u][t] = t < tiles ? load16(brow[t], k, k_end, bvalid[t], al)
+                                                                     : v4i{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int u = 0; u < kOverlapAhead; ++u) {
+      if (count_a) asum += byte_sum4(a[u].x) + byte_sum4(a[u].y) + byte_sum4(a[u].z) + byte_sum4(a[u].w);
+#pragma unroll
+      for (int t = 0; t < kOverlapGTiles; ++t) {
+        if (t >= tiles) break;
+        if (count_b)
+          bsum[t] += byte_sum4(bv[u][t].x) + byte_sum4(bv[u][t].y) + byte_sum4(bv[u][t].z) + byte_sum4(bv[u][t].w);
+        acc[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[u], bv[u][t], acc[t], 0, 0, 0);
+      }
     }
   }
   // D fragment: lane holds rows 4*(lane/16) + r, column lane % 16.
@@ -125,11 +173,11 @@ __global__ __launch_bounds__(64) void mask_overlap_mfma(const uint8_t* pm, int D
 #pragma unroll
     for (int t = 0; t < kOverlapGTiles; ++t) {
       if (t >= tiles) break;
-      int s = bsum[t];
-      s += __shfl_xor(s, 16, kWave);
-      s += __shfl_xor(s, 32, kWave);
+      int sum = bsum[t];
+      sum += __shfl_xor(sum, 16, kWave);
+      sum += __shfl_xor(sum, 32, kWave);
       const int g = (ggroup * kOverlapGTiles + t) * 16 + row;
-      if (kgroup == 0 && g < G && s) atomicAdd(ga + g, s);
+      if (kgroup == 0 && g < G && sum) atomicAdd(ga + g, sum);
     }
   }
 }
@@ -137,15 +185,36 @@ __global__ __launch_bounds__(64) void mask_overlap_mfma(const uint8_t* pm, int D
 struct OverlapVariant {
   int gtiles;
   int chunk;
-  void (*kernel)(const uint8_t*, int, const uint8_t*, int, i64, int32_t*, int32_t*, int32_t*, int);
+  void (*kernel)(OverlapArgs);
 };
 // Order matches the tuning registry's descriptions.
 const OverlapVariant kOverlap[] = {
-    {8, 16384, mask_overlap_mfma<8, 16384>},
-    {4, 16384, mask_overlap_mfma<4, 16384>},
-    {8, 8192, mask_overlap_mfma<8, 8192>},
-    {8, 32768, mask_overlap_mfma<8, 32768>},
+    {8, 16384, mask_overlap_mfma<8, 16384, 1>}, {4, 16384, mask_overlap_mfma<4, 16384, 1>},
+    {8, 8192, mask_overlap_mfma<8, 8192, 1>},   {8, 32768, mask_overlap_mfma<8, 32768, 1>},
+    {4, 8192, mask_overlap_mfma<4, 8192, 2>},   {4, 8192, mask_overlap_mfma<4, 8192, 4>},
+    {4, 16384, mask_overlap_mfma<4, 16384, 4>}, {4, 32768, mask_overlap_mfma<4, 32768, 4>},
 };
+
+int launch_overlap(OverlapArgs o, int64_t B, int64_t max_d, int64_t max_g, int64_t max_hw, int64_t total_d,
+                   int64_t total_g, int64_t total_inter, hipStream_t stream) {
+  int rc;
+  if (total_inter && (rc = launch_zero(o.inter, 4 * static_cast<size_t>(total_inter), stream))) return rc;
+  if (o.pa && total_d && (rc = launch_zero(o.pa, 4 * static_cast<size_t>(total_d), stream))) return rc;
+  if (o.ga && total_g && (rc = launch_zero(o.ga, 4 * static_cast<size_t>(total_g), stream))) return rc;
+  if (B == 0 || max_hw == 0 || (max_d == 0 && max_g == 0)) return kOk;
+  const OverlapVariant& v = kOverlap[tuned_variant(kTuneMaskOverlap)];
+  const i64 chunks = (max_hw + v.chunk - 1) / v.chunk;
+  const i64 dtiles = std::max<i64>(1, (max_d + 15) / 16);
+  const i64 ggroups = std::max<i64>(1, (max_g + 16 * v.gtiles - 1) / (16 * v.gtiles));
+  if (dtiles >= INT_MAX || ggroups >= 65536 || B * chunks >= 65536)
+    return fail(kUnsupportedShape, "too many masks or pixels for one overlap launch");
+  o.chunks = static_cast<int>(chunks);
+  dim3 grid(static_cast<unsigned>(dtiles), static_cast<unsigned>(ggroups), static_cast<unsigned>(B * chunks));
+  ProfileScope timing("mask_overlap_mfma", stream);
+  v.kernel<<<grid, 64, 0, stream>>>(o);
+  PE_LAUNCH_CHECK();
+  return kOk;
+}
 
 // ------------------------------------------------- detection x slot counts
 
@@ -778,27 +847,42 @@ using namespace pe;
 extern "C" int pe_mask_overlap(const uint8_t* pm, int64_t D, const uint8_t* gm, int64_t G, int64_t HW,
                                int32_t* inter, int32_t* pa, int32_t* ga, void* stream_ptr) {
   PE_REQUIRE(D >= 0 && G >= 0 && HW >= 0 && D < INT_MAX && G < INT_MAX && HW < INT_MAX, "invalid mask shapes");
-  hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
-  int rc;
-  if (D * G && (rc = launch_zero(inter, 4 * static_cast<size_t>(D * G), stream))) return rc;
-  if (pa && D && (rc = launch_zero(pa, 4 * static_cast<size_t>(D), stream))) return rc;
-  if (ga && G && (rc = launch_zero(ga, 4 * static_cast<size_t>(G), stream))) return rc;
-  if (HW == 0 || (D == 0 && G == 0)) return kOk;
-  const bool aligned = HW % 16 == 0 && reinterpret_cast<uintptr_t>(pm) % 16 == 0 &&
-                       reinterpret_cast<uintptr_t>(gm) % 16 == 0;
-  const OverlapVariant& v = kOverlap[tuned_variant(kTuneMaskOverlap)];
-  const i64 chunks = (HW + v.chunk - 1) / v.chunk;
-  PE_REQUIRE(chunks < INT_MAX, "image too large");
-  // With D == 0 or G == 0 still count the other side's areas.
-  const i64 dtiles = std::max<i64>(1, (D + 15) / 16);
-  const i64 ggroups = std::max<i64>(1, (G + 16 * v.gtiles - 1) / (16 * v.gtiles));
-  PE_REQUIRE(dtiles < 65536 && ggroups < 65536, "too many masks");
-  dim3 grid(static_cast<unsigned>(chunks), static_cast<unsigned>(dtiles), static_cast<unsigned>(ggroups));
-  ProfileScope timing("mask_overlap_mfma", stream);
-  v.kernel<<<grid, 64, 0, stream>>>(pm, static_cast<int>(D), gm, static_cast<int>(G), HW, inter, pa, ga,
-                                    aligned ? 1 : 0);
-  PE_LAUNCH_CHECK();
-  return kOk;
+  OverlapArgs o{};
+  o.pm = pm;
+  o.gm = gm;
+  o.D = static_cast<int>(D);
+  o.G = static_cast<int>(G);
+  o.HW = HW;
+  o.inter = inter;
+  o.pa = pa;
+  o.ga = ga;
+  o.aligned = HW % 16 == 0 && reinterpret_cast<uintptr_t>(pm) % 16 == 0 && reinterpret_cast<uintptr_t>(gm) % 16 == 0;
+  return launch_overlap(o, 1, D, G, HW, D, G, D * G, static_cast<hipStream_t>(stream_ptr));
+}
+
+extern "C" int pe_mask_overlap_batch(int64_t B, const uint8_t* const* pm, const uint8_t* const* gm,
+                                     const int64_t* det_off, const int64_t* gt_off, const int64_t* inter_off,
+                                     const int64_t* pixels, int64_t HW, int64_t max_d, int64_t max_g,
+                                     int64_t max_hw, int64_t total_d, int64_t total_g, int64_t total_inter,
+                                     int32_t* inter, int32_t* pa, int32_t* ga, int aligned, void* stream_ptr) {
+  PE_REQUIRE(B >= 0 && B < INT_MAX && max_d >= 0 && max_d < INT_MAX && max_g >= 0 && max_g < INT_MAX &&
+                 max_hw >= 0 && total_d >= 0 && total_g >= 0 && total_inter >= 0,
+             "invalid mask batch shapes");
+  PE_REQUIRE(pixels || HW == max_hw, "uniform batches pass HW == max_hw");
+  OverlapArgs o{};
+  o.pm_table = pm;
+  o.gm_table = gm;
+  o.det_off = reinterpret_cast<const i64*>(det_off);
+  o.gt_off = reinterpret_cast<const i64*>(gt_off);
+  o.inter_off = reinterpret_cast<const i64*>(inter_off);
+  o.pixels = reinterpret_cast<const i64*>(pixels);
+  o.HW = HW;
+  o.inter = inter;
+  o.pa = pa;
+  o.ga = ga;
+  o.aligned = aligned;
+  return launch_overlap(o, B, max_d, max_g, max_hw, total_d, total_g, total_inter,
+                        static_cast<hipStream_t>(stream_ptr));
 }
 
 extern "C" int64_t pe_detection_histogram_workspace_bytes(int64_t B, int64_t packed_pixels) {

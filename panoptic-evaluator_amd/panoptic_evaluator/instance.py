@@ -19,6 +19,8 @@ from . import _native
 _i64, _int, _ptr = ctypes.c_int64, ctypes.c_int, ctypes.c_void_p
 for _name, _sig in {
     "pe_mask_overlap": (_int, [_ptr, _i64, _ptr, _i64, _i64, _ptr, _ptr, _ptr, _ptr]),
+    "pe_mask_overlap_batch": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _i64, _i64, _i64,
+                                     _i64, _ptr, _ptr, _ptr, _int, _ptr]),
     "pe_detection_histogram_workspace_bytes": (_i64, [_i64, _i64]),
     "pe_detection_histogram": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _i64, _ptr, _int,
                                       _ptr, _i64, _ptr]),
@@ -199,51 +201,61 @@ class InstanceEvaluator:
         stream = _native.stream(device)
         ranked = score_order(predictions, device)
         det_cls, scores, det_cls32 = ranked.classes, ranked.scores, ranked.classes32
-        # Intersections and pixel areas per image (one MFMA launch each),
-        # in input (mask) order.
-        inters, det_areas, gt_areas = [], [], []
-        for p, g in zip(predictions, targets):
-            D, G = p.masks.shape[0], g.masks.shape[0]
-            HW = p.masks.shape[1] * p.masks.shape[2]
-            inter = torch.empty(D * G, dtype=torch.int32, device=device)
-            pa = torch.empty(D, dtype=torch.int32, device=device)
-            ga = torch.empty(G, dtype=torch.int32, device=device)
-            _native.call("pe_mask_overlap", p.masks.data_ptr(), D, g.masks.data_ptr(), G, HW, inter.data_ptr(),
-                         pa.data_ptr(), ga.data_ptr(), stream)
-            inters.append(inter)
-            det_areas.append(pa)
-            gt_areas.append(ga.double())
+        # Intersections and pixel areas of every image in one MFMA launch, in
+        # input (mask) order.
+        det_counts = ranked.counts
+        gt_counts = [g.masks.shape[0] for g in targets]
+        pixels = [p.masks.shape[1] * p.masks.shape[2] for p in predictions]
+        inter_counts = [d * g for d, g in zip(det_counts, gt_counts)]
+        N, total_g, total_inter = sum(det_counts), sum(gt_counts), sum(inter_counts)
+        gt_off, inter_off = offsets(gt_counts, device), offsets(inter_counts, device)
+        uniform = all(n == pixels[0] for n in pixels)
+        inter = torch.empty(total_inter, dtype=torch.int32, device=device)
+        pa = torch.empty(N, dtype=torch.int32, device=device)
+        ga = torch.empty(total_g, dtype=torch.int32, device=device)
+        aligned = int(all(n % 16 == 0 for n in pixels)
+                      and all(x.masks.data_ptr() % 16 == 0 for x in predictions + targets))
+        _native.call("pe_mask_overlap_batch", len(predictions),
+                     _native.device_table([p.masks.data_ptr() for p in predictions], device).data_ptr(),
+                     _native.device_table([g.masks.data_ptr() for g in targets], device).data_ptr(),
+                     ranked.det_off.data_ptr(), gt_off.data_ptr(), inter_off.data_ptr(),
+                     None if uniform else _native.device_table(pixels, device).data_ptr(),
+                     pixels[0] if pixels else 0, max(det_counts, default=0), max(gt_counts, default=0),
+                     max(pixels, default=0), N, total_g, total_inter, inter.data_ptr(), pa.data_ptr(),
+                     ga.data_ptr(), aligned, stream)
         # Detection pixel areas in score order.
         first, base = 0, []
         for count in ranked.counts:
             base.extend([first] * count)
             first += count
-        det_area = _cat(det_areas, torch.float64, device)[ranked.order + _native.device_table(base, device)]
+        det_area = pa.double()[ranked.order + _native.device_table(base, device)]
         gt_cls = _cat([g.classes for g in targets], torch.int64, device)
-        gt_area = _cat(gt_areas, torch.float64, device)
-        ann = _cat([g.annotation_areas if g.annotation_areas is not None else a
-                    for g, a in zip(targets, gt_areas)], torch.float64, device)
+        gt_area = ga.double()
+        # Annotation areas drive the area ranges; images without them use the
+        # pixel areas (the matcher reads gt_area when none are given at all).
+        ann = None
+        if any(g.annotation_areas is not None for g in targets):
+            parts, start = [], 0
+            for g, count in zip(targets, gt_counts):
+                parts.append(g.annotation_areas if g.annotation_areas is not None else gt_area[start:start + count])
+                start += count
+            ann = _cat(parts, torch.float64, device)
         if self.validate:
             _native.call("pe_instance_checks", det_cls.numel(), det_cls.data_ptr(), scores.data_ptr(),
-                         det_area.data_ptr(), gt_cls.numel(), gt_cls.data_ptr(), gt_area.data_ptr(), ann.data_ptr(),
+                         det_area.data_ptr(), gt_cls.numel(), gt_cls.data_ptr(), gt_area.data_ptr(), _native.ptr(ann),
                          C, None, 0, None, self._error.data_ptr(), stream)
             code = int(self._error.item())
             if code:
                 raise_instance_error(code)
-        det_counts = ranked.counts
-        gt_counts = [g.masks.shape[0] for g in targets]
-        det_off, gt_off = ranked.det_off, offsets(gt_counts, device)
-        inter_off = offsets([d * g for d, g in zip(det_counts, gt_counts)], device)
-        inter = _cat(inters, torch.int32, device)
+        det_off = ranked.det_off
         det_row = ranked.order  # intersection rows are in mask order
-        N = det_cls.numel()
         status = torch.empty((N, 40), dtype=torch.int8, device=device)
         ranks = torch.empty(N, dtype=torch.int32, device=device)
         gt_cls32 = gt_cls.int()
         crowd = _cat([g.crowd for g in targets], torch.bool, device)
         _native.call("pe_coco_match", len(predictions), C, det_off.data_ptr(), gt_off.data_ptr(),
                      inter_off.data_ptr(), det_cls32.data_ptr(), det_area.data_ptr(), det_row.data_ptr(),
-                     gt_cls32.data_ptr(), gt_area.data_ptr(), crowd.data_ptr(), ann.data_ptr(), inter.data_ptr(), 4,
+                     gt_cls32.data_ptr(), gt_area.data_ptr(), crowd.data_ptr(), _native.ptr(ann), inter.data_ptr(), 4,
                      N, max(gt_counts, default=0), status.data_ptr(), ranks.data_ptr(), self._counts.data_ptr(),
                      None, None, None, 0, stream)
         self._commit((scores, det_cls32, ranks, status))

@@ -391,3 +391,31 @@ def test_ap_accumulation_paths(pe, N):
     assert_exact(got["recall"], recall, "recall")
     for key in SUMMARY:
         assert_close(got[key], expected[key], what=key)
+
+
+def test_instance_batch_of_uneven_images(pe):
+    """One update over images of different sizes (odd, not multiples of 16),
+    with an image without detections and one without ground truth: the single
+    batched MFMA launch against the oracle."""
+    rng = np.random.default_rng(5)
+    sizes = [(24, 37), (40, 64), (17, 29), (33, 50)]
+    dets, gts = [9, 0, 21, 6], [6, 4, 0, 11]
+    pm, gm, pcl, gcl, sc, cr = [], [], [], [], [], []
+    for (h, w), d, g in zip(sizes, dets, gts):
+        gmask = rng.random((g, h, w)) > 0.6
+        pmask = rng.random((d, h, w)) > 0.6
+        if d and g:
+            pmask[0] = gmask[0]
+        gm.append(gmask)
+        pm.append(pmask)
+        gcl.append(rng.integers(0, 3, g))
+        pcl.append(rng.integers(0, 3, d))
+        sc.append(rng.choice([0.2, 0.6, 0.9], d))
+        cr.append(rng.random(g) < 0.2)
+    ev = pe.InstanceEvaluator(3)
+    ev.update(pe.InstanceBatch([cuda(m) for m in pm], [cuda(c) for c in pcl], [cuda(s) for s in sc]),
+              pe.InstanceBatch([cuda(m) for m in gm], [cuda(c) for c in gcl], crowd=[cuda(c) for c in cr]))
+    state = oinst.InstanceState(3)
+    state.update([dict(masks=m, classes=c, scores=s) for m, c, s in zip(pm, pcl, sc)],
+                 [dict(masks=m, classes=c, crowd=k) for m, c, k in zip(gm, gcl, cr)])
+    compare_instance(to_numpy(ev.compute()), state.compute())
