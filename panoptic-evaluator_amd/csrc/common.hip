@@ -225,10 +225,15 @@ namespace {
 const char* const kDetVariants[] = {"1 chunk x 4 masks", "1 chunk x 8 masks", "2 chunks x 2 masks",
                                     "2 chunks x 4 masks", "4 chunks x 2 masks"};
 const char* const kJointVariants[] = {"auto chunk", "8192 px", "16384 px", "32768 px", "65532 px"};
-const char* const kOverlapVariants[] = {"8 GT tiles x 16384 px", "4 GT tiles x 16384 px", "8 GT tiles x 8192 px",
-                                        "8 GT tiles x 32768 px", "4 GT tiles x 8192 px, 2 steps ahead",
-                                        "4 GT tiles x 8192 px, 4 steps ahead", "4 GT tiles x 16384 px, 4 steps ahead",
-                                        "4 GT tiles x 32768 px, 4 steps ahead"};
+const char* const kOverlapVariants[] = {
+    "LDS-staged 128 dets x 64 GT x 8192 px, 256 px blocks",
+    "LDS-staged 128 dets x 64 GT x 4096 px, 256 px blocks",
+    "LDS-staged 64 dets x 64 GT x 8192 px, 256 px blocks",
+    "LDS-staged 128 dets x 32 GT x 8192 px, 512 px blocks",
+    "registers 16 dets x 64 GT x 4096 px, 4 steps ahead, XCD-grouped",
+    "registers 16 dets x 64 GT x 8192 px, 4 steps ahead, XCD-grouped",
+    "registers 16 dets x 128 GT x 16384 px, XCD-grouped",
+    "registers 16 dets x 64 GT x 8192 px, 4 steps ahead, spread"};
 const char* const kGridVariants[] = {"one workgroup per work item", "persistent: 8 workgroups per CU",
                                      "persistent: 4 workgroups per CU"};
 const char* const kGroupVariants[] = {"LDS budget (96 KB)", "128 detections", "64 detections", "32 detections"};

@@ -74,35 +74,93 @@ struct OverlapArgs {
   int D, G;
   i64 HW;
   int chunks;             // pixel chunks of the largest image
+  int dgroups, ggroups;   // work items: dgroups x ggroups x (images * chunks)
+  int items;
+  int xcd_span;           // > 0: work items per XCD (grid padded to kXcds * xcd_span)
   int32_t* inter;
   int32_t* pa;            // may be null
   int32_t* ga;            // may be null
   int aligned;
 };
 
+// XCDs of a gfx950 device; the dispatcher deals workgroup i to XCD i % kXcds.
+constexpr int kXcds = 8;
+
+// One work item of an overlap launch: detection group (fastest), GT group,
+// then image * chunks + chunk, so the workgroups reading one chunk's GT masks
+// are neighbours.  With xcd_span > 0 neighbouring items run on one XCD
+// (workgroup i takes item (i % kXcds) * xcd_span + i / kXcds) and share its
+// L2; otherwise the dispatcher spreads them over every XCD.
+struct OverlapItem {
+  int dgroup, ggroup, chunk;
+  int D, G;
+  i64 HW;
+  const uint8_t* pm;
+  const uint8_t* gm;
+  int32_t* inter;
+  int32_t* pa;
+  int32_t* ga;
+};
+
+__device__ __forceinline__ bool overlap_item(const OverlapArgs& o, int dets_per_group, int gts_per_group,
+                                             i64 chunk_px, OverlapItem& it) {
+  const int wg = static_cast<int>(blockIdx.x);
+  const int item = o.xcd_span > 0 ? (wg % kXcds) * o.xcd_span + wg / kXcds : wg;
+  if (item >= o.items) return false;
+  it.dgroup = item % o.dgroups;
+  const int rest = item / o.dgroups;
+  it.ggroup = rest % o.ggroups;
+  const int z = rest / o.ggroups;
+  const int b = z / o.chunks;
+  it.chunk = z - b * o.chunks;
+  const bool table = o.pm_table != nullptr;
+  const i64 d_first = table ? o.det_off[b] : 0, g_first = table ? o.gt_off[b] : 0;
+  it.D = table ? static_cast<int>(o.det_off[b + 1] - d_first) : o.D;
+  it.G = table ? static_cast<int>(o.gt_off[b + 1] - g_first) : o.G;
+  it.HW = o.pixels ? o.pixels[b] : o.HW;
+  // With D == 0 or G == 0 the first group of the other side still counts areas.
+  if (it.chunk * chunk_px >= it.HW || it.dgroup * dets_per_group >= max(it.D, 1) ||
+      it.ggroup * gts_per_group >= max(it.G, 1))
+    return false;
+  it.pm = table ? o.pm_table[b] : o.pm;
+  it.gm = table ? o.gm_table[b] : o.gm;
+  it.inter = o.inter + (table ? o.inter_off[b] : 0);
+  it.pa = o.pa ? o.pa + d_first : nullptr;
+  it.ga = o.ga ? o.ga + g_first : nullptr;
+  return true;
+}
+
+// inter[d0 + i, g0 + j] += one 16 x 16 MFMA result tile; lane holds rows
+// 4 * (lane / 16) + r, column lane % 16.
+__device__ __forceinline__ void add_overlap_tile(v4i acc, int lane, int d0, int g0, int D, int G,
+                                                 int32_t* inter) {
+  const int g = g0 + (lane & 15);
+  if (g >= G) return;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int d = d0 + 4 * (lane >> 4) + r;
+    const int v = acc[r];
+    if (d < D && v) atomicAdd(inter + static_cast<i64>(d) * G + g, v);
+  }
+}
+
 // inter[d, g] += popcount(pm[d] & gm[g]) over one wave's pixel chunk, for a
 // 16-detection tile and up to kOverlapGTiles 16-GT tiles (kOverlapChunk
-// pixels per wave, a multiple of 64).  Grid: x = detection tile (fastest, so
-// the waves reading one chunk's GT tiles run together and share them in L2),
-// y = GT tile group, z = image * chunks + chunk.
+// pixels per wave, a multiple of 64), loaded straight into MFMA operands.
 template <int kOverlapGTiles, int kOverlapChunk, int kOverlapAhead>
 __global__ __launch_bounds__(64) void mask_overlap_mfma(OverlapArgs o) {
   const int lane = threadIdx.x;
-  const int dtile = blockIdx.x, ggroup = blockIdx.y;
-  const int b = blockIdx.z / o.chunks, chunk = blockIdx.z - b * o.chunks;
-  const bool table = o.pm_table != nullptr;
-  const i64 d_first = table ? o.det_off[b] : 0, g_first = table ? o.gt_off[b] : 0;
-  const int D = table ? static_cast<int>(o.det_off[b + 1] - d_first) : o.D;
-  const int G = table ? static_cast<int>(o.gt_off[b + 1] - g_first) : o.G;
-  const i64 HW = o.pixels ? o.pixels[b] : o.HW;
-  const i64 k_begin = static_cast<i64>(chunk) * kOverlapChunk;
-  // With D == 0 or G == 0 the first tile of the other side still counts areas.
-  if (k_begin >= HW || dtile * 16 >= max(D, 1) || ggroup * kOverlapGTiles * 16 >= max(G, 1)) return;
-  const uint8_t* pm = table ? o.pm_table[b] : o.pm;
-  const uint8_t* gm = table ? o.gm_table[b] : o.gm;
-  int32_t* inter = o.inter + (table ? o.inter_off[b] : 0);
-  int32_t* pa = o.pa ? o.pa + d_first : nullptr;
-  int32_t* ga = o.ga ? o.ga + g_first : nullptr;
+  OverlapItem it;
+  if (!overlap_item(o, 16, 16 * kOverlapGTiles, kOverlapChunk, it)) return;
+  const int dtile = it.dgroup, ggroup = it.ggroup;
+  const int D = it.D, G = it.G;
+  const i64 HW = it.HW;
+  const i64 k_begin = static_cast<i64>(it.chunk) * kOverlapChunk;
+  const uint8_t* pm = it.pm;
+  const uint8_t* gm = it.gm;
+  int32_t* inter = it.inter;
+  int32_t* pa = it.pa;
+  int32_t* ga = it.ga;
   const int row = lane & 15, kgroup = lane >> 4;
   const int d = dtile * 16 + row;
   const bool dvalid = d < D;
@@ -150,18 +208,10 @@ __global__ __launch_bounds__(64) void mask_overlap_mfma(OverlapArgs o) {
       }
     }
   }
-  // D fragment: lane holds rows 4*(lane/16) + r, column lane % 16.
 #pragma unroll
   for (int t = 0; t < kOverlapGTiles; ++t) {
     if (t >= tiles) break;
-    const int g = (ggroup * kOverlapGTiles + t) * 16 + (lane & 15);
-    if (g >= G) continue;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int dd = dtile * 16 + 4 * (lane >> 4) + r;
-      const int v = acc[t][r];
-      if (dd < D && v) atomicAdd(inter + static_cast<i64>(dd) * G + g, v);
-    }
+    add_overlap_tile(acc[t], lane, dtile * 16, (ggroup * kOverlapGTiles + t) * 16, D, G, inter);
   }
   // Areas: the four lanes of a row (kgroups) hold disjoint pixels.
   if (count_a) {
@@ -182,17 +232,130 @@ __global__ __launch_bounds__(64) void mask_overlap_mfma(OverlapArgs o) {
   }
 }
 
+// The same contraction with both operands staged in LDS: a workgroup of DT
+// waves covers 16 * DT detections x 16 * GT GT masks over a Chunk-pixel
+// range, in blocks of KB pixels.  Every row of a block is loaded as one
+// contiguous KB-byte run (16 bytes per lane, neighbouring lanes on
+// neighbouring bytes), so a load instruction touches few rows (pages, L2
+// channels) and each mask byte is fetched once per workgroup; wave w then
+// multiplies detection tile w by every GT tile out of LDS.  Rows are padded
+// by 16 bytes so that the 16 rows of an MFMA operand fall in distinct banks.
+// The next block's loads are in flight while the current one is multiplied.
+template <int DT, int GT, int KB, int Chunk>
+__global__ __launch_bounds__(64 * DT) void mask_overlap_staged(OverlapArgs o) {
+  constexpr int kThreads = 64 * DT;
+  constexpr int kRows = 16 * (DT + GT);
+  constexpr int kStride = KB + 16;
+  constexpr int kPerRow = KB / 16;  // 16-byte pieces of one row's block
+  constexpr int kSlots = kRows * kPerRow / kThreads;
+  static_assert(KB % 64 == 0 && Chunk % KB == 0, "blocks are whole MFMA steps");
+  static_assert(kThreads % kPerRow == 0 && kPerRow <= kWave && (kPerRow & (kPerRow - 1)) == 0,
+                "the pieces of one row sit in one wave");
+  static_assert(kRows * kPerRow % kThreads == 0, "every thread loads the same number of pieces");
+  __shared__ __align__(16) uint8_t tile[kRows * kStride];
+  OverlapItem it;
+  if (!overlap_item(o, 16 * DT, 16 * GT, Chunk, it)) return;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const int D = it.D, G = it.G;
+  const int d_base = it.dgroup * 16 * DT, g_base = it.ggroup * 16 * GT;
+  const i64 HW = it.HW;
+  const i64 k_begin = static_cast<i64>(it.chunk) * Chunk;
+  const i64 k_end = min(k_begin + Chunk, HW);
+  const bool al = o.aligned != 0;
+  // Slot i of this thread: piece tid + i * kThreads of the block, i.e. row
+  // (tid + i * kThreads) / kPerRow at byte (tid % kPerRow) * 16.
+  const int col = (tid % kPerRow) * 16;
+  const uint8_t* rowp[kSlots];
+  bool rvalid[kSlots];
+  int lds[kSlots];
+  int area[kSlots];
+#pragma unroll
+  for (int i = 0; i < kSlots; ++i) {
+    const int r = (tid + i * kThreads) / kPerRow;
+    const bool det = r < 16 * DT;
+    const int idx = det ? d_base + r : g_base + r - 16 * DT;
+    rvalid[i] = idx < (det ? D : G);
+    rowp[i] = (det ? it.pm : it.gm) + static_cast<i64>(rvalid[i] ? idx : 0) * HW;
+    lds[i] = r * kStride + col;
+    area[i] = 0;
+  }
+  int tiles = G - g_base;
+  tiles = tiles <= 0 ? 0 : min(GT, (tiles + 15) / 16);
+  const bool compute = d_base + wave * 16 < D;
+  const int row = lane & 15, kgroup = lane >> 4;
+  const uint8_t* a_row = tile + (wave * 16 + row) * kStride + kgroup * 16;
+  const uint8_t* b_row = tile + (16 * DT + row) * kStride + kgroup * 16;
+  v4i acc[GT];
+#pragma unroll
+  for (int t = 0; t < GT; ++t) acc[t] = v4i{0, 0, 0, 0};
+  v4i buf[kSlots];
+#pragma unroll
+  for (int i = 0; i < kSlots; ++i) buf[i] = load16(rowp[i], k_begin + col, k_end, rvalid[i], al);
+  for (i64 k = k_begin; k < k_end; k += KB) {
+    __syncthreads();  // the previous block has been multiplied
+#pragma unroll
+    for (int i = 0; i < kSlots; ++i) {
+      *reinterpret_cast<v4i*>(tile + lds[i]) = buf[i];
+      area[i] += byte_sum4(buf[i].x) + byte_sum4(buf[i].y) + byte_sum4(buf[i].z) + byte_sum4(buf[i].w);
+    }
+    __syncthreads();
+    if (k + KB < k_end) {
+#pragma unroll
+      for (int i = 0; i < kSlots; ++i) buf[i] = load16(rowp[i], k + KB + col, k_end, rvalid[i], al);
+    }
+    if (compute) {
+#pragma unroll
+      for (int step = 0; step < KB / 64; ++step) {
+        const v4i a = *reinterpret_cast<const v4i*>(a_row + step * 64);
+#pragma unroll
+        for (int t = 0; t < GT; ++t) {
+          if (t >= tiles) break;
+          const v4i b = *reinterpret_cast<const v4i*>(b_row + t * 16 * kStride + step * 64);
+          acc[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc[t], 0, 0, 0);
+        }
+      }
+    }
+  }
+  if (compute) {
+#pragma unroll
+    for (int t = 0; t < GT; ++t) {
+      if (t >= tiles) break;
+      add_overlap_tile(acc[t], lane, d_base + wave * 16, g_base + t * 16, D, G, it.inter);
+    }
+  }
+  // Areas: the kPerRow neighbouring lanes of a slot share its row.
+  const bool count_a = it.pa != nullptr && it.ggroup == 0;
+  const bool count_b = it.ga != nullptr && it.dgroup == 0;
+#pragma unroll
+  for (int i = 0; i < kSlots; ++i) {
+    int sum = area[i];
+#pragma unroll
+    for (int m = 1; m < kPerRow; m <<= 1) sum += __shfl_xor(sum, m, kWave);
+    const int r = (tid + i * kThreads) / kPerRow;
+    const bool det = r < 16 * DT;
+    if (col == 0 && rvalid[i] && sum && (det ? count_a : count_b))
+      atomicAdd(det ? it.pa + d_base + r : it.ga + g_base + r - 16 * DT, sum);
+  }
+}
+
 struct OverlapVariant {
-  int gtiles;
-  int chunk;
+  int dets;     // detections per work item
+  int gts;      // GT masks per work item
+  int chunk;    // pixels per work item
+  int threads;
+  bool xcd;
   void (*kernel)(OverlapArgs);
 };
 // Order matches the tuning registry's descriptions.
 const OverlapVariant kOverlap[] = {
-    {8, 16384, mask_overlap_mfma<8, 16384, 1>}, {4, 16384, mask_overlap_mfma<4, 16384, 1>},
-    {8, 8192, mask_overlap_mfma<8, 8192, 1>},   {8, 32768, mask_overlap_mfma<8, 32768, 1>},
-    {4, 8192, mask_overlap_mfma<4, 8192, 2>},   {4, 8192, mask_overlap_mfma<4, 8192, 4>},
-    {4, 16384, mask_overlap_mfma<4, 16384, 4>}, {4, 32768, mask_overlap_mfma<4, 32768, 4>},
+    {128, 64, 8192, 512, true, mask_overlap_staged<8, 4, 256, 8192>},
+    {128, 64, 4096, 512, true, mask_overlap_staged<8, 4, 256, 4096>},
+    {64, 64, 8192, 256, true, mask_overlap_staged<4, 4, 256, 8192>},
+    {128, 32, 8192, 512, true, mask_overlap_staged<8, 2, 512, 8192>},
+    {16, 64, 4096, 64, true, mask_overlap_mfma<4, 4096, 4>},
+    {16, 64, 8192, 64, true, mask_overlap_mfma<4, 8192, 4>},
+    {16, 128, 16384, 64, true, mask_overlap_mfma<8, 16384, 1>},
+    {16, 64, 8192, 64, false, mask_overlap_mfma<4, 8192, 4>},
 };
 
 int launch_overlap(OverlapArgs o, int64_t B, int64_t max_d, int64_t max_g, int64_t max_hw, int64_t total_d,
@@ -204,14 +367,18 @@ int launch_overlap(OverlapArgs o, int64_t B, int64_t max_d, int64_t max_g, int64
   if (B == 0 || max_hw == 0 || (max_d == 0 && max_g == 0)) return kOk;
   const OverlapVariant& v = kOverlap[tuned_variant(kTuneMaskOverlap)];
   const i64 chunks = (max_hw + v.chunk - 1) / v.chunk;
-  const i64 dtiles = std::max<i64>(1, (max_d + 15) / 16);
-  const i64 ggroups = std::max<i64>(1, (max_g + 16 * v.gtiles - 1) / (16 * v.gtiles));
-  if (dtiles >= INT_MAX || ggroups >= 65536 || B * chunks >= 65536)
-    return fail(kUnsupportedShape, "too many masks or pixels for one overlap launch");
+  const i64 dgroups = std::max<i64>(1, (max_d + v.dets - 1) / v.dets);
+  const i64 ggroups = std::max<i64>(1, (max_g + v.gts - 1) / v.gts);
+  const i64 items = dgroups * ggroups * B * chunks;
+  if (items >= INT_MAX - kXcds) return fail(kUnsupportedShape, "too many masks or pixels for one overlap launch");
   o.chunks = static_cast<int>(chunks);
-  dim3 grid(static_cast<unsigned>(dtiles), static_cast<unsigned>(ggroups), static_cast<unsigned>(B * chunks));
+  o.dgroups = static_cast<int>(dgroups);
+  o.ggroups = static_cast<int>(ggroups);
+  o.items = static_cast<int>(items);
+  o.xcd_span = v.xcd ? static_cast<int>((items + kXcds - 1) / kXcds) : 0;
+  const i64 grid = v.xcd ? static_cast<i64>(o.xcd_span) * kXcds : items;
   ProfileScope timing("mask_overlap_mfma", stream);
-  v.kernel<<<grid, 64, 0, stream>>>(o);
+  v.kernel<<<static_cast<unsigned>(grid), v.threads, 0, stream>>>(o);
   PE_LAUNCH_CHECK();
   return kOk;
 }

@@ -77,7 +77,7 @@ def test_detection_histogram_variants(pe, tuning, H, W):
             assert_exact(result["instance"][key], expected["instance"][key], f"{key} variant {variant}")
 
 
-@pytest.mark.parametrize("D,G,H,W", [(17, 140, 24, 37), (33, 20, 160, 300)])
+@pytest.mark.parametrize("D,G,H,W", [(17, 140, 24, 37), (33, 20, 160, 300), (150, 70, 40, 52), (0, 9, 30, 31)])
 def test_mask_overlap_variants(pe, tuning, D, G, H, W):
     from panoptic_evaluator import _native
     rng = np.random.default_rng(D + G)
