@@ -62,6 +62,11 @@ int64_t pe_update_workspace_bytes(int64_t B, int64_t G, int64_t P, int64_t C, in
  *   gt, pred        B x H x W maps; element size map_bytes 4 (int32) or 8
  *                   (int64).  Compact slots, or original IDs when the
  *                   matching ID/slot lookup tables (B x G / B x P) are given.
+ *                   map_bytes 3: decoded COCO panoptic PNGs, B x H x W x 3
+ *                   uint8 whose ID is R + 256 G + 65536 B (README.md:67-68),
+ *                   decoded in the kernel; needs the lookup table and pairs
+ *                   with an RGB or int32 map (no reference counterpart: its
+ *                   callers decode on the host).
  *   pix_off         NULL, or B + 1 device pixel offsets of a packed batch of
  *                   uneven images stored one after another (image b covers
  *                   pix_off[b] .. pix_off[b + 1]; then H * W is the pixel
@@ -120,6 +125,7 @@ int pe_image_pq(const int64_t* tp, const int64_t* fp, const int64_t* fn, const d
 
 /* Original segment IDs -> compact int32 slots (0 on failure, error |= 1).
  * Replaces ``pack`` (bindings.cpp:12; pack_cuda at histogram_kernel.cu:343-366).
+ * maps: map_bytes 4 / 8 integer IDs, or 3: RGB-encoded IDs as in pe_update.
  * ids: B x S sorted int64 (slot 0 = id 0), slots: B x S int32. */
 int pe_pack(const void* maps, int map_bytes, int64_t B, int64_t HW,
             const int64_t* ids, const int32_t* slots, int64_t S,

@@ -49,6 +49,20 @@ def from_coco(maps, segments_info, category_ids):
             np.array(classes, np.int32).reshape(shape), np.array(crowd, bool).reshape(shape))
 
 
+def rgb_to_id(rgb):
+    """Decoded COCO panoptic PNG (... x 3 uint8) -> segment IDs: R + 256 * G
+    + 65536 * B (README.md:67-68)."""
+    rgb = np.asarray(rgb).astype(np.int64)
+    return rgb[..., 0] + 256 * rgb[..., 1] + 65536 * rgb[..., 2]
+
+
+def id_to_rgb(ids):
+    """Segment IDs (< 2**24) -> the RGB bytes a COCO panoptic PNG stores; the
+    encoder of the reference's tests (test_evaluator.py:39)."""
+    ids = np.asarray(ids).astype(np.int64)
+    return np.stack((ids % 256, ids // 256 % 256, ids // 65536 % 256), -1).astype(np.uint8)
+
+
 def compact(maps, lookup_ids, lookup_slots):
     """Original IDs -> compact slots (evaluator.py:147-157, CPU branch).
 

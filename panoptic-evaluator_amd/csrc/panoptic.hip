@@ -30,6 +30,70 @@ namespace pe {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
+// ------------------------------------------------------------- map pixels
+
+// A decoded COCO panoptic PNG pixel: bytes R, G, B whose segment ID is
+// R + 256 * G + 65536 * B (README.md:67-68), i.e. the little-endian 24-bit
+// integer of the three bytes.  Maps are B x H x W x 3 uint8.
+struct Rgb24 {
+  uint8_t c[3];
+};
+static_assert(sizeof(Rgb24) == 3, "an RGB pixel is three bytes");
+
+// The integer a map pixel holds (an ID or a compact slot).
+template <typename T>
+struct MapValue {
+  using type = T;
+};
+template <>
+struct MapValue<Rgb24> {
+  using type = int32_t;
+};
+
+template <typename T>
+__device__ __forceinline__ typename MapValue<T>::type read_pixel(const T* map, i64 i) {
+  return as_global(map)[i];
+}
+template <>
+__device__ __forceinline__ int32_t read_pixel<Rgb24>(const Rgb24* map, i64 i) {
+  const auto* p = as_global(reinterpret_cast<const uint8_t*>(map) + 3 * i);
+  return static_cast<int32_t>(p[0] | (p[1] << 8) | (p[2] << 16));
+}
+
+// Pixels 4q .. 4q + 3 with vector loads that bypass the caches (maps are read
+// once): int32 one 16-byte load, int64 two, RGB three dwords (12 bytes).  The
+// map base must be 16-byte aligned (4-byte for RGB).
+__device__ __forceinline__ void load_quad(const int32_t* map, i64 q, int32_t (&v)[4]) {
+  const v4i x = __builtin_nontemporal_load(as_global(reinterpret_cast<const v4i*>(map)) + q);
+  v[0] = x.x;
+  v[1] = x.y;
+  v[2] = x.z;
+  v[3] = x.w;
+}
+__device__ __forceinline__ void load_quad(const i64* map, i64 q, i64 (&v)[4]) {
+  typedef long long v2l __attribute__((ext_vector_type(2)));
+  const auto* p = as_global(reinterpret_cast<const v2l*>(map)) + 2 * q;
+  const v2l a = __builtin_nontemporal_load(p), b = __builtin_nontemporal_load(p + 1);
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = b.x;
+  v[3] = b.y;
+}
+__device__ __forceinline__ void load_quad(const Rgb24* map, i64 q, int32_t (&v)[4]) {
+  const auto* p = as_global(reinterpret_cast<const uint32_t*>(reinterpret_cast<const uint8_t*>(map) + 12 * q));
+  const uint32_t w0 = __builtin_nontemporal_load(p), w1 = __builtin_nontemporal_load(p + 1),
+                 w2 = __builtin_nontemporal_load(p + 2);
+  v[0] = static_cast<int32_t>(w0 & 0xFFFFFFu);
+  v[1] = static_cast<int32_t>((w0 >> 24) | ((w1 & 0xFFFFu) << 8));
+  v[2] = static_cast<int32_t>((w1 >> 16) | ((w2 & 0xFFu) << 16));
+  v[3] = static_cast<int32_t>(w2 >> 8);
+}
+
+// Base alignment load_quad needs for a map of pixel width `bytes`.
+inline bool quad_aligned(const void* map, int bytes) {
+  return reinterpret_cast<uintptr_t>(map) % (bytes == 3 ? 4 : 16) == 0;
+}
+
 // ------------------------------------------------------------ slot lookup
 
 template <typename T>
@@ -56,12 +120,13 @@ __device__ __forceinline__ int lookup_slot(i64 key, const i64* ids, const int* s
 
 template <typename T, bool LOOKUP>
 struct SlotResolver {
+  using V = typename MapValue<T>::type;
   const i64* ids;
   const int* slots;
   int S;
   i64 last_key = LLONG_MIN;
   int last_slot = -1;
-  __device__ __forceinline__ int operator()(T v) {
+  __device__ __forceinline__ int operator()(V v) {
     if (!LOOKUP || ids == nullptr) return compact_slot(v, S);
     i64 key = static_cast<i64>(v);
     if (key != last_key) {
@@ -118,7 +183,9 @@ __device__ __forceinline__ void histogram_chunk(const HistArgs& a, int b, i64 be
   KeyCache<GLOBAL> cache;
   bool bad = false;
   const uint32_t P = static_cast<uint32_t>(a.P);
-  auto visit = [&](GT gv, PR pv) {
+  using GV = typename MapValue<GT>::type;
+  using PV = typename MapValue<PR>::type;
+  auto visit = [&](GV gv, PV pv) {
     int g = gslot(gv);
     int p = pslot(pv);
     if ((g | p) < 0) {
@@ -128,43 +195,41 @@ __device__ __forceinline__ void histogram_chunk(const HistArgs& a, int b, i64 be
     cache.add(static_cast<uint32_t>(g) * P + static_cast<uint32_t>(p), bins, pairs);
   };
   if (VEC) {
-    // 16-byte vectors of four int32 pixels; scalar head/tail of <= 3 pixels.
+    // Quads of four pixels (load_quad); scalar head/tail of <= 3 pixels.
     const i64 va = (begin + 3) & ~static_cast<i64>(3);
     const i64 vb = end & ~static_cast<i64>(3);
     const i64 head_end = va < end ? va : end;
-    for (i64 i = begin + tid; i < head_end; i += kHistThreads) visit(gt[i], pred[i]);
+    for (i64 i = begin + tid; i < head_end; i += kHistThreads) visit(read_pixel(gt, i), read_pixel(pred, i));
     if (va < vb) {
-      const v4i* g4 = reinterpret_cast<const v4i*>(gt + va);
-      const v4i* p4 = reinterpret_cast<const v4i*>(pred + va);
-      const i64 n4 = (vb - va) >> 2;
+      const i64 q_end = vb >> 2;
       constexpr int U = 4;
-      for (i64 base = tid; base < n4; base += static_cast<i64>(kHistThreads) * U) {
-        v4i gv[U], pv[U];
+      for (i64 base = (va >> 2) + tid; base < q_end; base += static_cast<i64>(kHistThreads) * U) {
+        GV gv[U][4];
+        PV pv[U][4];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          i64 k = base + static_cast<i64>(u) * kHistThreads;
-          if (k < n4) {
-            gv[u] = __builtin_nontemporal_load(g4 + k);
-            pv[u] = __builtin_nontemporal_load(p4 + k);
+          const i64 q = base + static_cast<i64>(u) * kHistThreads;
+          if (q < q_end) {
+            load_quad(gt, q, gv[u]);
+            load_quad(pred, q, pv[u]);
           }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          i64 k = base + static_cast<i64>(u) * kHistThreads;
-          if (k < n4) {
-            visit(static_cast<GT>(gv[u].x), static_cast<PR>(pv[u].x));
-            visit(static_cast<GT>(gv[u].y), static_cast<PR>(pv[u].y));
-            visit(static_cast<GT>(gv[u].z), static_cast<PR>(pv[u].z));
-            visit(static_cast<GT>(gv[u].w), static_cast<PR>(pv[u].w));
+          const i64 q = base + static_cast<i64>(u) * kHistThreads;
+          if (q < q_end) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) visit(gv[u][j], pv[u][j]);
           }
         }
       }
-      for (i64 i = (vb > head_end ? vb : head_end) + tid; i < end; i += kHistThreads) visit(gt[i], pred[i]);
+      for (i64 i = (vb > head_end ? vb : head_end) + tid; i < end; i += kHistThreads)
+        visit(read_pixel(gt, i), read_pixel(pred, i));
     } else {
-      for (i64 i = head_end + tid; i < end; i += kHistThreads) visit(gt[i], pred[i]);
+      for (i64 i = head_end + tid; i < end; i += kHistThreads) visit(read_pixel(gt, i), read_pixel(pred, i));
     }
   } else {
-    for (i64 i = begin + tid; i < end; i += kHistThreads) visit(gt[i], pred[i]);
+    for (i64 i = begin + tid; i < end; i += kHistThreads) visit(read_pixel(gt, i), read_pixel(pred, i));
   }
   cache.drain(bins, pairs);
   if (bad && a.error) atomicOr(a.error, 1);
@@ -307,14 +372,17 @@ int launch_joint_histogram(HistArgs a, int gt_bytes, int pred_bytes, int chunk, 
   if (global) lds = 0;
   a.lds_bytes = lds;
   bool lookup = a.gt_ids || a.pred_ids;
-  bool vec = gt_bytes == 4 && pred_bytes == 4 && (reinterpret_cast<uintptr_t>(a.gt) % 16) == 0 &&
-             (reinterpret_cast<uintptr_t>(a.pred) % 16) == 0;
+  bool vec = quad_aligned(a.gt, gt_bytes) && quad_aligned(a.pred, pred_bytes);
   int nb = static_cast<int>(blocks);
 #define PE_HIST(GT, PR)                                                                   \
   do {                                                                                    \
     return lookup ? launch_hist_typed<GT, PR, true>(a, nb, lds, vec, global, stream)     \
                   : launch_hist_typed<GT, PR, false>(a, nb, lds, vec, global, stream);   \
   } while (0)
+  // RGB maps always carry original IDs (the caller checked the lookup table).
+  if (gt_bytes == 3 && pred_bytes == 3) return launch_hist_typed<Rgb24, Rgb24, true>(a, nb, lds, vec, global, stream);
+  if (gt_bytes == 3 && pred_bytes == 4) return launch_hist_typed<Rgb24, int32_t, true>(a, nb, lds, vec, global, stream);
+  if (gt_bytes == 4 && pred_bytes == 3) return launch_hist_typed<int32_t, Rgb24, true>(a, nb, lds, vec, global, stream);
   if (gt_bytes == 4 && pred_bytes == 4) PE_HIST(int32_t, int32_t);
   if (gt_bytes == 4 && pred_bytes == 8) PE_HIST(int32_t, int64_t);
   if (gt_bytes == 8 && pred_bytes == 4) PE_HIST(int64_t, int32_t);
@@ -1006,7 +1074,7 @@ __global__ __launch_bounds__(256) void image_pq(const i64* tp, const i64* fp, co
 
 // ---------------------------------------------------------------- packing
 
-template <typename T>
+template <typename T, bool VEC>
 __global__ __launch_bounds__(256) void pack_slots(const T* maps, i64 HW, const i64* ids, const int* slots,
                                                   int S, int blocks_per_image, int32_t* out, int* error) {
   const int b = blockIdx.x / blocks_per_image;
@@ -1016,10 +1084,24 @@ __global__ __launch_bounds__(256) void pack_slots(const T* maps, i64 HW, const i
   const i64 end = static_cast<i64>(b) * HW + min(HW, (part + 1) * span);
   SlotResolver<T, true> resolve{ids + static_cast<i64>(b) * S, slots + static_cast<i64>(b) * S, S};
   bool bad = false;
-  for (i64 i = begin + threadIdx.x; i < end; i += blockDim.x) {
-    int s = resolve(maps[i]);
+  auto put = [&](i64 i, typename MapValue<T>::type v) {
+    const int s = resolve(v);
     out[i] = s >= 0 ? s : 0;
     bad |= s < 0;
+  };
+  if (VEC) {
+    // Whole quads with vector loads; the scalar head/tail has <= 3 pixels.
+    const i64 va = min((begin + 3) & ~static_cast<i64>(3), end), vb = max(end & ~static_cast<i64>(3), va);
+    for (i64 i = begin + threadIdx.x; i < va; i += blockDim.x) put(i, read_pixel(maps, i));
+    for (i64 q = (va >> 2) + threadIdx.x; q < (vb >> 2); q += blockDim.x) {
+      typename MapValue<T>::type v[4];
+      load_quad(maps, q, v);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) put(4 * q + j, v[j]);
+    }
+    for (i64 i = vb + threadIdx.x; i < end; i += blockDim.x) put(i, read_pixel(maps, i));
+  } else {
+    for (i64 i = begin + threadIdx.x; i < end; i += blockDim.x) put(i, read_pixel(maps, i));
   }
   if (bad) atomicOr(error, 1);
 }
@@ -1048,9 +1130,13 @@ extern "C" int pe_update(const void* gt, int gt_bytes, const void* pred, int pre
   PE_REQUIRE(B >= 0 && H >= 0 && W >= 0, "negative shape");
   PE_REQUIRE(G > 0 && G < INT_MAX && P > 0 && P < INT_MAX && C > 0 && C < INT_MAX, "invalid capacities");
   PE_REQUIRE(B < INT_MAX && G * P < (1LL << 32), "pair histogram too large");
-  PE_REQUIRE((gt_bytes == 4 || gt_bytes == 8) && (pred_bytes == 4 || pred_bytes == 8), "maps must be int32/int64");
+  auto width_ok = [](int bytes) { return bytes == 3 || bytes == 4 || bytes == 8; };
+  PE_REQUIRE(width_ok(gt_bytes) && width_ok(pred_bytes), "maps must be int32/int64 or RGB uint8 x 3");
   PE_REQUIRE((gt_ids == nullptr) == (gt_slots == nullptr) && (pred_ids == nullptr) == (pred_slots == nullptr),
              "lookup IDs and slots must be supplied together");
+  PE_REQUIRE((gt_bytes != 3 || gt_ids) && (pred_bytes != 3 || pred_ids), "RGB maps need their ID lookup table");
+  PE_REQUIRE(!(gt_bytes == 3 && pred_bytes == 8) && !(gt_bytes == 8 && pred_bytes == 3),
+             "an RGB map pairs with an RGB or int32 map");
   const i64 packed_pixels = pix_off ? H * W : 0;
   PE_REQUIRE(workspace_bytes >= pe_update_workspace_bytes(B, G, P, C, packed_pixels), "workspace too small");
   PE_REQUIRE(!validate || error, "validation needs the error word");
@@ -1144,7 +1230,7 @@ extern "C" int pe_compute(const int64_t* tp, const int64_t* fp, const int64_t* f
 
 extern "C" int pe_pack(const void* maps, int map_bytes, int64_t B, int64_t HW, const int64_t* ids,
                        const int32_t* slots, int64_t S, int32_t* out, int32_t* error, void* stream_ptr) {
-  PE_REQUIRE(map_bytes == 4 || map_bytes == 8, "maps must be int32/int64");
+  PE_REQUIRE(map_bytes == 3 || map_bytes == 4 || map_bytes == 8, "maps must be int32/int64 or RGB uint8 x 3");
   PE_REQUIRE(S > 0 && S < INT_MAX && B >= 0 && HW >= 0, "invalid lookup shapes");
   hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
   int rc = launch_zero(error, 4, stream);
@@ -1152,12 +1238,14 @@ extern "C" int pe_pack(const void* maps, int map_bytes, int64_t B, int64_t HW, c
   i64 per_image = std::max<i64>(1, std::min<i64>((HW + 8191) / 8192, (4LL * compute_units() + B - 1) / B));
   PE_REQUIRE(B * per_image < INT_MAX, "too many images");
   int blocks = static_cast<int>(B * per_image);
-  if (map_bytes == 4)
-    pack_slots<int32_t><<<blocks, 256, 0, stream>>>(static_cast<const int32_t*>(maps), HW, ids, slots,
-                                                    static_cast<int>(S), static_cast<int>(per_image), out, error);
-  else
-    pack_slots<int64_t><<<blocks, 256, 0, stream>>>(static_cast<const int64_t*>(maps), HW, ids, slots,
-                                                    static_cast<int>(S), static_cast<int>(per_image), out, error);
+  const bool vec = quad_aligned(maps, map_bytes);
+#define PE_PACK(T)                                                                                          \
+  (vec ? pack_slots<T, true> : pack_slots<T, false>)<<<blocks, 256, 0, stream>>>(                           \
+      static_cast<const T*>(maps), HW, ids, slots, static_cast<int>(S), static_cast<int>(per_image), out, error)
+  if (map_bytes == 3) PE_PACK(Rgb24);
+  else if (map_bytes == 4) PE_PACK(int32_t);
+  else PE_PACK(int64_t);
+#undef PE_PACK
   PE_LAUNCH_CHECK();
   return kOk;
 }

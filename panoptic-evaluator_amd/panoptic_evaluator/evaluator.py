@@ -25,6 +25,22 @@ MSG_CLASSES = "classes must be -1 or valid category indices; slot 0 must be void
 MSG_ACTIVE = "nonvoid map slots and active metadata must correspond exactly"
 
 
+def is_rgb(maps: torch.Tensor) -> bool:
+    """Decoded COCO panoptic PNGs: B x H x W x 3 uint8, segment ID
+    R + 256 * G + 65536 * B (README.md:67-68)."""
+    return maps.dtype == torch.uint8 and maps.ndim == 4 and maps.shape[-1] == 3
+
+
+def id_map_ok(maps: torch.Tensor) -> bool:
+    """Original-ID maps: int32/int64 B x H x W, or RGB B x H x W x 3 uint8."""
+    return (maps.ndim == 3 and maps.dtype in (torch.int32, torch.int64)) or is_rgb(maps)
+
+
+def map_bytes(maps: torch.Tensor) -> int:
+    """Bytes per pixel as the kernels read them (3 for RGB)."""
+    return 3 if is_rgb(maps) else maps.element_size()
+
+
 def raise_validation(code: int):
     """Map the kernels' validation bitmask to the reference's messages
     (evaluator.py:365-376)."""
@@ -39,7 +55,8 @@ def raise_validation(code: int):
 class PanopticBatch:
     """Segment maps and metadata; optional lookup tables allow original IDs.
 
-    segment_map  B x H x W; compact int32 slots, or original int32/int64 IDs
+    segment_map  B x H x W; compact int32 slots, or original int32/int64 IDs,
+                 or original IDs as decoded RGB PNGs (B x H x W x 3 uint8)
     classes      B x S int32; slot 0 is void, class -1 is padding
     crowd        optional B x S bool (ground truth only)
     lookup_ids   optional B x S sorted int64 original IDs
@@ -97,8 +114,11 @@ class PanopticBatch:
         official "last crowd segment per category" rule applies.  With
         compact=False the maps keep their original IDs and the lookup is fused
         into the histogram kernel; pixel validation then happens in update().
+        ``maps`` may also be decoded COCO panoptic PNGs (B x H x W x 3 uint8 on
+        the device): the ID R + 256 * G + 65536 * B is decoded inside the same
+        kernels, so an RGB map is read once at 3 bytes per pixel.
         """
-        if maps.ndim != 3 or maps.dtype not in (torch.int32, torch.int64):
+        if not id_map_ok(maps):
             raise ValueError("maps must be integer B x H x W tensors")
         if len(segments_info) != maps.shape[0] or len(set(category_ids)) != len(category_ids):
             raise ValueError("batch length or category IDs are invalid")
@@ -142,7 +162,8 @@ class PanopticBatch:
             if maps.shape != self.segment_map.shape or maps.device != self.classes.device:
                 raise ValueError("new packed maps must have the same pixel count and metadata device")
             return replace(self, segment_map=maps)
-        if maps.ndim != 3 or maps.shape[0] != self.classes.shape[0] or maps.device != self.classes.device:
+        if (maps.ndim != 3 and not is_rgb(maps)) or maps.shape[0] != self.classes.shape[0] \
+                or maps.device != self.classes.device:
             raise ValueError("new maps must have the same batch size and metadata device")
         return replace(self, segment_map=maps)
 
@@ -151,7 +172,7 @@ class PanopticBatch:
         if self.lookup_ids is None:
             return self
         maps = self.segment_map
-        if maps.ndim != 3 or maps.dtype not in (torch.int32, torch.int64):
+        if not id_map_ok(maps):
             raise ValueError("maps must be integer B x H x W tensors")
         if (self.lookup_slots is None or self.lookup_ids.shape != self.classes.shape
                 or self.lookup_slots.shape != self.classes.shape):
@@ -163,10 +184,10 @@ class PanopticBatch:
         maps = maps.contiguous()
         ids = self.lookup_ids.contiguous()
         slots = self.lookup_slots.contiguous()
-        packed = torch.empty(maps.shape, dtype=torch.int32, device=maps.device)
+        B, H, W = maps.shape[:3]
+        packed = torch.empty((B, H, W), dtype=torch.int32, device=maps.device)
         error = torch.empty(1, dtype=torch.int32, device=maps.device)
-        B, H, W = maps.shape
-        _native.call("pe_pack", maps.data_ptr(), maps.element_size(), B, H * W, ids.data_ptr(),
+        _native.call("pe_pack", maps.data_ptr(), map_bytes(maps), B, H * W, ids.data_ptr(),
                      slots.data_ptr(), ids.shape[1], packed.data_ptr(), error.data_ptr(),
                      _native.stream(maps.device))
         if error.item():
@@ -185,13 +206,13 @@ def _prepare(batch: PanopticBatch, num_classes: int, validate: bool, *, ignore_c
             raise ValueError("a packed batch needs flat maps of sum(H_i * W_i) pixels and classes B x S")
         if batch.lookup_ids is not None:
             raise ValueError("packed batches take compact slots (compact() them image by image first)")
-    elif maps.ndim != 3 or classes.ndim != 2 or classes.shape[0] != maps.shape[0] or classes.shape[1] < 1:
+    elif ((maps.ndim != 3 and not is_rgb(maps)) or classes.ndim != 2 or classes.shape[0] != maps.shape[0]
+          or classes.shape[1] < 1):
         raise ValueError("expected maps B x H x W and classes B x S with S >= 1")
     original_ids = batch.lookup_ids is not None
     if (batch.lookup_slots is not None) != original_ids:
         raise ValueError("lookup IDs and slots must be supplied together")
-    map_dtypes = (torch.int32, torch.int64) if original_ids else (torch.int32,)
-    if maps.dtype not in map_dtypes or classes.dtype != torch.int32:
+    if not (id_map_ok(maps) if original_ids else maps.dtype == torch.int32) or classes.dtype != torch.int32:
         raise ValueError("classes and compact maps must be int32; original ID maps may also be int64")
     if original_ids:
         for lookup, dtype in ((batch.lookup_ids, torch.int64), (batch.lookup_slots, torch.int32)):
@@ -226,9 +247,11 @@ def _geometry(prediction: PanopticBatch, target: PanopticBatch, gt: torch.Tensor
             total += h * w
             offsets.append(total)
         return len(target.image_sizes), 1, total, _native.device_table(offsets, gt.device), total
-    if gt.shape != pred.shape or gt.device != pred.device:
+    if gt.shape[:3] != pred.shape[:3] or gt.device != pred.device:
         raise ValueError("prediction and target must share shape and device")
-    B, H, W = gt.shape
+    if is_rgb(gt) and pred.dtype == torch.int64 or is_rgb(pred) and gt.dtype == torch.int64:
+        raise ValueError("an RGB map pairs with an RGB or int32 map")
+    B, H, W = gt.shape[:3]
     return B, H, W, None, 0
 
 
@@ -305,7 +328,7 @@ class PanopticEvaluator:
         gt_ids, gt_slots = _lookup(target)
         pred_ids, pred_slots = _lookup(prediction)
         code = _native.call(
-            "pe_update", gt.data_ptr(), gt.element_size(), pred.data_ptr(), pred.element_size(), B, H, W,
+            "pe_update", gt.data_ptr(), map_bytes(gt), pred.data_ptr(), map_bytes(pred), B, H, W,
             _native.ptr(pix_off), gc.data_ptr(), pc.data_ptr(), crowd.data_ptr(), G, P, self.num_classes,
             pairs.data_ptr(), areas.data_ptr(), matched.data_ptr(), self.tp.data_ptr(),
             self.fp.data_ptr(), self.fn.data_ptr(), self.iou_sum.data_ptr(), self.confusion.data_ptr(),
@@ -389,7 +412,7 @@ def panoptic_quality(prediction: PanopticBatch, target: PanopticBatch, num_class
     pred_ids, pred_slots = _lookup(prediction)
     stream = _native.stream(device)
     code = _native.call(
-        "pe_update", gt.data_ptr(), gt.element_size(), pred.data_ptr(), pred.element_size(), B, H, W,
+        "pe_update", gt.data_ptr(), map_bytes(gt), pred.data_ptr(), map_bytes(pred), B, H, W,
         _native.ptr(pix_off), gc.data_ptr(), pc.data_ptr(), crowd.data_ptr(), G, P, C, pairs.data_ptr(),
         areas.data_ptr(),
         matched.data_ptr(), tp.data_ptr(), fp.data_ptr(), fn.data_ptr(), sums.data_ptr(), None,

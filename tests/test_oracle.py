@@ -438,5 +438,20 @@ def test_accumulate_brute_force_single_class():
         assert recall[0, 0, 0, 2] == rcs[-1]
 
 
+def test_rgb_decode():
+    """README.md:67-68 decode against the reference tests' encoder
+    (test_evaluator.py:39) and hand values; each byte weight checked alone."""
+    assert opan.rgb_to_id(np.array([1, 0, 0], np.uint8)) == 1
+    assert opan.rgb_to_id(np.array([0, 1, 0], np.uint8)) == 256
+    assert opan.rgb_to_id(np.array([0, 0, 1], np.uint8)) == 65536
+    assert opan.rgb_to_id(np.array([1, 2, 3], np.uint8)) == 1 + 2 * 256 + 3 * 65536
+    assert opan.rgb_to_id(np.array([255, 255, 255], np.uint8)) == 2**24 - 1
+    ids = np.random.default_rng(4).integers(0, 2**24, (3, 5, 7))
+    rgb = opan.id_to_rgb(ids)
+    assert rgb.dtype == np.uint8 and rgb.shape == (3, 5, 7, 3)
+    assert_exact(opan.rgb_to_id(rgb), ids)
+    assert_exact(rgb[..., 0], ids % 256)  # R is the low byte
+
+
 def test_oracle_is_test_only():
     assert oracle.__doc__ and "TEST INFRASTRUCTURE ONLY" in oracle.__doc__
