@@ -99,6 +99,71 @@ class PanopticConfig:
         return done / elapsed, f"{done} of {self.images} images, oracle update+compute"
 
 
+class RgbConfig(PanopticConfig):
+    """PanopticEvaluator over c3's maps stored as decoded COCO panoptic PNGs:
+    every slot gets a random 24-bit segment ID, maps are B x H x W x 3 uint8
+    (ID = R + 256 G + 65536 B) and PanopticBatch.from_coco(compact=False)
+    keeps them that way, so each update decodes and looks up the IDs inside
+    the histogram kernel."""
+
+    dtype = "u8 RGB maps (24-bit segment IDs; int64 counts, f64 ratios)"
+
+    def setup(self, device):
+        import torch
+        import workloads
+        import panoptic_evaluator as pe
+        pair = self.maker(device=device)
+        self.pair = pair
+        self.isthing = workloads.isthing_vector(self.classes, self.things)
+        gen = torch.Generator().manual_seed(17)
+        B, H, W = pair.gt.shape
+
+        def encode(maps, classes, crowd):
+            S = classes.shape[1]
+            ids = (torch.randperm(2**24 - 1, generator=gen)[:B * S] + 1).view(B, S)
+            ids[:, 0] = 0
+            table = ids.to(device)
+            segment = torch.gather(table, 1, maps.reshape(B, -1).long()).view(B, H, W)
+            rgb = torch.stack([(segment >> shift) & 255 for shift in (0, 8, 16)], -1).to(torch.uint8)
+            klass, flags = classes.cpu(), None if crowd is None else crowd.cpu()
+            info = [[{"id": int(ids[b, s]), "category_id": int(klass[b, s]),
+                      "iscrowd": int(flags is not None and bool(flags[b, s]))}
+                     for s in range(1, S) if klass[b, s] >= 0] for b in range(B)]
+            return rgb.contiguous(), info
+
+        self.rgb_gt, self.gt_info = encode(pair.gt, pair.gc, pair.crowd)
+        self.rgb_pred, self.pred_info = encode(pair.pred, pair.pc, None)
+        categories = range(self.classes)
+        self.target = pe.PanopticBatch.from_coco(self.rgb_gt, self.gt_info, categories, compact=False)
+        self.prediction = pe.PanopticBatch.from_coco(self.rgb_pred, self.pred_info, categories, compact=False)
+        self.evaluator = pe.PanopticEvaluator(self.classes, isthing=self.isthing, validate=False)
+        self.images = B
+        self.bytes_per_launch = 6 * B * H * W  # RGB GT + prediction maps
+        self.config = {"workload": self.description, "images": B, "height": H, "width": W,
+                       "classes": self.classes, "gt_segments": sum(map(len, self.gt_info)),
+                       "pred_segments": sum(map(len, self.pred_info)), "validate": False}
+
+    def cpu_sample(self, budget_s):
+        from oracle import panoptic as opan
+        gt, pred = self.rgb_gt.cpu().numpy(), self.rgb_pred.cpu().numpy()
+        categories = list(range(self.classes))
+        done, start = 0, time.perf_counter()
+        state = opan.PanopticState(self.classes, self.isthing)
+        while done < self.images:
+            b = done
+            g_ids, g_slots, gc, crowd = opan.from_coco(gt[b:b + 1, ..., 0], self.gt_info[b:b + 1], categories)
+            p_ids, p_slots, pc, _ = opan.from_coco(pred[b:b + 1, ..., 0], self.pred_info[b:b + 1], categories)
+            state.update(opan.compact(opan.rgb_to_id(gt[b:b + 1]), g_ids, g_slots),
+                         opan.compact(opan.rgb_to_id(pred[b:b + 1]), p_ids, p_slots), gc, pc, crowd,
+                         validate=False)
+            done += 1
+            if time.perf_counter() - start > budget_s:
+                break
+        state.compute()
+        elapsed = time.perf_counter() - start
+        return done / elapsed, f"{done} of {self.images} images, oracle RGB decode + ID compaction + update+compute"
+
+
 class CombinedConfig:
     """SegmentationEvaluator reset/update/compute over one uniform batch: PQ,
     mIoU and mask AP with the panoptic target's thing segments as AP ground
@@ -401,6 +466,9 @@ def make_configs():
                              "c0: PanopticEvaluator, 8 x 512x512, 133 classes, 40 Voronoi seeds"),
         "c3": PanopticConfig("c3", workloads.config3, 19, 8,
                              "c3: PanopticEvaluator, 64 x 1024x2048, 19 classes, 150 Voronoi seeds"),
+        "c3r": RgbConfig("c3r", workloads.config3, 19, 8,
+                         "c3r: PanopticEvaluator, c3's 64 x 1024x2048 maps as decoded COCO panoptic PNGs "
+                         "(RGB uint8, 24-bit segment IDs, fused decode + ID lookup), 19 classes"),
         "c4": ServingConfig("c4", "c4: RewardServer, Poisson arrivals; requests of 1..64 images of "
                                   "256^2..1024^2, 10..100 segments; half per-image PQ, half prefix rewards "
                                   "over 8..128 masks (every prefix or every 8th)"),

@@ -21,6 +21,7 @@
 //    (image, slot) order, so iou_sum is deterministic and bit-identical to the
 //    reference CPU path's in-order scatter_add_.
 #include <climits>
+#include <type_traits>
 
 #include "common.h"
 #include "panoptic_kernels.h"
@@ -104,7 +105,8 @@ __device__ __forceinline__ int compact_slot(T v, int S) {
 // Original ID -> metadata slot: binary search of the sorted per-image ID
 // table (evaluator.py:147-157 semantics: ID 0 is void; padding IDs are
 // INT64_MAX with slot 0 and never resolve).
-__device__ __forceinline__ int lookup_slot(i64 key, const i64* ids, const int* slots, int S) {
+template <typename IdPtr, typename SlotPtr>
+__device__ __forceinline__ int lookup_slot(i64 key, IdPtr ids, SlotPtr slots, int S) {
   if (key == 0) return 0;
   if (key < 0 || key == LLONG_MAX) return -1;
   int lo = 1, hi = S;
@@ -118,20 +120,60 @@ __device__ __forceinline__ int lookup_slot(i64 key, const i64* ids, const int* s
   return (s > 0 && s < S) ? s : -1;
 }
 
-template <typename T, bool LOOKUP>
-struct SlotResolver {
-  using V = typename MapValue<T>::type;
+// Sorted per-image (ID, slot) table in global memory: binary search.
+struct SortedIds {
   const i64* ids;
   const int* slots;
   int S;
-  i64 last_key = LLONG_MIN;
+  __device__ __forceinline__ int find(i64 key) const { return lookup_slot(key, ids, slots, S); }
+};
+
+typedef const __attribute__((address_space(3))) i64* LdsIds;
+typedef const __attribute__((address_space(3))) int* LdsInts;
+constexpr i64 kEmptyId = -1;
+
+// Multiplicative hash of an ID folded to 32 bits; `shift` = 32 - log2(size).
+__device__ __forceinline__ uint32_t hash_id(i64 key, int shift) {
+  const uint32_t x = static_cast<uint32_t>(key) ^ static_cast<uint32_t>(static_cast<u64>(key) >> 32);
+  return (x * 0x9E3779B1u) >> shift;
+}
+
+// The same table as an open-addressing hash in LDS (linear probing, load
+// factor <= 1/2): keys[h] is an ID (kEmptyId when free) and slot[h] what the
+// binary search returns for it (the slot at the lowest sorted index holding
+// the ID, or -1 when that slot is out of range).  A lookup is one round of
+// two independent LDS reads instead of log2(S) dependent ones; that matters
+// because a wave waits for its slowest lane whenever any lane's key changes.
+struct LdsHashIds {
+  LdsIds keys;
+  LdsInts slot;
+  uint32_t mask;
+  int shift;
+  __device__ __forceinline__ int find(i64 key) const {
+    if (key == 0) return 0;
+    if (key < 0 || key == LLONG_MAX) return -1;
+    for (uint32_t h = hash_id(key, shift);; h = (h + 1) & mask) {
+      const i64 k = keys[h];
+      const int s = slot[h];
+      if (k == key) return s;
+      if (k == kEmptyId) return -1;
+    }
+  }
+};
+
+template <typename T, bool LOOKUP, typename Table = SortedIds>
+struct SlotResolver {
+  using V = typename MapValue<T>::type;
+  Table table;
+  int S;
+  bool mapped;  // the map holds original IDs (else compact slots)
+  V last_key = -1;  // never a key that resolves: IDs are >= 0
   int last_slot = -1;
   __device__ __forceinline__ int operator()(V v) {
-    if (!LOOKUP || ids == nullptr) return compact_slot(v, S);
-    i64 key = static_cast<i64>(v);
-    if (key != last_key) {
-      last_key = key;
-      last_slot = lookup_slot(key, ids, slots, S);
+    if (!LOOKUP || !mapped) return compact_slot(v, S);
+    if (v != last_key) {
+      last_key = v;
+      last_slot = table.find(static_cast<i64>(v));
     }
     return last_slot;
   }
@@ -170,16 +212,23 @@ struct KeyCache {
   }
 };
 
-template <typename GT, typename PR, bool LOOKUP, bool VEC, bool GLOBAL>
-__device__ __forceinline__ void histogram_chunk(const HistArgs& a, int b, i64 begin, i64 end, uint32_t* bins) {
+// The image's ID tables: hashes in LDS (STAGE) or sorted tables in global memory.
+template <bool STAGE>
+struct ImageTables {
+  using Table = typename std::conditional<STAGE, LdsHashIds, SortedIds>::type;
+  Table gt, pred;
+};
+
+template <typename GT, typename PR, bool LOOKUP, bool VEC, bool GLOBAL, bool STAGE>
+__device__ __forceinline__ void histogram_chunk(const HistArgs& a, int b, i64 begin, i64 end, uint32_t* bins,
+                                                const ImageTables<STAGE>& t) {
   const GT* gt = static_cast<const GT*>(a.gt);
   const PR* pred = static_cast<const PR*>(a.pred);
   const int tid = threadIdx.x;
   u64* pairs = a.pairs + static_cast<i64>(b) * a.G * a.P;
-  SlotResolver<GT, LOOKUP> gslot{LOOKUP && a.gt_ids ? a.gt_ids + static_cast<i64>(b) * a.G : nullptr,
-                                 LOOKUP && a.gt_ids ? a.gt_slots + static_cast<i64>(b) * a.G : nullptr, a.G};
-  SlotResolver<PR, LOOKUP> pslot{LOOKUP && a.pred_ids ? a.pred_ids + static_cast<i64>(b) * a.P : nullptr,
-                                 LOOKUP && a.pred_ids ? a.pred_slots + static_cast<i64>(b) * a.P : nullptr, a.P};
+  using Table = typename ImageTables<STAGE>::Table;
+  SlotResolver<GT, LOOKUP, Table> gslot{t.gt, a.G, LOOKUP && a.gt_ids != nullptr};
+  SlotResolver<PR, LOOKUP, Table> pslot{t.pred, a.P, LOOKUP && a.pred_ids != nullptr};
   KeyCache<GLOBAL> cache;
   bool bad = false;
   const uint32_t P = static_cast<uint32_t>(a.P);
@@ -239,7 +288,7 @@ __device__ __forceinline__ void histogram_chunk(const HistArgs& a, int b, i64 be
 // pair table.  The pair table must be zero on entry (the scorer re-zeroes what
 // it read).  Completion is published by the kernel boundary: a per-workgroup
 // device-scope fence would write back the XCD's L2 for every chunk.
-template <typename GT, typename PR, bool LOOKUP, bool VEC, bool GLOBAL>
+template <typename GT, typename PR, bool LOOKUP, bool VEC, bool GLOBAL, bool STAGE>
 __device__ __forceinline__ void histogram_item(const HistArgs& a, int item, uint4* lds4) {
   uint32_t* bins = reinterpret_cast<uint32_t*>(lds4);
   const int tid = threadIdx.x;
@@ -251,9 +300,57 @@ __device__ __forceinline__ void histogram_item(const HistArgs& a, int item, uint
   const i64 end = image + min(static_cast<i64>(chunk + 1) * a.tiles.tile, pixels);
   if (!GLOBAL) {
     for (int w = tid; w < a.words4; w += kHistThreads) lds4[w] = make_uint4(0, 0, 0, 0);
-    __syncthreads();
   }
-  histogram_chunk<GT, PR, LOOKUP, VEC, GLOBAL>(a, b, begin, end, bins);
+  ImageTables<STAGE> t{};
+  const i64 gt_row = static_cast<i64>(b) * a.G, pred_row = static_cast<i64>(b) * a.P;
+  if constexpr (STAGE) {
+    // Hash the image's ID tables into LDS after the bins: Tg + Tp int64 keys,
+    // then Tg + Tp int32 words (lowest sorted index while building, then the
+    // resolved slot).
+    const int Tg = 1 << a.gt_hash_bits, Tp = 1 << a.pred_hash_bits;
+    auto* keys = reinterpret_cast<i64*>(reinterpret_cast<char*>(lds4) + a.table_offset);
+    auto* word = reinterpret_cast<int*>(keys + Tg + Tp);
+    for (int i = tid; i < Tg + Tp; i += kHistThreads) {
+      keys[i] = kEmptyId;
+      word[i] = INT_MAX;
+    }
+    __syncthreads();
+    for (int i = tid; i < a.G + a.P; i += kHistThreads) {
+      const bool g = i < a.G;
+      const i64* src = g ? a.gt_ids : a.pred_ids;
+      const int index = g ? i : i - a.G;
+      if (!src || index == 0) continue;  // index 0 is the void entry (ID 0)
+      const i64 key = src[g ? gt_row + index : pred_row + index];
+      if (key <= 0 || key == LLONG_MAX) continue;  // padding never resolves
+      i64* table = g ? keys : keys + Tg;
+      int* lowest = g ? word : word + Tg;
+      const uint32_t mask = static_cast<uint32_t>((g ? Tg : Tp) - 1);
+      for (uint32_t h = hash_id(key, 32 - (g ? a.gt_hash_bits : a.pred_hash_bits));; h = (h + 1) & mask) {
+        const u64 prev = atomicCAS(reinterpret_cast<unsigned long long*>(table + h),
+                                   static_cast<unsigned long long>(kEmptyId), static_cast<unsigned long long>(key));
+        if (static_cast<i64>(prev) == kEmptyId || static_cast<i64>(prev) == key) {
+          atomicMin(lowest + h, index);
+          break;
+        }
+      }
+    }
+    __syncthreads();
+    for (int i = tid; i < Tg + Tp; i += kHistThreads) {
+      if (keys[i] == kEmptyId) continue;
+      const bool g = i < Tg;
+      const int S = g ? a.G : a.P;
+      const int s = g ? a.gt_slots[gt_row + word[i]] : a.pred_slots[pred_row + word[i]];
+      word[i] = (s > 0 && s < S) ? s : -1;
+    }
+    t.gt = LdsHashIds{(LdsIds)(keys), (LdsInts)(word), static_cast<uint32_t>(Tg - 1), 32 - a.gt_hash_bits};
+    t.pred = LdsHashIds{(LdsIds)(keys + Tg), (LdsInts)(word + Tg), static_cast<uint32_t>(Tp - 1),
+                        32 - a.pred_hash_bits};
+  } else {
+    if (a.gt_ids) t.gt = SortedIds{a.gt_ids + gt_row, a.gt_slots + gt_row, a.G};
+    if (a.pred_ids) t.pred = SortedIds{a.pred_ids + pred_row, a.pred_slots + pred_row, a.P};
+  }
+  if (!GLOBAL || STAGE) __syncthreads();
+  histogram_chunk<GT, PR, LOOKUP, VEC, GLOBAL, STAGE>(a, b, begin, end, bins, t);
   if (!GLOBAL) {
     __syncthreads();
     u64* pairs = a.pairs + static_cast<i64>(b) * a.G * a.P;
@@ -275,17 +372,18 @@ __device__ __forceinline__ void histogram_item(const HistArgs& a, int item, uint
 // One workgroup per chunk (surplus workgroups of a packed batch, whose chunk
 // count is only known on the device, exit at once), or a persistent grid
 // that walks the chunk list with a grid stride.
-template <typename GT, typename PR, bool LOOKUP, bool VEC, bool GLOBAL, bool PERSIST>
+template <typename GT, typename PR, bool LOOKUP, bool VEC, bool GLOBAL, bool PERSIST, bool STAGE>
 __global__ __launch_bounds__(kHistThreads) void joint_histogram(HistArgs a) {
   extern __shared__ uint4 lds4[];
   const int total = a.tiles.count(a.B);
   if (!PERSIST) {
-    if (static_cast<int>(blockIdx.x) < total) histogram_item<GT, PR, LOOKUP, VEC, GLOBAL>(a, blockIdx.x, lds4);
+    if (static_cast<int>(blockIdx.x) < total)
+      histogram_item<GT, PR, LOOKUP, VEC, GLOBAL, STAGE>(a, blockIdx.x, lds4);
     return;
   }
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
-    histogram_item<GT, PR, LOOKUP, VEC, GLOBAL>(a, item, lds4);
-    __syncthreads();  // the next chunk re-zeroes the bins
+    histogram_item<GT, PR, LOOKUP, VEC, GLOBAL, STAGE>(a, item, lds4);
+    __syncthreads();  // the next chunk re-zeroes the bins and re-stages the tables
   }
 }
 
@@ -298,8 +396,15 @@ static bool allow_lds(K kernel, size_t bytes) {
 
 template <typename GT, typename PR, bool LOOKUP, bool VEC, bool GLOBAL>
 static int launch_hist_kernel(const HistArgs& a, int blocks, size_t lds, hipStream_t stream) {
-  auto kernel = a.persist ? joint_histogram<GT, PR, LOOKUP, VEC, GLOBAL, true>
-                          : joint_histogram<GT, PR, LOOKUP, VEC, GLOBAL, false>;
+  // Original-ID batches are uniform (never packed), so they never take the
+  // persistent grid; their ID tables go to LDS when they fit.
+  void (*kernel)(HistArgs);
+  if constexpr (LOOKUP)
+    kernel = a.stage_tables ? joint_histogram<GT, PR, LOOKUP, VEC, GLOBAL, false, true>
+                            : joint_histogram<GT, PR, LOOKUP, VEC, GLOBAL, false, false>;
+  else
+    kernel = a.persist ? joint_histogram<GT, PR, LOOKUP, VEC, GLOBAL, true, false>
+                       : joint_histogram<GT, PR, LOOKUP, VEC, GLOBAL, false, false>;
   if (!allow_lds(kernel, lds)) return fail(kUnsupportedShape, "LDS request of %zu bytes refused", lds);
   ProfileScope timing("joint_histogram", stream);
   kernel<<<blocks, kHistThreads, lds, stream>>>(a);
@@ -370,8 +475,30 @@ int launch_joint_histogram(HistArgs a, int gt_bytes, int pred_bytes, int chunk, 
   size_t lds = static_cast<size_t>(a.words4) * 16;
   bool global = lds > kMaxHistLds;
   if (global) lds = 0;
-  a.lds_bytes = lds;
   bool lookup = a.gt_ids || a.pred_ids;
+  if (lookup && a.persist) return fail(kBadArgument, "original-ID maps cannot be packed");
+  // ID hashes (12 bytes per entry, at least two entries per ID) after the
+  // bins, if the total fits.
+  auto hash_bits = [](int n) {
+    int bits = 4;
+    while (bits < 30 && (static_cast<i64>(1) << bits) < 2 * static_cast<i64>(n)) ++bits;
+    return bits;
+  };
+  a.gt_hash_bits = hash_bits(a.G);
+  a.pred_hash_bits = hash_bits(a.P);
+  const size_t table_bytes =
+      align_up((static_cast<size_t>(1) << a.gt_hash_bits) * 12 + (static_cast<size_t>(1) << a.pred_hash_bits) * 12,
+               16);
+  a.stage_tables = lookup && lds + table_bytes <= kMaxHistLds + 16 * 1024;
+  if (lookup && !a.stage_tables && !global && table_bytes <= kMaxHistLds) {
+    // The bins and the tables do not fit together: bins go to global memory.
+    global = true;
+    lds = 0;
+    a.stage_tables = true;
+  }
+  a.table_offset = static_cast<int>(lds);
+  if (a.stage_tables) lds += table_bytes;
+  a.lds_bytes = lds;
   bool vec = quad_aligned(a.gt, gt_bytes) && quad_aligned(a.pred, pred_bytes);
   int nb = static_cast<int>(blocks);
 #define PE_HIST(GT, PR)                                                                   \
@@ -1082,7 +1209,8 @@ __global__ __launch_bounds__(256) void pack_slots(const T* maps, i64 HW, const i
   const i64 span = (HW + blocks_per_image - 1) / blocks_per_image;
   const i64 begin = static_cast<i64>(b) * HW + part * span;
   const i64 end = static_cast<i64>(b) * HW + min(HW, (part + 1) * span);
-  SlotResolver<T, true> resolve{ids + static_cast<i64>(b) * S, slots + static_cast<i64>(b) * S, S};
+  SlotResolver<T, true> resolve{SortedIds{ids + static_cast<i64>(b) * S, slots + static_cast<i64>(b) * S, S}, S,
+                                true};
   bool bad = false;
   auto put = [&](i64 i, typename MapValue<T>::type v) {
     const int s = resolve(v);

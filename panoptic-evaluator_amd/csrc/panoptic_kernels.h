@@ -29,6 +29,10 @@ struct HistArgs {
   const int* pred_slots;
   int* error;            // nullptr: no validation
   size_t lds_bytes;
+  bool stage_tables;     // original-ID tables hashed into LDS after the bins
+  int table_offset;      // their byte offset in LDS
+  int gt_hash_bits;      // log2 of the hash sizes (>= 2 entries per ID)
+  int pred_hash_bits;
 };
 
 struct ScoreArgs {
