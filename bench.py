@@ -453,6 +453,52 @@ class ServingConfig:
         return done / elapsed, f"{done} images of PQ and prefix requests, oracle panoptic_quality/prefixes"
 
 
+class PackedCombinedConfig(CombinedConfig):
+    """c1 with the detection masks arriving bit-packed (PackedMasks, one bit
+    per pixel): the detection x slot histogram reads 1/8 of the mask bytes.
+    Packing happens once in setup, outside the timed steps."""
+
+    dtype = "bit-packed masks / int32 maps (int64 counts, f64 ratios)"
+
+    def setup(self, device):
+        import panoptic_evaluator as pe
+        super().setup(device)
+        self.detections = pe.InstanceBatch([pe.pack_masks(m) for m in self.masks], self.classes, self.scores)
+        B, H, W = self.pair.gt.shape
+        dets = sum(m.shape[0] for m in self.masks)
+        # bit rows of the detection masks + the int32 target map
+        self.bytes_per_launch = 4 * mask_words(H * W) * dets + 4 * H * W * B
+        self.config["mask_format"] = "bits"
+        self.config["update_bytes_per_image"] = 8 * H * W + 4 * mask_words(H * W) * dets // B
+
+
+class PackedInstanceConfig(InstanceConfig):
+    """c1i with bit-packed detection and GT masks: intersections are popcounts
+    of ANDed words (mask_overlap_bits) instead of the int8 MFMA contraction.
+    Packing happens once in setup, outside the timed steps."""
+
+    dominant = "mask_overlap_bits"
+    tuned = ("mask_overlap_bits",)
+    dtype = "bit-packed masks (popcount, int32 accumulate, f64 ratios)"
+
+    def setup(self, device):
+        import panoptic_evaluator as pe
+        super().setup(device)
+        self.prediction = pe.InstanceBatch([pe.pack_masks(m) for m in self.masks], self.prediction.classes,
+                                           self.prediction.scores)
+        self.target = pe.InstanceBatch([pe.pack_masks(m) for m in self.gt_masks], self.target.classes,
+                                       crowd=self.target.crowd)
+        H, W = self.config["height"], self.config["width"]
+        rows = sum(m.shape[0] for m in self.masks) + sum(m.shape[0] for m in self.gt_masks)
+        self.bytes_per_launch = 4 * mask_words(H * W) * rows
+        self.mfma_ops = None
+        self.config["mask_format"] = "bits"
+
+
+def mask_words(pixels):
+    return (pixels + 127) // 128 * 4
+
+
 def make_configs():
     import workloads
     return {
@@ -462,6 +508,10 @@ def make_configs():
                                  "[320, 1344]), 5..120 segments, 0..300 ellipse detections, 133 classes"),
         "c1i": InstanceConfig("c1i", "c1i: InstanceEvaluator, 16 x 800x1088, 100 ellipse detections vs the "
                                      "target's thing segments as GT masks, 133 classes"),
+        "c1p": PackedCombinedConfig("c1p", "c1p: c1 (SegmentationEvaluator, 16 x 800x1088, 100 ellipse detections "
+                                           "per image) with bit-packed detection masks"),
+        "c1ip": PackedInstanceConfig("c1ip", "c1ip: c1i (InstanceEvaluator, 16 x 800x1088, 100 detections vs the "
+                                             "target's thing segments) with bit-packed masks on both sides"),
         "c0": PanopticConfig("c0", workloads.config0, 133, 80,
                              "c0: PanopticEvaluator, 8 x 512x512, 133 classes, 40 Voronoi seeds"),
         "c3": PanopticConfig("c3", workloads.config3, 19, 8,

@@ -158,6 +158,25 @@ int pe_mask_overlap_batch(int64_t B, const uint8_t* const* pm, const uint8_t* co
                           int64_t total_d, int64_t total_g, int64_t total_inter, int32_t* inter, int32_t* pa,
                           int32_t* ga, int aligned, void* stream);
 
+/* Bit-packed masks (no reference counterpart: the reference takes bool
+ * bytes; its compressed-mask input is a listed extension).  A mask of
+ * `pixels` pixels is a row of pe_mask_words(pixels) 32-bit words, multiple
+ * of 4: bit p % 32 of word p / 32 is pixel p, bits past `pixels` are zero,
+ * rows are 16-byte aligned.  pe_pack_masks packs N rows of HW bool bytes
+ * (any nonzero byte is set) into bits (N x pe_mask_words(HW) words). */
+int64_t pe_mask_words(int64_t pixels);
+int pe_pack_masks(const uint8_t* masks, int64_t N, int64_t HW, uint32_t* bits, void* stream);
+
+/* pe_mask_overlap_batch over bit-packed masks: pm/gm are device arrays of B
+ * pointers to bit rows (16-byte aligned); intersections are popcounts of
+ * ANDed words.  Same arguments, outputs and replaced reference interface as
+ * pe_mask_overlap_batch (no `aligned`: bit rows always are). */
+int pe_mask_overlap_bits_batch(int64_t B, const uint32_t* const* pm, const uint32_t* const* gm,
+                               const int64_t* det_off, const int64_t* gt_off, const int64_t* inter_off,
+                               const int64_t* pixels, int64_t HW, int64_t max_d, int64_t max_g, int64_t max_hw,
+                               int64_t total_d, int64_t total_g, int64_t total_inter, int32_t* inter,
+                               int32_t* pa, int32_t* ga, void* stream);
+
 /* Scratch bytes of pe_detection_histogram for a packed batch of
  * packed_pixels pixels (0 for a batch of one image size: no scratch). */
 int64_t pe_detection_histogram_workspace_bytes(int64_t B, int64_t packed_pixels);
@@ -177,6 +196,14 @@ int pe_detection_histogram(int64_t B, const uint8_t* const* masks, const int64_t
                            const int64_t* order, const int32_t* target, const int64_t* pix_off,
                            int64_t HW, int64_t S, int64_t max_dets, int64_t total_dets, int64_t* pairs,
                            int aligned, void* workspace, int64_t workspace_bytes, void* stream);
+
+/* pe_detection_histogram over bit-packed detection masks (rows as for
+ * pe_pack_masks, 16-byte aligned): same arguments and outputs, no
+ * `aligned`.  Replaces the same ``instance_histogram`` interface. */
+int pe_detection_histogram_bits(int64_t B, const uint32_t* const* masks, const int64_t* det_off,
+                                const int64_t* order, const int32_t* target, const int64_t* pix_off,
+                                int64_t HW, int64_t S, int64_t max_dets, int64_t total_dets, int64_t* pairs,
+                                void* workspace, int64_t workspace_bytes, void* stream);
 
 /* COCO greedy matching of a batch of images.  Per image b, detections
  * det_off[b]..det_off[b+1] are in score order (det_cls int32, det_area f64

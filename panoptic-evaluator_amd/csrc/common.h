@@ -152,13 +152,18 @@ struct ProfileScope {
   ~ProfileScope();
 };
 
+// Bit-packed masks: a mask of `pixels` pixels is a row of mask_words(pixels)
+// 32-bit words, bit p % 32 of word p / 32 being pixel p; rows are padded with
+// zero bits to a multiple of 128 pixels so that every row is 16-byte aligned.
+__host__ __device__ inline i64 mask_words(i64 pixels) { return (pixels + 127) / 128 * 4; }
+
 // Launch-shape tuning.  Each tunable kernel has a fixed list of variants
 // (template instantiations and launch shapes, all computing identical
 // results); the selected variant is used by every later launch in the
 // process.  Selection is process-wide and meant to be set once, before
 // launches are issued (the autotuner times each variant and keeps the best).
 enum TuneKernel { kTuneDetectionHistogram = 0, kTuneJointHistogram, kTuneMaskOverlap, kTunePackedGrid,
-                  kTuneDetectionGroup, kTuneKernels };
+                  kTuneDetectionGroup, kTuneMaskOverlapBits, kTuneKernels };
 struct TuneSpec {
   const char* name;
   int count;

@@ -383,13 +383,210 @@ int launch_overlap(OverlapArgs o, int64_t B, int64_t max_d, int64_t max_g, int64
   return kOk;
 }
 
+// --------------------------------------------------------- bit-packed masks
+
+// Four mask bytes (any nonzero byte is set) -> four bits, byte j -> bit j.
+__device__ __forceinline__ uint32_t byte_bits4(uint32_t x) {
+  const uint32_t set = (((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u;  // high bit of nonzero bytes
+  return ((set >> 7) * 0x01020408u) >> 24;
+}
+
+// Bool masks (N rows of HW bytes) -> bit rows of `stride` = mask_words(HW)
+// words; bits past HW are zero.
+__global__ __launch_bounds__(256) void pack_mask_bits(const uint8_t* masks, i64 N, i64 HW, i64 stride, int aligned,
+                                                      uint32_t* out) {
+  const i64 total = N * stride;
+  for (i64 w = static_cast<i64>(blockIdx.x) * 256 + threadIdx.x; w < total; w += static_cast<i64>(gridDim.x) * 256) {
+    const i64 n = w / stride, k = (w - n * stride) * 32;
+    uint32_t word = 0;
+    if (k < HW) {
+      const uint8_t* row = masks + n * HW;
+      if (aligned && k + 32 <= HW) {
+        const auto* p = as_global(reinterpret_cast<const v4i*>(row + k));
+        const v4i lo = __builtin_nontemporal_load(p), hi = __builtin_nontemporal_load(p + 1);
+        const uint32_t d[8] = {static_cast<uint32_t>(lo.x), static_cast<uint32_t>(lo.y),
+                               static_cast<uint32_t>(lo.z), static_cast<uint32_t>(lo.w),
+                               static_cast<uint32_t>(hi.x), static_cast<uint32_t>(hi.y),
+                               static_cast<uint32_t>(hi.z), static_cast<uint32_t>(hi.w)};
+#pragma unroll
+        for (int j = 0; j < 8; ++j) word |= byte_bits4(d[j]) << (4 * j);
+      } else {
+        const auto* g = as_global(row);
+        for (int j = 0; j < 32 && k + j < HW; ++j) word |= static_cast<uint32_t>(g[k + j] != 0) << j;
+      }
+    }
+    out[w] = word;
+  }
+}
+
+// inter[d, g] += popcount(pm[d] & gm[g]) over bit rows: a workgroup of 256
+// threads covers 16 * DR detections x 16 * GR GT masks over ChunkW words
+// (32 pixels each), staging blocks of KW words of every row in LDS (each row
+// block is a contiguous run, loaded 16 bytes per lane).  Thread (td, tg)
+// owns detections td + 16 j and GT masks tg + 16 i, so the 16 lanes of one
+// LDS read pass touch 16 consecutive rows (distinct banks with the 4-word
+// row padding) or one row (a broadcast).  Areas are popcounts of the staged
+// words.  The next block's loads are in flight during the popcounts.
+template <int DR, int GR, int KW, int ChunkW>
+__global__ __launch_bounds__(256) void mask_overlap_bits(OverlapArgs o) {
+  constexpr int kThreads = 256;
+  constexpr int kDets = 16 * DR, kGts = 16 * GR;
+  constexpr int kRows = kDets + kGts;
+  constexpr int kStride = KW + 4;  // words
+  constexpr int kPerRow = KW / 4;  // 16-byte pieces of one row block
+  constexpr int kSlots = kRows * kPerRow / kThreads;
+  static_assert(KW % 4 == 0 && ChunkW % KW == 0, "blocks are whole 16-byte pieces");
+  static_assert(kThreads % kPerRow == 0 && kPerRow <= kWave && (kPerRow & (kPerRow - 1)) == 0,
+                "the pieces of one row sit in one wave");
+  static_assert(kRows * kPerRow % kThreads == 0, "every thread loads the same number of pieces");
+  __shared__ __align__(16) uint32_t tile[kRows * kStride];
+  OverlapItem it;
+  if (!overlap_item(o, kDets, kGts, static_cast<i64>(ChunkW) * 32, it)) return;
+  const int tid = threadIdx.x;
+  const int D = it.D, G = it.G;
+  const int d_base = it.dgroup * kDets, g_base = it.ggroup * kGts;
+  const i64 words = mask_words(it.HW);
+  const i64 w_begin = static_cast<i64>(it.chunk) * ChunkW;
+  const i64 w_end = min(w_begin + ChunkW, words);  // both multiples of 4
+  const auto* pm = reinterpret_cast<const uint32_t*>(it.pm);
+  const auto* gm = reinterpret_cast<const uint32_t*>(it.gm);
+  // Slot i of this thread: row tid / kPerRow + i * kRowStep of the block, at
+  // word (tid % kPerRow) * 4.  Rows and pointers are recomputed per load to
+  // keep registers for the counts.
+  constexpr int kRowStep = kThreads / kPerRow;
+  const int col = (tid % kPerRow) * 4, row0 = tid / kPerRow;
+  auto slot_row = [&](int i, int& idx, bool& det) {
+    const int r = row0 + i * kRowStep;
+    det = r < kDets;
+    idx = det ? d_base + r : g_base + r - kDets;
+    return r;
+  };
+  auto load = [&](int i, i64 w) {
+    int idx;
+    bool det;
+    slot_row(i, idx, det);
+    if (idx >= (det ? D : G) || w + col >= w_end) return v4i{0, 0, 0, 0};
+    const uint32_t* row = (det ? pm : gm) + static_cast<i64>(idx) * words;
+    return __builtin_nontemporal_load(as_global(reinterpret_cast<const v4i*>(row + w + col)));
+  };
+  int area[kSlots];
+#pragma unroll
+  for (int i = 0; i < kSlots; ++i) area[i] = 0;
+  // Detection blocks j and GT blocks i that hold any mask (uniform).
+  const int jd = min(DR, (D - d_base + 15) / 16), ig = min(GR, max(0, (G - g_base + 15) / 16));
+  const int td = tid >> 4, tg = tid & 15;
+  int acc[DR][GR];
+#pragma unroll
+  for (int j = 0; j < DR; ++j)
+#pragma unroll
+    for (int i = 0; i < GR; ++i) acc[j][i] = 0;
+  v4i buf[kSlots];
+#pragma unroll
+  for (int i = 0; i < kSlots; ++i) buf[i] = load(i, w_begin);
+  typedef unsigned v2u __attribute__((ext_vector_type(2)));
+  for (i64 w = w_begin; w < w_end; w += KW) {
+    __syncthreads();  // the previous block has been counted
+#pragma unroll
+    for (int i = 0; i < kSlots; ++i) {
+      *reinterpret_cast<v4i*>(tile + (row0 + i * kRowStep) * kStride + col) = buf[i];
+      area[i] += __builtin_popcount(static_cast<uint32_t>(buf[i].x)) +
+                 __builtin_popcount(static_cast<uint32_t>(buf[i].y)) +
+                 __builtin_popcount(static_cast<uint32_t>(buf[i].z)) +
+                 __builtin_popcount(static_cast<uint32_t>(buf[i].w));
+    }
+    __syncthreads();
+    if (w + KW < w_end) {
+#pragma unroll
+      for (int i = 0; i < kSlots; ++i) buf[i] = load(i, w + KW);
+    }
+#pragma unroll 2
+    for (int c = 0; c < KW; c += 2) {
+      v2u dv[DR], gv[GR];
+#pragma unroll
+      for (int j = 0; j < DR; ++j)
+        if (j < jd) dv[j] = *reinterpret_cast<const v2u*>(tile + (td + 16 * j) * kStride + c);
+#pragma unroll
+      for (int i = 0; i < GR; ++i)
+        if (i < ig) gv[i] = *reinterpret_cast<const v2u*>(tile + (kDets + tg + 16 * i) * kStride + c);
+#pragma unroll
+      for (int j = 0; j < DR; ++j) {
+        if (j >= jd) break;
+#pragma unroll
+        for (int i = 0; i < GR; ++i) {
+          if (i >= ig) break;
+          acc[j][i] += __builtin_popcount(dv[j].x & gv[i].x) + __builtin_popcount(dv[j].y & gv[i].y);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < DR; ++j) {
+    const int d = d_base + td + 16 * j;
+#pragma unroll
+    for (int i = 0; i < GR; ++i) {
+      const int g = g_base + tg + 16 * i;
+      if (d < D && g < G && acc[j][i]) atomicAdd(it.inter + static_cast<i64>(d) * G + g, acc[j][i]);
+    }
+  }
+  // Areas: the kPerRow neighbouring lanes of a slot share its row.
+  const bool count_a = it.pa != nullptr && it.ggroup == 0;
+  const bool count_b = it.ga != nullptr && it.dgroup == 0;
+#pragma unroll
+  for (int i = 0; i < kSlots; ++i) {
+    int sum = area[i];
+#pragma unroll
+    for (int m = 1; m < kPerRow; m <<= 1) sum += __shfl_xor(sum, m, kWave);
+    int idx;
+    bool det;
+    slot_row(i, idx, det);
+    if (col == 0 && idx < (det ? D : G) && sum && (det ? count_a : count_b))
+      atomicAdd(det ? it.pa + idx : it.ga + idx, sum);
+  }
+}
+
+struct BitsVariant {
+  int dets, gts, chunk_words;
+  void (*kernel)(OverlapArgs);
+};
+// Order matches the tuning registry's descriptions.
+const BitsVariant kOverlapBits[] = {
+    {128, 64, 512, mask_overlap_bits<8, 4, 32, 512>},
+    {128, 64, 1024, mask_overlap_bits<8, 4, 32, 1024>},
+    {64, 64, 512, mask_overlap_bits<4, 4, 32, 512>},
+    {128, 32, 512, mask_overlap_bits<8, 2, 32, 512>},
+};
+
+int launch_overlap_bits(OverlapArgs o, int64_t B, int64_t max_d, int64_t max_g, int64_t max_hw, int64_t total_d,
+                        int64_t total_g, int64_t total_inter, hipStream_t stream) {
+  int rc;
+  if (total_inter && (rc = launch_zero(o.inter, 4 * static_cast<size_t>(total_inter), stream))) return rc;
+  if (o.pa && total_d && (rc = launch_zero(o.pa, 4 * static_cast<size_t>(total_d), stream))) return rc;
+  if (o.ga && total_g && (rc = launch_zero(o.ga, 4 * static_cast<size_t>(total_g), stream))) return rc;
+  if (B == 0 || max_hw == 0 || (max_d == 0 && max_g == 0)) return kOk;
+  const BitsVariant& v = kOverlapBits[tuned_variant(kTuneMaskOverlapBits)];
+  const i64 chunks = (mask_words(max_hw) + v.chunk_words - 1) / v.chunk_words;
+  const i64 dgroups = std::max<i64>(1, (max_d + v.dets - 1) / v.dets);
+  const i64 ggroups = std::max<i64>(1, (max_g + v.gts - 1) / v.gts);
+  const i64 items = dgroups * ggroups * B * chunks;
+  if (items >= INT_MAX - kXcds) return fail(kUnsupportedShape, "too many masks or pixels for one overlap launch");
+  o.chunks = static_cast<int>(chunks);
+  o.dgroups = static_cast<int>(dgroups);
+  o.ggroups = static_cast<int>(ggroups);
+  o.items = static_cast<int>(items);
+  o.xcd_span = static_cast<int>((items + kXcds - 1) / kXcds);
+  ProfileScope timing("mask_overlap_bits", stream);
+  v.kernel<<<static_cast<unsigned>(static_cast<i64>(o.xcd_span) * kXcds), 256, 0, stream>>>(o);
+  PE_LAUNCH_CHECK();
+  return kOk;
+}
+
 // ------------------------------------------------- detection x slot counts
 
 constexpr int kDetThreads = 256;
 constexpr size_t kDetMaxLds = 96 * 1024;
 
 struct DetHistArgs {
-  const uint8_t* const* masks;   // B device pointers (D_b x pixels_b bools)
+  const uint8_t* const* masks;   // B device pointers (D_b x pixels_b bools, or bit rows)
   const i64* det_off;            // B + 1
   const i64* order;              // per detection: mask index within its image
   const int32_t* target;         // slots of every image, one after another
@@ -422,6 +619,23 @@ __device__ __forceinline__ void count_chunk(v4i m, const int* slots, int S, int 
   if (run_key >= 0) atomicAdd(bins + (run_key >> 1), static_cast<uint32_t>(run) << ((run_key & 1) * 16));
 }
 
+// The same count over 16 pixels given as bits (bit j = pixel j).
+__device__ __forceinline__ void count_chunk_bits(uint32_t on, const int* slots, int S, int rowbase, uint32_t* bins) {
+  int run_key = -1, run = 0;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int s = slots[j];
+    const int key = (((on >> j) & 1u) && s >= 0 && s < S) ? rowbase + s : -1;
+    if (key != run_key) {
+      if (run_key >= 0) atomicAdd(bins + (run_key >> 1), static_cast<uint32_t>(run) << ((run_key & 1) * 16));
+      run_key = key;
+      run = 0;
+    }
+    run += key >= 0;
+  }
+  if (run_key >= 0) atomicAdd(bins + (run_key >> 1), static_cast<uint32_t>(run) << ((run_key & 1) * 16));
+}
+
 // Masks are read exactly once: stream them past the caches.
 __device__ __forceinline__ v4i load16_stream(const uint8_t* row, i64 k, i64 HW, bool aligned) {
   const auto* g = as_global(row);
@@ -437,7 +651,9 @@ __device__ __forceinline__ v4i load16_stream(const uint8_t* row, i64 k, i64 HW, 
 
 // CH = 16-pixel chunks per lane (tile = 16 * 256 * CH pixels); U = masks
 // whose loads are in flight together.  One (tile, detection group) item.
-template <int CH, int U>
+// BITS: masks are bit rows of mask_words(pixels) 32-bit words (bit p % 32 of
+// word p / 32 is pixel p); a lane's 16-pixel chunk is half a word.
+template <int CH, int U, bool BITS>
 __device__ __forceinline__ void detection_item(const DetHistArgs& a, int item, uint4* lds4) {
   constexpr int kTile = 16 * kDetThreads * CH;
   uint32_t* bins = reinterpret_cast<uint32_t*>(lds4);
@@ -467,25 +683,47 @@ __device__ __forceinline__ void detection_item(const DetHistArgs& a, int item, u
 #pragma unroll
     for (int j = 0; j < 16; ++j) slots[h][j] = base[h] + j < HW ? target[base[h] + j] : -1;
   }
-  for (int r0 = 0; r0 < nd; r0 += U) {
-    v4i m[U][CH];
+  if constexpr (BITS) {
+    const i64 stride = mask_words(HW);
+    const auto* rows = as_global(reinterpret_cast<const uint32_t*>(masks));
+    for (int r0 = 0; r0 < nd; r0 += U) {
+      uint32_t m[U][CH];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (r0 + u < nd) {
-        const uint8_t* mask = masks + order[r0 + u] * HW;
+      for (int u = 0; u < U; ++u) {
+        const auto* mask = rows + (r0 + u < nd ? order[r0 + u] : 0) * stride;
 #pragma unroll
-        for (int h = 0; h < CH; ++h) m[u][h] = load16_stream(mask, base[h], HW, aligned);
-      } else {
-#pragma unroll
-        for (int h = 0; h < CH; ++h) m[u][h] = v4i{0, 0, 0, 0};
+        for (int h = 0; h < CH; ++h)
+          m[u][h] = r0 + u < nd && base[h] < HW
+                        ? (__builtin_nontemporal_load(mask + (base[h] >> 5)) >> (base[h] & 16)) & 0xFFFFu
+                        : 0u;
       }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int h = 0; h < CH; ++h)
+          if (m[u][h]) count_chunk_bits(m[u][h], slots[h], a.S, (r0 + u) * a.S, bins);
     }
+  } else {
+    for (int r0 = 0; r0 < nd; r0 += U) {
+      v4i m[U][CH];
 #pragma unroll
-    for (int u = 0; u < U; ++u)
+      for (int u = 0; u < U; ++u) {
+        if (r0 + u < nd) {
+          const uint8_t* mask = masks + order[r0 + u] * HW;
 #pragma unroll
-      for (int h = 0; h < CH; ++h)
-        if ((m[u][h].x | m[u][h].y | m[u][h].z | m[u][h].w) != 0)
-          count_chunk(m[u][h], slots[h], a.S, (r0 + u) * a.S, bins);
+          for (int h = 0; h < CH; ++h) m[u][h] = load16_stream(mask, base[h], HW, aligned);
+        } else {
+#pragma unroll
+          for (int h = 0; h < CH; ++h) m[u][h] = v4i{0, 0, 0, 0};
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int h = 0; h < CH; ++h)
+          if ((m[u][h].x | m[u][h].y | m[u][h].z | m[u][h].w) != 0)
+            count_chunk(m[u][h], slots[h], a.S, (r0 + u) * a.S, bins);
+    }
   }
   __syncthreads();
   u64* out = a.pairs + (first + d0) * a.S;
@@ -501,33 +739,35 @@ __device__ __forceinline__ void detection_item(const DetHistArgs& a, int item, u
 // One workgroup per item (grid >= item count; surplus workgroups of a packed
 // batch, whose count is only known on the device, exit at once), or a
 // persistent grid that walks the items with a grid stride.
-template <int CH, int U, bool PERSIST>
+template <int CH, int U, bool PERSIST, bool BITS>
 __global__ __launch_bounds__(kDetThreads) void detection_histogram(DetHistArgs a) {
   extern __shared__ uint4 lds4[];
   const int total = a.tiles.count(a.B) * a.dgroups;
   if (!PERSIST) {
-    if (static_cast<int>(blockIdx.x) < total) detection_item<CH, U>(a, blockIdx.x, lds4);
+    if (static_cast<int>(blockIdx.x) < total) detection_item<CH, U, BITS>(a, blockIdx.x, lds4);
     return;
   }
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
-    detection_item<CH, U>(a, item, lds4);
+    detection_item<CH, U, BITS>(a, item, lds4);
     __syncthreads();  // the next item re-zeroes the bins
   }
 }
 
 struct DetVariant {
   int chunks;
-  void (*kernel)(DetHistArgs);
-  void (*persistent)(DetHistArgs);
+  void (*kernel[2])(DetHistArgs);      // [bits]
+  void (*persistent[2])(DetHistArgs);
 };
+#define PE_DET_VARIANT(CH, U)                                                                \
+  {                                                                                          \
+    CH, {detection_histogram<CH, U, false, false>, detection_histogram<CH, U, false, true>}, \
+        {detection_histogram<CH, U, true, false>, detection_histogram<CH, U, true, true>}    \
+  }
 // Order matches the tuning registry's descriptions.
 const DetVariant kDetHist[] = {
-    {1, detection_histogram<1, 4, false>, detection_histogram<1, 4, true>},
-    {1, detection_histogram<1, 8, false>, detection_histogram<1, 8, true>},
-    {2, detection_histogram<2, 2, false>, detection_histogram<2, 2, true>},
-    {2, detection_histogram<2, 4, false>, detection_histogram<2, 4, true>},
-    {4, detection_histogram<4, 2, false>, detection_histogram<4, 2, true>},
+    PE_DET_VARIANT(1, 4), PE_DET_VARIANT(1, 8), PE_DET_VARIANT(2, 2), PE_DET_VARIANT(2, 4), PE_DET_VARIANT(4, 2),
 };
+#undef PE_DET_VARIANT
 
 // ------------------------------------------------------------ COCO matching
 
@@ -1052,17 +1292,56 @@ extern "C" int pe_mask_overlap_batch(int64_t B, const uint8_t* const* pm, const 
                         static_cast<hipStream_t>(stream_ptr));
 }
 
+extern "C" int64_t pe_mask_words(int64_t pixels) { return pixels < 0 ? -1 : mask_words(pixels); }
+
+extern "C" int pe_pack_masks(const uint8_t* masks, int64_t N, int64_t HW, uint32_t* bits, void* stream_ptr) {
+  PE_REQUIRE(N >= 0 && HW >= 0, "invalid mask shapes");
+  const i64 stride = mask_words(HW), total = N * stride;
+  if (total == 0) return kOk;
+  const int aligned = HW % 16 == 0 && reinterpret_cast<uintptr_t>(masks) % 16 == 0;
+  const i64 blocks = std::min<i64>((total + 255) / 256, 64LL * compute_units());
+  pack_mask_bits<<<static_cast<unsigned>(blocks), 256, 0, static_cast<hipStream_t>(stream_ptr)>>>(
+      masks, N, HW, stride, aligned, bits);
+  PE_LAUNCH_CHECK();
+  return kOk;
+}
+
+extern "C" int pe_mask_overlap_bits_batch(int64_t B, const uint32_t* const* pm, const uint32_t* const* gm,
+                                          const int64_t* det_off, const int64_t* gt_off, const int64_t* inter_off,
+                                          const int64_t* pixels, int64_t HW, int64_t max_d, int64_t max_g,
+                                          int64_t max_hw, int64_t total_d, int64_t total_g, int64_t total_inter,
+                                          int32_t* inter, int32_t* pa, int32_t* ga, void* stream_ptr) {
+  PE_REQUIRE(B >= 0 && B < INT_MAX && max_d >= 0 && max_d < INT_MAX && max_g >= 0 && max_g < INT_MAX &&
+                 max_hw >= 0 && total_d >= 0 && total_g >= 0 && total_inter >= 0,
+             "invalid mask batch shapes");
+  PE_REQUIRE(pixels || HW == max_hw, "uniform batches pass HW == max_hw");
+  PE_REQUIRE(B == 0 || (pm && gm && det_off && gt_off && inter_off), "missing batch tables");
+  OverlapArgs o{};
+  o.pm_table = reinterpret_cast<const uint8_t* const*>(pm);
+  o.gm_table = reinterpret_cast<const uint8_t* const*>(gm);
+  o.det_off = reinterpret_cast<const i64*>(det_off);
+  o.gt_off = reinterpret_cast<const i64*>(gt_off);
+  o.inter_off = reinterpret_cast<const i64*>(inter_off);
+  o.pixels = reinterpret_cast<const i64*>(pixels);
+  o.HW = HW;
+  o.inter = inter;
+  o.pa = pa;
+  o.ga = ga;
+  return launch_overlap_bits(o, B, max_d, max_g, max_hw, total_d, total_g, total_inter,
+                             static_cast<hipStream_t>(stream_ptr));
+}
+
 extern "C" int64_t pe_detection_histogram_workspace_bytes(int64_t B, int64_t packed_pixels) {
   if (packed_pixels <= 0) return 0;
   return static_cast<int64_t>(align_up(4 * static_cast<size_t>(tile_list_ints(B, packed_pixels, 16 * kDetThreads)),
                                        256));
 }
 
-extern "C" int pe_detection_histogram(int64_t B, const uint8_t* const* masks, const int64_t* det_off,
+static int detection_histogram_launch(int64_t B, const uint8_t* const* masks, const int64_t* det_off,
                                       const int64_t* order, const int32_t* target, const int64_t* pix_off,
                                       int64_t HW, int64_t S, int64_t max_dets, int64_t total_dets,
-                                      int64_t* pairs, int aligned, void* workspace, int64_t workspace_bytes,
-                                      void* stream_ptr) {
+                                      int64_t* pairs, int aligned, bool bits, void* workspace,
+                                      int64_t workspace_bytes, void* stream_ptr) {
   PE_REQUIRE(B >= 0 && B < INT_MAX && HW > 0 && S > 0 && S < INT_MAX && max_dets >= 0 && total_dets >= 0,
              "invalid shapes");
   PE_REQUIRE(!pix_off || workspace_bytes >= pe_detection_histogram_workspace_bytes(B, HW),
@@ -1118,7 +1397,7 @@ extern "C" int pe_detection_histogram(int64_t B, const uint8_t* const* masks, co
   }
   PE_REQUIRE(blocks < INT_MAX, "too many workgroups");
   const size_t lds = static_cast<size_t>((group * S + 1) / 2 + 3) / 4 * 16;
-  auto kernel = persist ? v.persistent : v.kernel;
+  auto kernel = persist ? v.persistent[bits] : v.kernel[bits];
   if (lds > 64 * 1024)
     PE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
                                static_cast<int>(lds)));
@@ -1126,6 +1405,25 @@ extern "C" int pe_detection_histogram(int64_t B, const uint8_t* const* masks, co
   kernel<<<static_cast<unsigned>(blocks), kDetThreads, lds, stream>>>(a);
   PE_LAUNCH_CHECK();
   return kOk;
+}
+
+extern "C" int pe_detection_histogram(int64_t B, const uint8_t* const* masks, const int64_t* det_off,
+                                      const int64_t* order, const int32_t* target, const int64_t* pix_off,
+                                      int64_t HW, int64_t S, int64_t max_dets, int64_t total_dets,
+                                      int64_t* pairs, int aligned, void* workspace, int64_t workspace_bytes,
+                                      void* stream_ptr) {
+  return detection_histogram_launch(B, masks, det_off, order, target, pix_off, HW, S, max_dets, total_dets, pairs,
+                                    aligned, false, workspace, workspace_bytes, stream_ptr);
+}
+
+extern "C" int pe_detection_histogram_bits(int64_t B, const uint32_t* const* masks, const int64_t* det_off,
+                                           const int64_t* order, const int32_t* target, const int64_t* pix_off,
+                                           int64_t HW, int64_t S, int64_t max_dets, int64_t total_dets,
+                                           int64_t* pairs, void* workspace, int64_t workspace_bytes,
+                                           void* stream_ptr) {
+  return detection_histogram_launch(B, reinterpret_cast<const uint8_t* const*>(masks), det_off, order, target,
+                                    pix_off, HW, S, max_dets, total_dets, pairs, 1, true, workspace,
+                                    workspace_bytes, stream_ptr);
 }
 
 extern "C" int pe_coco_match(int64_t B, int64_t C, const int64_t* det_off, const int64_t* gt_off,

@@ -6,7 +6,7 @@ hand-written HIP kernels behind a C ABI (libpanoptic_hip.so).
 """
 
 from .evaluator import PanopticBatch, PanopticEvaluator, panoptic_quality
-from .instance import InstanceBatch, InstanceEvaluator
+from .instance import InstanceBatch, InstanceEvaluator, PackedMasks, pack_masks
 from .combined import SegmentationEvaluator
 from .prefix import segmentation_prefixes
 
@@ -18,4 +18,6 @@ __all__ = [
     "InstanceEvaluator",
     "SegmentationEvaluator",
     "segmentation_prefixes",
+    "PackedMasks",
+    "pack_masks",
 ]

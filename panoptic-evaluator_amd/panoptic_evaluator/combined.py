@@ -15,7 +15,7 @@ import torch
 
 from . import _native
 from .evaluator import PanopticBatch, PanopticEvaluator, _prepare
-from .instance import InstanceBatch, InstanceEvaluator, offsets, raise_instance_error, score_order
+from .instance import InstanceBatch, InstanceEvaluator, PackedMasks, as_bits, offsets, raise_instance_error, score_order
 
 
 class SegmentationEvaluator:
@@ -60,9 +60,10 @@ class SegmentationEvaluator:
         B, S = len(sizes), classes.shape[1]
         ranked = score_order(prepared, device)
         det_counts, N, det_off = ranked.counts, sum(ranked.counts), ranked.det_off
-        pointers = _native.device_table([image.masks.data_ptr() for image in prepared], device)
-        aligned = int(all((h * w) % 16 == 0 for h, w in sizes)
-                      and all(image.masks.data_ptr() % 16 == 0 for image in prepared))
+        bits = any(isinstance(image.masks, PackedMasks) for image in prepared)
+        masks = [as_bits(image.masks) for image in prepared] if bits else [image.masks for image in prepared]
+        pointers = _native.device_table([m.data_ptr() for m in masks], device)
+        aligned = int(all((h * w) % 16 == 0 for h, w in sizes) and all(m.data_ptr() % 16 == 0 for m in masks))
         pairs = torch.empty((N, S), dtype=torch.int64, device=device)
         if target.packed:
             # Packed images: pixel offsets, the pixel total and a tile-list workspace.
@@ -72,10 +73,14 @@ class SegmentationEvaluator:
             workspace = torch.empty(max(size, 256), dtype=torch.uint8, device=device)
         else:
             pix_off, HW, workspace = None, sizes[0][0] * sizes[0][1], None
-        _native.call("pe_detection_histogram", B, pointers.data_ptr(), det_off.data_ptr(), ranked.order.data_ptr(),
-                     maps.data_ptr(), _native.ptr(pix_off), HW, S, max(det_counts, default=0), N,
-                     pairs.data_ptr(), aligned, _native.ptr(workspace),
-                     0 if workspace is None else workspace.numel(), stream)
+        common = (B, pointers.data_ptr(), det_off.data_ptr(), ranked.order.data_ptr(), maps.data_ptr(),
+                  _native.ptr(pix_off), HW, S, max(det_counts, default=0), N, pairs.data_ptr())
+        ws = (_native.ptr(workspace), 0 if workspace is None else workspace.numel(), stream)
+        if bits:
+            # Bit rows (bool images are packed first): 1/8 of the mask bytes.
+            _native.call("pe_detection_histogram_bits", *common, *ws)
+        else:
+            _native.call("pe_detection_histogram", *common, aligned, *ws)
         # Full detection areas, including void and stuff pixels.
         det_area = torch.empty(N, dtype=torch.float64, device=device)
         _native.call("pe_row_sums", pairs.data_ptr(), N, S, det_area.data_ptr(), stream)

@@ -21,6 +21,12 @@ for _name, _sig in {
     "pe_mask_overlap": (_int, [_ptr, _i64, _ptr, _i64, _i64, _ptr, _ptr, _ptr, _ptr]),
     "pe_mask_overlap_batch": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _i64, _i64, _i64,
                                      _i64, _ptr, _ptr, _ptr, _int, _ptr]),
+    "pe_mask_words": (_i64, [_i64]),
+    "pe_pack_masks": (_int, [_ptr, _i64, _i64, _ptr, _ptr]),
+    "pe_mask_overlap_bits_batch": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _i64, _i64,
+                                          _i64, _i64, _ptr, _ptr, _ptr, _ptr]),
+    "pe_detection_histogram_bits": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _i64, _ptr,
+                                           _ptr, _i64, _ptr]),
     "pe_detection_histogram_workspace_bytes": (_i64, [_i64, _i64]),
     "pe_detection_histogram": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _i64, _ptr, _int,
                                       _ptr, _i64, _ptr]),
@@ -40,10 +46,91 @@ SUMMARY_KEYS = ("ap", "ap50", "ap75", "ap_small", "ap_medium", "ap_large", "ar1"
                 "ar_medium", "ar_large")
 
 
+def mask_words(pixels: int) -> int:
+    """32-bit words per bit-packed mask row of ``pixels`` pixels (rows are
+    zero-padded to a multiple of 128 pixels, so every row is 16-byte aligned)."""
+    return (int(pixels) + 127) // 128 * 4
+
+
+@dataclass(frozen=True)
+class PackedMasks:
+    """N x H x W instance masks stored as bits, one bit per pixel (no
+    counterpart in the reference, whose masks are bool bytes): ``bits`` is a
+    contiguous int32 N x mask_words(H * W) device tensor in which bit p % 32
+    of word p // 32 of row n is pixel p = y * W + x of mask n; bits past H * W
+    are zero.  InstanceBatch accepts it wherever it accepts a bool mask
+    tensor, and the evaluators then read 1/8 of the bytes."""
+
+    bits: torch.Tensor
+    height: int
+    width: int
+
+    def __post_init__(self):
+        if self.height <= 0 or self.width <= 0:
+            raise ValueError("packed masks need positive image dimensions")
+        words = mask_words(self.height * self.width)
+        bits = self.bits
+        if (bits.ndim != 2 or bits.dtype != torch.int32 or bits.shape[1] != words or not bits.is_contiguous()
+                or bits.data_ptr() % 16):
+            raise ValueError("packed mask bits must be a contiguous 16-byte aligned int32 N x mask_words(H * W) "
+                             "tensor")
+
+    @classmethod
+    def from_bool(cls, masks: torch.Tensor) -> PackedMasks:
+        """Pack an N x H x W bool device tensor with one streaming kernel."""
+        if masks.ndim != 3 or masks.dtype != torch.bool or min(masks.shape[1:]) <= 0:
+            raise ValueError("masks must be bool N x H x W with positive image dimensions")
+        device = _native.require_gpu(masks.device)
+        masks = masks.contiguous()
+        N, H, W = masks.shape
+        bits = torch.empty((N, mask_words(H * W)), dtype=torch.int32, device=device)
+        _native.call("pe_pack_masks", masks.data_ptr(), N, H * W, bits.data_ptr(), _native.stream(device))
+        return cls(bits, H, W)
+
+    def to_bool(self) -> torch.Tensor:
+        """The N x H x W bool masks (torch ops; for inspection and tests)."""
+        shifts = torch.arange(32, dtype=torch.int32, device=self.bits.device)
+        flags = (self.bits[:, :, None] >> shifts) & 1
+        N, HW = self.bits.shape[0], self.height * self.width
+        return flags.reshape(N, -1)[:, :HW].bool().reshape(N, self.height, self.width)
+
+    @property
+    def shape(self) -> torch.Size:
+        return torch.Size((self.bits.shape[0], self.height, self.width))
+
+    @property
+    def ndim(self) -> int:
+        return 3
+
+    @property
+    def dtype(self) -> torch.dtype:
+        return self.bits.dtype
+
+    @property
+    def device(self) -> torch.device:
+        return self.bits.device
+
+    def data_ptr(self) -> int:
+        return self.bits.data_ptr()
+
+    def contiguous(self) -> PackedMasks:
+        return self
+
+
+def pack_masks(masks: torch.Tensor) -> PackedMasks:
+    """Bit-pack an N x H x W bool device tensor (PackedMasks.from_bool)."""
+    return PackedMasks.from_bool(masks)
+
+
+def as_bits(masks) -> PackedMasks:
+    return masks if isinstance(masks, PackedMasks) else PackedMasks.from_bool(masks)
+
+
 @dataclass(frozen=True)
 class InstanceBatch:
     """One N x H x W bool tensor per image, with N category indices.
 
+    A PackedMasks may stand in for any image's bool tensor (one bit per pixel).
     Predictions require floating point scores.  Targets accept crowd flags and
     annotation areas (COCO's area filtering uses annotation area for targets).
     Masks may overlap.  Empty images use tensors with N = 0.
@@ -157,7 +244,8 @@ class InstanceEvaluator:
             raise ValueError("predictions require scores")
         prepared = []
         for b, (masks, classes) in enumerate(zip(batch.masks, batch.classes)):
-            if masks.ndim != 3 or masks.dtype != torch.bool or min(masks.shape[1:]) <= 0:
+            if not isinstance(masks, PackedMasks) and (
+                    masks.ndim != 3 or masks.dtype != torch.bool or min(masks.shape[1:]) <= 0):
                 raise ValueError("masks must be bool N x H x W with positive image dimensions")
             n = masks.shape[0]
             if classes.shape != (n,) or classes.dtype not in (torch.int32, torch.int64):
@@ -213,16 +301,27 @@ class InstanceEvaluator:
         inter = torch.empty(total_inter, dtype=torch.int32, device=device)
         pa = torch.empty(N, dtype=torch.int32, device=device)
         ga = torch.empty(total_g, dtype=torch.int32, device=device)
-        aligned = int(all(n % 16 == 0 for n in pixels)
-                      and all(x.masks.data_ptr() % 16 == 0 for x in predictions + targets))
-        _native.call("pe_mask_overlap_batch", len(predictions),
-                     _native.device_table([p.masks.data_ptr() for p in predictions], device).data_ptr(),
-                     _native.device_table([g.masks.data_ptr() for g in targets], device).data_ptr(),
-                     ranked.det_off.data_ptr(), gt_off.data_ptr(), inter_off.data_ptr(),
-                     None if uniform else _native.device_table(pixels, device).data_ptr(),
-                     pixels[0] if pixels else 0, max(det_counts, default=0), max(gt_counts, default=0),
-                     max(pixels, default=0), N, total_g, total_inter, inter.data_ptr(), pa.data_ptr(),
-                     ga.data_ptr(), aligned, stream)
+        shape_args = (ranked.det_off.data_ptr(), gt_off.data_ptr(), inter_off.data_ptr(),
+                      None if uniform else _native.device_table(pixels, device).data_ptr(),
+                      pixels[0] if pixels else 0, max(det_counts, default=0), max(gt_counts, default=0),
+                      max(pixels, default=0), N, total_g, total_inter, inter.data_ptr(), pa.data_ptr(),
+                      ga.data_ptr())
+        if any(isinstance(x.masks, PackedMasks) for x in predictions + targets):
+            # Bit rows on both sides (bool images are packed first): popcounts
+            # of ANDed words instead of the int8 MFMA contraction.
+            pred_bits = [as_bits(p.masks) for p in predictions]
+            gt_bits = [as_bits(g.masks) for g in targets]
+            _native.call("pe_mask_overlap_bits_batch", len(predictions),
+                         _native.device_table([m.data_ptr() for m in pred_bits], device).data_ptr(),
+                         _native.device_table([m.data_ptr() for m in gt_bits], device).data_ptr(),
+                         *shape_args, stream)
+        else:
+            aligned = int(all(n % 16 == 0 for n in pixels)
+                          and all(x.masks.data_ptr() % 16 == 0 for x in predictions + targets))
+            _native.call("pe_mask_overlap_batch", len(predictions),
+                         _native.device_table([p.masks.data_ptr() for p in predictions], device).data_ptr(),
+                         _native.device_table([g.masks.data_ptr() for g in targets], device).data_ptr(),
+                         *shape_args, aligned, stream)
         # Detection pixel areas in score order.
         first, base = 0, []
         for count in ranked.counts:

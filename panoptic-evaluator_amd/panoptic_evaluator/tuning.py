@@ -1,11 +1,11 @@
 """Launch-shape autotuner for the streaming kernels.
 
 Each tuning knob of libpanoptic_hip.so ("joint_histogram",
-"detection_histogram", "mask_overlap_mfma", "packed_grid", "detection_group")
-has a fixed list of variants -- pixel tile per workgroup, masks in flight per
-lane, GT tiles and pixel chunk per MFMA wave, persistent grid size over packed
-batches, detections per workgroup (LDS bins) -- that compute identical
-results.  ``autotune`` times every
+"detection_histogram", "mask_overlap_mfma", "packed_grid", "detection_group",
+"mask_overlap_bits") has a fixed list of variants -- pixel tile per
+workgroup, masks in flight per lane, mask groups and pixel range per
+contraction workgroup, persistent grid size over packed batches, detections
+per workgroup (LDS bins) -- that compute identical results.  ``autotune`` times every
 variant on the caller's own workload with HIP events recorded around the
 kernel's launches and keeps the fastest; the selection is process-wide.
 Choices can be saved as JSON and applied again with ``apply``.
@@ -21,7 +21,8 @@ import torch
 
 from . import _native
 
-KERNELS = ("joint_histogram", "detection_histogram", "mask_overlap_mfma", "packed_grid", "detection_group")
+KERNELS = ("joint_histogram", "detection_histogram", "mask_overlap_mfma", "packed_grid", "detection_group",
+           "mask_overlap_bits")
 
 # Kernels whose time decides each knob ("packed_grid" shapes the persistent
 # grids of both histograms over packed batches of uneven images).
