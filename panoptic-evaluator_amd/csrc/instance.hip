@@ -28,6 +28,7 @@
 
 #include <cfloat>
 #include <climits>
+#include <type_traits>
 #include <cmath>
 
 #include "common.h"
@@ -241,14 +242,32 @@ __global__ __launch_bounds__(64) void mask_overlap_mfma(OverlapArgs o) {
 // multiplies detection tile w by every GT tile out of LDS.  Rows are padded
 // by 16 bytes so that the 16 rows of an MFMA operand fall in distinct banks.
 // The next block's loads are in flight while the current one is multiplied.
-template <int DT, int GT, int KB, int Chunk>
+// Expands 32 mask bits into 32 bytes of 0/1 (byte j = bit j): each nibble
+// n becomes (n * 0x00204081) & 0x01010101, the four shifted copies of n
+// (by 0, 7, 14, 21 bits) never overlapping.
+__device__ __forceinline__ void expand_bits(uint32_t w, v4i& lo, v4i& hi) {
+  uint32_t d[8];
+#pragma unroll
+  for (int n = 0; n < 8; ++n) d[n] = (((w >> (4 * n)) & 0xFu) * 0x00204081u) & 0x01010101u;
+  lo = v4i{static_cast<int>(d[0]), static_cast<int>(d[1]), static_cast<int>(d[2]), static_cast<int>(d[3])};
+  hi = v4i{static_cast<int>(d[4]), static_cast<int>(d[5]), static_cast<int>(d[6]), static_cast<int>(d[7])};
+}
+
+// BITS: the masks are bit rows (mask_words(HW) words each); every 32-pixel
+// word is expanded to 0/1 bytes on its way into LDS, so the same int8 MFMA
+// loop runs on 1/8 of the HBM bytes.  Ahead: blocks whose loads are in
+// flight (bit rows are so short per block that one block of look-ahead does
+// not cover the memory latency).
+template <int DT, int GT, int KB, int Chunk, bool BITS, int Ahead = 1>
 __global__ __launch_bounds__(64 * DT) void mask_overlap_staged(OverlapArgs o) {
   constexpr int kThreads = 64 * DT;
   constexpr int kRows = 16 * (DT + GT);
   constexpr int kStride = KB + 16;
-  constexpr int kPerRow = KB / 16;  // 16-byte pieces of one row's block
+  constexpr int kPiecePx = BITS ? 32 : 16;  // pixels per loaded piece (one word / 16 bytes)
+  constexpr int kPerRow = KB / kPiecePx;    // pieces of one row's block
   constexpr int kSlots = kRows * kPerRow / kThreads;
-  static_assert(KB % 64 == 0 && Chunk % KB == 0, "blocks are whole MFMA steps");
+  using Piece = typename std::conditional<BITS, uint32_t, v4i>::type;
+  static_assert(KB % 64 == 0 && Chunk % (KB * Ahead) == 0, "blocks are whole MFMA steps");
   static_assert(kThreads % kPerRow == 0 && kPerRow <= kWave && (kPerRow & (kPerRow - 1)) == 0,
                 "the pieces of one row sit in one wave");
   static_assert(kRows * kPerRow % kThreads == 0, "every thread loads the same number of pieces");
@@ -259,12 +278,13 @@ __global__ __launch_bounds__(64 * DT) void mask_overlap_staged(OverlapArgs o) {
   const int D = it.D, G = it.G;
   const int d_base = it.dgroup * 16 * DT, g_base = it.ggroup * 16 * GT;
   const i64 HW = it.HW;
+  const i64 row_len = BITS ? 4 * mask_words(HW) : HW;  // bytes per mask row
   const i64 k_begin = static_cast<i64>(it.chunk) * Chunk;
   const i64 k_end = min(k_begin + Chunk, HW);
   const bool al = o.aligned != 0;
   // Slot i of this thread: piece tid + i * kThreads of the block, i.e. row
-  // (tid + i * kThreads) / kPerRow at byte (tid % kPerRow) * 16.
-  const int col = (tid % kPerRow) * 16;
+  // (tid + i * kThreads) / kPerRow at pixel (tid % kPerRow) * kPiecePx.
+  const int col = (tid % kPerRow) * kPiecePx;
   const uint8_t* rowp[kSlots];
   bool rvalid[kSlots];
   int lds[kSlots];
@@ -275,10 +295,20 @@ __global__ __launch_bounds__(64 * DT) void mask_overlap_staged(OverlapArgs o) {
     const bool det = r < 16 * DT;
     const int idx = det ? d_base + r : g_base + r - 16 * DT;
     rvalid[i] = idx < (det ? D : G);
-    rowp[i] = (det ? it.pm : it.gm) + static_cast<i64>(rvalid[i] ? idx : 0) * HW;
+    rowp[i] = (det ? it.pm : it.gm) + static_cast<i64>(rvalid[i] ? idx : 0) * row_len;
     lds[i] = r * kStride + col;
     area[i] = 0;
   }
+  auto load = [&](int i, i64 k) -> Piece {
+    if constexpr (BITS) {
+      // Bits past HW are zero in every row, so a partial last word is whole.
+      return rvalid[i] && k < k_end
+                 ? __builtin_nontemporal_load(as_global(reinterpret_cast<const uint32_t*>(rowp[i])) + (k >> 5))
+                 : 0u;
+    } else {
+      return load16(rowp[i], k, k_end, rvalid[i], al);
+    }
+  };
   int tiles = G - g_base;
   tiles = tiles <= 0 ? 0 : min(GT, (tiles + 15) / 16);
   const bool compute = d_base + wave * 16 < D;
@@ -288,30 +318,46 @@ __global__ __launch_bounds__(64 * DT) void mask_overlap_staged(OverlapArgs o) {
   v4i acc[GT];
 #pragma unroll
   for (int t = 0; t < GT; ++t) acc[t] = v4i{0, 0, 0, 0};
-  v4i buf[kSlots];
+  Piece buf[Ahead][kSlots];
 #pragma unroll
-  for (int i = 0; i < kSlots; ++i) buf[i] = load16(rowp[i], k_begin + col, k_end, rvalid[i], al);
-  for (i64 k = k_begin; k < k_end; k += KB) {
-    __syncthreads();  // the previous block has been multiplied
+  for (int ahead = 0; ahead < Ahead; ++ahead)
 #pragma unroll
-    for (int i = 0; i < kSlots; ++i) {
-      *reinterpret_cast<v4i*>(tile + lds[i]) = buf[i];
-      area[i] += byte_sum4(buf[i].x) + byte_sum4(buf[i].y) + byte_sum4(buf[i].z) + byte_sum4(buf[i].w);
-    }
-    __syncthreads();
-    if (k + KB < k_end) {
+    for (int i = 0; i < kSlots; ++i) buf[ahead][i] = load(i, k_begin + ahead * KB + col);
+  for (i64 k0 = k_begin; k0 < k_end; k0 += KB * Ahead) {
 #pragma unroll
-      for (int i = 0; i < kSlots; ++i) buf[i] = load16(rowp[i], k + KB + col, k_end, rvalid[i], al);
-    }
-    if (compute) {
+    for (int ahead = 0; ahead < Ahead; ++ahead) {
+      const i64 k = k0 + ahead * KB;
+      if (k >= k_end) break;  // uniform
+      __syncthreads();        // the previous block has been multiplied
 #pragma unroll
-      for (int step = 0; step < KB / 64; ++step) {
-        const v4i a = *reinterpret_cast<const v4i*>(a_row + step * 64);
+      for (int i = 0; i < kSlots; ++i) {
+        if constexpr (BITS) {
+          v4i lo, hi;
+          expand_bits(buf[ahead][i], lo, hi);
+          *reinterpret_cast<v4i*>(tile + lds[i]) = lo;
+          *reinterpret_cast<v4i*>(tile + lds[i] + 16) = hi;
+          area[i] += __builtin_popcount(buf[ahead][i]);
+        } else {
+          const v4i v = buf[ahead][i];
+          *reinterpret_cast<v4i*>(tile + lds[i]) = v;
+          area[i] += byte_sum4(v.x) + byte_sum4(v.y) + byte_sum4(v.z) + byte_sum4(v.w);
+        }
+      }
+      __syncthreads();
+      if (k + Ahead * KB < k_end) {
 #pragma unroll
-        for (int t = 0; t < GT; ++t) {
-          if (t >= tiles) break;
-          const v4i b = *reinterpret_cast<const v4i*>(b_row + t * 16 * kStride + step * 64);
-          acc[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc[t], 0, 0, 0);
+        for (int i = 0; i < kSlots; ++i) buf[ahead][i] = load(i, k + Ahead * KB + col);
+      }
+      if (compute) {
+#pragma unroll
+        for (int step = 0; step < KB / 64; ++step) {
+          const v4i a = *reinterpret_cast<const v4i*>(a_row + step * 64);
+#pragma unroll
+          for (int t = 0; t < GT; ++t) {
+            if (t >= tiles) break;
+            const v4i b = *reinterpret_cast<const v4i*>(b_row + t * 16 * kStride + step * 64);
+            acc[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc[t], 0, 0, 0);
+          }
         }
       }
     }
@@ -348,10 +394,10 @@ struct OverlapVariant {
 };
 // Order matches the tuning registry's descriptions.
 const OverlapVariant kOverlap[] = {
-    {128, 64, 8192, 512, true, mask_overlap_staged<8, 4, 256, 8192>},
-    {128, 64, 4096, 512, true, mask_overlap_staged<8, 4, 256, 4096>},
-    {64, 64, 8192, 256, true, mask_overlap_staged<4, 4, 256, 8192>},
-    {128, 32, 8192, 512, true, mask_overlap_staged<8, 2, 512, 8192>},
+    {128, 64, 8192, 512, true, mask_overlap_staged<8, 4, 256, 8192, false>},
+    {128, 64, 4096, 512, true, mask_overlap_staged<8, 4, 256, 4096, false>},
+    {64, 64, 8192, 256, true, mask_overlap_staged<4, 4, 256, 8192, false>},
+    {128, 32, 8192, 512, true, mask_overlap_staged<8, 2, 512, 8192, false>},
     {16, 64, 4096, 64, true, mask_overlap_mfma<4, 4096, 4>},
     {16, 64, 8192, 64, true, mask_overlap_mfma<4, 8192, 4>},
     {16, 128, 16384, 64, true, mask_overlap_mfma<8, 16384, 1>},
@@ -545,15 +591,19 @@ __global__ __launch_bounds__(256) void mask_overlap_bits(OverlapArgs o) {
 }
 
 struct BitsVariant {
-  int dets, gts, chunk_words;
+  int dets, gts;
+  i64 chunk_px;
+  int threads;
   void (*kernel)(OverlapArgs);
 };
 // Order matches the tuning registry's descriptions.
 const BitsVariant kOverlapBits[] = {
-    {128, 64, 512, mask_overlap_bits<8, 4, 32, 512>},
-    {128, 64, 1024, mask_overlap_bits<8, 4, 32, 1024>},
-    {64, 64, 512, mask_overlap_bits<4, 4, 32, 512>},
-    {128, 32, 512, mask_overlap_bits<8, 2, 32, 512>},
+    {128, 64, 4096, 512, mask_overlap_staged<8, 4, 256, 4096, true, 1>},
+    {128, 64, 4096, 512, mask_overlap_staged<8, 4, 256, 4096, true, 4>},
+    {128, 64, 8192, 512, mask_overlap_staged<8, 4, 256, 8192, true, 4>},
+    {128, 64, 16384, 512, mask_overlap_staged<8, 4, 256, 16384, true, 8>},
+    {128, 64, 16384, 256, mask_overlap_bits<8, 4, 32, 512>},
+    {64, 64, 16384, 256, mask_overlap_bits<4, 4, 32, 512>},
 };
 
 int launch_overlap_bits(OverlapArgs o, int64_t B, int64_t max_d, int64_t max_g, int64_t max_hw, int64_t total_d,
@@ -564,7 +614,7 @@ int launch_overlap_bits(OverlapArgs o, int64_t B, int64_t max_d, int64_t max_g, 
   if (o.ga && total_g && (rc = launch_zero(o.ga, 4 * static_cast<size_t>(total_g), stream))) return rc;
   if (B == 0 || max_hw == 0 || (max_d == 0 && max_g == 0)) return kOk;
   const BitsVariant& v = kOverlapBits[tuned_variant(kTuneMaskOverlapBits)];
-  const i64 chunks = (mask_words(max_hw) + v.chunk_words - 1) / v.chunk_words;
+  const i64 chunks = (max_hw + v.chunk_px - 1) / v.chunk_px;
   const i64 dgroups = std::max<i64>(1, (max_d + v.dets - 1) / v.dets);
   const i64 ggroups = std::max<i64>(1, (max_g + v.gts - 1) / v.gts);
   const i64 items = dgroups * ggroups * B * chunks;
@@ -575,7 +625,7 @@ int launch_overlap_bits(OverlapArgs o, int64_t B, int64_t max_d, int64_t max_g, 
   o.items = static_cast<int>(items);
   o.xcd_span = static_cast<int>((items + kXcds - 1) / kXcds);
   ProfileScope timing("mask_overlap_bits", stream);
-  v.kernel<<<static_cast<unsigned>(static_cast<i64>(o.xcd_span) * kXcds), 256, 0, stream>>>(o);
+  v.kernel<<<static_cast<unsigned>(static_cast<i64>(o.xcd_span) * kXcds), v.threads, 0, stream>>>(o);
   PE_LAUNCH_CHECK();
   return kOk;
 }

@@ -92,7 +92,7 @@ class PackedMasks:
         shifts = torch.arange(32, dtype=torch.int32, device=self.bits.device)
         flags = (self.bits[:, :, None] >> shifts) & 1
         N, HW = self.bits.shape[0], self.height * self.width
-        return flags.reshape(N, -1)[:, :HW].bool().reshape(N, self.height, self.width)
+        return flags.reshape(N, 32 * self.bits.shape[1])[:, :HW].bool().reshape(N, self.height, self.width)
 
     @property
     def shape(self) -> torch.Size:
