@@ -1207,34 +1207,47 @@ __global__ __launch_bounds__(kApThreads) void ap_small(ApSmallArgs s) {
   int32_t* idx = reinterpret_cast<int32_t*>(keys + s.sort_slots);
   uint8_t* s_rank = reinterpret_cast<uint8_t*>(idx + s.sort_slots);
   int8_t* s_flag = reinterpret_cast<int8_t*>(s_rank + kApChunk);
-  __shared__ int wave_count[kWaves];
-  __shared__ int total;
-  if (tid == 0) total = 0;
-  __syncthreads();
-  // Gather this class's records in index order.
-  for (int base = 0; base < s.N; base += kApThreads) {
-    const int i = base + tid;
-    bool mine = false;
-    if (i < s.N) {
-      const int r = a.ranks[i];
-      mine = s.classes[i] == c && r > 0 && r <= 100;
-    }
-    const u64 ballot = __ballot(mine);
-    if (lane == 0) wave_count[wave] = __popcll(ballot);
-    __syncthreads();
-    int offset = total;
-    for (int w = 0; w < wave; ++w) offset += wave_count[w];
-    if (mine) {
-      const int slot = offset + __popcll(ballot & ((1ULL << lane) - 1));
-      idx[slot] = i;
-      keys[slot] = score_key_desc(s.scores[i]);
-    }
-    __syncthreads();
-    if (tid == 0)
-      for (int w = 0; w < kWaves; ++w) total += wave_count[w];
-    __syncthreads();
+  // Gather this class's records in index order: every thread issues its
+  // loads for all of its records (i = it * kApThreads + tid) at once, then a
+  // single barrier publishes the per-(round, wave) counts for the offsets.
+  constexpr int kRounds = kApSmall / kApThreads;
+  __shared__ int round_count[kRounds][kWaves];
+  uint32_t mine = 0;
+  int rank[kRounds], klass[kRounds];
+#pragma unroll
+  for (int it = 0; it < kRounds; ++it) {
+    const int i = it * kApThreads + tid;
+    rank[it] = i < s.N ? a.ranks[i] : 0;
+    klass[it] = i < s.N ? s.classes[i] : -1;
   }
-  const int n = total;
+#pragma unroll
+  for (int it = 0; it < kRounds; ++it)
+    mine |= static_cast<uint32_t>(klass[it] == c && rank[it] > 0 && rank[it] <= 100) << it;
+  double score[kRounds];
+#pragma unroll
+  for (int it = 0; it < kRounds; ++it) score[it] = (mine >> it) & 1u ? s.scores[it * kApThreads + tid] : 0.0;
+  u64 ballot[kRounds];
+#pragma unroll
+  for (int it = 0; it < kRounds; ++it) {
+    ballot[it] = __ballot((mine >> it) & 1u);
+    if (lane == 0) round_count[it][wave] = __popcll(ballot[it]);
+  }
+  __syncthreads();
+  int n = 0;
+#pragma unroll
+  for (int it = 0; it < kRounds; ++it) {
+    int before = n;
+    for (int w = 0; w < kWaves; ++w) {
+      const int count = round_count[it][w];
+      if (w < wave) before += count;
+      n += count;
+    }
+    if ((mine >> it) & 1u) {
+      const int slot = before + __popcll(ballot[it] & ((1ULL << lane) - 1));
+      idx[slot] = it * kApThreads + tid;
+      keys[slot] = score_key_desc(score[it]);
+    }
+  }
   int m = 1;
   while (m < n) m <<= 1;
   for (int e = n + tid; e < m; e += kApThreads) {
