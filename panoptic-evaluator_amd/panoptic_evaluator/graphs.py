@@ -57,7 +57,9 @@ class CapturedUpdate:
         self._record = None
         if instance is not None:
             # The captured update's detection records live in graph memory;
-            # every replay appends a copy of them.
+            # every replay appends them, and they are copied out only if the
+            # evaluator still holds them when the next replay would overwrite
+            # them (a reset() in between makes the copy unnecessary).
             instance._commit = self._keep
         try:
             with torch.cuda.graph(self.graph):
@@ -90,7 +92,12 @@ class CapturedUpdate:
 
     def replay(self):
         """Run the recorded update once more (on the current stream's device)."""
+        if self._record is not None:
+            records = self._instance._records
+            for i, record in enumerate(records):
+                if record is self._record:
+                    records[i] = tuple(t.clone() for t in record)
         self.graph.replay()
         if self._record is not None:
-            self._instance._records.append(tuple(t.clone() for t in self._record))
+            self._instance._records.append(self._record)
         return self.evaluator

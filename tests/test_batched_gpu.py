@@ -193,6 +193,30 @@ def test_graph_replay_accumulates_like_repeated_updates(pe):
         assert_exact(got["instance"][key], want["instance"][key], key)
 
 
+def test_graph_replay_keeps_records_of_refilled_inputs(pe):
+    """Inputs refilled in place between replays: the first replay's detection
+    records must survive the second replay (they are copied out of graph
+    memory before it overwrites them)."""
+    from panoptic_evaluator.graphs import CapturedUpdate
+    batch = uneven(23, 4, detections=(2, 20))
+    isthing = [c < 80 for c in range(133)]
+    prediction, target = packed(pe, batch)
+    scores = [s.cuda() for s in batch.scores]
+    dets = pe.InstanceBatch([m.cuda() for m in batch.masks], [c.cuda() for c in batch.classes], scores)
+    ev = pe.SegmentationEvaluator(133, isthing=isthing, validate=False)
+    captured = CapturedUpdate(ev, prediction, dets, target)
+    eager = pe.SegmentationEvaluator(133, isthing=isthing, validate=False)
+    captured.replay()
+    eager.update(prediction, dets, target)
+    for s in scores:
+        s.copy_(1.0 - s)  # reverse every image's score order in place
+    captured.replay()
+    eager.update(prediction, dets, target)
+    got, want = to_numpy(ev.compute()), to_numpy(eager.compute())
+    for key in ("precision", "recall"):
+        assert_exact(got["instance"][key], want["instance"][key], key)
+
+
 def test_capture_rejects_validation(pe):
     from panoptic_evaluator.graphs import CapturedUpdate
     import workloads
