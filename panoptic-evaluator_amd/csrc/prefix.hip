@@ -352,7 +352,28 @@ __device__ __forceinline__ PrefixView global_view(const PrefixArgs& a, int b, i6
   return v;
 }
 
-// Panoptic and semantic statistics of the visible canvas (snapshot row r).
+// Instance outputs of checkpoint row r for thresholds t0, t0 + step, ...:
+// counts, FN = positives - TP, precision and recall per (threshold, class),
+// over the lanes `first`, `first + stride`, ...
+__device__ __forceinline__ void prefix_instance_row(const PrefixArgs& a, const PrefixView& v, i64 r, int t0, int t_step,
+                                                    int first, int stride) {
+  const int C = a.C, TC = a.T * C;
+  for (int t = t0; t < a.T; t += t_step)
+    for (int c = first; c < C; c += stride) {
+      const int e = t * C + c;
+      const i64 t_ = v.itp[e], f_ = v.ifp[e], n_ = v.positives[c];
+      a.out_itp[r * TC + e] = t_;
+      a.out_ifp[r * TC + e] = f_;
+      a.out_ifn[r * TC + e] = n_ - t_;
+      a.precision[r * TC + e] = static_cast<double>(t_) / static_cast<double>(t_ + f_ < 1 ? 1 : t_ + f_);
+      a.recall[r * TC + e] = static_cast<double>(t_) / static_cast<double>(n_ < 1 ? 1 : n_);
+    }
+}
+
+// Panoptic and semantic statistics of the visible canvas (snapshot row r),
+// plus the instance outputs when INSTANCE (otherwise written by
+// prefix_instance_waves).
+template <bool INSTANCE>
 __device__ __forceinline__ void prefix_snapshot(const PrefixArgs& a, int k, i64 r, int P1, const PrefixView& v,
                                                 const int32_t* cls) {
   const int G = a.G, C = a.C, tid = threadIdx.x;
@@ -417,16 +438,71 @@ __device__ __forceinline__ void prefix_snapshot(const PrefixArgs& a, int k, i64 
       if (v.lab[j] == c) acc += v.val[j];
     sums[c] = acc;
   }
-  const int TC = a.T * C;
-  for (int e = tid; e < TC; e += blockDim.x) {
-    const i64 t_ = v.itp[e], f_ = v.ifp[e], n_ = v.positives[e % C];
-    a.out_itp[r * TC + e] = t_;
-    a.out_ifp[r * TC + e] = f_;
-    a.out_ifn[r * TC + e] = n_ - t_;
-    a.precision[r * TC + e] = static_cast<double>(t_) / static_cast<double>(t_ + f_ < 1 ? 1 : t_ + f_);
-    a.recall[r * TC + e] = static_cast<double>(t_) / static_cast<double>(n_ < 1 ? 1 : n_);
-  }
+  if (INSTANCE) prefix_instance_row(a, v, r, 0, 1, tid, blockDim.x);
   __syncthreads();
+}
+
+// Independent-mask greedy matching of every mask in generation order
+// (prefix.py semantics: same-class non-crowd GT not yet taken, IoU >=
+// threshold, the highest IoU and then the lowest slot wins; a crowd match
+// only ignores), with wave w owning thresholds w, w + waves, ...: the lanes
+// cover the GT slots, a wave argmax picks the match and lane 0 counts it.
+// The matching never reads the canvas, so it runs apart from the canvas
+// loop, without block barriers, and writes its checkpoint rows itself.
+// `used` entries are only ever touched by the lane owning that slot, and the
+// counts only by lane 0, in LDS, so a wave sees its own writes in order.
+__device__ __forceinline__ void prefix_instance_waves(const PrefixArgs& a, const PrefixView& v, int N,
+                                                      const int32_t* raw, const int32_t* cls, i64 kfirst,
+                                                      i64 klast) {
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, waves = blockDim.x / kWave;
+  const int G = a.G, C = a.C;
+  for (int t = wave; t < a.T; t += waves) {
+    const double thr = a.thresholds[t];
+    i64 q = kfirst;
+    if (q < klast && a.points[q] == 0) prefix_instance_row(a, v, q++, t, a.T, lane, kWave);
+    for (int i = 0; i < N; ++i) {
+      const int c = cls[i];
+      if (c >= 0 && c < C && a.things[c]) {
+        const int32_t* row = raw + static_cast<i64>(i) * G;
+        i64 s = 0;
+        for (int g = lane; g < G; g += kWave) s += row[g];
+        const double area = static_cast<double>(wave_sum(s));
+        int best = INT_MAX;
+        double best_iou = -1.0;
+        bool ignored = false;
+        for (int g = lane; g < G; g += kWave) {
+          if (v.gc[g] != c) continue;
+          const i64 inter = row[g];
+          double uni = v.crowd[g] ? area : area + static_cast<double>(v.ga[g]) - static_cast<double>(inter);
+          if (uni < 1.0) uni = 1.0;
+          const double iou = static_cast<double>(inter) / uni;
+          if (iou < thr) continue;
+          if (v.crowd[g]) ignored = true;
+          else if (!v.used[t * G + g] && iou > best_iou) {
+            best = g;
+            best_iou = iou;
+          }
+        }
+#pragma unroll
+        for (int m = kWave / 2; m > 0; m >>= 1) {
+          const double o_iou = __shfl_xor(best_iou, m, kWave);
+          const int o_best = __shfl_xor(best, m, kWave);
+          if (o_iou > best_iou || (o_iou == best_iou && o_best < best)) {
+            best_iou = o_iou;
+            best = o_best;
+          }
+        }
+        ignored = __ballot(ignored) != 0;
+        if (best != INT_MAX) {
+          if (lane == (best & (kWave - 1))) v.used[t * G + best] = 1;
+          if (lane == 0) ++v.itp[t * C + c];
+        } else if (!ignored && lane == 0) {
+          ++v.ifp[t * C + c];
+        }
+      }
+      if (q < klast && a.points[q] == i + 1) prefix_instance_row(a, v, q++, t, a.T, lane, kWave);
+    }
+  }
 }
 
 template <bool STAGED>
@@ -481,25 +557,41 @@ __device__ __forceinline__ void prefix_score_image(const PrefixArgs& a, int b, c
     v.positives[c] = n;
   }
   __syncthreads();
+  if (STAGED) {
+    prefix_instance_waves(a, v, N, raw, cls, kfirst, klast);
+    __syncthreads();
+  }
   i64 q = kfirst;
   if (q < klast && a.points[q] == 0) {
-    prefix_snapshot(a, 0, q, P1, v, cls);
+    prefix_snapshot<!STAGED>(a, 0, q, P1, v, cls);
     ++q;
   }
   for (int i = 0; i < N; ++i) {
     const int32_t* dec_i = dec + static_cast<i64>(G) * tri(i);
     const int width = i + 1;
-    for (int e = tid; e < G * width; e += blockDim.x) {
-      const int32_t d = dec_i[e];
-      if (d) {
+    // Earlier masks lose what mask i covers: G x (i + 1) decrements, eight
+    // loads in flight per thread.
+    constexpr int U = 8;
+    const int total = G * width;
+    for (int e0 = tid; e0 < total; e0 += kPrefixThreads * U) {
+      int32_t d[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = e0 + u * kPrefixThreads;
+        d[u] = e < total ? dec_i[e] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (!d[u]) continue;
+        const int e = e0 + u * kPrefixThreads;
         const int g = e / width;
-        v.pairs[g * P1 + (e - g * width)] -= d;
+        v.pairs[g * P1 + (e - g * width)] -= d[u];
       }
     }
     for (int g = tid; g < G; g += blockDim.x) v.pairs[g * P1 + i + 1] = raw[static_cast<i64>(i) * G + g];
     __syncthreads();
     const int c = cls[i];
-    if (c >= 0 && c < C && a.things[c]) {
+    if (!STAGED && c >= 0 && c < C && a.things[c]) {
       double area = 0.0;
       {
         i64 s = 0;
@@ -532,9 +624,9 @@ __device__ __forceinline__ void prefix_score_image(const PrefixArgs& a, int b, c
         }
       }
     }
-    __syncthreads();
+    if (!STAGED) __syncthreads();
     if (q < klast && a.points[q] == i + 1) {
-      prefix_snapshot(a, i + 1, q, P1, v, cls);
+      prefix_snapshot<!STAGED>(a, i + 1, q, P1, v, cls);
       ++q;
     }
   }
