@@ -377,10 +377,16 @@ template <bool INSTANCE>
 __device__ __forceinline__ void prefix_snapshot(const PrefixArgs& a, int k, i64 r, int P1, const PrefixView& v,
                                                 const int32_t* cls) {
   const int G = a.G, C = a.C, tid = threadIdx.x;
-  for (int j = tid; j <= k; j += blockDim.x) {
+  // kLanes neighbouring lanes share one canvas column and split the GT
+  // slots; their partial results combine with shuffles inside the group.
+  constexpr int kLanes = 8;
+  const int sub = tid % kLanes, first_col = tid / kLanes, col_step = blockDim.x / kLanes;
+  for (int j = first_col; j <= k; j += col_step) {
     i64 s = 0;
-    for (int g = 0; g < G; ++g) s += v.pairs[static_cast<i64>(g) * P1 + j];
-    v.pa[j] = s;
+    for (int g = sub; g < G; g += kLanes) s += v.pairs[static_cast<i64>(g) * P1 + j];
+#pragma unroll
+    for (int m = 1; m < kLanes; m <<= 1) s += __shfl_xor(s, m, kWave);
+    if (sub == 0) v.pa[j] = s;
   }
   for (int g = tid; g < G; g += blockDim.x) v.matched[g] = 0;
   __syncthreads();
@@ -388,14 +394,19 @@ __device__ __forceinline__ void prefix_snapshot(const PrefixArgs& a, int k, i64 
   u64* fp = reinterpret_cast<u64*>(a.fp + r * C);
   u64* fn = reinterpret_cast<u64*>(a.fn + r * C);
   u64* conf = reinterpret_cast<u64*>(a.confusion + r * C * (C + 1));
-  for (int j = tid; j <= k; j += blockDim.x) {
-    const int label = (j > 0 && v.pa[j] > 0) ? cls[j - 1] : -1;
+  for (int j = first_col; j <= k; j += col_step) {
+    const i64 pa = v.pa[j];
+    const int label = (j > 0 && pa > 0) ? cls[j - 1] : -1;
     const bool valid_pred = label >= 0 && label < C;
     const i64 void_overlap = v.pairs[j];
-    i64 ignored = void_overlap;
+    // At most one GT slot can hold more than half of the union (GT segments
+    // are disjoint), so the matched IoU sum has at most one term; for crowd
+    // slots of the label's class the last one (highest slot) decides.
     bool pm = false;
     double sum = 0.0;
-    for (int g = 0; g < G; ++g) {
+    int crowd_g = -1;
+    i64 crowd_inter = 0;
+    for (int g = sub; g < G; g += kLanes) {
       const int tcls = v.gc[g];
       if (tcls < 0 || tcls >= C) continue;
       const i64 inter = v.pairs[static_cast<i64>(g) * P1 + j];
@@ -404,9 +415,10 @@ __device__ __forceinline__ void prefix_snapshot(const PrefixArgs& a, int k, i64 
                                         static_cast<u64>(inter));
       if (!valid_pred || tcls != label) continue;
       if (is_crowd) {
-        ignored = void_overlap + inter;
+        crowd_g = g;
+        crowd_inter = inter;
       } else if (inter) {
-        const i64 uni = static_cast<i64>(v.ga[g]) + v.pa[j] - inter - void_overlap;
+        const i64 uni = static_cast<i64>(v.ga[g]) + pa - inter - void_overlap;
         if (inter > uni / 2) {
           pm = true;
           sum += static_cast<double>(inter) / static_cast<double>(uni);
@@ -414,28 +426,61 @@ __device__ __forceinline__ void prefix_snapshot(const PrefixArgs& a, int k, i64 
         }
       }
     }
-    int out = -1;
-    if (valid_pred) {
-      if (pm) {
-        atomicAdd(tp + label, 1ULL);
-        out = label;
-      } else if (static_cast<double>(ignored) <= 0.5 * static_cast<double>(v.pa[j])) {
-        atomicAdd(fp + label, 1ULL);
+#pragma unroll
+    for (int m = 1; m < kLanes; m <<= 1) {
+      pm |= __shfl_xor(static_cast<int>(pm), m, kWave) != 0;
+      sum += __shfl_xor(sum, m, kWave);
+      const int o_g = __shfl_xor(crowd_g, m, kWave);
+      const i64 o_inter = __shfl_xor(crowd_inter, m, kWave);
+      if (o_g > crowd_g) {
+        crowd_g = o_g;
+        crowd_inter = o_inter;
       }
     }
-    v.lab[j] = out;
-    v.val[j] = sum;
+    if (sub == 0) {
+      const i64 ignored = crowd_g >= 0 ? void_overlap + crowd_inter : void_overlap;
+      int out = -1;
+      if (valid_pred) {
+        if (pm) {
+          atomicAdd(tp + label, 1ULL);
+          out = label;
+        } else if (static_cast<double>(ignored) <= 0.5 * static_cast<double>(pa)) {
+          atomicAdd(fp + label, 1ULL);
+        }
+      }
+      v.lab[j] = out;
+      v.val[j] = sum;
+    }
   }
   __syncthreads();
   for (int g = tid; g < G; g += blockDim.x) {
     const int c = v.gc[g];
     if (c >= 0 && c < C && !v.crowd[g] && !v.matched[g]) atomicAdd(fn + c, 1ULL);
   }
+  // Matched columns in increasing order (wave 0, reusing pa's storage), so
+  // each class folds its IoUs in column order over that short list.
+  int* list = reinterpret_cast<int*>(v.pa);
+  __shared__ int list_len;
+  if (tid < kWave) {
+    int count = 0;
+    for (int j0 = 0; j0 <= k; j0 += kWave) {
+      const int j = j0 + tid;
+      const bool hit = j <= k && v.lab[j] >= 0;
+      const u64 ballot = __ballot(hit);
+      if (hit) list[count + __popcll(ballot & ((1ULL << tid) - 1))] = j;
+      count += __popcll(ballot);
+    }
+    if (tid == 0) list_len = count;
+  }
+  __syncthreads();
   double* sums = a.sums + r * C;
+  const int matched_cols = list_len;
   for (int c = tid; c < C; c += blockDim.x) {
     double acc = 0.0;
-    for (int j = 0; j <= k; ++j)
+    for (int e = 0; e < matched_cols; ++e) {
+      const int j = list[e];
       if (v.lab[j] == c) acc += v.val[j];
+    }
     sums[c] = acc;
   }
   if (INSTANCE) prefix_instance_row(a, v, r, 0, 1, tid, blockDim.x);
