@@ -233,7 +233,8 @@ const char* const kOverlapVariants[] = {
     "registers 16 dets x 64 GT x 4096 px, 4 steps ahead, XCD-grouped",
     "registers 16 dets x 64 GT x 8192 px, 4 steps ahead, XCD-grouped",
     "registers 16 dets x 128 GT x 16384 px, XCD-grouped",
-    "registers 16 dets x 64 GT x 8192 px, 4 steps ahead, spread"};
+    "registers 16 dets x 64 GT x 8192 px, 4 steps ahead, spread",
+    "LDS-staged 128 dets x 64 GT x 4096 px, 256 px blocks, 2 detection tiles per wave"};
 const char* const kGridVariants[] = {"one workgroup per work item", "persistent: 8 workgroups per CU",
                                      "persistent: 4 workgroups per CU"};
 const char* const kGroupVariants[] = {"LDS budget (96 KB)", "128 detections", "64 detections", "32 detections"};
@@ -242,15 +243,16 @@ const char* const kBitsVariants[] = {
     "bits expanded to int8 MFMA operands in LDS, 128 dets x 64 GT x 4096 px, 4 blocks ahead",
     "bits expanded to int8 MFMA operands in LDS, 128 dets x 64 GT x 8192 px, 4 blocks ahead",
     "bits expanded to int8 MFMA operands in LDS, 128 dets x 64 GT x 16384 px, 8 blocks ahead",
+    "bits expanded to int8 MFMA operands in LDS, 128 dets x 64 GT x 4096 px, 2 detection tiles per wave",
     "popcounts of ANDed words, 128 dets x 64 GT x 16384 px",
     "popcounts of ANDed words, 64 dets x 64 GT x 16384 px"};
 const TuneSpec kSpecs[kTuneKernels] = {
     {"detection_histogram", 5, 0, kDetVariants},
     {"joint_histogram", 5, 0, kJointVariants},
-    {"mask_overlap_mfma", 8, 0, kOverlapVariants},
+    {"mask_overlap_mfma", 9, 0, kOverlapVariants},
     {"packed_grid", 3, 0, kGridVariants},
     {"detection_group", 4, 0, kGroupVariants},
-    {"mask_overlap_bits", 6, 0, kBitsVariants},
+    {"mask_overlap_bits", 7, 0, kBitsVariants},
 };
 std::atomic<int> g_selected[kTuneKernels] = {{-1}, {-1}, {-1}, {-1}, {-1}, {-1}};
 

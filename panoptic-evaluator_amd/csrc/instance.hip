@@ -258,9 +258,12 @@ __device__ __forceinline__ void expand_bits(uint32_t w, v4i& lo, v4i& hi) {
 // loop runs on 1/8 of the HBM bytes.  Ahead: blocks whose loads are in
 // flight (bit rows are so short per block that one block of look-ahead does
 // not cover the memory latency).
-template <int DT, int GT, int KB, int Chunk, bool BITS, int Ahead = 1>
-__global__ __launch_bounds__(64 * DT) void mask_overlap_staged(OverlapArgs o) {
-  constexpr int kThreads = 64 * DT;
+// DPW: detection tiles per wave (2 halves the LDS reads of the GT operands
+// per MFMA, with half the waves).
+template <int DT, int GT, int KB, int Chunk, bool BITS, int Ahead = 1, int DPW = 1>
+__global__ __launch_bounds__(64 * DT / DPW) void mask_overlap_staged(OverlapArgs o) {
+  static_assert(DT % DPW == 0, "whole detection tiles per wave");
+  constexpr int kThreads = 64 * DT / DPW;
   constexpr int kRows = 16 * (DT + GT);
   constexpr int kStride = KB + 16;
   constexpr int kPiecePx = BITS ? 32 : 16;  // pixels per loaded piece (one word / 16 bytes)
@@ -311,13 +314,17 @@ __global__ __launch_bounds__(64 * DT) void mask_overlap_staged(OverlapArgs o) {
   };
   int tiles = G - g_base;
   tiles = tiles <= 0 ? 0 : min(GT, (tiles + 15) / 16);
-  const bool compute = d_base + wave * 16 < D;
+  // Wave w multiplies detection tiles w * DPW .. w * DPW + DPW - 1.
+  const int dtiles = min(DPW, (D - d_base - wave * DPW * 16 + 15) / 16);  // uniform per wave; <= 0: idle
+  const bool compute = dtiles > 0;
   const int row = lane & 15, kgroup = lane >> 4;
-  const uint8_t* a_row = tile + (wave * 16 + row) * kStride + kgroup * 16;
+  const uint8_t* a_row = tile + (wave * DPW * 16 + row) * kStride + kgroup * 16;
   const uint8_t* b_row = tile + (16 * DT + row) * kStride + kgroup * 16;
-  v4i acc[GT];
+  v4i acc[DPW][GT];
 #pragma unroll
-  for (int t = 0; t < GT; ++t) acc[t] = v4i{0, 0, 0, 0};
+  for (int d = 0; d < DPW; ++d)
+#pragma unroll
+    for (int t = 0; t < GT; ++t) acc[d][t] = v4i{0, 0, 0, 0};
   Piece buf[Ahead][kSlots];
 #pragma unroll
   for (int ahead = 0; ahead < Ahead; ++ahead)
@@ -351,12 +358,17 @@ __global__ __launch_bounds__(64 * DT) void mask_overlap_staged(OverlapArgs o) {
       if (compute) {
 #pragma unroll
         for (int step = 0; step < KB / 64; ++step) {
-          const v4i a = *reinterpret_cast<const v4i*>(a_row + step * 64);
+          v4i a[DPW];
+#pragma unroll
+          for (int d = 0; d < DPW; ++d)
+            a[d] = d < dtiles ? *reinterpret_cast<const v4i*>(a_row + d * 16 * kStride + step * 64) : v4i{0, 0, 0, 0};
 #pragma unroll
           for (int t = 0; t < GT; ++t) {
             if (t >= tiles) break;
             const v4i b = *reinterpret_cast<const v4i*>(b_row + t * 16 * kStride + step * 64);
-            acc[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc[t], 0, 0, 0);
+#pragma unroll
+            for (int d = 0; d < DPW; ++d)
+              if (d < dtiles) acc[d][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[d], b, acc[d][t], 0, 0, 0);
           }
         }
       }
@@ -364,9 +376,13 @@ __global__ __launch_bounds__(64 * DT) void mask_overlap_staged(OverlapArgs o) {
   }
   if (compute) {
 #pragma unroll
-    for (int t = 0; t < GT; ++t) {
-      if (t >= tiles) break;
-      add_overlap_tile(acc[t], lane, d_base + wave * 16, g_base + t * 16, D, G, it.inter);
+    for (int d = 0; d < DPW; ++d) {
+      if (d >= dtiles) break;
+#pragma unroll
+      for (int t = 0; t < GT; ++t) {
+        if (t >= tiles) break;
+        add_overlap_tile(acc[d][t], lane, d_base + (wave * DPW + d) * 16, g_base + t * 16, D, G, it.inter);
+      }
     }
   }
   // Areas: the kPerRow neighbouring lanes of a slot share its row.
@@ -402,6 +418,7 @@ const OverlapVariant kOverlap[] = {
     {16, 64, 8192, 64, true, mask_overlap_mfma<4, 8192, 4>},
     {16, 128, 16384, 64, true, mask_overlap_mfma<8, 16384, 1>},
     {16, 64, 8192, 64, false, mask_overlap_mfma<4, 8192, 4>},
+    {128, 64, 4096, 256, true, mask_overlap_staged<8, 4, 256, 4096, false, 1, 2>},
 };
 
 int launch_overlap(OverlapArgs o, int64_t B, int64_t max_d, int64_t max_g, int64_t max_hw, int64_t total_d,
@@ -602,6 +619,7 @@ const BitsVariant kOverlapBits[] = {
     {128, 64, 4096, 512, mask_overlap_staged<8, 4, 256, 4096, true, 4>},
     {128, 64, 8192, 512, mask_overlap_staged<8, 4, 256, 8192, true, 4>},
     {128, 64, 16384, 512, mask_overlap_staged<8, 4, 256, 16384, true, 8>},
+    {128, 64, 4096, 256, mask_overlap_staged<8, 4, 256, 4096, true, 1, 2>},
     {128, 64, 16384, 256, mask_overlap_bits<8, 4, 32, 512>},
     {64, 64, 16384, 256, mask_overlap_bits<4, 4, 32, 512>},
 };
