@@ -104,8 +104,24 @@ def ptr(tensor):
     return None if tensor is None else tensor.data_ptr()
 
 
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def stream(device=None) -> int:
-    return torch.cuda.current_stream(device).cuda_stream
+    """The current HIP stream of `device` as an integer handle (every launch
+    asks, so this avoids building a torch Stream object when torch exposes
+    the raw handle)."""
+    if _raw_stream is None:
+        return torch.cuda.current_stream(device).cuda_stream
+    if isinstance(device, torch.device) and device.index is not None:
+        index = device.index
+    elif isinstance(device, int):
+        index = device
+    elif device is None or torch.device(device).index is None:
+        index = torch.cuda.current_device()
+    else:
+        index = torch.device(device).index
+    return _raw_stream(index)
 
 
 def check(code: int) -> int:

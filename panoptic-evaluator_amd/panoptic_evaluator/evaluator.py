@@ -370,14 +370,18 @@ def finalize(tp, fp, fn, sums, confusion, things):
     """Run the FP64 finaliser over K accumulator rows; return K result dicts."""
     K = tp.shape[0]
     per_class, summaries, counts, miou, copies = finalize_arrays(tp, fp, fn, sums, confusion, things)
-    out_tp, out_fp, out_fn, out_sums, out_confusion = copies
+    # Result views come from a few unbind calls (one dispatch each) rather
+    # than one indexing op per entry.
+    summary_rows = summaries.reshape(K, 9).unbind(0)
+    count_rows, class_rows, miou_rows = counts.unbind(0), per_class.unbind(0), miou.unbind(0)
+    out_tp, out_fp, out_fn, out_sums, out_confusion = (t.unbind(0) for t in copies)
     results = []
     for k in range(K):
-        result = {name: {"pq": summaries[k, i, 0], "sq": summaries[k, i, 1], "rq": summaries[k, i, 2],
-                         "n": counts[k, i]}
+        s, n = summary_rows[k].unbind(0), count_rows[k].unbind(0)
+        result = {name: {"pq": s[3 * i], "sq": s[3 * i + 1], "rq": s[3 * i + 2], "n": n[i]}
                   for i, name in enumerate(("All", "Things", "Stuff"))}
-        result.update(per_class={name: per_class[k, i] for i, name in enumerate(("pq", "sq", "rq", "iou"))},
-                      miou=miou[k], tp=out_tp[k], fp=out_fp[k], fn=out_fn[k], iou_sum=out_sums[k],
+        result.update(per_class=dict(zip(("pq", "sq", "rq", "iou"), class_rows[k].unbind(0))),
+                      miou=miou_rows[k], tp=out_tp[k], fp=out_fp[k], fn=out_fn[k], iou_sum=out_sums[k],
                       confusion=out_confusion[k])
         results.append(result)
     return results
