@@ -148,14 +148,17 @@ def segmentation_prefixes(prediction: InstanceBatch, target, num_classes: int, *
     itp, ifp, ifn, precision, recall = out["itp"], out["ifp"], out["ifn"], out["precision"], out["recall"]
     threshold_tensor = out["thresholds"]
     snapshots = finalize(tp, fp, fn, sums, confusion, things) if rows else []
+    # Per-checkpoint views of the instance arrays, one unbind each.
+    itp_rows, ifp_rows, ifn_rows, precision_rows, recall_rows = (
+        (t.unbind(0) if rows else ()) for t in (itp, ifp, ifn, precision, recall))
     results, row = [], 0
     for image_points in points:
         image = []
         for k in image_points:
             result = snapshots[row]
             result["prefix"] = k
-            result["instance"] = dict(iou_thresholds=threshold_tensor, tp=itp[row], fp=ifp[row], fn=ifn[row],
-                                      precision=precision[row], recall=recall[row])
+            result["instance"] = dict(iou_thresholds=threshold_tensor, tp=itp_rows[row], fp=ifp_rows[row],
+                                      fn=ifn_rows[row], precision=precision_rows[row], recall=recall_rows[row])
             image.append(result)
             row += 1
         results.append(image)
