@@ -187,3 +187,28 @@ def test_multi_image_every_prefix(pe):
             isthing=[True, True, True, False, False], checkpoints=points))
         for a, e in zip(actual, expected):
             compare_snapshots(a, e)
+
+
+def test_scoring_state_beyond_lds():
+    """200 target segments x 160 masks: the G x (N + 1) canvas table no
+    longer fits LDS, so scoring keeps its state in global memory (the
+    unstaged path, matching inside the mask loop) -- same results."""
+    import workloads
+    import panoptic_evaluator as pe
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    gen = torch.Generator().manual_seed(21)
+    pair = workloads.panoptic_batch(21, 1, 64, 64, 200, 5, 3)
+    masks = [workloads.ellipse_masks(gen, 160, 64, 64, min_axis=2.0, max_axis=12.0)]
+    classes = [torch.randint(0, 5, (160,), generator=gen).int()]
+    isthing = [True, True, True, False, False]
+    points = list(range(0, 161, 8))
+    expected = opre.prefixes([m.numpy() for m in masks], [c.numpy() for c in classes], pair.gt.numpy(),
+                             pair.gc.numpy(), pair.crowd.numpy(), 5, isthing=isthing, checkpoints=points)
+    actual = to_numpy(pe.segmentation_prefixes(
+        pe.InstanceBatch([m.cuda() for m in masks], [c.cuda() for c in classes]),
+        pe.PanopticBatch(pair.gt.cuda(), pair.gc.cuda(), pair.crowd.cuda()), 5, isthing=isthing,
+        checkpoints=points))
+    assert pair.gc.shape[1] * 161 * 4 > 120 * 1024  # beyond the LDS stage
+    for a, e in zip(actual, expected):
+        compare_snapshots(a, e)

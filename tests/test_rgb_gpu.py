@@ -119,3 +119,22 @@ def test_rgb_panoptic_quality_and_unknown_ids(pe):
     ev = pe.PanopticEvaluator(len(CATS), validate=True)
     with pytest.raises(ValueError, match="unknown segment ID"):
         ev.update(prediction, unknown)
+
+
+def test_rgb_lookup_with_global_pair_bins(pe):
+    """~300 segments per side: the G x P pair table exceeds the LDS bins, so the
+    histogram counts into global memory while the ID hashes stay in LDS."""
+    gt, gi = id_batch(7, 2, 96, 96, 300)
+    pred, _ = id_batch(8, 2, 96, 96, 320)
+    pred[:, :48] = gt[:, :48]
+    rng = np.random.default_rng(9)
+    pi = [[dict(e, iscrowd=0) for e in segments_of(p, rng, g)] for p, g in zip(pred, gi)]
+    G, P = 1 + max(map(len, gi)), 1 + max(map(len, pi))
+    assert G * P * 2 > 128 * 1024  # 16-bit bins beyond the LDS budget
+    expected = expected_state(gt, pred, gi, pi)
+    target = pe.PanopticBatch.from_coco(rgb_tensor(gt), gi, CATS, compact=False)
+    prediction = pe.PanopticBatch.from_coco(rgb_tensor(pred), pi, CATS, compact=False)
+    for validate in (False, True):
+        ev = pe.PanopticEvaluator(len(CATS), isthing=[True, True, False, False], validate=validate)
+        ev.update(prediction, target)
+        compare_panoptic(to_numpy(ev.compute()), expected)
