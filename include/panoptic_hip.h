@@ -298,7 +298,7 @@ int64_t pe_prefix_workspace_bytes(int64_t B, int64_t G, int64_t C, int64_t T, in
  * precision/recall (T x C f64) of greedy instance matching.  error bits: 1
  * target slot outside 0..G-1, 2 bad target classes / slot 0, 4 active target
  * metadata and pixels disagree, 8 generated class outside 0..C-1, 16 empty
- * mask.  Replaces ``prefix`` (bindings.cpp:24-26; prefix_cuda,
+ * mask, 32 checkpoints not increasing or beyond N_b.  Replaces ``prefix`` (bindings.cpp:24-26; prefix_cuda,
  * prefix_kernel.cu:142-199; CPU segmentation_prefixes, prefix.py:11-122). */
 int pe_prefix(int64_t B, const int32_t* target, const int64_t* pix_off, int64_t HW, int64_t G, int64_t C,
               const uint8_t* const* masks, const int64_t* n_off, int64_t sum_n, int64_t sum_tri, int64_t max_n,

@@ -13,11 +13,17 @@
 //    pixel is a transfer (mask i, GT slot g, previous top p), counted in
 //    raw[i][g] (intersection of the original mask) and in the triangular
 //    per-step table dec[i][g][p] (p <= i).  GT areas come from the same pass.
-//  * scoring: one workgroup per image replays the steps: column i+1 of the
-//    G x (N+1) pair table becomes raw[i], dec[i] is subtracted, the greedy
-//    instance match of mask i runs with one lane per IoU threshold, and at
-//    each checkpoint the full PQ/confusion of the visible canvas is computed
-//    (iou_sum folded in slot order, as the reference's scatter_add_).
+//  * scoring: the G x (N+1) pair table of the visible canvas at step k is
+//    a sum: column j >= 1 is raw[j-1] minus the decrements of the steps
+//    j..k-1, column 0 the GT areas minus those of steps 0..k-1.  So the
+//    checkpoints of an image are split into slices, one workgroup each: it
+//    builds the table at its first checkpoint in one pass over the earlier
+//    steps' counts (LDS atomics, no per-step barrier), then moves from
+//    checkpoint to checkpoint the same way and computes the full
+//    PQ/confusion of each (iou_sum folded in slot order, as the reference's
+//    scatter_add_).  The greedy instance matching (one wave per IoU
+//    threshold) runs in workgroups of its own.  When the table does not fit
+//    in LDS, one workgroup per image replays every step in global scratch.
 #include <climits>
 
 #include "common.h"
@@ -64,6 +70,11 @@ struct PrefixArgs {
   i64* out_itp; i64* out_ifp; i64* out_ifn; double* precision; double* recall;
   int32_t* error;
   int aligned;
+  // Staged scoring: B x match_groups matching workgroups, then one workgroup
+  // per slice of at most slice_rows checkpoints (slices.pix_off = k_off).
+  int match_groups;
+  int slice_rows;
+  TileList slices;
 };
 
 __device__ __forceinline__ v4i load16(const uint8_t* row, i64 k, i64 HW, bool aligned) {
@@ -490,18 +501,18 @@ __device__ __forceinline__ void prefix_snapshot(const PrefixArgs& a, int k, i64 
 // Independent-mask greedy matching of every mask in generation order
 // (prefix.py semantics: same-class non-crowd GT not yet taken, IoU >=
 // threshold, the highest IoU and then the lowest slot wins; a crowd match
-// only ignores), with wave w owning thresholds w, w + waves, ...: the lanes
-// cover the GT slots, a wave argmax picks the match and lane 0 counts it.
-// The matching never reads the canvas, so it runs apart from the canvas
-// loop, without block barriers, and writes its checkpoint rows itself.
+// only ignores), with wave w owning thresholds t_first + w, + t_step, ...:
+// the lanes cover the GT slots, a wave argmax picks the match and lane 0
+// counts it.  The matching never reads the canvas, so it runs apart from
+// it, without block barriers, and writes its checkpoint rows itself.
 // `used` entries are only ever touched by the lane owning that slot, and the
 // counts only by lane 0, in LDS, so a wave sees its own writes in order.
 __device__ __forceinline__ void prefix_instance_waves(const PrefixArgs& a, const PrefixView& v, int N,
                                                       const int32_t* raw, const int32_t* cls, i64 kfirst,
-                                                      i64 klast) {
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, waves = blockDim.x / kWave;
+                                                      i64 klast, int t_first, int t_step) {
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   const int G = a.G, C = a.C;
-  for (int t = wave; t < a.T; t += waves) {
+  for (int t = t_first + wave; t < a.T; t += t_step) {
     const double thr = a.thresholds[t];
     i64 q = kfirst;
     if (q < klast && a.points[q] == 0) prefix_instance_row(a, v, q++, t, a.T, lane, kWave);
@@ -550,29 +561,24 @@ __device__ __forceinline__ void prefix_instance_waves(const PrefixArgs& a, const
   }
 }
 
-template <bool STAGED>
-__device__ __forceinline__ void prefix_score_image(const PrefixArgs& a, int b, const PrefixView& v) {
-  const int tid = threadIdx.x;
-  const int G = a.G, C = a.C, T = a.T;
-  const i64 n0 = a.n_off[b];
-  const int N = static_cast<int>(a.n_off[b + 1] - n0);
-  const int P1 = N + 1;
-  const int32_t* raw = a.raw + n0 * G;
-  const int32_t* dec = a.dec + dec_base(a, b);
-  const int32_t* cls = a.cls + n0;
-  const i64 kfirst = a.k_off[b], klast = a.k_off[b + 1];
-  if (STAGED) {
-    const int32_t* gc = a.gc + static_cast<i64>(b) * G;
-    const uint8_t* crowd = a.crowd + static_cast<i64>(b) * G;
-    const int32_t* ga = a.ga + static_cast<i64>(b) * G;
-    for (int g = tid; g < G; g += blockDim.x) {
-      v.gc[g] = gc[g];
-      v.crowd[g] = crowd[g];
-      v.ga[g] = ga[g];
-    }
-    __syncthreads();
+// GT metadata of image b into the LDS view.
+__device__ __forceinline__ void prefix_stage_targets(const PrefixArgs& a, int b, const PrefixView& v) {
+  const int G = a.G;
+  const int32_t* gc = a.gc + static_cast<i64>(b) * G;
+  const uint8_t* crowd = a.crowd + static_cast<i64>(b) * G;
+  const int32_t* ga = a.ga + static_cast<i64>(b) * G;
+  for (int g = threadIdx.x; g < G; g += blockDim.x) {
+    v.gc[g] = gc[g];
+    v.crowd[g] = crowd[g];
+    v.ga[g] = ga[g];
   }
-  // Validation of the target metadata and the generated classes.
+  __syncthreads();
+}
+
+// Validation of the target metadata and the generated classes (error bits).
+__device__ __forceinline__ void prefix_validate(const PrefixArgs& a, const PrefixView& v, int N, const int32_t* raw,
+                                                const int32_t* cls) {
+  const int G = a.G, C = a.C, tid = threadIdx.x;
   int bits = 0;
   for (int g = tid; g < G; g += blockDim.x) {
     const int c = v.gc[g];
@@ -586,10 +592,11 @@ __device__ __forceinline__ void prefix_score_image(const PrefixArgs& a, int b, c
     if (area == 0) bits |= 16;
   }
   if (bits) atomicOr(a.error, bits);
-  for (int e = tid; e < G * P1; e += blockDim.x) {
-    const int g = e / P1;
-    v.pairs[e] = (e - g * P1) == 0 ? v.ga[g] : 0;
-  }
+}
+
+// Matching state: nothing taken, no counts, positives per thing class.
+__device__ __forceinline__ void prefix_clear_matching(const PrefixArgs& a, const PrefixView& v) {
+  const int G = a.G, C = a.C, T = a.T, tid = threadIdx.x;
   for (int e = tid; e < T * G; e += blockDim.x) v.used[e] = 0;
   for (int e = tid; e < T * C; e += blockDim.x) {
     v.itp[e] = 0;
@@ -601,14 +608,139 @@ __device__ __forceinline__ void prefix_score_image(const PrefixArgs& a, int b, c
       for (int g = 0; g < G; ++g) n += v.gc[g] == c && !v.crowd[g];
     v.positives[c] = n;
   }
-  __syncthreads();
-  if (STAGED) {
-    prefix_instance_waves(a, v, N, raw, cls, kfirst, klast);
-    __syncthreads();
+}
+
+// Empty canvas: column 0 (not covered) holds the GT areas.
+__device__ __forceinline__ void prefix_clear_canvas(const PrefixView& v, int G, int P1) {
+  for (int e = threadIdx.x; e < G * P1; e += blockDim.x) {
+    const int g = e / P1;
+    v.pairs[e] = (e - g * P1) == 0 ? v.ga[g] : 0;
   }
+}
+
+// Step i whose triangle holds cell y (tri(i) <= y < tri(i + 1)).
+__device__ __forceinline__ int prefix_step_of(i64 y) {
+  int i = static_cast<int>((sqrt(8.0 * static_cast<double>(y) + 1.0) - 1.0) * 0.5);
+  while (tri(i + 1) <= y) ++i;
+  while (i > 0 && tri(i) > y) --i;
+  return i;
+}
+
+constexpr int kPrefixLoads = 8;  // loads in flight per thread while moving the canvas
+
+// Moves the LDS canvas from step `from` to step `to` (from <= to): the
+// columns of masks from..to-1 gain their raw counts and every earlier column
+// loses the decrements of steps from..to-1.  Each entry is a sum of these
+// terms, so all of them are added at once with LDS atomics.  The steps'
+// triangles are one contiguous range of dec, read as 16-byte vectors from the
+// 16-byte boundary at or below its start (the scratch section after the
+// triangles keeps the last vector inside the workspace).
+__device__ __forceinline__ void prefix_advance(const PrefixArgs& a, const PrefixView& v, const int32_t* raw,
+                                               const int32_t* dec, int from, int to, int P1) {
+  const int G = a.G, tid = threadIdx.x;
+  const i64 rlo = static_cast<i64>(from) * G, rcount = static_cast<i64>(to - from) * G;
+  for (i64 e0 = tid; e0 < rcount; e0 += kPrefixThreads * kPrefixLoads) {
+    int32_t r[kPrefixLoads];
+#pragma unroll
+    for (int u = 0; u < kPrefixLoads; ++u) {
+      const i64 e = e0 + u * kPrefixThreads;
+      r[u] = e < rcount ? raw[rlo + e] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPrefixLoads; ++u) {
+      if (!r[u]) continue;
+      const i64 x = rlo + e0 + u * kPrefixThreads;
+      const int i = static_cast<int>(x / G), g = static_cast<int>(x - static_cast<i64>(i) * G);
+      atomicAdd(v.pairs + g * P1 + i + 1, r[u]);
+    }
+  }
+  const i64 lo = G * tri(from), count = G * tri(to) - lo;
+  if (count <= 0) return;
+  const uintptr_t at = reinterpret_cast<uintptr_t>(dec + lo);
+  const v4i* first = reinterpret_cast<const v4i*>(at & ~static_cast<uintptr_t>(15));
+  const int lead = static_cast<int>((at & 15) / 4);
+  const i64 vectors = (lead + count + 3) / 4;
+  for (i64 v0 = tid; v0 < vectors; v0 += kPrefixThreads * kPrefixLoads) {
+    v4i d[kPrefixLoads];
+#pragma unroll
+    for (int u = 0; u < kPrefixLoads; ++u) {
+      const i64 vi = v0 + u * kPrefixThreads;
+      d[u] = vi < vectors ? __builtin_nontemporal_load(as_global(first + vi)) : v4i{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int u = 0; u < kPrefixLoads; ++u)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int n = d[u][c];
+        const i64 x = 4 * (v0 + u * kPrefixThreads) + c - lead;
+        if (!n || x < 0 || x >= count) continue;  // outside: counts of other steps
+        const i64 cell = lo + x;
+        const int i = prefix_step_of(cell / G);
+        const int r = static_cast<int>(cell - G * tri(i)), g = r / (i + 1);
+        atomicSub(v.pairs + g * P1 + (r - g * (i + 1)), n);
+      }
+  }
+}
+
+// Matching workgroup `group` of image b: thresholds group * waves + w.
+__device__ __forceinline__ void prefix_match_job(const PrefixArgs& a, int b, int group, const PrefixView& v) {
+  const i64 n0 = a.n_off[b];
+  const int N = static_cast<int>(a.n_off[b + 1] - n0);
+  const int32_t* raw = a.raw + n0 * a.G;
+  const int32_t* cls = a.cls + n0;
+  const int waves = kPrefixThreads / kWave;
+  prefix_stage_targets(a, b, v);
+  if (group == 0) prefix_validate(a, v, N, raw, cls);
+  prefix_clear_matching(a, v);
+  __syncthreads();
+  prefix_instance_waves(a, v, N, raw, cls, a.k_off[b], a.k_off[b + 1], group * waves, a.match_groups * waves);
+}
+
+// Canvas workgroup of image b for checkpoint rows q0..q1-1.
+__device__ __forceinline__ void prefix_canvas_job(const PrefixArgs& a, int b, i64 q0, i64 q1, const PrefixView& v) {
+  const i64 n0 = a.n_off[b];
+  const int P1 = static_cast<int>(a.n_off[b + 1] - n0) + 1;
+  const int32_t* raw = a.raw + n0 * a.G;
+  const int32_t* dec = a.dec + dec_base(a, b);
+  const int32_t* cls = a.cls + n0;
+  prefix_stage_targets(a, b, v);
+  prefix_clear_canvas(v, a.G, P1);
+  __syncthreads();
+  int at = 0;
+  for (i64 q = q0; q < q1; ++q) {
+    const int k = a.points[q];
+    if (k < at || k >= P1) {  // not increasing or beyond the masks: rows left unscored
+      if (threadIdx.x == 0) atomicOr(a.error, 32);
+      return;
+    }
+    prefix_advance(a, v, raw, dec, at, k, P1);
+    __syncthreads();
+    prefix_snapshot<false>(a, k, q, P1, v, cls);  // ends with a barrier
+    at = k;
+  }
+}
+
+// One workgroup per image with the scoring state in the global scratch:
+// column i + 1 becomes raw[i], dec[i] is subtracted, the greedy match of
+// mask i runs with one lane per IoU threshold, and every checkpoint is
+// scored on the way.
+__device__ __forceinline__ void prefix_score_image(const PrefixArgs& a, int b, const PrefixView& v) {
+  const int tid = threadIdx.x;
+  const int G = a.G, C = a.C, T = a.T;
+  const i64 n0 = a.n_off[b];
+  const int N = static_cast<int>(a.n_off[b + 1] - n0);
+  const int P1 = N + 1;
+  const int32_t* raw = a.raw + n0 * G;
+  const int32_t* dec = a.dec + dec_base(a, b);
+  const int32_t* cls = a.cls + n0;
+  const i64 kfirst = a.k_off[b], klast = a.k_off[b + 1];
+  prefix_validate(a, v, N, raw, cls);
+  prefix_clear_canvas(v, G, P1);
+  prefix_clear_matching(a, v);
+  __syncthreads();
   i64 q = kfirst;
   if (q < klast && a.points[q] == 0) {
-    prefix_snapshot<!STAGED>(a, 0, q, P1, v, cls);
+    prefix_snapshot<true>(a, 0, q, P1, v, cls);
     ++q;
   }
   for (int i = 0; i < N; ++i) {
@@ -636,7 +768,7 @@ __device__ __forceinline__ void prefix_score_image(const PrefixArgs& a, int b, c
     for (int g = tid; g < G; g += blockDim.x) v.pairs[g * P1 + i + 1] = raw[static_cast<i64>(i) * G + g];
     __syncthreads();
     const int c = cls[i];
-    if (!STAGED && c >= 0 && c < C && a.things[c]) {
+    if (c >= 0 && c < C && a.things[c]) {
       double area = 0.0;
       {
         i64 s = 0;
@@ -669,27 +801,43 @@ __device__ __forceinline__ void prefix_score_image(const PrefixArgs& a, int b, c
         }
       }
     }
-    if (!STAGED) __syncthreads();
+    __syncthreads();
     if (q < klast && a.points[q] == i + 1) {
-      prefix_snapshot<!STAGED>(a, i + 1, q, P1, v, cls);
+      prefix_snapshot<true>(a, i + 1, q, P1, v, cls);
       ++q;
     }
   }
 }
 
-// One workgroup per image.  STAGED keeps the whole scoring state in LDS
-// (sized for the largest image of the launch); otherwise it lives in the
-// global scratch.
+// STAGED: the scoring state of every workgroup lives in LDS (sized for the
+// largest image of the launch); matching workgroups come first, then the
+// canvas slices (surplus workgroups, whose count is only known on the
+// device, exit at once).  Otherwise one workgroup per image in the global
+// scratch.
 template <bool STAGED>
 __global__ __launch_bounds__(kPrefixThreads) void prefix_score(PrefixArgs a) {
   extern __shared__ uint4 lds4[];
-  const int b = blockIdx.x;
-  if (STAGED) {
-    const i64 n0 = a.n_off[b];
-    const int P1 = static_cast<int>(a.n_off[b + 1] - n0) + 1;
-    prefix_score_image<true>(a, b, lds_view(reinterpret_cast<char*>(lds4), a.G, P1, a.C, a.T));
+  int job = blockIdx.x;
+  if (!STAGED) {
+    prefix_score_image(a, job, global_view(a, job, a.n_off[job]));
+    return;
+  }
+  const int match_jobs = a.B * a.match_groups;
+  int b;
+  if (job < match_jobs) {
+    b = job / a.match_groups;
   } else {
-    prefix_score_image<false>(a, b, global_view(a, b, a.n_off[b]));
+    job -= match_jobs;
+    if (job >= a.slices.count(a.B)) return;
+    b = a.slices.image_of(job);
+  }
+  const int P1 = static_cast<int>(a.n_off[b + 1] - a.n_off[b]) + 1;
+  const PrefixView v = lds_view(reinterpret_cast<char*>(lds4), a.G, P1, a.C, a.T);
+  if (static_cast<int>(blockIdx.x) < match_jobs) {
+    prefix_match_job(a, b, job - b * a.match_groups, v);
+  } else {
+    const i64 q0 = a.k_off[b] + static_cast<i64>(job - a.slices.first_tile(b)) * a.slice_rows;
+    prefix_canvas_job(a, b, q0, min(q0 + a.slice_rows, a.k_off[b + 1]), v);
   }
 }
 
@@ -712,6 +860,7 @@ struct PrefixScratch {
   int32_t* lab;
   double* val;
   int* tile_list;     // packed batches: tile_off then tile_image
+  int* slice_list;    // staged scoring: slice_off then slice_image
   size_t zero_bytes;  // raw + dec + ga, contiguous at the start
   size_t used_bytes;
 };
@@ -720,7 +869,7 @@ PrefixScratch carve_prefix(void* base, i64 B, i64 G, i64 C, i64 T, i64 sum_n, i6
   Carve carve(base);
   PrefixScratch s{};
   s.raw = carve.take<int32_t>(sum_n * G);
-  s.dec = carve.take<int32_t>(sum_tri * G);
+  s.dec = carve.take<int32_t>(sum_tri * G);  // followed by ga: 16-byte reads past its end stay inside
   s.ga = carve.take<int32_t>(B * G);
   s.zero_bytes = carve.used;
   s.pairs = carve.take<int32_t>((sum_n + B) * G);
@@ -733,6 +882,8 @@ PrefixScratch carve_prefix(void* base, i64 B, i64 G, i64 C, i64 T, i64 sum_n, i6
   s.ifp = carve.take<i64>(B * T * C);
   s.positives = carve.take<i64>(B * C);
   s.tile_list = packed_pixels > 0 ? carve.take<int>(tile_list_ints(B, packed_pixels, kPrefixTile)) : nullptr;
+  // At most N_b + 1 checkpoints per image, slices of at least one row.
+  s.slice_list = carve.take<int>(tile_list_ints(B, sum_n + B, 1));
   s.used_bytes = carve.used;
   return s;
 }
@@ -755,7 +906,8 @@ extern "C" int pe_prefix(int64_t B, const int32_t* target, const int64_t* pix_of
                          int64_t workspace_bytes, int aligned, void* stream_ptr) {
   PE_REQUIRE(B >= 0 && HW > 0 && G > 0 && G < INT_MAX && C > 0 && C < INT_MAX && T > 0 && T * C < INT_MAX,
              "prefix dimensions outside supported range");
-  PE_REQUIRE(sum_n >= 0 && sum_tri >= 0 && total_rows >= 0 && max_n >= 0 && max_n <= sum_n,
+  PE_REQUIRE(sum_n >= 0 && sum_tri >= 0 && total_rows >= 0 && total_rows <= sum_n + B && max_n >= 0 &&
+                 max_n <= sum_n,
              "invalid prefix sizes");
   PE_REQUIRE(G * (max_n + 1) < INT_MAX && G * (max_n * (max_n + 1) / 2) < INT_MAX, "prefix tables too large");
   const i64 packed_pixels = pix_off ? HW : 0;
@@ -855,11 +1007,28 @@ extern "C" int pe_prefix(int64_t B, const int32_t* target, const int64_t* pix_of
                         hipFuncSetAttribute(reinterpret_cast<const void*>(prefix_score<true>),
                                             hipFuncAttributeMaxDynamicSharedMemorySize,
                                             static_cast<int>(kPrefixStageMax)) == hipSuccess;
-    ProfileScope timing("prefix_score", stream);
-    if (staged)
-      prefix_score<true><<<static_cast<unsigned>(B), kPrefixThreads, view, stream>>>(a);
-    else
+    if (staged) {
+      // Slices of about total_rows / (2 x CUs) checkpoints (two staged
+      // workgroups fit a CU), matching workgroups of one wave per threshold.
+      const i64 waves = kPrefixThreads / kWave;
+      const i64 target = 2 * static_cast<i64>(compute_units());
+      a.match_groups = static_cast<int>((T + waves - 1) / waves);
+      a.slice_rows = static_cast<int>(std::max<i64>(1, (total_rows + target - 1) / target));
+      TileList& sl = a.slices;
+      sl.pix_off = a.k_off;
+      sl.tile = a.slice_rows;
+      sl.tile_off = s.slice_list;
+      sl.tile_image = s.slice_list + B + 1;
+      if ((rc = launch_tile_list(a.k_off, a.B, a.slice_rows, s.slice_list, s.slice_list + B + 1, stream)))
+        return rc;
+      const i64 jobs = B * a.match_groups + tile_list_bound(B, total_rows, a.slice_rows);
+      PE_REQUIRE(jobs < INT_MAX, "too many workgroups");
+      ProfileScope timing("prefix_score", stream);
+      prefix_score<true><<<static_cast<unsigned>(jobs), kPrefixThreads, view, stream>>>(a);
+    } else {
+      ProfileScope timing("prefix_score", stream);
       prefix_score<false><<<static_cast<unsigned>(B), kPrefixThreads, 0, stream>>>(a);
+    }
     PE_LAUNCH_CHECK();
   }
   return kOk;

@@ -2,8 +2,10 @@
 
 Public surface of the reference's panoptic_evaluator/prefix.py
 (segmentation_prefixes :11-122).  All images of a call run in two kernel
-launches (one pixel pass over every mask, one scoring workgroup per image)
-plus the FP64 finaliser over every checkpoint row.
+launches (one pixel pass over every mask; one scoring launch whose
+workgroups each take a slice of an image's checkpoints or the instance
+matching of a few IoU thresholds) plus the FP64 finaliser over every
+checkpoint row.
 """
 
 from __future__ import annotations
@@ -46,6 +48,8 @@ def _raise_prefix_error(code):
         raise ValueError(MSG_ACTIVE)
     if code & 8:
         raise ValueError("classes must be valid category indices")
+    if code & 32:
+        raise ValueError("checkpoints must be strictly increasing nonnegative integers")
     raise ValueError("unknown target slot or empty prediction mask")
 
 

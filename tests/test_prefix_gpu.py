@@ -212,3 +212,47 @@ def test_scoring_state_beyond_lds():
     assert pair.gc.shape[1] * 161 * 4 > 120 * 1024  # beyond the LDS stage
     for a, e in zip(actual, expected):
         compare_snapshots(a, e)
+
+
+def test_checkpoint_slices_of_several_rows(pe):
+    """1031 checkpoint rows over 8 images (one without masks): more rows than
+    two per compute unit, so every scoring workgroup takes a slice of
+    several consecutive checkpoints and moves its canvas between them."""
+    import workloads
+    gen = torch.Generator().manual_seed(9)
+    ns = (150, 0, 140, 160, 130, 150, 145, 155)
+    pair = workloads.panoptic_batch(9, len(ns), 48, 64, 14, 5, 3)
+    masks = [workloads.ellipse_masks(gen, n, 48, 64, min_axis=2.0, max_axis=20.0) for n in ns]
+    classes = [torch.randint(0, 5, (len(m),), generator=gen).int() for m in masks]
+    isthing = [True, True, True, False, False]
+    rows = sum(n if n else 1 for n in ns)
+    assert rows > 2 * torch.cuda.get_device_properties(0).multi_processor_count
+    expected = opre.prefixes([m.numpy() for m in masks], [c.numpy() for c in classes], pair.gt.numpy(),
+                             pair.gc.numpy(), pair.crowd.numpy(), 5, isthing=isthing)
+    actual = to_numpy(pe.segmentation_prefixes(
+        pe.InstanceBatch([m.cuda() for m in masks], [c.cuda() for c in classes]),
+        pe.PanopticBatch(pair.gt.cuda(), pair.gc.cuda(), pair.crowd.cuda()), 5, isthing=isthing))
+    assert sum(len(a) for a in actual) == rows
+    for a, e in zip(actual, expected):
+        compare_snapshots(a, e)
+
+
+def test_native_checkpoint_beyond_masks_is_reported(pe):
+    """At the native boundary a checkpoint past the image's mask count is
+    caught by the scoring kernel's own range check (error bit 32); the
+    Python entry point rejects it before any launch."""
+    from panoptic_evaluator.evaluator import _prepare
+    from panoptic_evaluator.prefix import launch_prefix
+    tgt = target(pe).compact()
+    maps, gc, crowd = _prepare(tgt, 2, False)
+    masks = [torch.zeros((3,) + tuple(maps.shape[1:]), dtype=torch.bool, device="cuda")]
+    masks[0][:, 0, :2] = True
+    classes = [torch.zeros(3, dtype=torch.int32, device="cuda")]
+    things = torch.ones(2, dtype=torch.bool, device="cuda")
+    sizes = [tuple(maps.shape[1:])] * maps.shape[0]
+    out = launch_prefix(masks, classes, maps, gc, crowd, sizes, False, [[1, 5]], things, (0.5,), 2,
+                        torch.device("cuda"))
+    assert int(out["error"].item()) & 32
+    ok = launch_prefix(masks, classes, maps, gc, crowd, sizes, False, [[1, 3]], things, (0.5,), 2,
+                       torch.device("cuda"))
+    assert int(ok["error"].item()) == 0
