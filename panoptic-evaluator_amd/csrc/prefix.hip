@@ -8,7 +8,7 @@
 // Design (MI355X-first; replaces 2 launches per mask with 2 launches per
 // batch of images):
 //  * events: one pixel pass over every mask of every image.  Each lane owns
-//    32 pixels, keeps their GT slots and current top mask in registers and
+//    16 pixels, keeps their GT slots and current top mask in registers and
 //    streams the masks in generation order with 16-byte loads.  Every covered
 //    pixel is a transfer (mask i, GT slot g, previous top p), counted in
 //    raw[i][g] (intersection of the original mask) and in the triangular
@@ -34,7 +34,10 @@ namespace pe {
 typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr int kPrefixThreads = 256;
-constexpr int kPrefixTile = 16 * kPrefixThreads * 2;
+// One 16-pixel vector per lane keeps the per-pixel state small (about 60
+// VGPRs, 7 waves per SIMD): occupancy hides the mask loads' latency.
+constexpr int kPrefixVecs = 1;  // 16-pixel vectors per lane
+constexpr int kPrefixTile = 16 * kPrefixThreads * kPrefixVecs;
 static_assert(kPrefixTile < 65536, "a step's count per pixel tile must fit the 16-bit table entries");
 
 struct PrefixArgs {
@@ -99,21 +102,23 @@ __device__ __forceinline__ i64 dec_base(const PrefixArgs& a, int b) {
 
 // Masks whose loads are in flight together (processing stays in generation
 // order: only the loads run ahead).
-constexpr int kPrefixAhead = 4;
+constexpr int kPrefixAhead = 6;
 
 // Per-step event table in LDS (TABLE): the transitions of step i are counted
-// in a G x (N + 1) table, the keys touched first are listed, and after a
-// barrier the workgroup adds each touched entry once to the global raw/dec
-// tables.  Without it (large G x N) every pixel run goes to global atomics.
-constexpr int kPrefixList = 2048;                 // touched keys listed per step
+// in a G x (N + 1) table and marked in a bitmap of the same keys (no-return
+// LDS atomics only), and after a barrier the workgroup adds each marked
+// entry once to the global raw/dec tables.  Without it (large G x N) every
+// pixel run goes to global atomics.
 constexpr size_t kPrefixStageMax = 120 * 1024;    // LDS for the staged scoring state
 constexpr int kPrefixTableMax = 32768;            // G x (N + 1) 16-bit entries in LDS
 
 // A step adds at most one tile's pixels (kPrefixTile < 2^16) to an entry, so
-// the table packs two 16-bit counts per 32-bit word.
+// the table packs two 16-bit counts per 32-bit word; the bitmap word of key
+// k is k / 32, so both keys of a table word share one bitmap word.
 __host__ __device__ inline size_t prefix_lds_bytes(int G, int P1, bool table) {
+  const size_t keys = static_cast<size_t>(G) * P1;
   size_t bytes = 4 * static_cast<size_t>(G);
-  if (table) bytes += 4 * ((static_cast<size_t>(G) * P1 + 1) / 2 + kPrefixList + 2);
+  if (table) bytes += 4 * ((keys + 1) / 2 + (keys + 31) / 32);
   return (bytes + 15) & ~static_cast<size_t>(15);
 }
 
@@ -134,11 +139,15 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
   const i64 n0 = a.n_off[b];
   const int N = static_cast<int>(a.n_off[b + 1] - n0);
   const auto* target = as_global(a.target + a.tiles.begin(b));
-  int slots[2][16], top[2][16];
-  i64 base[2];
+  // Per pixel: its GT slot and the mask on top so far (0: none).  With the
+  // table G and P1 are at most 2^15, so both share one register (state =
+  // slot | top << 16, slot 0xFFFF for none); otherwise slots / top.
+  uint32_t state[kPrefixVecs][16];
+  int slots[kPrefixVecs][16], top[kPrefixVecs][16];
+  i64 base[kPrefixVecs];
   bool bad = false;
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < kPrefixVecs; ++h) {
     base[h] = static_cast<i64>(tile) * kPrefixTile + (static_cast<i64>(h) * kPrefixThreads + threadIdx.x) * 16;
     int run_g = -1, run = 0;
 #pragma unroll
@@ -151,8 +160,12 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
           s = -1;
         }
       }
-      slots[h][j] = s;
-      top[h][j] = 0;
+      if (TABLE) {
+        state[h][j] = s < 0 ? 0xFFFFu : static_cast<uint32_t>(s);
+      } else {
+        slots[h][j] = s;
+        top[h][j] = 0;
+      }
       if (s != run_g) {
         if (run_g >= 0) atomicAdd(ga + run_g, static_cast<uint32_t>(run));
         run_g = s;
@@ -168,28 +181,22 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
   int32_t* dec = a.dec + dec_base(a, b);
   const int P1 = N + 1;
   const bool aligned = a.aligned != 0;
-  // LDS step table (TABLE): G x P1 counts, the touched-key list and two
-  // alternating list lengths.  Only steps whose mask covers some pixel of the
-  // tile are processed; processed step number ps uses counter ps & 1 and
-  // clears the other one for the next processed step.
+  // LDS step table (TABLE): G x P1 counts and the bitmap of the keys counted
+  // in the current step (both all zero between steps).  Only steps whose
+  // mask covers some pixel of the tile are processed.
+  const int table_words = (G * P1 + 1) / 2, mark_words = (G * P1 + 31) / 32;
   uint32_t* table = ga + G;
-  uint32_t* list = table + (static_cast<size_t>(G) * P1 + 1) / 2;
-  uint32_t* listed = list + kPrefixList;
+  uint32_t* marks = table + table_words;
   __shared__ uint32_t wave_touch[2][kPrefixThreads / kWave];
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  int ps = 0;
   if (TABLE) {
-    for (int e = threadIdx.x; e < (G * P1 + 1) / 2; e += kPrefixThreads) table[e] = 0;
-    if (threadIdx.x < 2) listed[threadIdx.x] = 0;
+    for (int e = threadIdx.x; e < table_words + mark_words; e += kPrefixThreads) table[e] = 0;
     __syncthreads();
   }
   auto add_run = [&](int i, int32_t* dec_i, int key, int run) {
     if (TABLE) {
-      const uint32_t shift = (key & 1) * 16;
-      if (((atomicAdd(table + (key >> 1), static_cast<uint32_t>(run) << shift) >> shift) & 0xFFFFu) == 0) {
-        const uint32_t slot = atomicAdd(listed + (ps & 1), 1u);
-        if (slot < kPrefixList) list[slot] = static_cast<uint32_t>(key);
-      }
+      atomicAdd(table + (key >> 1), static_cast<uint32_t>(run) << ((key & 1) * 16));
+      atomicOr(marks + (key >> 5), 1u << (key & 31));
     } else {
       const int rg = key / P1, prev = key - rg * P1;
       atomicAdd(raw + static_cast<i64>(i) * G + rg, run);
@@ -197,16 +204,16 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
     }
   };
   for (int i0 = 0; i0 < N; i0 += kPrefixAhead) {
-    v4i ahead[kPrefixAhead][2];
+    v4i ahead[kPrefixAhead][kPrefixVecs];
 #pragma unroll
     for (int u = 0; u < kPrefixAhead; ++u) {
       if (i0 + u < N) {
         const uint8_t* mask = masks + static_cast<i64>(i0 + u) * HW;
 #pragma unroll
-        for (int h = 0; h < 2; ++h) ahead[u][h] = load16(mask, base[h], HW, aligned);
+        for (int h = 0; h < kPrefixVecs; ++h) ahead[u][h] = load16(mask, base[h], HW, aligned);
       } else {
 #pragma unroll
-        for (int h = 0; h < 2; ++h) ahead[u][h] = v4i{0, 0, 0, 0};
+        for (int h = 0; h < kPrefixVecs; ++h) ahead[u][h] = v4i{0, 0, 0, 0};
       }
     }
     // Which of these steps touch the tile at all (one barrier per group; the
@@ -217,7 +224,7 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
 #pragma unroll
       for (int u = 0; u < kPrefixAhead; ++u)
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
+        for (int h = 0; h < kPrefixVecs; ++h)
           if ((ahead[u][h].x | ahead[u][h].y | ahead[u][h].z | ahead[u][h].w) != 0) mine |= 1u << u;
       uint32_t bits = 0;
 #pragma unroll
@@ -237,12 +244,8 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
     if (!((touched >> u) & 1u)) continue;  // uniform: no pixel of mask i in this tile
     const v4i* m = ahead[u];
     int32_t* dec_i = dec + static_cast<i64>(G) * tri(i);
-    // Counter (ps + 1) & 1 was last read in processed step ps - 1 before a
-    // barrier (or holds 0 already): clear it for the next processed step,
-    // whose users start after the next barrier.
-    if (TABLE && threadIdx.x == 0) listed[(ps + 1) & 1] = 0;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < kPrefixVecs; ++h) {
       if ((m[h].x | m[h].y | m[h].z | m[h].w) == 0) continue;
       const uint32_t words[4] = {static_cast<uint32_t>(m[h].x), static_cast<uint32_t>(m[h].y),
                                  static_cast<uint32_t>(m[h].z), static_cast<uint32_t>(m[h].w)};
@@ -250,11 +253,19 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         const bool on = (words[j >> 2] >> ((j & 3) * 8)) & 0xFFu;
-        const int g = slots[h][j];
         int key = -1;
-        if (on && g >= 0) {
-          key = g * P1 + top[h][j];
-          top[h][j] = i + 1;
+        if (TABLE) {
+          const uint32_t st = state[h][j], g = st & 0xFFFFu;
+          if (on && g != 0xFFFFu) {
+            key = static_cast<int>(g * P1 + (st >> 16));
+            state[h][j] = g | (static_cast<uint32_t>(i + 1) << 16);
+          }
+        } else {
+          const int g = slots[h][j];
+          if (on && g >= 0) {
+            key = g * P1 + top[h][j];
+            top[h][j] = i + 1;
+          }
         }
         if (key != run_key) {
           if (run_key >= 0) add_run(i, dec_i, run_key, run);
@@ -266,28 +277,33 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
       if (run_key >= 0) add_run(i, dec_i, run_key, run);
     }
     if (TABLE) {
-      // Flush this step's touched entries once each (skipped when the tile
-      // saw no pixel of mask i: the decision is uniform after the barrier).
+      // Flush this step's marked entries once each: the owner of a bitmap
+      // word reads and clears it and the table words it covers, then adds
+      // their counts to the global tables.  Skipped entirely when the tile
+      // saw no pixel of mask i (the decision is uniform after the barrier).
       __syncthreads();
-      const uint32_t n = listed[ps & 1];
-      ++ps;
-      if (n) {
-        auto flush = [&](uint32_t key) {
-          const uint32_t shift = (key & 1) * 16;
-          const uint32_t v = (table[key >> 1] >> shift) & 0xFFFFu;
-          if (!v) return;
-          atomicAnd(table + (key >> 1), ~(0xFFFFu << shift));  // the other half may be flushed concurrently
-          const int rg = static_cast<int>(key) / P1, prev = static_cast<int>(key) - rg * P1;
-          atomicAdd(raw + static_cast<i64>(i) * G + rg, static_cast<int32_t>(v));
-          atomicAdd(dec_i + static_cast<i64>(rg) * (i + 1) + prev, static_cast<int32_t>(v));
-        };
-        if (n <= kPrefixList) {
-          for (uint32_t e = threadIdx.x; e < n; e += kPrefixThreads) flush(list[e]);
-        } else {
-          for (int e = threadIdx.x; e < G * P1; e += kPrefixThreads) flush(static_cast<uint32_t>(e));
+      for (int w = threadIdx.x; w < mark_words; w += kPrefixThreads) {
+        const uint32_t marked = marks[w];
+        if (!marked) continue;
+        marks[w] = 0;
+        uint32_t pairs = (marked | (marked >> 1)) & 0x55555555u;  // even bit p: table word 16 w + p / 2
+        while (pairs) {
+          const int p = __builtin_ctz(pairs);
+          pairs &= pairs - 1;
+          const int word = 16 * w + p / 2;
+          const uint32_t counts = table[word];
+          table[word] = 0;
+#pragma unroll
+          for (int half = 0; half < 2; ++half) {
+            const int32_t v = static_cast<int32_t>((counts >> (16 * half)) & 0xFFFFu);
+            if (!v) continue;
+            const int key = 2 * word + half, rg = key / P1, prev = key - rg * P1;
+            atomicAdd(raw + static_cast<i64>(i) * G + rg, v);
+            atomicAdd(dec_i + static_cast<i64>(rg) * (i + 1) + prev, v);
+          }
         }
-        __syncthreads();
       }
+      __syncthreads();
     }
     }
   }

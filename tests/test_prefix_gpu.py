@@ -256,3 +256,28 @@ def test_native_checkpoint_beyond_masks_is_reported(pe):
     ok = launch_prefix(masks, classes, maps, gc, crowd, sizes, False, [[1, 3]], things, (0.5,), 2,
                        torch.device("cuda"))
     assert int(ok["error"].item()) == 0
+
+
+def test_events_beyond_lds_table():
+    """260 target segments x 130 masks: G x (N + 1) exceeds the events
+    kernel's LDS step table, so every pixel run goes to global atomics (and
+    scoring keeps its state in global memory) -- same results."""
+    import workloads
+    import panoptic_evaluator as pe
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    gen = torch.Generator().manual_seed(23)
+    pair = workloads.panoptic_batch(23, 1, 64, 64, 260, 5, 3)
+    masks = [workloads.ellipse_masks(gen, 130, 64, 64, min_axis=2.0, max_axis=14.0)]
+    classes = [torch.randint(0, 5, (130,), generator=gen).int()]
+    isthing = [True, True, True, False, False]
+    points = [0, 1, 2, 3] + list(range(10, 131, 10))
+    assert pair.gc.shape[1] * 131 > 32768
+    expected = opre.prefixes([m.numpy() for m in masks], [c.numpy() for c in classes], pair.gt.numpy(),
+                             pair.gc.numpy(), pair.crowd.numpy(), 5, isthing=isthing, checkpoints=points)
+    actual = to_numpy(pe.segmentation_prefixes(
+        pe.InstanceBatch([m.cuda() for m in masks], [c.cuda() for c in classes]),
+        pe.PanopticBatch(pair.gt.cuda(), pair.gc.cuda(), pair.crowd.cuda()), 5, isthing=isthing,
+        checkpoints=points))
+    for a, e in zip(actual, expected):
+        compare_snapshots(a, e)
