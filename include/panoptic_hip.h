@@ -27,6 +27,17 @@ extern "C" {
 const char* pe_last_error(void);
 int pe_abi_version(void);
 
+/* Stream fork and join (no counterpart in the reference: its extension runs
+ * every kernel on the current stream).  pe_stream_wait records `event` on
+ * `signalling` and makes `waiting` wait for it; recorded inside HIP graph
+ * capture it forks a side stream into the capture or joins it back.  Events
+ * come from pe_event_create (timing disabled) and are reusable once the wait
+ * has been issued.  Used by the combined evaluator to overlap its panoptic
+ * and instance branches. */
+int pe_event_create(void** event);
+int pe_event_destroy(void* event);
+int pe_stream_wait(void* waiting, void* signalling, void* event);
+
 /* Benchmark timing hooks (no counterpart in the reference): while enabled,
  * HIP events are recorded on the launch stream around every launch of the
  * library's dominant kernels (by kernel name, e.g. "joint_histogram");

@@ -127,6 +127,28 @@ extern "C" const char* pe_last_error(void) { return pe::error_buffer(); }
 
 extern "C" int pe_abi_version(void) { return PE_ABI_VERSION; }
 
+// ---------------------------------------------------- stream fork and join
+
+extern "C" int pe_event_create(void** event) {
+  PE_REQUIRE(event, "event output pointer is null");
+  hipEvent_t e = nullptr;
+  PE_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  *event = e;
+  return pe::kOk;
+}
+
+extern "C" int pe_event_destroy(void* event) {
+  if (event) PE_HIP(hipEventDestroy(static_cast<hipEvent_t>(event)));
+  return pe::kOk;
+}
+
+extern "C" int pe_stream_wait(void* waiting, void* signalling, void* event) {
+  PE_REQUIRE(event, "event is null");
+  PE_HIP(hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(signalling)));
+  PE_HIP(hipStreamWaitEvent(static_cast<hipStream_t>(waiting), static_cast<hipEvent_t>(event), 0));
+  return pe::kOk;
+}
+
 // ------------------------------------------------------------ timing hooks
 
 #include <atomic>

@@ -11,6 +11,7 @@ import collections
 import ctypes
 import pathlib
 import threading
+import weakref
 
 import torch
 
@@ -38,6 +39,9 @@ _SIGNATURES = {
                           _ptr, _ptr, _ptr, _ptr, _ptr, _ptr]),
     "pe_image_pq": (_int, [_ptr, _ptr, _ptr, _ptr, _i64, _i64, _ptr, _ptr, _ptr]),
     "pe_pack": (_int, [_ptr, _int, _i64, _i64, _ptr, _ptr, _i64, _ptr, _ptr, _ptr]),
+    "pe_event_create": (_int, [ctypes.POINTER(_ptr)]),
+    "pe_event_destroy": (_int, [_ptr]),
+    "pe_stream_wait": (_int, [_ptr, _ptr, _ptr]),
 }
 
 _lock = threading.Lock()
@@ -74,7 +78,7 @@ def library():
     return _library
 
 
-_tables: "collections.OrderedDict[tuple, torch.Tensor]" = collections.OrderedDict()
+_tables: "collections.OrderedDict[tuple, tuple[torch.Tensor, set]]" = collections.OrderedDict()
 _TABLE_CACHE = 256
 
 
@@ -82,22 +86,63 @@ def device_table(values, device) -> torch.Tensor:
     """Read-only int64 device copy of a short host list (offsets, pointer
     tables), cached by content: repeated batch layouts upload nothing, so an
     update issued after a warm-up call performs no host-to-device copy (as HIP
-    graph capture requires).  Callers must never write to the result."""
+    graph capture requires).  Callers must never write to the result.
+
+    A table may be handed out on several streams; each entry remembers the
+    streams it was requested on, and an evicted entry is marked as used by
+    them, so the caching allocator reuses its memory only after their work."""
     device = torch.device(device)
     if device.index is None:
         device = torch.device(device.type, torch.cuda.current_device())
     key = (device.index, tuple(int(v) for v in values))
+    current = stream(device)
     with _lock:
-        table = _tables.get(key)
-        if table is not None:
+        entry = _tables.get(key)
+        if entry is not None:
             _tables.move_to_end(key)
-            return table
+            entry[1].add(current)
+            return entry[0]
     table = torch.tensor(key[1], dtype=torch.int64).to(device)
+    evicted = []
     with _lock:
-        _tables[key] = table
+        _tables[key] = (table, {current})
         while len(_tables) > _TABLE_CACHE:
-            _tables.popitem(last=False)
+            evicted.append(_tables.popitem(last=False)[1])
+    for old, streams in evicted:
+        for handle in streams:
+            old.record_stream(torch.cuda.ExternalStream(handle, device=old.device))
     return table
+
+
+class SideStream:
+    """A second stream of default priority on ``device`` with two reusable
+    events: ``fork(main)`` orders it after the work issued so far on the raw
+    stream handle ``main``; ``join(main)`` makes ``main`` wait for it.  Both
+    are plain event record/wait pairs, so they also fork and join a side
+    branch inside HIP graph capture."""
+
+    def __init__(self, device):
+        self.stream = torch.cuda.Stream(device)
+        self.handle = self.stream.cuda_stream
+        lib = library()
+        self._events = []
+        for _ in range(2):
+            event = _ptr()
+            check(lib.pe_event_create(ctypes.byref(event)))
+            self._events.append(event.value)
+        weakref.finalize(self, _destroy_events, list(self._events))
+
+    def fork(self, main: int):
+        call("pe_stream_wait", self.handle, main, self._events[0])
+
+    def join(self, main: int):
+        call("pe_stream_wait", main, self.handle, self._events[1])
+
+
+def _destroy_events(events):
+    if _library is not None:
+        for event in events:
+            _library.pe_event_destroy(event)
 
 
 def ptr(tensor):

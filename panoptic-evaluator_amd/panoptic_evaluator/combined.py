@@ -5,6 +5,15 @@ Public surface of the reference's panoptic_evaluator/combined.py
 the detection x target-slot histogram (one launch), the device-side input
 checks (one host read when validating), the panoptic update, and one COCO
 matching launch; AP ground truth is exactly the target's thing segments.
+
+The panoptic branch (joint histogram and segment scoring) and the instance
+branch (score order and detection histogram) share no data until the matcher.
+While an update is captured into a HIP graph (validate=False), the panoptic
+branch is launched on a second stream of default priority that joins the
+caller's stream before the matcher, so the graph holds the two branches side
+by side.  Eager updates keep one stream: a cross-stream event wait costs more
+on the host and device than the overlap saves at these sizes.  Every tensor is
+allocated on the caller's stream.
 """
 
 from __future__ import annotations
@@ -29,6 +38,12 @@ class SegmentationEvaluator:
         self.num_classes = num_classes
         self.validate = validate
         self._error = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self._side = None
+
+    def _side_stream(self):
+        if self._side is None:
+            self._side = _native.SideStream(self.device)
+        return self._side
 
     def reset(self):
         self.panoptic.reset()
@@ -56,6 +71,12 @@ class SegmentationEvaluator:
             raise ValueError("detections and target must have matching image counts and dimensions")
         self._validate_areas(instance_areas, classes)
         device, C = self.device, self.num_classes
+        fork = not self.validate and torch.cuda.is_current_stream_capturing()
+        if fork:
+            # The panoptic branch needs nothing the instance branch makes; its
+            # scratch is allocated once per shape and kept by the evaluator.
+            side = self._side_stream()
+            areas = self.panoptic._update(prediction, target, return_areas=True, side=side)
         stream = _native.stream(device)
         B, S = len(sizes), classes.shape[1]
         ranked = score_order(prepared, device)
@@ -96,7 +117,10 @@ class SegmentationEvaluator:
         # Panoptic totals change only after the instance inputs passed.  The
         # matcher reads the target slot areas straight from the panoptic
         # update's int64 area table and keeps only thing-class slots.
-        areas = self.panoptic._update(prediction, target, return_areas=True)
+        if fork:
+            side.join(stream)
+        else:
+            areas = self.panoptic._update(prediction, target, return_areas=True)
         status = torch.empty((N, 40), dtype=torch.int8, device=device)
         ranks = torch.empty(N, dtype=torch.int32, device=device)
         gt_off = _native.device_table(range(0, (B + 1) * S, S), device)

@@ -312,7 +312,10 @@ class PanopticEvaluator:
         return self._scratch
 
     @torch.no_grad()
-    def _update(self, prediction: PanopticBatch, target: PanopticBatch, *, return_areas=False):
+    def _update(self, prediction: PanopticBatch, target: PanopticBatch, *, return_areas=False, side=None):
+        """``side`` (a ``_native.SideStream``): fork it from the current stream
+        once every input and scratch buffer is ready and launch there; the
+        caller joins it back."""
         if target.crowd is None:
             if (self._default_crowd is None or self._default_crowd.shape != target.classes.shape
                     or self._default_crowd.device != target.classes.device):
@@ -327,6 +330,10 @@ class PanopticEvaluator:
         pairs, areas, matched = self._scratch_for(B, G, P, packed_pixels)
         gt_ids, gt_slots = _lookup(target)
         pred_ids, pred_slots = _lookup(prediction)
+        stream = _native.stream(self.device)
+        if side is not None:
+            side.fork(stream)
+            stream = side.handle
         code = _native.call(
             "pe_update", gt.data_ptr(), map_bytes(gt), pred.data_ptr(), map_bytes(pred), B, H, W,
             _native.ptr(pix_off), gc.data_ptr(), pc.data_ptr(), crowd.data_ptr(), G, P, self.num_classes,
@@ -334,7 +341,7 @@ class PanopticEvaluator:
             self.fp.data_ptr(), self.fn.data_ptr(), self.iou_sum.data_ptr(), self.confusion.data_ptr(),
             _native.ptr(gt_ids), _native.ptr(gt_slots), _native.ptr(pred_ids), _native.ptr(pred_slots),
             self._validation_error.data_ptr(), self._iou_exact.data_ptr(), self._workspace.data_ptr(),
-            self._workspace.numel(), int(self.validate), 0, 1, _native.stream(self.device))
+            self._workspace.numel(), int(self.validate), 0, 1, stream)
         if code:
             raise_validation(code)
         return areas if return_areas else self

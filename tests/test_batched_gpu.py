@@ -248,3 +248,47 @@ def test_packed_prefix_equals_per_image_oracle(pe):
                                      batch.gc[i][None].numpy(), batch.crowd[i][None].numpy(), 133,
                                      isthing=isthing, checkpoints=points)
             compare_snapshots(actual[i], expected[0])
+
+
+def test_device_tables_survive_eviction_across_streams(pe):
+    """Tables handed out on two streams keep their values while the cache
+    evicts and reallocates around them (evicted entries are marked as used by
+    every stream that requested them)."""
+    from panoptic_evaluator import _native
+    side = torch.cuda.Stream()
+    held = []
+    for i in range(3 * _native._TABLE_CACHE):
+        values = [i, i + 1, 7 * i]
+        if i % 2:
+            with torch.cuda.stream(side):
+                table = _native.device_table(values, "cuda")
+        else:
+            table = _native.device_table(values, "cuda")
+        if i % 97 == 0:
+            held.append((values, table))
+    torch.cuda.synchronize()
+    for values, table in held:
+        assert table.cpu().tolist() == values
+
+
+def test_combined_fork_equals_sequential_at_c1_shape(pe):
+    """A captured update holds the panoptic branch on a second stream; two
+    replays equal two updates of the validating (one-stream) evaluator."""
+    import workloads
+    from panoptic_evaluator.graphs import CapturedUpdate
+    pair, masks, classes, scores = workloads.config1("cuda", seed=3, B=2)
+    isthing = [c < 80 for c in range(133)]
+    args = (pe.PanopticBatch(pair.pred, pair.pc), pe.InstanceBatch(masks, classes, scores),
+            pe.PanopticBatch(pair.gt, pair.gc, pair.crowd))
+    sequential = pe.SegmentationEvaluator(133, isthing=isthing, validate=True)
+    sequential.update(*args)
+    sequential.update(*args)
+    forked = pe.SegmentationEvaluator(133, isthing=isthing, validate=False)
+    captured = CapturedUpdate(forked, *args)
+    captured.replay()
+    torch.cuda.synchronize()
+    captured.replay()
+    got, want = to_numpy(forked.compute()), to_numpy(sequential.compute())
+    compare_panoptic(got, want, exact_iou_sum=True)
+    for key in ("precision", "recall"):
+        assert_exact(got["instance"][key], want["instance"][key], key)
