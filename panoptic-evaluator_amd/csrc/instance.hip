@@ -1041,12 +1041,24 @@ struct ApArgs {
 constexpr int kApChunk = 1024;
 constexpr int kApThreads = 256;
 
+// LDS precision table of one class (TABLE form): for each recall threshold r
+// and job j, the running counts (tp << 16 | fp) of the record whose precision
+// is the envelope there, and per job the last threshold the scan reached
+// (-1: every threshold is 0; -2: no positives, every threshold is -1).
+constexpr int kApJobs = 120;
+constexpr size_t kApTableBytes = sizeof(uint32_t) * 101 * kApJobs + sizeof(int) * kApJobs;
+
 // The accumulation of class c over its records order[s0 .. s1) (global score
 // order, ties by record index).  ORDER is the address space of ``order``
 // (a global array or an LDS list), kept apart so accesses stay typed.
-template <typename OrderPtr>
+// TABLE: the scan writes its precision values as counts into an LDS table
+// (``codes``, ``last``) and the whole workgroup then writes the class's
+// 10 x 101 x 12 precision values, 12 contiguous doubles per (t, r); otherwise
+// each job writes its own 101 values as it goes (counts of at most 4096
+// records fit the table's 16-bit fields; the TABLE form is for small batches).
+template <bool TABLE, typename OrderPtr>
 __device__ __forceinline__ void ap_class(const ApArgs& a, int c, OrderPtr order, int s0, int s1, uint8_t* s_rank,
-                                         int8_t* s_flag) {
+                                         int8_t* s_flag, uint32_t* codes = nullptr, int* last = nullptr) {
   const int job = threadIdx.x;
   const bool active = job < 120;
   const int m = job % 3, ar = (job / 3) % 4, t = job / 12;
@@ -1059,24 +1071,22 @@ __device__ __forceinline__ void ap_class(const ApArgs& a, int c, OrderPtr order,
   const int lane_flag = ar * 10 + t;
   // Records of one chunk -> LDS: the rank and the 40 flags (five 8-byte words,
   // rows are 8-byte aligned) of each record, four gathers in flight per lane.
+  // Out-of-range lanes load the last word (indices are clamped rather than
+  // branched around, so every load of a step is issued before the first wait).
   auto stage = [&](int first, int count) {
     uint64_t* flag_words = reinterpret_cast<uint64_t*>(s_flag);
     const int words = count * 5;
     constexpr int U = 4;
     for (int e0 = threadIdx.x; e0 < words; e0 += kApThreads * U) {
       uint64_t v[U];
-      int rank[U];
+      int rank[U], idx[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) idx[u] = order[first + min(e0 + u * kApThreads, words - 1) / 5];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int e = e0 + u * kApThreads;
-        v[u] = 0;
-        rank[u] = 0;
-        if (e < words) {
-          const int i = e / 5, w = e - i * 5;
-          const int idx = order[first + i];
-          v[u] = reinterpret_cast<const uint64_t*>(a.status + static_cast<i64>(idx) * 40)[w];
-          if (w == 0) rank[u] = a.ranks[idx];
-        }
+        const int e = min(e0 + u * kApThreads, words - 1);
+        v[u] = reinterpret_cast<const uint64_t*>(a.status + static_cast<i64>(idx[u]) * 40)[e % 5];
+        rank[u] = a.ranks[idx[u]];
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -1088,7 +1098,9 @@ __device__ __forceinline__ void ap_class(const ApArgs& a, int c, OrderPtr order,
       }
     }
   };
-  // forward: totals
+  // forward: totals.  Records are read V at a time with every LDS load in
+  // flight before any is used; the counts take no branches.
+  constexpr int V = 4;
   i64 tp = 0, fp = 0, n = 0;
   for (int first = s0; first < s1; first += kApChunk) {
     const int count = min(kApChunk, s1 - first);
@@ -1096,26 +1108,56 @@ __device__ __forceinline__ void ap_class(const ApArgs& a, int c, OrderPtr order,
     stage(first, count);
     __syncthreads();
     if (active && positives)
-      for (int i = 0; i < count; ++i) {
-        if (s_rank[i] > limit) continue;
-        const int f = s_flag[i * 40 + lane_flag];
-        tp += f == 1;
-        fp += f == 0;
-        ++n;
+      for (int i0 = 0; i0 < count; i0 += 2 * V) {
+        int rank[2 * V], flag[2 * V];
+#pragma unroll
+        for (int v = 0; v < 2 * V; ++v) {
+          const int iv = min(i0 + v, count - 1);
+          rank[v] = s_rank[iv];
+          flag[v] = s_flag[iv * 40 + lane_flag];
+        }
+#pragma unroll
+        for (int v = 0; v < 2 * V; ++v)
+          if (i0 + v >= count) rank[v] = 255;
+#pragma unroll
+        for (int v = 0; v < 2 * V; ++v) {
+          const bool counted = rank[v] <= limit;
+          tp += counted && flag[v] == 1;
+          fp += counted && flag[v] == 0;
+          n += counted;
+        }
       }
   }
   const bool scan = active && positives && n > 0;
   const double npos = static_cast<double>(positives);
   double env = 0.0, sum = 0.0;
+  uint32_t env_code = 0;  // counts behind env (0/(0+eps) = 0.0)
   int r = 100;
   i64 ctp = tp, cfp = fp, remaining = n;
-  if (scan)
-    while (r >= 0 && static_cast<double>(r) * 0.01 > static_cast<double>(ctp) / npos) {
-      prec[r * pstride] = 0.0;
-      --r;
+  if (scan) {
+    // Thresholds above the final recall get precision 0.
+    const double final_recall = static_cast<double>(ctp) / npos;
+    if constexpr (TABLE) {
+      // The largest r with r * 0.01 <= final recall (the loop below, in O(1)).
+      r = min(max(static_cast<int>(final_recall * 100.0), 0), 100);
+      while (r < 100 && static_cast<double>(r + 1) * 0.01 <= final_recall) ++r;
+      while (r >= 0 && static_cast<double>(r) * 0.01 > final_recall) --r;
+    } else {
+      while (r >= 0 && static_cast<double>(r) * 0.01 > final_recall) {
+        prec[r * pstride] = 0.0;
+        --r;
+      }
     }
+  }
+  if constexpr (TABLE) {
+    if (active) last[job] = !positives ? -2 : scan ? r : -1;
+  }
   // backward: precision[r] = max(pr[pos_r:]), pos_r = first record with
   // recall >= r / 100 (reusing the staged chunk when the class fits in one).
+  // V records per step: their LDS loads and the 2V FP64 divisions (which
+  // depend only on the integer running counts) are independent, so they
+  // overlap; the envelope and the recall-threshold writes then follow in
+  // record order, exactly as one record at a time.
   const int last_first = s1 > s0 ? s0 + ((s1 - s0 - 1) / kApChunk) * kApChunk : s0;
   for (int first = last_first; first >= s0 && s1 > s0; first -= kApChunk) {
     const int count = min(kApChunk, s1 - first);
@@ -1125,28 +1167,56 @@ __device__ __forceinline__ void ap_class(const ApArgs& a, int c, OrderPtr order,
       __syncthreads();
     }
     if (scan)
-      for (int i = count - 1; i >= 0 && r >= 0; --i) {
-        if (s_rank[i] > limit) continue;
-        const double pr = static_cast<double>(ctp) / (static_cast<double>(ctp + cfp) + DBL_EPSILON);
-        env = fmax(env, pr);
-        const int f = s_flag[i * 40 + lane_flag];
-        ctp -= f == 1;
-        cfp -= f == 0;
-        --remaining;
-        const double prev = remaining > 0 ? static_cast<double>(ctp) / npos : -1.0;
-        while (r >= 0 && static_cast<double>(r) * 0.01 > prev) {
-          prec[r * pstride] = env;
-          sum += env;
-          --r;
+      for (int i = count - 1; i >= 0 && r >= 0; i -= V) {
+        int rank[V], flag[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const int iv = max(i - v, 0);
+          rank[v] = s_rank[iv];
+          flag[v] = s_flag[iv * 40 + lane_flag];
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+          if (i - v < 0) rank[v] = 255;
+        double pr[V], prev[V];
+        bool counted[V];
+        uint32_t code[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          counted[v] = rank[v] <= limit;
+          code[v] = static_cast<uint32_t>(ctp << 16 | cfp);
+          pr[v] = static_cast<double>(ctp) / (static_cast<double>(ctp + cfp) + DBL_EPSILON);
+          ctp -= counted[v] && flag[v] == 1;
+          cfp -= counted[v] && flag[v] == 0;
+          remaining -= counted[v];
+          prev[v] = remaining > 0 ? static_cast<double>(ctp) / npos : -1.0;
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          if (!counted[v]) continue;
+          if (pr[v] > env) {
+            env = pr[v];
+            env_code = code[v];
+          }
+          while (r >= 0 && static_cast<double>(r) * 0.01 > prev[v]) {
+            if constexpr (TABLE)
+              codes[r * kApJobs + job] = env_code;
+            else
+              prec[r * pstride] = env;
+            sum += env;
+            --r;
+          }
         }
       }
   }
   double part_sum = 0.0, part_n = 0.0, rv = -1.0;
   if (active) {
     if (!positives) {
-      for (int q = 0; q <= 100; ++q) prec[q * pstride] = -1.0;
+      if constexpr (!TABLE)
+        for (int q = 0; q <= 100; ++q) prec[q * pstride] = -1.0;
     } else if (n == 0) {
-      for (int q = 0; q <= 100; ++q) prec[q * pstride] = 0.0;
+      if constexpr (!TABLE)
+        for (int q = 0; q <= 100; ++q) prec[q * pstride] = 0.0;
       rv = 0.0;
       part_n = 101.0;
     } else {
@@ -1159,6 +1229,35 @@ __device__ __forceinline__ void ap_class(const ApArgs& a, int c, OrderPtr order,
   // This class's share of the 12 summaries: 24 (sum, count) accumulators,
   // each folded over the 120 jobs in job order.
   __syncthreads();
+  if constexpr (TABLE) {
+    // (t, r, q) in memory order: q = ar * 3 + m is the job within t.  Eight
+    // values per thread and step, every LDS read issued before the divisions.
+    double* base = a.precision + c * 12;
+    constexpr int kValues = 10 * 101 * 12, W = 8;
+    for (int e0 = threadIdx.x; e0 < kValues; e0 += kApThreads * W) {
+      uint32_t counts[W];
+      int reached[W];
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const int e = min(e0 + w * kApThreads, kValues - 1);
+        const int q = e % 12, tr = e / 12, rr = tr % 101, tt = tr / 101;
+        const int j = tt * 12 + q;
+        reached[w] = last[j];
+        counts[w] = codes[rr * kApJobs + j];
+      }
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const int e = e0 + w * kApThreads;
+        if (e >= kValues) break;
+        const int q = e % 12, tr = e / 12, rr = tr % 101, tt = tr / 101;
+        const uint32_t code = rr <= reached[w] ? counts[w] : 0u;
+        const i64 etp = code >> 16, efp = code & 0xFFFFu;
+        const double value = reached[w] < -1 ? -1.0
+                                             : static_cast<double>(etp) / (static_cast<double>(etp + efp) + DBL_EPSILON);
+        base[(static_cast<i64>(tt) * 101 + rr) * pstride + q] = value;
+      }
+    }
+  }
   double* sh = reinterpret_cast<double*>(s_flag);
   if (active) {
     sh[3 * job] = part_sum;
@@ -1199,7 +1298,7 @@ __global__ __launch_bounds__(kApThreads) void ap_accumulate(ApArgs a) {
   __shared__ uint8_t s_rank[kApChunk];
   __shared__ __align__(16) int8_t s_flag[kApChunk * 40];
   const int c = blockIdx.x;
-  ap_class(a, c, as_global(a.order), a.start[c], a.end[c], s_rank, s_flag);
+  ap_class<false>(a, c, as_global(a.order), a.start[c], a.end[c], s_rank, s_flag);
 }
 
 // Batches of at most kApSmall records: each class workgroup gathers its own
@@ -1230,20 +1329,33 @@ __global__ __launch_bounds__(kApThreads) void ap_small(ApSmallArgs s) {
   // single barrier publishes the per-(round, wave) counts for the offsets.
   constexpr int kRounds = kApSmall / kApThreads;
   __shared__ int round_count[kRounds][kWaves];
+  // Scores are loaded with the ranks and classes (one round trip).
+  // Indices past N are clamped (not branched around) so that all 3 x kRounds
+  // loads are in flight together; their values are then discarded.
   uint32_t mine = 0;
   int rank[kRounds], klass[kRounds];
+  double score[kRounds];
 #pragma unroll
   for (int it = 0; it < kRounds; ++it) {
-    const int i = it * kApThreads + tid;
-    rank[it] = i < s.N ? a.ranks[i] : 0;
-    klass[it] = i < s.N ? s.classes[i] : -1;
+    rank[it] = 0;
+    klass[it] = -1;
+    score[it] = 0.0;
+  }
+  if (s.N > 0) {
+#pragma unroll
+    for (int it = 0; it < kRounds; ++it) {
+      const int i = min(it * kApThreads + tid, s.N - 1);
+      rank[it] = a.ranks[i];
+      klass[it] = s.classes[i];
+      score[it] = s.scores[i];
+    }
+#pragma unroll
+    for (int it = 0; it < kRounds; ++it)
+      if (it * kApThreads + tid >= s.N) klass[it] = -1;
   }
 #pragma unroll
   for (int it = 0; it < kRounds; ++it)
     mine |= static_cast<uint32_t>(klass[it] == c && rank[it] > 0 && rank[it] <= 100) << it;
-  double score[kRounds];
-#pragma unroll
-  for (int it = 0; it < kRounds; ++it) score[it] = (mine >> it) & 1u ? s.scores[it * kApThreads + tid] : 0.0;
   u64 ballot[kRounds];
 #pragma unroll
   for (int it = 0; it < kRounds; ++it) {
@@ -1291,10 +1403,12 @@ __global__ __launch_bounds__(kApThreads) void ap_small(ApSmallArgs s) {
       }
       __syncthreads();
     }
-  ap_class(a, c, static_cast<const int32_t*>(idx), 0, n, s_rank, s_flag);
+  uint32_t* codes = reinterpret_cast<uint32_t*>(s_flag + 40 * kApChunk);
+  ap_class<true>(a, c, static_cast<const int32_t*>(idx), 0, n, s_rank, s_flag, codes,
+                 reinterpret_cast<int*>(codes + 101 * kApJobs));
 }
 
-size_t ap_small_lds(int slots) { return 12 * static_cast<size_t>(slots) + kApChunk + 40 * kApChunk; }
+size_t ap_small_lds(int slots) { return 12 * static_cast<size_t>(slots) + kApChunk + 40 * kApChunk + kApTableBytes; }
 
 // instance.py:236-248: the 12 summaries as means over valid entries (>= 0),
 // from the per-class accumulators; classes are folded in a fixed order.

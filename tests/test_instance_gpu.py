@@ -366,20 +366,21 @@ def test_config1_parity_subset(pe):
     compare_instance(result["instance"], expected["instance"])
 
 
-@pytest.mark.parametrize("N", [3000, 9000])
-def test_ap_accumulation_paths(pe, N):
+@pytest.mark.parametrize("N,C", [(3000, 7), (4000, 2), (9000, 7)])
+def test_ap_accumulation_paths(pe, N, C):
     """Synthetic match records (tied scores, ranks 0..100, all flag values) on
-    both AP paths: one-launch per-class LDS sort (N <= 4096) and the two
-    global radix sorts (larger N)."""
+    both AP paths: one-launch per-class LDS sort (N <= 4096; with C=2 a class
+    holds more records than one LDS chunk) and the two global radix sorts
+    (larger N)."""
     from panoptic_evaluator.instance import accumulate
     rng = np.random.default_rng(N)
-    C = 7
     scores = rng.integers(0, 50, N).astype(np.float64) / 50.0   # many ties
     classes = rng.integers(0, C, N).astype(np.int32)
     ranks = rng.integers(0, 101, N).astype(np.int32)
     status = rng.integers(-1, 2, (N, 40)).astype(np.int8)
     counts = rng.integers(0, 40, (C, 4)).astype(np.int64)
-    counts[2] = 0                                                # a class without positives
+    if C > 2:
+        counts[2] = 0                                            # a class without positives
     half = N // 2                                                # two update records
     records = [(cuda(scores[:half]), cuda(classes[:half]), cuda(ranks[:half]), cuda(status[:half])),
                (cuda(scores[half:]), cuda(classes[half:]), cuda(ranks[half:]), cuda(status[half:]))]
