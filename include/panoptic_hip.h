@@ -27,6 +27,15 @@ extern "C" {
 const char* pe_last_error(void);
 int pe_abi_version(void);
 
+/* Streaming-read ceiling (no counterpart in the reference): reads `bytes`
+ * of device memory at `data` (16-byte aligned, a multiple of 16) with
+ * non-temporal 16-byte loads on `blocks` 256-thread workgroups and nothing
+ * else; `sink` is a 4-byte device buffer, written only if the folded data
+ * happens to equal a fixed sentinel (the write keeps the loads alive).
+ * Timed by the profiling hooks as "stream_read": bench.py reports it as the
+ * measured read bandwidth of the same byte count beside each HBM roofline. */
+int pe_stream_read(const void* data, int64_t bytes, int blocks, int* sink, void* stream);
+
 /* Stream fork and join (no counterpart in the reference: its extension runs
  * every kernel on the current stream).  pe_stream_wait records `event` on
  * `signalling` and makes `waiting` wait for it; recorded inside HIP graph

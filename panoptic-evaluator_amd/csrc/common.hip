@@ -127,6 +127,44 @@ extern "C" const char* pe_last_error(void) { return pe::error_buffer(); }
 
 extern "C" int pe_abi_version(void) { return PE_ABI_VERSION; }
 
+// ------------------------------------------------- streaming-read ceiling
+
+namespace pe {
+typedef int v4i_read __attribute__((ext_vector_type(4)));
+
+// Reads n16 16-byte vectors with four non-temporal loads in flight per lane
+// (grid-stride) and folds them into a value that is stored only if it equals
+// a sentinel the compiler cannot see (a kernel argument), so the loads stay.
+__global__ __launch_bounds__(256) void stream_read(const v4i_read* data, i64 n16, int sentinel, int* sink) {
+  constexpr int U = 4;
+  int acc = 0;
+  const i64 stride = static_cast<i64>(gridDim.x) * blockDim.x;
+  for (i64 i = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; i < n16; i += stride * U) {
+    v4i_read v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const i64 k = i + u * stride;
+      v[u] = k < n16 ? __builtin_nontemporal_load(data + k) : v4i_read{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc += v[u].x ^ v[u].y ^ v[u].z ^ v[u].w;
+  }
+  if (acc == sentinel) sink[0] = acc;
+}
+}  // namespace pe
+
+extern "C" int pe_stream_read(const void* data, int64_t bytes, int blocks, int* sink, void* stream) {
+  PE_REQUIRE(data && sink, "null buffer");
+  PE_REQUIRE(reinterpret_cast<uintptr_t>(data) % 16 == 0 && bytes > 0 && bytes % 16 == 0,
+             "data must be 16-byte aligned and a positive multiple of 16 bytes");
+  PE_REQUIRE(blocks > 0 && blocks <= 65536, "blocks must be in 1..65536");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  pe::ProfileScope timing("stream_read", s);
+  pe::stream_read<<<blocks, 256, 0, s>>>(static_cast<const pe::v4i_read*>(data), bytes / 16, 0x5eed1234, sink);
+  PE_LAUNCH_CHECK();
+  return pe::kOk;
+}
+
 // ---------------------------------------------------- stream fork and join
 
 extern "C" int pe_event_create(void** event) {

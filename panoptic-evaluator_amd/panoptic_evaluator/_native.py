@@ -39,6 +39,7 @@ _SIGNATURES = {
                           _ptr, _ptr, _ptr, _ptr, _ptr, _ptr]),
     "pe_image_pq": (_int, [_ptr, _ptr, _ptr, _ptr, _i64, _i64, _ptr, _ptr, _ptr]),
     "pe_pack": (_int, [_ptr, _int, _i64, _i64, _ptr, _ptr, _i64, _ptr, _ptr, _ptr]),
+    "pe_stream_read": (_int, [_ptr, _i64, _int, _ptr, _ptr]),
     "pe_event_create": (_int, [ctypes.POINTER(_ptr)]),
     "pe_event_destroy": (_int, [_ptr]),
     "pe_stream_wait": (_int, [_ptr, _ptr, _ptr]),
