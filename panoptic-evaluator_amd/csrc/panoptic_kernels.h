@@ -7,7 +7,7 @@
 namespace pe {
 
 constexpr int kHistThreads = 512;
-constexpr int kScoreThreads = 256;
+constexpr int kScoreThreads = 1024;
 constexpr int kMaxChunk = 65532;                 // packed 16-bit LDS counters stay exact
 constexpr size_t kMaxHistLds = 128 * 1024;       // beyond this, global atomics
 constexpr size_t kMaxStageLds = 120 * 1024;      // per-image scoring metadata in LDS

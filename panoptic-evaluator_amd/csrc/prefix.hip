@@ -149,12 +149,17 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_events(PrefixArgs a) {
 #pragma unroll
   for (int h = 0; h < kPrefixVecs; ++h) {
     base[h] = static_cast<i64>(tile) * kPrefixTile + (static_cast<i64>(h) * kPrefixThreads + threadIdx.x) * 16;
+    // All sixteen loads issue before the first is used (indices past the
+    // image are clamped to its last pixel and their values dropped).
+    int raw[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) raw[j] = target[min(base[h] + j, HW - 1)];
     int run_g = -1, run = 0;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       int s = -1;
       if (base[h] + j < HW) {
-        s = target[base[h] + j];
+        s = raw[j];
         if (s < 0 || s >= G) {
           bad = true;
           s = -1;
