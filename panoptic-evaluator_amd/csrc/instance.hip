@@ -902,17 +902,31 @@ __global__ __launch_bounds__(64) void coco_match(MatchArgs a) {
   const int D = static_cast<int>(a.det_off[b + 1] - d0);
   const int G = static_cast<int>(a.gt_off[b + 1] - g0);
   // Detections of class c in score order; the first 100 get ranks 1..100.
+  // Every other detection gets rank 0 and match flags -1 here as well (those
+  // of class c beyond the first 100 from this wave, those of no valid class
+  // from the class-0 wave); the top 100 get their flags from the chains below.
   int nd = 0;
+  auto unranked = [&](i64 d) {
+    a.ranks[d] = 0;
+    int8_t* row = a.status + d * 40;
+#pragma unroll
+    for (int k = 0; k < 40; ++k) row[k] = -1;
+  };
   for (int base = 0; base < D; base += kWave) {
     const int i = base + lane;
-    const bool sel = i < D && a.det_cls[d0 + i] == c;
+    const int cls = i < D ? a.det_cls[d0 + i] : -1;
+    const bool sel = i < D && cls == c;
     const u64 mask = __ballot(sel);
     if (sel) {
       const int rank = nd + __popcll(mask & ((1ULL << lane) - 1));
       if (rank < 100) {
         dets[rank] = i;
         a.ranks[d0 + i] = rank + 1;
+      } else {
+        unranked(d0 + i);
       }
+    } else if (i < D && c == 0 && (cls < 0 || cls >= a.C)) {
+      unranked(d0 + i);
     }
     nd += __popcll(mask);
   }
@@ -1632,12 +1646,8 @@ extern "C" int pe_coco_match(int64_t B, int64_t C, const int64_t* det_off, const
   PE_REQUIRE(inter_bytes == 4 || inter_bytes == 8, "intersections must be int32 or int64");
   PE_REQUIRE(max_gt == 0 || gt_area || gt_area_i64, "GT pixel areas are required");
   hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
-  int rc;
-  // status = -1, ranks = 0 for every detection (ranks > 100 stay 0 as on the CPU path)
-  if (total_dets) {
-    if ((rc = launch_zero(ranks, 4 * static_cast<size_t>(total_dets), stream))) return rc;
-    if ((rc = launch_fill_u8(status, 0xFF, static_cast<size_t>(total_dets) * 40, stream))) return rc;
-  }
+  // The kernel writes every detection's rank and flags (rank 0 and flags -1
+  // past the first 100 of a class, as on the CPU path).
   if (B == 0) return kOk;
   MatchArgs a{};
   a.B = static_cast<int>(B);

@@ -420,3 +420,22 @@ def test_instance_batch_of_uneven_images(pe):
     state.update([dict(masks=m, classes=c, scores=s) for m, c, s in zip(pm, pcl, sc)],
                  [dict(masks=m, classes=c, crowd=k) for m, c, k in zip(gm, gcl, cr)])
     compare_instance(to_numpy(ev.compute()), state.compute())
+
+
+def test_more_than_100_detections_of_one_class(pe):
+    """The matcher ranks only the first 100 detections of a class (in score
+    order); the rest get rank 0 and flags -1 from the same kernel."""
+    rng = np.random.default_rng(11)
+    h, w, d, g = 20, 24, 130, 9
+    pm = rng.random((d, h, w)) > 0.7
+    gm = rng.random((g, h, w)) > 0.6
+    pm[:g] = gm                                  # exact matches among the detections
+    pcl = np.where(np.arange(d) < 120, 1, 0)     # 120 detections of class 1, 10 of class 0
+    gcl = rng.integers(0, 2, g)
+    sc = rng.random(d)
+    ev = pe.InstanceEvaluator(2)
+    ev.update(pe.InstanceBatch([cuda(pm)], [cuda(pcl)], [cuda(sc)]),
+              pe.InstanceBatch([cuda(gm)], [cuda(gcl)]))
+    state = oinst.InstanceState(2)
+    state.update([dict(masks=pm, classes=pcl, scores=sc)], [dict(masks=gm, classes=gcl)])
+    compare_instance(to_numpy(ev.compute()), state.compute())
