@@ -79,7 +79,7 @@ def library():
     return _library
 
 
-_tables: "collections.OrderedDict[tuple, tuple[torch.Tensor, set]]" = collections.OrderedDict()
+_tables: "collections.OrderedDict[tuple, tuple[torch.Tensor, int, set]]" = collections.OrderedDict()
 _TABLE_CACHE = 256
 
 
@@ -90,8 +90,10 @@ def device_table(values, device) -> torch.Tensor:
     graph capture requires).  Callers must never write to the result.
 
     A table may be handed out on several streams; each entry remembers the
-    streams it was requested on, and an evicted entry is marked as used by
-    them, so the caching allocator reuses its memory only after their work."""
+    streams other than its allocation stream that requested it, and an
+    evicted entry is marked as used by them, so the caching allocator reuses
+    its memory only after their work (the allocation stream is ordered by the
+    allocator itself)."""
     device = torch.device(device)
     if device.index is None:
         device = torch.device(device.type, torch.cuda.current_device())
@@ -101,15 +103,16 @@ def device_table(values, device) -> torch.Tensor:
         entry = _tables.get(key)
         if entry is not None:
             _tables.move_to_end(key)
-            entry[1].add(current)
+            if current != entry[1]:
+                entry[2].add(current)
             return entry[0]
     table = torch.tensor(key[1], dtype=torch.int64).to(device)
     evicted = []
     with _lock:
-        _tables[key] = (table, {current})
+        _tables[key] = (table, current, set())
         while len(_tables) > _TABLE_CACHE:
             evicted.append(_tables.popitem(last=False)[1])
-    for old, streams in evicted:
+    for old, _, streams in evicted:
         for handle in streams:
             old.record_stream(torch.cuda.ExternalStream(handle, device=old.device))
     return table

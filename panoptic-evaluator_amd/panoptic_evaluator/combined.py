@@ -39,6 +39,13 @@ class SegmentationEvaluator:
         self.validate = validate
         self._error = torch.zeros(1, dtype=torch.int32, device=self.device)
         self._side = None
+        # The panoptic accumulators and the AP positive counts share one
+        # buffer, so reset() is a single fill.
+        words = PanopticEvaluator.state_words(num_classes)
+        shared = torch.zeros(words + 4 * num_classes, dtype=torch.int64, device=self.device)
+        self.panoptic._bind_state(shared[:words])
+        self.instance._counts = shared[words:].view(num_classes, 4)
+        self._shared = shared
 
     def _side_stream(self):
         if self._side is None:
@@ -46,8 +53,8 @@ class SegmentationEvaluator:
         return self._side
 
     def reset(self):
-        self.panoptic.reset()
-        self.instance.reset()
+        self._shared.zero_()
+        self.instance._records = []
 
     def _validate_areas(self, areas, classes):
         if areas is None:

@@ -281,10 +281,20 @@ class PanopticEvaluator:
         if hasattr(self, "tp"):
             self._state.zero_()  # one fill for all accumulators
             return
-        C = self.num_classes
         # One 8-byte-element buffer holds every accumulator, including the
         # exact 2^-53 fixed-point IoU sums behind iou_sum (2 words per class).
-        self._state = torch.zeros(6 * C + C * (C + 1), dtype=torch.int64, device=self.device)
+        self._bind_state(torch.zeros(self.state_words(self.num_classes), dtype=torch.int64, device=self.device))
+
+    @staticmethod
+    def state_words(num_classes: int) -> int:
+        """int64 words of the accumulator buffer for ``num_classes``."""
+        return 6 * num_classes + num_classes * (num_classes + 1)
+
+    def _bind_state(self, state: torch.Tensor):
+        """Use ``state`` (state_words int64 words, zero or a copy of the
+        current totals) as the accumulator buffer."""
+        C = self.num_classes
+        self._state = state
         self.tp = self._state[0:C]
         self.fp = self._state[C:2 * C]
         self.fn = self._state[2 * C:3 * C]

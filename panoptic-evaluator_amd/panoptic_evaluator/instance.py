@@ -168,8 +168,14 @@ def score_order(images, device) -> ScoreOrder:
     counts = [image.masks.shape[0] for image in images]
     B, N = len(images), sum(counts)
     det_off = offsets(counts, device)
-    scores = _cat([image.scores for image in images], torch.float64, device)
-    classes = _cat([image.classes for image in images], torch.int64, device)
+    if B == 1:
+        # One image: its own arrays (converted only if they are not float64 /
+        # int64 already), no concatenated copy.
+        scores = images[0].scores.reshape(-1).to(torch.float64).contiguous()
+        classes = images[0].classes.reshape(-1).to(torch.int64).contiguous()
+    else:
+        scores = _cat([image.scores for image in images], torch.float64, device)
+        classes = _cat([image.classes for image in images], torch.int64, device)
     order = torch.empty(N, dtype=torch.int64, device=device)
     sorted_scores = torch.empty(N, dtype=torch.float64, device=device)
     sorted_classes = torch.empty(N, dtype=torch.int64, device=device)

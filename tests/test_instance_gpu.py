@@ -439,3 +439,38 @@ def test_more_than_100_detections_of_one_class(pe):
     state = oinst.InstanceState(2)
     state.update([dict(masks=pm, classes=pcl, scores=sc)], [dict(masks=gm, classes=gcl)])
     compare_instance(to_numpy(ev.compute()), state.compute())
+
+
+def test_score_order_input_dtypes(pe):
+    """float64 or float32 scores and int32 or int64 classes (also mixed across
+    images, and a single image passed without a copy) give the oracle's
+    results."""
+    rng = np.random.default_rng(12)
+    sizes, dets, gts = [(16, 21), (16, 21), (16, 21)], [14, 0, 9], [5, 3, 6]
+    pm = [rng.random((d, 16, 21)) > 0.65 for d in dets]
+    gm = [rng.random((g, 16, 21)) > 0.6 for g in gts]
+    pcl = [rng.integers(0, 3, d) for d in dets]
+    gcl = [rng.integers(0, 3, g) for g in gts]
+    sc = [rng.integers(0, 8, d) / 8.0 for d in dets]  # ties; exact in float32
+    state = oinst.InstanceState(3)
+    state.update([dict(masks=m, classes=c, scores=s) for m, c, s in zip(pm, pcl, sc)],
+                 [dict(masks=m, classes=c) for m, c in zip(gm, gcl)])
+    expected = state.compute()
+    targets = pe.InstanceBatch([cuda(m) for m in gm], [cuda(c) for c in gcl])
+    forms = {
+        "int64 classes": ([cuda(c) for c in pcl], [cuda(s) for s in sc]),
+        "int32 classes": ([cuda(c).int() for c in pcl], [cuda(s) for s in sc]),
+        "float32 scores": ([cuda(c) for c in pcl], [cuda(s).float() for s in sc]),
+        "mixed class dtypes": ([cuda(c).int() if i == 0 else cuda(c) for i, c in enumerate(pcl)],
+                               [cuda(s) for s in sc]),
+    }
+    for name, (classes, scores) in forms.items():
+        ev = pe.InstanceEvaluator(3)
+        ev.update(pe.InstanceBatch([cuda(m) for m in pm], classes, scores), targets)
+        compare_instance(to_numpy(ev.compute()), expected)
+        # The same images one update at a time (single-image batches).
+        ev = pe.InstanceEvaluator(3)
+        for b in range(len(pm)):
+            ev.update(pe.InstanceBatch([cuda(pm[b])], [classes[b]], [scores[b]]),
+                      pe.InstanceBatch([targets.masks[b]], [targets.classes[b]]))
+        compare_instance(to_numpy(ev.compute()), expected)
