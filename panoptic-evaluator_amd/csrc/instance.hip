@@ -912,41 +912,69 @@ __global__ __launch_bounds__(64) void coco_match(MatchArgs a) {
 #pragma unroll
     for (int k = 0; k < 40; ++k) row[k] = -1;
   };
-  for (int base = 0; base < D; base += kWave) {
-    const int i = base + lane;
-    const int cls = i < D ? a.det_cls[d0 + i] : -1;
-    const bool sel = i < D && cls == c;
-    const u64 mask = __ballot(sel);
-    if (sel) {
-      const int rank = nd + __popcll(mask & ((1ULL << lane) - 1));
-      if (rank < 100) {
-        dets[rank] = i;
-        a.ranks[d0 + i] = rank + 1;
-      } else {
+  // Four wave-wide chunks per step, their class loads issued together
+  // (indices clamped into the image; lanes past D are then ignored).
+  constexpr int kScan = 4;
+  for (int base = 0; base < D; base += kWave * kScan) {
+    int cls[kScan];
+#pragma unroll
+    for (int k = 0; k < kScan; ++k) cls[k] = a.det_cls[d0 + min(base + k * kWave + lane, D - 1)];
+#pragma unroll
+    for (int k = 0; k < kScan; ++k) {
+      const int i = base + k * kWave + lane;
+      const bool sel = i < D && cls[k] == c;
+      const u64 mask = __ballot(sel);
+      if (sel) {
+        const int rank = nd + __popcll(mask & ((1ULL << lane) - 1));
+        if (rank < 100) {
+          dets[rank] = i;
+          a.ranks[d0 + i] = rank + 1;
+        } else {
+          unranked(d0 + i);
+        }
+      } else if (i < D && c == 0 && (cls[k] < 0 || cls[k] >= a.C)) {
         unranked(d0 + i);
       }
-    } else if (i < D && c == 0 && (cls < 0 || cls >= a.C)) {
-      unranked(d0 + i);
+      nd += __popcll(mask);
     }
-    nd += __popcll(mask);
   }
   nd = min(nd, 100);
   // Ground truth of class c (metadata order) and positives per area range.
+  // As above: class, crowd flag and annotation area of four chunks loaded
+  // together (then the thing flags of those classes), before any is used.
   int ng = 0;
   int positives[4] = {0, 0, 0, 0};
-  for (int base = 0; base < G; base += kWave) {
-    const int j = base + lane;
-    const bool sel = j < G && gt_class(a, g0 + j) == c;
-    const u64 mask = __ballot(sel);
-    if (sel) gts[ng + __popcll(mask & ((1ULL << lane) - 1))] = j;
-    ng += __popcll(mask);
-    const bool eligible = sel && !(a.gt_crowd && a.gt_crowd[g0 + j]);
-    const double area = eligible ? gt_ann_area(a, b, g0, j) : 0.0;
+  for (int base = 0; base < G; base += kWave * kScan) {
+    int cls[kScan];
+    bool crowded[kScan];
+    double ann[kScan];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      double lo, hi;
-      area_range(r, lo, hi);
-      positives[r] += __popcll(__ballot(eligible && area >= lo && area <= hi));
+    for (int k = 0; k < kScan; ++k) {
+      const int j = min(base + k * kWave + lane, G - 1);
+      cls[k] = a.gt_cls[g0 + j];
+      crowded[k] = a.gt_crowd && a.gt_crowd[g0 + j];
+      ann[k] = gt_ann_area(a, b, g0, j);
+    }
+    if (a.gt_things) {
+#pragma unroll
+      for (int k = 0; k < kScan; ++k)
+        if (cls[k] >= 0 && cls[k] < a.C && !a.gt_things[cls[k]]) cls[k] = -1;
+    }
+#pragma unroll
+    for (int k = 0; k < kScan; ++k) {
+      const int j = base + k * kWave + lane;
+      const bool sel = j < G && cls[k] >= 0 && cls[k] < a.C && cls[k] == c;
+      const u64 mask = __ballot(sel);
+      if (sel) gts[ng + __popcll(mask & ((1ULL << lane) - 1))] = j;
+      ng += __popcll(mask);
+      const bool eligible = sel && !crowded[k];
+      const double area = eligible ? ann[k] : 0.0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        double lo, hi;
+        area_range(r, lo, hi);
+        positives[r] += __popcll(__ballot(eligible && area >= lo && area <= hi));
+      }
     }
   }
   if (lane == 0 && ng)

@@ -1082,15 +1082,17 @@ __global__ __launch_bounds__(kFinalThreads) void finalize(FinalArgs a) {
   double* per_class = a.per_class + k * 4 * C;
   for (int c = tid; c < 2 * C; c += kFinalThreads) rowsum[c] = 0;
   __syncthreads();
-  // Eight loads in flight per lane: the sweep is latency-bound otherwise.
-  constexpr int U = 8;
+  // Twenty-four loads in flight per lane (one step covers C up to 156): the
+  // sweep is latency-bound otherwise.  Indices past the table are clamped
+  // and their values dropped, so no load waits behind a branch.
+  constexpr int U = 24;
   for (int i0 = tid; i0 < CC; i0 += kFinalThreads * U) {
     i64 v[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int i = i0 + u * kFinalThreads;
-      v[u] = i < CC ? conf[i] : 0;
-    }
+    for (int u = 0; u < U; ++u) v[u] = conf[min(i0 + u * kFinalThreads, CC - 1)];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i0 + u * kFinalThreads >= CC) v[u] = 0;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int i = i0 + u * kFinalThreads;
@@ -1107,8 +1109,11 @@ __global__ __launch_bounds__(kFinalThreads) void finalize(FinalArgs a) {
 #pragma unroll
   for (int j = 0; j < 14; ++j) part[j] = 0.0;
   for (int c = tid; c < C; c += kFinalThreads) {
+    // Every load of the class first (one round trip), then the ratios.
     const i64 t = tp[c], f = fp[c], n = fn[c];
     const double s = sums[c];
+    const i64 diag = conf[c * C1 + c];
+    const bool thing = a.things[c] != 0;
     if (a.out_tp) {
       a.out_tp[k * C + c] = t;
       a.out_fp[k * C + c] = f;
@@ -1123,7 +1128,6 @@ __global__ __launch_bounds__(kFinalThreads) void finalize(FinalArgs a) {
     per_class[c] = pq;
     per_class[C + c] = sq;
     per_class[2 * C + c] = rq;
-    const i64 diag = conf[c * C1 + c];
     const i64 uni = static_cast<i64>(rowsum[c]) + static_cast<i64>(colsum[c]) - diag;
     const double iou = uni > 0 ? static_cast<double>(diag) / static_cast<double>(uni) : __builtin_nan("");
     per_class[3 * C + c] = iou;
@@ -1132,20 +1136,25 @@ __global__ __launch_bounds__(kFinalThreads) void finalize(FinalArgs a) {
       part[13] += 1.0;
     }
     if (denom > 0) {
-      const int group = a.things[c] ? 1 : 2;
+      const int group = thing ? 1 : 2;
       part[0] += pq; part[1] += sq; part[2] += rq; part[3] += 1.0;
       part[group * 4] += pq; part[group * 4 + 1] += sq; part[group * 4 + 2] += rq; part[group * 4 + 3] += 1.0;
     }
   }
+  // Only waves that own classes reduce (the others' partials are +0.0, and
+  // adding +0.0 changes no sum).
+  const int active_waves = min(kWaves, (C + kWave - 1) / kWave);
+  if (wave < active_waves) {
 #pragma unroll
-  for (int j = 0; j < 14; ++j) part[j] = wave_sum(part[j]);
-  if (lane == 0)
+    for (int j = 0; j < 14; ++j) part[j] = wave_sum(part[j]);
+    if (lane == 0)
 #pragma unroll
-    for (int j = 0; j < 14; ++j) red[j][wave] = part[j];
+      for (int j = 0; j < 14; ++j) red[j][wave] = part[j];
+  }
   __syncthreads();
   if (tid < 14) {
     double s = 0.0;
-    for (int w = 0; w < kWaves; ++w) s += red[tid][w];
+    for (int w = 0; w < active_waves; ++w) s += red[tid][w];
     total[tid] = s;
   }
   __syncthreads();
