@@ -1081,7 +1081,8 @@ struct ApArgs {
 // totals; backward pass: precision envelope at the 101 recall thresholds
 // (instance.py:300-331).
 constexpr int kApChunk = 1024;
-constexpr int kApThreads = 256;
+constexpr int kApThreads = 256;       // ap_accumulate (large batches)
+constexpr int kApSmallThreads = 1024; // ap_small: more lanes for the gathers, the sort and the writer
 
 // LDS precision table of one class (TABLE form): for each recall threshold r
 // and job j, the running counts (tp << 16 | fp) of the record whose precision
@@ -1098,7 +1099,7 @@ constexpr size_t kApTableBytes = sizeof(uint32_t) * 101 * kApJobs + sizeof(int) 
 // 10 x 101 x 12 precision values, 12 contiguous doubles per (t, r); otherwise
 // each job writes its own 101 values as it goes (counts of at most 4096
 // records fit the table's 16-bit fields; the TABLE form is for small batches).
-template <bool TABLE, typename OrderPtr>
+template <bool TABLE, int THREADS, typename OrderPtr>
 __device__ __forceinline__ void ap_class(const ApArgs& a, int c, OrderPtr order, int s0, int s1, uint8_t* s_rank,
                                          int8_t* s_flag, uint32_t* codes = nullptr, int* last = nullptr) {
   const int job = threadIdx.x;
@@ -1119,20 +1120,20 @@ __device__ __forceinline__ void ap_class(const ApArgs& a, int c, OrderPtr order,
     uint64_t* flag_words = reinterpret_cast<uint64_t*>(s_flag);
     const int words = count * 5;
     constexpr int U = 4;
-    for (int e0 = threadIdx.x; e0 < words; e0 += kApThreads * U) {
+    for (int e0 = threadIdx.x; e0 < words; e0 += THREADS * U) {
       uint64_t v[U];
       int rank[U], idx[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) idx[u] = order[first + min(e0 + u * kApThreads, words - 1) / 5];
+      for (int u = 0; u < U; ++u) idx[u] = order[first + min(e0 + u * THREADS, words - 1) / 5];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int e = min(e0 + u * kApThreads, words - 1);
+        const int e = min(e0 + u * THREADS, words - 1);
         v[u] = reinterpret_cast<const uint64_t*>(a.status + static_cast<i64>(idx[u]) * 40)[e % 5];
         rank[u] = a.ranks[idx[u]];
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int e = e0 + u * kApThreads;
+        const int e = e0 + u * THREADS;
         if (e >= words) break;
         const int i = e / 5, w = e - i * 5;
         flag_words[e] = v[u];
@@ -1276,12 +1277,12 @@ __device__ __forceinline__ void ap_class(const ApArgs& a, int c, OrderPtr order,
     // values per thread and step, every LDS read issued before the divisions.
     double* base = a.precision + c * 12;
     constexpr int kValues = 10 * 101 * 12, W = 8;
-    for (int e0 = threadIdx.x; e0 < kValues; e0 += kApThreads * W) {
+    for (int e0 = threadIdx.x; e0 < kValues; e0 += THREADS * W) {
       uint32_t counts[W];
       int reached[W];
 #pragma unroll
       for (int w = 0; w < W; ++w) {
-        const int e = min(e0 + w * kApThreads, kValues - 1);
+        const int e = min(e0 + w * THREADS, kValues - 1);
         const int q = e % 12, tr = e / 12, rr = tr % 101, tt = tr / 101;
         const int j = tt * 12 + q;
         reached[w] = last[j];
@@ -1289,7 +1290,7 @@ __device__ __forceinline__ void ap_class(const ApArgs& a, int c, OrderPtr order,
       }
 #pragma unroll
       for (int w = 0; w < W; ++w) {
-        const int e = e0 + w * kApThreads;
+        const int e = e0 + w * THREADS;
         if (e >= kValues) break;
         const int q = e % 12, tr = e / 12, rr = tr % 101, tt = tr / 101;
         const uint32_t code = rr <= reached[w] ? counts[w] : 0u;
@@ -1340,7 +1341,7 @@ __global__ __launch_bounds__(kApThreads) void ap_accumulate(ApArgs a) {
   __shared__ uint8_t s_rank[kApChunk];
   __shared__ __align__(16) int8_t s_flag[kApChunk * 40];
   const int c = blockIdx.x;
-  ap_class<false>(a, c, as_global(a.order), a.start[c], a.end[c], s_rank, s_flag);
+  ap_class<false, kApThreads>(a, c, as_global(a.order), a.start[c], a.end[c], s_rank, s_flag);
 }
 
 // Batches of at most kApSmall records: each class workgroup gathers its own
@@ -1357,19 +1358,19 @@ struct ApSmallArgs {
   int sort_slots;        // power of two >= N
 };
 
-__global__ __launch_bounds__(kApThreads) void ap_small(ApSmallArgs s) {
+__global__ __launch_bounds__(kApSmallThreads) void ap_small(ApSmallArgs s) {
   extern __shared__ uint4 lds4[];
   const ApArgs& a = s.ap;
   const int c = blockIdx.x, tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  constexpr int kWaves = kApThreads / kWave;
+  constexpr int kWaves = kApSmallThreads / kWave;
   u64* keys = reinterpret_cast<u64*>(lds4);
   int32_t* idx = reinterpret_cast<int32_t*>(keys + s.sort_slots);
   uint8_t* s_rank = reinterpret_cast<uint8_t*>(idx + s.sort_slots);
   int8_t* s_flag = reinterpret_cast<int8_t*>(s_rank + kApChunk);
   // Gather this class's records in index order: every thread issues its
-  // loads for all of its records (i = it * kApThreads + tid) at once, then a
+  // loads for all of its records (i = it * kApSmallThreads + tid) at once, then a
   // single barrier publishes the per-(round, wave) counts for the offsets.
-  constexpr int kRounds = kApSmall / kApThreads;
+  constexpr int kRounds = kApSmall / kApSmallThreads;
   __shared__ int round_count[kRounds][kWaves];
   // Scores are loaded with the ranks and classes (one round trip).
   // Indices past N are clamped (not branched around) so that all 3 x kRounds
@@ -1386,14 +1387,14 @@ __global__ __launch_bounds__(kApThreads) void ap_small(ApSmallArgs s) {
   if (s.N > 0) {
 #pragma unroll
     for (int it = 0; it < kRounds; ++it) {
-      const int i = min(it * kApThreads + tid, s.N - 1);
+      const int i = min(it * kApSmallThreads + tid, s.N - 1);
       rank[it] = a.ranks[i];
       klass[it] = s.classes[i];
       score[it] = s.scores[i];
     }
 #pragma unroll
     for (int it = 0; it < kRounds; ++it)
-      if (it * kApThreads + tid >= s.N) klass[it] = -1;
+      if (it * kApSmallThreads + tid >= s.N) klass[it] = -1;
   }
 #pragma unroll
   for (int it = 0; it < kRounds; ++it)
@@ -1416,13 +1417,13 @@ __global__ __launch_bounds__(kApThreads) void ap_small(ApSmallArgs s) {
     }
     if ((mine >> it) & 1u) {
       const int slot = before + __popcll(ballot[it] & ((1ULL << lane) - 1));
-      idx[slot] = it * kApThreads + tid;
+      idx[slot] = it * kApSmallThreads + tid;
       keys[slot] = score_key_desc(score[it]);
     }
   }
   int m = 1;
   while (m < n) m <<= 1;
-  for (int e = n + tid; e < m; e += kApThreads) {
+  for (int e = n + tid; e < m; e += kApSmallThreads) {
     keys[e] = ~0ULL;
     idx[e] = INT_MAX;
   }
@@ -1430,7 +1431,7 @@ __global__ __launch_bounds__(kApThreads) void ap_small(ApSmallArgs s) {
   // Bitonic sort of (key, index) ascending: descending score, then index.
   for (int k = 2; k <= m; k <<= 1)
     for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int e = tid; e < m; e += kApThreads) {
+      for (int e = tid; e < m; e += kApSmallThreads) {
         const int f = e ^ j;
         if (f <= e) continue;
         const u64 ke = keys[e], kf = keys[f];
@@ -1446,7 +1447,7 @@ __global__ __launch_bounds__(kApThreads) void ap_small(ApSmallArgs s) {
       __syncthreads();
     }
   uint32_t* codes = reinterpret_cast<uint32_t*>(s_flag + 40 * kApChunk);
-  ap_class<true>(a, c, static_cast<const int32_t*>(idx), 0, n, s_rank, s_flag, codes,
+  ap_class<true, kApSmallThreads>(a, c, static_cast<const int32_t*>(idx), 0, n, s_rank, s_flag, codes,
                  reinterpret_cast<int*>(codes + 101 * kApJobs));
 }
 
@@ -1801,7 +1802,7 @@ extern "C" int pe_ap_compute(int64_t N, const double* scores, const int32_t* cla
                                static_cast<int>(lds)));
     {
       ProfileScope timing("ap_accumulate", stream);
-      ap_small<<<static_cast<unsigned>(C), kApThreads, lds, stream>>>(sa);
+      ap_small<<<static_cast<unsigned>(C), kApSmallThreads, lds, stream>>>(sa);
       PE_LAUNCH_CHECK();
     }
     ap_summaries<<<1, 256, 0, stream>>>(static_cast<int>(C), w.partial, summary);
