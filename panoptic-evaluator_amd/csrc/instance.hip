@@ -890,13 +890,27 @@ __device__ __forceinline__ double iou_threshold(int t) {
   return table[t];
 }
 
+// Matching state staged in LDS before the chains run (when it fits): the
+// intersections of every (ranked detection, candidate GT) pair and the GT
+// candidates' crowd flags and areas.  The chains are serial over detections;
+// staged, each step reads LDS instead of waiting on a global load.
+constexpr int kMatchStagePairs = 512;
+constexpr int kMatchStageGts = 128;
+constexpr size_t kMatchStageBytes = 8 * kMatchStagePairs + 17 * kMatchStageGts;
+
 __global__ __launch_bounds__(64) void coco_match(MatchArgs a) {
   extern __shared__ uint4 lds4[];
   if (a.guard && *a.guard) return;
   const int b = blockIdx.x / a.C, c = blockIdx.x - (blockIdx.x / a.C) * a.C;
   const int lane = threadIdx.x;
-  int* dets = reinterpret_cast<int*>(lds4);            // 100
-  int* gts = dets + 100;                               // max_gt
+  double* det_pa = reinterpret_cast<double*>(lds4);   // 100: pixel area of ranked detection r
+  i64* det_rw = reinterpret_cast<i64*>(det_pa + 100);  // 100: its intersection row
+  int* dets = reinterpret_cast<int*>(det_rw + 100);    // 100
+  double* pair_inter = reinterpret_cast<double*>(dets + 100);  // kMatchStagePairs
+  double* gt_px = pair_inter + kMatchStagePairs;               // kMatchStageGts: pixel areas
+  double* gt_an = gt_px + kMatchStageGts;                      // kMatchStageGts: annotation areas
+  uint8_t* gt_cr = reinterpret_cast<uint8_t*>(gt_an + kMatchStageGts);  // kMatchStageGts
+  int* gts = reinterpret_cast<int*>(gt_cr + kMatchStageGts);   // max_gt
   uint32_t* used = reinterpret_cast<uint32_t*>(gts + a.max_gt);  // 40 x words
   const i64 d0 = a.det_off[b], g0 = a.gt_off[b];
   const int D = static_cast<int>(a.det_off[b + 1] - d0);
@@ -917,8 +931,15 @@ __global__ __launch_bounds__(64) void coco_match(MatchArgs a) {
   constexpr int kScan = 4;
   for (int base = 0; base < D; base += kWave * kScan) {
     int cls[kScan];
+    double area[kScan];
+    i64 row[kScan];
 #pragma unroll
-    for (int k = 0; k < kScan; ++k) cls[k] = a.det_cls[d0 + min(base + k * kWave + lane, D - 1)];
+    for (int k = 0; k < kScan; ++k) {
+      const int i = min(base + k * kWave + lane, D - 1);
+      cls[k] = a.det_cls[d0 + i];
+      area[k] = a.det_area[d0 + i];
+      row[k] = a.det_row ? a.det_row[d0 + i] : i;
+    }
 #pragma unroll
     for (int k = 0; k < kScan; ++k) {
       const int i = base + k * kWave + lane;
@@ -928,6 +949,8 @@ __global__ __launch_bounds__(64) void coco_match(MatchArgs a) {
         const int rank = nd + __popcll(mask & ((1ULL << lane) - 1));
         if (rank < 100) {
           dets[rank] = i;
+          det_pa[rank] = area[k];
+          det_rw[rank] = row[k];
           a.ranks[d0 + i] = rank + 1;
         } else {
           unranked(d0 + i);
@@ -941,10 +964,12 @@ __global__ __launch_bounds__(64) void coco_match(MatchArgs a) {
   nd = min(nd, 100);
   // Ground truth of class c (metadata order) and positives per area range.
   // As above: class, crowd flag and annotation area of four chunks loaded
-  // together (then the thing flags of those classes), before any is used.
+  // together before any is used.  A GT is a candidate only if its class is
+  // c, so the thing-class filter needs only c's own flag.
   int ng = 0;
   int positives[4] = {0, 0, 0, 0};
-  for (int base = 0; base < G; base += kWave * kScan) {
+  const bool c_candidate = !a.gt_things || a.gt_things[c];
+  for (int base = 0; base < G && c_candidate; base += kWave * kScan) {
     int cls[kScan];
     bool crowded[kScan];
     double ann[kScan];
@@ -955,15 +980,10 @@ __global__ __launch_bounds__(64) void coco_match(MatchArgs a) {
       crowded[k] = a.gt_crowd && a.gt_crowd[g0 + j];
       ann[k] = gt_ann_area(a, b, g0, j);
     }
-    if (a.gt_things) {
-#pragma unroll
-      for (int k = 0; k < kScan; ++k)
-        if (cls[k] >= 0 && cls[k] < a.C && !a.gt_things[cls[k]]) cls[k] = -1;
-    }
 #pragma unroll
     for (int k = 0; k < kScan; ++k) {
       const int j = base + k * kWave + lane;
-      const bool sel = j < G && cls[k] >= 0 && cls[k] < a.C && cls[k] == c;
+      const bool sel = j < G && cls[k] == c;
       const u64 mask = __ballot(sel);
       if (sel) gts[ng + __popcll(mask & ((1ULL << lane) - 1))] = j;
       ng += __popcll(mask);
@@ -982,6 +1002,32 @@ __global__ __launch_bounds__(64) void coco_match(MatchArgs a) {
       if (positives[r]) atomicAdd(reinterpret_cast<u64*>(a.counts) + c * 4 + r, static_cast<u64>(positives[r]));
   if (nd == 0) return;
   for (int w = lane; w < 40 * a.words; w += kWave) used[w] = 0;
+  const i64 ib = a.inter_off[b];
+  const bool staged = ng <= kMatchStageGts && nd * ng <= kMatchStagePairs;
+  if (staged && ng > 0) {
+    __syncthreads();  // gts[] complete
+    for (int q = lane; q < ng; q += kWave) {
+      const int j = gts[q];
+      gt_cr[q] = a.gt_crowd && a.gt_crowd[g0 + j] != 0;
+      gt_px[q] = gt_pixel_area(a, b, g0, j);
+      gt_an[q] = gt_ann_area(a, b, g0, j);
+    }
+    // Four pairs per lane per step, loads issued together (clamped).
+    const int pairs = nd * ng;
+    for (int e0 = lane; e0 < pairs; e0 += 4 * kWave) {
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int e = min(e0 + u * kWave, pairs - 1), r = e / ng, q = e - r * ng;
+        const i64 idx = ib + det_rw[r] * G + gts[q];
+        v[u] = a.inter_bytes == 8 ? static_cast<double>(static_cast<const i64*>(a.inter)[idx])
+                                  : static_cast<double>(static_cast<const int32_t*>(a.inter)[idx]);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (e0 + u * kWave < pairs) pair_inter[e0 + u * kWave] = v[u];
+    }
+  }
   __syncthreads();
   if (lane >= 40) return;
   const int range = lane / 10, t = lane - (lane / 10) * 10;
@@ -989,31 +1035,53 @@ __global__ __launch_bounds__(64) void coco_match(MatchArgs a) {
   area_range(range, lo, hi);
   const double thr = iou_threshold(t);
   uint32_t* taken = used + lane * a.words;
-  const i64 ib = a.inter_off[b];
   for (int r = 0; r < nd; ++r) {
     const int i = dets[r];
-    const i64 row = a.det_row ? a.det_row[d0 + i] : i;
-    const double pa = a.det_area[d0 + i];
+    const i64 row = det_rw[r];
+    const double pa = det_pa[r];
     int best = -1, best_ignore = 2;
     double best_iou = thr;
-    for (int q = 0; q < ng; ++q) {
-      const int j = gts[q];
-      const bool crowd = a.gt_crowd && a.gt_crowd[g0 + j] != 0;
-      if (!crowd && ((taken[q >> 5] >> (q & 31)) & 1u)) continue;
-      const i64 idx = ib + row * G + j;
-      const double inter = a.inter_bytes == 8 ? static_cast<double>(static_cast<const i64*>(a.inter)[idx])
-                                              : static_cast<double>(static_cast<const int32_t*>(a.inter)[idx]);
-      double uni = crowd ? pa : pa + gt_pixel_area(a, b, g0, j) - inter;
-      if (uni < 1.0) uni = 1.0;
-      const double iou = inter / uni;
-      const double ann = gt_ann_area(a, b, g0, j);
-      const int ignore = crowd || ann < lo || ann > hi;
-      // Non-ignored GT first, then larger IoU (>= running best), later
-      // metadata entry on equal IoU: COCO's scan order.
-      if (iou >= thr && (ignore < best_ignore || (ignore == best_ignore && iou >= best_iou))) {
-        best = q;
-        best_ignore = ignore;
-        best_iou = iou;
+    // Candidates four at a time: their loads (crowd flag, intersection, pixel
+    // and annotation areas) issue together; the greedy choice then visits
+    // them in metadata order, as one at a time.
+    constexpr int Q = 4;
+    for (int q0 = 0; q0 < ng; q0 += Q) {
+      bool crowd[Q];
+      double inter[Q], garea[Q], ann[Q];
+#pragma unroll
+      for (int u = 0; u < Q; ++u) {
+        const int q = min(q0 + u, ng - 1);
+        if (staged) {
+          crowd[u] = gt_cr[q] != 0;
+          inter[u] = pair_inter[r * ng + q];
+          garea[u] = gt_px[q];
+          ann[u] = gt_an[q];
+        } else {
+          const int j = gts[q];
+          crowd[u] = a.gt_crowd && a.gt_crowd[g0 + j] != 0;
+          const i64 idx = ib + row * G + j;
+          inter[u] = a.inter_bytes == 8 ? static_cast<double>(static_cast<const i64*>(a.inter)[idx])
+                                        : static_cast<double>(static_cast<const int32_t*>(a.inter)[idx]);
+          garea[u] = gt_pixel_area(a, b, g0, j);
+          ann[u] = gt_ann_area(a, b, g0, j);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < Q; ++u) {
+        const int q = q0 + u;
+        if (q >= ng) break;
+        if (!crowd[u] && ((taken[q >> 5] >> (q & 31)) & 1u)) continue;
+        double uni = crowd[u] ? pa : pa + garea[u] - inter[u];
+        if (uni < 1.0) uni = 1.0;
+        const double iou = inter[u] / uni;
+        const int ignore = crowd[u] || ann[u] < lo || ann[u] > hi;
+        // Non-ignored GT first, then larger IoU (>= running best), later
+        // metadata entry on equal IoU: COCO's scan order.
+        if (iou >= thr && (ignore < best_ignore || (ignore == best_ignore && iou >= best_iou))) {
+          best = q;
+          best_ignore = ignore;
+          best_iou = iou;
+        }
       }
     }
     int8_t flag = -1;
@@ -1702,7 +1770,8 @@ extern "C" int pe_coco_match(int64_t B, int64_t C, const int64_t* det_off, const
   a.gt_things = gt_things;
   a.gt_area_i64 = reinterpret_cast<const i64*>(gt_area_i64);
   a.gt_area_stride = gt_area_stride;
-  const size_t lds = 4 * (100 + static_cast<size_t>(max_gt)) + 4 * 40 * static_cast<size_t>(a.words) + 16;
+  const size_t lds = 16 * 100 + 4 * (100 + static_cast<size_t>(max_gt)) + kMatchStageBytes +
+                     4 * 40 * static_cast<size_t>(a.words) + 16;
   PE_REQUIRE(lds <= 150 * 1024, "too many ground-truth instances per image for the matcher");
   if (lds > 64 * 1024)
     PE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(coco_match), hipFuncAttributeMaxDynamicSharedMemorySize,

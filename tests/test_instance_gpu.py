@@ -474,3 +474,26 @@ def test_score_order_input_dtypes(pe):
             ev.update(pe.InstanceBatch([cuda(pm[b])], [classes[b]], [scores[b]]),
                       pe.InstanceBatch([targets.masks[b]], [targets.classes[b]]))
         compare_instance(to_numpy(ev.compute()), expected)
+
+
+def test_matcher_with_many_pairs_of_one_class(pe):
+    """40 detections x 20 ground-truth masks of one class exceed the matcher's
+    LDS staging (512 pairs), so its chains read global memory; a second image
+    stays within the staging.  Both against the oracle."""
+    rng = np.random.default_rng(13)
+    h, w = 18, 22
+    pm = [rng.random((40, h, w)) > 0.6, rng.random((6, h, w)) > 0.6]
+    gm = [rng.random((20, h, w)) > 0.55, rng.random((4, h, w)) > 0.55]
+    pm[0][:20] = gm[0]
+    pm[0][20:25] &= gm[0][:5]
+    pcl = [np.zeros(40, int), rng.integers(0, 2, 6)]
+    gcl = [np.zeros(20, int), rng.integers(0, 2, 4)]
+    sc = [rng.random(40), rng.random(6)]
+    cr = [rng.random(20) < 0.1, np.zeros(4, bool)]
+    ev = pe.InstanceEvaluator(2)
+    ev.update(pe.InstanceBatch([cuda(m) for m in pm], [cuda(c) for c in pcl], [cuda(s) for s in sc]),
+              pe.InstanceBatch([cuda(m) for m in gm], [cuda(c) for c in gcl], crowd=[cuda(c) for c in cr]))
+    state = oinst.InstanceState(2)
+    state.update([dict(masks=m, classes=c, scores=s) for m, c, s in zip(pm, pcl, sc)],
+                 [dict(masks=m, classes=c, crowd=k) for m, c, k in zip(gm, gcl, cr)])
+    compare_instance(to_numpy(ev.compute()), state.compute())
