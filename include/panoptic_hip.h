@@ -255,9 +255,11 @@ int64_t pe_score_order_workspace_bytes(int64_t N, int64_t B);
 /* Per-image stable descending score order (ties keep input order) of N
  * detections grouped by det_off (B + 1, device): order[i] = within-image
  * index of sorted position i; scores/classes gathered into that order
- * (classes also as int32).  Replaces the per-image
+ * (classes also as int32).  max_dets: the largest image's detection count
+ * (at most 4096: one in-LDS sort per image; 0 or more: a segmented radix
+ * sort).  Replaces the per-image
  * torch.argsort(scores, descending=True, stable=True) of instance.py:140. */
-int pe_score_order(int64_t B, int64_t N, const int64_t* det_off, const double* scores,
+int pe_score_order(int64_t B, int64_t N, int64_t max_dets, const int64_t* det_off, const double* scores,
                    const int64_t* classes, int64_t* order, double* sorted_scores,
                    int64_t* sorted_classes, int32_t* sorted_classes32, void* workspace,
                    int64_t workspace_bytes, void* stream);

@@ -1782,6 +1782,56 @@ extern "C" int pe_coco_match(int64_t B, int64_t C, const int64_t* det_off, const
   return kOk;
 }
 
+// Images with at most kOrderLds detections: one workgroup per image sorts
+// (score key, index) pairs with a bitonic network in LDS and writes the
+// outputs directly (one launch instead of keys + segmented sort + gather).
+constexpr int kOrderLds = 4096;
+
+__global__ __launch_bounds__(1024) void score_order_lds(const i64* det_off, const double* scores, const i64* classes,
+                                                        int slots, i64* order, double* sorted_scores,
+                                                        i64* sorted_classes, int32_t* sorted_classes32) {
+  extern __shared__ uint4 lds4[];
+  u64* keys = reinterpret_cast<u64*>(lds4);
+  int32_t* idx = reinterpret_cast<int32_t*>(keys + slots);
+  const int b = blockIdx.x, tid = threadIdx.x, nthreads = blockDim.x;
+  const i64 first = det_off[b];
+  const int D = static_cast<int>(det_off[b + 1] - first);
+  if (D <= 0) return;  // uniform
+  int m = 1;
+  while (m < D) m <<= 1;
+  for (int e = tid; e < m; e += nthreads) {
+    keys[e] = e < D ? score_key_desc(scores[first + e]) : ~0ULL;
+    idx[e] = e < D ? e : INT_MAX;
+  }
+  __syncthreads();
+  // Ascending (key, index): descending score, ties in input order.
+  for (int k = 2; k <= m; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int e = tid; e < m; e += nthreads) {
+        const int f = e ^ j;
+        if (f <= e) continue;
+        const u64 ke = keys[e], kf = keys[f];
+        const int ie = idx[e], jf = idx[f];
+        const bool greater = ke > kf || (ke == kf && ie > jf);
+        if (greater == ((e & k) == 0)) {
+          keys[e] = kf;
+          keys[f] = ke;
+          idx[e] = jf;
+          idx[f] = ie;
+        }
+      }
+      __syncthreads();
+    }
+  for (int e = tid; e < D; e += nthreads) {
+    const int src = idx[e];
+    order[first + e] = src;
+    sorted_scores[first + e] = scores[first + src];
+    const i64 c = classes[first + src];
+    sorted_classes[first + e] = c;
+    sorted_classes32[first + e] = static_cast<int32_t>(c);
+  }
+}
+
 namespace {
 struct ApBuffers {
   u64* keys_in;
@@ -2035,7 +2085,7 @@ extern "C" int64_t pe_score_order_workspace_bytes(int64_t N, int64_t B) {
   return static_cast<int64_t>(align_up(carve.used, 256));
 }
 
-extern "C" int pe_score_order(int64_t B, int64_t N, const int64_t* det_off, const double* scores,
+extern "C" int pe_score_order(int64_t B, int64_t N, int64_t max_dets, const int64_t* det_off, const double* scores,
                               const int64_t* classes, int64_t* order, double* sorted_scores,
                               int64_t* sorted_classes, int32_t* sorted_classes32, void* workspace,
                               int64_t workspace_bytes, void* stream_ptr) {
@@ -2043,6 +2093,20 @@ extern "C" int pe_score_order(int64_t B, int64_t N, const int64_t* det_off, cons
   PE_REQUIRE(workspace_bytes >= pe_score_order_workspace_bytes(N, B), "workspace too small");
   if (N == 0 || B == 0) return kOk;
   hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+  if (max_dets > 0 && max_dets <= kOrderLds) {
+    int slots = 64;
+    while (slots < max_dets) slots <<= 1;
+    const int threads = std::min(1024, std::max(64, slots / 2));
+    const size_t lds = 12 * static_cast<size_t>(slots);
+    if (lds > 64 * 1024)
+      PE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(score_order_lds),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    score_order_lds<<<static_cast<unsigned>(B), threads, lds, stream>>>(
+        reinterpret_cast<const i64*>(det_off), scores, reinterpret_cast<const i64*>(classes), slots,
+        reinterpret_cast<i64*>(order), sorted_scores, reinterpret_cast<i64*>(sorted_classes), sorted_classes32);
+    PE_LAUNCH_CHECK();
+    return kOk;
+  }
   Carve carve(workspace);
   u64* keys_in = carve.take<u64>(N);
   u64* keys_out = carve.take<u64>(N);

@@ -33,7 +33,7 @@ for _name, _sig in {
     "pe_coco_match": (_int, [_i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _int,
                              _i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _ptr]),
     "pe_score_order_workspace_bytes": (_i64, [_i64, _i64]),
-    "pe_score_order": (_int, [_i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _ptr]),
+    "pe_score_order": (_int, [_i64, _i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _ptr]),
     "pe_ap_workspace_bytes": (_i64, [_i64, _i64]),
     "pe_ap_compute": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _ptr, _ptr, _ptr, _ptr, _i64, _ptr]),
     "pe_row_sums": (_int, [_ptr, _i64, _i64, _ptr, _ptr]),
@@ -183,8 +183,8 @@ def score_order(images, device) -> ScoreOrder:
     if N:
         size = max(_native.library().pe_score_order_workspace_bytes(N, B), 256)
         workspace = torch.empty(size, dtype=torch.uint8, device=device)
-        _native.call("pe_score_order", B, N, det_off.data_ptr(), scores.data_ptr(), classes.data_ptr(),
-                     order.data_ptr(), sorted_scores.data_ptr(), sorted_classes.data_ptr(),
+        _native.call("pe_score_order", B, N, max(counts), det_off.data_ptr(), scores.data_ptr(),
+                     classes.data_ptr(), order.data_ptr(), sorted_scores.data_ptr(), sorted_classes.data_ptr(),
                      sorted_classes32.data_ptr(), workspace.data_ptr(), workspace.numel(), _native.stream(device))
     return ScoreOrder(counts, det_off, order, sorted_scores, sorted_classes, sorted_classes32)
 

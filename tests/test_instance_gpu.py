@@ -497,3 +497,23 @@ def test_matcher_with_many_pairs_of_one_class(pe):
     state.update([dict(masks=m, classes=c, scores=s) for m, c, s in zip(pm, pcl, sc)],
                  [dict(masks=m, classes=c, crowd=k) for m, c, k in zip(gm, gcl, cr)])
     compare_instance(to_numpy(ev.compute()), state.compute())
+
+
+def test_score_order_beyond_the_lds_sort(pe):
+    """An image with more detections than the in-LDS per-image sort holds
+    (4096) takes the segmented radix sort; both images against the oracle."""
+    rng = np.random.default_rng(14)
+    h, w = 6, 7
+    dets = [5000, 37]
+    pm = [rng.random((d, h, w)) > 0.5 for d in dets]
+    gm = [rng.random((3, h, w)) > 0.5 for _ in dets]
+    pcl = [rng.integers(0, 2, d) for d in dets]
+    gcl = [rng.integers(0, 2, 3) for _ in dets]
+    sc = [rng.integers(0, 300, d) / 300.0 for d in dets]  # many ties
+    ev = pe.InstanceEvaluator(2)
+    ev.update(pe.InstanceBatch([cuda(m) for m in pm], [cuda(c) for c in pcl], [cuda(s) for s in sc]),
+              pe.InstanceBatch([cuda(m) for m in gm], [cuda(c) for c in gcl]))
+    state = oinst.InstanceState(2)
+    state.update([dict(masks=m, classes=c, scores=s) for m, c, s in zip(pm, pcl, sc)],
+                 [dict(masks=m, classes=c) for m, c in zip(gm, gcl)])
+    compare_instance(to_numpy(ev.compute()), state.compute())
