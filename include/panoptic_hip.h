@@ -264,6 +264,15 @@ int pe_score_order(int64_t B, int64_t N, int64_t max_dets, const int64_t* det_of
                    int64_t* sorted_classes, int32_t* sorted_classes32, void* workspace,
                    int64_t workspace_bytes, void* stream);
 
+/* pe_score_order for at most 64 images of at most 4096 detections each, read
+ * in place: scores and classes are HOST arrays of B device pointers (image b's
+ * f64 scores and its int32 or int64 classes, per class_bytes), copied into the
+ * kernel's arguments, so no concatenated copy and no upload is needed.  No
+ * workspace. */
+int pe_score_order_images(int64_t B, int64_t max_dets, const int64_t* det_off, const double* const* scores,
+                          const void* const* classes, int class_bytes, int64_t* order, double* sorted_scores,
+                          int64_t* sorted_classes, int32_t* sorted_classes32, void* stream);
+
 /* Scratch bytes of pe_ap_compute for N detection records and C classes. */
 int64_t pe_ap_workspace_bytes(int64_t N, int64_t C);
 

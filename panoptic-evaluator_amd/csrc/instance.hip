@@ -1787,9 +1787,23 @@ extern "C" int pe_coco_match(int64_t B, int64_t C, const int64_t* det_off, const
 // outputs directly (one launch instead of keys + segmented sort + gather).
 constexpr int kOrderLds = 4096;
 
+// Per-image input arrays passed by value in the kernel arguments (no
+// concatenated copy and no device table): image b's scores (f64) and classes
+// (int32 or int64) start at scores[b] and classes[b].
+constexpr int kOrderArgImages = 64;
+struct ImageArrays {
+  const double* scores[kOrderArgImages];
+  const void* classes[kOrderArgImages];
+  int class_bytes;
+};
+
+// Scores and classes come either concatenated (scores/classes, image b at
+// det_off[b]) or per image (arrays, when ARRAYS).
+template <bool ARRAYS>
 __global__ __launch_bounds__(1024) void score_order_lds(const i64* det_off, const double* scores, const i64* classes,
-                                                        int slots, i64* order, double* sorted_scores,
-                                                        i64* sorted_classes, int32_t* sorted_classes32) {
+                                                        ImageArrays arrays, int slots, i64* order,
+                                                        double* sorted_scores, i64* sorted_classes,
+                                                        int32_t* sorted_classes32) {
   extern __shared__ uint4 lds4[];
   u64* keys = reinterpret_cast<u64*>(lds4);
   int32_t* idx = reinterpret_cast<int32_t*>(keys + slots);
@@ -1797,10 +1811,11 @@ __global__ __launch_bounds__(1024) void score_order_lds(const i64* det_off, cons
   const i64 first = det_off[b];
   const int D = static_cast<int>(det_off[b + 1] - first);
   if (D <= 0) return;  // uniform
+  const double* s = ARRAYS ? arrays.scores[b] : scores + first;
   int m = 1;
   while (m < D) m <<= 1;
   for (int e = tid; e < m; e += nthreads) {
-    keys[e] = e < D ? score_key_desc(scores[first + e]) : ~0ULL;
+    keys[e] = e < D ? score_key_desc(s[e]) : ~0ULL;
     idx[e] = e < D ? e : INT_MAX;
   }
   __syncthreads();
@@ -1825,11 +1840,36 @@ __global__ __launch_bounds__(1024) void score_order_lds(const i64* det_off, cons
   for (int e = tid; e < D; e += nthreads) {
     const int src = idx[e];
     order[first + e] = src;
-    sorted_scores[first + e] = scores[first + src];
-    const i64 c = classes[first + src];
+    sorted_scores[first + e] = s[src];
+    i64 c;
+    if (!ARRAYS)
+      c = classes[first + src];
+    else if (arrays.class_bytes == 8)
+      c = static_cast<const i64*>(arrays.classes[b])[src];
+    else
+      c = static_cast<const int32_t*>(arrays.classes[b])[src];
     sorted_classes[first + e] = c;
     sorted_classes32[first + e] = static_cast<int32_t>(c);
   }
+}
+
+static int launch_score_order_lds(bool arrays, int64_t B, int64_t max_dets, const int64_t* det_off,
+                                  const double* scores, const int64_t* classes, const ImageArrays& images,
+                                  int64_t* order, double* sorted_scores, int64_t* sorted_classes,
+                                  int32_t* sorted_classes32, hipStream_t stream) {
+  int slots = 64;
+  while (slots < max_dets) slots <<= 1;
+  const int threads = std::min(1024, std::max(64, slots / 2));
+  const size_t lds = 12 * static_cast<size_t>(slots);
+  auto kernel = arrays ? score_order_lds<true> : score_order_lds<false>;
+  if (lds > 64 * 1024)
+    PE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               static_cast<int>(lds)));
+  kernel<<<static_cast<unsigned>(B), threads, lds, stream>>>(
+      reinterpret_cast<const i64*>(det_off), scores, reinterpret_cast<const i64*>(classes), images, slots,
+      reinterpret_cast<i64*>(order), sorted_scores, reinterpret_cast<i64*>(sorted_classes), sorted_classes32);
+  PE_LAUNCH_CHECK();
+  return kOk;
 }
 
 namespace {
@@ -2093,20 +2133,9 @@ extern "C" int pe_score_order(int64_t B, int64_t N, int64_t max_dets, const int6
   PE_REQUIRE(workspace_bytes >= pe_score_order_workspace_bytes(N, B), "workspace too small");
   if (N == 0 || B == 0) return kOk;
   hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
-  if (max_dets > 0 && max_dets <= kOrderLds) {
-    int slots = 64;
-    while (slots < max_dets) slots <<= 1;
-    const int threads = std::min(1024, std::max(64, slots / 2));
-    const size_t lds = 12 * static_cast<size_t>(slots);
-    if (lds > 64 * 1024)
-      PE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(score_order_lds),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    score_order_lds<<<static_cast<unsigned>(B), threads, lds, stream>>>(
-        reinterpret_cast<const i64*>(det_off), scores, reinterpret_cast<const i64*>(classes), slots,
-        reinterpret_cast<i64*>(order), sorted_scores, reinterpret_cast<i64*>(sorted_classes), sorted_classes32);
-    PE_LAUNCH_CHECK();
-    return kOk;
-  }
+  if (max_dets > 0 && max_dets <= kOrderLds)
+    return launch_score_order_lds(false, B, max_dets, det_off, scores, classes, ImageArrays{}, order,
+                                  sorted_scores, sorted_classes, sorted_classes32, stream);
   Carve carve(workspace);
   u64* keys_in = carve.take<u64>(N);
   u64* keys_out = carve.take<u64>(N);
@@ -2128,4 +2157,23 @@ extern "C" int pe_score_order(int64_t B, int64_t N, int64_t max_dets, const int6
       reinterpret_cast<i64*>(sorted_classes), sorted_classes32);
   PE_LAUNCH_CHECK();
   return kOk;
+}
+
+extern "C" int pe_score_order_images(int64_t B, int64_t max_dets, const int64_t* det_off,
+                                     const double* const* scores, const void* const* classes, int class_bytes,
+                                     int64_t* order, double* sorted_scores, int64_t* sorted_classes,
+                                     int32_t* sorted_classes32, void* stream_ptr) {
+  PE_REQUIRE(B >= 0 && B <= kOrderArgImages, "at most 64 images per call");
+  PE_REQUIRE(max_dets >= 0 && max_dets <= kOrderLds, "at most 4096 detections per image");
+  PE_REQUIRE(class_bytes == 4 || class_bytes == 8, "classes must be int32 or int64");
+  PE_REQUIRE(B == 0 || (scores && classes), "null per-image arrays");
+  if (B == 0 || max_dets == 0) return kOk;
+  ImageArrays images{};
+  for (int64_t b = 0; b < B; ++b) {
+    images.scores[b] = scores[b];
+    images.classes[b] = classes[b];
+  }
+  images.class_bytes = class_bytes;
+  return launch_score_order_lds(true, B, max_dets, det_off, nullptr, nullptr, images, order, sorted_scores,
+                                sorted_classes, sorted_classes32, static_cast<hipStream_t>(stream_ptr));
 }

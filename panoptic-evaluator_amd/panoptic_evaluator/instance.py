@@ -34,6 +34,7 @@ for _name, _sig in {
                              _i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _ptr]),
     "pe_score_order_workspace_bytes": (_i64, [_i64, _i64]),
     "pe_score_order": (_int, [_i64, _i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _ptr]),
+    "pe_score_order_images": (_int, [_i64, _i64, _ptr, _ptr, _ptr, _int, _ptr, _ptr, _ptr, _ptr, _ptr]),
     "pe_ap_workspace_bytes": (_i64, [_i64, _i64]),
     "pe_ap_compute": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _ptr, _ptr, _ptr, _ptr, _i64, _ptr]),
     "pe_row_sums": (_int, [_ptr, _i64, _i64, _ptr, _ptr]),
@@ -163,11 +164,32 @@ class ScoreOrder(NamedTuple):
     classes32: torch.Tensor       # int32, sorted (matching, records)
 
 
+ORDER_ARG_IMAGES = 64      # images whose arrays pe_score_order_images reads in place
+ORDER_LDS_DETS = 4096      # detections per image of the in-LDS sort
+
+
 def score_order(images, device) -> ScoreOrder:
     """One segmented radix sort over all images (instance.py:140 per image)."""
     counts = [image.masks.shape[0] for image in images]
     B, N = len(images), sum(counts)
     det_off = offsets(counts, device)
+    class_dtype = images[0].classes.dtype if images else torch.int64
+    if (N and B <= ORDER_ARG_IMAGES and max(counts) <= ORDER_LDS_DETS
+            and all(image.scores.dtype == torch.float64 and image.scores.is_contiguous()
+                    and image.classes.dtype == class_dtype and image.classes.is_contiguous() for image in images)):
+        # Every image's arrays read in place: their addresses travel in the
+        # kernel's arguments (no concatenation, no upload).
+        order = torch.empty(N, dtype=torch.int64, device=device)
+        sorted_scores = torch.empty(N, dtype=torch.float64, device=device)
+        sorted_classes = torch.empty(N, dtype=torch.int64, device=device)
+        sorted_classes32 = torch.empty(N, dtype=torch.int32, device=device)
+        pointers = ctypes.c_void_p * B
+        _native.call("pe_score_order_images", B, max(counts), det_off.data_ptr(),
+                     pointers(*(image.scores.data_ptr() for image in images)),
+                     pointers(*(image.classes.data_ptr() for image in images)),
+                     8 if class_dtype == torch.int64 else 4, order.data_ptr(), sorted_scores.data_ptr(),
+                     sorted_classes.data_ptr(), sorted_classes32.data_ptr(), _native.stream(device))
+        return ScoreOrder(counts, det_off, order, sorted_scores, sorted_classes, sorted_classes32)
     if B == 1:
         # One image: its own arrays (converted only if they are not float64 /
         # int64 already), no concatenated copy.
