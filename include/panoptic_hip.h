@@ -290,6 +290,21 @@ int pe_ap_compute(int64_t N, const double* scores, const int32_t* classes, const
                   double* recall, double* summary, void* workspace, int64_t workspace_bytes,
                   void* stream);
 
+/* pe_compute and pe_ap_compute in one call (no reference counterpart; used by
+ * the combined evaluator's compute()).  The first 17 arguments are
+ * pe_compute's without its stream, the rest pe_ap_compute's.  For at most
+ * 4096 records the finaliser rows run as extra workgroups of the AP launch,
+ * beside the AP classes; otherwise the finaliser is launched first.  Same
+ * outputs as the two calls. */
+int pe_compute_with_ap(const int64_t* tp, const int64_t* fp, const int64_t* fn, const double* sums,
+                       const int64_t* confusion, const uint8_t* things, int64_t K, int64_t C,
+                       double* per_class, double* summaries, int64_t* counts, double* miou,
+                       int64_t* out_tp, int64_t* out_fp, int64_t* out_fn, double* out_sums,
+                       int64_t* out_confusion, int64_t N, const double* scores, const int32_t* classes,
+                       const int32_t* ranks, const int8_t* status, const int64_t* ap_counts, int64_t ap_C,
+                       double* precision, double* recall, double* summary, void* workspace,
+                       int64_t workspace_bytes, void* stream);
+
 /* out[r] = sum of int64 row r as f64 (detection areas of a shared target
  * histogram, combined.py:107-108). */
 int pe_row_sums(const int64_t* pairs, int64_t rows, int64_t cols, double* out, void* stream);

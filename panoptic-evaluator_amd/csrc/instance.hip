@@ -32,6 +32,7 @@
 #include <cmath>
 
 #include "common.h"
+#include "finalize.h"
 #include "../../include/panoptic_hip.h"
 
 namespace pe {
@@ -1426,10 +1427,9 @@ struct ApSmallArgs {
   int sort_slots;        // power of two >= N
 };
 
-__global__ __launch_bounds__(kApSmallThreads) void ap_small(ApSmallArgs s) {
-  extern __shared__ uint4 lds4[];
+__device__ __forceinline__ void ap_small_class(const ApSmallArgs& s, int c, uint4* lds4) {
   const ApArgs& a = s.ap;
-  const int c = blockIdx.x, tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   constexpr int kWaves = kApSmallThreads / kWave;
   u64* keys = reinterpret_cast<u64*>(lds4);
   int32_t* idx = reinterpret_cast<int32_t*>(keys + s.sort_slots);
@@ -1517,6 +1517,23 @@ __global__ __launch_bounds__(kApSmallThreads) void ap_small(ApSmallArgs s) {
   uint32_t* codes = reinterpret_cast<uint32_t*>(s_flag + 40 * kApChunk);
   ap_class<true, kApSmallThreads>(a, c, static_cast<const int32_t*>(idx), 0, n, s_rank, s_flag, codes,
                  reinterpret_cast<int*>(codes + 101 * kApJobs));
+}
+
+__global__ __launch_bounds__(kApSmallThreads) void ap_small(ApSmallArgs s) {
+  extern __shared__ uint4 lds4[];
+  ap_small_class(s, blockIdx.x, lds4);
+}
+
+// The same, with the panoptic finaliser's rows as extra workgroups of the
+// launch (both use 1024 threads), so the two run side by side.
+static_assert(kApSmallThreads == kFinalThreads, "the AP classes and the finaliser share one launch shape");
+__global__ __launch_bounds__(kApSmallThreads) void ap_small_final(ApSmallArgs s, FinalArgs f) {
+  extern __shared__ uint4 lds4[];
+  const int block = blockIdx.x;
+  if (block < s.ap.C)
+    ap_small_class(s, block, lds4);
+  else
+    finalize_row(f, block - s.ap.C, lds4);
 }
 
 size_t ap_small_lds(int slots) { return 12 * static_cast<size_t>(slots) + kApChunk + 40 * kApChunk + kApTableBytes; }
@@ -1925,10 +1942,12 @@ extern "C" int64_t pe_ap_workspace_bytes(int64_t N, int64_t C) {
   return static_cast<int64_t>(align_up(used, 256));
 }
 
-extern "C" int pe_ap_compute(int64_t N, const double* scores, const int32_t* classes, const int32_t* ranks,
-                             const int8_t* status, const int64_t* counts, int64_t C, double* precision,
-                             double* recall, double* summary, void* workspace, int64_t workspace_bytes,
-                             void* stream_ptr) {
+// AP accumulation; with `fin`, also the K panoptic finaliser rows (in the
+// same launch as the AP classes for small batches, else just before).
+static int ap_compute(int64_t N, const double* scores, const int32_t* classes, const int32_t* ranks,
+                      const int8_t* status, const int64_t* counts, int64_t C, double* precision, double* recall,
+                      double* summary, void* workspace, int64_t workspace_bytes, void* stream_ptr,
+                      const FinalArgs* fin, int64_t K) {
   PE_REQUIRE(N >= 0 && N < INT_MAX && C > 0 && C <= INT_MAX / 240, "invalid AP shapes");
   PE_REQUIRE(workspace_bytes >= pe_ap_workspace_bytes(N, C), "workspace too small");
   PE_REQUIRE(reinterpret_cast<uintptr_t>(status) % 8 == 0, "match flags must be 8-byte aligned");
@@ -1956,10 +1975,18 @@ extern "C" int pe_ap_compute(int64_t N, const double* scores, const int32_t* cla
     sa.N = static_cast<int>(N);
     sa.sort_slots = 64;  // keeps the flag staging area 16-byte aligned
     while (sa.sort_slots < N) sa.sort_slots <<= 1;
-    const size_t lds = ap_small_lds(sa.sort_slots);
-    PE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(ap_small), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               static_cast<int>(lds)));
-    {
+    size_t lds = ap_small_lds(sa.sort_slots);
+    if (fin && K > 0) {
+      lds = std::max(lds, 16 * static_cast<size_t>(fin->C));
+      PE_REQUIRE(lds <= 160 * 1024, "too many categories for the finaliser");
+      PE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(ap_small_final),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+      ProfileScope timing("ap_accumulate", stream);
+      ap_small_final<<<static_cast<unsigned>(C + K), kApSmallThreads, lds, stream>>>(sa, *fin);
+      PE_LAUNCH_CHECK();
+    } else {
+      PE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(ap_small), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 static_cast<int>(lds)));
       ProfileScope timing("ap_accumulate", stream);
       ap_small<<<static_cast<unsigned>(C), kApSmallThreads, lds, stream>>>(sa);
       PE_LAUNCH_CHECK();
@@ -1968,6 +1995,7 @@ extern "C" int pe_ap_compute(int64_t N, const double* scores, const int32_t* cla
     PE_LAUNCH_CHECK();
     return kOk;
   }
+  if (fin && (rc = launch_finalize(*fin, K, stream))) return rc;
   if ((rc = launch_zero(w.start, 4 * static_cast<size_t>(C), stream))) return rc;
   if ((rc = launch_zero(w.end, 4 * static_cast<size_t>(C), stream))) return rc;
   if (N > 0) {
@@ -1994,6 +2022,30 @@ extern "C" int pe_ap_compute(int64_t N, const double* scores, const int32_t* cla
   ap_summaries<<<1, 256, 0, stream>>>(static_cast<int>(C), w.partial, summary);
   PE_LAUNCH_CHECK();
   return kOk;
+}
+
+extern "C" int pe_ap_compute(int64_t N, const double* scores, const int32_t* classes, const int32_t* ranks,
+                             const int8_t* status, const int64_t* counts, int64_t C, double* precision,
+                             double* recall, double* summary, void* workspace, int64_t workspace_bytes,
+                             void* stream_ptr) {
+  return ap_compute(N, scores, classes, ranks, status, counts, C, precision, recall, summary, workspace,
+                    workspace_bytes, stream_ptr, nullptr, 0);
+}
+
+extern "C" int pe_compute_with_ap(const int64_t* tp, const int64_t* fp, const int64_t* fn, const double* sums,
+                                  const int64_t* confusion, const uint8_t* things, int64_t K, int64_t C,
+                                  double* per_class, double* summaries, int64_t* counts, double* miou,
+                                  int64_t* out_tp, int64_t* out_fp, int64_t* out_fn, double* out_sums,
+                                  int64_t* out_confusion, int64_t N, const double* scores, const int32_t* classes,
+                                  const int32_t* ranks, const int8_t* status, const int64_t* ap_counts,
+                                  int64_t ap_C, double* precision, double* recall, double* summary,
+                                  void* workspace, int64_t workspace_bytes, void* stream) {
+  FinalArgs fin;
+  const int rc = make_final_args(tp, fp, fn, sums, confusion, things, K, C, per_class, summaries, counts, miou,
+                                 out_tp, out_fp, out_fn, out_sums, out_confusion, &fin);
+  if (rc) return rc;
+  return ap_compute(N, scores, classes, ranks, status, ap_counts, ap_C, precision, recall, summary, workspace,
+                    workspace_bytes, stream, &fin, K);
 }
 
 // ------------------------------------------------------- small helpers

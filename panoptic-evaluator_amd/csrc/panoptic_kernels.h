@@ -82,5 +82,10 @@ int launch_joint_histogram(HistArgs a, int gt_bytes, int pred_bytes, int chunk, 
 int launch_prepare(const ScoreArgs& a, int* error, hipStream_t stream);
 int launch_score(ScoreArgs a, hipStream_t stream);
 int launch_finalize(const FinalArgs& a, i64 K, hipStream_t stream);
+// Checks pe_compute's arguments and fills `out` (0, or a negative status).
+int make_final_args(const int64_t* tp, const int64_t* fp, const int64_t* fn, const double* sums,
+                    const int64_t* confusion, const uint8_t* things, int64_t K, int64_t C, double* per_class,
+                    double* summaries, int64_t* counts, double* miou, int64_t* out_tp, int64_t* out_fp,
+                    int64_t* out_fn, double* out_sums, int64_t* out_confusion, FinalArgs* out);
 
 }  // namespace pe

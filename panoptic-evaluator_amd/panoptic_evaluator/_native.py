@@ -37,6 +37,9 @@ _SIGNATURES = {
                          _ptr, _ptr, _ptr, _i64, _int, _int, _int, _ptr]),
     "pe_compute": (_int, [_ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _ptr, _ptr, _ptr, _ptr,
                           _ptr, _ptr, _ptr, _ptr, _ptr, _ptr]),
+    "pe_compute_with_ap": (_int, [_ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _ptr, _ptr, _ptr, _ptr,
+                                  _ptr, _ptr, _ptr, _ptr, _ptr,
+                                  _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _ptr, _ptr, _ptr, _ptr, _i64, _ptr]),
     "pe_image_pq": (_int, [_ptr, _ptr, _ptr, _ptr, _i64, _i64, _ptr, _ptr, _ptr]),
     "pe_pack": (_int, [_ptr, _int, _i64, _i64, _ptr, _ptr, _i64, _ptr, _ptr, _ptr]),
     "pe_stream_read": (_int, [_ptr, _i64, _int, _ptr, _ptr]),

@@ -23,8 +23,9 @@ from typing import Sequence
 import torch
 
 from . import _native
-from .evaluator import PanopticBatch, PanopticEvaluator, _prepare
-from .instance import InstanceBatch, InstanceEvaluator, PackedMasks, as_bits, offsets, raise_instance_error, score_order
+from .evaluator import PanopticBatch, PanopticEvaluator, _prepare, final_launch_args, final_results
+from .instance import (InstanceBatch, InstanceEvaluator, PackedMasks, ap_launch_args, ap_result, as_bits, offsets,
+                       raise_instance_error, score_order)
 
 
 class SegmentationEvaluator:
@@ -142,6 +143,13 @@ class SegmentationEvaluator:
 
     def compute(self):
         """Panoptic/semantic results plus an 'instance' AP/AR dictionary."""
-        result = self.panoptic.compute()
-        result["instance"] = self.instance.compute()
+        # One launch runs the PQ finaliser beside the AP classes.
+        pan = self.panoptic
+        final_outputs, (final_args, _inputs) = final_launch_args(
+            pan.tp[None], pan.fp[None], pan.fn[None], pan.iou_sum[None], pan.confusion[None], pan.isthing)
+        ap_outputs, ap_args, _keep = ap_launch_args(self.instance._records, self.instance._counts,
+                                                    self.num_classes, self.device)
+        _native.call("pe_compute_with_ap", *final_args, *ap_args, _native.stream(self.device))
+        result = final_results(*final_outputs)[0]
+        result["instance"] = ap_result(*ap_outputs)
         return result

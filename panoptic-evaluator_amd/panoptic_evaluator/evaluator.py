@@ -364,10 +364,11 @@ class PanopticEvaluator:
                         self.confusion[None], self.isthing)[0]
 
 
-def finalize_arrays(tp, fp, fn, sums, confusion, things, *, copies=True):
-    """FP64 finaliser over K accumulator rows as device arrays: per_class
-    (K x 4 x C), summaries (K x 3 x 3: All/Things/Stuff x pq/sq/rq), counts
-    (K x 3), miou (K), and copies of the five state tensors (or None)."""
+def final_launch_args(tp, fp, fn, sums, confusion, things, *, copies=True):
+    """Outputs of the FP64 finaliser over K accumulator rows, allocated:
+    ((per_class K x 4 x C, summaries K x 3 x 3, counts K x 3, miou K, copies
+    of the five state tensors or None), pe_compute's arguments without the
+    stream)."""
     K, C = tp.shape
     device = tp.device
     doubles = dict(dtype=torch.float64, device=device)
@@ -377,16 +378,29 @@ def finalize_arrays(tp, fp, fn, sums, confusion, things, *, copies=True):
     miou = torch.empty(K, **doubles)
     state = [torch.empty_like(t) for t in (tp, fp, fn, sums, confusion)] if copies else [None] * 5
     tensors = [t.contiguous() for t in (tp, fp, fn, sums, confusion)]
-    _native.call("pe_compute", *(t.data_ptr() for t in tensors), things.data_ptr(), K, C,
-                 per_class.data_ptr(), summaries.data_ptr(), counts.data_ptr(), miou.data_ptr(),
-                 *(_native.ptr(t) for t in state), _native.stream(device))
-    return per_class, summaries, counts, miou, state
+    args = (*(t.data_ptr() for t in tensors), things.data_ptr(), K, C, per_class.data_ptr(),
+            summaries.data_ptr(), counts.data_ptr(), miou.data_ptr(), *(_native.ptr(t) for t in state))
+    # The inputs stay referenced until the call is issued.
+    return (per_class, summaries, counts, miou, state), (args, tensors)
+
+
+def finalize_arrays(tp, fp, fn, sums, confusion, things, *, copies=True):
+    """FP64 finaliser over K accumulator rows as device arrays: per_class
+    (K x 4 x C), summaries (K x 3 x 3: All/Things/Stuff x pq/sq/rq), counts
+    (K x 3), miou (K), and copies of the five state tensors (or None)."""
+    outputs, (args, _) = final_launch_args(tp, fp, fn, sums, confusion, things, copies=copies)
+    _native.call("pe_compute", *args, _native.stream(tp.device))
+    return outputs
 
 
 def finalize(tp, fp, fn, sums, confusion, things):
     """Run the FP64 finaliser over K accumulator rows; return K result dicts."""
-    K = tp.shape[0]
-    per_class, summaries, counts, miou, copies = finalize_arrays(tp, fp, fn, sums, confusion, things)
+    return final_results(*finalize_arrays(tp, fp, fn, sums, confusion, things))
+
+
+def final_results(per_class, summaries, counts, miou, copies):
+    """K result dictionaries over the finaliser's output arrays."""
+    K = per_class.shape[0]
     # Result views come from a few unbind calls (one dispatch each) rather
     # than one indexing op per entry.
     summary_rows = summaries.reshape(K, 9).unbind(0)

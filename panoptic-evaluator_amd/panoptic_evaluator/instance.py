@@ -397,6 +397,20 @@ class InstanceEvaluator:
 def accumulate(records, counts, num_classes, device):
     """Sort all records by score and accumulate AP/AR on the device
     (instance.py:205-248)."""
+    outputs, args, keep = ap_launch_args(records, counts, num_classes, device)
+    _native.call("pe_ap_compute", *args, _native.stream(device))
+    return ap_result(*outputs)
+
+
+def ap_result(summary, precision, recall):
+    result = {key: summary[i] for i, key in enumerate(SUMMARY_KEYS)}
+    result.update(precision=precision, recall=recall)
+    return result
+
+
+def ap_launch_args(records, counts, num_classes, device):
+    """AP outputs (summary, precision, recall), pe_ap_compute's arguments
+    without the stream, and the tensors those arguments point into."""
     C = num_classes
     if len(records) == 1:
         scores, classes, ranks, status = records[0]
@@ -416,9 +430,6 @@ def accumulate(records, counts, num_classes, device):
     summary = torch.empty(12, dtype=torch.float64, device=device)
     size = max(_native.library().pe_ap_workspace_bytes(N, C), 256)
     workspace = torch.empty(size, dtype=torch.uint8, device=device)
-    _native.call("pe_ap_compute", N, scores.data_ptr(), classes.data_ptr(), ranks.data_ptr(), status.data_ptr(),
-                 counts.data_ptr(), C, precision.data_ptr(), recall.data_ptr(), summary.data_ptr(),
-                 workspace.data_ptr(), workspace.numel(), _native.stream(device))
-    result = {key: summary[i] for i, key in enumerate(SUMMARY_KEYS)}
-    result.update(precision=precision, recall=recall)
-    return result
+    args = (N, scores.data_ptr(), classes.data_ptr(), ranks.data_ptr(), status.data_ptr(), counts.data_ptr(), C,
+            precision.data_ptr(), recall.data_ptr(), summary.data_ptr(), workspace.data_ptr(), workspace.numel())
+    return (summary, precision, recall), args, (scores, classes, ranks, status, workspace)

@@ -517,3 +517,29 @@ def test_score_order_beyond_the_lds_sort(pe):
     state.update([dict(masks=m, classes=c, scores=s) for m, c, s in zip(pm, pcl, sc)],
                  [dict(masks=m, classes=c) for m, c in zip(gm, gcl)])
     compare_instance(to_numpy(ev.compute()), state.compute())
+
+
+def test_combined_compute_beyond_the_small_ap_path(pe):
+    """More than 4096 detection records: compute() launches the finaliser on
+    its own before the radix-sort AP path (below that, both run in one
+    launch, as every other combined test checks)."""
+    rng = np.random.default_rng(15)
+    B, h, w, D = 2, 10, 12, 2100
+    maps = rng.integers(0, 6, (B, h, w)).astype(np.int32)
+    classes = np.array([[-1, 0, 1, 2, 0, 3]] * B, dtype=np.int32)
+    crowd = np.zeros((B, 6), bool)
+    pred_maps = np.where(rng.random((B, h, w)) < 0.8, maps, rng.integers(0, 6, (B, h, w))).astype(np.int32)
+    masks = [rng.random((D, h, w)) > 0.6 for _ in range(B)]
+    labels = [rng.integers(0, 3, D) for _ in range(B)]
+    scores = [rng.integers(0, 500, D) / 500.0 for _ in range(B)]
+    isthing = [True, True, True, False]
+    joint = pe.SegmentationEvaluator(4, isthing=isthing, validate=False)
+    joint.update(pe.PanopticBatch(cuda(pred_maps), cuda(classes)),
+                 pe.InstanceBatch([cuda(m) for m in masks], [cuda(x) for x in labels], [cuda(s) for s in scores]),
+                 pe.PanopticBatch(cuda(maps), cuda(classes), cuda(crowd)))
+    state = ocomb.SegmentationState(4, isthing, validate=False)
+    state.update(pred_maps, classes, [dict(masks=m, classes=x, scores=s) for m, x, s in zip(masks, labels, scores)],
+                 maps, classes, crowd)
+    result, expected = to_numpy(joint.compute()), state.compute()
+    compare_panoptic(result, expected)
+    compare_instance(result["instance"], expected["instance"])
