@@ -14,6 +14,7 @@ from dataclasses import dataclass, replace
 from numbers import Integral
 from typing import Sequence
 
+import numpy as np
 import torch
 
 from . import _native
@@ -86,21 +87,27 @@ class PanopticBatch:
                 raise ValueError("packed maps must be int32 H x W and classes int32 S tensors with S >= 1")
             if m.device != device or c.device != device:
                 raise ValueError("batch tensors must share a device")
-        S = max(c.numel() for c in classes)
-        B = len(maps)
-        flat = torch.cat([m.reshape(-1) for m in maps])
-        padded = torch.full((B, S), -1, dtype=torch.int32, device=device)
-        for b, c in enumerate(classes):
-            padded[b, :c.numel()] = c
-        padded_crowd = None
         if crowd is not None:
-            padded_crowd = torch.zeros((B, S), dtype=torch.bool, device=device)
-            for b, (c, k) in enumerate(zip(classes, crowd)):
+            for c, k in zip(classes, crowd):
                 if k.shape != c.shape or k.dtype != torch.bool or k.device != device:
                     raise ValueError("crowd must be bool with the same shape as classes")
-                padded_crowd[b, :k.numel()] = k
+        lengths = np.fromiter((c.numel() for c in classes), dtype=np.int64, count=len(classes))
+        S, B = int(lengths.max()), len(maps)
+        flat = torch.cat([m.reshape(-1) for m in maps])
+        # Every image's metadata lands in its padded row with one scatter:
+        # element j of image b goes to b * S + j.
+        starts = np.arange(B, dtype=np.int64) * S - (np.cumsum(lengths) - lengths)
+        index = torch.from_numpy(np.arange(int(lengths.sum()), dtype=np.int64) + np.repeat(starts, lengths))
+        index = index.to(device)
+        padded = torch.full((B * S,), -1, dtype=torch.int32, device=device)
+        padded.index_copy_(0, index, torch.cat(list(classes)))
+        padded_crowd = None
+        if crowd is not None:
+            padded_crowd = torch.zeros(B * S, dtype=torch.bool, device=device)
+            padded_crowd.index_copy_(0, index, torch.cat(list(crowd)))
+            padded_crowd = padded_crowd.view(B, S)
         sizes = tuple((int(m.shape[0]), int(m.shape[1])) for m in maps)
-        return cls(flat, padded, padded_crowd, image_sizes=sizes)
+        return cls(flat, padded.view(B, S), padded_crowd, image_sizes=sizes)
 
     @property
     def packed(self) -> bool:

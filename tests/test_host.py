@@ -71,3 +71,21 @@ def test_from_coco_keeps_rgb_maps_and_builds_the_id_tables():
     assert maps.dtype == torch.uint8 and maps.shape == (1, 2, 3, 3)
     with pytest.raises(ValueError, match="compact maps must be int32"):
         _prepare(pe.PanopticBatch(rgb, batch.classes), 2, False)
+
+
+def test_pack_pads_metadata_rows():
+    """PanopticBatch.pack (CPU tensors): maps one after another, classes and
+    crowd padded to the largest S with -1 / False, sizes kept."""
+    maps = [torch.arange(6, dtype=torch.int32).reshape(2, 3), torch.zeros((1, 4), dtype=torch.int32),
+            torch.ones((3, 1), dtype=torch.int32)]
+    classes = [torch.tensor([-1, 4], dtype=torch.int32), torch.tensor([-1, 1, 2, 3], dtype=torch.int32),
+               torch.tensor([-1], dtype=torch.int32)]
+    crowd = [torch.tensor([False, True]), torch.tensor([False, False, True, False]), torch.tensor([False])]
+    batch = pe.PanopticBatch.pack(maps, classes, crowd)
+    assert batch.image_sizes == ((2, 3), (1, 4), (3, 1))
+    assert batch.segment_map.tolist() == [0, 1, 2, 3, 4, 5, 0, 0, 0, 0, 1, 1, 1]
+    assert batch.classes.tolist() == [[-1, 4, -1, -1], [-1, 1, 2, 3], [-1, -1, -1, -1]]
+    assert batch.crowd.tolist() == [[False, True, False, False], [False, False, True, False],
+                                    [False, False, False, False]]
+    with pytest.raises(ValueError, match="crowd"):
+        pe.PanopticBatch.pack(maps, classes, [torch.tensor([1, 0], dtype=torch.int32)] * 3)
