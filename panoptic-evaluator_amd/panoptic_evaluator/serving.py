@@ -259,17 +259,23 @@ class RewardServer:
         stream.wait_stream(torch.cuda.current_stream(self.device))
         with torch.cuda.stream(stream):
             preds, targets, split = [], [], []
-            for ticket in tickets:
-                p = _images(ticket.request.prediction, crowd=False)
-                g = _images(ticket.request.target, crowd=True)
-                if [s for *_, s in p] != [s for *_, s in g]:
-                    raise ValueError("prediction and target must have matching image sizes")
-                preds += p
-                targets += g
-                split.append(len(g))
-            prediction = PanopticBatch.pack([m for m, *_ in preds], [k for _, k, *_ in preds])
-            target = PanopticBatch.pack([m for m, *_ in targets], [k for _, k, *_ in targets],
-                                        _crowd_rows(targets))
+            if len(tickets) == 1:
+                # A batch of one request is evaluated as submitted (no repacking
+                # copy); per-image results do not depend on the batch.
+                prediction, target = tickets[0].request.prediction, tickets[0].request.target
+                split.append(target.segment_map.shape[0] if not target.packed else len(target.image_sizes))
+            else:
+                for ticket in tickets:
+                    p = _images(ticket.request.prediction, crowd=False)
+                    g = _images(ticket.request.target, crowd=True)
+                    if [s for *_, s in p] != [s for *_, s in g]:
+                        raise ValueError("prediction and target must have matching image sizes")
+                    preds += p
+                    targets += g
+                    split.append(len(g))
+                prediction = PanopticBatch.pack([m for m, *_ in preds], [k for _, k, *_ in preds])
+                target = PanopticBatch.pack([m for m, *_ in targets], [k for _, k, *_ in targets],
+                                            _crowd_rows(targets))
             pq = panoptic_quality(prediction, target, self.num_classes, validate=False)
             launch.host = torch.empty(pq.shape, dtype=pq.dtype, pin_memory=True)
             launch.host.copy_(pq, non_blocking=True)
@@ -297,11 +303,20 @@ class RewardServer:
                 image_points = [self._points(request, m.shape[0]) for m in request.prediction.masks]
                 points += image_points
                 split.append(image_points)
-            target = PanopticBatch.pack([m for m, *_ in targets], [k for _, k, *_ in targets],
-                                        _crowd_rows(targets))
+            if len(tickets) == 1:
+                # One request: its own target maps, without a repacking copy.
+                target = tickets[0].request.target.compact()
+                if target.crowd is None:
+                    target = PanopticBatch(target.segment_map, target.classes,
+                                           torch.zeros_like(target.classes, dtype=torch.bool),
+                                           image_sizes=target.image_sizes)
+            else:
+                target = PanopticBatch.pack([m for m, *_ in targets], [k for _, k, *_ in targets],
+                                            _crowd_rows(targets))
             maps, gc, crowd = _prepare(target, self.num_classes, False)
-            sizes = list(target.image_sizes)
-            out = launch_prefix(masks, classes, maps, gc, crowd, sizes, True, points, self.things,
+            sizes = (list(target.image_sizes) if target.packed
+                     else [tuple(target.segment_map.shape[1:])] * target.segment_map.shape[0])
+            out = launch_prefix(masks, classes, maps, gc, crowd, sizes, target.packed, points, self.things,
                                 self.thresholds, self.num_classes, self.device)
             _, summaries, _, miou, _ = finalize_arrays(out["tp"], out["fp"], out["fn"], out["sums"],
                                                        out["confusion"], self.things, copies=False)

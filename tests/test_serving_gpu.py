@@ -92,3 +92,27 @@ def test_checkpoint_beyond_sequence_is_rejected(pe):
     server = RewardServer(133)
     with pytest.raises(ValueError, match="checkpoint"):
         server.submit(bad)
+
+
+def test_single_request_batches_packed_and_uniform(pe):
+    """A batch of one request is evaluated as submitted (packed, or a
+    uniform B x H x W batch without crowd flags); results equal the plain API."""
+    import workloads
+    from panoptic_evaluator.serving import PQRequest, PrefixRequest, RewardServer
+    isthing = [c < 80 for c in range(133)]
+    pair = workloads.config0("cuda", seed=4, B=3, H=40, W=56)
+    gen = torch.Generator(device="cuda").manual_seed(9)
+    masks = [workloads.ellipse_masks(gen, 7, 40, 56, device="cuda", max_axis=20.0) for _ in range(3)]
+    classes = [torch.randint(0, 133, (7,), device="cuda", generator=gen).int() for _ in range(3)]
+    uniform_target = pe.PanopticBatch(pair.gt, pair.gc)                  # no crowd flags
+    requests = [PQRequest(pe.PanopticBatch(pair.pred, pair.pc), pe.PanopticBatch(pair.gt, pair.gc, pair.crowd)),
+                PQRequest(pe.PanopticBatch(pair.pred, pair.pc), uniform_target),
+                PrefixRequest(pe.InstanceBatch(masks, classes), uniform_target, None),
+                PrefixRequest(pe.InstanceBatch(masks, classes), pe.PanopticBatch(pair.gt, pair.gc, pair.crowd),
+                              [0, 3, 7])]
+    requests += make_requests(pe, 8, 4)                                  # packed requests
+    server = RewardServer(133, isthing=isthing, streams=2)
+    for request in requests:
+        ticket = server.submit(request)
+        server.drain()                                                   # one request per batch
+        assert_same(ticket.result, alone(pe, request, isthing))
