@@ -278,6 +278,7 @@ class PanopticEvaluator:
         self.isthing = torch.tensor(list(isthing) if isthing is not None else [True] * num_classes,
                                     dtype=torch.bool, device=self.device)
         self._scratch = None
+        self._scratch_key = None
         self._workspace = None
         self._default_crowd = None
         self._validation_error = torch.zeros(1, dtype=torch.int32, device=self.device)
@@ -316,6 +317,9 @@ class PanopticEvaluator:
 
     def _scratch_for(self, B, G, P, packed_pixels=0):
         shape = (B, G, P)
+        if self._scratch is not None and self._scratch_key == (B, G, P, packed_pixels):
+            return self._scratch  # same layout as the previous update
+        self._scratch_key = (B, G, P, packed_pixels)
         size = _native.library().pe_update_workspace_bytes(B, G, P, self.num_classes, packed_pixels)
         if self._workspace is not None and self._workspace.numel() < size:
             self._scratch = None
