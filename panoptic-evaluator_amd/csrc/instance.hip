@@ -1876,7 +1876,7 @@ static int launch_score_order_lds(bool arrays, int64_t B, int64_t max_dets, cons
                                   int32_t* sorted_classes32, hipStream_t stream) {
   int slots = 64;
   while (slots < max_dets) slots <<= 1;
-  const int threads = std::min(1024, std::max(64, slots / 2));
+  const int threads = std::min(1024, slots);  // one element per lane and stage
   const size_t lds = 12 * static_cast<size_t>(slots);
   auto kernel = arrays ? score_order_lds<true> : score_order_lds<false>;
   if (lds > 64 * 1024)
