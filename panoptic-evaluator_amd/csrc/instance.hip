@@ -1346,6 +1346,7 @@ __device__ __forceinline__ void ap_class(const ApArgs& a, int c, OrderPtr order,
     // values per thread and step, every LDS read issued before the divisions.
     double* base = a.precision + c * 12;
     constexpr int kValues = 10 * 101 * 12, W = 8;
+    auto rr_of = [](int e) { return (min(e, kValues - 1) / 12) % 101; };
     for (int e0 = threadIdx.x; e0 < kValues; e0 += THREADS * W) {
       uint32_t counts[W];
       int reached[W];
@@ -1357,6 +1358,13 @@ __device__ __forceinline__ void ap_class(const ApArgs& a, int c, OrderPtr order,
         reached[w] = last[j];
         counts[w] = codes[rr * kApJobs + j];
       }
+      // Envelopes with no true positive are exactly 0 (0 / (fp + eps)); a
+      // wave whose values are all such skips the divisions.
+      bool divide = false;
+#pragma unroll
+      for (int w = 0; w < W; ++w) divide |= reached[w] >= -1 && rr_of(e0 + w * THREADS) <= reached[w] &&
+                                            (counts[w] >> 16) != 0;
+      const bool any_divide = __ballot(divide) != 0;
 #pragma unroll
       for (int w = 0; w < W; ++w) {
         const int e = e0 + w * THREADS;
@@ -1364,8 +1372,9 @@ __device__ __forceinline__ void ap_class(const ApArgs& a, int c, OrderPtr order,
         const int q = e % 12, tr = e / 12, rr = tr % 101, tt = tr / 101;
         const uint32_t code = rr <= reached[w] ? counts[w] : 0u;
         const i64 etp = code >> 16, efp = code & 0xFFFFu;
-        const double value = reached[w] < -1 ? -1.0
-                                             : static_cast<double>(etp) / (static_cast<double>(etp + efp) + DBL_EPSILON);
+        double value = reached[w] < -1 ? -1.0 : 0.0;
+        if (any_divide && reached[w] >= -1)
+          value = static_cast<double>(etp) / (static_cast<double>(etp + efp) + DBL_EPSILON);
         base[(static_cast<i64>(tt) * 101 + rr) * pstride + q] = value;
       }
     }
