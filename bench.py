@@ -652,7 +652,7 @@ def serve(cfg, args):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic_for(cfg.name),
             "algorithmic_bytes_per_launch": round(out["streamed"] / max(out["launches"], 1)),
             "avg_kernel_ms": round(out["kernel_ms"] / max(out["launches"], 1), 5),
             "launches": out["launches"],
