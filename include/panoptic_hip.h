@@ -101,11 +101,17 @@ int64_t pe_update_workspace_bytes(int64_t B, int64_t G, int64_t P, int64_t C, in
  *   tp, fp, fn, iou_sum   C (or B x C when reward != 0) accumulators.
  *   confusion       C x (C + 1) int64 accumulator (ignored when reward != 0).
  *   error           1 int32 scratch.
- *   iou_exact       C x 2 uint64 accumulator (dataset mode; NULL for reward):
- *                   the exact sum of matched IoUs in units of 2^-53 (every
+ *   iou_exact       C x 2 uint64 accumulator (dataset mode) or NULL: the
+ *                   exact sum of matched IoUs in units of 2^-53 (every
  *                   matched IoU is an exact multiple), as low-32-bit and
  *                   high-bit partial sums; iou_sum is rewritten as its
- *                   correctly rounded FP64 value after every update.
+ *                   correctly rounded FP64 value after every update, so the
+ *                   caller zeroes these words together with iou_sum.  NULL
+ *                   (the reference's state, which has no such words): this
+ *                   update's exact sums live in the workspace and their
+ *                   correctly rounded value is added to iou_sum once per
+ *                   class, so a caller that zeroes only its own tensors on
+ *                   reset gets the right totals.  Ignored when reward != 0.
  *   reward != 0 selects per-image statistics (panoptic_quality): empty
  *   prediction slots are ignored and no confusion is accumulated.
  * validate != 0 synchronizes the stream once to read the error bitmask.
@@ -257,7 +263,9 @@ int64_t pe_score_order_workspace_bytes(int64_t N, int64_t B);
  * index of sorted position i; scores/classes gathered into that order
  * (classes also as int32).  max_dets: the largest image's detection count
  * (at most 4096: one in-LDS sort per image; 0 or more: a segmented radix
- * sort).  Replaces the per-image
+ * sort).  max_dets must be at least every image's count: an image with more
+ * detections than the in-LDS sort was sized for is not sorted and gets
+ * order entries of -1.  Replaces the per-image
  * torch.argsort(scores, descending=True, stable=True) of instance.py:140. */
 int pe_score_order(int64_t B, int64_t N, int64_t max_dets, const int64_t* det_off, const double* scores,
                    const int64_t* classes, int64_t* order, double* sorted_scores,

@@ -7,11 +7,15 @@
 //
 // Design (MI355X-first):
 //  * mask x mask intersections: bool masks are bytes 0/1, i.e. valid int8
-//    MFMA operands as they lie in HBM.  One wave owns a 16-detection tile and
-//    up to 8 GT tiles for a K (pixel) chunk; each K step is one 16x16x64 int8
-//    MFMA per GT tile with exact int32 accumulation.  Partial tiles are added
-//    atomically (counts never exceed the pixel count).  Mask areas come from
-//    the same loads (byte sums), so every mask byte is read from HBM once.
+//    MFMA operands as they lie in HBM.  Default (staged) form: a 512-thread
+//    workgroup covers 128 detections x 64 GT masks over a pixel range; each
+//    256-pixel block of every row is staged in LDS and wave w multiplies
+//    detection tile w by every GT tile with 16x16x64 int8 MFMAs (exact int32
+//    accumulation).  Register form (tuner alternative): one wave per
+//    16-detection tile with operands loaded straight into VGPRs.  Partial
+//    tiles are added atomically (counts never exceed the pixel count).  Mask
+//    areas come from the same loads (byte sums), so every mask byte is read
+//    from HBM once.
 //  * detection x slot histogram: one workgroup owns a pixel tile of one image,
 //    keeps the target slots of the tile in registers and streams every
 //    detection mask over it with 16-byte loads (all-zero chunks skip at once),
@@ -19,11 +23,12 @@
 //  * matching: one wave per (image, class); lane L < 40 owns the chain
 //    (area range L / 10, IoU threshold L % 10); detections are processed in
 //    score order with a per-chain GT bitmask in LDS.
-//  * AP: two stable radix sorts (score descending, then class) group each
-//    class's detections in global score order; one thread per
-//    (class, area, maxDets, threshold) runs the cumulative TP/FP scan forward
-//    and the precision envelope backward, matching the reference's
-//    searchsorted/cummax result exactly.
+//  * AP: up to 4096 records, one workgroup per class gathers its records,
+//    sorts them by (score descending, input index) with a bitonic network in
+//    LDS and scans them per (threshold, area, maxDets) job; larger record sets
+//    are first grouped per class in global score order by stable sorts.  The
+//    cumulative TP/FP scan runs forward and the precision envelope backward,
+//    matching the reference's searchsorted/cummax result exactly.
 #include <hipcub/hipcub.hpp>
 
 #include <cfloat>
@@ -1837,6 +1842,13 @@ __global__ __launch_bounds__(1024) void score_order_lds(const i64* det_off, cons
   const i64 first = det_off[b];
   const int D = static_cast<int>(det_off[b + 1] - first);
   if (D <= 0) return;  // uniform
+  if (D > slots) {
+    // The caller's max_dets understated this image's count: the LDS holds
+    // only `slots` entries, so nothing is sorted and every order entry of the
+    // image is -1 (an index no caller accepts).
+    for (int e = tid; e < D; e += nthreads) order[first + e] = -1;
+    return;
+  }
   const double* s = ARRAYS ? arrays.scores[b] : scores + first;
   int m = 1;
   while (m < D) m <<= 1;

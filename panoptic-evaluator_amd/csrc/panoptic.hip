@@ -16,10 +16,13 @@
 //    nonzero bins are flushed with 64-bit global atomics.
 //  * score: one workgroup per image computes areas, matches every prediction
 //    slot against the GT slots (integer strict-half test, IEEE FP64 IoU),
-//    adds confusion directly from the pair counts, and writes per-image,
-//    class-sorted matched IoUs.  The last workgroup to finish folds iou_sum in
-//    (image, slot) order, so iou_sum is deterministic and bit-identical to the
-//    reference CPU path's in-order scatter_add_.
+//    adds confusion directly from the pair counts and accumulates the matched
+//    IoUs.  Dataset iou_sum: every matched IoU is an exact multiple of 2^-53,
+//    so the class sums are kept exactly as 2^-53 fixed point (two u64 words
+//    per class) and iou_sum is the correctly rounded FP64 of that exact total:
+//    deterministic, independent of batching and launch order, and within
+//    1e-12 relative of the reference's FP64 fold.  Per-image (reward) sums
+//    fold in prediction-slot order as the reference does.
 #include <climits>
 #include <type_traits>
 
@@ -1006,9 +1009,16 @@ __device__ __forceinline__ double exact_to_double(u64 lo, u64 hi) {
   return (static_cast<double>(hi) * 4294967296.0 + static_cast<double>(lo)) * 0x1p-53;
 }
 
+// Caller-kept exact words (iou_add == 0): iou_sum is rewritten from the
+// running exact total.  Per-update words in the workspace (iou_add != 0, the
+// caller keeps no exact words): this update's exact sum is rounded once and
+// added to the caller's iou_sum, one FP64 add per class and update, so a
+// caller that zeroes iou_sum itself (reset) starts from zero.
 __device__ void convert_iou_sum(const ScoreArgs& a) {
-  for (int c = threadIdx.x; c < a.C; c += blockDim.x)
-    a.iou_sum[c] = exact_to_double(a.iou_exact[2 * c], a.iou_exact[2 * c + 1]);
+  for (int c = threadIdx.x; c < a.C; c += blockDim.x) {
+    const double update = exact_to_double(a.iou_exact[2 * c], a.iou_exact[2 * c + 1]);
+    a.iou_sum[c] = a.iou_add ? a.iou_sum[c] + update : update;
+  }
 }
 
 __global__ __launch_bounds__(kScoreThreads) void score_images(ScoreArgs a) {
@@ -1149,6 +1159,7 @@ extern "C" int64_t pe_update_workspace_bytes(int64_t B, int64_t G, int64_t P, in
   carve.take<int>(B * P);        // rec_lab (prediction labels, large G/P only)
   carve.take<double>(B * P);     // rec_val
   if (packed_pixels > 0) carve.take<int>(tile_list_ints(B, packed_pixels, 4096));  // chunk list
+  carve.take<u64>(2 * C);        // per-update exact IoU words (iou_exact == NULL)
   return static_cast<int64_t>(align_up(carve.used, 256));
 }
 
@@ -1174,7 +1185,7 @@ extern "C" int pe_update(const void* gt, int gt_bytes, const void* pred, int pre
   const i64 packed_pixels = pix_off ? H * W : 0;
   PE_REQUIRE(workspace_bytes >= pe_update_workspace_bytes(B, G, P, C, packed_pixels), "workspace too small");
   PE_REQUIRE(!validate || error, "validation needs the error word");
-  PE_REQUIRE(reward || (iou_exact && confusion), "dataset mode needs the confusion and exact IoU accumulators");
+  PE_REQUIRE(reward || confusion, "dataset mode needs the confusion accumulator");
   hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
   if (B == 0) return kOk;
   Carve carve(workspace);
@@ -1217,6 +1228,14 @@ extern "C" int pe_update(const void* gt, int gt_bytes, const void* pred, int pre
   h.pred_slots = pred_slots;
   h.error = validate ? error : nullptr;
   int* tile_ws = pix_off ? carve.take<int>(tile_list_ints(B, packed_pixels, 4096)) : nullptr;
+  u64* update_exact = carve.take<u64>(2 * C);
+  if (!reward && !iou_exact) {
+    // No caller-kept exact words: accumulate this update's exact sums in the
+    // workspace and add their rounded value to iou_sum.
+    s.iou_exact = update_exact;
+    s.iou_add = 1;
+    if ((rc = launch_zero(update_exact, 16 * static_cast<size_t>(C), stream))) return rc;
+  }
   if ((rc = launch_joint_histogram(h, gt_bytes, pred_bytes, 0, tile_ws, stream))) return rc;
   if (validate) {
     if ((rc = launch_prepare(s, error, stream))) return rc;

@@ -50,6 +50,7 @@ struct ScoreArgs {
   int* rec_lab;
   double* rec_val;
   u64* iou_exact;        // C x 2: exact 2^-53 fixed-point IoU sums (dataset mode)
+  int iou_add;           // 1: iou_exact holds this update only; add it to iou_sum
   const int* guard;
   int B, G, P, C;
   int reward;

@@ -8,6 +8,7 @@ the call fails loudly.
 from __future__ import annotations
 
 import collections
+import contextlib
 import ctypes
 import pathlib
 import threading
@@ -84,6 +85,28 @@ def library():
 
 _tables: "collections.OrderedDict[tuple, tuple[torch.Tensor, int, set]]" = collections.OrderedDict()
 _TABLE_CACHE = 256
+_pins: "list | None" = None   # tables handed out while a capture is recorded
+
+
+@contextlib.contextmanager
+def pinned_tables():
+    """Collect every table device_table hands out inside the block (cache
+    hits included) in the yielded list.  A HIP graph captured inside the
+    block bakes those tables' addresses into its kernels, so its owner keeps
+    the list for the graph's lifetime: the cache may evict the entries, but
+    the caching allocator cannot reuse their memory while the list holds
+    them."""
+    global _pins
+    with _lock:
+        if _pins is not None:
+            raise RuntimeError("pinned_tables blocks do not nest")
+        _pins = []
+        pins = _pins
+    try:
+        yield pins
+    finally:
+        with _lock:
+            _pins = None
 
 
 def device_table(values, device) -> torch.Tensor:
@@ -108,10 +131,14 @@ def device_table(values, device) -> torch.Tensor:
             _tables.move_to_end(key)
             if current != entry[1]:
                 entry[2].add(current)
+            if _pins is not None:
+                _pins.append(entry[0])
             return entry[0]
     table = torch.tensor(key[1], dtype=torch.int64).to(device)
     evicted = []
     with _lock:
+        if _pins is not None:
+            _pins.append(table)
         _tables[key] = (table, current, set())
         while len(_tables) > _TABLE_CACHE:
             evicted.append(_tables.popitem(last=False)[1])

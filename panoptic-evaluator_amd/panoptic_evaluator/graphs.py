@@ -18,6 +18,8 @@ from __future__ import annotations
 
 import torch
 
+from . import _native
+
 
 def _parts(evaluator):
     """(panoptic evaluator or None, instance evaluator or None)."""
@@ -62,8 +64,11 @@ class CapturedUpdate:
             # them (a reset() in between makes the copy unnecessary).
             instance._commit = self._keep
         try:
-            with torch.cuda.graph(self.graph):
+            # The graph's kernels read the device metadata tables by address:
+            # keep every table the captured update used for the graph's life.
+            with _native.pinned_tables() as tables, torch.cuda.graph(self.graph):
                 evaluator.update(*args, **kwargs)
+            self._tables = tables
         finally:
             if instance is not None:
                 del instance._commit  # back to the class's method

@@ -70,6 +70,7 @@ class Ticket:
     launched: float | None = None
     completed: float | None = None
     result: object = None
+    error: BaseException | None = None   # set instead of result when the launch failed
 
     @property
     def done(self) -> bool:
@@ -90,6 +91,7 @@ class Admission:
         self.budget = int(budget)
         self.in_use = 0
         self.running = 0
+        self.blocked = 0   # admission passes at which the head request waited for memory
         self.waiting: collections.deque = collections.deque()
 
     def offer(self, ticket: Ticket):
@@ -101,6 +103,7 @@ class Admission:
             ticket = self.waiting[0]
             fits = self.in_use + ticket.footprint <= self.budget
             if not fits and not (self.running == 0 and not admitted):
+                self.blocked += 1
                 break
             self.waiting.popleft()
             self.in_use += ticket.footprint
@@ -233,11 +236,26 @@ class RewardServer:
             pq = [t for t in self._pending if isinstance(t.request, PQRequest)]
             prefix = [t for t in self._pending if isinstance(t.request, PrefixRequest)]
             self._pending = []
-            if pq:
-                self._launch_pq(pq)
-            if prefix:
-                self._launch_prefix(prefix)
+            done = 0
+            for tickets, launch in ((pq, self._launch_pq), (prefix, self._launch_prefix)):
+                if not tickets:
+                    continue
+                try:
+                    launch(tickets)
+                except Exception as error:  # noqa: BLE001 - reported on the tickets
+                    done += self._fail(tickets, error)
+            return done + self._complete()
         return self._complete()
+
+    def _fail(self, tickets, error) -> int:
+        """A launch that raised: its tickets complete with the error and give
+        their memory back, and the server keeps serving."""
+        now = time.perf_counter()
+        for ticket in tickets:
+            ticket.error = error
+            ticket.completed = now
+            self.admission.release(ticket)
+        return len(tickets)
 
     def drain(self):
         """Poll until every submitted request has completed."""

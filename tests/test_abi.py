@@ -54,3 +54,35 @@ def test_workspace_queries_are_host_only():
     det.restype = ctypes.c_int64
     det.argtypes = [ctypes.c_int64] * 2
     assert det(16, 0) == 0 and det(16, 16 * 800 * 1088) >= 4 * (16 * 800 * 1088 // 4096)
+
+
+REFERENCE_BINDINGS = ("update", "prefix", "compute", "pack", "instance_update", "instance_compute",
+                      "instance_histogram", "instance_match_histogram")  # bindings.cpp:29-56
+
+
+def test_shim_module_has_the_reference_bindings():
+    import inspect
+
+    from panoptic_evaluator import _cuda_evaluator
+    for name in REFERENCE_BINDINGS:
+        assert callable(getattr(_cuda_evaluator, name)), name
+    # Argument order of the reference's update binding (bindings.cpp:30-49).
+    params = list(inspect.signature(_cuda_evaluator.update).parameters)
+    assert params == ["gt", "pred", "gc", "pc", "crowd", "pairs", "areas", "matched", "tp", "fp", "fn", "iou_sum",
+                      "confusion", "gt_ids", "gt_slots", "pred_ids", "pred_slots", "error", "validate", "reward"]
+    assert inspect.signature(_cuda_evaluator.update).parameters["reward"].default is False
+
+
+@pytest.mark.skipif(not LIB.exists(), reason="library not built")
+def test_shim_rejects_host_tensors_like_torch_check():
+    import torch
+
+    from panoptic_evaluator import _cuda_evaluator
+    maps = torch.zeros((1, 4, 4), dtype=torch.int32)
+    ids = torch.zeros((1, 2), dtype=torch.int64)
+    with pytest.raises(RuntimeError):
+        _cuda_evaluator.pack(maps, ids, ids.int())
+    with pytest.raises(RuntimeError):
+        _cuda_evaluator.compute(*(torch.zeros(3, dtype=torch.int64) for _ in range(3)),
+                                torch.zeros(3, dtype=torch.float64), torch.zeros((3, 4), dtype=torch.int64),
+                                torch.zeros(3, dtype=torch.bool))
