@@ -292,3 +292,36 @@ def test_combined_fork_equals_sequential_at_c1_shape(pe):
     compare_panoptic(got, want, exact_iou_sum=True)
     for key in ("precision", "recall"):
         assert_exact(got["instance"][key], want["instance"][key], key)
+
+
+def test_captured_update_survives_table_cache_turnover(pe):
+    """A captured update reads device metadata tables by address.  After the
+    cache has handed out and evicted more than three times its capacity of new
+    tables (whose memory the allocator reuses), replays still equal eager
+    updates: the graph keeps its own tables."""
+    from panoptic_evaluator import _native
+    from panoptic_evaluator.graphs import CapturedUpdate
+    batch = uneven(31, 5, detections=(1, 9))
+    isthing = [c < 80 for c in range(133)]
+    prediction, target = packed(pe, batch)
+    dets = pe.InstanceBatch([m.cuda() for m in batch.masks], [c.cuda() for c in batch.classes],
+                            [s.cuda() for s in batch.scores])
+    eager = pe.SegmentationEvaluator(133, isthing=isthing, validate=False)
+    eager.update(prediction, dets, target)
+    want = to_numpy(eager.compute())
+    replayed = pe.SegmentationEvaluator(133, isthing=isthing, validate=False)
+    captured = CapturedUpdate(replayed, prediction, dets, target)
+    churn = []
+    for i in range(3 * _native._TABLE_CACHE + 7):
+        churn.append(torch.full((4096,), -1, dtype=torch.int64, device="cuda"))  # reuse freed blocks
+        _native.device_table([10 ** 9 + i, i, 2 * i, 5], "cuda")
+        if len(churn) > 8:
+            churn.pop(0)
+    torch.cuda.synchronize()
+    for _ in range(2):
+        replayed.reset()
+        captured.replay()
+        got = to_numpy(replayed.compute())
+        compare_panoptic(got, want, exact_iou_sum=True)
+        for key in ("precision", "recall"):
+            assert_exact(got["instance"][key], want["instance"][key], key)
