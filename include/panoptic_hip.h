@@ -128,6 +128,29 @@ int pe_update(const void* gt, int gt_bytes, const void* pred, int pred_bytes,
               int32_t* error, uint64_t* iou_exact, void* workspace, int64_t workspace_bytes,
               int validate, int reward, int pairs_zeroed, void* stream);
 
+/* pe_update without a workspace argument and without any allocation.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, behaviour and results as pe_update,
+ * minus workspace/workspace_bytes: the workspace is a static device array of
+ * pe_update_noalloc_max_workspace_bytes() bytes (8 MiB) per GPU, held until
+ * the process ends; batches whose pe_update_workspace_bytes exceed it are
+ * rejected (kWorkspaceTooSmall, < 0) before anything runs.  Call it with the
+ * stream's device current.  All calls on one GPU share that array: calls
+ * that overlap on one GPU (on different streams or host threads) race, so
+ * serialise them. */
+int64_t pe_update_noalloc_max_workspace_bytes(void);
+int pe_update_noalloc(const void* gt, int gt_bytes, const void* pred, int pred_bytes,
+                      int64_t B, int64_t H, int64_t W, const int64_t* pix_off,
+                      const int32_t* gc, const int32_t* pc, const uint8_t* crowd,
+                      int64_t G, int64_t P, int64_t C,
+                      int64_t* pairs, int64_t* areas, int32_t* matched,
+                      int64_t* tp, int64_t* fp, int64_t* fn, double* iou_sum, int64_t* confusion,
+                      const int64_t* gt_ids, const int32_t* gt_slots,
+                      const int64_t* pred_ids, const int32_t* pred_slots,
+                      int32_t* error, uint64_t* iou_exact, int validate, int reward, int pairs_zeroed,
+                      void* stream);
+
 /* FP64 finaliser over K independent accumulator rows.  Replaces ``compute``
  * (bindings.cpp:10-11; compute_cuda at compute_kernel.cu:66-95; K > 1 is
  * compute_prefix_cuda, compute_kernel.cu:98-118).

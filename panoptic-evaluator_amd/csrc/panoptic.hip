@@ -24,6 +24,7 @@
 //    1e-12 relative of the reference's FP64 fold.  Per-image (reward) sums
 //    fold in prediction-slot order as the reference does.
 #include <climits>
+#include <mutex>
 #include <type_traits>
 
 #include "common.h"
@@ -1163,15 +1164,14 @@ extern "C" int64_t pe_update_workspace_bytes(int64_t B, int64_t G, int64_t P, in
   return static_cast<int64_t>(align_up(carve.used, 256));
 }
 
-extern "C" int pe_update(const void* gt, int gt_bytes, const void* pred, int pred_bytes, int64_t B,
-                         int64_t H, int64_t W, const int64_t* pix_off, const int32_t* gc, const int32_t* pc,
-                         const uint8_t* crowd,
-                         int64_t G, int64_t P, int64_t C, int64_t* pairs, int64_t* areas, int32_t* matched,
-                         int64_t* tp, int64_t* fp, int64_t* fn, double* iou_sum, int64_t* confusion,
-                         const int64_t* gt_ids, const int32_t* gt_slots, const int64_t* pred_ids,
-                         const int32_t* pred_slots, int32_t* error, uint64_t* iou_exact, void* workspace,
-                         int64_t workspace_bytes, int validate, int reward, int pairs_zeroed,
-                         void* stream_ptr) {
+namespace pe {
+int update_impl(const void* gt, int gt_bytes, const void* pred, int pred_bytes, int64_t B, int64_t H, int64_t W,
+                const int64_t* pix_off, const int32_t* gc, const int32_t* pc, const uint8_t* crowd, int64_t G,
+                int64_t P, int64_t C, int64_t* pairs, int64_t* areas, int32_t* matched, int64_t* tp, int64_t* fp,
+                int64_t* fn, double* iou_sum, int64_t* confusion, const int64_t* gt_ids, const int32_t* gt_slots,
+                const int64_t* pred_ids, const int32_t* pred_slots, int32_t* error, uint64_t* iou_exact,
+                void* workspace, int64_t workspace_bytes, int validate, int reward, int pairs_zeroed,
+                void* stream_ptr) {
   PE_REQUIRE(B >= 0 && H >= 0 && W >= 0, "negative shape");
   PE_REQUIRE(G > 0 && G < INT_MAX && P > 0 && P < INT_MAX && C > 0 && C < INT_MAX, "invalid capacities");
   PE_REQUIRE(B < INT_MAX && G * P < (1LL << 32), "pair histogram too large");
@@ -1250,6 +1250,81 @@ extern "C" int pe_update(const void* gt, int gt_bytes, const void* pred, int pre
     s.areas_ready = 1;
   }
   return launch_score(s, stream);
+}
+}  // namespace pe
+
+extern "C" int pe_update(const void* gt, int gt_bytes, const void* pred, int pred_bytes, int64_t B,
+                         int64_t H, int64_t W, const int64_t* pix_off, const int32_t* gc, const int32_t* pc,
+                         const uint8_t* crowd,
+                         int64_t G, int64_t P, int64_t C, int64_t* pairs, int64_t* areas, int32_t* matched,
+                         int64_t* tp, int64_t* fp, int64_t* fn, double* iou_sum, int64_t* confusion,
+                         const int64_t* gt_ids, const int32_t* gt_slots, const int64_t* pred_ids,
+                         const int32_t* pred_slots, int32_t* error, uint64_t* iou_exact, void* workspace,
+                         int64_t workspace_bytes, int validate, int reward, int pairs_zeroed,
+                         void* stream_ptr) {
+  return update_impl(gt, gt_bytes, pred, pred_bytes, B, H, W, pix_off, gc, pc, crowd, G, P, C, pairs, areas,
+                     matched, tp, fp, fn, iou_sum, confusion, gt_ids, gt_slots, pred_ids, pred_slots, error,
+                     iou_exact, workspace, workspace_bytes, validate, reward, pairs_zeroed, stream_ptr);
+}
+
+// ------------------------------------------------ update without allocation
+//
+// pe_update_noalloc's workspace: one array per GPU, held until the process
+// ends.  Host code never passes the array's name to a kernel (on the host it
+// is only a stand-in symbol); its device address is looked up once per
+// device with hipGetSymbolAddress and used everywhere instead.
+constexpr int64_t kStaticScratchBytes = 8LL << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex scratch_mutex;
+void* scratch_address[kMaxScratchDevices] = {};
+
+// Device address of the current device's scratch array (looked up on the
+// first call on that device, with that device current).
+int current_scratch(void** out) {
+  int device = -1;
+  PE_HIP(hipGetDevice(&device));
+  PE_REQUIRE(device >= 0 && device < kMaxScratchDevices, "device index %d outside the scratch table", device);
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (!scratch_address[device]) {
+    void* address = nullptr;
+    PE_HIP(hipGetSymbolAddress(&address, HIP_SYMBOL(scratch)));
+    PE_REQUIRE(address != nullptr, "no device address for the scratch array");
+    scratch_address[device] = address;
+  }
+  *out = scratch_address[device];
+  return kOk;
+}
+}  // namespace
+
+extern "C" int64_t pe_update_noalloc_max_workspace_bytes(void) { return kStaticScratchBytes; }
+
+extern "C" int pe_update_noalloc(const void* gt, int gt_bytes, const void* pred, int pred_bytes, int64_t B,
+                                 int64_t H, int64_t W, const int64_t* pix_off, const int32_t* gc, const int32_t* pc,
+                                 const uint8_t* crowd, int64_t G, int64_t P, int64_t C, int64_t* pairs,
+                                 int64_t* areas, int32_t* matched, int64_t* tp, int64_t* fp, int64_t* fn,
+                                 double* iou_sum, int64_t* confusion, const int64_t* gt_ids,
+                                 const int32_t* gt_slots, const int64_t* pred_ids, const int32_t* pred_slots,
+                                 int32_t* error, uint64_t* iou_exact, int validate, int reward, int pairs_zeroed,
+                                 void* stream_ptr) {
+  PE_REQUIRE(B >= 0 && H >= 0 && W >= 0 && G > 0 && P > 0 && C > 0 && C < INT_MAX, "invalid shape");
+  const int64_t needed = pe_update_workspace_bytes(B, G, P, C, pix_off ? H * W : 0);
+  if (needed > kStaticScratchBytes)
+    return fail(kWorkspaceTooSmall,
+                "pe_update_noalloc: this batch needs %lld workspace bytes, above the static scratch maximum of "
+                "%lld bytes; use pe_update with a caller workspace",
+                static_cast<long long>(needed), static_cast<long long>(kStaticScratchBytes));
+  void* workspace = nullptr;
+  const int rc = current_scratch(&workspace);
+  if (rc) return rc;
+  // Nothing in the workspace needs to start at zero: the per-update exact IoU
+  // words are cleared by update_impl, the other regions are written before
+  // they are read.
+  return update_impl(gt, gt_bytes, pred, pred_bytes, B, H, W, pix_off, gc, pc, crowd, G, P, C, pairs, areas,
+                     matched, tp, fp, fn, iou_sum, confusion, gt_ids, gt_slots, pred_ids, pred_slots, error,
+                     iou_exact, workspace, kStaticScratchBytes, validate, reward, pairs_zeroed, stream_ptr);
 }
 
 namespace pe {
