@@ -250,7 +250,7 @@ class InstanceConfig:
             gt_classes.append(gc[index.long()].long())
             gt_crowd.append(pair.crowd[b][index.long()])
         self.masks, self.gt_masks = masks, gt_masks
-        self.evaluator = pe.InstanceEvaluator(133, validate=False)
+        self.evaluator = pe.InstanceEvaluator(133, device="cuda", validate=False)
         self.prediction = pe.InstanceBatch(masks, classes, scores)
         self.target = pe.InstanceBatch(gt_masks, gt_classes, crowd=gt_crowd)
         B, H, W = pair.gt.shape

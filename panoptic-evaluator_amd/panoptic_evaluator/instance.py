@@ -242,7 +242,9 @@ class InstanceEvaluator:
     """COCO mask AP/AR with MFMA intersections and on-device greedy matching
     (instance.py:46-227).  Only scores, matches and counts are retained."""
 
-    def __init__(self, num_classes: int, *, device: str | torch.device = "cuda", validate: bool = True):
+    def __init__(self, num_classes: int, *, device: str | torch.device = "cpu", validate: bool = True):
+        """The reference's signature, default device included (instance.py:53-58);
+        this build evaluates on AMD GPUs only, so pass device="cuda"."""
         if not isinstance(num_classes, int) or isinstance(num_classes, bool) or num_classes <= 0:
             raise ValueError("num_classes must be a positive integer")
         self.num_classes = num_classes

@@ -127,7 +127,7 @@ def test_instance_evaluator_packed_masks(pe, packed_side):
         pred_masks = [pe.pack_masks(m) for m in pred_masks]
     if packed_side in ("both", "target"):
         gt_masks = [pe.pack_masks(m) for m in gt_masks]
-    ev = pe.InstanceEvaluator(3)
+    ev = pe.InstanceEvaluator(3, device="cuda")
     ev.update(pe.InstanceBatch(pred_masks, [cuda(c) for c in pcl], [cuda(s) for s in sc]),
               pe.InstanceBatch(gt_masks, [cuda(c) for c in gcl], crowd=[cuda(c) for c in cr]))
     state = oinst.InstanceState(3)

@@ -89,3 +89,15 @@ def test_pack_pads_metadata_rows():
                                     [False, False, False, False]]
     with pytest.raises(ValueError, match="crowd"):
         pe.PanopticBatch.pack(maps, classes, [torch.tensor([1, 0], dtype=torch.int32)] * 3)
+
+
+def test_evaluator_signatures_follow_the_reference():
+    """Default devices of the reference (evaluator.py:254, instance.py:57,
+    combined.py:30); the CPU default of InstanceEvaluator is refused with the
+    build's GPU-only error."""
+    import inspect
+    assert inspect.signature(pe.PanopticEvaluator).parameters["device"].default == "cuda"
+    assert inspect.signature(pe.SegmentationEvaluator).parameters["device"].default == "cuda"
+    assert inspect.signature(pe.InstanceEvaluator).parameters["device"].default == "cpu"
+    with pytest.raises(ValueError, match="AMD GPUs"):
+        pe.InstanceEvaluator(3)

@@ -96,7 +96,7 @@ def test_golden_coco_matcher(pe):
 
 def test_golden_instance_evaluator(pe):
     data = load("instance_eval")
-    ev = pe.InstanceEvaluator(4)
+    ev = pe.InstanceEvaluator(4, device="cuda")
     for image in range(int(data["num_images"])):
         inp = group(data, f"in{image}")
         ev.update(pe.InstanceBatch([cuda(inp["pm"])], [cuda(inp["pc"])], [cuda(inp["scores"])]),
@@ -143,7 +143,7 @@ def instance_batch(pe, scores=None, crowd=None):
 
 
 def test_instance_known_answers(pe):  # test_instance.py:27-69
-    ev = pe.InstanceEvaluator(2)
+    ev = pe.InstanceEvaluator(2, device="cuda")
     ev.update(instance_batch(pe, [0.9, 0.8]), instance_batch(pe))
     result = ev.compute()
     assert abs(result["ap"].item() - 1.0) < 1e-12 and result["precision"].shape == (10, 101, 2, 4, 3)
@@ -152,20 +152,20 @@ def test_instance_known_answers(pe):  # test_instance.py:27-69
     assert ev.compute()["ap"].item() == -1.0 and abs(result["ap"].item() - 1.0) < 1e-12
     empty = pe.InstanceBatch([torch.empty((0, 16, 16), dtype=torch.bool, device="cuda")],
                              [torch.empty(0, dtype=torch.int64, device="cuda")], [torch.empty(0, device="cuda")])
-    ev = pe.InstanceEvaluator(1)
+    ev = pe.InstanceEvaluator(1, device="cuda")
     ev.update(empty, instance_batch(pe))
     assert ev.compute()["ap"].item() == 0.0
-    ev = pe.InstanceEvaluator(1)
+    ev = pe.InstanceEvaluator(1, device="cuda")
     ev.update(instance_batch(pe, [0.9, 0.8]), instance_batch(pe, crowd=[True, False]))
     ev.update(instance_batch(pe, [0.9, 0.8]), instance_batch(pe))
     assert abs(ev.compute()["ap"].item() - 1.0) < 1e-12
-    ev = pe.InstanceEvaluator(1)
+    ev = pe.InstanceEvaluator(1, device="cuda")
     with pytest.raises(ValueError):
         ev.update(instance_batch(pe), instance_batch(pe))
     assert ev.compute()["ap"].item() == -1.0
     target = instance_batch(pe)
     target = pe.InstanceBatch([target.masks[0][:1]], [target.classes[0][:1]])
-    ev = pe.InstanceEvaluator(1)
+    ev = pe.InstanceEvaluator(1, device="cuda")
     ev.update(instance_batch(pe, [0.8, 0.9]), target)
     assert abs(ev.compute()["ap"].item() - 0.5) < 1e-12
 
@@ -174,7 +174,7 @@ def test_max_detections_per_category(pe):  # test_instance.py:179-197
     masks = torch.ones((106, 1, 1), dtype=torch.bool, device="cuda")
     labels = torch.zeros(106, dtype=torch.int64, device="cuda")
     labels[-1] = 1
-    ev = pe.InstanceEvaluator(2)
+    ev = pe.InstanceEvaluator(2, device="cuda")
     ev.update(pe.InstanceBatch([masks], [labels], [torch.ones(106, device="cuda")]), pe.InstanceBatch([masks], [labels]))
     r = ev.compute()
     assert abs(r["ar1"].item() - (1 / 105 + 1) / 2) < 1e-12
@@ -187,7 +187,7 @@ def test_exact_iou_threshold_and_empty_target(pe):  # test_instance.py:199-218
     gm = torch.tensor([[[True, True]]], device="cuda")
     label = torch.tensor([0], device="cuda")
     pred = pe.InstanceBatch([pm], [label], [torch.tensor([0.9], device="cuda")])
-    ev = pe.InstanceEvaluator(1)
+    ev = pe.InstanceEvaluator(1, device="cuda")
     ev.update(pred, pe.InstanceBatch([gm], [label]))
     r = ev.compute()
     assert abs(r["ap50"].item() - 1.0) < 1e-12 and abs(r["ap"].item() - 0.1) < 1e-12
@@ -204,7 +204,7 @@ def test_nondefault_stream_snapshots_and_validation(pe):  # test_instance.py:220
     with torch.cuda.stream(stream):
         prediction = instance_batch(pe, [0.9, 0.8])
         target = instance_batch(pe)
-        ev = pe.InstanceEvaluator(1, validate=False)
+        ev = pe.InstanceEvaluator(1, device="cuda", validate=False)
         ev.update(prediction, target)
         prediction.masks[0].zero_()
         prediction.scores[0].zero_()
@@ -212,7 +212,7 @@ def test_nondefault_stream_snapshots_and_validation(pe):  # test_instance.py:220
         ev.reset()
     stream.synchronize()
     assert abs(result["ap"].item() - 1.0) < 1e-12
-    ev = pe.InstanceEvaluator(1)
+    ev = pe.InstanceEvaluator(1, device="cuda")
     with pytest.raises(ValueError):
         ev.update(prediction, target)
     assert ev.compute()["ap"].item() == -1.0
@@ -223,7 +223,7 @@ def test_randomized_instance_parity(pe):
     edges (the COCOeval parity inputs of test_instance.py:80-177) against the
     oracle, over several updates."""
     rng = np.random.default_rng(47)
-    ev = pe.InstanceEvaluator(4)
+    ev = pe.InstanceEvaluator(4, device="cuda")
     state = oinst.InstanceState(4)
     for _ in range(12):
         h, w = 24, 37
@@ -413,7 +413,7 @@ def test_instance_batch_of_uneven_images(pe):
         pcl.append(rng.integers(0, 3, d))
         sc.append(rng.choice([0.2, 0.6, 0.9], d))
         cr.append(rng.random(g) < 0.2)
-    ev = pe.InstanceEvaluator(3)
+    ev = pe.InstanceEvaluator(3, device="cuda")
     ev.update(pe.InstanceBatch([cuda(m) for m in pm], [cuda(c) for c in pcl], [cuda(s) for s in sc]),
               pe.InstanceBatch([cuda(m) for m in gm], [cuda(c) for c in gcl], crowd=[cuda(c) for c in cr]))
     state = oinst.InstanceState(3)
@@ -433,7 +433,7 @@ def test_more_than_100_detections_of_one_class(pe):
     pcl = np.where(np.arange(d) < 120, 1, 0)     # 120 detections of class 1, 10 of class 0
     gcl = rng.integers(0, 2, g)
     sc = rng.random(d)
-    ev = pe.InstanceEvaluator(2)
+    ev = pe.InstanceEvaluator(2, device="cuda")
     ev.update(pe.InstanceBatch([cuda(pm)], [cuda(pcl)], [cuda(sc)]),
               pe.InstanceBatch([cuda(gm)], [cuda(gcl)]))
     state = oinst.InstanceState(2)
@@ -465,11 +465,11 @@ def test_score_order_input_dtypes(pe):
                                [cuda(s) for s in sc]),
     }
     for name, (classes, scores) in forms.items():
-        ev = pe.InstanceEvaluator(3)
+        ev = pe.InstanceEvaluator(3, device="cuda")
         ev.update(pe.InstanceBatch([cuda(m) for m in pm], classes, scores), targets)
         compare_instance(to_numpy(ev.compute()), expected)
         # The same images one update at a time (single-image batches).
-        ev = pe.InstanceEvaluator(3)
+        ev = pe.InstanceEvaluator(3, device="cuda")
         for b in range(len(pm)):
             ev.update(pe.InstanceBatch([cuda(pm[b])], [classes[b]], [scores[b]]),
                       pe.InstanceBatch([targets.masks[b]], [targets.classes[b]]))
@@ -490,7 +490,7 @@ def test_matcher_with_many_pairs_of_one_class(pe):
     gcl = [np.zeros(20, int), rng.integers(0, 2, 4)]
     sc = [rng.random(40), rng.random(6)]
     cr = [rng.random(20) < 0.1, np.zeros(4, bool)]
-    ev = pe.InstanceEvaluator(2)
+    ev = pe.InstanceEvaluator(2, device="cuda")
     ev.update(pe.InstanceBatch([cuda(m) for m in pm], [cuda(c) for c in pcl], [cuda(s) for s in sc]),
               pe.InstanceBatch([cuda(m) for m in gm], [cuda(c) for c in gcl], crowd=[cuda(c) for c in cr]))
     state = oinst.InstanceState(2)
@@ -510,7 +510,7 @@ def test_score_order_beyond_the_lds_sort(pe):
     pcl = [rng.integers(0, 2, d) for d in dets]
     gcl = [rng.integers(0, 2, 3) for _ in dets]
     sc = [rng.integers(0, 300, d) / 300.0 for d in dets]  # many ties
-    ev = pe.InstanceEvaluator(2)
+    ev = pe.InstanceEvaluator(2, device="cuda")
     ev.update(pe.InstanceBatch([cuda(m) for m in pm], [cuda(c) for c in pcl], [cuda(s) for s in sc]),
               pe.InstanceBatch([cuda(m) for m in gm], [cuda(c) for c in gcl]))
     state = oinst.InstanceState(2)
