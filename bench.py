@@ -19,6 +19,11 @@ kernel and launch shape as in the graph) and reported against the MI355X HBM
 roofline; the CPU oracle is timed on a bounded sample of the same workload on
 the host as ``cpu_baseline``.
 
+The default run measures c1 (the value) and then, in the same process, the
+rest of the metric as sub-objects of the same line: "c2" (the uneven 256-image
+batch, batched vs the plain per-image loop) and "c4" (the reward server's
+throughput and p50/p99 latency).  ``--extras none`` skips them.
+
 Other modes: --copy-calibration runs a plain copy of known size (the reference
 point that scales rocprofv3 FETCH_SIZE/WRITE_SIZE into bytes).
 """
@@ -424,7 +429,8 @@ class ServingConfig:
         latencies = np.array([t.latency for t in tickets]) * 1e3
         images = sum(t.images for t in tickets)
         return dict(images=images, elapsed=elapsed, latencies=latencies, kernel_ms=kernel_ms, launches=launches,
-                    streamed=server.prefix_pixels_streamed, batches=dict(server.launches), tickets=tickets)
+                    streamed=server.prefix_pixels_streamed, batches=dict(server.launches), tickets=tickets,
+                    admission_waits=server.admission.blocked)
 
     def cpu_sample(self, budget_s):
         from oracle import panoptic as opan
@@ -609,42 +615,50 @@ def timed(fn, steps):
     return time.perf_counter() - t0
 
 
-def serve(cfg, args):
-    """c4: one JSON line with throughput and p50/p99 request latency."""
+def serve(cfg, args, *, requests=None, rate=None, cpu_budget=None):
+    """c4: throughput and p50/p99 request latency of the serving loop as a
+    bench-line dictionary."""
     import numpy as np
+    requests = requests or args.requests
+    rate = rate or args.rate
+    cpu_budget = args.cpu_budget if cpu_budget is None else cpu_budget
     log(f"[bench] warm-up: {min(len(cfg.pool), 8)} requests")
     cfg.run(rate=1e6, requests=min(len(cfg.pool), 8), budget=args.hbm_budget, streams=args.streams)
-    log(f"[bench] serving {args.requests} requests at {args.rate} req/s offered")
-    out = cfg.run(rate=args.rate, requests=args.requests, budget=args.hbm_budget, streams=args.streams)
+    log(f"[bench] serving {requests} requests at {rate} req/s offered")
+    out = cfg.run(rate=rate, requests=requests, budget=args.hbm_budget, streams=args.streams)
     lat = out["latencies"]
     achieved = out["streamed"] / (out["kernel_ms"] * 1e-3) / 1e9 if out["launches"] else None
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU oracle sample")
-        rate, sample = cfg.cpu_sample(args.cpu_budget)
-        cpu = {"value": round(rate, 4), "unit": "images/sec", "cores": 1, "kind": "port",
+        cpu_rate, sample = cfg.cpu_sample(cpu_budget)
+        cpu = {"value": round(cpu_rate, 4), "unit": "images/sec", "cores": 1, "kind": "port",
                "sample": sample, "host_threads_available": os.cpu_count()}
-    line = {
+    batches = out["batches"]
+    launches = sum(batches.values())
+    return {
         "metric": METRIC,
         "value": round(out["images"] / out["elapsed"], 3),
         "unit": "images/sec",
         "n_gpus": 1,
-        "steps": args.requests,
+        "steps": requests,
         "warmup": min(len(cfg.pool), 8),
-        "ms_per_step": round(out["elapsed"] * 1e3 / args.requests, 4),
+        "ms_per_step": round(out["elapsed"] * 1e3 / requests, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": cfg.dtype,
         "data": "synthetic",
-        "config": dict(cfg.config, offered_load_rps=args.rate, requests=args.requests,
+        "config": dict(cfg.config, offered_load_rps=rate, requests=requests,
                        hbm_budget_bytes=args.hbm_budget, streams=args.streams,
                        form="serving: continuous batching"),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3),
                        "mean": round(float(lat.mean()), 3), "max": round(float(lat.max()), 3)},
-        "throughput_rps": round(args.requests / out["elapsed"], 3),
-        "batches": out["batches"],
+        "throughput_rps": round(requests / out["elapsed"], 3),
+        "batches": batches,
+        "requests_per_launch": round(requests / max(launches, 1), 3),
+        "admission_waits": out.get("admission_waits"),
         "roofline": with_ceiling({
             "bound": "hbm",
             "kernel": cfg.dominant,
@@ -659,45 +673,13 @@ def serve(cfg, args):
         }),
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
-    return 0
 
 
-def main(argv=None):
-    parser = argparse.ArgumentParser(description=__doc__)
-    parser.add_argument("--gpus", type=int, default=1)
-    parser.add_argument("--steps", type=int, default=20)
-    parser.add_argument("--warmup", type=int, default=3)
-    parser.add_argument("--config", default=DEFAULT_CONFIG)
-    parser.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU oracle sample")
-    parser.add_argument("--no-cpu-baseline", action="store_true")
-    parser.add_argument("--copy-calibration", action="store_true")
-    parser.add_argument("--no-autotune", action="store_true",
-                        help="keep the default launch shapes instead of tuning them on this workload")
-    parser.add_argument("--plain-only", action="store_true", help="time only the plain eager form")
-    parser.add_argument("--rate", type=float, default=200.0, help="c4: offered load, requests/sec")
-    parser.add_argument("--requests", type=int, default=400, help="c4: requests served")
-    parser.add_argument("--hbm-budget", type=int, default=16 << 30, help="c4: admission budget, bytes")
-    parser.add_argument("--streams", type=int, default=3, help="c4: streams for overlapping batches")
-    args = parser.parse_args(argv)
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the evaluation path shards by image with one "
-              "exchange at compute(), and no multi-GPU figure is produced here.", file=sys.stderr)
-        return 2
+def measure_steps(cfg, args, *, cpu_budget=None):
+    """Plain and batched forms of a step config as a bench-line dictionary."""
     import torch
-    if args.copy_calibration:
-        copy_calibration()
-        return 0
-    import panoptic_evaluator as pe  # noqa: F401
     from panoptic_evaluator import _native
-
-    configs = make_configs()
-    cfg = configs[args.config]
-    log(f"[bench] setting up {cfg.name} on cuda:0")
-    cfg.setup("cuda")
-    torch.cuda.synchronize()
-    if isinstance(cfg, ServingConfig):
-        return serve(cfg, args)
+    cpu_budget = args.cpu_budget if cpu_budget is None else cpu_budget
     eager_batched = getattr(cfg, "eager_batched_step", cfg.step)
     for _ in range(args.warmup):
         cfg.step()
@@ -706,7 +688,7 @@ def main(argv=None):
     tuning_report = None
     if not args.no_autotune:
         from panoptic_evaluator import tuning
-        log("[bench] autotuning launch shapes (untimed)")
+        log(f"[bench] {cfg.name}: autotuning launch shapes (untimed)")
         report = tuning.autotune(eager_batched, cfg.tuned, repeats=3)
         tuning_report = {kernel: {"variant": report["selection"][kernel],
                                   "describe": report["describe"][kernel][report["selection"][kernel]],
@@ -719,7 +701,7 @@ def main(argv=None):
         torch.cuda.synchronize()
     lib = _native.library()
 
-    log(f"[bench] timing {args.steps} plain steps")
+    log(f"[bench] {cfg.name}: timing {args.steps} plain steps")
     plain_elapsed = timed(cfg.step, args.steps)
     # The dominant kernel under the library's HIP events, in eager steps of
     # the batched update (the kernels and launch shapes the graph replays).
@@ -734,7 +716,7 @@ def main(argv=None):
     batched_elapsed, batched_form, probe = None, None, None
     if not args.plain_only:
         from panoptic_evaluator.graphs import CapturedUpdate
-        log("[bench] capturing the batched update as a HIP graph")
+        log(f"[bench] {cfg.name}: capturing the batched update as a HIP graph")
         captured = CapturedUpdate(cfg.evaluator, *cfg.update_args())
 
         def graph_step():
@@ -752,7 +734,7 @@ def main(argv=None):
         probe_steps = max(3, args.steps // 4)
         probe = {name: round(timed(fn, probe_steps) * 1e3 / probe_steps, 4) for name, fn in forms.items()}
         batched_form = min(probe, key=probe.get)
-        log(f"[bench] timing {args.steps} batched steps ({batched_form})")
+        log(f"[bench] {cfg.name}: timing {args.steps} batched steps ({batched_form})")
         batched_elapsed = timed(forms[batched_form], args.steps)
 
     plain_rate = cfg.images * args.steps / plain_elapsed
@@ -760,8 +742,8 @@ def main(argv=None):
     images_per_s = cfg.images * args.steps / value_elapsed
     cpu = None
     if not args.no_cpu_baseline:
-        log("[bench] timing the CPU oracle sample")
-        rate, sample = cfg.cpu_sample(args.cpu_budget)
+        log(f"[bench] {cfg.name}: timing the CPU oracle sample")
+        rate, sample = cfg.cpu_sample(cpu_budget)
         cpu = {"value": round(rate, 4), "unit": "images/sec", "cores": 1, "kind": "port",
                "sample": sample, "host_threads_available": os.cpu_count()}
     traffic = traffic_for(cfg.name)
@@ -808,6 +790,83 @@ def main(argv=None):
         line["mfma"] = {"kernel": cfg.dominant, "dense_equivalent_ops_per_launch": cfg.mfma_ops,
                         "achieved": round(tops, 2), "peak": MFMA_INT8_PEAK_TOPS, "unit": "TOP/s",
                         "frac": round(tops / MFMA_INT8_PEAK_TOPS, 4)}
+    return line
+
+
+# Sub-objects of the default line: the rest of the metric, measured in the
+# same process after the headline configuration.
+EXTRA_KEYS = ("value", "unit", "ms_per_step", "steps", "dtype", "plain", "speedup_vs_plain", "latency_ms",
+              "throughput_rps", "batches", "requests_per_launch", "admission_waits", "roofline", "cpu_baseline",
+              "tuning")
+
+
+def extra(line):
+    out = {key: line[key] for key in EXTRA_KEYS if key in line}
+    out["workload"] = line["config"]["workload"]
+    out["form"] = line["config"]["form"]
+    for key in ("offered_load_rps", "hbm_budget_bytes", "streams", "requests"):
+        if key in line["config"]:
+            out[key] = line["config"][key]
+    return out
+
+
+def run_config(name, args, configs, *, cpu_budget=None, requests=None):
+    import gc
+    import torch
+    cfg = configs[name]
+    log(f"[bench] setting up {cfg.name} on cuda:0")
+    cfg.setup("cuda")
+    torch.cuda.synchronize()
+    try:
+        if isinstance(cfg, ServingConfig):
+            return serve(cfg, args, cpu_budget=cpu_budget, requests=requests)
+        return measure_steps(cfg, args, cpu_budget=cpu_budget)
+    finally:
+        configs[name] = None
+        del cfg
+        gc.collect()
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+
+
+def main(argv=None):
+    parser = argparse.ArgumentParser(description=__doc__)
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=20)
+    parser.add_argument("--warmup", type=int, default=3)
+    parser.add_argument("--config", default=DEFAULT_CONFIG)
+    parser.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU oracle sample")
+    parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--copy-calibration", action="store_true")
+    parser.add_argument("--no-autotune", action="store_true",
+                        help="keep the default launch shapes instead of tuning them on this workload")
+    parser.add_argument("--plain-only", action="store_true", help="time only the plain eager form")
+    parser.add_argument("--rate", type=float, default=200.0, help="c4: offered load, requests/sec")
+    parser.add_argument("--requests", type=int, default=400, help="c4: requests served")
+    parser.add_argument("--hbm-budget", type=int, default=16 << 30, help="c4: admission budget, bytes")
+    parser.add_argument("--streams", type=int, default=3, help="c4: streams for overlapping batches")
+    parser.add_argument("--extras", default=None,
+                        help="comma-separated configs measured after the main one and reported as "
+                             "sub-objects of the same line (default for c1: c2,c4; 'none' for none)")
+    args = parser.parse_args(argv)
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the evaluation path shards by image with one "
+              "exchange at compute(), and no multi-GPU figure is produced here.", file=sys.stderr)
+        return 2
+    if args.copy_calibration:
+        copy_calibration()
+        return 0
+    import panoptic_evaluator as pe  # noqa: F401
+
+    configs = make_configs()
+    extras = args.extras if args.extras is not None else ("c2,c4" if args.config == DEFAULT_CONFIG else "none")
+    extras = [] if extras == "none" else [name for name in extras.split(",") if name]
+    line = run_config(args.config, args, configs)
+    for name in extras:
+        # The rest of the metric: batched vs plain on the uneven batch, and the
+        # serving loop's latency; half the CPU-sample budget each.
+        sub = run_config(name, args, configs, cpu_budget=args.cpu_budget / 2)
+        line[name] = extra(sub)
     print(json.dumps(line), flush=True)
     return 0
 
