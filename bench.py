@@ -470,6 +470,7 @@ class PackedCombinedConfig(CombinedConfig):
         import panoptic_evaluator as pe
         super().setup(device)
         self.detections = pe.InstanceBatch([pe.pack_masks(m) for m in self.masks], self.classes, self.scores)
+        self.tuned = ("detection_group", "detection_histogram_bits", "joint_histogram")
         B, H, W = self.pair.gt.shape
         dets = sum(m.shape[0] for m in self.masks)
         # bit rows of the detection masks + the int32 target map

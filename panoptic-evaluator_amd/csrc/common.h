@@ -163,7 +163,7 @@ __host__ __device__ inline i64 mask_words(i64 pixels) { return (pixels + 127) / 
 // process.  Selection is process-wide and meant to be set once, before
 // launches are issued (the autotuner times each variant and keeps the best).
 enum TuneKernel { kTuneDetectionHistogram = 0, kTuneJointHistogram, kTuneMaskOverlap, kTunePackedGrid,
-                  kTuneDetectionGroup, kTuneMaskOverlapBits, kTuneKernels };
+                  kTuneDetectionGroup, kTuneMaskOverlapBits, kTuneDetectionHistogramBits, kTuneKernels };
 struct TuneSpec {
   const char* name;
   int count;

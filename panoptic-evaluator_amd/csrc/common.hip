@@ -306,6 +306,9 @@ const char* const kBitsVariants[] = {
     "bits expanded to int8 MFMA operands in LDS, 128 dets x 64 GT x 4096 px, 2 detection tiles per wave",
     "popcounts of ANDed words, 128 dets x 64 GT x 16384 px",
     "popcounts of ANDed words, 64 dets x 64 GT x 16384 px"};
+const char* const kDetBitsVariants[] = {
+    "2 words x 8 masks", "2 words x 16 masks", "1 word x 16 masks", "4 words x 4 masks", "4 words x 8 masks",
+    "1 word x 8 masks", "2 words x 8 masks, wave-summed bins"};
 const TuneSpec kSpecs[kTuneKernels] = {
     {"detection_histogram", 5, 0, kDetVariants},
     {"joint_histogram", 5, 0, kJointVariants},
@@ -313,8 +316,9 @@ const TuneSpec kSpecs[kTuneKernels] = {
     {"packed_grid", 3, 0, kGridVariants},
     {"detection_group", 4, 0, kGroupVariants},
     {"mask_overlap_bits", 7, 0, kBitsVariants},
+    {"detection_histogram_bits", 7, 0, kDetBitsVariants},
 };
-std::atomic<int> g_selected[kTuneKernels] = {{-1}, {-1}, {-1}, {-1}, {-1}, {-1}};
+std::atomic<int> g_selected[kTuneKernels] = {{-1}, {-1}, {-1}, {-1}, {-1}, {-1}, {-1}};
 
 int tune_index(const char* name) {
   if (!name) return -1;

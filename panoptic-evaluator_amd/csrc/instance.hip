@@ -673,41 +673,166 @@ struct DetHistArgs {
   int aligned;
 };
 
-// Slot keys of one 16-pixel chunk: counts runs of equal slots per lane.
-__device__ __forceinline__ void count_chunk(v4i m, const int* slots, int S, int rowbase, uint32_t* bins) {
-  const uint32_t words[4] = {static_cast<uint32_t>(m.x), static_cast<uint32_t>(m.y),
-                             static_cast<uint32_t>(m.z), static_cast<uint32_t>(m.w)};
-  int run_key = -1, run = 0;
+// Pixel-by-pixel count of a chunk given as bits (bit j = pixel base + j),
+// for the rare chunks with more distinct slots than SlotGroups keeps: one
+// LDS add per set pixel, each slot read from the (cached) map when needed,
+// so no per-pixel arrays stay live in registers.
+template <typename TargetPtr>
+__device__ __forceinline__ void count_pixels(uint32_t on, TargetPtr target, i64 base, i64 HW, int S, int rowbase,
+                                             uint32_t* bins) {
+  // Eight slots are fetched at once (indices clamped into the image), so a
+  // chunk costs a few load latencies rather than one per set pixel.
+  for (int j0 = 0; j0 < 32 && (on >> j0); j0 += 8) {
+    int s[8];
 #pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const bool on = (words[j >> 2] >> ((j & 3) * 8)) & 0xFFu;
-    const int s = slots[j];
-    const int key = (on && s >= 0 && s < S) ? rowbase + s : -1;
-    if (key != run_key) {
-      if (run_key >= 0) atomicAdd(bins + (run_key >> 1), static_cast<uint32_t>(run) << ((run_key & 1) * 16));
-      run_key = key;
-      run = 0;
+    for (int j = 0; j < 8; ++j) s[j] = target[min(base + j0 + j, HW - 1)];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int slot = s[j];
+      if (!((on >> (j0 + j)) & 1u) || base + j0 + j >= HW || slot < 0 || slot >= S) continue;
+      const int key = rowbase + slot;
+      atomicAdd(bins + (key >> 1), 1u << ((key & 1) * 16));
     }
-    run += key >= 0;
   }
-  if (run_key >= 0) atomicAdd(bins + (run_key >> 1), static_cast<uint32_t>(run) << ((run_key & 1) * 16));
 }
 
-// The same count over 16 pixels given as bits (bit j = pixel j).
-__device__ __forceinline__ void count_chunk_bits(uint32_t on, const int* slots, int S, int rowbase, uint32_t* bins) {
-  int run_key = -1, run = 0;
+// The distinct valid target slots of an N-pixel chunk (N <= 32) with the
+// pixels each covers (bit j = pixel j), found once per tile.  Voronoi-like
+// maps give one segment plus scattered void pixels, sometimes a second
+// segment or two at a boundary: up to kGroups slots are kept; a chunk with more is
+// `complex` and is counted pixel by pixel.
+constexpr int kGroups = 5;
+struct SlotGroups {
+  int slot[kGroups];
+  uint32_t mask[kGroups];
+  bool complex;
+};
+
+// Adds slot s at pixel j of a chunk to its groups.
+__device__ __forceinline__ void group_insert(SlotGroups& g, int s, int j, int S) {
+  if (s < 0 || s >= S) return;
+  bool placed = false;
 #pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const int s = slots[j];
-    const int key = (((on >> j) & 1u) && s >= 0 && s < S) ? rowbase + s : -1;
-    if (key != run_key) {
-      if (run_key >= 0) atomicAdd(bins + (run_key >> 1), static_cast<uint32_t>(run) << ((run_key & 1) * 16));
-      run_key = key;
-      run = 0;
+  for (int k = 0; k < kGroups; ++k) {
+    if (placed) continue;
+    if (g.mask[k] == 0) {
+      g.slot[k] = s;
+      g.mask[k] = 1u << j;
+      placed = true;
+    } else if (g.slot[k] == s) {
+      g.mask[k] |= 1u << j;
+      placed = true;
     }
-    run += key >= 0;
   }
-  if (run_key >= 0) atomicAdd(bins + (run_key >> 1), static_cast<uint32_t>(run) << ((run_key & 1) * 16));
+  g.complex |= !placed;
+}
+
+// Slot groups of the N-pixel chunk starting at pixel `base` of the map.  A
+// chunk inside a 16-byte aligned map is read as N / 4 vectors from one
+// address, all in flight together; a chunk at the image's end (at most one
+// lane per image) is read pixel by pixel.
+template <int N, typename TargetPtr>
+__device__ __forceinline__ SlotGroups chunk_groups(TargetPtr target, i64 base, i64 HW, bool vec, int S) {
+  SlotGroups g;
+#pragma unroll
+  for (int k = 0; k < kGroups; ++k) {
+    g.slot[k] = 0;
+    g.mask[k] = 0;
+  }
+  g.complex = false;
+  if (vec && base + N <= HW) {
+    const auto* v = reinterpret_cast<const __attribute__((address_space(1))) v4i*>(target + base);
+    v4i raw[N / 4];
+#pragma unroll
+    for (int q = 0; q < N / 4; ++q) raw[q] = v[q];
+#pragma unroll
+    for (int q = 0; q < N / 4; ++q) {
+      group_insert(g, raw[q].x, 4 * q, S);
+      group_insert(g, raw[q].y, 4 * q + 1, S);
+      group_insert(g, raw[q].z, 4 * q + 2, S);
+      group_insert(g, raw[q].w, 4 * q + 3, S);
+    }
+  } else {
+#pragma unroll 1
+    for (int j = 0; j < N; ++j)
+      if (base + j < HW) group_insert(g, target[base + j], j, S);
+  }
+  return g;
+}
+
+// One lane's count of a detection's N-pixel chunk (bit j = pixel j set): one
+// popcount and at most one LDS add per distinct slot of the chunk.  A complex
+// chunk re-reads its target slots (cached: the tile's map was just read).
+template <typename TargetPtr>
+__device__ __forceinline__ void count_lane(uint32_t on, const SlotGroups& g, TargetPtr target, i64 base, i64 HW,
+                                           int S, int rowbase, uint32_t* bins) {
+  if (g.complex) {
+    count_pixels(on, target, base, HW, S, rowbase, bins);
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < kGroups; ++k) {
+    const uint32_t c = __popc(on & g.mask[k]);
+    if (c) {
+      const int key = rowbase + g.slot[k];
+      atomicAdd(bins + (key >> 1), c << ((key & 1) * 16));
+    }
+  }
+}
+
+__device__ __forceinline__ bool any_count(const uint32_t* c) {
+  uint32_t any = 0;
+#pragma unroll
+  for (int k = 0; k < kGroups; ++k) any |= c[k];
+  return any != 0;
+}
+
+// The same count for every lane of the wave together (`on` = 0 for lanes
+// with nothing to add).  The lanes of a wave cover neighbouring pixels, so
+// most add to the same (detection, slot) bin, and lane-by-lane adds would
+// serialise on that LDS address.  Instead each distinct key is summed across
+// the wave (DPP reduction) and added once; a wave holds few distinct keys
+// (the segments under the mask plus void).
+template <typename TargetPtr>
+__device__ __forceinline__ void count_wave(uint32_t on, const SlotGroups& g, TargetPtr target, i64 base, i64 HW,
+                                           int S, int rowbase, uint32_t* bins) {
+  uint32_t c[kGroups];
+  int key[kGroups];
+#pragma unroll
+  for (int k = 0; k < kGroups; ++k) {
+    c[k] = g.complex ? 0u : static_cast<uint32_t>(__popc(on & g.mask[k]));
+    key[k] = rowbase + g.slot[k];
+  }
+  if (g.complex && on) count_pixels(on, target, base, HW, S, rowbase, bins);
+  u64 pending = __ballot(any_count(c));
+  while (pending) {
+    const int leader = __builtin_ctzll(pending);
+    int mine = key[kGroups - 1];
+#pragma unroll
+    for (int k = kGroups - 2; k >= 0; --k) mine = c[k] ? key[k] : mine;
+    const int k_u = __builtin_amdgcn_readlane(mine, leader);
+    uint32_t contrib = 0;
+#pragma unroll
+    for (int k = 0; k < kGroups; ++k)
+      if (key[k] == k_u) {
+        contrib += c[k];
+        c[k] = 0;
+      }
+    const uint32_t total = __reduce_add_sync(~0ULL, contrib);
+    if (lane_id() == leader) atomicAdd(bins + (k_u >> 1), total << ((k_u & 1) * 16));
+    pending = __ballot(any_count(c));
+  }
+}
+
+// 16 mask bytes (bool: each 0 or 1) as 16 bits: in each 32-bit word the product
+// with 0x01020408 moves byte i's bit to bit 24 + i and leaves no carry into
+// bits 24..27.
+__device__ __forceinline__ uint32_t byte_bits(v4i m) {
+  const uint32_t w0 = (static_cast<uint32_t>(m.x) * 0x01020408u) >> 24;
+  const uint32_t w1 = (static_cast<uint32_t>(m.y) * 0x01020408u) >> 24;
+  const uint32_t w2 = (static_cast<uint32_t>(m.z) * 0x01020408u) >> 24;
+  const uint32_t w3 = (static_cast<uint32_t>(m.w) * 0x01020408u) >> 24;
+  return (w0 & 0xFu) | ((w1 & 0xFu) << 4) | ((w2 & 0xFu) << 8) | ((w3 & 0xFu) << 12);
 }
 
 // Masks are read exactly once: stream them past the caches.
@@ -723,59 +848,102 @@ __device__ __forceinline__ v4i load16_stream(const uint8_t* row, i64 k, i64 HW, 
   return v;
 }
 
-// CH = 16-pixel chunks per lane (tile = 16 * 256 * CH pixels); U = masks
-// whose loads are in flight together.  One (tile, detection group) item.
-// BITS: masks are bit rows of mask_words(pixels) 32-bit words (bit p % 32 of
-// word p / 32 is pixel p); a lane's 16-pixel chunk is half a word.
-template <int CH, int U, bool BITS>
-__device__ __forceinline__ void detection_item(const DetHistArgs& a, int item, uint4* lds4) {
-  constexpr int kTile = 16 * kDetThreads * CH;
-  uint32_t* bins = reinterpret_cast<uint32_t*>(lds4);
+// uint4 units of the packed 16-bit (detection, slot) bins of a group.
+__host__ __device__ constexpr i64 det_bins4(i64 group, i64 S) { return ((group * S + 1) / 2 + 3) / 4; }
+
+// Shared set-up of one (tile, detection group) item: zeroed bins and the
+// group's mask row offsets (element units) staged in LDS behind them, so the
+// row loop reads them from LDS instead of waiting on order[] before every
+// batch of mask loads.  Returns false when the group is empty.
+struct DetItem {
+  int b, tile, nd;
+  i64 first, d0, HW;
+  uint32_t* bins;
+  i64* row_off;
+  int words;
+};
+
+__device__ __forceinline__ bool begin_item(const DetHistArgs& a, int item, uint4* lds4, bool bits, DetItem& it) {
   const int tile_item = item / a.dgroups, dgroup = item - tile_item * a.dgroups;
-  const int b = __builtin_amdgcn_readfirstlane(a.tiles.image_of(tile_item));
-  const int tile = tile_item - a.tiles.first_tile(b);
-  const i64 first = a.det_off[b];
-  const int D = static_cast<int>(a.det_off[b + 1] - first);
-  const int d0 = dgroup * a.group;
-  const int nd = min(a.group, D - d0);
-  if (nd <= 0) return;  // uniform across the workgroup
-  const bool aligned = a.aligned != 0;
-  const int words = (nd * a.S + 1) / 2;
-  const int words4 = (words + 3) / 4;
+  it.b = __builtin_amdgcn_readfirstlane(a.tiles.image_of(tile_item));
+  it.tile = tile_item - a.tiles.first_tile(it.b);
+  it.first = a.det_off[it.b];
+  const int D = static_cast<int>(a.det_off[it.b + 1] - it.first);
+  it.d0 = static_cast<i64>(dgroup) * a.group;
+  it.nd = min(a.group, D - static_cast<int>(it.d0));
+  if (it.nd <= 0) return false;  // uniform across the workgroup
+  it.HW = a.tiles.pixels(it.b);
+  it.bins = reinterpret_cast<uint32_t*>(lds4);
+  it.row_off = reinterpret_cast<i64*>(lds4 + det_bins4(a.group, a.S));
+  it.words = (it.nd * a.S + 1) / 2;
+  const int words4 = (it.words + 3) / 4;
+  const i64 stride = bits ? mask_words(it.HW) : it.HW;
   for (int w = threadIdx.x; w < words4; w += kDetThreads) lds4[w] = make_uint4(0, 0, 0, 0);
+  for (int d = threadIdx.x; d < it.nd; d += kDetThreads) it.row_off[d] = a.order[it.first + it.d0 + d] * stride;
   __syncthreads();
-  const i64 HW = a.tiles.pixels(b);
-  const i64 p0 = static_cast<i64>(tile) * kTile;
-  const auto* target = as_global(a.target + a.tiles.begin(b));
-  const uint8_t* masks = a.masks[b];
-  const i64* order = a.order + first + d0;
-  int slots[CH][16];
+  return true;
+}
+
+// Flush the item's nonzero 16-bit bins into the (detection, slot) table.
+__device__ __forceinline__ void end_item(const DetHistArgs& a, const DetItem& it) {
+  __syncthreads();
+  u64* out = a.pairs + (it.first + it.d0) * a.S;
+  for (int w = threadIdx.x; w < it.words; w += kDetThreads) {
+    const uint32_t v = it.bins[w];
+    if (!v) continue;
+    const uint32_t lo = v & 0xFFFFu, hi = v >> 16;
+    if (lo) atomicAdd(out + 2 * w, static_cast<u64>(lo));
+    if (hi) atomicAdd(out + 2 * w + 1, static_cast<u64>(hi));
+  }
+}
+
+// Bool masks.  CH = 16-pixel chunks per lane (tile = 16 * 256 * CH pixels);
+// U = masks whose loads are in flight together.  Every load is unconditional,
+// with its index clamped into the image or group and the value masked after:
+// a load behind a branch gets its own wait, which would serialise the loads.
+template <int CH, int U>
+__device__ __forceinline__ void detection_item_bytes(const DetHistArgs& a, int item, uint4* lds4) {
+  constexpr int kTile = 16 * kDetThreads * CH;
+  DetItem it;
+  if (!begin_item(a, item, lds4, false, it)) return;
+  const int nd = it.nd;
+  const i64 HW = it.HW;
+  const i64 p0 = static_cast<i64>(it.tile) * kTile;
+  const auto* target = as_global(a.target + a.tiles.begin(it.b));
+  const bool vec = (reinterpret_cast<uintptr_t>(target) & 15) == 0;  // base is a multiple of 16 pixels
+  const uint8_t* masks = a.masks[it.b];
   i64 base[CH];
+  bool inside[CH];
+  SlotGroups groups[CH];
 #pragma unroll
   for (int h = 0; h < CH; ++h) {
     base[h] = p0 + (static_cast<i64>(h) * kDetThreads + threadIdx.x) * 16;
-#pragma unroll
-    for (int j = 0; j < 16; ++j) slots[h][j] = base[h] + j < HW ? target[base[h] + j] : -1;
+    inside[h] = base[h] < HW;
+    groups[h] = chunk_groups<16>(target, base[h], HW, vec, a.S);
   }
-  if constexpr (BITS) {
-    const i64 stride = mask_words(HW);
-    const auto* rows = as_global(reinterpret_cast<const uint32_t*>(masks));
+  if (a.aligned) {
+    // 16-byte aligned rows whose pixel counts are multiples of 16: a lane's
+    // chunk inside the image is one whole aligned vector.
+    i64 offset[CH];
+#pragma unroll
+    for (int h = 0; h < CH; ++h) offset[h] = inside[h] ? base[h] : 0;
     for (int r0 = 0; r0 < nd; r0 += U) {
-      uint32_t m[U][CH];
+      v4i raw[U][CH];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const auto* mask = rows + (r0 + u < nd ? order[r0 + u] : 0) * stride;
+        const uint8_t* mask = masks + it.row_off[min(r0 + u, nd - 1)];
 #pragma unroll
         for (int h = 0; h < CH; ++h)
-          m[u][h] = r0 + u < nd && base[h] < HW
-                        ? (__builtin_nontemporal_load(mask + (base[h] >> 5)) >> (base[h] & 16)) & 0xFFFFu
-                        : 0u;
+          raw[u][h] = __builtin_nontemporal_load(as_global(reinterpret_cast<const v4i*>(mask + offset[h])));
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
 #pragma unroll
-        for (int h = 0; h < CH; ++h)
-          if (m[u][h]) count_chunk_bits(m[u][h], slots[h], a.S, (r0 + u) * a.S, bins);
+        for (int h = 0; h < CH; ++h) {
+          const v4i& v = raw[u][h];
+          if (r0 + u < nd && inside[h] && (v.x | v.y | v.z | v.w) != 0)
+            count_lane(byte_bits(v), groups[h], target, base[h], HW, a.S, (r0 + u) * a.S, it.bins);
+        }
     }
   } else {
     for (int r0 = 0; r0 < nd; r0 += U) {
@@ -783,9 +951,9 @@ __device__ __forceinline__ void detection_item(const DetHistArgs& a, int item, u
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         if (r0 + u < nd) {
-          const uint8_t* mask = masks + order[r0 + u] * HW;
+          const uint8_t* mask = masks + it.row_off[r0 + u];
 #pragma unroll
-          for (int h = 0; h < CH; ++h) m[u][h] = load16_stream(mask, base[h], HW, aligned);
+          for (int h = 0; h < CH; ++h) m[u][h] = load16_stream(mask, base[h], HW, false);
         } else {
 #pragma unroll
           for (int h = 0; h < CH; ++h) m[u][h] = v4i{0, 0, 0, 0};
@@ -796,52 +964,137 @@ __device__ __forceinline__ void detection_item(const DetHistArgs& a, int item, u
 #pragma unroll
         for (int h = 0; h < CH; ++h)
           if ((m[u][h].x | m[u][h].y | m[u][h].z | m[u][h].w) != 0)
-            count_chunk(m[u][h], slots[h], a.S, (r0 + u) * a.S, bins);
+            count_lane(byte_bits(m[u][h]), groups[h], target, base[h], HW, a.S, (r0 + u) * a.S, it.bins);
     }
   }
-  __syncthreads();
-  u64* out = a.pairs + (first + d0) * a.S;
-  for (int w = threadIdx.x; w < words; w += kDetThreads) {
-    const uint32_t v = bins[w];
-    if (!v) continue;
-    const uint32_t lo = v & 0xFFFFu, hi = v >> 16;
-    if (lo) atomicAdd(out + 2 * w, static_cast<u64>(lo));
-    if (hi) atomicAdd(out + 2 * w + 1, static_cast<u64>(hi));
+  end_item(a, it);
+}
+
+// Bit-packed masks (rows of mask_words(pixels) 32-bit words, bit p % 32 of
+// word p / 32 = pixel p).  A lane owns W consecutive words (32 * W pixels,
+// tile = 32 * W * 256 pixels) and reads them with one vector load per mask,
+// so a workgroup keeps U * W * 1 KB of mask bits in flight.  AGG: sum each
+// (detection, slot) key across the wave before the LDS add.
+template <int W>
+struct WordsT;
+template <>
+struct WordsT<1> {
+  using T = uint32_t;
+};
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+template <>
+struct WordsT<2> {
+  using T = u32x2;
+};
+template <>
+struct WordsT<4> {
+  using T = u32x4;
+};
+template <int W>
+__device__ __forceinline__ uint32_t word_of(const typename WordsT<W>::T& v, int w) {
+  if constexpr (W == 1) {
+    return v;
+  } else if constexpr (W == 2) {
+    return w == 0 ? v.x : v.y;
+  } else {
+    return w == 0 ? v.x : w == 1 ? v.y : w == 2 ? v.z : v.w;
   }
+}
+
+template <int W, int U, bool AGG>
+__device__ __forceinline__ void detection_item_bits(const DetHistArgs& a, int item, uint4* lds4) {
+  constexpr int kTile = 32 * W * kDetThreads;
+  using Vec = typename WordsT<W>::T;
+  DetItem it;
+  if (!begin_item(a, item, lds4, true, it)) return;
+  const int nd = it.nd;
+  const i64 HW = it.HW;
+  const i64 lane_px = static_cast<i64>(it.tile) * kTile + static_cast<i64>(threadIdx.x) * 32 * W;
+  const auto* target = as_global(a.target + a.tiles.begin(it.b));
+  const bool vec = (reinterpret_cast<uintptr_t>(target) & 15) == 0;  // lane_px is a multiple of 32
+  bool inside[W];
+  SlotGroups groups[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    inside[w] = lane_px + 32 * w < HW;
+    groups[w] = chunk_groups<32>(target, lane_px + 32 * w, HW, vec, a.S);
+  }
+  // Rows are padded to whole 16-byte units, so the W words of a lane whose
+  // first pixel is inside the image are inside the row.
+  const i64 word0 = inside[0] ? lane_px >> 5 : 0;
+  const auto* rows = as_global(reinterpret_cast<const uint32_t*>(a.masks[it.b]));
+  for (int r0 = 0; r0 < nd; r0 += U) {
+    Vec raw[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      raw[u] = __builtin_nontemporal_load(reinterpret_cast<const __attribute__((address_space(1))) Vec*>(
+          rows + it.row_off[min(r0 + u, nd - 1)] + word0));
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const uint32_t on = r0 + u < nd && inside[w] ? word_of<W>(raw[u], w) : 0u;
+        if constexpr (AGG) {
+          count_wave(on, groups[w], target, lane_px + 32 * w, HW, a.S, (r0 + u) * a.S, it.bins);
+        } else {
+          if (on) count_lane(on, groups[w], target, lane_px + 32 * w, HW, a.S, (r0 + u) * a.S, it.bins);
+        }
+      }
+  }
+  end_item(a, it);
 }
 
 // One workgroup per item (grid >= item count; surplus workgroups of a packed
 // batch, whose count is only known on the device, exit at once), or a
 // persistent grid that walks the items with a grid stride.
-template <int CH, int U, bool PERSIST, bool BITS>
+template <int CH, int U, bool PERSIST>
 __global__ __launch_bounds__(kDetThreads) void detection_histogram(DetHistArgs a) {
   extern __shared__ uint4 lds4[];
   const int total = a.tiles.count(a.B) * a.dgroups;
   if (!PERSIST) {
-    if (static_cast<int>(blockIdx.x) < total) detection_item<CH, U, BITS>(a, blockIdx.x, lds4);
+    if (static_cast<int>(blockIdx.x) < total) detection_item_bytes<CH, U>(a, blockIdx.x, lds4);
     return;
   }
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
-    detection_item<CH, U, BITS>(a, item, lds4);
+    detection_item_bytes<CH, U>(a, item, lds4);
     __syncthreads();  // the next item re-zeroes the bins
   }
 }
 
-struct DetVariant {
-  int chunks;
-  void (*kernel[2])(DetHistArgs);      // [bits]
-  void (*persistent[2])(DetHistArgs);
-};
-#define PE_DET_VARIANT(CH, U)                                                                \
-  {                                                                                          \
-    CH, {detection_histogram<CH, U, false, false>, detection_histogram<CH, U, false, true>}, \
-        {detection_histogram<CH, U, true, false>, detection_histogram<CH, U, true, true>}    \
+template <int W, int U, bool AGG, bool PERSIST>
+__global__ __launch_bounds__(kDetThreads) void detection_histogram_bits(DetHistArgs a) {
+  extern __shared__ uint4 lds4[];
+  const int total = a.tiles.count(a.B) * a.dgroups;
+  if (!PERSIST) {
+    if (static_cast<int>(blockIdx.x) < total) detection_item_bits<W, U, AGG>(a, blockIdx.x, lds4);
+    return;
   }
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    detection_item_bits<W, U, AGG>(a, item, lds4);
+    __syncthreads();
+  }
+}
+
+struct DetVariant {
+  int tile;                            // pixels per tile
+  void (*kernel)(DetHistArgs);
+  void (*persistent)(DetHistArgs);
+};
+#define PE_DET_VARIANT(CH, U) \
+  { 16 * kDetThreads * CH, detection_histogram<CH, U, false>, detection_histogram<CH, U, true> }
+#define PE_DET_BITS(W, U, AGG) \
+  { 32 * W * kDetThreads, detection_histogram_bits<W, U, AGG, false>, detection_histogram_bits<W, U, AGG, true> }
 // Order matches the tuning registry's descriptions.
 const DetVariant kDetHist[] = {
     PE_DET_VARIANT(1, 4), PE_DET_VARIANT(1, 8), PE_DET_VARIANT(2, 2), PE_DET_VARIANT(2, 4), PE_DET_VARIANT(4, 2),
 };
+const DetVariant kDetHistBits[] = {
+    PE_DET_BITS(2, 8, false), PE_DET_BITS(2, 16, false), PE_DET_BITS(1, 16, false), PE_DET_BITS(4, 4, false),
+    PE_DET_BITS(4, 8, false), PE_DET_BITS(1, 8, false), PE_DET_BITS(2, 8, true),
+};
 #undef PE_DET_VARIANT
+#undef PE_DET_BITS
 
 // ------------------------------------------------------------ COCO matching
 
@@ -1694,11 +1947,13 @@ static int detection_histogram_launch(int64_t B, const uint8_t* const* masks, co
   a.target = target;
   a.B = static_cast<int>(B);
   a.S = static_cast<int>(S);
-  const DetVariant& v = kDetHist[tuned_variant(kTuneDetectionHistogram)];
-  const i64 tile = 16 * kDetThreads * v.chunks;
-  // Detections per group: packed 16-bit counters of group x S within the LDS
-  // budget (a tile holds at most 16384 pixels, so no counter exceeds 0xFFFF).
-  i64 group = static_cast<i64>(kDetMaxLds * 2 / 4) / S;  // bins per group / S
+  const DetVariant& v = bits ? kDetHistBits[tuned_variant(kTuneDetectionHistogramBits)]
+                             : kDetHist[tuned_variant(kTuneDetectionHistogram)];
+  const i64 tile = v.tile;
+  // Detections per group: packed 16-bit counters of group x S plus one 8-byte
+  // row offset per detection within the LDS budget (a tile holds at most
+  // 16384 pixels, so no counter exceeds 0xFFFF).
+  i64 group = static_cast<i64>(kDetMaxLds) / (2 * S + 8);
   PE_REQUIRE(group >= 1, "too many target slots for the detection histogram");
   if (detection_group_limit() > 0) group = std::min<i64>(group, detection_group_limit());
   group = std::min<i64>(group, max_dets);
@@ -1733,8 +1988,8 @@ static int detection_histogram_launch(int64_t B, const uint8_t* const* masks, co
     blocks = B * t.tiles_per_image * a.dgroups;
   }
   PE_REQUIRE(blocks < INT_MAX, "too many workgroups");
-  const size_t lds = static_cast<size_t>((group * S + 1) / 2 + 3) / 4 * 16;
-  auto kernel = persist ? v.persistent[bits] : v.kernel[bits];
+  const size_t lds = static_cast<size_t>(det_bins4(group, S)) * 16 + 8 * static_cast<size_t>(group);
+  auto kernel = persist ? v.persistent : v.kernel;
   if (lds > 64 * 1024)
     PE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
                                static_cast<int>(lds)));

@@ -1,8 +1,8 @@
 """Launch-shape autotuner for the streaming kernels.
 
 Each tuning knob of libpanoptic_hip.so ("joint_histogram",
-"detection_histogram", "mask_overlap_mfma", "packed_grid", "detection_group",
-"mask_overlap_bits") has a fixed list of variants -- pixel tile per
+"detection_histogram", "detection_histogram_bits", "mask_overlap_mfma",
+"packed_grid", "detection_group", "mask_overlap_bits") has a fixed list of variants -- pixel tile per
 workgroup, masks in flight per lane, mask groups and pixel range per
 contraction workgroup, persistent grid size over packed batches, detections
 per workgroup (LDS bins) -- that compute identical results.  ``autotune`` times every
@@ -22,12 +22,13 @@ import torch
 from . import _native
 
 KERNELS = ("joint_histogram", "detection_histogram", "mask_overlap_mfma", "packed_grid", "detection_group",
-           "mask_overlap_bits")
+           "mask_overlap_bits", "detection_histogram_bits")
 
 # Kernels whose time decides each knob ("packed_grid" shapes the persistent
 # grids of both histograms over packed batches of uneven images).
 MEASURED = {"packed_grid": ("joint_histogram", "detection_histogram"),
-            "detection_group": ("detection_histogram",)}
+            "detection_group": ("detection_histogram",),
+            "detection_histogram_bits": ("detection_histogram",)}
 
 
 def variants(kernel: str) -> list[str]:

@@ -147,8 +147,8 @@ def test_combined_packed_detections_all_variants(pe, tuning):
     expected = state.compute()
     dets = pe.InstanceBatch([pe.pack_masks(m.cuda()) for m in masks], [c.cuda() for c in classes],
                             [s.cuda() for s in scores])
-    for variant in range(len(tuning.variants("detection_histogram"))):
-        tuning.select("detection_histogram", variant)
+    for variant in range(len(tuning.variants("detection_histogram_bits"))):
+        tuning.select("detection_histogram_bits", variant)
         joint = pe.SegmentationEvaluator(133, isthing=isthing)
         joint.update(pe.PanopticBatch(pair.pred.cuda(), pair.pc.cuda()), dets,
                      pe.PanopticBatch(pair.gt.cuda(), pair.gc.cuda(), pair.crowd.cuda()))
@@ -175,12 +175,14 @@ def test_combined_packed_uneven_batch(pe, tuning):
     prediction = pe.PanopticBatch.pack(dev(batch.pred), dev(batch.pc))
     dets = pe.InstanceBatch([pe.pack_masks(m.cuda()) for m in batch.masks], dev(batch.classes), dev(batch.scores))
     for grid in range(len(tuning.variants("packed_grid"))):
-        tuning.select("packed_grid", grid)
-        joint = pe.SegmentationEvaluator(133, isthing=isthing)
-        joint.update(prediction, dets, target)
-        result = to_numpy(joint.compute())
-        compare_panoptic(result, expected)
-        compare_instance(result["instance"], expected["instance"])
+        for variant in (0, 1, len(tuning.variants("detection_histogram_bits")) - 1):
+            tuning.select("packed_grid", grid)
+            tuning.select("detection_histogram_bits", variant)
+            joint = pe.SegmentationEvaluator(133, isthing=isthing)
+            joint.update(prediction, dets, target)
+            result = to_numpy(joint.compute())
+            compare_panoptic(result, expected)
+            compare_instance(result["instance"], expected["instance"])
 
 
 def test_packed_masks_rejected_where_unsupported(pe):
