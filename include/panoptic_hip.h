@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define PE_ABI_VERSION 2
+#define PE_ABI_VERSION 3
 
 const char* pe_last_error(void);
 int pe_abi_version(void);
@@ -231,20 +231,22 @@ int pe_mask_overlap_bits_batch(int64_t B, const uint32_t* const* pm, const uint3
 int64_t pe_detection_histogram_workspace_bytes(int64_t B, int64_t packed_pixels);
 
 /* Detection x target-slot intersections: pairs[row, s] = #pixels of mask
- * order[row] (within its image) whose target slot is s; rows are grouped per
- * image by det_off (B + 1, device).  masks: device array of B pointers to
+ * order[row] (within its image; order NULL: mask row) whose target slot is
+ * s; rows are grouped per image by det_off (B + 1, device).  masks: device array of B pointers to
  * D_b x pixels_b bool tensors; target: the images' int32 slots one after
  * another.  pix_off NULL: every image has HW pixels (target is B x HW);
  * otherwise B + 1 device pixel offsets of a packed batch of uneven images
  * and HW is the pixel total (workspace from the query above).  pairs:
- * (sum D_b) x S int64.  aligned != 0 promises that every image's pixel count
- * is a multiple of 16 and the mask pointers are 16-byte aligned.  Replaces
+ * (sum D_b) x S int64; pairs_zeroed != 0 promises it is all zero on entry
+ * (as pe_coco_match with zero_inter leaves it), which saves a fill.
+ * aligned != 0 promises that every image's pixel count is a multiple of 16
+ * and the mask pointers are 16-byte aligned.  Replaces
  * ``instance_histogram`` (bindings.cpp:18-19; instance_histogram_cuda,
  * instance_kernel.cu:335-363), which takes one image at a time. */
 int pe_detection_histogram(int64_t B, const uint8_t* const* masks, const int64_t* det_off,
                            const int64_t* order, const int32_t* target, const int64_t* pix_off,
                            int64_t HW, int64_t S, int64_t max_dets, int64_t total_dets, int64_t* pairs,
-                           int aligned, void* workspace, int64_t workspace_bytes, void* stream);
+                           int pairs_zeroed, int aligned, void* workspace, int64_t workspace_bytes, void* stream);
 
 /* pe_detection_histogram over bit-packed detection masks (rows as for
  * pe_pack_masks, 16-byte aligned): same arguments and outputs, no
@@ -252,13 +254,16 @@ int pe_detection_histogram(int64_t B, const uint8_t* const* masks, const int64_t
 int pe_detection_histogram_bits(int64_t B, const uint32_t* const* masks, const int64_t* det_off,
                                 const int64_t* order, const int32_t* target, const int64_t* pix_off,
                                 int64_t HW, int64_t S, int64_t max_dets, int64_t total_dets, int64_t* pairs,
-                                void* workspace, int64_t workspace_bytes, void* stream);
+                                int pairs_zeroed, void* workspace, int64_t workspace_bytes, void* stream);
 
 /* COCO greedy matching of a batch of images.  Per image b, detections
  * det_off[b]..det_off[b+1] are in score order (det_cls int32, det_area f64
  * pixel areas) and ground truth gt_off[b]..gt_off[b+1] in metadata order
  * (gt_cls int32, -1 = not a candidate; gt_area f64 pixel areas; gt_crowd;
- * gt_ann f64 annotation areas for the area ranges).  Intersections of image b
+ * gt_ann f64 annotation areas for the area ranges; det_area may be NULL for
+ * int64 intersections, then each detection's area is the sum of its
+ * intersection row, as for the shared-target table whose rows span every
+ * target slot).  Intersections of image b
  * start at inter_off[b] as a D_b x G_b block (int32 or int64, inter_bytes);
  * det_row (may be NULL) gives the block row of each detection.  Outputs:
  * status (sum D) x 40 int8 [area*10 + threshold], ranks (sum D) int32 (1..100,
@@ -266,7 +271,10 @@ int pe_detection_histogram_bits(int64_t B, const uint32_t* const* masks, const i
  * written when *guard != 0.  gt_things (C uint8, may be NULL) keeps only GT
  * of thing classes as candidates; gt_area_i64 (may be NULL, then gt_area is
  * used) reads int64 pixel areas at b * gt_area_stride + j; gt_ann may be NULL
- * (pixel areas drive the area ranges).  Replaces ``instance_update`` /
+ * (pixel areas drive the area ranges).  zero_inter != 0 (int64 rows only)
+ * zeroes every detection's intersection row after matching, so a caller that
+ * keeps the table can pass it to pe_detection_histogram with pairs_zeroed.
+ * Replaces ``instance_update`` /
  * ``instance_match_histogram`` matching (bindings.cpp:13-15, :20-22;
  * match_instances, instance_kernel.cu:41-174; CPU _match_ious_cpu,
  * instance.py:267-297). */
@@ -276,7 +284,7 @@ int pe_coco_match(int64_t B, int64_t C, const int64_t* det_off, const int64_t* g
                   const uint8_t* gt_crowd, const double* gt_ann, const void* inter, int inter_bytes,
                   int64_t total_dets, int64_t max_gt, int8_t* status, int32_t* ranks, int64_t* counts,
                   const int32_t* guard, const uint8_t* gt_things, const int64_t* gt_area_i64,
-                  int64_t gt_area_stride, void* stream);
+                  int64_t gt_area_stride, int zero_inter, void* stream);
 
 /* Scratch bytes of pe_score_order for N detections in B images. */
 int64_t pe_score_order_workspace_bytes(int64_t N, int64_t B);

@@ -879,7 +879,8 @@ __device__ __forceinline__ bool begin_item(const DetHistArgs& a, int item, uint4
   const int words4 = (it.words + 3) / 4;
   const i64 stride = bits ? mask_words(it.HW) : it.HW;
   for (int w = threadIdx.x; w < words4; w += kDetThreads) lds4[w] = make_uint4(0, 0, 0, 0);
-  for (int d = threadIdx.x; d < it.nd; d += kDetThreads) it.row_off[d] = a.order[it.first + it.d0 + d] * stride;
+  for (int d = threadIdx.x; d < it.nd; d += kDetThreads)
+    it.row_off[d] = (a.order ? a.order[it.first + it.d0 + d] : it.d0 + d) * stride;
   __syncthreads();
   return true;
 }
@@ -1121,6 +1122,7 @@ struct MatchArgs {
   const uint8_t* gt_things;   // optional: only GT of thing classes are candidates
   const i64* gt_area_i64;     // optional: int64 pixel areas at b * gt_area_stride + j
   i64 gt_area_stride;
+  int zero_inter;             // zero every detection's intersection row after matching
 };
 
 // GT candidate class (or -1), pixel area and annotation area of GT j of image b.
@@ -1156,6 +1158,33 @@ __device__ __forceinline__ double iou_threshold(int t) {
 constexpr int kMatchStagePairs = 512;
 constexpr int kMatchStageGts = 128;
 constexpr size_t kMatchStageBytes = 8 * kMatchStagePairs + 17 * kMatchStageGts;
+
+__device__ __forceinline__ void match_chains(const MatchArgs& a, int b, i64 d0, i64 g0, int G, i64 ib, int nd,
+                                             int ng, bool staged, const int* dets, const i64* det_rw,
+                                             const double* det_pa, const double* pair_inter, const double* gt_px,
+                                             const double* gt_an, const uint8_t* gt_cr, const int* gts,
+                                             uint32_t* used);
+
+// Zero the int64 intersection rows of image b's detections of class c (the
+// class-0 wave also takes detections of no valid class), so a caller that
+// keeps the table gets it back all-zero.
+__device__ __forceinline__ void zero_rows(const MatchArgs& a, int b, int c, i64 d0, int D, int G, i64 ib) {
+  const int lane = threadIdx.x;
+  i64* rows = static_cast<i64*>(const_cast<void*>(a.inter)) + ib;
+  for (int base = 0; base < D; base += kWave) {
+    const int i = min(base + lane, D - 1);
+    const int cls = a.det_cls[d0 + i];
+    const i64 row = a.det_row ? a.det_row[d0 + i] : i;
+    const bool mine = base + lane < D && (cls == c || (c == 0 && (cls < 0 || cls >= a.C)));
+    u64 pending = __ballot(mine);
+    while (pending) {
+      const int src = __builtin_ctzll(pending);
+      pending &= pending - 1;
+      const i64 r = __shfl(row, src, kWave);
+      for (int g = lane; g < G; g += kWave) rows[r * G + g] = 0;
+    }
+  }
+}
 
 __global__ __launch_bounds__(64) void coco_match(MatchArgs a) {
   extern __shared__ uint4 lds4[];
@@ -1196,7 +1225,7 @@ __global__ __launch_bounds__(64) void coco_match(MatchArgs a) {
     for (int k = 0; k < kScan; ++k) {
       const int i = min(base + k * kWave + lane, D - 1);
       cls[k] = a.det_cls[d0 + i];
-      area[k] = a.det_area[d0 + i];
+      area[k] = a.det_area ? a.det_area[d0 + i] : 0.0;
       row[k] = a.det_row ? a.det_row[d0 + i] : i;
     }
 #pragma unroll
@@ -1259,9 +1288,34 @@ __global__ __launch_bounds__(64) void coco_match(MatchArgs a) {
   if (lane == 0 && ng)
     for (int r = 0; r < 4; ++r)
       if (positives[r]) atomicAdd(reinterpret_cast<u64*>(a.counts) + c * 4 + r, static_cast<u64>(positives[r]));
-  if (nd == 0) return;
-  for (int w = lane; w < 40 * a.words; w += kWave) used[w] = 0;
   const i64 ib = a.inter_off[b];
+  if (!a.det_area && nd > 0) {
+    // Detection areas are the row sums of the int64 intersection rows (the
+    // shared-target table spans every target slot): four rows at a time, each
+    // lane's columns loaded together, then summed across the wave.
+    __syncthreads();  // det_rw[] complete
+    const i64* rows = static_cast<const i64*>(a.inter) + ib;
+    const int ranked = min(nd, 100);
+    for (int r0 = 0; r0 < ranked; r0 += 4) {
+      i64 sum[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const i64* row = rows + det_rw[min(r0 + u, ranked - 1)] * G;
+        sum[u] = 0;
+        for (int g = lane; g < G; g += kWave) sum[u] += row[g];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const i64 total = wave_sum(sum[u]);
+        if (lane == 0 && r0 + u < ranked) det_pa[r0 + u] = static_cast<double>(total);
+      }
+    }
+  }
+  if (nd == 0) {
+    if (a.zero_inter) zero_rows(a, b, c, d0, D, G, ib);
+    return;
+  }
+  for (int w = lane; w < 40 * a.words; w += kWave) used[w] = 0;
   const bool staged = ng <= kMatchStageGts && nd * ng <= kMatchStagePairs;
   if (staged && ng > 0) {
     __syncthreads();  // gts[] complete
@@ -1288,7 +1342,22 @@ __global__ __launch_bounds__(64) void coco_match(MatchArgs a) {
     }
   }
   __syncthreads();
-  if (lane >= 40) return;
+  if (lane < 40) match_chains(a, b, d0, g0, G, ib, nd, ng, staged, dets, det_rw, det_pa, pair_inter, gt_px, gt_an,
+                              gt_cr, gts, used);
+  if (a.zero_inter) {
+    __syncthreads();  // every intersection read is done
+    zero_rows(a, b, c, d0, D, G, ib);
+  }
+}
+
+// The 40 greedy chains of one (image, class) wave: lane = area range * 10 +
+// IoU threshold; detections in score order.
+__device__ __forceinline__ void match_chains(const MatchArgs& a, int b, i64 d0, i64 g0, int G, i64 ib, int nd,
+                                             int ng, bool staged, const int* dets, const i64* det_rw,
+                                             const double* det_pa, const double* pair_inter, const double* gt_px,
+                                             const double* gt_an, const uint8_t* gt_cr, const int* gts,
+                                             uint32_t* used) {
+  const int lane = threadIdx.x;
   const int range = lane / 10, t = lane - (lane / 10) * 10;
   double lo, hi;
   area_range(range, lo, hi);
@@ -1930,7 +1999,7 @@ extern "C" int64_t pe_detection_histogram_workspace_bytes(int64_t B, int64_t pac
 static int detection_histogram_launch(int64_t B, const uint8_t* const* masks, const int64_t* det_off,
                                       const int64_t* order, const int32_t* target, const int64_t* pix_off,
                                       int64_t HW, int64_t S, int64_t max_dets, int64_t total_dets,
-                                      int64_t* pairs, int aligned, bool bits, void* workspace,
+                                      int64_t* pairs, int pairs_zeroed, int aligned, bool bits, void* workspace,
                                       int64_t workspace_bytes, void* stream_ptr) {
   PE_REQUIRE(B >= 0 && B < INT_MAX && HW > 0 && S > 0 && S < INT_MAX && max_dets >= 0 && total_dets >= 0,
              "invalid shapes");
@@ -1938,7 +2007,8 @@ static int detection_histogram_launch(int64_t B, const uint8_t* const* masks, co
              "workspace too small for a packed batch");
   hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
   int rc;
-  if (total_dets && (rc = launch_zero(pairs, 8 * static_cast<size_t>(total_dets * S), stream))) return rc;
+  if (total_dets && !pairs_zeroed && (rc = launch_zero(pairs, 8 * static_cast<size_t>(total_dets * S), stream)))
+    return rc;
   if (B == 0 || max_dets == 0) return kOk;
   DetHistArgs a{};
   a.masks = masks;
@@ -2002,19 +2072,19 @@ static int detection_histogram_launch(int64_t B, const uint8_t* const* masks, co
 extern "C" int pe_detection_histogram(int64_t B, const uint8_t* const* masks, const int64_t* det_off,
                                       const int64_t* order, const int32_t* target, const int64_t* pix_off,
                                       int64_t HW, int64_t S, int64_t max_dets, int64_t total_dets,
-                                      int64_t* pairs, int aligned, void* workspace, int64_t workspace_bytes,
-                                      void* stream_ptr) {
+                                      int64_t* pairs, int pairs_zeroed, int aligned, void* workspace,
+                                      int64_t workspace_bytes, void* stream_ptr) {
   return detection_histogram_launch(B, masks, det_off, order, target, pix_off, HW, S, max_dets, total_dets, pairs,
-                                    aligned, false, workspace, workspace_bytes, stream_ptr);
+                                    pairs_zeroed, aligned, false, workspace, workspace_bytes, stream_ptr);
 }
 
 extern "C" int pe_detection_histogram_bits(int64_t B, const uint32_t* const* masks, const int64_t* det_off,
                                            const int64_t* order, const int32_t* target, const int64_t* pix_off,
                                            int64_t HW, int64_t S, int64_t max_dets, int64_t total_dets,
-                                           int64_t* pairs, void* workspace, int64_t workspace_bytes,
-                                           void* stream_ptr) {
+                                           int64_t* pairs, int pairs_zeroed, void* workspace,
+                                           int64_t workspace_bytes, void* stream_ptr) {
   return detection_histogram_launch(B, reinterpret_cast<const uint8_t* const*>(masks), det_off, order, target,
-                                    pix_off, HW, S, max_dets, total_dets, pairs, 1, true, workspace,
+                                    pix_off, HW, S, max_dets, total_dets, pairs, pairs_zeroed, 1, true, workspace,
                                     workspace_bytes, stream_ptr);
 }
 
@@ -2024,9 +2094,12 @@ extern "C" int pe_coco_match(int64_t B, int64_t C, const int64_t* det_off, const
                              const uint8_t* gt_crowd, const double* gt_ann, const void* inter, int inter_bytes,
                              int64_t total_dets, int64_t max_gt, int8_t* status, int32_t* ranks, int64_t* counts,
                              const int32_t* guard, const uint8_t* gt_things, const int64_t* gt_area_i64,
-                             int64_t gt_area_stride, void* stream_ptr) {
+                             int64_t gt_area_stride, int zero_inter, void* stream_ptr) {
   PE_REQUIRE(B >= 0 && C > 0 && C < INT_MAX && B * C < INT_MAX && max_gt >= 0, "invalid matching shapes");
   PE_REQUIRE(inter_bytes == 4 || inter_bytes == 8, "intersections must be int32 or int64");
+  PE_REQUIRE(det_area || inter_bytes == 8 || total_dets == 0,
+             "detection areas may be omitted only for int64 intersection rows");
+  PE_REQUIRE(!zero_inter || inter_bytes == 8, "only int64 intersection rows are zeroed");
   PE_REQUIRE(max_gt == 0 || gt_area || gt_area_i64, "GT pixel areas are required");
   hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
   // The kernel writes every detection's rank and flags (rank 0 and flags -1
@@ -2056,6 +2129,7 @@ extern "C" int pe_coco_match(int64_t B, int64_t C, const int64_t* det_off, const
   a.gt_things = gt_things;
   a.gt_area_i64 = reinterpret_cast<const i64*>(gt_area_i64);
   a.gt_area_stride = gt_area_stride;
+  a.zero_inter = zero_inter;
   const size_t lds = 16 * 100 + 4 * (100 + static_cast<size_t>(max_gt)) + kMatchStageBytes +
                      4 * 40 * static_cast<size_t>(a.words) + 16;
   PE_REQUIRE(lds <= 150 * 1024, "too many ground-truth instances per image for the matcher");

@@ -48,11 +48,11 @@ _SIGNATURES = {
     "pe_pack": (_int, [_ptr, _int, _i64, _i64, _ptr, _ptr, _i64, _ptr, _ptr, _ptr]),
     "pe_mask_overlap": (_int, [_ptr, _i64, _ptr, _i64, _i64, _ptr, _ptr, _ptr, _ptr]),
     "pe_coco_match": (_int, [_i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _int,
-                             _i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _ptr]),
+                             _i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _int, _ptr]),
     "pe_ap_workspace_bytes": (_i64, [_i64, _i64]),
     "pe_ap_compute": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _ptr, _ptr, _ptr, _ptr, _i64, _ptr]),
     "pe_detection_histogram": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _i64, _ptr, _int,
-                                      _ptr, _i64, _ptr]),
+                                      _int, _ptr, _i64, _ptr]),
     "pe_prefix_workspace_bytes": (_i64, [_i64] * 7),
     "pe_prefix": (_int, [_i64, _ptr, _ptr, _i64, _i64, _i64, _ptr, _ptr, _i64, _i64, _i64, _ptr, _ptr, _ptr, _ptr,
                          _ptr, _i64, _ptr, _ptr, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr,
@@ -242,7 +242,7 @@ def _match(inter, inter_bytes, pc, gc, pa, ga, crowd, areas, C):
     det_off, gt_off, inter_off = _table([0, D], device), _table([0, G], device), _table([0, D * G], device)
     _call("pe_coco_match", 1, C, det_off.data_ptr(), gt_off.data_ptr(), inter_off.data_ptr(), pc.data_ptr(),
           pa.data_ptr(), None, gc.data_ptr(), ga.data_ptr(), crowd.data_ptr(), areas.data_ptr(), inter.data_ptr(),
-          inter_bytes, D, G, status.data_ptr(), ranks.data_ptr(), counts.data_ptr(), None, None, None, 0,
+          inter_bytes, D, G, status.data_ptr(), ranks.data_ptr(), counts.data_ptr(), None, None, None, 0, 0,
           _stream(device))
     # The library writes one 40-flag row per detection; the reference's
     # layout is 40 x D (area * 10 + threshold, detection).
@@ -317,7 +317,7 @@ def instance_histogram(masks, target, order, S):
         det_off = _table([0, D], masks.device)
         aligned = int(HW % 16 == 0 and masks.data_ptr() % 16 == 0)
         _call("pe_detection_histogram", 1, pointers.data_ptr(), det_off.data_ptr(), order.data_ptr(),
-              target.data_ptr(), None, HW, S, D, D, pairs.data_ptr(), aligned, None, 0, _stream(masks.device))
+              target.data_ptr(), None, HW, S, D, D, pairs.data_ptr(), 0, aligned, None, 0, _stream(masks.device))
     return pairs
 
 

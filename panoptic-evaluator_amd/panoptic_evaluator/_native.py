@@ -17,7 +17,7 @@ import weakref
 import torch
 
 LIBRARY_PATH = pathlib.Path(__file__).with_name("libpanoptic_hip.so")
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 _i64, _int, _ptr, _dbl = ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_double
 

@@ -6,14 +6,16 @@ the detection x target-slot histogram (one launch), the device-side input
 checks (one host read when validating), the panoptic update, and one COCO
 matching launch; AP ground truth is exactly the target's thing segments.
 
-The panoptic branch (joint histogram and segment scoring) and the instance
-branch (score order and detection histogram) share no data until the matcher.
-While an update is captured into a HIP graph (validate=False), the panoptic
-branch is launched on a second stream of default priority that joins the
-caller's stream before the matcher, so the graph holds the two branches side
-by side.  Eager updates keep one stream: a cross-stream event wait costs more
-on the host and device than the overlap saves at these sizes.  Every tensor is
-allocated on the caller's stream.
+The detection x slot histogram counts rows in mask order, so it needs
+neither the score order nor the panoptic branch (joint histogram and segment
+scoring); the matcher joins them.  While an update is captured into a HIP
+graph (validate=False), the panoptic branch and the score order are launched
+on a second stream of default priority that joins the caller's stream before
+the matcher, so the graph holds them beside the detection histogram.  Eager
+updates keep one stream: a cross-stream event wait costs more on the host and
+device than the overlap saves at these sizes.  The detection x slot table is
+kept between updates and returned all-zero by the matcher, so no fill runs
+per update.
 """
 
 from __future__ import annotations
@@ -40,6 +42,8 @@ class SegmentationEvaluator:
         self.validate = validate
         self._error = torch.zeros(1, dtype=torch.int32, device=self.device)
         self._side = None
+        self._pairs = None
+        self._pairs_clean = False
         # The panoptic accumulators and the AP positive counts share one
         # buffer, so reset() is a single fill.
         words = PanopticEvaluator.state_words(num_classes)
@@ -47,6 +51,17 @@ class SegmentationEvaluator:
         self.panoptic._bind_state(shared[:words])
         self.instance._counts = shared[words:].view(num_classes, 4)
         self._shared = shared
+
+    def _pair_table(self, N, S):
+        """The N x S detection x slot table, a view of a buffer kept between
+        updates (grown when a batch needs more rows): every update leaves its
+        rows all-zero (the matcher zeroes the rows it consumed), so no fill
+        and no allocation run per update."""
+        if self._pairs is None or self._pairs.shape[1] != S or self._pairs.shape[0] < N:
+            rows = N if self._pairs is None or self._pairs.shape[1] != S else max(N, 2 * self._pairs.shape[0])
+            self._pairs = torch.zeros((max(rows, 1), S), dtype=torch.int64, device=self.device)
+            self._pairs_clean = True
+        return self._pairs[:N]
 
     def _side_stream(self):
         if self._side is None:
@@ -79,21 +94,26 @@ class SegmentationEvaluator:
             raise ValueError("detections and target must have matching image counts and dimensions")
         self._validate_areas(instance_areas, classes)
         device, C = self.device, self.num_classes
+        B, S = len(sizes), classes.shape[1]
+        counts = [image.masks.shape[0] for image in prepared]
+        N = sum(counts)
+        det_off = offsets(counts, device)
         fork = not self.validate and torch.cuda.is_current_stream_capturing()
+        stream = _native.stream(device)
+        ranked = None
         if fork:
-            # The panoptic branch needs nothing the instance branch makes; its
-            # scratch is allocated once per shape and kept by the evaluator.
+            # The panoptic branch and the score order need nothing the
+            # detection histogram makes: they run on a second stream beside it
+            # (the panoptic scratch is allocated once per shape and kept).
             side = self._side_stream()
             areas = self.panoptic._update(prediction, target, return_areas=True, side=side)
-        stream = _native.stream(device)
-        B, S = len(sizes), classes.shape[1]
-        ranked = score_order(prepared, device)
-        det_counts, N, det_off = ranked.counts, sum(ranked.counts), ranked.det_off
+            with torch.cuda.stream(side.stream):
+                ranked = score_order(prepared, device)
         bits = any(isinstance(image.masks, PackedMasks) for image in prepared)
         masks = [as_bits(image.masks) for image in prepared] if bits else [image.masks for image in prepared]
         pointers = _native.device_table([m.data_ptr() for m in masks], device)
         aligned = int(all((h * w) % 16 == 0 for h, w in sizes) and all(m.data_ptr() % 16 == 0 for m in masks))
-        pairs = torch.empty((N, S), dtype=torch.int64, device=device)
+        pairs = self._pair_table(N, S)
         if target.packed:
             # Packed images: pixel offsets, the pixel total and a tile-list workspace.
             pix_off = offsets([h * w for h, w in sizes], device)
@@ -102,20 +122,26 @@ class SegmentationEvaluator:
             workspace = torch.empty(max(size, 256), dtype=torch.uint8, device=device)
         else:
             pix_off, HW, workspace = None, sizes[0][0] * sizes[0][1], None
-        common = (B, pointers.data_ptr(), det_off.data_ptr(), ranked.order.data_ptr(), maps.data_ptr(),
-                  _native.ptr(pix_off), HW, S, max(det_counts, default=0), N, pairs.data_ptr())
+        # Rows in mask order (order NULL): the matcher maps each ranked
+        # detection to its row, so the histogram does not wait for the sort.
+        common = (B, pointers.data_ptr(), det_off.data_ptr(), None, maps.data_ptr(), _native.ptr(pix_off), HW, S,
+                  max(counts, default=0), N, pairs.data_ptr(), int(self._pairs_clean))
         ws = (_native.ptr(workspace), 0 if workspace is None else workspace.numel(), stream)
+        self._pairs_clean = False
         if bits:
             # Bit rows (bool images are packed first): 1/8 of the mask bytes.
             _native.call("pe_detection_histogram_bits", *common, *ws)
         else:
             _native.call("pe_detection_histogram", *common, aligned, *ws)
-        # Full detection areas, including void and stuff pixels.
-        det_area = torch.empty(N, dtype=torch.float64, device=device)
-        _native.call("pe_row_sums", pairs.data_ptr(), N, S, det_area.data_ptr(), stream)
+        if ranked is None:
+            ranked = score_order(prepared, device)
         det_cls, scores, det_cls32 = ranked.classes, ranked.scores, ranked.classes32
         inst = None if instance_areas is None else instance_areas.double().contiguous()
         if self.validate:
+            # Full detection areas (row sums, mask order: only emptiness is
+            # checked here; the matcher sums the rows it ranks itself).
+            det_area = torch.empty(N, dtype=torch.float64, device=device)
+            _native.call("pe_row_sums", pairs.data_ptr(), N, S, det_area.data_ptr(), stream)
             _native.call("pe_instance_checks", N, det_cls.data_ptr(), scores.data_ptr(), det_area.data_ptr(), 0,
                          None, None, None, C, self.panoptic.isthing.data_ptr(),
                          0 if inst is None else inst.numel(), _native.ptr(inst), self._error.data_ptr(), stream)
@@ -132,12 +158,16 @@ class SegmentationEvaluator:
         status = torch.empty((N, 40), dtype=torch.int8, device=device)
         ranks = torch.empty(N, dtype=torch.int32, device=device)
         gt_off = _native.device_table(range(0, (B + 1) * S, S), device)
-        inter_off = offsets([d * S for d in det_counts], device)
+        inter_off = offsets([d * S for d in counts], device)
+        # Detection areas are the matcher's own row sums (det_area NULL), and
+        # it zeroes every row it consumed (zero_inter), so the table is clean
+        # for the next update.
         _native.call("pe_coco_match", B, C, det_off.data_ptr(), gt_off.data_ptr(), inter_off.data_ptr(),
-                     det_cls32.data_ptr(), det_area.data_ptr(), None, classes.data_ptr(), None,
+                     det_cls32.data_ptr(), None, ranked.order.data_ptr(), classes.data_ptr(), None,
                      _native.ptr(crowd), _native.ptr(inst), pairs.data_ptr(), 8, N, S, status.data_ptr(),
                      ranks.data_ptr(), self.instance._counts.data_ptr(), None, self.panoptic.isthing.data_ptr(),
-                     areas.data_ptr(), areas.stride(0), stream)
+                     areas.data_ptr(), areas.stride(0), 1, stream)
+        self._pairs_clean = True
         self.instance._commit((scores, det_cls32, ranks, status))
         return self
 

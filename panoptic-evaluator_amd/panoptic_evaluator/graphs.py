@@ -35,6 +35,18 @@ def _parts(evaluator):
     raise TypeError("expected a PanopticEvaluator, InstanceEvaluator or SegmentationEvaluator")
 
 
+def _scratch_of(evaluator):
+    """The per-shape scratch tensors an evaluator keeps between updates."""
+    from .combined import SegmentationEvaluator
+    keep = []
+    panoptic, _ = _parts(evaluator)
+    if panoptic is not None:
+        keep += [panoptic._scratch, panoptic._workspace, panoptic._default_crowd]
+    if isinstance(evaluator, SegmentationEvaluator):
+        keep.append(evaluator._pairs)
+    return keep
+
+
 class CapturedUpdate:
     """One update() of ``evaluator`` recorded as a HIP graph."""
 
@@ -69,6 +81,10 @@ class CapturedUpdate:
             with _native.pinned_tables() as tables, torch.cuda.graph(self.graph):
                 evaluator.update(*args, **kwargs)
             self._tables = tables
+            # Scratch the evaluator allocated outside the capture (per-shape
+            # buffers, kept between updates) is read by the graph's kernels by
+            # address: keep it alive even if the evaluator later replaces it.
+            self._scratch = _scratch_of(evaluator)
         finally:
             if instance is not None:
                 del instance._commit  # back to the class's method

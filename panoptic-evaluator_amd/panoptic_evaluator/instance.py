@@ -26,12 +26,12 @@ for _name, _sig in {
     "pe_mask_overlap_bits_batch": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _i64, _i64,
                                           _i64, _i64, _ptr, _ptr, _ptr, _ptr]),
     "pe_detection_histogram_bits": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _i64, _ptr,
-                                           _ptr, _i64, _ptr]),
+                                           _int, _ptr, _i64, _ptr]),
     "pe_detection_histogram_workspace_bytes": (_i64, [_i64, _i64]),
     "pe_detection_histogram": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _i64, _ptr, _int,
-                                      _ptr, _i64, _ptr]),
+                                      _int, _ptr, _i64, _ptr]),
     "pe_coco_match": (_int, [_i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _int,
-                             _i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _ptr]),
+                             _i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _int, _ptr]),
     "pe_score_order_workspace_bytes": (_i64, [_i64, _i64]),
     "pe_score_order": (_int, [_i64, _i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _ptr]),
     "pe_score_order_images": (_int, [_i64, _i64, _ptr, _ptr, _ptr, _int, _ptr, _ptr, _ptr, _ptr, _ptr]),
@@ -386,7 +386,7 @@ class InstanceEvaluator:
                      inter_off.data_ptr(), det_cls32.data_ptr(), det_area.data_ptr(), det_row.data_ptr(),
                      gt_cls32.data_ptr(), gt_area.data_ptr(), crowd.data_ptr(), _native.ptr(ann), inter.data_ptr(), 4,
                      N, max(gt_counts, default=0), status.data_ptr(), ranks.data_ptr(), self._counts.data_ptr(),
-                     None, None, None, 0, stream)
+                     None, None, None, 0, 0, stream)
         self._commit((scores, det_cls32, ranks, status))
         return self
 

@@ -171,7 +171,7 @@ def test_config2_invariants_whole_batch(pe, c2):
     workspace = torch.empty(size, dtype=torch.uint8, device=device)
     pairs = torch.empty((N, S), dtype=torch.int64, device=device)
     _native.call("pe_detection_histogram", len(counts), pointers.data_ptr(), det_off.data_ptr(), order.data_ptr(),
-                 target.segment_map.data_ptr(), pix_off.data_ptr(), HW, S, max(counts), N, pairs.data_ptr(), 1,
+                 target.segment_map.data_ptr(), pix_off.data_ptr(), HW, S, max(counts), N, pairs.data_ptr(), 0, 1,
                  workspace.data_ptr(), workspace.numel(), _native.stream(device))
     mask_areas = torch.cat([m.flatten(1).sum(1) for m in c2.masks])
     assert torch.equal(pairs.sum(1), mask_areas)

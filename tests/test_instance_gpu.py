@@ -88,7 +88,7 @@ def test_golden_coco_matcher(pe):
         _native.call("pe_coco_match", 1, C, det_off.data_ptr(), gt_off.data_ptr(), inter_off.data_ptr(),
                      args[0].data_ptr(), args[1].data_ptr(), None, args[2].data_ptr(), args[3].data_ptr(),
                      args[4].data_ptr(), args[5].data_ptr(), pairs.data_ptr(), 8, D, G, status.data_ptr(),
-                     ranks.data_ptr(), counts.data_ptr(), None, None, None, 0, _native.stream())
+                     ranks.data_ptr(), counts.data_ptr(), None, None, None, 0, 0, _native.stream())
         assert_exact(status.t(), data[f"out{index}.status"], f"status case {index}")
         assert_exact(ranks, data[f"out{index}.ranks"], f"ranks case {index}")
         assert_exact(counts, data[f"out{index}.counts"], f"counts case {index}")
