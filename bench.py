@@ -395,6 +395,10 @@ class ServingConfig:
         # interleave the two kinds
         half = len(self.pool) // 2
         self.pool = [r for pair in zip(self.pool[:half], self.pool[half:]) for r in pair]
+        # Algorithmic bytes of the prefix event pass per request: the target map
+        # and every mask once, H * W * (4 + K) per image (0 for PQ requests).
+        self.prefix_bytes = [sum(m.shape[1] * m.shape[2] * (4 + m.shape[0]) for m in r.prediction.masks)
+                             if isinstance(r, PrefixRequest) else 0 for r in self.pool]
         self.config = {"workload": self.description, "pool_requests": len(self.pool),
                        "pool_images": sum(len(images) for images in pq) + sum(len(images) for images, _ in prefix),
                        "classes": 133, "validate": False}
@@ -408,6 +412,7 @@ class ServingConfig:
         arrivals = np.cumsum(rng.exponential(1.0 / rate, requests))
         order = [self.pool[i % len(self.pool)] for i in range(requests)]
         server = RewardServer(133, isthing=self.isthing, hbm_budget=budget, streams=streams)
+        streamed = sum(self.prefix_bytes[i % len(self.pool)] for i in range(requests))
         lib = _native.library()
         torch.cuda.synchronize()
         lib.pe_profile_reset()
@@ -419,18 +424,21 @@ class ServingConfig:
             while i < requests and arrivals[i] <= now:
                 tickets.append(server.submit(order[i], arrival=start + arrivals[i]))
                 i += 1
-            server.poll()
-            if server.idle and i < requests:
-                time.sleep(max(0.0, min(0.0005, arrivals[i] - (time.perf_counter() - start))))
+            # Collect results, waiting at most until the next arrival is due.
+            wait = arrivals[i] - (time.perf_counter() - start) if i < requests else 0.002
+            server.poll(wait_us=int(max(0.0, min(wait, 0.002)) * 1e6))
         elapsed = max(t.completed for t in tickets) - start
+        stats = dict(launches=server.launches, launched=server.launched_requests,
+                     blocked=server.admission.blocked)
+        server.close()
         torch.cuda.synchronize()
         lib.pe_profile_enable(0)
         kernel_ms, launches = _native.profile_read(self.dominant)
         latencies = np.array([t.latency for t in tickets]) * 1e3
         images = sum(t.images for t in tickets)
         return dict(images=images, elapsed=elapsed, latencies=latencies, kernel_ms=kernel_ms, launches=launches,
-                    streamed=server.prefix_pixels_streamed, batches=dict(server.launches), tickets=tickets,
-                    admission_waits=server.admission.blocked)
+                    streamed=streamed, batches=dict(stats["launches"]), tickets=tickets,
+                    admission_waits=stats["blocked"], launched_requests=stats["launched"])
 
     def cpu_sample(self, budget_s):
         from oracle import panoptic as opan
@@ -637,6 +645,7 @@ def serve(cfg, args, *, requests=None, rate=None, cpu_budget=None):
                "sample": sample, "host_threads_available": os.cpu_count()}
     batches = out["batches"]
     launches = sum(batches.values())
+    launched = out.get("launched_requests") or requests
     return {
         "metric": METRIC,
         "value": round(out["images"] / out["elapsed"], 3),
@@ -658,7 +667,7 @@ def serve(cfg, args, *, requests=None, rate=None, cpu_budget=None):
                        "mean": round(float(lat.mean()), 3), "max": round(float(lat.max()), 3)},
         "throughput_rps": round(requests / out["elapsed"], 3),
         "batches": batches,
-        "requests_per_launch": round(requests / max(launches, 1), 3),
+        "requests_per_launch": round(launched / max(launches, 1), 3),
         "admission_waits": out.get("admission_waits"),
         "roofline": with_ceiling({
             "bound": "hbm",

@@ -394,6 +394,62 @@ int pe_prefix(int64_t B, const int32_t* target, const int64_t* pix_off, int64_t 
               double* precision, double* recall, int32_t* error, void* workspace,
               int64_t workspace_bytes, int aligned, void* stream);
 
+/* ------------------------------------------------------------- serving ---- */
+
+/* Reward server (no counterpart in the reference: the serving form of
+ * pe_update(reward) + pe_image_pq and pe_prefix + pe_compute).  A worker
+ * thread admits requests first-in first-out into a device arena of
+ * budget_bytes (the HBM budget, allocated once), merges every waiting request
+ * of a kind that fits into one packed launch on the next of `streams` streams
+ * of default priority, and completes them from their events; a request larger
+ * than the whole budget runs alone in memory of its own.  Inputs stay in the
+ * caller's device memory until the request completes; they are not validated.
+ *   pe_server_create: things (C uint8) and thresholds (T f64) are HOST
+ *     arrays; pinned_bytes sizes the pinned staging block; the budget is
+ *     capped at 80% of the device memory free at creation.  NULL on error.
+ *   pe_server_submit: kind 0 = per-image PQ, 1 = prefix rewards; `images` is a
+ *     HOST array of B x 9 int64 per image: target map, then PQ: prediction map,
+ *     target classes, prediction classes / prefix: N x H x W bool masks,
+ *     generated classes (int32), target classes; then target crowd flags (or
+ *     0), H, W, G (target slots), P (prediction slots) or N (masks).  Maps are
+ *     H x W int32 compact slots.  Prefix requests give each image's strictly
+ *     increasing checkpoints (HOST `points`, `point_counts` per image).
+ *     arrival_ns: CLOCK_MONOTONIC submission time (0: now).  Returns the
+ *     request id (> 0) or < 0.
+ *   pe_server_poll: copies up to max_ids ids of completed requests (waiting up
+ *     to wait_us for one); returns the count.
+ *   pe_server_take: copies a completed request's result into `out` (PQ: one
+ *     f64 per image; prefix: pq, sq, rq, mIoU per checkpoint row, images in
+ *     order) and `times` (arrival, launch, completion, CLOCK_MONOTONIC ns),
+ *     and forgets the request; returns 0, > 0 when its launch failed (text in
+ *     `message`, 256 bytes), < 0 on argument errors.
+ *   pe_server_stats: 9 int64: arena bytes in use, arena capacity, requests
+ *     waiting, launches in flight, admission passes that waited for memory,
+ *     PQ launches, prefix launches, requests completed, requests launched.
+ *   pe_server_footprint: device bytes a request takes when it runs alone
+ *     (host only; no server needed). */
+void* pe_server_create(int device, int64_t budget_bytes, int streams, int64_t C, const uint8_t* things,
+                       const double* thresholds, int64_t T, int64_t pinned_bytes);
+int pe_server_destroy(void* server);
+int64_t pe_server_submit(void* server, int kind, int64_t B, const int64_t* images, const int32_t* points,
+                         const int64_t* point_counts, int64_t arrival_ns);
+int64_t pe_server_poll(void* server, int64_t* ids, int64_t max_ids, int64_t wait_us);
+int pe_server_take(void* server, int64_t id, double* out, int64_t capacity, int64_t* length, int64_t* times,
+                   char* message);
+int pe_server_stats(void* server, int64_t* out);
+int64_t pe_server_footprint(int64_t C, int64_t T, int kind, int64_t B, const int64_t* images,
+                            const int32_t* points, const int64_t* point_counts);
+
+/* The server's first-fit arena as host-only bookkeeping (offsets in 256-byte
+ * granules, freed neighbours coalesced), for callers and tests that plan
+ * memory without a device: alloc returns the offset or -1. */
+void* pe_arena_create(int64_t capacity);
+int pe_arena_destroy(void* arena);
+int64_t pe_arena_alloc(void* arena, int64_t bytes);
+int pe_arena_release(void* arena, int64_t offset, int64_t bytes);
+int64_t pe_arena_in_use(void* arena);
+int64_t pe_arena_largest_free(void* arena);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,40 +4,59 @@ Requests are per-image panoptic quality (``PQRequest``: the reference's
 ``panoptic_quality``, evaluator.py:471-544) or generation-order prefix rewards
 (``PrefixRequest``: ``segmentation_prefixes``, prefix.py:11-122).  The server
 has no counterpart in the reference; it is the serving form of the same
-kernels:
+kernels.  Its loop runs in native code (a worker thread of
+libpanoptic_hip.so, ``pe_server_*``):
 
-* **admission by HBM budget** -- each request's device footprint (scratch,
-  statistics and packed copies) is estimated on submission; requests are
-  admitted first-in first-out while the in-use total stays within the budget,
-  and their memory returns to the budget when their results are on the host;
-* **continuous batching** -- every ``poll()`` merges all newly admitted
-  requests of a kind into one packed launch (images of any size, one kernel
-  sequence), issued on the next of a few streams, so work admitted while
-  earlier batches are still running overlaps with them on the device;
+* **admission by HBM budget** -- the budget is one device arena; a batch is
+  admitted when its merged layout fits, requests wait first-in first-out
+  while it does not, and a request larger than the whole budget runs alone;
+* **continuous batching** -- every pass of the loop merges all waiting
+  requests of a kind that fit into one packed launch (images of any size) on
+  the next of a few streams, so work admitted while earlier batches run
+  overlaps with them on the device;
 * **device-resident statistics** -- per-image sufficient statistics and the
-  FP64 metrics are computed on the GPU; only the final per-image numbers are
-  copied (asynchronously, into pinned memory) to the host.
+  FP64 metrics stay on the GPU; only per-image PQ (or per-checkpoint
+  PQ/SQ/RQ/mIoU) is copied to pinned host memory.
 
-Results do not depend on how requests are batched: per-image PQ folds each
-image's IoUs in slot order and prefix snapshots are per image, so a request
-gets bit-identical numbers alone or in any mix.  Inputs are not validated
-(validation reads a result on the host, which would serialise the loop);
-callers validate with the plain API beforehand if needed.
+This module checks request shapes on the host, hands the server per-image
+device addresses, and keeps the request's tensors alive until its result is
+back.  Results do not depend on how requests are batched: per-image PQ folds
+each image's IoUs in slot order and prefix snapshots are per image, so a
+request gets bit-identical numbers alone or in any mix.  Input values are not
+validated on the device (validation reads a result on the host, which would
+serialise the loop); callers validate with the plain API beforehand if needed.
 """
 
 from __future__ import annotations
 
 import collections
+import ctypes
 import time
 from dataclasses import dataclass, field
 from typing import Sequence
 
+import numpy as np
 import torch
 
 from . import _native
-from .evaluator import PanopticBatch, _prepare, finalize_arrays, panoptic_quality
+from .evaluator import PanopticBatch
 from .instance import InstanceBatch
-from .prefix import _check_arguments, launch_prefix
+from .prefix import _check_arguments
+
+_i64, _int, _ptr = ctypes.c_int64, ctypes.c_int, ctypes.c_void_p
+for _name, _sig in {
+    "pe_server_create": (_ptr, [_int, _i64, _int, _i64, _ptr, _ptr, _i64, _i64]),
+    "pe_server_destroy": (_int, [_ptr]),
+    "pe_server_submit": (_i64, [_ptr, _int, _i64, _ptr, _ptr, _ptr, _i64]),
+    "pe_server_poll": (_i64, [_ptr, _ptr, _i64, _i64]),
+    "pe_server_take": (_int, [_ptr, _i64, _ptr, _i64, _ptr, _ptr, _ptr]),
+    "pe_server_stats": (_int, [_ptr, _ptr]),
+    "pe_server_footprint": (_i64, [_i64, _i64, _int, _i64, _ptr, _ptr, _ptr]),
+}.items():
+    _native.register(_name, *_sig)
+
+PQ, PREFIX = 0, 1
+PINNED_BYTES = 64 << 20
 
 
 @dataclass
@@ -71,6 +90,9 @@ class Ticket:
     completed: float | None = None
     result: object = None
     error: BaseException | None = None   # set instead of result when the launch failed
+    id: int = -1
+    points: list = field(default_factory=list)
+    keep: list = field(default_factory=list)
 
     @property
     def done(self) -> bool:
@@ -81,73 +103,107 @@ class Ticket:
         return None if self.completed is None else self.completed - self.arrival
 
 
-class Admission:
-    """First-in first-out admission under a byte budget.  A request larger
-    than the whole budget is admitted alone once nothing else is in use."""
+@dataclass
+class AdmissionStats:
+    """The server's admission state (from the native loop)."""
 
-    def __init__(self, budget: int):
-        if budget <= 0:
-            raise ValueError("the HBM budget must be positive")
-        self.budget = int(budget)
-        self.in_use = 0
-        self.running = 0
-        self.blocked = 0   # admission passes at which the head request waited for memory
-        self.waiting: collections.deque = collections.deque()
-
-    def offer(self, ticket: Ticket):
-        self.waiting.append(ticket)
-
-    def admit(self) -> list:
-        admitted = []
-        while self.waiting:
-            ticket = self.waiting[0]
-            fits = self.in_use + ticket.footprint <= self.budget
-            if not fits and not (self.running == 0 and not admitted):
-                self.blocked += 1
-                break
-            self.waiting.popleft()
-            self.in_use += ticket.footprint
-            self.running += 1
-            admitted.append(ticket)
-        return admitted
-
-    def release(self, ticket: Ticket):
-        self.in_use -= ticket.footprint
-        self.running -= 1
+    budget: int
+    in_use: int = 0
+    waiting: int = 0
+    running: int = 0       # launches in flight
+    blocked: int = 0       # loop passes at which the head request waited for memory
 
 
-def _images(batch: PanopticBatch, *, crowd: bool):
-    """Per-image (H x W map, classes row, crowd row or None, (H, W)) views."""
+def _image_views(batch: PanopticBatch, *, crowd: bool):
+    """Per-image (H x W int32 map, classes row, crowd row or None, (H, W))
+    views of a compact batch (no copies)."""
     batch = batch.compact()
     maps, classes = batch.segment_map, batch.classes
+    if maps.dtype != torch.int32 or classes.dtype != torch.int32 or classes.ndim != 2:
+        raise ValueError("served maps must be compact int32 slots with int32 B x S classes")
+    if batch.crowd is not None and (batch.crowd.dtype != torch.bool or batch.crowd.shape != classes.shape):
+        raise ValueError("crowd must be bool with the same shape as classes")
+    maps, classes = maps.contiguous(), classes.contiguous()
+    flags = batch.crowd.contiguous() if crowd and batch.crowd is not None else None
     rows = []
     if batch.packed:
+        if maps.ndim != 1 or classes.shape[0] != len(batch.image_sizes) or \
+                maps.numel() != sum(h * w for h, w in batch.image_sizes):
+            raise ValueError("a packed batch needs flat maps of sum(H_i * W_i) pixels and classes B x S")
         start = 0
         for b, (h, w) in enumerate(batch.image_sizes):
-            rows.append((maps[start:start + h * w].view(h, w), classes[b],
-                         batch.crowd[b] if crowd and batch.crowd is not None else None, (h, w)))
+            rows.append((maps[start:start + h * w], classes[b], None if flags is None else flags[b], (h, w)))
             start += h * w
     else:
+        if maps.ndim != 3 or classes.shape[0] != maps.shape[0]:
+            raise ValueError("expected maps B x H x W and classes B x S")
         for b in range(maps.shape[0]):
-            rows.append((maps[b], classes[b], batch.crowd[b] if crowd and batch.crowd is not None else None,
-                         tuple(maps.shape[1:])))
-    return rows
+            rows.append((maps[b], classes[b], None if flags is None else flags[b], tuple(maps.shape[1:])))
+    return rows, [maps, classes, flags]
 
 
-def _crowd_rows(rows):
-    return [c if c is not None else torch.zeros_like(k, dtype=torch.bool) for _, k, c, _ in rows]
+def _points(checkpoints, n):
+    if checkpoints is None:
+        return list(range(1, n + 1)) if n else [0]
+    return sorted(set(checkpoints))
 
 
-class _Launch:
-    """One merged launch in flight: its tickets, device outputs kept alive,
-    pinned host copies and the completion event."""
+def describe(request, num_classes: int, thresholds):
+    """(kind, per-image descriptors int64 [B, 9], checkpoints per image,
+    tensors to keep alive) of a request; raises ValueError for requests the
+    server cannot evaluate.  Host-side shape checks only."""
+    if isinstance(request, PQRequest):
+        pred, keep_p = _image_views(request.prediction, crowd=False)
+        tgt, keep_t = _image_views(request.target, crowd=True)
+        if [s for *_, s in pred] != [s for *_, s in tgt]:
+            raise ValueError("prediction and target must have matching image sizes")
+        if not tgt:
+            raise ValueError("a request needs at least one image")
+        device = request.target.segment_map.device
+        rows = []
+        for (pm, pc, _, (h, w)), (gm, gc, gcrowd, _) in zip(pred, tgt):
+            if pm.device != device or gm.device != device:
+                raise ValueError("batch tensors must share a device")
+            rows.append([gm.data_ptr(), pm.data_ptr(), gc.data_ptr(), pc.data_ptr(),
+                         0 if gcrowd is None else gcrowd.data_ptr(), h, w, gc.numel(), pc.numel()])
+        return PQ, np.asarray(rows, np.int64), [], keep_p + keep_t
+    if isinstance(request, PrefixRequest):
+        _, requested = _check_arguments(thresholds, request.checkpoints)
+        tgt, keep_t = _image_views(request.target, crowd=True)
+        masks, classes = request.prediction.masks, request.prediction.classes
+        if len(masks) != len(tgt) or len(classes) != len(masks) or not tgt:
+            raise ValueError("prediction and target must have matching batch and image dimensions")
+        rows, points, keep = [], [], list(keep_t)
+        for m, c, (gm, gc, gcrowd, (h, w)) in zip(masks, classes, tgt):
+            if m.ndim != 3 or m.dtype != torch.bool or tuple(m.shape[1:]) != (h, w):
+                raise ValueError("masks must be bool N x H x W matching their target image")
+            if c.shape != (m.shape[0],) or c.dtype not in (torch.int32, torch.int64):
+                raise ValueError("classes must be int32/int64 N tensors")
+            if requested and requested[-1] > m.shape[0]:
+                raise ValueError("checkpoint exceeds prediction count")
+            m, c = m.contiguous(), c.to(torch.int32).contiguous()
+            keep += [m, c]
+            rows.append([gm.data_ptr(), m.data_ptr(), c.data_ptr(), gc.data_ptr(),
+                         0 if gcrowd is None else gcrowd.data_ptr(), h, w, gc.numel(), m.shape[0]])
+            points.append(_points(requested, m.shape[0]))
+        return PREFIX, np.asarray(rows, np.int64), points, keep
+    raise TypeError("requests are PQRequest or PrefixRequest")
 
-    def __init__(self, kind, tickets, stream):
-        self.kind, self.tickets, self.stream = kind, tickets, stream
-        self.event = torch.cuda.Event()
-        self.keep = []
-        self.host = None
-        self.split = None
+
+def _flat_points(points):
+    counts = np.asarray([len(p) for p in points], np.int64)
+    flat = np.asarray([q for p in points for q in p] or [0], np.int32)
+    return flat, counts
+
+
+def footprint(request, num_classes: int, iou_thresholds=(0.5, 0.75)) -> int:
+    """Device bytes the server reserves for ``request`` when it runs alone."""
+    thresholds, _ = _check_arguments(iou_thresholds, None)
+    kind, rows, points, _ = describe(request, num_classes, thresholds)
+    flat, counts = _flat_points(points)
+    size = _native.library().pe_server_footprint(num_classes, len(thresholds), kind, len(rows),
+                                                 rows.ctypes.data, flat.ctypes.data, counts.ctypes.data)
+    return _native.check(size)
 
 
 class RewardServer:
@@ -162,233 +218,119 @@ class RewardServer:
             raise ValueError("isthing must contain one flag per category")
         if streams < 1:
             raise ValueError("at least one stream is needed")
+        if hbm_budget <= 0:
+            raise ValueError("the HBM budget must be positive")
         self.num_classes = num_classes
         self.device = _native.require_gpu(device)
-        self.things = torch.tensor(list(isthing) if isthing is not None else [True] * num_classes,
-                                   dtype=torch.bool, device=self.device)
+        if self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
         self.thresholds, _ = _check_arguments(iou_thresholds, None)
-        self.admission = Admission(hbm_budget)
-        # Default-priority streams only.
-        self.streams = [torch.cuda.Stream(self.device) for _ in range(streams)]
-        self._next_stream = 0
-        self._pending: list = []
-        self._in_flight: collections.deque = collections.deque()
-        self.launches = collections.Counter()
-        self.prefix_pixels_streamed = 0   # algorithmic bytes of the prefix event passes
+        things = np.asarray(list(isthing) if isthing is not None else [True] * num_classes, np.uint8)
+        thresholds = np.asarray(self.thresholds, np.float64)
+        lib = _native.library()
+        handle = lib.pe_server_create(self.device.index, int(hbm_budget), int(streams), num_classes,
+                                      things.ctypes.data, thresholds.ctypes.data, len(thresholds), PINNED_BYTES)
+        if not handle:
+            raise RuntimeError(f"panoptic HIP kernels: {lib.pe_last_error().decode(errors='replace')}")
+        self._handle = handle
+        self.budget = int(hbm_budget)
+        self._tickets: dict[int, Ticket] = {}
+        self._ids = (ctypes.c_int64 * 256)()
 
-    # ------------------------------------------------------------ footprint
+    def close(self):
+        """Finish every submitted request and release the server."""
+        if getattr(self, "_handle", None):
+            self.drain()
+            _native.library().pe_server_destroy(self._handle)
+            self._handle = None
 
-    def _footprint(self, request) -> tuple[int, int]:
-        """(images, estimated device bytes) of a request."""
-        C = self.num_classes
-        if isinstance(request, PQRequest):
-            tgt, pred = request.target, request.prediction
-            B = len(tgt.image_sizes) if tgt.packed else tgt.segment_map.shape[0]
-            G, P = tgt.classes.shape[1], pred.classes.shape[1]
-            pixels = tgt.segment_map.numel()
-            bytes_ = (8 * B * G * P + 8 * B * (G + P) + 4 * B * G + 32 * B * C + 12 * B * P + 8 * pixels
-                      + 8 * B * C * 4)
-            return B, int(bytes_)
-        if isinstance(request, PrefixRequest):
-            tgt = request.target
-            B = len(request.prediction.masks)
-            G, T = tgt.classes.shape[1], len(self.thresholds)
-            counts = [m.shape[0] for m in request.prediction.masks]
-            rows = sum(len(self._points(request, n)) for n in counts)
-            sum_n, sum_tri = sum(counts), sum(n * (n + 1) // 2 for n in counts)
-            pixels = sum(m.shape[1] * m.shape[2] for m in request.prediction.masks)
-            workspace = _native.library().pe_prefix_workspace_bytes(B, G, C, T, sum_n, sum_tri, pixels)
-            per_row = 8 * (4 * C + C * (C + 1)) + 40 * T * C + 8 * (4 * C + 13)
-            return B, int(workspace + rows * per_row + 4 * pixels + 16 * B * G)
-        raise TypeError("requests are PQRequest or PrefixRequest")
-
-    @staticmethod
-    def _points(request, n):
-        if request.checkpoints is None:
-            return list(range(1, n + 1)) if n else [0]
-        return sorted(set(request.checkpoints))
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
 
     # ------------------------------------------------------------ interface
 
     def submit(self, request, arrival: float | None = None) -> Ticket:
-        """Queue a request; returns its ticket (``result`` filled on completion)."""
-        if isinstance(request, PrefixRequest) and request.checkpoints is not None:
-            _, requested = _check_arguments(self.thresholds, request.checkpoints)
-            if requested and any(requested[-1] > m.shape[0] for m in request.prediction.masks):
-                raise ValueError("checkpoint exceeds prediction count")
-        images, footprint = self._footprint(request)
-        ticket = Ticket(request, images, footprint, time.perf_counter() if arrival is None else arrival)
-        self.admission.offer(ticket)
+        """Queue a request; returns its ticket (``result`` filled on
+        completion).  Malformed requests raise ValueError here, before
+        anything is queued."""
+        kind, rows, points, keep = describe(request, self.num_classes, self.thresholds)
+        flat, counts = _flat_points(points)
+        arrival = time.perf_counter() if arrival is None else arrival
+        ticket = Ticket(request, len(rows), 0, arrival, points=points, keep=keep)
+        # Work already queued on the caller's stream (inputs, packing) comes
+        # first: the server's streams run independently of it.
+        torch.cuda.current_stream(self.device).synchronize()
+        ticket.id = _native.check(_native.library().pe_server_submit(
+            self._handle, kind, len(rows), rows.ctypes.data, flat.ctypes.data, counts.ctypes.data,
+            int(arrival * 1e9)))
+        self._tickets[ticket.id] = ticket
         return ticket
 
     @property
     def idle(self) -> bool:
-        return not self.admission.waiting and not self._pending and not self._in_flight
+        return not self._tickets
 
-    def poll(self) -> int:
-        """Admit, launch and complete without blocking; returns how many
-        requests completed."""
-        now = time.perf_counter()
-        for ticket in self.admission.admit():
-            ticket.admitted = now
-            self._pending.append(ticket)
-        if self._pending:
-            pq = [t for t in self._pending if isinstance(t.request, PQRequest)]
-            prefix = [t for t in self._pending if isinstance(t.request, PrefixRequest)]
-            self._pending = []
-            done = 0
-            for tickets, launch in ((pq, self._launch_pq), (prefix, self._launch_prefix)):
-                if not tickets:
-                    continue
-                try:
-                    launch(tickets)
-                except Exception as error:  # noqa: BLE001 - reported on the tickets
-                    done += self._fail(tickets, error)
-            return done + self._complete()
-        return self._complete()
-
-    def _fail(self, tickets, error) -> int:
-        """A launch that raised: its tickets complete with the error and give
-        their memory back, and the server keeps serving."""
-        now = time.perf_counter()
-        for ticket in tickets:
-            ticket.error = error
-            ticket.completed = now
-            self.admission.release(ticket)
-        return len(tickets)
+    def poll(self, wait_us: int = 0) -> int:
+        """Collect finished requests without blocking (or up to ``wait_us``
+        microseconds); returns how many completed."""
+        n = _native.library().pe_server_poll(self._handle, self._ids, len(self._ids), int(wait_us))
+        for k in range(n):
+            self._finish(self._tickets.pop(self._ids[k]))
+        return n
 
     def drain(self):
-        """Poll until every submitted request has completed."""
-        while not self.idle:
-            if not self.poll() and self._in_flight:
-                self._in_flight[0].event.synchronize()
+        """Wait until every submitted request has completed."""
+        while self._tickets:
+            self.poll(wait_us=2000)
 
-    # ------------------------------------------------------------ launches
+    @property
+    def admission(self) -> AdmissionStats:
+        s = self._stats()
+        return AdmissionStats(budget=int(s[1]), in_use=int(s[0]), waiting=int(s[2]), running=int(s[3]),
+                              blocked=int(s[4]))
 
-    def _stream(self):
-        stream = self.streams[self._next_stream]
-        self._next_stream = (self._next_stream + 1) % len(self.streams)
-        return stream
+    @property
+    def launches(self) -> collections.Counter:
+        s = self._stats()
+        return collections.Counter({"pq": int(s[5]), "prefix": int(s[6])})
 
-    @torch.no_grad()
-    def _launch_pq(self, tickets):
-        stream = self._stream()
-        launch = _Launch("pq", tickets, stream)
-        stream.wait_stream(torch.cuda.current_stream(self.device))
-        with torch.cuda.stream(stream):
-            preds, targets, split = [], [], []
-            if len(tickets) == 1:
-                # A batch of one request is evaluated as submitted (no repacking
-                # copy); per-image results do not depend on the batch.
-                prediction, target = tickets[0].request.prediction, tickets[0].request.target
-                split.append(target.segment_map.shape[0] if not target.packed else len(target.image_sizes))
-            else:
-                for ticket in tickets:
-                    p = _images(ticket.request.prediction, crowd=False)
-                    g = _images(ticket.request.target, crowd=True)
-                    if [s for *_, s in p] != [s for *_, s in g]:
-                        raise ValueError("prediction and target must have matching image sizes")
-                    preds += p
-                    targets += g
-                    split.append(len(g))
-                prediction = PanopticBatch.pack([m for m, *_ in preds], [k for _, k, *_ in preds])
-                target = PanopticBatch.pack([m for m, *_ in targets], [k for _, k, *_ in targets],
-                                            _crowd_rows(targets))
-            pq = panoptic_quality(prediction, target, self.num_classes, validate=False)
-            launch.host = torch.empty(pq.shape, dtype=pq.dtype, pin_memory=True)
-            launch.host.copy_(pq, non_blocking=True)
-            launch.event.record(stream)
-            launch.keep = [prediction, target, pq]
-        launch.split = split
-        self._started(launch)
+    @property
+    def launched_requests(self) -> int:
+        return int(self._stats()[8])
 
-    @torch.no_grad()
-    def _launch_prefix(self, tickets):
-        stream = self._stream()
-        launch = _Launch("prefix", tickets, stream)
-        stream.wait_stream(torch.cuda.current_stream(self.device))
-        with torch.cuda.stream(stream):
-            masks, classes, targets, points, split = [], [], [], [], []
-            for ticket in tickets:
-                request = ticket.request
-                g = _images(request.target, crowd=True)
-                if len(g) != len(request.prediction.masks) or any(
-                        tuple(m.shape[1:]) != s for m, (*_, s) in zip(request.prediction.masks, g)):
-                    raise ValueError("prediction and target must have matching batch and image dimensions")
-                masks += [m.contiguous() for m in request.prediction.masks]
-                classes += list(request.prediction.classes)
-                targets += g
-                image_points = [self._points(request, m.shape[0]) for m in request.prediction.masks]
-                points += image_points
-                split.append(image_points)
-            if len(tickets) == 1:
-                # One request: its own target maps, without a repacking copy.
-                target = tickets[0].request.target.compact()
-                if target.crowd is None:
-                    target = PanopticBatch(target.segment_map, target.classes,
-                                           torch.zeros_like(target.classes, dtype=torch.bool),
-                                           image_sizes=target.image_sizes)
-            else:
-                target = PanopticBatch.pack([m for m, *_ in targets], [k for _, k, *_ in targets],
-                                            _crowd_rows(targets))
-            maps, gc, crowd = _prepare(target, self.num_classes, False)
-            sizes = (list(target.image_sizes) if target.packed
-                     else [tuple(target.segment_map.shape[1:])] * target.segment_map.shape[0])
-            out = launch_prefix(masks, classes, maps, gc, crowd, sizes, target.packed, points, self.things,
-                                self.thresholds, self.num_classes, self.device)
-            _, summaries, _, miou, _ = finalize_arrays(out["tp"], out["fp"], out["fn"], out["sums"],
-                                                       out["confusion"], self.things, copies=False)
-            rows = out["rows"]
-            values = torch.cat([summaries[:, 0, :].reshape(rows, 3), miou.reshape(rows, 1)], 1)
-            launch.host = torch.empty(values.shape, dtype=values.dtype, pin_memory=True)
-            launch.host.copy_(values, non_blocking=True)
-            launch.event.record(stream)
-            launch.keep = [target, out, summaries, miou, values, masks]
-            self.prefix_pixels_streamed += sum(h * w * (4 + m.shape[0]) for (h, w), m in zip(sizes, masks))
-        launch.split = split
-        self._started(launch)
+    def _stats(self):
+        out = (ctypes.c_int64 * 9)()
+        _native.call("pe_server_stats", self._handle, out)
+        return out
 
-    def _started(self, launch):
-        now = time.perf_counter()
-        for ticket in launch.tickets:
-            ticket.launched = now
-        self.launches[launch.kind] += 1
-        self._in_flight.append(launch)
+    # ------------------------------------------------------------ results
 
-    # ------------------------------------------------------------ completion
-
-    def _complete(self) -> int:
-        done = 0
-        while self._in_flight:
-            # Launches complete in issue order per stream; check each.
-            ready = [launch for launch in self._in_flight if launch.event.query()]
-            if not ready:
-                break
-            now = time.perf_counter()
-            for launch in ready:
-                self._in_flight.remove(launch)
-                self._finish(launch, now)
-                done += len(launch.tickets)
-        return done
-
-    def _finish(self, launch, now):
-        host = launch.host.numpy()
-        if launch.kind == "pq":
-            start = 0
-            for ticket, count in zip(launch.tickets, launch.split):
-                ticket.result = host[start:start + count].copy()
-                start += count
-        else:
-            row = 0
-            for ticket, image_points in zip(launch.tickets, launch.split):
-                per_image = []
-                for pts in image_points:
-                    block = host[row:row + len(pts)]
-                    per_image.append({"prefix": list(pts), "pq": block[:, 0].copy(), "sq": block[:, 1].copy(),
-                                      "rq": block[:, 2].copy(), "miou": block[:, 3].copy()})
-                    row += len(pts)
-                ticket.result = per_image
-        for ticket in launch.tickets:
-            ticket.completed = now
-            self.admission.release(ticket)
-        launch.keep = None
+    def _finish(self, ticket: Ticket):
+        lib = _native.library()
+        length = ctypes.c_int64()
+        times = (ctypes.c_int64 * 3)()
+        message = ctypes.create_string_buffer(256)
+        capacity = ticket.images if isinstance(ticket.request, PQRequest) else 4 * sum(map(len, ticket.points))
+        out = np.empty(max(capacity, 1), np.float64)
+        code = _native.check(lib.pe_server_take(self._handle, ticket.id, out.ctypes.data, out.size,
+                                                ctypes.byref(length), times, message))
+        ticket.launched = ticket.admitted = times[1] * 1e-9 if times[1] else None
+        ticket.completed = times[2] * 1e-9
+        ticket.keep = []
+        if code:
+            ticket.error = RuntimeError(f"reward launch failed: {message.value.decode(errors='replace')}")
+            return
+        values = out[:length.value]
+        if isinstance(ticket.request, PQRequest):
+            ticket.result = values.copy()
+            return
+        per_image, row = [], 0
+        for pts in ticket.points:
+            block = values[4 * row:4 * (row + len(pts))].reshape(len(pts), 4)
+            per_image.append({"prefix": list(pts), "pq": block[:, 0].copy(), "sq": block[:, 1].copy(),
+                              "rq": block[:, 2].copy(), "miou": block[:, 3].copy()})
+            row += len(pts)
+        ticket.result = per_image

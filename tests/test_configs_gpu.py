@@ -202,7 +202,7 @@ def alone(pe, request):
 
 def test_config4_serving_at_size(pe):
     import workloads
-    from panoptic_evaluator.serving import PQRequest, PrefixRequest, RewardServer
+    from panoptic_evaluator.serving import PQRequest, PrefixRequest, RewardServer, footprint
     pq, prefix = workloads.config4(device="cuda", pool=3)
     requests = []
     for images in pq:
@@ -217,10 +217,9 @@ def test_config4_serving_at_size(pe):
     requests = [r for pair in zip(requests[:len(pq)], requests[len(pq):]) for r in pair]
     print("c4 requests:", [(type(r).__name__, len(r.target.image_sizes)) for r in requests], flush=True)
     expected = [alone(pe, r) for r in requests]
-    server = RewardServer(133, isthing=ISTHING, hbm_budget=1, streams=3)
-    footprints = [server._footprint(r)[1] for r in requests]
+    footprints = [footprint(r, 133) for r in requests]
     # A budget that holds the largest request but not every two: admission waits.
-    server.admission.budget = max(footprints) + min(footprints) // 2
+    server = RewardServer(133, isthing=ISTHING, hbm_budget=max(footprints) + min(footprints) // 2, streams=3)
     tickets = []
     for i, request in enumerate(requests):
         tickets.append(server.submit(request))
@@ -229,6 +228,7 @@ def test_config4_serving_at_size(pe):
     server.drain()
     assert server.idle and server.admission.in_use == 0
     assert server.admission.blocked > 0, "the budget never made admission wait"
+    print("c4 launches:", dict(server.launches), "blocked passes:", server.admission.blocked, flush=True)
     for ticket, want in zip(tickets, expected):
         assert ticket.done and ticket.error is None
         if isinstance(want, np.ndarray):

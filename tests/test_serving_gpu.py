@@ -116,3 +116,36 @@ def test_single_request_batches_packed_and_uniform(pe):
         ticket = server.submit(request)
         server.drain()                                                   # one request per batch
         assert_same(ticket.result, alone(pe, request, isthing))
+
+
+def test_malformed_request_rejected_and_server_keeps_serving(pe):
+    """A request the server cannot evaluate is refused at submission (nothing
+    queued, no memory held); later requests are served and drain returns."""
+    from panoptic_evaluator.serving import PQRequest, RewardServer
+    isthing = [c < 80 for c in range(133)]
+    requests = make_requests(pe, 6, 4)
+    server = RewardServer(133, isthing=isthing, hbm_budget=64 << 20)
+    good = requests[0]
+    bad = PQRequest(pe.PanopticBatch(good.prediction.segment_map.long(), good.prediction.classes,
+                                     image_sizes=good.prediction.image_sizes), good.target)
+    with pytest.raises(ValueError):
+        server.submit(bad)
+    tickets = [server.submit(r) for r in requests]
+    server.drain()
+    assert server.idle and server.admission.in_use == 0 and server.admission.waiting == 0
+    for ticket, request in zip(tickets, requests):
+        assert ticket.error is None
+        assert_same(ticket.result, alone(pe, request, isthing))
+
+
+def test_request_larger_than_budget_runs_alone(pe):
+    from panoptic_evaluator.serving import RewardServer, footprint
+    isthing = [c < 80 for c in range(133)]
+    requests = make_requests(pe, 7, 3)
+    sizes = [footprint(r, 133) for r in requests]
+    server = RewardServer(133, isthing=isthing, hbm_budget=min(sizes) // 2)
+    tickets = [server.submit(r) for r in requests]
+    server.drain()
+    assert server.admission.in_use == 0
+    for ticket, request in zip(tickets, requests):
+        assert_same(ticket.result, alone(pe, request, isthing))
