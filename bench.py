@@ -16,13 +16,17 @@ DEVELOPMENT.md:47-49).  Two forms are timed over exactly K steps each:
 duration is measured with HIP events recorded by the library on the kernel's
 own launch stream during K eager steps of the batched update (the same
 kernel and launch shape as in the graph) and reported against the MI355X HBM
-roofline; the CPU oracle is timed on a bounded sample of the same workload on
-the host as ``cpu_baseline``.
+roofline.  ``cpu_baseline`` times a bounded sample of the same workload on
+the host: for the SegmentationEvaluator configs, a torch-op restatement of the
+reference's CPU path at torch's intra-op threads (with one thread and the
+numpy oracle beside it); elsewhere the numpy oracle.
 
 The default run measures c1 (the value) and then, in the same process, the
 rest of the metric as sub-objects of the same line: "c2" (the uneven 256-image
 batch, batched vs the plain per-image loop) and "c4" (the reward server's
-throughput and p50/p99 latency).  ``--extras none`` skips them.
+throughput and p50/p99 latency at 200 req/s, then at the ``--load-sweep``
+points: higher offered loads, the last under a budget that makes admission
+wait).  ``--extras none`` skips them.
 
 Other modes: --copy-calibration runs a plain copy of known size (the reference
 point that scales rocprofv3 FETCH_SIZE/WRITE_SIZE into bytes).
@@ -227,6 +231,30 @@ class CombinedConfig:
         elapsed += time.perf_counter() - t0
         return done / elapsed, f"{done} of {self.images} images, oracle SegmentationEvaluator update+compute"
 
+    def torch_sample(self, budget_s, chunk=4):
+        import torch
+        from oracle import torch_cpu
+        gt, pred, gc, pc, crowd = (t.cpu() for t in (self.pair.gt, self.pair.pred, self.pair.gc, self.pair.pc,
+                                                     self.pair.crowd))
+        masks = [m.cpu() for m in self.masks]
+        classes, scores = [c.cpu() for c in self.classes], [s.cpu() for s in self.scores]
+        state = torch_cpu.SegmentationState(133, self.isthing)
+        done, elapsed = 0, 0.0
+        while done < self.images and elapsed < budget_s:
+            part = range(done, min(done + chunk, self.images))
+            t0 = time.perf_counter()
+            state.update(pred[part.start:part.stop].long(), pc[part.start:part.stop].long(),
+                         [(masks[b], classes[b].long(), scores[b].double()) for b in part],
+                         gt[part.start:part.stop].long(), gc[part.start:part.stop].long(),
+                         crowd[part.start:part.stop])
+            elapsed += time.perf_counter() - t0
+            done = part.stop
+        t0 = time.perf_counter()
+        state.compute()
+        elapsed += time.perf_counter() - t0
+        return done / elapsed, (f"{done} of {self.images} images in updates of {chunk}, torch-op restatement "
+                                f"of the reference CPU SegmentationEvaluator update+compute")
+
 
 class InstanceConfig:
     """InstanceEvaluator reset/update/compute: c1's 100 ellipse detections per
@@ -364,6 +392,25 @@ class UnevenConfig:
         state.compute()
         elapsed += time.perf_counter() - t0
         return done / elapsed, f"{done} of {self.images} images, oracle SegmentationEvaluator update+compute"
+
+    def torch_sample(self, budget_s):
+        from oracle import torch_cpu
+        u = self.batch
+        state = torch_cpu.SegmentationState(133, self.isthing)
+        done, elapsed = 0, 0.0
+        while done < self.images and elapsed < budget_s:
+            i = done  # one update per image: the reference cannot batch uneven sizes
+            gt, pred, gc, pc, crowd = (t.cpu()[None] for t in (u.gt[i], u.pred[i], u.gc[i], u.pc[i], u.crowd[i]))
+            det = [(u.masks[i].cpu(), u.classes[i].cpu().long(), u.scores[i].cpu().double())]
+            t0 = time.perf_counter()
+            state.update(pred.long(), pc.long(), det, gt.long(), gc.long(), crowd)
+            elapsed += time.perf_counter() - t0
+            done += 1
+        t0 = time.perf_counter()
+        state.compute()
+        elapsed += time.perf_counter() - t0
+        return done / elapsed, (f"{done} of {self.images} images, one update each, torch-op restatement of the "
+                                f"reference CPU SegmentationEvaluator update+compute")
 
 
 class ServingConfig:
@@ -646,6 +693,25 @@ def serve(cfg, args, *, requests=None, rate=None, cpu_budget=None):
     batches = out["batches"]
     launches = sum(batches.values())
     launched = out.get("launched_requests") or requests
+    sweep = []
+    for point in filter(None, (args.load_sweep or "").split(",")):
+        # Higher offered loads, with a budget of their own where one is given
+        # (rate:bytes): batch occupancy and admission waits under pressure.
+        point_rate, _, point_budget = point.partition(":")
+        point_rate, point_budget = float(point_rate), int(point_budget or args.hbm_budget)
+        log(f"[bench] serving {requests} requests at {point_rate} req/s offered, budget {point_budget} B")
+        run = cfg.run(rate=point_rate, requests=requests, budget=point_budget, streams=args.streams)
+        run_launches = sum(run["batches"].values())
+        sweep.append({
+            "offered_load_rps": point_rate, "hbm_budget_bytes": point_budget,
+            "throughput_rps": round(requests / run["elapsed"], 3),
+            "images_per_sec": round(run["images"] / run["elapsed"], 3),
+            "latency_ms": {"p50": round(float(np.percentile(run["latencies"], 50)), 3),
+                           "p99": round(float(np.percentile(run["latencies"], 99)), 3)},
+            "batches": run["batches"],
+            "requests_per_launch": round((run.get("launched_requests") or requests) / max(run_launches, 1), 3),
+            "admission_waits": run.get("admission_waits"),
+        })
     return {
         "metric": METRIC,
         "value": round(out["images"] / out["elapsed"], 3),
@@ -669,6 +735,7 @@ def serve(cfg, args, *, requests=None, rate=None, cpu_budget=None):
         "batches": batches,
         "requests_per_launch": round(launched / max(launches, 1), 3),
         "admission_waits": out.get("admission_waits"),
+        "load_sweep": sweep,
         "roofline": with_ceiling({
             "bound": "hbm",
             "kernel": cfg.dominant,
@@ -683,6 +750,38 @@ def serve(cfg, args, *, requests=None, rate=None, cpu_budget=None):
         }),
         "cpu_baseline": cpu,
     }
+
+
+def cpu_baselines(cfg, budget_s):
+    """The CPU baseline of a step config.  Where the config has a torch-op
+    restatement of the reference's CPU path (oracle.torch_cpu), that is the
+    headline, timed at torch's intra-op threads, with the same at one thread
+    and the single-threaded numpy oracle beside it; otherwise the numpy
+    oracle alone."""
+    import torch
+    if not hasattr(cfg, "torch_sample"):
+        log(f"[bench] {cfg.name}: timing the CPU oracle sample")
+        rate, sample = cfg.cpu_sample(budget_s)
+        return {"value": round(rate, 4), "unit": "images/sec", "cores": 1, "kind": "port",
+                "sample": sample, "host_threads_available": os.cpu_count()}
+    threads = torch.get_num_threads()
+    timings = {}
+    for n in (threads, 1):
+        log(f"[bench] {cfg.name}: timing the torch-op CPU restatement at {n} thread(s)")
+        torch.set_num_threads(n)
+        try:
+            timings[n] = cfg.torch_sample(budget_s / 3)
+        finally:
+            torch.set_num_threads(threads)
+    log(f"[bench] {cfg.name}: timing the numpy CPU oracle sample")
+    numpy_rate, numpy_sample = cfg.cpu_sample(budget_s / 3)
+    rate, sample = timings[threads]
+    one_rate, one_sample = timings[1]
+    return {"value": round(rate, 4), "unit": "images/sec", "cores": threads, "kind": "port",
+            "sample": sample, "restatement": "torch CPU ops (oracle/torch_cpu.py)",
+            "host_threads_available": os.cpu_count(),
+            "single_thread": {"value": round(one_rate, 4), "cores": 1, "sample": one_sample},
+            "numpy_oracle": {"value": round(numpy_rate, 4), "cores": 1, "sample": numpy_sample}}
 
 
 def measure_steps(cfg, args, *, cpu_budget=None):
@@ -752,10 +851,7 @@ def measure_steps(cfg, args, *, cpu_budget=None):
     images_per_s = cfg.images * args.steps / value_elapsed
     cpu = None
     if not args.no_cpu_baseline:
-        log(f"[bench] {cfg.name}: timing the CPU oracle sample")
-        rate, sample = cfg.cpu_sample(cpu_budget)
-        cpu = {"value": round(rate, 4), "unit": "images/sec", "cores": 1, "kind": "port",
-               "sample": sample, "host_threads_available": os.cpu_count()}
+        cpu = cpu_baselines(cfg, cpu_budget)
     traffic = traffic_for(cfg.name)
     uneven = isinstance(cfg, UnevenConfig)
     line = {
@@ -806,8 +902,8 @@ def measure_steps(cfg, args, *, cpu_budget=None):
 # Sub-objects of the default line: the rest of the metric, measured in the
 # same process after the headline configuration.
 EXTRA_KEYS = ("value", "unit", "ms_per_step", "steps", "dtype", "plain", "speedup_vs_plain", "latency_ms",
-              "throughput_rps", "batches", "requests_per_launch", "admission_waits", "roofline", "cpu_baseline",
-              "tuning")
+              "throughput_rps", "batches", "requests_per_launch", "admission_waits", "load_sweep", "roofline",
+              "cpu_baseline", "tuning")
 
 
 def extra(line):
@@ -855,6 +951,9 @@ def main(argv=None):
     parser.add_argument("--requests", type=int, default=400, help="c4: requests served")
     parser.add_argument("--hbm-budget", type=int, default=16 << 30, help="c4: admission budget, bytes")
     parser.add_argument("--streams", type=int, default=3, help="c4: streams for overlapping batches")
+    parser.add_argument("--load-sweep", default="800,3200:4294967296",
+                        help="c4: further offered loads served after the main one, rate[:budget bytes] "
+                             "comma-separated ('' for none)")
     parser.add_argument("--extras", default=None,
                         help="comma-separated configs measured after the main one and reported as "
                              "sub-objects of the same line (default for c1: c2,c4; 'none' for none)")

@@ -455,3 +455,72 @@ def test_rgb_decode():
 
 def test_oracle_is_test_only():
     assert oracle.__doc__ and "TEST INFRASTRUCTURE ONLY" in oracle.__doc__
+
+
+# ------------------------------------------- torch-op restatement (CPU baseline)
+
+def _torch_and_numpy(C, isthing, batches):
+    import torch
+    from oracle import torch_cpu
+    ours, theirs = torch_cpu.SegmentationState(C, isthing), ocomb.SegmentationState(C, isthing, validate=False)
+    for pred, pc, dets, gt, gc, crowd, areas in batches:
+        theirs.update(pred, pc, [dict(masks=m, classes=c, scores=s) for m, c, s in dets], gt, gc, crowd,
+                      instance_areas=areas)
+        t = lambda a, dtype=torch.int64: torch.from_numpy(np.asarray(a)).to(dtype)  # noqa: E731
+        ours.update(t(pred), t(pc), [(t(m, torch.bool), t(c), t(s, torch.float64)) for m, c, s in dets],
+                    t(gt), t(gc), t(crowd, torch.bool), None if areas is None else t(areas, torch.float64))
+    return ours.compute(), theirs.compute()
+
+
+def _assert_same_results(got, want):
+    for key in ("tp", "fp", "fn", "confusion"):
+        assert_exact(got[key].numpy(), want[key])
+    assert_close(got["iou_sum"].numpy(), want["iou_sum"])
+    assert_close(got["miou"].item(), want["miou"])
+    for group_name in ("All", "Things", "Stuff"):
+        for key in ("pq", "sq", "rq", "n"):
+            assert_close(got[group_name][key].item(), want[group_name][key])
+    for key in ("pq", "sq", "rq", "iou"):
+        assert_close(got["per_class"][key].numpy(), want["per_class"][key])
+    assert_exact(got["instance"]["precision"], want["instance"]["precision"])
+    assert_exact(got["instance"]["recall"], want["instance"]["recall"])
+
+
+def test_torch_restatement_golden_combined():
+    data = load("combined")
+    inp = group(data, "in")
+    dets = [(inp[f"masks{b}"], inp[f"labels{b}"], inp[f"scores{b}"]) for b in range(2)]
+    batches = [(inp["pred_maps"], inp["classes"], dets, inp["maps"], inp["classes"], inp["crowd"], areas)
+               for areas in (None, inp["areas"])]
+    got, want = _torch_and_numpy(3, [True, True, False], batches)
+    _assert_same_results(got, want)
+    out = group(data, "out")
+    for key in ("tp", "fp", "fn", "confusion", "iou_sum"):
+        assert_exact(got[key].numpy(), out[key])
+
+
+def test_torch_restatement_random_batches():
+    rng = np.random.default_rng(31)
+    C, isthing = 6, [True, True, True, False, True, False]
+    batches = []
+    for _ in range(3):
+        B, H, W, S = 3, 20, 24, 7
+        gt = rng.integers(0, S, (B, H, W))
+        pred = np.where(rng.random((B, H, W)) < 0.3, rng.integers(0, S, (B, H, W)), gt)
+        gc = np.where(rng.random((B, S)) < 0.85, rng.integers(0, C, (B, S)), -1)
+        gc[:, 0] = -1
+        pc = np.where(rng.random((B, S)) < 0.7, gc, rng.integers(0, C, (B, S)))
+        pc[:, 0] = -1
+        crowd = (rng.random((B, S)) < 0.2) & (gc >= 0)
+        dets = []
+        for b in range(B):
+            D = int(rng.integers(0, 9))
+            masks = (gt[b][None] == rng.integers(1, S, (D, 1, 1))) ^ (rng.random((D, H, W)) < 0.1)
+            masks[:, 0, 0] = True
+            classes = rng.choice([c for c in range(C) if isthing[c]], D)
+            scores = np.round(rng.random(D), 1)  # ties exercise the stable order
+            dets.append((masks, classes, scores))
+        batches.append((pred, pc, dets, gt, gc, crowd, None))
+    got, want = _torch_and_numpy(C, isthing, batches)
+    _assert_same_results(got, want)
+    assert int(got["tp"].sum()) > 0 and (got["instance"]["precision"] > 0).any()
