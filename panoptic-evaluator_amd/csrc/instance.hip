@@ -1471,42 +1471,128 @@ struct ApArgs {
   double* partial;           // C x 24 summary accumulators
 };
 
-// One workgroup per class; thread j < 120 owns (t, area, maxDets) =
-// (j / 12, (j / 3) % 4, j % 3).  The class's records (score order) are staged
-// through LDS in chunks: rank and the 40 match flags per record.  Forward pass:
-// totals; backward pass: precision envelope at the 101 recall thresholds
-// (instance.py:300-331).
+// One workgroup per class (instance.py:300-331).  Its first four waves scan
+// the class's records, wave ar for area range ar, lane l < 30 for
+// (t, maxDets) = (l / 3, l % 3): job j = t * 12 + ar * 3 + maxDets indexes
+// the tables and the precision layout.  The records (score order) are staged
+// through LDS in chunks: rank and the 40 match flags per record.  Forward
+// pass: totals; backward pass: the precision envelope at the 101 recall
+// thresholds.
 constexpr int kApChunk = 1024;
-constexpr int kApThreads = 256;       // ap_accumulate (large batches)
+constexpr int kApThreads = 1024;      // ap_accumulate (large batches)
 constexpr int kApSmallThreads = 1024; // ap_small: more lanes for the gathers, the sort and the writer
-
-// LDS precision table of one class (TABLE form): for each recall threshold r
-// and job j, the running counts (tp << 16 | fp) of the record whose precision
-// is the envelope there, and per job the last threshold the scan reached
-// (-1: every threshold is 0; -2: no positives, every threshold is -1).
 constexpr int kApJobs = 120;
-constexpr size_t kApTableBytes = sizeof(uint32_t) * 101 * kApJobs + sizeof(int) * kApJobs;
+
+// LDS table of one class: for each recall target k and job j, the counts
+// (tp, fp) of the record whose precision is the envelope there, packed in
+// one Code (16 + 16 bits when counts stay below 65536, else 32 + 32), and per
+// job the highest threshold its final recall reaches (-1: every threshold is
+// 0; -2: no positives, every threshold is -1).
+template <typename Code>
+constexpr size_t ap_table_bytes() { return sizeof(Code) * 101 * kApJobs + sizeof(int) * kApJobs; }
+constexpr size_t kApTableBytes = ap_table_bytes<uint32_t>();
+
+// The least true-positive count t >= 1 with t / positives >= r * 0.01, both
+// as the FP64 expressions the reference compares (t = positives qualifies:
+// 100 * 0.01 rounds to 1.0).  Threshold r takes the envelope at the record
+// where the running count reaches it (the first counted record for r = 0,
+// where the envelope already equals the one at the first true positive:
+// records before it have precision 0).
+__device__ __forceinline__ int32_t recall_target(int r, i64 positives) {
+  const double thr = static_cast<double>(r) * 0.01;
+  const double npos = static_cast<double>(positives);
+  i64 t = static_cast<i64>(ceil(thr * npos));
+  t = t < 1 ? 1 : t > positives ? positives : t;
+  while (t > 1 && static_cast<double>(t - 1) / npos >= thr) --t;
+  while (t < positives && static_cast<double>(t) / npos < thr) ++t;
+  return static_cast<int32_t>(t > INT_MAX ? INT_MAX : t);
+}
+
+// Whether precision tp / (tp + fp + eps) of (t, f) exceeds that of (et, ef)
+// as FP64 values, compared on the integers.  For n = tp + fp >= 2, n + eps
+// rounds to n, so the values are the correctly rounded fractions t / n;
+// distinct fractions keep their order (or round equal, and then either
+// record gives the same value).  n = 1 gives 1 / (1 + eps), just below the
+// exact 1 of t = n >= 2 and above every other fraction; n = 0 gives 0.
+// Count is int (counts below 4096: 24-bit products) or i64.
+template <typename Count>
+__device__ __forceinline__ bool precision_above(Count t, Count f, Count et, Count ef) {
+  const Count n = t + f, en = et + ef;
+  bool above, equal;
+  if constexpr (sizeof(Count) == 4) {
+    const uint32_t lhs = __umul24(static_cast<uint32_t>(t), static_cast<uint32_t>(en > 1 ? en : 1));
+    const uint32_t rhs = __umul24(static_cast<uint32_t>(et), static_cast<uint32_t>(n > 1 ? n : 1));
+    above = lhs > rhs;
+    equal = lhs == rhs;
+  } else {
+    const u64 lhs = static_cast<u64>(t) * static_cast<u64>(en > 1 ? en : 1);
+    const u64 rhs = static_cast<u64>(et) * static_cast<u64>(n > 1 ? n : 1);
+    above = lhs > rhs;
+    equal = lhs == rhs;
+  }
+  return above || (equal && n >= 2 && t == n && en == 1 && et == 1);
+}
+
+template <typename Code>
+__device__ __forceinline__ double code_precision(Code code) {
+  constexpr int kShift = 4 * sizeof(Code);
+  const i64 tp = static_cast<i64>(code >> kShift), fp = static_cast<i64>(code & ((Code(1) << kShift) - 1));
+  return static_cast<double>(tp) / (static_cast<double>(tp + fp) + DBL_EPSILON);
+}
 
 // The accumulation of class c over its records order[s0 .. s1) (global score
 // order, ties by record index).  ORDER is the address space of ``order``
 // (a global array or an LDS list), kept apart so accesses stay typed.
-// TABLE: the scan writes its precision values as counts into an LDS table
-// (``codes``, ``last``) and the whole workgroup then writes the class's
-// 10 x 101 x 12 precision values, 12 contiguous doubles per (t, r); otherwise
-// each job writes its own 101 values as it goes (counts of at most 4096
-// records fit the table's 16-bit fields; the TABLE form is for small batches).
-template <bool TABLE, int THREADS, typename OrderPtr>
+// The envelope (the running maximum precision, scanning backward) only
+// changes at true positives, and a threshold only takes it at the true
+// positive where the count reaches the threshold's target, so the backward
+// pass does integer work at true positives and one LDS write per distinct
+// target.  The workgroup then writes the class's 10 x 101 x 12 precision
+// values (12 contiguous doubles per (t, r)) and each job folds its own
+// thresholds, highest first (the order of the reference's loop).  STAGED:
+// the caller has already put the records (at most one chunk) in LDS.
+template <int THREADS, typename Code, typename OrderPtr>
 __device__ __forceinline__ void ap_class(const ApArgs& a, int c, OrderPtr order, int s0, int s1, uint8_t* s_rank,
-                                         int8_t* s_flag, uint32_t* codes = nullptr, int* last = nullptr) {
-  const int job = threadIdx.x;
-  const bool active = job < 120;
-  const int m = job % 3, ar = (job / 3) % 4, t = job / 12;
+                                         int8_t* s_flag, Code* codes, int* last, bool staged = false) {
+  static_assert(THREADS >= 4 * kWave, "four scan waves");
+  using Count = std::conditional_t<sizeof(Code) == 4, int, i64>;
+  constexpr int kShift = 4 * sizeof(Code);
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  // Recall targets of each area range: s_target[ar][k] the distinct targets
+  // in descending order (0 ends the list), s_kr[ar][r] threshold r's k.
+  __shared__ int32_t s_target[4][102];
+  __shared__ uint8_t s_kr[4][101];
+  if (wave < 4) {
+    const i64 pos = a.counts[c * 4 + wave];
+    const int x_lo = pos > 0 ? recall_target(lane, pos) : 1;                               // r = lane
+    const int x_hi = pos > 0 && lane <= 36 ? recall_target(64 + lane, pos) : INT_MAX;      // r = 64 + lane
+    // a threshold starts a new target where its target differs from r + 1's
+    // (every lane takes part in both shuffles)
+    const int down_lo = __shfl_down(x_lo, 1, kWave);
+    const int first_hi = __shfl(x_hi, 0, kWave);
+    const int next_hi = __shfl_down(x_hi, 1, kWave);
+    const int next_lo = lane < 63 ? down_lo : first_hi;
+    const bool new_lo = x_lo != next_lo;
+    const bool new_hi = lane <= 36 && (lane == 36 || x_hi != next_hi);
+    const u64 hi = __ballot(new_hi), lo = __ballot(new_lo);
+    const int k_hi = __popcll(hi >> lane) - 1;
+    const int k_lo = __popcll(lo >> lane) + __popcll(hi) - 1;
+    if (lane <= 36) {
+      s_kr[wave][64 + lane] = static_cast<uint8_t>(k_hi);
+      if (new_hi) s_target[wave][k_hi] = x_hi;
+    }
+    s_kr[wave][lane] = static_cast<uint8_t>(k_lo);
+    if (new_lo) s_target[wave][k_lo] = x_lo;
+    if (lane == 0) s_target[wave][__popcll(hi) + __popcll(lo)] = 0;
+  }
+  // Scan lanes (the tables above are published by the forward pass's first
+  // barrier, which runs whenever a scan does).
+  const int ar = wave, t = lane / 3, m = lane % 3;
+  const bool active = wave < 4 && lane < 30;
+  const int job = t * 12 + ar * 3 + m;
   const int limit = m == 0 ? 1 : m == 1 ? 10 : 100;
   const i64 positives = active ? a.counts[c * 4 + ar] : 0;
   const i64 pstride = static_cast<i64>(a.C) * 12;
-  const i64 global_job = static_cast<i64>(t) * pstride + c * 12 + ar * 3 + m;
-  double* prec = a.precision + static_cast<i64>(t) * 101 * pstride + (c * 4 + ar) * 3 + m;
-  double* rec = a.recall + global_job;
   const int lane_flag = ar * 10 + t;
   // Records of one chunk -> LDS: the rank and the 40 flags (five 8-byte words,
   // rows are 8-byte aligned) of each record, four gathers in flight per lane.
@@ -1516,7 +1602,7 @@ __device__ __forceinline__ void ap_class(const ApArgs& a, int c, OrderPtr order,
     uint64_t* flag_words = reinterpret_cast<uint64_t*>(s_flag);
     const int words = count * 5;
     constexpr int U = 4;
-    for (int e0 = threadIdx.x; e0 < words; e0 += THREADS * U) {
+    for (int e0 = tid; e0 < words; e0 += THREADS * U) {
       uint64_t v[U];
       int rank[U], idx[U];
 #pragma unroll
@@ -1540,11 +1626,11 @@ __device__ __forceinline__ void ap_class(const ApArgs& a, int c, OrderPtr order,
   // forward: totals.  Records are read V at a time with every LDS load in
   // flight before any is used; the counts take no branches.
   constexpr int V = 4;
-  i64 tp = 0, fp = 0, n = 0;
+  Count tp = 0, fp = 0, n = 0;
   for (int first = s0; first < s1; first += kApChunk) {
     const int count = min(kApChunk, s1 - first);
     __syncthreads();
-    stage(first, count);
+    if (!staged) stage(first, count);
     __syncthreads();
     if (active && positives)
       for (int i0 = 0; i0 < count; i0 += 2 * V) {
@@ -1567,36 +1653,26 @@ __device__ __forceinline__ void ap_class(const ApArgs& a, int c, OrderPtr order,
         }
       }
   }
-  const bool scan = active && positives && n > 0;
+  const bool scan = active && positives && tp > 0;
   const double npos = static_cast<double>(positives);
-  double env = 0.0, sum = 0.0;
-  uint32_t env_code = 0;  // counts behind env (0/(0+eps) = 0.0)
-  int r = 100;
-  i64 ctp = tp, cfp = fp, remaining = n;
+  // Thresholds above the final recall get precision 0: the highest threshold
+  // taken is the largest r with r * 0.01 <= final recall.
+  int top = -1;
   if (scan) {
-    // Thresholds above the final recall get precision 0.
-    const double final_recall = static_cast<double>(ctp) / npos;
-    if constexpr (TABLE) {
-      // The largest r with r * 0.01 <= final recall (the loop below, in O(1)).
-      r = min(max(static_cast<int>(final_recall * 100.0), 0), 100);
-      while (r < 100 && static_cast<double>(r + 1) * 0.01 <= final_recall) ++r;
-      while (r >= 0 && static_cast<double>(r) * 0.01 > final_recall) --r;
-    } else {
-      while (r >= 0 && static_cast<double>(r) * 0.01 > final_recall) {
-        prec[r * pstride] = 0.0;
-        --r;
-      }
-    }
+    const double final_recall = static_cast<double>(tp) / npos;
+    top = min(max(static_cast<int>(final_recall * 100.0), 0), 100);
+    while (top < 100 && static_cast<double>(top + 1) * 0.01 <= final_recall) ++top;
+    while (top >= 0 && static_cast<double>(top) * 0.01 > final_recall) --top;
   }
-  if constexpr (TABLE) {
-    if (active) last[job] = !positives ? -2 : scan ? r : -1;
-  }
-  // backward: precision[r] = max(pr[pos_r:]), pos_r = first record with
-  // recall >= r / 100 (reusing the staged chunk when the class fits in one).
-  // V records per step: their LDS loads and the 2V FP64 divisions (which
-  // depend only on the integer running counts) are independent, so they
-  // overlap; the envelope and the recall-threshold writes then follow in
-  // record order, exactly as one record at a time.
+  if (active) last[job] = !positives ? -2 : n == 0 ? -1 : top;
+  // backward.  The next target (and the one after it, prefetched) stay in
+  // registers; the scan of a lane ends with its last target.
+  int k = scan ? s_kr[ar][top] : 0;
+  int target = scan ? s_target[ar][k] : 0;
+  int after = scan ? s_target[ar][k + 1] : 0;
+  Count et = 0, ef = 0;  // the envelope's record (0 / (0 + eps) = 0.0)
+  Code env_code = 0;
+  Count ctp = tp, cfp = fp;
   const int last_first = s1 > s0 ? s0 + ((s1 - s0 - 1) / kApChunk) * kApChunk : s0;
   for (int first = last_first; first >= s0 && s1 > s0; first -= kApChunk) {
     const int count = min(kApChunk, s1 - first);
@@ -1606,7 +1682,7 @@ __device__ __forceinline__ void ap_class(const ApArgs& a, int c, OrderPtr order,
       __syncthreads();
     }
     if (scan)
-      for (int i = count - 1; i >= 0 && r >= 0; i -= V) {
+      for (int i = count - 1; i >= 0 && target > 0; i -= V) {
         int rank[V], flag[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) {
@@ -1617,65 +1693,36 @@ __device__ __forceinline__ void ap_class(const ApArgs& a, int c, OrderPtr order,
 #pragma unroll
         for (int v = 0; v < V; ++v)
           if (i - v < 0) rank[v] = 255;
-        double pr[V], prev[V];
-        bool counted[V];
-        uint32_t code[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-          counted[v] = rank[v] <= limit;
-          code[v] = static_cast<uint32_t>(ctp << 16 | cfp);
-          pr[v] = static_cast<double>(ctp) / (static_cast<double>(ctp + cfp) + DBL_EPSILON);
-          ctp -= counted[v] && flag[v] == 1;
-          cfp -= counted[v] && flag[v] == 0;
-          remaining -= counted[v];
-          prev[v] = remaining > 0 ? static_cast<double>(ctp) / npos : -1.0;
-        }
-#pragma unroll
-        for (int v = 0; v < V; ++v) {
-          if (!counted[v]) continue;
-          if (pr[v] > env) {
-            env = pr[v];
-            env_code = code[v];
+          const bool counted = rank[v] <= limit;
+          const bool hit = counted && flag[v] == 1;
+          // (ctp, cfp): the counts up to and including this record
+          const bool up = hit && precision_above<Count>(ctp, cfp, et, ef);
+          et = up ? ctp : et;
+          ef = up ? cfp : ef;
+          env_code = up ? static_cast<Code>(static_cast<Code>(ctp) << kShift | static_cast<Code>(cfp)) : env_code;
+          if (hit && ctp == target) {
+            codes[k * kApJobs + job] = env_code;
+            ++k;
+            target = after;
+            after = s_target[ar][min(k + 1, 101)];
           }
-          while (r >= 0 && static_cast<double>(r) * 0.01 > prev[v]) {
-            if constexpr (TABLE)
-              codes[r * kApJobs + job] = env_code;
-            else
-              prec[r * pstride] = env;
-            sum += env;
-            --r;
-          }
+          ctp -= hit;
+          cfp -= counted && flag[v] == 0;
         }
       }
   }
-  double part_sum = 0.0, part_n = 0.0, rv = -1.0;
-  if (active) {
-    if (!positives) {
-      if constexpr (!TABLE)
-        for (int q = 0; q <= 100; ++q) prec[q * pstride] = -1.0;
-    } else if (n == 0) {
-      if constexpr (!TABLE)
-        for (int q = 0; q <= 100; ++q) prec[q * pstride] = 0.0;
-      rv = 0.0;
-      part_n = 101.0;
-    } else {
-      rv = static_cast<double>(tp) / npos;
-      part_sum = sum;
-      part_n = 101.0;
-    }
-    *rec = rv;
-  }
-  // This class's share of the 12 summaries: 24 (sum, count) accumulators,
-  // each folded over the 120 jobs in job order.
   __syncthreads();
-  if constexpr (TABLE) {
-    // (t, r, q) in memory order: q = ar * 3 + m is the job within t.  Eight
-    // values per thread and step, every LDS read issued before the divisions.
-    double* base = a.precision + c * 12;
+  // The class's precision values: (t, r, q) in memory order, q = ar * 3 + m
+  // the job within t.  Eight values per thread and step, every LDS read
+  // issued before the divisions.
+  double* base = a.precision + c * 12;
+  {
     constexpr int kValues = 10 * 101 * 12, W = 8;
     auto rr_of = [](int e) { return (min(e, kValues - 1) / 12) % 101; };
-    for (int e0 = threadIdx.x; e0 < kValues; e0 += THREADS * W) {
-      uint32_t counts[W];
+    for (int e0 = tid; e0 < kValues; e0 += THREADS * W) {
+      Code counts[W];
       int reached[W];
 #pragma unroll
       for (int w = 0; w < W; ++w) {
@@ -1683,28 +1730,50 @@ __device__ __forceinline__ void ap_class(const ApArgs& a, int c, OrderPtr order,
         const int q = e % 12, tr = e / 12, rr = tr % 101, tt = tr / 101;
         const int j = tt * 12 + q;
         reached[w] = last[j];
-        counts[w] = codes[rr * kApJobs + j];
+        counts[w] = codes[s_kr[q / 3][rr] * kApJobs + j];
       }
       // Envelopes with no true positive are exactly 0 (0 / (fp + eps)); a
       // wave whose values are all such skips the divisions.
       bool divide = false;
 #pragma unroll
       for (int w = 0; w < W; ++w) divide |= reached[w] >= -1 && rr_of(e0 + w * THREADS) <= reached[w] &&
-                                            (counts[w] >> 16) != 0;
+                                            (counts[w] >> kShift) != 0;
       const bool any_divide = __ballot(divide) != 0;
 #pragma unroll
       for (int w = 0; w < W; ++w) {
         const int e = e0 + w * THREADS;
         if (e >= kValues) break;
         const int q = e % 12, tr = e / 12, rr = tr % 101, tt = tr / 101;
-        const uint32_t code = rr <= reached[w] ? counts[w] : 0u;
-        const i64 etp = code >> 16, efp = code & 0xFFFFu;
+        const Code code = rr <= reached[w] ? counts[w] : Code(0);
         double value = reached[w] < -1 ? -1.0 : 0.0;
-        if (any_divide && reached[w] >= -1)
-          value = static_cast<double>(etp) / (static_cast<double>(etp + efp) + DBL_EPSILON);
+        if (any_divide && reached[w] >= -1) value = code_precision(code);
         base[(static_cast<i64>(tt) * 101 + rr) * pstride + q] = value;
       }
     }
+  }
+  __syncthreads();
+  // Each job's threshold sum (highest first, read back from the values just
+  // written), recall, and the class's share of the 12 summaries: 24 (sum,
+  // count) accumulators, each folded over the 120 jobs in job order.
+  double part_sum = 0.0, part_n = 0.0, rv = -1.0;
+  if (active) {
+    if (positives) {
+      double sum = 0.0;
+      const double* mine = base + static_cast<i64>(t) * 101 * pstride + ar * 3 + m;
+      constexpr int R = 8;  // loads in flight; the adds keep the order
+      for (int q0 = top; q0 >= 0; q0 -= R) {
+        double value[R];
+#pragma unroll
+        for (int u = 0; u < R; ++u) value[u] = mine[max(q0 - u, 0) * pstride];
+#pragma unroll
+        for (int u = 0; u < R; ++u)
+          if (q0 - u >= 0) sum += value[u];
+      }
+      rv = static_cast<double>(tp) / npos;
+      part_sum = sum;
+      part_n = 101.0;
+    }
+    a.recall[static_cast<i64>(t) * pstride + c * 12 + ar * 3 + m] = rv;
   }
   double* sh = reinterpret_cast<double*>(s_flag);
   if (active) {
@@ -1717,42 +1786,53 @@ __device__ __forceinline__ void ap_class(const ApArgs& a, int c, OrderPtr order,
   // 0 ap, 1 ap50, 2 ap75 (all areas, maxDets 100), 3-5 ap small/medium/large,
   // 6-8 ar1/ar10/ar100 (all areas), 9-11 ar small/medium/large (maxDets 100).
   // Each folds its jobs over thresholds t in increasing order.
-  if (threadIdx.x < 24) {
-    const int q = threadIdx.x, k = q >> 1;
+  if (tid < 24) {
+    const int q = tid, kk = q >> 1;
     const bool count = q & 1;
-    const bool precision_side = k < 6;
-    int ar = 0, m = 2, t0 = 0, t1 = 10;
-    if (k == 1) t1 = 1;
-    else if (k == 2) t0 = 5, t1 = 6;
-    else if (k >= 3 && k < 6) ar = k - 2;
-    else if (k >= 6 && k < 9) m = k - 6;
-    else if (k >= 9) ar = k - 8;
+    const bool precision_side = kk < 6;
+    int sar = 0, sm = 2, t0 = 0, t1 = 10;
+    if (kk == 1) t1 = 1;
+    else if (kk == 2) t0 = 5, t1 = 6;
+    else if (kk >= 3 && kk < 6) sar = kk - 2;
+    else if (kk >= 6 && kk < 9) sm = kk - 6;
+    else if (kk >= 9) sar = kk - 8;
     double acc = 0.0;
-    for (int t = t0; t < t1; ++t) {
-      const int j = t * 12 + ar * 3 + m;
+    for (int st = t0; st < t1; ++st) {
+      const int j = st * 12 + sar * 3 + sm;
       if (precision_side) {
         acc += count ? sh[3 * j + 1] : sh[3 * j];
       } else {
-        const double rv = sh[3 * j + 2];
-        if (rv >= 0) acc += count ? 1.0 : rv;
+        const double srv = sh[3 * j + 2];
+        if (srv >= 0) acc += count ? 1.0 : srv;
       }
     }
     a.partial[c * 24 + q] = acc;
   }
 }
 
-// Classes whose records were grouped by the two radix sorts.
+// Classes whose records were grouped by the global sort.  Dynamic LDS: the
+// chunk's ranks and flags, then the 64-bit table.
+constexpr size_t kApAccumulateLds = kApChunk + 40 * kApChunk + ap_table_bytes<u64>();
 __global__ __launch_bounds__(kApThreads) void ap_accumulate(ApArgs a) {
-  __shared__ uint8_t s_rank[kApChunk];
-  __shared__ __align__(16) int8_t s_flag[kApChunk * 40];
+  extern __shared__ uint4 lds4[];
+  int8_t* s_flag = reinterpret_cast<int8_t*>(lds4);
+  uint8_t* s_rank = reinterpret_cast<uint8_t*>(s_flag + 40 * kApChunk);
+  u64* codes = reinterpret_cast<u64*>(s_rank + kApChunk);
+  int* last = reinterpret_cast<int*>(codes + 101 * kApJobs);
   const int c = blockIdx.x;
-  ap_class<false, kApThreads>(a, c, as_global(a.order), a.start[c], a.end[c], s_rank, s_flag);
+  const int s0 = a.start[c], s1 = a.end[c];
+  // Counts below 4096 take the 32-bit codes and 24-bit products.
+  if (s1 - s0 < 4096)
+    ap_class<kApThreads>(a, c, as_global(a.order), s0, s1, s_rank, s_flag, reinterpret_cast<uint32_t*>(codes), last);
+  else
+    ap_class<kApThreads>(a, c, as_global(a.order), s0, s1, s_rank, s_flag, codes, last);
 }
 
 // Batches of at most kApSmall records: each class workgroup gathers its own
 // records (valid rank, this class) in index order, sorts them by (score
-// descending, index) with a bitonic network in LDS, and accumulates -- the
-// same order the two global radix sorts give, in one launch.
+// descending, index) in LDS (a counted rank per record when they fit one per
+// lane, else a bitonic network), and accumulates -- the same order the
+// global sort of larger batches gives, in one launch.
 constexpr int kApSmall = 4096;
 
 struct ApSmallArgs {
@@ -1825,34 +1905,66 @@ __device__ __forceinline__ void ap_small_class(const ApSmallArgs& s, int c, uint
       keys[slot] = score_key_desc(score[it]);
     }
   }
-  int m = 1;
-  while (m < n) m <<= 1;
-  for (int e = n + tid; e < m; e += kApSmallThreads) {
-    keys[e] = ~0ULL;
-    idx[e] = INT_MAX;
-  }
   __syncthreads();
-  // Bitonic sort of (key, index) ascending: descending score, then index.
-  for (int k = 2; k <= m; k <<= 1)
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int e = tid; e < m; e += kApSmallThreads) {
-        const int f = e ^ j;
-        if (f <= e) continue;
-        const u64 ke = keys[e], kf = keys[f];
-        const int ie = idx[e], jf = idx[f];
-        const bool greater = ke > kf || (ke == kf && ie > jf);
-        if (greater == ((e & k) == 0)) {
-          keys[e] = kf;
-          keys[f] = ke;
-          idx[e] = jf;
-          idx[f] = ie;
-        }
+  if (n <= kApSmallThreads) {
+    // Up to one record per lane: its sorted position is the number of
+    // records before it in (key, index) order, counted over broadcast LDS
+    // reads -- one barrier instead of a network of them.
+    // The record's rank and 40 flags load meanwhile and go straight to its
+    // sorted place in the staging area.
+    int pos = 0, mine_idx = 0, mine_rank = 0;
+    uint64_t words[5] = {0, 0, 0, 0, 0};
+    if (tid < n) {
+      const u64 k = keys[tid];
+      mine_idx = idx[tid];
+      const uint64_t* row = reinterpret_cast<const uint64_t*>(a.status + static_cast<i64>(mine_idx) * 40);
+#pragma unroll
+      for (int w = 0; w < 5; ++w) words[w] = row[w];
+      mine_rank = a.ranks[mine_idx];
+      for (int j = 0; j < n; ++j) {
+        const u64 kj = keys[j];
+        pos += kj < k || (kj == k && idx[j] < mine_idx);
       }
-      __syncthreads();
     }
+    __syncthreads();
+    if (tid < n) {
+      idx[pos] = mine_idx;
+      uint64_t* flag_words = reinterpret_cast<uint64_t*>(s_flag) + pos * 5;
+#pragma unroll
+      for (int w = 0; w < 5; ++w) flag_words[w] = words[w];
+      s_rank[pos] = static_cast<uint8_t>(mine_rank);
+    }
+    __syncthreads();
+  } else {
+    int m = 1;
+    while (m < n) m <<= 1;
+    for (int e = n + tid; e < m; e += kApSmallThreads) {
+      keys[e] = ~0ULL;
+      idx[e] = INT_MAX;
+    }
+    __syncthreads();
+    // Bitonic sort of (key, index) ascending: descending score, then index.
+    for (int k = 2; k <= m; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int e = tid; e < m; e += kApSmallThreads) {
+          const int f = e ^ j;
+          if (f <= e) continue;
+          const u64 ke = keys[e], kf = keys[f];
+          const int ie = idx[e], jf = idx[f];
+          const bool greater = ke > kf || (ke == kf && ie > jf);
+          if (greater == ((e & k) == 0)) {
+            keys[e] = kf;
+            keys[f] = ke;
+            idx[e] = jf;
+            idx[f] = ie;
+          }
+        }
+        __syncthreads();
+      }
+  }
   uint32_t* codes = reinterpret_cast<uint32_t*>(s_flag + 40 * kApChunk);
-  ap_class<true, kApSmallThreads>(a, c, static_cast<const int32_t*>(idx), 0, n, s_rank, s_flag, codes,
-                 reinterpret_cast<int*>(codes + 101 * kApJobs));
+  ap_class<kApSmallThreads>(a, c, static_cast<const int32_t*>(idx), 0, n, s_rank, s_flag, codes,
+                            reinterpret_cast<int*>(codes + 101 * kApJobs), n <= kApSmallThreads);
 }
 
 __global__ __launch_bounds__(kApSmallThreads) void ap_small(ApSmallArgs s) {
@@ -2365,8 +2477,10 @@ static int ap_compute(int64_t N, const double* scores, const int32_t* classes, c
     PE_LAUNCH_CHECK();
   }
   {
+    PE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(ap_accumulate),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kApAccumulateLds)));
     ProfileScope timing("ap_accumulate", stream);
-    ap_accumulate<<<static_cast<unsigned>(C), kApThreads, 0, stream>>>(a);
+    ap_accumulate<<<static_cast<unsigned>(C), kApThreads, kApAccumulateLds, stream>>>(a);
     PE_LAUNCH_CHECK();
   }
   ap_summaries<<<1, 256, 0, stream>>>(static_cast<int>(C), w.partial, summary);
