@@ -29,7 +29,6 @@
 //    are first grouped per class in global score order by stable sorts.  The
 //    cumulative TP/FP scan runs forward and the precision envelope backward,
 //    matching the reference's searchsorted/cummax result exactly.
-#include <hipcub/hipcub.hpp>
 
 #include <cfloat>
 #include <climits>
@@ -38,6 +37,7 @@
 
 #include "common.h"
 #include "finalize.h"
+#include "merge_sort.h"
 #include "../../include/panoptic_hip.h"
 
 namespace pe {
@@ -1432,29 +1432,26 @@ __device__ __forceinline__ u64 score_key_desc(double s) {
   return ~asc;  // ascending radix order == descending score
 }
 
-__global__ void score_keys(const double* scores, i64 N, u64* keys, int32_t* idx) {
+// Sort items of the AP records: class (records with ranks outside 1..100,
+// which no maxDets setting counts, take class C and sort last), descending
+// score, record index.
+__global__ void ap_items(const double* scores, const int32_t* classes, const int32_t* ranks, i64 N, int C,
+                         SortItem* items) {
   for (i64 i = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += static_cast<i64>(gridDim.x) * blockDim.x) {
-    keys[i] = score_key_desc(scores[i]);
-    idx[i] = static_cast<int32_t>(i);
+    const int c = classes[i], r = ranks[i];
+    const bool valid = r > 0 && r <= 100 && c >= 0 && c < C;
+    items[i] = SortItem{score_key_desc(scores[i]), static_cast<uint32_t>(valid ? c : C), static_cast<int32_t>(i)};
   }
 }
 
-// Class key of each score-sorted record; ranks outside 1..100 sort last (C).
-__global__ void class_keys(const int32_t* perm, const int32_t* classes, const int32_t* ranks, i64 N, int C,
-                           uint32_t* keys) {
+// Record order and each class's [start, end) from the sorted items.
+__global__ void ap_order(const SortItem* sorted, i64 N, int C, int32_t* order, int32_t* start, int32_t* end) {
   for (i64 i = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += static_cast<i64>(gridDim.x) * blockDim.x) {
-    const int r = perm[i];
-    const int c = classes[r];
-    keys[i] = (ranks[r] > 0 && ranks[r] <= 100 && c >= 0 && c < C) ? static_cast<uint32_t>(c) : static_cast<uint32_t>(C);
-  }
-}
-
-__global__ void class_bounds(const uint32_t* keys, i64 N, int C, int32_t* start, int32_t* end) {
-  for (i64 i = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += static_cast<i64>(gridDim.x) * blockDim.x) {
-    const uint32_t k = keys[i];
+    const uint32_t k = sorted[i].group;
+    order[i] = sorted[i].index;
     if (k >= static_cast<uint32_t>(C)) continue;
-    if (i == 0 || keys[i - 1] != k) start[k] = static_cast<int32_t>(i);
-    if (i == N - 1 || keys[i + 1] != k) end[k] = static_cast<int32_t>(i + 1);
+    if (i == 0 || sorted[i - 1].group != k) start[k] = static_cast<int32_t>(i);
+    if (i == N - 1 || sorted[i + 1].group != k) end[k] = static_cast<int32_t>(i + 1);
   }
 }
 
@@ -2353,46 +2350,25 @@ static int launch_score_order_lds(bool arrays, int64_t B, int64_t max_dets, cons
 
 namespace {
 struct ApBuffers {
-  u64* keys_in;
-  u64* keys_out;
-  int32_t* idx_in;
-  int32_t* perm;
-  uint32_t* ckeys_in;
-  uint32_t* ckeys_out;
+  SortItem* items;
+  SortItem* spare;
   int32_t* order;
   int32_t* start;
   int32_t* end;
   double* partial;
-  void* temp;
-  size_t temp_bytes;
 };
 
-size_t sort_temp_bytes(i64 N) {
-  size_t a = 0, b = 0;
-  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, a, static_cast<u64*>(nullptr), static_cast<u64*>(nullptr),
-                                     static_cast<int32_t*>(nullptr), static_cast<int32_t*>(nullptr),
-                                     static_cast<int>(N));
-  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, b, static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr),
-                                     static_cast<int32_t*>(nullptr), static_cast<int32_t*>(nullptr),
-                                     static_cast<int>(N));
-  return std::max(a, b);
-}
-
-ApBuffers carve_ap(void* workspace, i64 N, i64 C, size_t temp_bytes, size_t* used = nullptr) {
+ApBuffers carve_ap(void* workspace, i64 N, i64 C, size_t* used = nullptr) {
   Carve carve(workspace);
   ApBuffers w{};
-  w.keys_in = carve.take<u64>(N);
-  w.keys_out = carve.take<u64>(N);
-  w.idx_in = carve.take<int32_t>(N);
-  w.perm = carve.take<int32_t>(N);
-  w.ckeys_in = carve.take<uint32_t>(N);
-  w.ckeys_out = carve.take<uint32_t>(N);
-  w.order = carve.take<int32_t>(N);
+  // The per-class gather of small batches needs no sort buffers.
+  const i64 sorted = N > kApSmall ? N : 0;
+  w.items = carve.take<SortItem>(sorted);
+  w.spare = carve.take<SortItem>(sorted);
+  w.order = carve.take<int32_t>(sorted);
   w.start = carve.take<int32_t>(C);
   w.end = carve.take<int32_t>(C);
   w.partial = carve.take<double>(C * 24);
-  w.temp = carve.take<char>(temp_bytes);
-  w.temp_bytes = temp_bytes;
   if (used) *used = carve.used;
   return w;
 }
@@ -2400,7 +2376,7 @@ ApBuffers carve_ap(void* workspace, i64 N, i64 C, size_t temp_bytes, size_t* use
 
 extern "C" int64_t pe_ap_workspace_bytes(int64_t N, int64_t C) {
   size_t used = 0;
-  carve_ap(nullptr, N, C, N > 0 ? sort_temp_bytes(N) : 0, &used);
+  carve_ap(nullptr, N, C, &used);
   return static_cast<int64_t>(align_up(used, 256));
 }
 
@@ -2414,8 +2390,7 @@ static int ap_compute(int64_t N, const double* scores, const int32_t* classes, c
   PE_REQUIRE(workspace_bytes >= pe_ap_workspace_bytes(N, C), "workspace too small");
   PE_REQUIRE(reinterpret_cast<uintptr_t>(status) % 8 == 0, "match flags must be 8-byte aligned");
   hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
-  const size_t temp = N > 0 ? sort_temp_bytes(N) : 0;
-  ApBuffers w = carve_ap(workspace, N, C, temp);
+  ApBuffers w = carve_ap(workspace, N, C);
   int rc;
   ApArgs a{};
   a.C = static_cast<int>(C);
@@ -2460,20 +2435,14 @@ static int ap_compute(int64_t N, const double* scores, const int32_t* classes, c
   if (fin && (rc = launch_finalize(*fin, K, stream))) return rc;
   if ((rc = launch_zero(w.start, 4 * static_cast<size_t>(C), stream))) return rc;
   if ((rc = launch_zero(w.end, 4 * static_cast<size_t>(C), stream))) return rc;
-  if (N > 0) {
+  {
+    // One sort by (class, descending score, record index).
     const int blocks = static_cast<int>(std::min<i64>((N + 255) / 256, 4096));
-    score_keys<<<blocks, 256, 0, stream>>>(scores, N, w.keys_in, w.idx_in);
+    ap_items<<<blocks, 256, 0, stream>>>(scores, classes, ranks, N, static_cast<int>(C), w.items);
     PE_LAUNCH_CHECK();
-    size_t bytes = w.temp_bytes;
-    PE_HIP(hipcub::DeviceRadixSort::SortPairs(w.temp, bytes, w.keys_in, w.keys_out, w.idx_in, w.perm,
-                                              static_cast<int>(N), 0, 64, stream));
-    class_keys<<<blocks, 256, 0, stream>>>(w.perm, classes, ranks, N, static_cast<int>(C), w.ckeys_in);
-    PE_LAUNCH_CHECK();
-    bytes = w.temp_bytes;
-    const int class_bits = 32 - __builtin_clz(static_cast<unsigned>(C) | 1u);
-    PE_HIP(hipcub::DeviceRadixSort::SortPairs(w.temp, bytes, w.ckeys_in, w.ckeys_out, w.perm, w.order,
-                                              static_cast<int>(N), 0, class_bits, stream));
-    class_bounds<<<blocks, 256, 0, stream>>>(w.ckeys_out, N, static_cast<int>(C), w.start, w.end);
+    SortItem* sorted = nullptr;
+    if ((rc = sort_items(w.items, w.spare, N, stream, &sorted))) return rc;
+    ap_order<<<blocks, 256, 0, stream>>>(sorted, N, static_cast<int>(C), w.order, w.start, w.end);
     PE_LAUNCH_CHECK();
   }
   {
@@ -2593,23 +2562,24 @@ extern "C" int pe_instance_checks(int64_t n_det, const int64_t* det_cls, const d
 
 namespace pe {
 
-__global__ void score_sort_keys(const double* scores, i64 N, u64* keys, int32_t* idx) {
-  for (i64 i = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += static_cast<i64>(gridDim.x) * blockDim.x) {
-    keys[i] = score_key_desc(scores[i]);
-    idx[i] = static_cast<int32_t>(i);
-  }
+// Sort items of one image per workgroup: image, descending score, index.
+__global__ __launch_bounds__(256) void score_items(const i64* det_off, const double* scores, SortItem* items) {
+  const int b = blockIdx.x;
+  const i64 first = det_off[b], last = det_off[b + 1];
+  for (i64 i = first + threadIdx.x; i < last; i += blockDim.x)
+    items[i] = SortItem{score_key_desc(scores[i]), static_cast<uint32_t>(b), static_cast<int32_t>(i)};
 }
 
 // One workgroup per image: gather classes/scores into score order and record
 // the within-image mask index of every sorted position.
-__global__ __launch_bounds__(256) void gather_score_order(const i64* det_off, const int32_t* sorted_idx,
+__global__ __launch_bounds__(256) void gather_score_order(const i64* det_off, const SortItem* sorted,
                                                           const double* scores, const i64* classes, i64* order,
                                                           double* sorted_scores, i64* sorted_classes,
                                                           int32_t* sorted_classes32) {
   const int b = blockIdx.x;
   const i64 first = det_off[b], last = det_off[b + 1];
   for (i64 i = first + threadIdx.x; i < last; i += blockDim.x) {
-    const int32_t src = sorted_idx[i];
+    const int32_t src = sorted[i].index;
     order[i] = src - first;
     sorted_scores[i] = scores[src];
     const i64 c = classes[src];
@@ -2618,26 +2588,13 @@ __global__ __launch_bounds__(256) void gather_score_order(const i64* det_off, co
   }
 }
 
-namespace {
-size_t segmented_sort_bytes(i64 N, i64 B) {
-  size_t bytes = 0;
-  (void)hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, bytes, static_cast<u64*>(nullptr), static_cast<u64*>(nullptr),
-                                              static_cast<int32_t*>(nullptr), static_cast<int32_t*>(nullptr),
-                                              static_cast<int>(N), static_cast<int>(B),
-                                              static_cast<const i64*>(nullptr), static_cast<const i64*>(nullptr));
-  return bytes;
-}
-}  // namespace
 
 }  // namespace pe
 
 extern "C" int64_t pe_score_order_workspace_bytes(int64_t N, int64_t B) {
   Carve carve(nullptr);
-  carve.take<u64>(N);
-  carve.take<u64>(N);
-  carve.take<int32_t>(N);
-  carve.take<int32_t>(N);
-  carve.take<char>(N > 0 && B > 0 ? segmented_sort_bytes(N, B) : 0);
+  carve.take<SortItem>(N > 0 && B > 0 ? N : 0);
+  carve.take<SortItem>(N > 0 && B > 0 ? N : 0);
   return static_cast<int64_t>(align_up(carve.used, 256));
 }
 
@@ -2653,23 +2610,17 @@ extern "C" int pe_score_order(int64_t B, int64_t N, int64_t max_dets, const int6
     return launch_score_order_lds(false, B, max_dets, det_off, scores, classes, ImageArrays{}, order,
                                   sorted_scores, sorted_classes, sorted_classes32, stream);
   Carve carve(workspace);
-  u64* keys_in = carve.take<u64>(N);
-  u64* keys_out = carve.take<u64>(N);
-  int32_t* idx_in = carve.take<int32_t>(N);
-  int32_t* idx_out = carve.take<int32_t>(N);
-  size_t temp_bytes = segmented_sort_bytes(N, B);
-  void* temp = carve.take<char>(temp_bytes);
-  const int blocks = static_cast<int>(std::min<i64>((N + 255) / 256, 4096));
-  score_sort_keys<<<blocks, 256, 0, stream>>>(scores, N, keys_in, idx_in);
-  PE_LAUNCH_CHECK();
-  // Stable LSD radix sort per image: descending score, ties in input order
+  SortItem* items = carve.take<SortItem>(N);
+  SortItem* spare = carve.take<SortItem>(N);
+  // Per image: descending score, ties in input order
   // (torch.argsort(descending=True, stable=True), instance.py:140).
   const i64* off = reinterpret_cast<const i64*>(det_off);
-  PE_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(temp, temp_bytes, keys_in, keys_out, idx_in, idx_out,
-                                                     static_cast<int>(N), static_cast<int>(B), off, off + 1, 0, 64,
-                                                     stream));
+  score_items<<<static_cast<unsigned>(B), 256, 0, stream>>>(off, scores, items);
+  PE_LAUNCH_CHECK();
+  SortItem* sorted = nullptr;
+  if (const int rc = sort_items(items, spare, N, stream, &sorted)) return rc;
   gather_score_order<<<static_cast<unsigned>(B), 256, 0, stream>>>(
-      off, idx_out, scores, reinterpret_cast<const i64*>(classes), reinterpret_cast<i64*>(order), sorted_scores,
+      off, sorted, scores, reinterpret_cast<const i64*>(classes), reinterpret_cast<i64*>(order), sorted_scores,
       reinterpret_cast<i64*>(sorted_classes), sorted_classes32);
   PE_LAUNCH_CHECK();
   return kOk;

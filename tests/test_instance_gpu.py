@@ -501,7 +501,7 @@ def test_matcher_with_many_pairs_of_one_class(pe):
 
 def test_score_order_beyond_the_lds_sort(pe):
     """An image with more detections than the in-LDS per-image sort holds
-    (4096) takes the segmented radix sort; both images against the oracle."""
+    (4096) takes the global merge sort; both images against the oracle."""
     rng = np.random.default_rng(14)
     h, w = 6, 7
     dets = [5000, 37]
@@ -519,9 +519,32 @@ def test_score_order_beyond_the_lds_sort(pe):
     compare_instance(to_numpy(ev.compute()), state.compute())
 
 
+def test_sorts_over_many_merge_passes(pe):
+    """Three images of 14000 detections: the per-image score order and the AP
+    record order each sort 42000 items (11 tiles, four merge passes with a
+    partial last pair); against the oracle."""
+    rng = np.random.default_rng(16)
+    h, w = 3, 4
+    dets = [14000, 14000, 14000]
+    pm = [rng.random((d, h, w)) > 0.5 for d in dets]
+    gm = [rng.random((4, h, w)) > 0.5 for _ in dets]
+    for m in pm + gm:
+        m[:, 0, 0] = True  # no empty mask
+    pcl = [rng.integers(0, 3, d) for d in dets]
+    gcl = [rng.integers(0, 3, 4) for _ in dets]
+    sc = [rng.integers(0, 1000, d) / 1000.0 for d in dets]  # many ties
+    ev = pe.InstanceEvaluator(3, device="cuda")
+    ev.update(pe.InstanceBatch([cuda(m) for m in pm], [cuda(c) for c in pcl], [cuda(s) for s in sc]),
+              pe.InstanceBatch([cuda(m) for m in gm], [cuda(c) for c in gcl]))
+    state = oinst.InstanceState(3)
+    state.update([dict(masks=m, classes=c, scores=s) for m, c, s in zip(pm, pcl, sc)],
+                 [dict(masks=m, classes=c) for m, c in zip(gm, gcl)])
+    compare_instance(to_numpy(ev.compute()), state.compute())
+
+
 def test_combined_compute_beyond_the_small_ap_path(pe):
     """More than 4096 detection records: compute() launches the finaliser on
-    its own before the radix-sort AP path (below that, both run in one
+    its own before the sorted AP path (below that, both run in one
     launch, as every other combined test checks)."""
     rng = np.random.default_rng(15)
     B, h, w, D = 2, 10, 12, 2100
