@@ -216,6 +216,30 @@ int pe_mask_overlap_batch(int64_t B, const uint8_t* const* pm, const uint8_t* co
 int64_t pe_mask_words(int64_t pixels);
 int pe_pack_masks(const uint8_t* masks, int64_t N, int64_t HW, uint32_t* bits, void* stream);
 
+/* COCO run-length encoded masks decoded into bit rows (the layout above).
+ * No counterpart in the reference, which takes bool bytes and hands its masks
+ * to pycocotools as RLE only in its parity tests (tests/test_instance.py:110,
+ * ``mask_utils.encode(np.asfortranarray(pixels))``); this is the compressed
+ * mask input of its listed extensions.  Format: pycocotools 2.0's
+ * (common/maskApi.c) -- column-major runs of 0s then 1s summing to H * W,
+ * either as the compressed string or as int32 counts.
+ *
+ * `table` is a device array of N x 8 int64, one row per mask: byte offset of
+ * its characters / int32 counts in `data`, their number (`units`), word
+ * offset of its bit row in `bits`, offset of its W entries in the column
+ * table, H, W, format (0: string, 1: int32 counts; counts 4-byte aligned)
+ * and offset of its run table (`units` entries).  `units` and `columns` are
+ * the totals of the units and W entries, `max_width` the largest W.  `bits`
+ * must be zero-filled (only set words are written).  A malformed mask (bad
+ * character, a count of more than seven 5-bit groups, a negative count or a
+ * total other than H * W) is left all-zero and ORs a bit into the device
+ * word *error (1 character, 2 group length, 4 negative count, 8 total).
+ * Workspace: pe_rle_workspace_bytes(N, units, columns). */
+int64_t pe_rle_workspace_bytes(int64_t N, int64_t units, int64_t columns);
+int pe_rle_decode(int64_t N, const int64_t* table, const uint8_t* data, int64_t units, int64_t columns,
+                  int64_t max_width, uint32_t* bits, int32_t* error, void* workspace, int64_t workspace_bytes,
+                  void* stream);
+
 /* pe_mask_overlap_batch over bit-packed masks: pm/gm are device arrays of B
  * pointers to bit rows (16-byte aligned); intersections are popcounts of
  * ANDed words.  Same arguments, outputs and replaced reference interface as

@@ -23,4 +23,4 @@ vectors written from the reference in the development container
 (tests/golden/make_golden.py).
 """
 
-from . import panoptic, instance, combined, prefix  # noqa: F401
+from . import panoptic, instance, combined, prefix, rle  # noqa: F401

@@ -1,0 +1,281 @@
+// COCO run-length encoded (RLE) instance masks decoded into bit rows on
+// gfx950.
+//
+// The reference hands its masks to pycocotools as RLE in its parity tests
+// (tests/test_instance.py:110, examples/benchmark_instance.py:42) and takes
+// bool bytes otherwise.  Here a caller holding COCO annotations keeps them
+// compressed up to the GPU: the strings (or count lists) are uploaded as
+// they are and decoded into the bit rows of bit-packed masks (bit p % 32 of
+// word p / 32 is pixel p = y * W + x), which the instance and combined
+// evaluators read directly.
+//
+// Format (pycocotools 2.0, common/maskApi.c): pixels in column-major order
+// (pixel (y, x) at x * H + y); counts are alternating runs of 0s and 1s,
+// starting with 0s, summing to H * W.  The compressed string stores count
+// i > 2 as its difference from count i - 2, every value as little-endian
+// 5-bit groups of its two's complement, one character 48 + group (+ 32 when
+// a group follows), the last group's bit 4 being the sign.
+//
+// Kernels:
+//  * rle_runs: one wave per mask.  The string is read 64 characters per step:
+//    a ballot of the groups without continuation bit marks where counts end,
+//    each ending lane sums its count's groups from the lanes before it
+//    (shuffles), and the delta coding resolves as two inclusive scans, one
+//    over even and one over odd count indices (count i > 2 is the sum of
+//    the stored values of indices i, i - 2, ..., down to 1 or 2).  A third
+//    scan gives every run's end position.  The lane that ends a run also
+//    writes, for every column whose first pixel lies in that run, the run's
+//    index (a per-column entry point for the raster kernel).  Malformed
+//    input (a character outside the alphabet, a count of more than seven
+//    groups, a negative count, a total other than H * W) marks the mask and
+//    sets an error bit; the raster kernel leaves such masks all-zero.
+//  * rle_raster: four waves per (mask, 256 columns), lane = column.  Each
+//    lane walks its column's runs down the image 32 rows at a time and forms
+//    the column's 32-bit word of the block; 32 ballots turn the 64 column
+//    words of a wave into 32 rows of 64 pixels, and lane t writes row t's
+//    segment.  Only nonzero words are written into the zero-filled rows:
+//    with W a multiple of 32 every word has one writer (plain stores), else
+//    segments straddle words and are merged with atomicOr.
+#include <climits>
+
+#include "common.h"
+#include "../../include/panoptic_hip.h"
+
+namespace pe {
+
+// Per-mask table row (int64 x 8), written by the host.
+struct RleMask {
+  i64 data_off;  // byte offset of the mask's characters or int32 counts in `data`
+  i64 units;     // characters (format 0) or counts (format 1)
+  i64 out_off;   // word offset of the mask's bit row in `bits`
+  i64 col_off;   // offset of the mask's W column entries in the column table
+  i64 height;
+  i64 width;
+  i64 format;    // 0: compressed string, 1: int32 counts
+  i64 run_off;   // offset of the mask's run ends (units entries)
+};
+static_assert(sizeof(RleMask) == 64, "the host writes eight int64 per mask");
+
+constexpr int kRleErrBadChar = 1;      // a character outside '0'..'o'
+constexpr int kRleErrLongCount = 2;    // a count of more than seven groups, or an unterminated last count
+constexpr int kRleErrNegative = 4;     // a negative count after delta decoding
+constexpr int kRleErrTotal = 8;        // counts not summing to H * W
+
+__device__ __forceinline__ i64 wave_inclusive_scan(i64 v, int lane) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const i64 t = __shfl_up(v, d, kWave);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(256) void rle_runs(i64 N, const RleMask* table, const uint8_t* data, int32_t* ends,
+                                                int32_t* col_run, int32_t* nruns, int32_t* error) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const i64 n = static_cast<i64>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (n >= N) return;  // uniform per wave
+  const RleMask m = table[n];
+  const i64 H = m.height, W = m.width, total = H * W;
+  const uint8_t* chars = data + m.data_off;
+  const int32_t* given = reinterpret_cast<const int32_t*>(data + m.data_off);
+  int32_t* run_end = ends + m.run_off;
+  int32_t* cols = col_run + m.col_off;
+  int bad = 0;
+  i64 count_base = 0;                  // counts completed so far
+  i64 pend_val = 0;                    // groups of a count still open at the end of a step
+  int pend_k = 0;                      // how many groups it has
+  i64 chain_even = 0, chain_odd = 0;   // running sums of the stored values (delta decoding)
+  i64 position = 0;                    // end position of the last completed run
+  for (i64 base = 0; base < m.units; base += kWave) {
+    const i64 u = base + lane;
+    const bool in = u < m.units;
+    i64 value = 0;    // this lane's count (valid where `ends_here`)
+    bool ends_here;
+    u64 emask;
+    if (m.format == 0) {
+      const int v = in ? static_cast<int>(chars[u]) - 48 : 0;
+      if (in && (v < 0 || v > 63)) bad |= kRleErrBadChar;
+      const int group = v & 0x1F;
+      ends_here = in && !(v & 0x20);
+      emask = __ballot(ends_here);
+      // The lanes of this step that belong to this lane's count: from the
+      // lane after the previous ending lane (or lane 0, continuing the count
+      // left open by the previous step).
+      const u64 before = emask & ((1ULL << lane) - 1);
+      const int start = before ? 64 - __clzll(before) : 0;
+      const int k = lane - start + (start == 0 ? pend_k : 0);
+      if (in && k > 6) bad |= kRleErrLongCount;
+      const i64 term = in ? static_cast<i64>(static_cast<u64>(group) << (5 * min(k, 11))) : 0;
+      i64 sum = term;
+#pragma unroll
+      for (int d = 1; d <= 6; ++d) {
+        const i64 t = __shfl_up(term, d, kWave);
+        if (lane - d >= start) sum += t;
+      }
+      if (start == 0) sum += pend_val;
+      value = sum;
+      if (ends_here && (group & 0x10)) value |= static_cast<i64>(-1) << (5 * min(k + 1, 62));
+      // The count left open at the end of this step (if the last character
+      // carries on): its groups so far.
+      const int last = static_cast<int>(min<i64>(kWave, m.units - base)) - 1;
+      const bool open = !((emask >> last) & 1ULL);
+      const i64 open_sum = __shfl(sum, last, kWave);
+      const int open_k = __shfl(k, last, kWave) + 1;
+      pend_val = open ? open_sum : 0;
+      pend_k = open ? open_k : 0;
+    } else {
+      ends_here = in;
+      emask = __ballot(in);
+      value = in ? static_cast<i64>(given[u]) : 0;
+    }
+    // Count index of each ending lane, and the delta decoding of the string
+    // form: count i > 2 adds count i - 2, i.e. the stored values of i, i - 2,
+    // ... down to index 1 (odd) or 2 (even).
+    const i64 idx = count_base + __popcll(emask & ((1ULL << lane) - 1));
+    i64 count = value;
+    if (m.format == 0) {
+      const bool chained = ends_here && idx >= 1;
+      const i64 even = wave_inclusive_scan(chained && !(idx & 1) ? value : 0, lane);
+      const i64 odd = wave_inclusive_scan(chained && (idx & 1) ? value : 0, lane);
+      if (chained) count = (idx & 1) ? chain_odd + odd : chain_even + even;
+      chain_even += __shfl(even, kWave - 1, kWave);
+      chain_odd += __shfl(odd, kWave - 1, kWave);
+    }
+    if (ends_here && count < 0) bad |= kRleErrNegative;
+    const i64 clamped = ends_here && count > 0 ? count : 0;
+    const i64 end = position + wave_inclusive_scan(clamped, lane);
+    position = __shfl(end, kWave - 1, kWave);
+    if (ends_here) {
+      const i64 lo = end - clamped;
+      if (end > total) bad |= kRleErrTotal;
+      if (idx < m.units) run_end[idx] = static_cast<int32_t>(min<i64>(end, INT_MAX));
+      // Columns whose first pixel lies in this run.
+      if (clamped > 0 && end <= total) {
+        const i64 x0 = (lo + H - 1) / H, x1 = (end - 1) / H;
+        for (i64 x = x0; x <= x1; ++x) cols[x] = static_cast<int32_t>(idx);
+      }
+    }
+    count_base += __popcll(emask);
+  }
+  if (pend_k) bad |= kRleErrLongCount;
+  if (position != total) bad |= kRleErrTotal;
+  // Any lane's error marks the whole mask.
+  const bool any_bad = __ballot(bad != 0) != 0;
+  int bits = bad;
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) bits |= __shfl_xor(bits, d, kWave);
+  if (lane == 0) {
+    nruns[n] = any_bad ? -1 : static_cast<int32_t>(count_base);
+    if (any_bad) atomicOr(error, bits);
+  }
+}
+
+// Four waves per (mask, 256-column group); lane = column x.  Rows are
+// produced 32 at a time: the column's word of the block from its runs, then
+// 32 ballots give 64-pixel row segments.
+__global__ __launch_bounds__(256) void rle_raster(i64 N, int groups, const RleMask* table, const int32_t* ends,
+                                                  const int32_t* col_run, const int32_t* nruns, uint32_t* bits) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const i64 n = blockIdx.x / groups;
+  const int band = (blockIdx.x - static_cast<int>(n) * groups) * 4 + (threadIdx.x >> 6);
+  if (n >= N) return;
+  const RleMask m = table[n];
+  const int runs = nruns[n];
+  const i64 H = m.height, W = m.width;
+  const i64 x0 = static_cast<i64>(band) * kWave;
+  if (runs <= 0 || x0 >= W) return;  // uniform per wave
+  const i64 x = x0 + lane;
+  const bool column = x < W;
+  const int32_t* run_end = ends + m.run_off;
+  uint32_t* out = bits + m.out_off;
+  // The run holding the column's current row, with its [start, end).
+  int r = column ? col_run[m.col_off + x] : 0;
+  i64 r_end = column ? run_end[r] : 0;
+  i64 r_start = column && r > 0 ? run_end[r - 1] : 0;
+  const bool words_aligned = (W & 31) == 0;
+  for (i64 y0 = 0; y0 < H; y0 += 32) {
+    const int rows = static_cast<int>(min<i64>(32, H - y0));
+    uint32_t word = 0;
+    if (column) {
+      const i64 lo = x * H + y0, hi = lo + rows;
+      // Runs overlapping [lo, hi): set the 1-runs' bits, advance past the
+      // runs that end inside the block.
+      while (true) {
+        if (r & 1) {
+          const i64 a = max(r_start, lo), b = min(r_end, hi);
+          if (b > a) {
+            const int s = static_cast<int>(a - lo), len = static_cast<int>(b - a);
+            word |= (len == 32 ? 0xFFFFFFFFu : ((1u << len) - 1u)) << s;
+          }
+        }
+        if (r_end >= hi || r + 1 >= runs) break;
+        ++r;
+        r_start = r_end;
+        r_end = run_end[r];
+      }
+    }
+    // Row t of the block: bit j = column x0 + j.
+    u64 segment = 0;
+#pragma unroll
+    for (int t = 0; t < 32; ++t) {
+      const u64 row_bits = __ballot((word >> t) & 1u);
+      if (lane == t) segment = row_bits;
+    }
+    if (lane < rows && segment) {
+      const i64 p = (y0 + lane) * W + x0;  // first pixel of the segment
+      const i64 w0 = p >> 5;
+      const int shift = static_cast<int>(p & 31);
+      if (words_aligned) {
+        const uint32_t lo32 = static_cast<uint32_t>(segment), hi32 = static_cast<uint32_t>(segment >> 32);
+        if (lo32) out[w0] = lo32;
+        if (hi32) out[w0 + 1] = hi32;
+      } else {
+        const uint32_t parts[3] = {static_cast<uint32_t>(segment << shift),
+                                   static_cast<uint32_t>(shift ? segment >> (32 - shift) : segment >> 32),
+                                   static_cast<uint32_t>(shift ? segment >> (64 - shift) : 0)};
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          if (parts[k]) atomicOr(out + w0 + k, parts[k]);
+      }
+    }
+  }
+}
+
+}  // namespace pe
+
+using namespace pe;
+
+extern "C" int64_t pe_rle_workspace_bytes(int64_t N, int64_t units, int64_t columns) {
+  if (N < 0 || units < 0 || columns < 0) return -1;
+  Carve carve(nullptr);
+  carve.take<int32_t>(units);
+  carve.take<int32_t>(columns);
+  carve.take<int32_t>(N);
+  carve.take<int32_t>(1);
+  return static_cast<int64_t>(align_up(carve.used, 256));
+}
+
+extern "C" int pe_rle_decode(int64_t N, const int64_t* table, const uint8_t* data, int64_t units, int64_t columns,
+                             int64_t max_width, uint32_t* bits, int32_t* error, void* workspace,
+                             int64_t workspace_bytes, void* stream_ptr) {
+  PE_REQUIRE(N >= 0 && N < INT_MAX / 4 && units >= 0 && columns >= 0 && max_width >= 0 && max_width < INT_MAX,
+             "invalid RLE shapes");
+  PE_REQUIRE(workspace_bytes >= pe_rle_workspace_bytes(N, units, columns), "workspace too small");
+  PE_REQUIRE(error != nullptr, "the error word is required");
+  if (N == 0) return kOk;
+  hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+  Carve carve(workspace);
+  int32_t* ends = carve.take<int32_t>(units);
+  int32_t* col_run = carve.take<int32_t>(columns);
+  int32_t* nruns = carve.take<int32_t>(N);
+  const auto* rows = reinterpret_cast<const RleMask*>(table);
+  rle_runs<<<static_cast<unsigned>((N + 3) / 4), 256, 0, stream>>>(N, rows, data, ends, col_run, nruns, error);
+  PE_LAUNCH_CHECK();
+  const int groups = static_cast<int>((max_width + 4 * kWave - 1) / (4 * kWave));
+  if (groups == 0) return kOk;
+  PE_REQUIRE(N * groups < INT_MAX, "too many RLE masks for one launch");
+  rle_raster<<<static_cast<unsigned>(N * groups), 256, 0, stream>>>(N, groups, rows, ends, col_run, nruns, bits);
+  PE_LAUNCH_CHECK();
+  return kOk;
+}

@@ -9,6 +9,7 @@ from .evaluator import PanopticBatch, PanopticEvaluator, panoptic_quality
 from .instance import InstanceBatch, InstanceEvaluator, PackedMasks, pack_masks
 from .combined import SegmentationEvaluator
 from .prefix import segmentation_prefixes
+from .rle import decode_rle, decode_rle_batch
 
 __all__ = [
     "PanopticBatch",
@@ -20,4 +21,6 @@ __all__ = [
     "segmentation_prefixes",
     "PackedMasks",
     "pack_masks",
+    "decode_rle",
+    "decode_rle_batch",
 ]

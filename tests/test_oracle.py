@@ -524,3 +524,44 @@ def test_torch_restatement_random_batches():
     got, want = _torch_and_numpy(C, isthing, batches)
     _assert_same_results(got, want)
     assert int(got["tp"].sum()) > 0 and (got["instance"]["precision"] > 0).any()
+
+
+# ---------------------------------------------------------------- COCO RLE
+
+def test_rle_string_known_answers():
+    """Hand-derived strings of pycocotools' compressed RLE: single groups,
+    a multi-group count, the delta from count i - 2 (only for i > 2) and a
+    negative delta with its sign bit."""
+    from oracle import rle
+    cases = [
+        ([4], b"4"),              # all-zero 2 x 2
+        ([0, 4], b"04"),          # all-one 2 x 2: an empty leading zero run
+        ([40], b"X1"),            # 40 = 8 + 32 * 1: group 8 with the continuation bit, then 1
+        ([3, 2, 5, 2], b"3250"),  # count 3 is stored as 2 - 2 = 0; count 2 as it is
+        ([1, 5, 1, 1], b"151L"),  # count 3: 1 - 5 = -4, one group 28 whose bit 4 is the sign
+        ([16], b"`0"),            # 16 has bit 4 set: it takes the continuation bit (96), then group 0
+        ([0, 1, 0, 0, 1], b"010O1"),  # count 3: 0 - 1 = -1 is group 31 (79); count 4: 1 - 0
+    ]
+    for counts, text in cases:
+        assert rle.counts_to_string(counts) == text, counts
+        assert rle.counts_from_string(text) == counts, text
+
+
+def test_rle_round_trip_and_identities():
+    from oracle import rle
+    rng = np.random.default_rng(11)
+    for h, w in ((1, 1), (1, 7), (5, 1), (3, 4), (17, 33), (32, 64), (40, 50)):
+        for density in (0.0, 0.05, 0.5, 0.97, 1.0):
+            mask = rng.random((h, w)) < density
+            code = rle.encode(mask)
+            assert code["size"] == [h, w]
+            counts = rle.counts_from_string(code["counts"])
+            assert sum(counts) == h * w
+            assert rle.area(code) == int(mask.sum())
+            assert np.array_equal(rle.decode(code), mask)
+            assert np.array_equal(rle.decode({"size": [h, w], "counts": counts}), mask)
+            # Column-major: the first run covers column 0 from the top.
+            first = mask[:, 0].argmax() if mask[:, 0].any() else h
+            assert counts[0] == (first if first < h else counts[0])
+    with pytest.raises(ValueError):
+        rle.decode({"size": [2, 2], "counts": [1, 2]})
