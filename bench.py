@@ -9,8 +9,11 @@ DEVELOPMENT.md:47-49).  Two forms are timed over exactly K steps each:
 * plain   -- the drop-in API called eagerly: one update() per batch for the
              uniform configs, one update() per image for the uneven batch
              (the reference cannot batch uneven sizes, README.md:57-58);
-* batched -- the whole batch (packed when uneven) in one update() recorded as
-             a HIP graph and replayed, then compute().
+* batched -- the whole batch (packed when uneven) in one update(): issued
+             eagerly, with the update replayed from a HIP graph and compute()
+             eager, or with reset, update and compute replayed as one HIP
+             graph (graphs.CapturedStep) -- whichever an untimed probe finds
+             fastest; the line names the form.
 
 ``value`` is the batched form's images/sec.  The dominant kernel's average
 duration is measured with HIP events recorded by the library on the kernel's
@@ -784,6 +787,13 @@ def cpu_baselines(cfg, budget_s):
             "numpy_oracle": {"value": round(numpy_rate, 4), "cores": 1, "sample": numpy_sample}}
 
 
+BATCHED_FORMS = {
+    "eager": "batched: reset(), one update() of the whole batch and compute(), eager",
+    "graph": "batched: reset(), one update() of the whole batch replayed as a HIP graph, compute()",
+    "step_graph": "batched: reset(), one update() of the whole batch and compute(), replayed as one HIP graph",
+}
+
+
 def measure_steps(cfg, args, *, cpu_budget=None):
     """Plain and batched forms of a step config as a bench-line dictionary."""
     import torch
@@ -824,19 +834,21 @@ def measure_steps(cfg, args, *, cpu_budget=None):
 
     batched_elapsed, batched_form, probe = None, None, None
     if not args.plain_only:
-        from panoptic_evaluator.graphs import CapturedUpdate
-        log(f"[bench] {cfg.name}: capturing the batched update as a HIP graph")
+        from panoptic_evaluator.graphs import CapturedStep, CapturedUpdate
+        log(f"[bench] {cfg.name}: capturing the batched update, and the whole step, as HIP graphs")
         captured = CapturedUpdate(cfg.evaluator, *cfg.update_args())
+        stepped = CapturedStep(cfg.evaluator, *cfg.update_args())
 
         def graph_step():
             cfg.evaluator.reset()
             captured.replay()
             return cfg.evaluator.compute()
 
-        # The batched update is issued eagerly or replayed from the graph,
-        # whichever an untimed probe finds faster on this workload (graph
+        # The batched step is issued eagerly, with its update replayed from a
+        # graph, or replayed whole (reset, update and compute in one graph),
+        # whichever an untimed probe finds fastest on this workload (graph
         # launch costs more than a few direct launches when the batch is small).
-        forms = {"eager": eager_batched, "graph": graph_step}
+        forms = {"eager": eager_batched, "graph": graph_step, "step_graph": stepped.replay}
         for fn in forms.values():
             for _ in range(args.warmup):
                 fn()
@@ -867,9 +879,7 @@ def measure_steps(cfg, args, *, cpu_budget=None):
         "vs_baseline": None,
         "dtype": cfg.dtype,
         "data": "synthetic",
-        "config": dict(cfg.config, form=("plain" if batched_elapsed is None else
-                                         "batched: one update() of the whole batch, "
-                                         + ("replayed as a HIP graph" if batched_form == "graph" else "eager")),
+        "config": dict(cfg.config, form=("plain" if batched_elapsed is None else BATCHED_FORMS[batched_form]),
                        batched_probe_ms_per_step=probe),
         "plain": {"value": round(plain_rate, 3), "ms_per_step": round(plain_elapsed / args.steps * 1e3, 4),
                   "form": "drop-in API, eager, " + ("one update() per image" if uneven else "one update() per batch")},

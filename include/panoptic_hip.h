@@ -47,6 +47,16 @@ int pe_event_create(void** event);
 int pe_event_destroy(void* event);
 int pe_stream_wait(void* waiting, void* signalling, void* event);
 
+/* Marks (on != 0) or unmarks a stream whose launches run beside a long
+ * kernel on another stream (no counterpart in the reference).  The
+ * library's latency-bound kernels (segment scoring, iou conversion) then
+ * launch 256-thread workgroups on it instead of 1024-thread ones: a
+ * 1024-thread workgroup needs 16 free wave slots on one compute unit and
+ * waits until the other kernel drains, a narrow one fits beside it.
+ * Results do not depend on the mark.  Used for the combined evaluator's
+ * side stream. */
+int pe_stream_set_concurrent(void* stream, int on);
+
 /* Benchmark timing hooks (no counterpart in the reference): while enabled,
  * HIP events are recorded on the launch stream around every launch of the
  * library's dominant kernels (by kernel name, e.g. "joint_histogram");
@@ -310,6 +320,40 @@ int pe_coco_match(int64_t B, int64_t C, const int64_t* det_off, const int64_t* g
                   const int32_t* guard, const uint8_t* gt_things, const int64_t* gt_area_i64,
                   int64_t gt_area_stride, int zero_inter, void* stream);
 
+/* pe_coco_match for the shared-target combined path, split in two so that
+ * the part needing no intersections runs beside the detection histogram.
+ * Replaces the same matching interface as pe_coco_match (the reference's
+ * combined.py:90-161 matching over the target's thing segments;
+ * instance_kernel.cu:41-174), restricted to its combined-path case: GT are
+ * the S target slots of each image (gt_cls B x S int32, -1 = not a
+ * candidate), intersections are the detection x slot int64 rows of
+ * pe_detection_histogram (image b's rows start at row det_off[b]; row of a
+ * ranked detection = det_row[i], its mask index), and the rows are zeroed
+ * after matching.
+ *
+ * pe_match_groups (no intersections needed): per image, groups the
+ * detections (score order within a class; det_cls int32 score order) and
+ * the candidate slots (class in range and, if gt_things is given, a thing
+ * class) by class into `groups` (pe_match_groups_bytes), and writes every
+ * detection's rank (1..100 within its class, else 0) and the flags (-1) of
+ * the unranked ones.  At most 4096 detections and slots per image.
+ *
+ * pe_coco_match_grouped (after the histogram): one wave per (image, class)
+ * reads its lists, counts the class's positives per area range into
+ * counts (C x 4, accumulated), matches the ranked detections (flags into
+ * status, (sum D) x 40) and zeroes their rows.  Detection areas are the
+ * row sums; GT pixel areas are gt_area_i64[b * gt_area_stride + j]; gt_ann
+ * (B x S f64, may be NULL) replaces them for the area ranges. */
+int64_t pe_match_groups_bytes(int64_t B, int64_t C, int64_t total_dets, int64_t S);
+int pe_match_groups(int64_t B, int64_t C, int64_t S, const int64_t* det_off, int64_t total_dets,
+                    int64_t max_dets, const int32_t* det_cls, const int64_t* det_row, const int32_t* gt_cls,
+                    const uint8_t* gt_things, int8_t* status, int32_t* ranks, void* groups,
+                    int64_t groups_bytes, void* stream);
+int pe_coco_match_grouped(int64_t B, int64_t C, int64_t S, const int64_t* det_off, int64_t total_dets,
+                          const void* groups, int64_t groups_bytes, const uint8_t* gt_crowd,
+                          const double* gt_ann, const int64_t* gt_area_i64, int64_t gt_area_stride,
+                          int64_t* pairs, int8_t* status, int64_t* counts, void* stream);
+
 /* Scratch bytes of pe_score_order for N detections in B images. */
 int64_t pe_score_order_workspace_bytes(int64_t N, int64_t B);
 
@@ -367,6 +411,33 @@ int pe_compute_with_ap(const int64_t* tp, const int64_t* fp, const int64_t* fn, 
                        const int32_t* ranks, const int8_t* status, const int64_t* ap_counts, int64_t ap_C,
                        double* precision, double* recall, double* summary, void* workspace,
                        int64_t workspace_bytes, void* stream);
+
+/* pe_compute_with_ap with the grouped matching of the last update done in
+ * the same launch (no reference counterpart; used by the combined
+ * evaluator's compute() after a graph-replayed update, whose matching it
+ * defers).  The records (N, scores, classes, ranks, status) are that
+ * update's: ranks and the unranked flags come from pe_match_groups, and the
+ * ranked flags are written into `status` here; positives are added into
+ * ap_counts.  The arguments after `workspace_bytes` are
+ * pe_coco_match_grouped's (B, S, det_off, groups, groups_bytes, gt_crowd,
+ * gt_ann, gt_area_i64, gt_area_stride, pairs; total_dets = N).  With at
+ * most 4096 records each AP class workgroup first matches its class in every
+ * image, then accumulates; otherwise the matcher is launched first.  `done`
+ * (optional): a device int32 that is zero between calls; the last AP class
+ * workgroup to finish then folds the summaries in the same launch and zeroes
+ * it again (calls sharing it must not overlap).  Same outputs as
+ * pe_coco_match_grouped followed by pe_compute_with_ap. */
+int pe_compute_with_ap_match(const int64_t* tp, const int64_t* fp, const int64_t* fn, const double* sums,
+                             const int64_t* confusion, const uint8_t* things, int64_t K, int64_t C,
+                             double* per_class, double* summaries, int64_t* counts, double* miou,
+                             int64_t* out_tp, int64_t* out_fp, int64_t* out_fn, double* out_sums,
+                             int64_t* out_confusion, int64_t N, const double* scores, const int32_t* classes,
+                             const int32_t* ranks, int8_t* status, int64_t* ap_counts, int64_t ap_C,
+                             double* precision, double* recall, double* summary, void* workspace,
+                             int64_t workspace_bytes, int64_t B, int64_t S, const int64_t* det_off,
+                             const void* groups, int64_t groups_bytes, const uint8_t* gt_crowd,
+                             const double* gt_ann, const int64_t* gt_area_i64, int64_t gt_area_stride,
+                             int64_t* pairs, int32_t* done, void* stream);
 
 /* out[r] = sum of int64 row r as f64 (detection areas of a shared target
  * histogram, combined.py:107-108). */

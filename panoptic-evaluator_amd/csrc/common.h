@@ -55,6 +55,13 @@ int fail(int code, const char* fmt, ...);
 // Compute units of the current device (cached per device ordinal).
 int compute_units();
 
+// Whether `stream` was marked (pe_stream_set_concurrent) as running beside
+// another stream's long kernel: latency-bound kernels launched on it then
+// use workgroups small enough to fit beside that kernel's workgroups.
+bool stream_concurrent(hipStream_t stream);
+// Workgroup size of the latency-bound kernels on a concurrent stream.
+constexpr int kConcurrentThreads = 256;
+
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 // Bump allocator over a caller-provided workspace.  Every slice is 256-byte

@@ -2,7 +2,10 @@
 // and the zero-fill kernel used instead of hipMemsetAsync (so that callers may
 // capture any entry point in a HIP graph).
 #include <cstdarg>
+#include <algorithm>
+#include <atomic>
 #include <mutex>
+#include <vector>
 
 #include "common.h"
 #include "../../include/panoptic_hip.h"
@@ -185,6 +188,36 @@ extern "C" int pe_stream_wait(void* waiting, void* signalling, void* event) {
   PE_HIP(hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(signalling)));
   PE_HIP(hipStreamWaitEvent(static_cast<hipStream_t>(waiting), static_cast<hipEvent_t>(event), 0));
   return pe::kOk;
+}
+
+// ------------------------------------------------- concurrent streams
+
+namespace pe {
+namespace {
+std::mutex concurrent_lock;
+std::vector<hipStream_t> concurrent_streams;
+std::atomic<int> concurrent_count{0};
+}  // namespace
+
+bool stream_concurrent(hipStream_t stream) {
+  if (concurrent_count.load(std::memory_order_relaxed) == 0) return false;
+  std::lock_guard<std::mutex> guard(concurrent_lock);
+  for (hipStream_t s : concurrent_streams)
+    if (s == stream) return true;
+  return false;
+}
+}  // namespace pe
+
+extern "C" int pe_stream_set_concurrent(void* stream, int on) {
+  using namespace pe;
+  PE_REQUIRE(stream, "the default stream cannot be marked");
+  std::lock_guard<std::mutex> guard(concurrent_lock);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  auto it = std::find(concurrent_streams.begin(), concurrent_streams.end(), s);
+  if (on && it == concurrent_streams.end()) concurrent_streams.push_back(s);
+  if (!on && it != concurrent_streams.end()) concurrent_streams.erase(it);
+  concurrent_count.store(static_cast<int>(concurrent_streams.size()), std::memory_order_relaxed);
+  return kOk;
 }
 
 // ------------------------------------------------------------ timing hooks

@@ -1123,6 +1123,13 @@ struct MatchArgs {
   const i64* gt_area_i64;     // optional: int64 pixel areas at b * gt_area_stride + j
   i64 gt_area_stride;
   int zero_inter;             // zero every detection's intersection row after matching
+  // Grouped form (coco_match_grouped): per-image class lists made beforehand
+  // by match_groups; GT are the S target slots of each image and the rows
+  // are int64 rows of S columns (image b's block starts at row det_off[b]).
+  const int4* group_hdr;      // B x (C + 1): det start, det count, gt start, gt count
+  const int2* group_det;      // sum D: (score-order index, row) grouped by class
+  const int32_t* group_gt;    // B x S: slot indices grouped by class
+  int S;
 };
 
 // GT candidate class (or -1), pixel area and annotation area of GT j of image b.
@@ -1357,7 +1364,7 @@ __device__ __forceinline__ void match_chains(const MatchArgs& a, int b, i64 d0, 
                                              const double* det_pa, const double* pair_inter, const double* gt_px,
                                              const double* gt_an, const uint8_t* gt_cr, const int* gts,
                                              uint32_t* used) {
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
   const int range = lane / 10, t = lane - (lane / 10) * 10;
   double lo, hi;
   area_range(range, lo, hi);
@@ -1423,6 +1430,256 @@ __device__ __forceinline__ void match_chains(const MatchArgs& a, int b, i64 d0, 
   }
 }
 
+// ------------------------------------------------ grouped COCO matching
+//
+// The shared-target (combined) path splits matching in two.  match_groups
+// needs only the score order and the target slot classes, so it runs beside
+// the detection histogram: per image it groups the detections (score order
+// kept within a class) and the candidate GT slots by class, writes every
+// detection's rank and the flags of the unranked ones.  coco_match_grouped,
+// after the histogram, then reads its (image, class) lists directly: three
+// dependent rounds of loads (header, lists, intersections and GT metadata)
+// instead of scanning the image's detections and slots.
+
+struct GroupArgs {
+  int B, C, S;
+  const i64* det_off;
+  const int32_t* det_cls;    // score order
+  const i64* det_row;        // score order -> row (mask index within the image)
+  const int32_t* gt_cls;     // B x S
+  const uint8_t* gt_things;  // C or NULL: only GT of thing classes are candidates
+  int8_t* status;            // (sum D) x 40
+  int32_t* ranks;            // sum D
+  int4* hdr;
+  int2* det_list;
+  int32_t* gt_list;
+  int det_slots, gt_slots;   // LDS sort sizes (powers of two >= the largest D, >= S)
+};
+
+// Ascending bitonic sort of m (a power of two) 32-bit keys in LDS.
+__device__ __forceinline__ void bitonic_u32(uint32_t* keys, int m) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int k = 2; k <= m; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int e = tid; e < m; e += nt) {
+        const int f = e ^ j;
+        if (f <= e) continue;
+        const uint32_t ke = keys[e], kf = keys[f];
+        if ((ke > kf) == ((e & k) == 0)) {
+          keys[e] = kf;
+          keys[f] = ke;
+        }
+      }
+      __syncthreads();
+    }
+}
+
+// Keys are (class bucket << 12) | index: a stable grouping by class of at
+// most 4096 entries; bucket C holds the entries of no (candidate) class.
+constexpr int kGroupIndexBits = 12;
+constexpr int kGroupMaxEntries = 1 << kGroupIndexBits;
+
+__global__ __launch_bounds__(256) void match_groups(GroupArgs g) {
+  extern __shared__ uint4 lds4[];
+  uint32_t* dkey = reinterpret_cast<uint32_t*>(lds4);
+  uint32_t* gkey = dkey + g.det_slots;
+  int* dstart = reinterpret_cast<int*>(gkey + g.gt_slots);
+  int* dend = dstart + (g.C + 1);
+  int* gstart = dend + (g.C + 1);
+  int* gend = gstart + (g.C + 1);
+  const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x, C = g.C, S = g.S;
+  const i64 d0 = g.det_off[b];
+  const int D = static_cast<int>(min<i64>(g.det_off[b + 1] - d0, g.det_slots));  // the host sizes det_slots
+  for (int c = tid; c <= C; c += nt) dstart[c] = dend[c] = gstart[c] = gend[c] = 0;
+  for (int e = tid; e < g.det_slots; e += nt) {
+    uint32_t key = 0xFFFFFFFFu;
+    if (e < D) {
+      const int c = g.det_cls[d0 + e];
+      key = (static_cast<uint32_t>(c >= 0 && c < C ? c : C) << kGroupIndexBits) | static_cast<uint32_t>(e);
+    }
+    dkey[e] = key;
+  }
+  for (int j = tid; j < g.gt_slots; j += nt) {
+    uint32_t key = 0xFFFFFFFFu;
+    if (j < S) {
+      const int c = g.gt_cls[static_cast<i64>(b) * S + j];
+      const bool candidate = c >= 0 && c < C && (!g.gt_things || g.gt_things[c]);
+      key = (static_cast<uint32_t>(candidate ? c : C) << kGroupIndexBits) | static_cast<uint32_t>(j);
+    }
+    gkey[j] = key;
+  }
+  __syncthreads();
+  bitonic_u32(dkey, g.det_slots);
+  bitonic_u32(gkey, g.gt_slots);
+  for (int p = tid; p < D; p += nt) {
+    const uint32_t k = dkey[p] >> kGroupIndexBits;
+    if (p == 0 || (dkey[p - 1] >> kGroupIndexBits) != k) dstart[k] = p;
+    if (p == D - 1 || (dkey[p + 1] >> kGroupIndexBits) != k) dend[k] = p + 1;
+  }
+  for (int p = tid; p < S; p += nt) {
+    const uint32_t k = gkey[p] >> kGroupIndexBits;
+    if (p == 0 || (gkey[p - 1] >> kGroupIndexBits) != k) gstart[k] = p;
+    if (p == S - 1 || (gkey[p + 1] >> kGroupIndexBits) != k) gend[k] = p + 1;
+  }
+  __syncthreads();
+  for (int p = tid; p < D; p += nt) {
+    const uint32_t key = dkey[p];
+    const int k = static_cast<int>(key >> kGroupIndexBits), e = static_cast<int>(key & (kGroupMaxEntries - 1));
+    const int rank = p - dstart[k];
+    g.det_list[d0 + p] = int2{e, static_cast<int>(g.det_row[d0 + e])};
+    const bool ranked = k < C && rank < 100;
+    g.ranks[d0 + e] = ranked ? rank + 1 : 0;
+    if (!ranked) {
+      // Rank 0 and flags -1 (five 8-byte words of -1; rows are 8-byte aligned).
+      uint64_t* row = reinterpret_cast<uint64_t*>(g.status + (d0 + e) * 40);
+#pragma unroll
+      for (int w = 0; w < 5; ++w) row[w] = ~0ULL;
+    }
+  }
+  for (int p = tid; p < S; p += nt)
+    g.gt_list[static_cast<i64>(b) * S + p] = static_cast<int32_t>(gkey[p] & (kGroupMaxEntries - 1));
+  int4* hdr = g.hdr + static_cast<i64>(b) * (C + 1);
+  for (int c = tid; c <= C; c += nt) hdr[c] = int4{dstart[c], dend[c] - dstart[c], gstart[c], gend[c] - gstart[c]};
+}
+
+// Zero the int64 rows (S columns) of `count` listed detections of image b.
+__device__ __forceinline__ void zero_listed_rows(const MatchArgs& a, const int2* list, int count, i64 block) {
+  i64* rows = static_cast<i64*>(const_cast<void*>(a.inter)) + block;
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int base = 0; base < count; base += kWave) {
+    const int row = list[min(base + lane, count - 1)].y;
+    const int here = min(kWave, count - base);
+    for (int k = 0; k < here; ++k) {
+      const i64 r = __shfl(row, k, kWave);
+      for (int s = lane; s < a.S; s += kWave) rows[r * a.S + s] = 0;
+    }
+  }
+}
+
+// LDS of one matching wave (bytes, 16-byte multiple) for S target slots.
+__host__ __device__ inline size_t match_wave_lds(int S) {
+  const size_t words = (static_cast<size_t>(S) + 31) / 32;
+  return align_up(20 * 100 + 8 * static_cast<size_t>(kMatchStagePairs) + 20 * static_cast<size_t>(S) +
+                      align_up(static_cast<size_t>(S), 16) + 4 * 40 * words,
+                  16);
+}
+
+// Orders a wave's LDS writes before its later LDS reads by other lanes (the
+// waves of a workgroup may be matching different images).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// One wave matches class c of image b from the lists of match_groups:
+// round 1 the class's header, round 2 its lists, round 3 GT metadata,
+// intersections and detection row sums, then the chains; last it zeroes the
+// rows of the class's detections.  `lds` is this wave's match_wave_lds(S).
+__device__ void match_grouped_wave(const MatchArgs& a, int b, int c, char* lds) {
+  const int C = a.C, S = a.S;
+  const int lane = threadIdx.x & (kWave - 1);
+  double* det_pa = reinterpret_cast<double*>(lds);     // 100
+  i64* det_rw = reinterpret_cast<i64*>(det_pa + 100);  // 100
+  int* dets = reinterpret_cast<int*>(det_rw + 100);    // 100
+  double* pair_inter = reinterpret_cast<double*>(dets + 100);  // kMatchStagePairs
+  double* gt_px = pair_inter + kMatchStagePairs;               // S
+  double* gt_an = gt_px + S;                                   // S
+  int* gts = reinterpret_cast<int*>(gt_an + S);                // S
+  uint8_t* gt_cr = reinterpret_cast<uint8_t*>(gts + S);        // S
+  uint32_t* used = reinterpret_cast<uint32_t*>(gt_cr + ((S + 15) & ~15));  // 40 x words
+  // Round 1: the class's header (and the no-class bucket's for class 0).
+  const int4* hdr = a.group_hdr + static_cast<i64>(b) * (C + 1);
+  const int4 h = hdr[c];
+  const int4 hx = hdr[c == 0 ? C : c];
+  const i64 d0 = a.det_off[b], g0 = static_cast<i64>(b) * S, ib = d0 * S;
+  const int nd_all = h.y, ng = h.w, extra = c == 0 ? hx.y : 0;
+  if (nd_all == 0 && ng == 0 && extra == 0) return;  // uniform
+  const int nd = min(nd_all, 100);
+  // Round 2: the lists.
+  for (int r = lane; r < nd; r += kWave) {
+    const int2 v = a.group_det[d0 + h.x + r];
+    dets[r] = v.x;
+    det_rw[r] = v.y;
+  }
+  for (int q = lane; q < ng; q += kWave) gts[q] = a.group_gt[g0 + h.z + q];
+  wave_sync();
+  // Round 3: GT metadata (areas, crowd flags) and positives per area range.
+  int positives[4] = {0, 0, 0, 0};
+  for (int q0 = 0; q0 < ng; q0 += kWave) {
+    const int q = q0 + lane;
+    const int j = gts[min(q, ng - 1)];
+    const bool crowded = a.gt_crowd && a.gt_crowd[g0 + j];
+    const double px = static_cast<double>(a.gt_area_i64[b * a.gt_area_stride + j]);
+    const double ann = a.gt_ann ? a.gt_ann[g0 + j] : px;
+    if (q < ng) {
+      gt_cr[q] = crowded;
+      gt_px[q] = px;
+      gt_an[q] = ann;
+    }
+    const bool eligible = q < ng && !crowded;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      double lo, hi;
+      area_range(r, lo, hi);
+      positives[r] += __popcll(__ballot(eligible && ann >= lo && ann <= hi));
+    }
+  }
+  if (lane == 0 && ng)
+    for (int r = 0; r < 4; ++r)
+      if (positives[r]) atomicAdd(reinterpret_cast<u64*>(a.counts) + c * 4 + r, static_cast<u64>(positives[r]));
+  const bool staged = nd * ng <= kMatchStagePairs;
+  if (nd > 0) {
+    const i64* rows = static_cast<const i64*>(a.inter) + ib;
+    // Intersections of every (ranked detection, candidate) pair, four per
+    // lane per step, loads issued together.
+    if (staged && ng > 0) {
+      const int pairs = nd * ng;
+      for (int e0 = lane; e0 < pairs; e0 += 4 * kWave) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int e = min(e0 + u * kWave, pairs - 1), r = e / ng, q = e - r * ng;
+          v[u] = static_cast<double>(rows[det_rw[r] * S + gts[q]]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (e0 + u * kWave < pairs) pair_inter[e0 + u * kWave] = v[u];
+      }
+    }
+    // Detection areas: the sums of their rows, four rows at a time.
+    for (int r0 = 0; r0 < nd; r0 += 4) {
+      i64 sum[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const i64* row = rows + det_rw[min(r0 + u, nd - 1)] * S;
+        sum[u] = 0;
+        for (int s = lane; s < S; s += kWave) sum[u] += row[s];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const i64 total = wave_sum(sum[u]);
+        if (lane == 0 && r0 + u < nd) det_pa[r0 + u] = static_cast<double>(total);
+      }
+    }
+    for (int w = lane; w < 40 * a.words; w += kWave) used[w] = 0;
+    wave_sync();
+    if (lane < 40)
+      match_chains(a, b, d0, g0, S, ib, nd, ng, staged, dets, det_rw, det_pa, pair_inter, gt_px, gt_an, gt_cr, gts,
+                   used);
+  }
+  // Every intersection read is done: zero the rows of the class's detections
+  // (and, for class 0, of the detections of no valid class).
+  wave_sync();
+  if (nd_all) zero_listed_rows(a, a.group_det + d0 + h.x, nd_all, ib);
+  if (extra) zero_listed_rows(a, a.group_det + d0 + hx.x, extra, ib);
+}
+
+__global__ __launch_bounds__(64) void coco_match_grouped(MatchArgs a) {
+  extern __shared__ uint4 lds4[];
+  const int b = blockIdx.x / a.C, c = blockIdx.x - (blockIdx.x / a.C) * a.C;
+  match_grouped_wave(a, b, c, reinterpret_cast<char*>(lds4));
+}
+
 // ------------------------------------------------------------ AP / AR
 
 __device__ __forceinline__ u64 score_key_desc(double s) {
@@ -1466,6 +1723,10 @@ struct ApArgs {
   double* precision;         // 10 x 101 x C x 4 x 3
   double* recall;            // 10 x C x 4 x 3
   double* partial;           // C x 24 summary accumulators
+  // Optional: a device word that is zero between launches.  The last class
+  // workgroup to finish (counted on it) folds the summaries and zeroes it.
+  int* done;
+  double* summary;           // 12 summaries (with done)
 };
 
 // One workgroup per class (instance.py:300-331).  Its first four waves scan
@@ -1803,7 +2064,15 @@ __device__ __forceinline__ void ap_class(const ApArgs& a, int c, OrderPtr order,
         if (srv >= 0) acc += count ? 1.0 : srv;
       }
     }
-    a.partial[c * 24 + q] = acc;
+    if (a.done) {
+      // Read by the last class workgroup, possibly on another XCD: the
+      // exchange completes at the device's coherence point before this
+      // workgroup counts itself done.
+      atomicExch(reinterpret_cast<unsigned long long*>(a.partial + c * 24 + q),
+                 static_cast<unsigned long long>(__double_as_longlong(acc)));
+    } else {
+      a.partial[c * 24 + q] = acc;
+    }
   }
 }
 
@@ -1985,7 +2254,10 @@ size_t ap_small_lds(int slots) { return 12 * static_cast<size_t>(slots) + kApChu
 
 // instance.py:236-248: the 12 summaries as means over valid entries (>= 0),
 // from the per-class accumulators; classes are folded in a fixed order.
-__global__ __launch_bounds__(256) void ap_summaries(int C, const double* class_part, double* out) {
+// COHERENT: the accumulators were written by other workgroups of the same
+// launch (read at the device's coherence point).  Threads 0..255.
+template <bool COHERENT>
+__device__ __forceinline__ void fold_summaries(int C, const double* class_part, double* out) {
   __shared__ double red[24][10];
   const int tid = threadIdx.x;
   if (tid < 240) {
@@ -1996,11 +2268,16 @@ __global__ __launch_bounds__(256) void ap_summaries(int C, const double* class_p
       double v[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int c = c0 + 10 * u;
-        v[u] = c < C ? class_part[static_cast<i64>(c) * 24 + q] : 0.0;
+        const int c = min(c0 + 10 * u, C - 1);
+        const double* p = class_part + static_cast<i64>(c) * 24 + q;
+        v[u] = COHERENT ? __longlong_as_double(static_cast<long long>(__hip_atomic_load(
+                              reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT)))
+                        : *p;
       }
 #pragma unroll
-      for (int u = 0; u < U; ++u) acc += v[u];
+      for (int u = 0; u < U; ++u)
+        if (c0 + 10 * u < C) acc += v[u];
     }
     red[q][k] = acc;
   }
@@ -2012,6 +2289,53 @@ __global__ __launch_bounds__(256) void ap_summaries(int C, const double* class_p
       n += red[2 * tid + 1][k];
     }
     out[tid] = n > 0 ? s / n : -1.0;
+  }
+}
+
+__global__ __launch_bounds__(256) void ap_summaries(int C, const double* class_part, double* out) {
+  fold_summaries<false>(C, class_part, out);
+}
+
+// With a completion counter: the class workgroup that finishes last folds the
+// summaries (no separate launch) and zeroes the counter for the next launch.
+__device__ __forceinline__ void summaries_if_last(const ApArgs& a) {
+  __shared__ int last;
+  __syncthreads();  // this workgroup's accumulators are written (exchanged)
+  if (threadIdx.x == 0) last = atomicAdd(a.done, 1) == a.C - 1;
+  __syncthreads();
+  if (!last) return;
+  fold_summaries<true>(a.C, a.partial, a.summary);
+  if (threadIdx.x == 0) atomicExch(a.done, 0);
+}
+
+// Matching deferred to the AP launch: class workgroup c first matches class c
+// in every image of an update (wave w takes images w, w + waves, ..., each
+// wave in its own LDS region), then accumulates its records; the finaliser
+// rows run as extra workgroups, and (with a completion counter) the last
+// class workgroup folds the summaries.  One launch replaces the matcher, the
+// AP classes, the summaries and the finaliser.
+struct MatchPhase {
+  MatchArgs m;
+  int waves;       // matching waves per workgroup
+  int wave_lds;    // bytes of LDS per matching wave
+};
+
+__global__ __launch_bounds__(kApSmallThreads) void ap_small_match_final(ApSmallArgs s, FinalArgs f, MatchPhase p) {
+  extern __shared__ uint4 lds4[];
+  const int block = blockIdx.x;
+  if (block < s.ap.C) {
+    const int wave = threadIdx.x / kWave;
+    if (wave < p.waves)
+      for (int b = wave; b < p.m.B; b += p.waves)
+        match_grouped_wave(p.m, b, block, reinterpret_cast<char*>(lds4) + static_cast<size_t>(wave) * p.wave_lds);
+    // Flags and positives written above are read by other waves of this
+    // workgroup below (one compute unit: the barrier's workgroup-scope fence
+    // suffices).
+    __syncthreads();
+    ap_small_class(s, block, lds4);
+    if (s.ap.done) summaries_if_last(s.ap);
+  } else {
+    finalize_row(f, block - s.ap.C, lds4);
   }
 }
 
@@ -2251,6 +2575,136 @@ extern "C" int pe_coco_match(int64_t B, int64_t C, const int64_t* det_off, const
   return kOk;
 }
 
+namespace {
+struct GroupBuffers {
+  int4* hdr;
+  int2* det_list;
+  int32_t* gt_list;
+};
+
+GroupBuffers carve_groups(void* base, i64 B, i64 C, i64 total_dets, i64 S, size_t* used = nullptr) {
+  Carve carve(base);
+  GroupBuffers g{};
+  g.hdr = carve.take<int4>(B * (C + 1));
+  g.det_list = carve.take<int2>(total_dets);
+  g.gt_list = carve.take<int32_t>(B * S);
+  if (used) *used = carve.used;
+  return g;
+}
+
+int pow2_at_least(i64 n, int floor) {
+  int m = floor;
+  while (m < n) m <<= 1;
+  return m;
+}
+}  // namespace
+
+extern "C" int64_t pe_match_groups_bytes(int64_t B, int64_t C, int64_t total_dets, int64_t S) {
+  if (B < 0 || C < 0 || total_dets < 0 || S < 0) return -1;
+  size_t used = 0;
+  carve_groups(nullptr, B, C, total_dets, S, &used);
+  return static_cast<int64_t>(align_up(used, 256));
+}
+
+extern "C" int pe_match_groups(int64_t B, int64_t C, int64_t S, const int64_t* det_off, int64_t total_dets,
+                               int64_t max_dets, const int32_t* det_cls, const int64_t* det_row,
+                               const int32_t* gt_cls, const uint8_t* gt_things, int8_t* status, int32_t* ranks,
+                               void* groups, int64_t groups_bytes, void* stream_ptr) {
+  PE_REQUIRE(B >= 0 && B < INT_MAX && C > 0 && S >= 0 && total_dets >= 0 && max_dets >= 0,
+             "invalid grouping shapes");
+  PE_REQUIRE(max_dets <= kGroupMaxEntries && S <= kGroupMaxEntries && C < (1 << (32 - kGroupIndexBits)) - 1,
+             "grouping holds at most 4096 detections and target slots per image");
+  PE_REQUIRE(reinterpret_cast<uintptr_t>(status) % 8 == 0, "match flags must be 8-byte aligned");
+  PE_REQUIRE(groups_bytes >= pe_match_groups_bytes(B, C, total_dets, S), "group buffer too small");
+  const int det_slots = pow2_at_least(max_dets, 64), gt_slots = pow2_at_least(S, 64);
+  const size_t lds = 4 * static_cast<size_t>(det_slots + gt_slots) + 16 * static_cast<size_t>(C + 1);
+  PE_REQUIRE(lds <= 160 * 1024, "too many classes for the grouped matcher");
+  if (B == 0) return kOk;
+  const GroupBuffers w = carve_groups(groups, B, C, total_dets, S);
+  GroupArgs g{};
+  g.B = static_cast<int>(B);
+  g.C = static_cast<int>(C);
+  g.S = static_cast<int>(S);
+  g.det_off = reinterpret_cast<const i64*>(det_off);
+  g.det_cls = det_cls;
+  g.det_row = reinterpret_cast<const i64*>(det_row);
+  g.gt_cls = gt_cls;
+  g.gt_things = gt_things;
+  g.status = status;
+  g.ranks = ranks;
+  g.hdr = w.hdr;
+  g.det_list = w.det_list;
+  g.gt_list = w.gt_list;
+  g.det_slots = det_slots;
+  g.gt_slots = gt_slots;
+  hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+  if (lds > 64 * 1024)
+    PE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(match_groups),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+  match_groups<<<static_cast<unsigned>(B), 256, lds, stream>>>(g);
+  PE_LAUNCH_CHECK();
+  return kOk;
+}
+
+namespace {
+int make_match_grouped(int64_t B, int64_t C, int64_t S, const int64_t* det_off, int64_t total_dets,
+                       const void* groups, int64_t groups_bytes, const uint8_t* gt_crowd, const double* gt_ann,
+                       const int64_t* gt_area_i64, int64_t gt_area_stride, int64_t* pairs, int8_t* status,
+                       int64_t* counts, MatchArgs* out) {
+  PE_REQUIRE(B >= 0 && C > 0 && C < INT_MAX && B * C < INT_MAX && S >= 0 && S <= kGroupMaxEntries &&
+                 total_dets >= 0,
+             "invalid matching shapes");
+  PE_REQUIRE(groups_bytes >= pe_match_groups_bytes(B, C, total_dets, S), "group buffer too small");
+  PE_REQUIRE(S == 0 || gt_area_i64, "GT pixel areas are required");
+  PE_REQUIRE(match_wave_lds(static_cast<int>(S)) <= 150 * 1024, "too many target slots per image for the matcher");
+  const GroupBuffers w = carve_groups(const_cast<void*>(groups), B, C, total_dets, S);
+  MatchArgs a{};
+  a.B = static_cast<int>(B);
+  a.C = static_cast<int>(C);
+  a.S = static_cast<int>(S);
+  a.det_off = reinterpret_cast<const i64*>(det_off);
+  a.gt_crowd = gt_crowd;
+  a.gt_ann = gt_ann;
+  a.gt_area_i64 = reinterpret_cast<const i64*>(gt_area_i64);
+  a.gt_area_stride = gt_area_stride;
+  a.inter = pairs;
+  a.inter_bytes = 8;
+  a.status = status;
+  a.counts = reinterpret_cast<i64*>(counts);
+  a.max_gt = static_cast<int>(S);
+  a.words = static_cast<int>((S + 31) / 32);
+  a.zero_inter = 1;
+  a.group_hdr = w.hdr;
+  a.group_det = w.det_list;
+  a.group_gt = w.gt_list;
+  *out = a;
+  return kOk;
+}
+}  // namespace
+
+int launch_match_grouped(const MatchArgs& a, hipStream_t stream) {
+  if (a.B == 0) return kOk;
+  const size_t lds = match_wave_lds(a.S);
+  if (lds > 64 * 1024)
+    PE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(coco_match_grouped),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+  ProfileScope timing("coco_match", stream);
+  coco_match_grouped<<<static_cast<unsigned>(static_cast<i64>(a.B) * a.C), 64, lds, stream>>>(a);
+  PE_LAUNCH_CHECK();
+  return kOk;
+}
+
+extern "C" int pe_coco_match_grouped(int64_t B, int64_t C, int64_t S, const int64_t* det_off, int64_t total_dets,
+                                     const void* groups, int64_t groups_bytes, const uint8_t* gt_crowd,
+                                     const double* gt_ann, const int64_t* gt_area_i64, int64_t gt_area_stride,
+                                     int64_t* pairs, int8_t* status, int64_t* counts, void* stream_ptr) {
+  MatchArgs a;
+  const int rc = make_match_grouped(B, C, S, det_off, total_dets, groups, groups_bytes, gt_crowd, gt_ann,
+                                    gt_area_i64, gt_area_stride, pairs, status, counts, &a);
+  if (rc) return rc;
+  return launch_match_grouped(a, static_cast<hipStream_t>(stream_ptr));
+}
+
 // Images with at most kOrderLds detections: one workgroup per image sorts
 // (score key, index) pairs with a bitonic network in LDS and writes the
 // outputs directly (one launch instead of keys + segmented sort + gather).
@@ -2335,7 +2789,8 @@ static int launch_score_order_lds(bool arrays, int64_t B, int64_t max_dets, cons
                                   int32_t* sorted_classes32, hipStream_t stream) {
   int slots = 64;
   while (slots < max_dets) slots <<= 1;
-  const int threads = std::min(1024, slots);  // one element per lane and stage
+  // One element per lane and stage; narrower beside another stream's kernel.
+  const int threads = std::min(stream_concurrent(stream) ? kConcurrentThreads : 1024, slots);
   const size_t lds = 12 * static_cast<size_t>(slots);
   auto kernel = arrays ? score_order_lds<true> : score_order_lds<false>;
   if (lds > 64 * 1024)
@@ -2385,7 +2840,7 @@ extern "C" int64_t pe_ap_workspace_bytes(int64_t N, int64_t C) {
 static int ap_compute(int64_t N, const double* scores, const int32_t* classes, const int32_t* ranks,
                       const int8_t* status, const int64_t* counts, int64_t C, double* precision, double* recall,
                       double* summary, void* workspace, int64_t workspace_bytes, void* stream_ptr,
-                      const FinalArgs* fin, int64_t K) {
+                      const FinalArgs* fin, int64_t K, const MatchArgs* match = nullptr, int32_t* done = nullptr) {
   PE_REQUIRE(N >= 0 && N < INT_MAX && C > 0 && C <= INT_MAX / 240, "invalid AP shapes");
   PE_REQUIRE(workspace_bytes >= pe_ap_workspace_bytes(N, C), "workspace too small");
   PE_REQUIRE(reinterpret_cast<uintptr_t>(status) % 8 == 0, "match flags must be 8-byte aligned");
@@ -2413,7 +2868,29 @@ static int ap_compute(int64_t N, const double* scores, const int32_t* classes, c
     sa.sort_slots = 64;  // keeps the flag staging area 16-byte aligned
     while (sa.sort_slots < N) sa.sort_slots <<= 1;
     size_t lds = ap_small_lds(sa.sort_slots);
-    if (fin && K > 0) {
+    MatchPhase phase{};
+    if (match && fin && K > 0) {
+      // As many matching waves as fit beside nothing else in LDS (at most
+      // the workgroup's 16); the AP phase reuses the same LDS afterwards.
+      phase.m = *match;
+      phase.wave_lds = static_cast<int>(match_wave_lds(match->S));
+      phase.waves = static_cast<int>(std::min<size_t>(kApSmallThreads / kWave, (150 * 1024) / phase.wave_lds));
+    }
+    if (phase.waves > 0) {
+      lds = std::max({lds, 16 * static_cast<size_t>(fin->C), static_cast<size_t>(phase.waves) * phase.wave_lds});
+      PE_REQUIRE(lds <= 160 * 1024, "too many categories for the finaliser");
+      PE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(ap_small_match_final),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+      // With a completion counter the last class workgroup also folds the
+      // summaries.
+      sa.ap.done = done;
+      sa.ap.summary = summary;
+      ProfileScope timing("ap_accumulate", stream);
+      ap_small_match_final<<<static_cast<unsigned>(C + K), kApSmallThreads, lds, stream>>>(sa, *fin, phase);
+      PE_LAUNCH_CHECK();
+      if (done) return kOk;
+    } else if (fin && K > 0) {
+      if (match && (rc = launch_match_grouped(*match, stream))) return rc;
       lds = std::max(lds, 16 * static_cast<size_t>(fin->C));
       PE_REQUIRE(lds <= 160 * 1024, "too many categories for the finaliser");
       PE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(ap_small_final),
@@ -2422,6 +2899,7 @@ static int ap_compute(int64_t N, const double* scores, const int32_t* classes, c
       ap_small_final<<<static_cast<unsigned>(C + K), kApSmallThreads, lds, stream>>>(sa, *fin);
       PE_LAUNCH_CHECK();
     } else {
+      if (match && (rc = launch_match_grouped(*match, stream))) return rc;
       PE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(ap_small), hipFuncAttributeMaxDynamicSharedMemorySize,
                                  static_cast<int>(lds)));
       ProfileScope timing("ap_accumulate", stream);
@@ -2432,6 +2910,7 @@ static int ap_compute(int64_t N, const double* scores, const int32_t* classes, c
     PE_LAUNCH_CHECK();
     return kOk;
   }
+  if (match && (rc = launch_match_grouped(*match, stream))) return rc;
   if (fin && (rc = launch_finalize(*fin, K, stream))) return rc;
   if ((rc = launch_zero(w.start, 4 * static_cast<size_t>(C), stream))) return rc;
   if ((rc = launch_zero(w.end, 4 * static_cast<size_t>(C), stream))) return rc;
@@ -2479,6 +2958,27 @@ extern "C" int pe_compute_with_ap(const int64_t* tp, const int64_t* fp, const in
   if (rc) return rc;
   return ap_compute(N, scores, classes, ranks, status, ap_counts, ap_C, precision, recall, summary, workspace,
                     workspace_bytes, stream, &fin, K);
+}
+
+extern "C" int pe_compute_with_ap_match(
+    const int64_t* tp, const int64_t* fp, const int64_t* fn, const double* sums, const int64_t* confusion,
+    const uint8_t* things, int64_t K, int64_t C, double* per_class, double* summaries, int64_t* counts, double* miou,
+    int64_t* out_tp, int64_t* out_fp, int64_t* out_fn, double* out_sums, int64_t* out_confusion, int64_t N,
+    const double* scores, const int32_t* classes, const int32_t* ranks, int8_t* status, int64_t* ap_counts,
+    int64_t ap_C, double* precision, double* recall, double* summary, void* workspace, int64_t workspace_bytes,
+    int64_t B, int64_t S, const int64_t* det_off, const void* groups, int64_t groups_bytes,
+    const uint8_t* gt_crowd, const double* gt_ann, const int64_t* gt_area_i64, int64_t gt_area_stride,
+    int64_t* pairs, int32_t* done, void* stream) {
+  FinalArgs fin;
+  int rc = make_final_args(tp, fp, fn, sums, confusion, things, K, C, per_class, summaries, counts, miou, out_tp,
+                           out_fp, out_fn, out_sums, out_confusion, &fin);
+  if (rc) return rc;
+  MatchArgs match;
+  if ((rc = make_match_grouped(B, ap_C, S, det_off, N, groups, groups_bytes, gt_crowd, gt_ann, gt_area_i64,
+                               gt_area_stride, pairs, status, ap_counts, &match)))
+    return rc;
+  return ap_compute(N, scores, classes, ranks, status, ap_counts, ap_C, precision, recall, summary, workspace,
+                    workspace_bytes, stream, &fin, K, &match, done);
 }
 
 // ------------------------------------------------------- small helpers

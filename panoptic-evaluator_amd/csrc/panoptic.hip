@@ -229,7 +229,7 @@ __device__ __forceinline__ void histogram_chunk(const HistArgs& a, int b, i64 be
                                                 const ImageTables<STAGE>& t) {
   const GT* gt = static_cast<const GT*>(a.gt);
   const PR* pred = static_cast<const PR*>(a.pred);
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, nt = blockDim.x;  // 512, or 256 beside another stream's kernel
   u64* pairs = a.pairs + static_cast<i64>(b) * a.G * a.P;
   using Table = typename ImageTables<STAGE>::Table;
   SlotResolver<GT, LOOKUP, Table> gslot{t.gt, a.G, LOOKUP && a.gt_ids != nullptr};
@@ -253,16 +253,16 @@ __device__ __forceinline__ void histogram_chunk(const HistArgs& a, int b, i64 be
     const i64 va = (begin + 3) & ~static_cast<i64>(3);
     const i64 vb = end & ~static_cast<i64>(3);
     const i64 head_end = va < end ? va : end;
-    for (i64 i = begin + tid; i < head_end; i += kHistThreads) visit(read_pixel(gt, i), read_pixel(pred, i));
+    for (i64 i = begin + tid; i < head_end; i += nt) visit(read_pixel(gt, i), read_pixel(pred, i));
     if (va < vb) {
       const i64 q_end = vb >> 2;
       constexpr int U = 4;
-      for (i64 base = (va >> 2) + tid; base < q_end; base += static_cast<i64>(kHistThreads) * U) {
+      for (i64 base = (va >> 2) + tid; base < q_end; base += static_cast<i64>(nt) * U) {
         GV gv[U][4];
         PV pv[U][4];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          const i64 q = base + static_cast<i64>(u) * kHistThreads;
+          const i64 q = base + static_cast<i64>(u) * nt;
           if (q < q_end) {
             load_quad(gt, q, gv[u]);
             load_quad(pred, q, pv[u]);
@@ -270,20 +270,20 @@ __device__ __forceinline__ void histogram_chunk(const HistArgs& a, int b, i64 be
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          const i64 q = base + static_cast<i64>(u) * kHistThreads;
+          const i64 q = base + static_cast<i64>(u) * nt;
           if (q < q_end) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) visit(gv[u][j], pv[u][j]);
           }
         }
       }
-      for (i64 i = (vb > head_end ? vb : head_end) + tid; i < end; i += kHistThreads)
+      for (i64 i = (vb > head_end ? vb : head_end) + tid; i < end; i += nt)
         visit(read_pixel(gt, i), read_pixel(pred, i));
     } else {
-      for (i64 i = head_end + tid; i < end; i += kHistThreads) visit(read_pixel(gt, i), read_pixel(pred, i));
+      for (i64 i = head_end + tid; i < end; i += nt) visit(read_pixel(gt, i), read_pixel(pred, i));
     }
   } else {
-    for (i64 i = begin + tid; i < end; i += kHistThreads) visit(read_pixel(gt, i), read_pixel(pred, i));
+    for (i64 i = begin + tid; i < end; i += nt) visit(read_pixel(gt, i), read_pixel(pred, i));
   }
   cache.drain(bins, pairs);
   if (bad && a.error) atomicOr(a.error, 1);
@@ -296,7 +296,7 @@ __device__ __forceinline__ void histogram_chunk(const HistArgs& a, int b, i64 be
 template <typename GT, typename PR, bool LOOKUP, bool VEC, bool GLOBAL, bool STAGE>
 __device__ __forceinline__ void histogram_item(const HistArgs& a, int item, uint4* lds4) {
   uint32_t* bins = reinterpret_cast<uint32_t*>(lds4);
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, nt = blockDim.x;
   const int b = __builtin_amdgcn_readfirstlane(a.tiles.image_of(item));
   const int chunk = item - a.tiles.first_tile(b);
   const i64 image = a.tiles.begin(b);
@@ -304,7 +304,7 @@ __device__ __forceinline__ void histogram_item(const HistArgs& a, int item, uint
   const i64 begin = image + static_cast<i64>(chunk) * a.tiles.tile;
   const i64 end = image + min(static_cast<i64>(chunk + 1) * a.tiles.tile, pixels);
   if (!GLOBAL) {
-    for (int w = tid; w < a.words4; w += kHistThreads) lds4[w] = make_uint4(0, 0, 0, 0);
+    for (int w = tid; w < a.words4; w += nt) lds4[w] = make_uint4(0, 0, 0, 0);
   }
   ImageTables<STAGE> t{};
   const i64 gt_row = static_cast<i64>(b) * a.G, pred_row = static_cast<i64>(b) * a.P;
@@ -315,12 +315,12 @@ __device__ __forceinline__ void histogram_item(const HistArgs& a, int item, uint
     const int Tg = 1 << a.gt_hash_bits, Tp = 1 << a.pred_hash_bits;
     auto* keys = reinterpret_cast<i64*>(reinterpret_cast<char*>(lds4) + a.table_offset);
     auto* word = reinterpret_cast<int*>(keys + Tg + Tp);
-    for (int i = tid; i < Tg + Tp; i += kHistThreads) {
+    for (int i = tid; i < Tg + Tp; i += nt) {
       keys[i] = kEmptyId;
       word[i] = INT_MAX;
     }
     __syncthreads();
-    for (int i = tid; i < a.G + a.P; i += kHistThreads) {
+    for (int i = tid; i < a.G + a.P; i += nt) {
       const bool g = i < a.G;
       const i64* src = g ? a.gt_ids : a.pred_ids;
       const int index = g ? i : i - a.G;
@@ -340,7 +340,7 @@ __device__ __forceinline__ void histogram_item(const HistArgs& a, int item, uint
       }
     }
     __syncthreads();
-    for (int i = tid; i < Tg + Tp; i += kHistThreads) {
+    for (int i = tid; i < Tg + Tp; i += nt) {
       if (keys[i] == kEmptyId) continue;
       const bool g = i < Tg;
       const int S = g ? a.G : a.P;
@@ -359,7 +359,7 @@ __device__ __forceinline__ void histogram_item(const HistArgs& a, int item, uint
   if (!GLOBAL) {
     __syncthreads();
     u64* pairs = a.pairs + static_cast<i64>(b) * a.G * a.P;
-    for (int w = tid; w < a.words4; w += kHistThreads) {
+    for (int w = tid; w < a.words4; w += nt) {
       uint4 v = lds4[w];
       uint32_t words[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
@@ -412,7 +412,10 @@ static int launch_hist_kernel(const HistArgs& a, int blocks, size_t lds, hipStre
                        : joint_histogram<GT, PR, LOOKUP, VEC, GLOBAL, false, false>;
   if (!allow_lds(kernel, lds)) return fail(kUnsupportedShape, "LDS request of %zu bytes refused", lds);
   ProfileScope timing("joint_histogram", stream);
-  kernel<<<blocks, kHistThreads, lds, stream>>>(a);
+  // Beside another stream's kernel, a narrower workgroup fits where one of
+  // that kernel's workgroups retired.
+  const int threads = stream_concurrent(stream) ? kConcurrentThreads : kHistThreads;
+  kernel<<<blocks, threads, lds, stream>>>(a);
   PE_LAUNCH_CHECK();
   return kOk;
 }
@@ -1048,13 +1051,16 @@ int launch_score(ScoreArgs a, hipStream_t stream) {
   if (lds > kMaxStageLds) lds = 0;
   if (!allow_lds(score_images, lds)) lds = 0;
   a.lds_bytes = lds;
+  // Beside another stream's kernel, 1024-thread workgroups wait until a
+  // compute unit has 16 wave slots free; narrow ones fit beside it.
+  const int threads = stream_concurrent(stream) ? kConcurrentThreads : kScoreThreads;
   {
     ProfileScope timing("score_images", stream);
-    score_images<<<a.B, kScoreThreads, lds, stream>>>(a);
+    score_images<<<a.B, threads, lds, stream>>>(a);
     PE_LAUNCH_CHECK();
   }
   if (!a.reward) {
-    convert_iou<<<1, kScoreThreads, 0, stream>>>(a);
+    convert_iou<<<1, threads, 0, stream>>>(a);
     PE_LAUNCH_CHECK();
   }
   return kOk;

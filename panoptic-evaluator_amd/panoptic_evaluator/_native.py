@@ -41,12 +41,17 @@ _SIGNATURES = {
     "pe_compute_with_ap": (_int, [_ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _ptr, _ptr, _ptr, _ptr,
                                   _ptr, _ptr, _ptr, _ptr, _ptr,
                                   _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _ptr, _ptr, _ptr, _ptr, _i64, _ptr]),
+    "pe_compute_with_ap_match": (_int, [_ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _ptr, _ptr, _ptr, _ptr,
+                                        _ptr, _ptr, _ptr, _ptr, _ptr,
+                                        _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _ptr, _ptr, _ptr, _ptr, _i64,
+                                        _i64, _i64, _ptr, _ptr, _i64, _ptr, _ptr, _ptr, _i64, _ptr, _ptr, _ptr]),
     "pe_image_pq": (_int, [_ptr, _ptr, _ptr, _ptr, _i64, _i64, _ptr, _ptr, _ptr]),
     "pe_pack": (_int, [_ptr, _int, _i64, _i64, _ptr, _ptr, _i64, _ptr, _ptr, _ptr]),
     "pe_stream_read": (_int, [_ptr, _i64, _int, _ptr, _ptr]),
     "pe_event_create": (_int, [ctypes.POINTER(_ptr)]),
     "pe_event_destroy": (_int, [_ptr]),
     "pe_stream_wait": (_int, [_ptr, _ptr, _ptr]),
+    "pe_stream_set_concurrent": (_int, [_ptr, _int]),
 }
 
 _lock = threading.Lock()
@@ -164,7 +169,11 @@ class SideStream:
             event = _ptr()
             check(lib.pe_event_create(ctypes.byref(event)))
             self._events.append(event.value)
-        weakref.finalize(self, _destroy_events, list(self._events))
+        # Its latency-bound launches use narrow workgroups, which fit beside
+        # the main stream's kernel (torch reuses pooled stream handles, so the
+        # mark is dropped with this object).
+        check(lib.pe_stream_set_concurrent(self.handle, 1))
+        weakref.finalize(self, _destroy_events, list(self._events), self.handle)
 
     def fork(self, main: int):
         call("pe_stream_wait", self.handle, main, self._events[0])
@@ -173,10 +182,12 @@ class SideStream:
         call("pe_stream_wait", main, self.handle, self._events[1])
 
 
-def _destroy_events(events):
+def _destroy_events(events, handle=None):
     if _library is not None:
         for event in events:
             _library.pe_event_destroy(event)
+        if handle is not None:
+            _library.pe_stream_set_concurrent(handle, 0)
 
 
 def ptr(tensor):

@@ -20,7 +20,7 @@ per update.
 
 from __future__ import annotations
 
-from typing import Sequence
+from typing import NamedTuple, Sequence
 
 import torch
 
@@ -28,6 +28,19 @@ from . import _native
 from .evaluator import PanopticBatch, PanopticEvaluator, _prepare, final_launch_args, final_results
 from .instance import (InstanceBatch, InstanceEvaluator, PackedMasks, ap_launch_args, ap_result, as_bits, offsets,
                        raise_instance_error, score_order)
+
+
+# Images up to this many detections and target slots use the grouped matcher.
+GROUP_LIMIT = 4096
+
+
+class _PendingMatch(NamedTuple):
+    """The grouped matching of an update, deferred: pe_coco_match_grouped's
+    arguments (without the stream), the record whose flags it completes, and
+    the tensors those arguments point into."""
+    args: tuple
+    record: tuple
+    keep: tuple
 
 
 class SegmentationEvaluator:
@@ -42,14 +55,23 @@ class SegmentationEvaluator:
         self.validate = validate
         self._error = torch.zeros(1, dtype=torch.int32, device=self.device)
         self._side = None
+        # Set while a whole step (reset, update, compute) is captured into one
+        # graph: reset() then clears the accumulators on the side stream, and
+        # the update's grouped matching runs inside compute()'s launch instead
+        # of at the end of the update.
+        self._capturing_step = False
+        self._pending = None
         self._pairs = None
         self._pairs_clean = False
         # The panoptic accumulators and the AP positive counts share one
-        # buffer, so reset() is a single fill.
+        # buffer, so reset() is a single fill.  One more word: the completion
+        # counter of compute()'s launch (zero between launches; the launch
+        # leaves it zero, and reset() zeroes it too).
         words = PanopticEvaluator.state_words(num_classes)
-        shared = torch.zeros(words + 4 * num_classes, dtype=torch.int64, device=self.device)
+        shared = torch.zeros(words + 4 * num_classes + 1, dtype=torch.int64, device=self.device)
         self.panoptic._bind_state(shared[:words])
-        self.instance._counts = shared[words:].view(num_classes, 4)
+        self.instance._counts = shared[words:words + 4 * num_classes].view(num_classes, 4)
+        self._done = shared[words + 4 * num_classes:].view(torch.int32)[:1]
         self._shared = shared
 
     def _pair_table(self, N, S):
@@ -68,8 +90,25 @@ class SegmentationEvaluator:
             self._side = _native.SideStream(self.device)
         return self._side
 
+    def _flush(self):
+        """Run the deferred matching of the last update, if any."""
+        pending, self._pending = self._pending, None
+        if pending is not None:
+            _native.call("pe_coco_match_grouped", *pending.args, _native.stream(self.device))
+
     def reset(self):
-        self._shared.zero_()
+        # A deferred match still zeroes the rows it owns (the table stays clean).
+        self._flush()
+        if self._capturing_step:
+            # The side stream's panoptic branch writes the accumulators first;
+            # the detection histogram need not wait for the fill.  The update
+            # captured next joins the side stream back.
+            side = self._side_stream()
+            side.fork(_native.stream(self.device))
+            with torch.cuda.stream(side.stream):
+                self._shared.zero_()
+        else:
+            self._shared.zero_()
         self.instance._records = []
 
     def _validate_areas(self, areas, classes):
@@ -83,6 +122,7 @@ class SegmentationEvaluator:
                instance_areas: torch.Tensor | None = None):
         """Accumulate all metrics (combined.py:63-161); optional annotation areas
         have shape B x S in target slot order and only drive AP area ranges."""
+        self._flush()
         target = target.compact()
         maps, classes, crowd = _prepare(target, self.num_classes, False)
         if maps.device != self.device:
@@ -100,15 +140,29 @@ class SegmentationEvaluator:
         det_off = offsets(counts, device)
         fork = not self.validate and torch.cuda.is_current_stream_capturing()
         stream = _native.stream(device)
+        max_d = max(counts, default=0)
+        # Captured updates match in two parts (images of at most 4096
+        # detections and slots): the per-class grouping needs only the score
+        # order and the slot classes, so it runs beside the histogram.  Eager
+        # updates keep one stream, where the one-part matcher is shorter.
+        grouped = fork and max_d <= GROUP_LIMIT and S <= GROUP_LIMIT
+        status = torch.empty((N, 40), dtype=torch.int8, device=device)
+        ranks = torch.empty(N, dtype=torch.int32, device=device)
+        groups = None
+        if grouped:
+            size = _native.library().pe_match_groups_bytes(B, C, N, S)
+            groups = torch.empty(max(size, 256), dtype=torch.uint8, device=device)
+
+        def order_and_group():
+            order = score_order(prepared, device)
+            if grouped:
+                _native.call("pe_match_groups", B, C, S, det_off.data_ptr(), N, max_d, order.classes32.data_ptr(),
+                             order.order.data_ptr(), classes.data_ptr(), self.panoptic.isthing.data_ptr(),
+                             status.data_ptr(), ranks.data_ptr(), groups.data_ptr(), groups.numel(),
+                             _native.stream(device))
+            return order
+
         ranked = None
-        if fork:
-            # The panoptic branch and the score order need nothing the
-            # detection histogram makes: they run on a second stream beside it
-            # (the panoptic scratch is allocated once per shape and kept).
-            side = self._side_stream()
-            areas = self.panoptic._update(prediction, target, return_areas=True, side=side)
-            with torch.cuda.stream(side.stream):
-                ranked = score_order(prepared, device)
         bits = any(isinstance(image.masks, PackedMasks) for image in prepared)
         masks = [as_bits(image.masks) for image in prepared] if bits else [image.masks for image in prepared]
         pointers = _native.device_table([m.data_ptr() for m in masks], device)
@@ -128,13 +182,24 @@ class SegmentationEvaluator:
                   max(counts, default=0), N, pairs.data_ptr(), int(self._pairs_clean))
         ws = (_native.ptr(workspace), 0 if workspace is None else workspace.numel(), stream)
         self._pairs_clean = False
+        if fork:
+            # The panoptic branch, the score order and the grouping need
+            # nothing the detection histogram makes: they run on a second
+            # stream beside it (the panoptic scratch is allocated once per
+            # shape and kept).  They are launched first, so that their
+            # workgroups are placed before the histogram's fill the device,
+            # and in narrow workgroups, which fit where the histogram's retire.
+            side = self._side_stream()
+            areas = self.panoptic._update(prediction, target, return_areas=True, side=side)
+            with torch.cuda.stream(side.stream):
+                ranked = order_and_group()
         if bits:
             # Bit rows (bool images are packed first): 1/8 of the mask bytes.
             _native.call("pe_detection_histogram_bits", *common, *ws)
         else:
             _native.call("pe_detection_histogram", *common, aligned, *ws)
         if ranked is None:
-            ranked = score_order(prepared, device)
+            ranked = order_and_group()
         det_cls, scores, det_cls32 = ranked.classes, ranked.scores, ranked.classes32
         inst = None if instance_areas is None else instance_areas.double().contiguous()
         if self.validate:
@@ -155,20 +220,30 @@ class SegmentationEvaluator:
             side.join(stream)
         else:
             areas = self.panoptic._update(prediction, target, return_areas=True)
-        status = torch.empty((N, 40), dtype=torch.int8, device=device)
-        ranks = torch.empty(N, dtype=torch.int32, device=device)
-        gt_off = _native.device_table(range(0, (B + 1) * S, S), device)
-        inter_off = offsets([d * S for d in counts], device)
-        # Detection areas are the matcher's own row sums (det_area NULL), and
-        # it zeroes every row it consumed (zero_inter), so the table is clean
-        # for the next update.
-        _native.call("pe_coco_match", B, C, det_off.data_ptr(), gt_off.data_ptr(), inter_off.data_ptr(),
-                     det_cls32.data_ptr(), None, ranked.order.data_ptr(), classes.data_ptr(), None,
-                     _native.ptr(crowd), _native.ptr(inst), pairs.data_ptr(), 8, N, S, status.data_ptr(),
-                     ranks.data_ptr(), self.instance._counts.data_ptr(), None, self.panoptic.isthing.data_ptr(),
-                     areas.data_ptr(), areas.stride(0), 1, stream)
+        record = (scores, det_cls32, ranks, status)
+        if grouped and not self._capturing_step:
+            _native.call("pe_coco_match_grouped", B, C, S, det_off.data_ptr(), N, groups.data_ptr(), groups.numel(),
+                         _native.ptr(crowd), _native.ptr(inst), areas.data_ptr(), areas.stride(0), pairs.data_ptr(),
+                         status.data_ptr(), self.instance._counts.data_ptr(), stream)
+        elif grouped:
+            # The matching is deferred: compute() does it in its own launch
+            # (or the next update, reset or compute flushes it first).
+            # Detection areas are the matcher's own row sums, and it zeroes
+            # every row it consumed, so the table is clean for the next update.
+            args = (B, C, S, det_off.data_ptr(), N, groups.data_ptr(), groups.numel(), _native.ptr(crowd),
+                    _native.ptr(inst), areas.data_ptr(), areas.stride(0), pairs.data_ptr(), status.data_ptr(),
+                    self.instance._counts.data_ptr())
+            self._pending = _PendingMatch(args, record, (det_off, groups, crowd, inst, areas, pairs))
+        else:
+            gt_off = _native.device_table(range(0, (B + 1) * S, S), device)
+            inter_off = offsets([d * S for d in counts], device)
+            _native.call("pe_coco_match", B, C, det_off.data_ptr(), gt_off.data_ptr(), inter_off.data_ptr(),
+                         det_cls32.data_ptr(), None, ranked.order.data_ptr(), classes.data_ptr(), None,
+                         _native.ptr(crowd), _native.ptr(inst), pairs.data_ptr(), 8, N, S, status.data_ptr(),
+                         ranks.data_ptr(), self.instance._counts.data_ptr(), None, self.panoptic.isthing.data_ptr(),
+                         areas.data_ptr(), areas.stride(0), 1, stream)
         self._pairs_clean = True
-        self.instance._commit((scores, det_cls32, ranks, status))
+        self.instance._commit(record)
         return self
 
     def compute(self):
@@ -177,9 +252,22 @@ class SegmentationEvaluator:
         pan = self.panoptic
         final_outputs, (final_args, _inputs) = final_launch_args(
             pan.tp[None], pan.fp[None], pan.fn[None], pan.iou_sum[None], pan.confusion[None], pan.isthing)
-        ap_outputs, ap_args, _keep = ap_launch_args(self.instance._records, self.instance._counts,
-                                                    self.num_classes, self.device)
-        _native.call("pe_compute_with_ap", *final_args, *ap_args, _native.stream(self.device))
+        records = self.instance._records
+        pending = self._pending
+        if pending is not None and not (len(records) == 1 and records[0] is pending.record):
+            self._flush()
+            pending = None
+        ap_outputs, ap_args, _keep = ap_launch_args(records, self.instance._counts, self.num_classes, self.device)
+        if pending is not None:
+            # The last update's matching runs in the same launch: each AP
+            # class workgroup matches its class in every image first.
+            self._pending = None
+            B, _C, S, det_off, _N, groups, groups_bytes, crowd, inst, areas, stride, pairs, _status, _counts = \
+                pending.args
+            _native.call("pe_compute_with_ap_match", *final_args, *ap_args, B, S, det_off, groups, groups_bytes,
+                         crowd, inst, areas, stride, pairs, self._done.data_ptr(), _native.stream(self.device))
+        else:
+            _native.call("pe_compute_with_ap", *final_args, *ap_args, _native.stream(self.device))
         result = final_results(*final_outputs)[0]
         result["instance"] = ap_result(*ap_outputs)
         return result

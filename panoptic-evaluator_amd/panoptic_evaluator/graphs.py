@@ -1,4 +1,4 @@
-"""HIP-graph capture of an evaluator's update().
+"""HIP-graph capture of an evaluator's update(), or of a whole step.
 
 ``CapturedUpdate(evaluator, *update_args)`` records one ``evaluator.update``
 (PanopticEvaluator, InstanceEvaluator or SegmentationEvaluator, built with
@@ -12,6 +12,11 @@ Capture needs no host work inside the graph: the update path allocates its
 scratch on the first call, keeps small metadata tables in a device cache, and
 clears memory with fill kernels (no memset nodes).  One untimed eager update
 runs first to populate those; the evaluator's state is restored afterwards.
+
+``CapturedStep(evaluator, *update_args)`` records ``reset(); update(...);
+compute()`` as one graph, for loops that evaluate batches of one layout one
+at a time (reward computation, per-batch metrics): a replay runs the three
+with no launch gaps between them and returns compute()'s result.
 """
 
 from __future__ import annotations
@@ -56,6 +61,11 @@ class CapturedUpdate:
             raise ValueError("capture needs validate=False: validation reads its result on the host")
         self.evaluator = evaluator
         self._instance = instance
+        # A combined evaluator defers the matching of captured updates to its
+        # next compute() (or flushes it before its next update or reset).
+        self._flush = getattr(evaluator, "_flush", None)
+        if self._flush is not None:
+            self._flush()
         device = (panoptic or instance).device
         saved = self._save(panoptic, instance)
         # Warm-up on a side stream (default priority): scratch allocation and
@@ -69,6 +79,7 @@ class CapturedUpdate:
         self._restore(panoptic, instance, saved)
         self.graph = torch.cuda.CUDAGraph()
         self._record = None
+        self._pending = None
         if instance is not None:
             # The captured update's detection records live in graph memory;
             # every replay appends them, and they are copied out only if the
@@ -81,6 +92,11 @@ class CapturedUpdate:
             with _native.pinned_tables() as tables, torch.cuda.graph(self.graph):
                 evaluator.update(*args, **kwargs)
             self._tables = tables
+            # The deferred matching of the captured update (in graph memory)
+            # is registered again after every replay.
+            self._pending = getattr(evaluator, "_pending", None)
+            if self._pending is not None:
+                evaluator._pending = None
             # Scratch the evaluator allocated outside the capture (per-shape
             # buffers, kept between updates) is read by the graph's kernels by
             # address: keep it alive even if the evaluator later replaces it.
@@ -113,6 +129,10 @@ class CapturedUpdate:
 
     def replay(self):
         """Run the recorded update once more (on the current stream's device)."""
+        if self._flush is not None:
+            # The previous update's deferred matching reads the table this
+            # replay refills, and completes the flags of its record.
+            self._flush()
         if self._record is not None:
             records = self._instance._records
             for i, record in enumerate(records):
@@ -121,4 +141,66 @@ class CapturedUpdate:
         self.graph.replay()
         if self._record is not None:
             self._instance._records.append(self._record)
+        if self._pending is not None:
+            self.evaluator._pending = self._pending
         return self.evaluator
+
+
+class CapturedStep:
+    """``reset(); update(*args); compute()`` of ``evaluator`` recorded as one
+    HIP graph.  ``replay()`` re-runs the three on the same input buffers
+    (refill them in place between replays) and returns compute()'s result,
+    whose tensors live in the graph's memory: the next replay overwrites them,
+    so clone what must outlive it.  After a replay the evaluator holds that
+    update's state, as after the eager calls (a later eager update() adds to
+    it, compute() recomputes it).  The batch layout is fixed at capture."""
+
+    def __init__(self, evaluator, *args, **kwargs):
+        panoptic, instance = _parts(evaluator)
+        if any(part is not None and part.validate for part in (panoptic, instance)):
+            raise ValueError("capture needs validate=False: validation reads its result on the host")
+        self.evaluator = evaluator
+        self._instance = instance
+        self._flush = getattr(evaluator, "_flush", None)
+        device = (panoptic or instance).device
+        # Warm-up step on a side stream (default priority): scratch,
+        # workspaces and the device metadata cache are filled outside the
+        # capture.
+        side = torch.cuda.Stream(device)
+        side.wait_stream(torch.cuda.current_stream(device))
+        with torch.cuda.stream(side):
+            evaluator.reset()
+            evaluator.update(*args, **kwargs)
+            evaluator.compute()
+        torch.cuda.current_stream(device).wait_stream(side)
+        torch.cuda.synchronize(device)
+        self.graph = torch.cuda.CUDAGraph()
+        # A combined evaluator clears its accumulators beside the detection
+        # histogram and matches inside compute()'s launch when the three are
+        # in one graph.
+        stepwise = hasattr(evaluator, "_capturing_step") and not evaluator.validate
+        if stepwise:
+            evaluator._capturing_step = True
+        try:
+            with _native.pinned_tables() as tables, torch.cuda.graph(self.graph):
+                evaluator.reset()
+                evaluator.update(*args, **kwargs)
+                self._result = evaluator.compute()
+        finally:
+            if stepwise:
+                evaluator._capturing_step = False
+        self._tables = tables
+        self._scratch = _scratch_of(evaluator)
+        self._records = list(instance._records) if instance is not None else None
+        torch.cuda.synchronize(device)
+
+    def replay(self):
+        """Run reset, update and compute once more; returns the result."""
+        if self._flush is not None:
+            # An eager update's deferred matching still owns rows of the
+            # detection x slot table that the replay refills.
+            self._flush()
+        self.graph.replay()
+        if self._instance is not None:
+            self._instance._records = list(self._records)
+        return self._result

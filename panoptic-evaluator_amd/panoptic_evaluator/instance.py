@@ -32,6 +32,11 @@ for _name, _sig in {
                                       _int, _ptr, _i64, _ptr]),
     "pe_coco_match": (_int, [_i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _int,
                              _i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _int, _ptr]),
+    "pe_match_groups_bytes": (_i64, [_i64, _i64, _i64, _i64]),
+    "pe_match_groups": (_int, [_i64, _i64, _i64, _ptr, _i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64,
+                               _ptr]),
+    "pe_coco_match_grouped": (_int, [_i64, _i64, _i64, _ptr, _i64, _ptr, _i64, _ptr, _ptr, _ptr, _i64, _ptr, _ptr,
+                                     _ptr, _ptr]),
     "pe_score_order_workspace_bytes": (_i64, [_i64, _i64]),
     "pe_score_order": (_int, [_i64, _i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _ptr]),
     "pe_score_order_images": (_int, [_i64, _i64, _ptr, _ptr, _ptr, _int, _ptr, _ptr, _ptr, _ptr, _ptr]),

@@ -325,3 +325,102 @@ def test_captured_update_survives_table_cache_turnover(pe):
         compare_panoptic(got, want, exact_iou_sum=True)
         for key in ("precision", "recall"):
             assert_exact(got["instance"][key], want["instance"][key], key)
+
+
+# ---------------------------------------------------------------- whole step
+
+def _step_replays_match_eager(pe, make, args, compare, replays=3):
+    """CapturedStep: every replay (eager work and a synchronisation between
+    replays) returns exactly the eager reset/update/compute result; after a
+    replay the evaluator's own compute() and a further eager update agree
+    with the eager sequence too."""
+    from panoptic_evaluator.graphs import CapturedStep
+    eager = make()
+    eager.update(*args)
+    want = to_numpy(eager.compute())
+    ev = make()
+    step = CapturedStep(ev, *args)
+    for _ in range(replays):
+        scratch = torch.randn(1 << 20, device="cuda").sum()
+        torch.cuda.synchronize()
+        got = to_numpy(step.replay())
+        assert scratch.isfinite()
+        compare(got, want)
+    compare(to_numpy(ev.compute()), want)
+    ev.update(*args)
+    eager.update(*args)
+    compare(to_numpy(ev.compute()), to_numpy(eager.compute()))
+    # A replay starts from reset, whatever the evaluator held.
+    compare(to_numpy(step.replay()), want)
+
+
+def test_step_graph_panoptic(pe):
+    import workloads
+    pair = workloads.config0(B=3)
+    isthing = workloads.isthing_vector(133, 80)
+    args = (pe.PanopticBatch(pair.pred.cuda(), pair.pc.cuda()),
+            pe.PanopticBatch(pair.gt.cuda(), pair.gc.cuda(), pair.crowd.cuda()))
+    make = lambda: pe.PanopticEvaluator(133, isthing=isthing, validate=False)  # noqa: E731
+    _step_replays_match_eager(pe, make, args, lambda got, want: compare_panoptic(got, want, exact_iou_sum=True))
+
+
+def _compare_combined_exact(got, want):
+    compare_panoptic(got, want, exact_iou_sum=True)
+    for key in ("precision", "recall", "ap", "ar100"):
+        assert_exact(got["instance"][key], want["instance"][key], key)
+
+
+def test_step_graph_combined_uniform(pe):
+    import workloads
+    pair, masks, classes, scores = workloads.config1(device="cuda", seed=4, B=3, H=160, W=224, detections=40)
+    isthing = workloads.isthing_vector(133, 80)
+    args = (pe.PanopticBatch(pair.pred, pair.pc), pe.InstanceBatch(masks, classes, scores),
+            pe.PanopticBatch(pair.gt, pair.gc, pair.crowd))
+    make = lambda: pe.SegmentationEvaluator(133, isthing=isthing, validate=False)  # noqa: E731
+    _step_replays_match_eager(pe, make, args, _compare_combined_exact)
+
+
+def test_step_graph_combined_packed(pe):
+    batch = uneven(29, 5, detections=(0, 30))
+    isthing = [c < 80 for c in range(133)]
+    prediction, target = packed(pe, batch)
+    dets = pe.InstanceBatch([m.cuda() for m in batch.masks], [c.cuda() for c in batch.classes],
+                            [s.cuda() for s in batch.scores])
+    make = lambda: pe.SegmentationEvaluator(133, isthing=isthing, validate=False)  # noqa: E731
+    _step_replays_match_eager(pe, make, (prediction, dets, target), _compare_combined_exact)
+
+
+def test_step_graph_instance(pe):
+    import workloads
+    pair, masks, classes, scores = workloads.config1(device="cuda", seed=6, B=2, H=96, W=128, detections=25)
+    gts = [(pair.gt[b][None] == torch.arange(1, 9, device="cuda")[:, None, None]) for b in range(2)]
+    keep = [m.flatten(1).any(1) for m in gts]
+    gts = [m[k] for m, k in zip(gts, keep)]
+    gcl = [torch.arange(len(m), device="cuda") % 7 for m in gts]
+    args = (pe.InstanceBatch(masks, classes, scores), pe.InstanceBatch(gts, gcl))
+    make = lambda: pe.InstanceEvaluator(133, device="cuda", validate=False)  # noqa: E731
+
+    def compare(got, want):
+        for key in ("precision", "recall", "ap", "ar100"):
+            assert_exact(got[key], want[key], key)
+    _step_replays_match_eager(pe, make, args, compare)
+
+
+def test_step_graph_with_refilled_inputs(pe):
+    """Inputs refilled in place between replays: each replay evaluates the
+    buffers' current contents."""
+    from panoptic_evaluator.graphs import CapturedStep
+    import workloads
+    pair, masks, classes, scores = workloads.config1(device="cuda", seed=8, B=2, H=128, W=160, detections=30)
+    isthing = workloads.isthing_vector(133, 80)
+    args = (pe.PanopticBatch(pair.pred, pair.pc), pe.InstanceBatch(masks, classes, scores),
+            pe.PanopticBatch(pair.gt, pair.gc, pair.crowd))
+    ev = pe.SegmentationEvaluator(133, isthing=isthing, validate=False)
+    step = CapturedStep(ev, *args)
+    for round_ in range(3):
+        for s in scores:
+            s.copy_(torch.rand_like(s))
+        pair.pred.copy_(torch.roll(pair.pred, shifts=round_ + 1, dims=2))
+        eager = pe.SegmentationEvaluator(133, isthing=isthing, validate=False)
+        eager.update(*args)
+        _compare_combined_exact(to_numpy(step.replay()), to_numpy(eager.compute()))

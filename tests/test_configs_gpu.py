@@ -68,7 +68,7 @@ def oracle_combined(gts, preds, gcs, pcs, crowds, masks, classes, scores):
 
 def test_config1_all_images(pe):
     import workloads
-    from panoptic_evaluator.graphs import CapturedUpdate
+    from panoptic_evaluator.graphs import CapturedStep, CapturedUpdate
     pair, masks, classes, scores = workloads.config1(device="cuda")
     print(f"c1: {pair.gt.shape[0]} images {tuple(pair.gt.shape[1:])}, {masks[0].shape[0]} detections", flush=True)
     args = (pe.PanopticBatch(pair.pred, pair.pc), pe.InstanceBatch(masks, classes, scores),
@@ -81,12 +81,15 @@ def test_config1_all_images(pe):
     replayed.reset()
     captured.replay()
     got_graph = to_numpy(replayed.compute())
+    stepped = pe.SegmentationEvaluator(133, isthing=ISTHING, validate=False)
+    got_step = to_numpy(CapturedStep(stepped, *args).replay())
     host = lambda ts: [t.cpu().numpy() for t in ts]  # noqa: E731
     expected = oracle_combined(pair.gt.cpu().numpy(), pair.pred.cpu().numpy(), pair.gc.cpu().numpy(),
                                pair.pc.cpu().numpy(), pair.crowd.cpu().numpy(), host(masks), host(classes),
                                host(scores))
     compare_combined(got_eager, expected)
     compare_combined(got_graph, got_eager, exact=True)
+    compare_combined(got_step, got_eager, exact=True)
 
 
 # ------------------------------------------------------------------------ c2
@@ -102,7 +105,7 @@ def c2(pe):
 
 
 def test_config2_batched_equals_plain_loop(pe, c2):
-    from panoptic_evaluator.graphs import CapturedUpdate
+    from panoptic_evaluator.graphs import CapturedStep, CapturedUpdate
     target = pe.PanopticBatch.pack(c2.gt, c2.gc, c2.crowd)
     prediction = pe.PanopticBatch.pack(c2.pred, c2.pc)
     detections = pe.InstanceBatch(c2.masks, c2.classes, c2.scores)
@@ -111,6 +114,12 @@ def test_config2_batched_equals_plain_loop(pe, c2):
     batched.reset()
     captured.replay()
     got = to_numpy(batched.compute())
+    # The whole step as one graph (matching inside compute()'s launches; the
+    # batch has more AP records than one launch accumulates in LDS).
+    stepped = pe.SegmentationEvaluator(133, isthing=ISTHING, validate=False)
+    step = CapturedStep(stepped, prediction, detections, target)
+    for _ in range(2):
+        compare_combined(to_numpy(step.replay()), got, exact=True)
     plain = pe.SegmentationEvaluator(133, isthing=ISTHING, validate=True)
     for i in range(len(c2.sizes)):
         plain.update(pe.PanopticBatch(c2.pred[i][None], c2.pc[i][None]),
