@@ -311,9 +311,9 @@ __global__ __launch_bounds__(64 * DT / DPW) void mask_overlap_staged(OverlapArgs
   auto load = [&](int i, i64 k) -> Piece {
     if constexpr (BITS) {
       // Bits past HW are zero in every row, so a partial last word is whole.
-      return rvalid[i] && k < k_end
-                 ? __builtin_nontemporal_load(as_global(reinterpret_cast<const uint32_t*>(rowp[i])) + (k >> 5))
-                 : 0u;
+      // Ordinary (cached) loads: a block takes 32 bytes of a row, and the
+      // next three blocks read the rest of the same 128-byte line.
+      return rvalid[i] && k < k_end ? *(as_global(reinterpret_cast<const uint32_t*>(rowp[i])) + (k >> 5)) : 0u;
     } else {
       return load16(rowp[i], k, k_end, rvalid[i], al);
     }
