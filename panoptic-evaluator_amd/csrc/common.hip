@@ -338,7 +338,10 @@ const char* const kBitsVariants[] = {
     "bits expanded to int8 MFMA operands in LDS, 128 dets x 64 GT x 16384 px, 8 blocks ahead",
     "bits expanded to int8 MFMA operands in LDS, 128 dets x 64 GT x 4096 px, 2 detection tiles per wave",
     "popcounts of ANDed words, 128 dets x 64 GT x 16384 px",
-    "popcounts of ANDed words, 64 dets x 64 GT x 16384 px"};
+    "popcounts of ANDed words, 64 dets x 64 GT x 16384 px",
+    "detection operands expanded in registers, GT in LDS, 128 dets x 64 GT x 4096 px, 2 detection tiles per wave",
+    "detection operands expanded in registers, GT in LDS, 128 dets x 64 GT x 4096 px, 4 detection tiles per wave",
+    "detection operands expanded in registers, GT in LDS, 128 dets x 64 GT x 8192 px, 2 detection tiles per wave"};
 const char* const kDetBitsVariants[] = {
     "2 words x 8 masks", "2 words x 16 masks", "1 word x 16 masks", "4 words x 4 masks", "4 words x 8 masks",
     "1 word x 8 masks", "2 words x 8 masks, wave-summed bins"};
@@ -348,7 +351,7 @@ const TuneSpec kSpecs[kTuneKernels] = {
     {"mask_overlap_mfma", 9, 0, kOverlapVariants},
     {"packed_grid", 3, 0, kGridVariants},
     {"detection_group", 4, 0, kGroupVariants},
-    {"mask_overlap_bits", 7, 0, kBitsVariants},
+    {"mask_overlap_bits", 10, 0, kBitsVariants},
     {"detection_histogram_bits", 7, 0, kDetBitsVariants},
 };
 std::atomic<int> g_selected[kTuneKernels] = {{-1}, {-1}, {-1}, {-1}, {-1}, {-1}, {-1}};

@@ -613,6 +613,179 @@ __global__ __launch_bounds__(256) void mask_overlap_bits(OverlapArgs o) {
   }
 }
 
+// Bit-packed masks with the detection operands built in registers: each wave
+// loads its detection rows' words straight from HBM and expands the 16 bits
+// an MFMA lane needs into its 16 operand bytes, so LDS holds only the GT
+// rows, which every wave shares.  Per 256-pixel block the LDS traffic is the
+// GT rows' expanded bytes (written once, read by every wave) instead of all
+// rows' (c1ip: ~80 KB instead of 144 KB per block with DPW = 2).
+// Workgroup: DT / DPW waves cover 16 * DT detections x 16 * GT GT masks over
+// a Chunk-pixel range; wave w owns detection tiles w * DPW .. w * DPW + DPW - 1.
+template <int DT, int GT, int Chunk, int DPW>
+__global__ __launch_bounds__(64 * DT / DPW) void mask_overlap_bits_reg(OverlapArgs o) {
+  static_assert(DT % DPW == 0, "whole detection tiles per wave");
+  constexpr int kThreads = 64 * DT / DPW;
+  constexpr int KB = 256;                       // pixels per block: 4 MFMA steps, 8 words per row
+  constexpr int kStride = KB + 16;              // expanded GT row in LDS, padded
+  constexpr int kRows = 16 * GT;
+  constexpr int kPerRow = KB / 32;              // words of one row's block
+  constexpr int kSlots = kRows * kPerRow / kThreads;
+  static_assert(Chunk % KB == 0, "blocks are whole MFMA steps");
+  static_assert(kThreads % kPerRow == 0 && kRows * kPerRow % kThreads == 0, "every thread stages whole words");
+  __shared__ __align__(16) uint8_t tile[kRows * kStride];
+  OverlapItem it;
+  if (!overlap_item(o, 16 * DT, 16 * GT, Chunk, it)) return;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const int D = it.D, G = it.G;
+  const int d_base = it.dgroup * 16 * DT, g_base = it.ggroup * 16 * GT;
+  const i64 HW = it.HW;
+  const i64 row_words = mask_words(HW);
+  const i64 k_begin = static_cast<i64>(it.chunk) * Chunk;
+  const i64 k_end = min(k_begin + Chunk, HW);
+  // GT staging: slot i of this thread is word (tid + i * kThreads) % 8 of GT
+  // row (tid + i * kThreads) / 8.
+  const int col = tid % kPerRow;
+  const uint32_t* grow[kSlots];
+  bool gvalid[kSlots];
+  int glds[kSlots];
+  int garea[kSlots];
+#pragma unroll
+  for (int i = 0; i < kSlots; ++i) {
+    const int r = (tid + i * kThreads) / kPerRow;
+    gvalid[i] = g_base + r < G;
+    grow[i] = reinterpret_cast<const uint32_t*>(it.gm) + static_cast<i64>(gvalid[i] ? g_base + r : 0) * row_words;
+    glds[i] = r * kStride + col * 32;
+    garea[i] = 0;
+  }
+  // Detection rows of this wave: row lane % 16 of tile d; the 16 operand
+  // bytes of step s are bits 16 * (kgroup & 1) .. + 15 of word
+  // 2 * s + (kgroup >> 1) of the block.
+  const int row = lane & 15, kgroup = lane >> 4;
+  const int dtiles = min(DPW, (D - d_base - wave * DPW * 16 + 15) / 16);  // uniform per wave; <= 0: idle
+  const uint32_t* drow[DPW];
+  bool dvalid[DPW];
+  int darea[DPW];
+#pragma unroll
+  for (int d = 0; d < DPW; ++d) {
+    const int idx = d_base + (wave * DPW + d) * 16 + row;
+    dvalid[d] = idx < D;
+    drow[d] = reinterpret_cast<const uint32_t*>(it.pm) + static_cast<i64>(dvalid[d] ? idx : 0) * row_words;
+    darea[d] = 0;
+  }
+  // One block's words: GT words for the LDS stage, detection words (8 per
+  // row, two 16-byte loads; the second only inside the row) for registers.
+  auto load_gt = [&](int i, i64 k0) -> uint32_t {
+    const i64 k = k0 + 32 * col;
+    return gvalid[i] && k < k_end ? *(as_global(grow[i]) + (k >> 5)) : 0u;
+  };
+  auto load_det = [&](int d, i64 k0, v4i& lo, v4i& hi) {
+    const bool on = dvalid[d] && k0 < k_end;
+    const auto* p = reinterpret_cast<const __attribute__((address_space(1))) v4i*>(as_global(drow[d]) + (k0 >> 5));
+    lo = on ? *p : v4i{0, 0, 0, 0};
+    hi = on && k0 + 128 < HW ? *(p + 1) : v4i{0, 0, 0, 0};
+  };
+  int tiles = G - g_base;
+  tiles = tiles <= 0 ? 0 : min(GT, (tiles + 15) / 16);
+  const bool compute = dtiles > 0;
+  const uint8_t* b_row = tile + row * kStride + kgroup * 16;
+  v4i acc[DPW][GT];
+#pragma unroll
+  for (int d = 0; d < DPW; ++d)
+#pragma unroll
+    for (int t = 0; t < GT; ++t) acc[d][t] = v4i{0, 0, 0, 0};
+  uint32_t gbuf[kSlots];
+  v4i dlo[DPW], dhi[DPW];
+#pragma unroll
+  for (int i = 0; i < kSlots; ++i) gbuf[i] = load_gt(i, k_begin);
+#pragma unroll
+  for (int d = 0; d < DPW; ++d) load_det(d, k_begin, dlo[d], dhi[d]);
+  for (i64 k0 = k_begin; k0 < k_end; k0 += KB) {
+    __syncthreads();  // the previous block has been multiplied
+#pragma unroll
+    for (int i = 0; i < kSlots; ++i) {
+      v4i lo, hi;
+      expand_bits(gbuf[i], lo, hi);
+      *reinterpret_cast<v4i*>(tile + glds[i]) = lo;
+      *reinterpret_cast<v4i*>(tile + glds[i] + 16) = hi;
+      garea[i] += __builtin_popcount(gbuf[i]);
+    }
+    // This block's detection words move to registers of their own, so the
+    // next block's loads can be issued before the multiplications.
+    uint32_t words[DPW][4];
+#pragma unroll
+    for (int d = 0; d < DPW; ++d) {
+      const bool odd = kgroup >> 1;
+      words[d][0] = static_cast<uint32_t>(odd ? dlo[d].y : dlo[d].x);
+      words[d][1] = static_cast<uint32_t>(odd ? dlo[d].w : dlo[d].z);
+      words[d][2] = static_cast<uint32_t>(odd ? dhi[d].y : dhi[d].x);
+      words[d][3] = static_cast<uint32_t>(odd ? dhi[d].w : dhi[d].z);
+    }
+    __syncthreads();
+    if (k0 + KB < k_end) {
+#pragma unroll
+      for (int i = 0; i < kSlots; ++i) gbuf[i] = load_gt(i, k0 + KB);
+#pragma unroll
+      for (int d = 0; d < DPW; ++d) load_det(d, k0 + KB, dlo[d], dhi[d]);
+    }
+    if (compute) {
+#pragma unroll
+      for (int step = 0; step < KB / 64; ++step) {
+        v4i a[DPW];
+#pragma unroll
+        for (int d = 0; d < DPW; ++d) {
+          const uint32_t half = (words[d][step] >> (16 * (kgroup & 1))) & 0xFFFFu;
+          darea[d] += __builtin_popcount(half);
+          uint32_t q[4];
+#pragma unroll
+          for (int n = 0; n < 4; ++n) q[n] = (((half >> (4 * n)) & 0xFu) * 0x00204081u) & 0x01010101u;
+          a[d] = v4i{static_cast<int>(q[0]), static_cast<int>(q[1]), static_cast<int>(q[2]), static_cast<int>(q[3])};
+        }
+#pragma unroll
+        for (int t = 0; t < GT; ++t) {
+          if (t >= tiles) break;
+          const v4i b = *reinterpret_cast<const v4i*>(b_row + t * 16 * kStride + step * 64);
+#pragma unroll
+          for (int d = 0; d < DPW; ++d)
+            if (d < dtiles) acc[d][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[d], b, acc[d][t], 0, 0, 0);
+        }
+      }
+    }
+  }
+  if (compute) {
+#pragma unroll
+    for (int d = 0; d < DPW; ++d) {
+      if (d >= dtiles) break;
+#pragma unroll
+      for (int t = 0; t < GT; ++t) {
+        if (t >= tiles) break;
+        add_overlap_tile(acc[d][t], lane, d_base + (wave * DPW + d) * 16, g_base + t * 16, D, G, it.inter);
+      }
+    }
+  }
+  // Areas.  Detection rows: the four kgroup lanes of a row hold disjoint
+  // quarters of its bits (counted only in waves that compute).  GT rows: the
+  // eight neighbouring lanes of a slot share its row.
+  if (it.pa != nullptr && it.ggroup == 0 && compute) {
+#pragma unroll
+    for (int d = 0; d < DPW; ++d) {
+      int sum = darea[d];
+      sum += __shfl_xor(sum, 16, kWave);
+      sum += __shfl_xor(sum, 32, kWave);
+      const int idx = d_base + (wave * DPW + d) * 16 + row;
+      if (kgroup == 0 && d < dtiles && dvalid[d] && sum) atomicAdd(it.pa + idx, sum);
+    }
+  }
+  const bool count_b = it.ga != nullptr && it.dgroup == 0;
+#pragma unroll
+  for (int i = 0; i < kSlots; ++i) {
+    int sum = garea[i];
+#pragma unroll
+    for (int m = 1; m < kPerRow; m <<= 1) sum += __shfl_xor(sum, m, kWave);
+    const int r = (tid + i * kThreads) / kPerRow;
+    if (col == 0 && gvalid[i] && sum && count_b) atomicAdd(it.ga + g_base + r, sum);
+  }
+}
+
 struct BitsVariant {
   int dets, gts;
   i64 chunk_px;
@@ -628,6 +801,9 @@ const BitsVariant kOverlapBits[] = {
     {128, 64, 4096, 256, mask_overlap_staged<8, 4, 256, 4096, true, 1, 2>},
     {128, 64, 16384, 256, mask_overlap_bits<8, 4, 32, 512>},
     {64, 64, 16384, 256, mask_overlap_bits<4, 4, 32, 512>},
+    {128, 64, 4096, 256, mask_overlap_bits_reg<8, 4, 4096, 2>},
+    {128, 64, 4096, 128, mask_overlap_bits_reg<8, 4, 4096, 4>},
+    {128, 64, 8192, 256, mask_overlap_bits_reg<8, 4, 8192, 2>},
 };
 
 int launch_overlap_bits(OverlapArgs o, int64_t B, int64_t max_d, int64_t max_g, int64_t max_hw, int64_t total_d,
