@@ -537,6 +537,71 @@ class PackedCombinedConfig(CombinedConfig):
         self.config["update_bytes_per_image"] = 8 * H * W + 4 * mask_words(H * W) * dets // B
 
 
+class RleCombinedConfig(CombinedConfig):
+    """c1 with the detection masks arriving as COCO RLE strings (pycocotools'
+    compressed form), uploaded to the device once: every step decodes them
+    on the GPU into bit rows (rle_runs + rle_raster), then runs c1's
+    evaluation on the bit-packed masks.  Encoding happens once in setup."""
+
+    dominant = "rle_raster"
+    dtype = "COCO RLE strings -> bit-packed masks / int32 maps (int64 counts, f64 ratios)"
+
+    def setup(self, device):
+        import workloads
+        import panoptic_evaluator as pe
+        super().setup(device)
+        log(f"[bench] {self.name}: encoding the detection masks as COCO RLE (untimed)")
+        images = [workloads.rle_encode(m) for m in self.masks]
+        self.rle_host = images
+        self.rle = pe.RleBatch(images, device)
+        self.rle.decode(validate=True)
+        self.detections = pe.InstanceBatch(self.rle.masks, self.classes, self.scores)
+        self.tuned = ("detection_group", "detection_histogram_bits", "joint_histogram")
+        B, H, W = self.pair.gt.shape
+        dets = sum(m.shape[0] for m in self.masks)
+        # The decode's bytes: RLE strings read, bit rows written.
+        self.bytes_per_launch = self.rle.input_bytes + 4 * mask_words(H * W) * dets
+        self.config["mask_format"] = "COCO RLE (compressed strings), decoded every step"
+        self.config["rle_bytes"] = self.rle.input_bytes
+        self.config["rle_bytes_per_mask"] = round(self.rle.input_bytes / max(dets, 1), 1)
+
+    def prepare(self):
+        self.rle.decode()
+
+    def step(self):
+        self.evaluator.reset()
+        self.prepare()
+        self.evaluator.update(*self.update_args())
+        return self.evaluator.compute()
+
+    def cpu_sample(self, budget_s):
+        """The oracle's RLE decode (pycocotools' algorithm) and the oracle
+        SegmentationEvaluator, one image at a time."""
+        from oracle import combined as ocomb
+        from oracle import rle as orle
+        state = ocomb.SegmentationState(133, self.isthing, validate=False)
+        done, elapsed = 0, 0.0
+        while done < self.images and elapsed < budget_s:
+            b = done  # the device-to-host copy of the inputs is not timed
+            pair = [t[b:b + 1].cpu().numpy() for t in (self.pair.gt, self.pair.pred, self.pair.gc, self.pair.pc,
+                                                       self.pair.crowd)]
+            classes, scores = self.classes[b].cpu().numpy(), self.scores[b].cpu().numpy()
+            t0 = time.perf_counter()
+            import numpy as np
+            masks = np.stack([orle.decode(r) for r in self.rle_host[b]])
+            state.update(pair[1], pair[3], [dict(masks=masks, classes=classes, scores=scores)], pair[0], pair[2],
+                         pair[4])
+            elapsed += time.perf_counter() - t0
+            done += 1
+        t0 = time.perf_counter()
+        state.compute()
+        elapsed += time.perf_counter() - t0
+        return done / elapsed, (f"{done} of {self.images} images, oracle RLE decode + SegmentationEvaluator "
+                                f"update+compute")
+
+    torch_sample = None
+
+
 class PackedInstanceConfig(InstanceConfig):
     """c1i with bit-packed detection and GT masks: intersections are popcounts
     of ANDed words (mask_overlap_bits) instead of the int8 MFMA contraction.
@@ -577,6 +642,9 @@ def make_configs():
                                            "per image) with bit-packed detection masks"),
         "c1ip": PackedInstanceConfig("c1ip", "c1ip: c1i (InstanceEvaluator, 16 x 800x1088, 100 detections vs the "
                                              "target's thing segments) with bit-packed masks on both sides"),
+        "c1r": RleCombinedConfig("c1r", "c1r: c1 (SegmentationEvaluator, 16 x 800x1088, 100 ellipse detections "
+                                        "per image) with the detections as COCO RLE strings decoded on the GPU "
+                                        "every step"),
         "c0": PanopticConfig("c0", workloads.config0, 133, 80,
                              "c0: PanopticEvaluator, 8 x 512x512, 133 classes, 40 Voronoi seeds"),
         "c3": PanopticConfig("c3", workloads.config3, 19, 8,
@@ -762,7 +830,7 @@ def cpu_baselines(cfg, budget_s):
     and the single-threaded numpy oracle beside it; otherwise the numpy
     oracle alone."""
     import torch
-    if not hasattr(cfg, "torch_sample"):
+    if getattr(cfg, "torch_sample", None) is None:
         log(f"[bench] {cfg.name}: timing the CPU oracle sample")
         rate, sample = cfg.cpu_sample(budget_s)
         return {"value": round(rate, 4), "unit": "images/sec", "cores": 1, "kind": "port",
@@ -800,6 +868,7 @@ def measure_steps(cfg, args, *, cpu_budget=None):
     from panoptic_evaluator import _native
     cpu_budget = args.cpu_budget if cpu_budget is None else cpu_budget
     eager_batched = getattr(cfg, "eager_batched_step", cfg.step)
+    prepare = getattr(cfg, "prepare", None)
     for _ in range(args.warmup):
         cfg.step()
         eager_batched()
@@ -837,10 +906,12 @@ def measure_steps(cfg, args, *, cpu_budget=None):
         from panoptic_evaluator.graphs import CapturedStep, CapturedUpdate
         log(f"[bench] {cfg.name}: capturing the batched update, and the whole step, as HIP graphs")
         captured = CapturedUpdate(cfg.evaluator, *cfg.update_args())
-        stepped = CapturedStep(cfg.evaluator, *cfg.update_args())
+        stepped = CapturedStep(cfg.evaluator, *cfg.update_args(), prepare=prepare)
 
         def graph_step():
             cfg.evaluator.reset()
+            if prepare is not None:
+                prepare()
             captured.replay()
             return cfg.evaluator.compute()
 

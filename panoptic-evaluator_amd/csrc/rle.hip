@@ -23,16 +23,19 @@
 //    (shuffles), and the delta coding resolves as two inclusive scans, one
 //    over even and one over odd count indices (count i > 2 is the sum of
 //    the stored values of indices i, i - 2, ..., down to 1 or 2).  A third
-//    scan gives every run's end position.  The lane that ends a run also
-//    writes, for every column whose first pixel lies in that run, the run's
-//    index (a per-column entry point for the raster kernel).  Malformed
+//    scan gives every run's end position.  For every column whose first
+//    pixel lies in a run, the wave writes the run's index (a per-column entry
+//    point for the raster kernel); a run of 0s can span hundreds of columns,
+//    so the entries of a step are spread over the lanes by a prefix sum of
+//    the runs' column counts.  Malformed
 //    input (a character outside the alphabet, a count of more than seven
 //    groups, a negative count, a total other than H * W) marks the mask and
 //    sets an error bit; the raster kernel leaves such masks all-zero.
 //  * rle_raster: four waves per (mask, 256 columns), lane = column.  Each
 //    lane walks its column's runs down the image 32 rows at a time and forms
-//    the column's 32-bit word of the block; 32 ballots turn the 64 column
-//    words of a wave into 32 rows of 64 pixels, and lane t writes row t's
+//    the column's 32-bit word of the block; a block with no set pixel in the
+//    wave's 64 columns is skipped after one ballot, otherwise 32 ballots turn
+//    the 64 column words into 32 rows of 64 pixels and lane t writes row t's
 //    segment.  Only nonzero words are written into the zero-filled rows:
 //    with W a multiple of 32 every word has one writer (plain stores), else
 //    segments straddle words and are merged with atomicOr.
@@ -70,9 +73,19 @@ __device__ __forceinline__ i64 wave_inclusive_scan(i64 v, int lane) {
   return v;
 }
 
+// Orders one wave's LDS writes and its later reads by other lanes.
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 __global__ __launch_bounds__(256) void rle_runs(i64 N, const RleMask* table, const uint8_t* data, int32_t* ends,
                                                 int32_t* col_run, int32_t* nruns, int32_t* error) {
+  __shared__ int32_t staging[4][3][kWave];  // per wave: prefix offsets, first columns, run indices
   const int lane = threadIdx.x & (kWave - 1);
+  int32_t* off_lds = staging[threadIdx.x >> 6][0];
+  int32_t* x0_lds = staging[threadIdx.x >> 6][1];
+  int32_t* idx_lds = staging[threadIdx.x >> 6][2];
   const i64 n = static_cast<i64>(blockIdx.x) * 4 + (threadIdx.x >> 6);
   if (n >= N) return;  // uniform per wave
   const RleMask m = table[n];
@@ -87,76 +100,114 @@ __global__ __launch_bounds__(256) void rle_runs(i64 N, const RleMask* table, con
   int pend_k = 0;                      // how many groups it has
   i64 chain_even = 0, chain_odd = 0;   // running sums of the stored values (delta decoding)
   i64 position = 0;                    // end position of the last completed run
-  for (i64 base = 0; base < m.units; base += kWave) {
-    const i64 u = base + lane;
-    const bool in = u < m.units;
-    i64 value = 0;    // this lane's count (valid where `ends_here`)
-    bool ends_here;
-    u64 emask;
-    if (m.format == 0) {
-      const int v = in ? static_cast<int>(chars[u]) - 48 : 0;
-      if (in && (v < 0 || v > 63)) bad |= kRleErrBadChar;
-      const int group = v & 0x1F;
-      ends_here = in && !(v & 0x20);
-      emask = __ballot(ends_here);
-      // The lanes of this step that belong to this lane's count: from the
-      // lane after the previous ending lane (or lane 0, continuing the count
-      // left open by the previous step).
-      const u64 before = emask & ((1ULL << lane) - 1);
-      const int start = before ? 64 - __clzll(before) : 0;
-      const int k = lane - start + (start == 0 ? pend_k : 0);
-      if (in && k > 6) bad |= kRleErrLongCount;
-      const i64 term = in ? static_cast<i64>(static_cast<u64>(group) << (5 * min(k, 11))) : 0;
-      i64 sum = term;
+  // Four steps' characters (or counts) are loaded together, indices clamped
+  // into the mask's units, before the first is used: the steps are serial,
+  // the loads need not be.
+  constexpr int kSteps = 4;
+  for (i64 block = 0; block < m.units; block += kSteps * kWave) {
+    int unit[kSteps];
 #pragma unroll
-      for (int d = 1; d <= 6; ++d) {
-        const i64 t = __shfl_up(term, d, kWave);
-        if (lane - d >= start) sum += t;
+    for (int s = 0; s < kSteps; ++s) {
+      const i64 u = min<i64>(block + s * kWave + lane, m.units - 1);
+      unit[s] = m.format == 0 ? static_cast<int>(chars[u]) : given[u];
+    }
+#pragma unroll
+    for (int s = 0; s < kSteps; ++s) {
+      const i64 base = block + s * kWave;
+      if (base >= m.units) break;  // uniform
+      const i64 u = base + lane;
+      const bool in = u < m.units;
+      i64 value = 0;    // this lane's count (valid where `ends_here`)
+      bool ends_here;
+      u64 emask;
+      if (m.format == 0) {
+        const int v = in ? unit[s] - 48 : 0;
+        if (in && (v < 0 || v > 63)) bad |= kRleErrBadChar;
+        const int group = v & 0x1F;
+        ends_here = in && !(v & 0x20);
+        emask = __ballot(ends_here);
+        // The lanes of this step that belong to this lane's count: from the
+        // lane after the previous ending lane (or lane 0, continuing the count
+        // left open by the previous step).
+        const u64 before = emask & ((1ULL << lane) - 1);
+        const int start = before ? 64 - __clzll(before) : 0;
+        const int k = lane - start + (start == 0 ? pend_k : 0);
+        if (in && k > 6) bad |= kRleErrLongCount;
+        const i64 term = in ? static_cast<i64>(static_cast<u64>(group) << (5 * min(k, 11))) : 0;
+        i64 sum = term;
+#pragma unroll
+        for (int d = 1; d <= 6; ++d) {
+          const i64 t = __shfl_up(term, d, kWave);
+          if (lane - d >= start) sum += t;
+        }
+        if (start == 0) sum += pend_val;
+        value = sum;
+        if (ends_here && (group & 0x10)) value |= static_cast<i64>(-1) << (5 * min(k + 1, 62));
+        // The count left open at the end of this step (if the last character
+        // carries on): its groups so far.
+        const int last = static_cast<int>(min<i64>(kWave, m.units - base)) - 1;
+        const bool open = !((emask >> last) & 1ULL);
+        const i64 open_sum = __shfl(sum, last, kWave);
+        const int open_k = __shfl(k, last, kWave) + 1;
+        pend_val = open ? open_sum : 0;
+        pend_k = open ? open_k : 0;
+      } else {
+        ends_here = in;
+        emask = __ballot(in);
+        value = in ? static_cast<i64>(unit[s]) : 0;
       }
-      if (start == 0) sum += pend_val;
-      value = sum;
-      if (ends_here && (group & 0x10)) value |= static_cast<i64>(-1) << (5 * min(k + 1, 62));
-      // The count left open at the end of this step (if the last character
-      // carries on): its groups so far.
-      const int last = static_cast<int>(min<i64>(kWave, m.units - base)) - 1;
-      const bool open = !((emask >> last) & 1ULL);
-      const i64 open_sum = __shfl(sum, last, kWave);
-      const int open_k = __shfl(k, last, kWave) + 1;
-      pend_val = open ? open_sum : 0;
-      pend_k = open ? open_k : 0;
-    } else {
-      ends_here = in;
-      emask = __ballot(in);
-      value = in ? static_cast<i64>(given[u]) : 0;
-    }
-    // Count index of each ending lane, and the delta decoding of the string
-    // form: count i > 2 adds count i - 2, i.e. the stored values of i, i - 2,
-    // ... down to index 1 (odd) or 2 (even).
-    const i64 idx = count_base + __popcll(emask & ((1ULL << lane) - 1));
-    i64 count = value;
-    if (m.format == 0) {
-      const bool chained = ends_here && idx >= 1;
-      const i64 even = wave_inclusive_scan(chained && !(idx & 1) ? value : 0, lane);
-      const i64 odd = wave_inclusive_scan(chained && (idx & 1) ? value : 0, lane);
-      if (chained) count = (idx & 1) ? chain_odd + odd : chain_even + even;
-      chain_even += __shfl(even, kWave - 1, kWave);
-      chain_odd += __shfl(odd, kWave - 1, kWave);
-    }
-    if (ends_here && count < 0) bad |= kRleErrNegative;
-    const i64 clamped = ends_here && count > 0 ? count : 0;
-    const i64 end = position + wave_inclusive_scan(clamped, lane);
-    position = __shfl(end, kWave - 1, kWave);
-    if (ends_here) {
-      const i64 lo = end - clamped;
-      if (end > total) bad |= kRleErrTotal;
-      if (idx < m.units) run_end[idx] = static_cast<int32_t>(min<i64>(end, INT_MAX));
-      // Columns whose first pixel lies in this run.
-      if (clamped > 0 && end <= total) {
-        const i64 x0 = (lo + H - 1) / H, x1 = (end - 1) / H;
-        for (i64 x = x0; x <= x1; ++x) cols[x] = static_cast<int32_t>(idx);
+      // Count index of each ending lane, and the delta decoding of the string
+      // form: count i > 2 adds count i - 2, i.e. the stored values of i, i - 2,
+      // ... down to index 1 (odd) or 2 (even).
+      const i64 idx = count_base + __popcll(emask & ((1ULL << lane) - 1));
+      i64 count = value;
+      if (m.format == 0) {
+        const bool chained = ends_here && idx >= 1;
+        const i64 even = wave_inclusive_scan(chained && !(idx & 1) ? value : 0, lane);
+        const i64 odd = wave_inclusive_scan(chained && (idx & 1) ? value : 0, lane);
+        if (chained) count = (idx & 1) ? chain_odd + odd : chain_even + even;
+        chain_even += __shfl(even, kWave - 1, kWave);
+        chain_odd += __shfl(odd, kWave - 1, kWave);
       }
+      if (ends_here && count < 0) bad |= kRleErrNegative;
+      const i64 clamped = ends_here && count > 0 ? count : 0;
+      const i64 end = position + wave_inclusive_scan(clamped, lane);
+      position = __shfl(end, kWave - 1, kWave);
+      // Columns whose first pixel lies in this lane's run: [x0, x0 + ncols).
+      int ncols = 0;
+      i64 x0 = 0;
+      if (ends_here) {
+        const i64 lo = end - clamped;
+        if (end > total) bad |= kRleErrTotal;
+        if (idx < m.units) run_end[idx] = static_cast<int32_t>(min<i64>(end, INT_MAX));
+        if (clamped > 0 && end <= total) {
+          x0 = (lo + H - 1) / H;
+          ncols = static_cast<int>(max<i64>(0, (end - 1) / H - x0 + 1));
+        }
+      }
+      // The step's column entries are written by the whole wave (a run of 0s
+      // can span hundreds of columns): entry q belongs to the last lane whose
+      // exclusive prefix of column counts is <= q.
+      const int incl = static_cast<int>(wave_inclusive_scan(ncols, lane));
+      const int entries = __shfl(incl, kWave - 1, kWave);
+      if (entries) {
+        wave_sync_lds();
+        off_lds[lane] = incl - ncols;
+        x0_lds[lane] = static_cast<int32_t>(x0);
+        idx_lds[lane] = static_cast<int32_t>(idx);
+        wave_sync_lds();
+        for (int q = lane; q < entries; q += kWave) {
+          int lo = 0, hi = kWave - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (off_lds[mid] <= q) lo = mid;
+            else hi = mid - 1;
+          }
+          cols[x0_lds[lo] + (q - off_lds[lo])] = idx_lds[lo];
+        }
+      }
+      count_base += __popcll(emask);
     }
-    count_base += __popcll(emask);
   }
   if (pend_k) bad |= kRleErrLongCount;
   if (position != total) bad |= kRleErrTotal;
@@ -189,10 +240,12 @@ __global__ __launch_bounds__(256) void rle_raster(i64 N, int groups, const RleMa
   const bool column = x < W;
   const int32_t* run_end = ends + m.run_off;
   uint32_t* out = bits + m.out_off;
-  // The run holding the column's current row, with its [start, end).
+  // The run holding the column's current row, with its [start, end), and the
+  // next run's end loaded one advance ahead of its use.
   int r = column ? col_run[m.col_off + x] : 0;
   i64 r_end = column ? run_end[r] : 0;
   i64 r_start = column && r > 0 ? run_end[r - 1] : 0;
+  i64 r_next = column && r + 1 < runs ? run_end[r + 1] : 0;
   const bool words_aligned = (W & 31) == 0;
   for (i64 y0 = 0; y0 < H; y0 += 32) {
     const int rows = static_cast<int>(min<i64>(32, H - y0));
@@ -212,9 +265,12 @@ __global__ __launch_bounds__(256) void rle_raster(i64 N, int groups, const RleMa
         if (r_end >= hi || r + 1 >= runs) break;
         ++r;
         r_start = r_end;
-        r_end = run_end[r];
+        r_end = r_next;
+        r_next = r + 1 < runs ? run_end[r + 1] : 0;
       }
     }
+    // Most blocks of a band hold no pixel of the mask: one ballot skips them.
+    if (__ballot(word != 0) == 0) continue;
     // Row t of the block: bit j = column x0 + j.
     u64 segment = 0;
 #pragma unroll
@@ -270,11 +326,15 @@ extern "C" int pe_rle_decode(int64_t N, const int64_t* table, const uint8_t* dat
   int32_t* col_run = carve.take<int32_t>(columns);
   int32_t* nruns = carve.take<int32_t>(N);
   const auto* rows = reinterpret_cast<const RleMask*>(table);
-  rle_runs<<<static_cast<unsigned>((N + 3) / 4), 256, 0, stream>>>(N, rows, data, ends, col_run, nruns, error);
-  PE_LAUNCH_CHECK();
+  {
+    ProfileScope timing("rle_runs", stream);
+    rle_runs<<<static_cast<unsigned>((N + 3) / 4), 256, 0, stream>>>(N, rows, data, ends, col_run, nruns, error);
+    PE_LAUNCH_CHECK();
+  }
   const int groups = static_cast<int>((max_width + 4 * kWave - 1) / (4 * kWave));
   if (groups == 0) return kOk;
   PE_REQUIRE(N * groups < INT_MAX, "too many RLE masks for one launch");
+  ProfileScope timing("rle_raster", stream);
   rle_raster<<<static_cast<unsigned>(N * groups), 256, 0, stream>>>(N, groups, rows, ends, col_run, nruns, bits);
   PE_LAUNCH_CHECK();
   return kOk;

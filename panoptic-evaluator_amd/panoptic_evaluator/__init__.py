@@ -9,7 +9,7 @@ from .evaluator import PanopticBatch, PanopticEvaluator, panoptic_quality
 from .instance import InstanceBatch, InstanceEvaluator, PackedMasks, pack_masks
 from .combined import SegmentationEvaluator
 from .prefix import segmentation_prefixes
-from .rle import decode_rle, decode_rle_batch
+from .rle import RleBatch, decode_rle, decode_rle_batch
 
 __all__ = [
     "PanopticBatch",
@@ -21,6 +21,7 @@ __all__ = [
     "segmentation_prefixes",
     "PackedMasks",
     "pack_masks",
+    "RleBatch",
     "decode_rle",
     "decode_rle_batch",
 ]

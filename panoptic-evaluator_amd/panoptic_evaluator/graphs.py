@@ -147,21 +147,25 @@ class CapturedUpdate:
 
 
 class CapturedStep:
-    """``reset(); update(*args); compute()`` of ``evaluator`` recorded as one
-    HIP graph.  ``replay()`` re-runs the three on the same input buffers
+    """``reset(); prepare(); update(*args); compute()`` of ``evaluator``
+    recorded as one HIP graph (``prepare``, optional: device work that makes
+    the update's inputs, such as decoding RLE masks).  ``replay()`` re-runs the three on the same input buffers
     (refill them in place between replays) and returns compute()'s result,
     whose tensors live in the graph's memory: the next replay overwrites them,
     so clone what must outlive it.  After a replay the evaluator holds that
     update's state, as after the eager calls (a later eager update() adds to
     it, compute() recomputes it).  The batch layout is fixed at capture."""
 
-    def __init__(self, evaluator, *args, **kwargs):
+    def __init__(self, evaluator, *args, prepare=None, **kwargs):
         panoptic, instance = _parts(evaluator)
         if any(part is not None and part.validate for part in (panoptic, instance)):
             raise ValueError("capture needs validate=False: validation reads its result on the host")
         self.evaluator = evaluator
         self._instance = instance
         self._flush = getattr(evaluator, "_flush", None)
+        # prepare(): device work that makes the update's inputs (such as
+        # RleBatch.decode), recorded between reset and update.
+        prepare = prepare or (lambda: None)
         device = (panoptic or instance).device
         # Warm-up step on a side stream (default priority): scratch,
         # workspaces and the device metadata cache are filled outside the
@@ -170,6 +174,7 @@ class CapturedStep:
         side.wait_stream(torch.cuda.current_stream(device))
         with torch.cuda.stream(side):
             evaluator.reset()
+            prepare()
             evaluator.update(*args, **kwargs)
             evaluator.compute()
         torch.cuda.current_stream(device).wait_stream(side)
@@ -184,6 +189,7 @@ class CapturedStep:
         try:
             with _native.pinned_tables() as tables, torch.cuda.graph(self.graph):
                 evaluator.reset()
+                prepare()
                 evaluator.update(*args, **kwargs)
                 self._result = evaluator.compute()
         finally:

@@ -58,7 +58,89 @@ def _payload(rle) -> tuple[int, bytes, int]:
     return 1, values.astype("<i4").tobytes(), int(values.size)
 
 
-@torch.no_grad()
+class RleBatch:
+    """The COCO RLE masks of a batch of images (one list per image, all masks
+    of an image of one size), uploaded to the device once, as they are, and
+    decoded on demand into bit-packed masks.  The decoded rows live in one
+    buffer owned by the batch (``masks`` are PackedMasks views of it), so
+    every decode writes to the same addresses: a HIP graph may capture
+    ``decode()``.  ``sizes`` (one (h, w) per image) is needed only for images
+    without masks."""
+
+    def __init__(self, images: Sequence[Sequence[dict]], device="cuda", *,
+                 sizes: Sequence[tuple[int, int]] | None = None):
+        device = _native.require_gpu(device)
+        if sizes is not None and len(sizes) != len(images):
+            raise ValueError("sizes must give one (h, w) per image")
+        rows, chunks = [], []
+        data_bytes = units_total = columns = out_words = 0
+        max_width = 0
+        shapes = []
+        for i, masks in enumerate(images):
+            shape = None if sizes is None else (int(sizes[i][0]), int(sizes[i][1]))
+            for rle in masks:
+                h, w = _size(rle)
+                if shape is not None and (h, w) != shape:
+                    raise ValueError("all RLE masks of one image must share its size")
+                shape = (h, w)
+            if shape is None:
+                raise ValueError("an image without masks needs its size")
+            h, w = shape
+            if h <= 0 or w <= 0:
+                raise ValueError("RLE masks need positive image dimensions")
+            words = mask_words(h * w)
+            shapes.append((len(masks), h, w, out_words))
+            for rle in masks:
+                form, payload, units = _payload(rle)
+                data_bytes = (data_bytes + 3) // 4 * 4  # int32 counts stay aligned
+                rows.append((data_bytes, units, out_words, columns, h, w, form, units_total))
+                chunks.append((data_bytes, payload))
+                data_bytes += len(payload)
+                units_total += units
+                columns += w
+                out_words += words
+                max_width = max(max_width, w)
+        self.device = device
+        self.count = len(rows)
+        self.units, self.columns, self.max_width = units_total, columns, max_width
+        self.input_bytes = data_bytes
+        self.bits = torch.zeros(max(out_words, 4), dtype=torch.int32, device=device)
+        self.error = torch.zeros(1, dtype=torch.int32, device=device)
+        self.table_bytes = 64 * self.count
+        # One host buffer, one copy: the table (N x 8 int64), then the data.
+        host = np.zeros(self.table_bytes + max(data_bytes, 1), dtype=np.uint8)
+        if rows:
+            host[:self.table_bytes] = np.frombuffer(np.asarray(rows, dtype="<i8").tobytes(), dtype=np.uint8)
+        for offset, payload in chunks:
+            start = self.table_bytes + offset
+            host[start:start + len(payload)] = np.frombuffer(payload, dtype=np.uint8)
+        self.upload = torch.from_numpy(host).to(device)
+        size = _native.library().pe_rle_workspace_bytes(self.count, units_total, columns)
+        self.workspace = torch.empty(max(size, 256), dtype=torch.uint8, device=device)
+        self.masks = [PackedMasks(self.bits[start:start + n * mask_words(h * w)].view(n, mask_words(h * w)), h, w)
+                      for n, h, w, start in shapes]
+
+    @torch.no_grad()
+    def decode(self, *, validate: bool = False) -> list[PackedMasks]:
+        """Decode every mask into ``masks`` (three launches: a fill of the bit
+        rows, then pe_rle_decode's two kernels).  ``validate`` reads back the
+        decoder's error word (a host synchronisation) and raises ValueError for
+        a malformed RLE; without it a malformed mask decodes as all-zero."""
+        self.bits.zero_()
+        if self.count:
+            self.error.zero_()
+            _native.call("pe_rle_decode", self.count, self.upload.data_ptr(),
+                         self.upload.data_ptr() + self.table_bytes, self.units, self.columns, self.max_width,
+                         self.bits.data_ptr(), self.error.data_ptr(), self.workspace.data_ptr(),
+                         self.workspace.numel(), _native.stream(self.device))
+            if validate:
+                code = int(self.error.item())
+                if code:
+                    reasons = ", ".join(text for bit, text in _ERRORS if code & bit)
+                    raise ValueError(f"malformed RLE mask: {reasons}")
+        return self.masks
+
+
 def decode_rle_batch(images: Sequence[Sequence[dict]], device="cuda", *,
                      sizes: Sequence[tuple[int, int]] | None = None, validate: bool = True) -> list[PackedMasks]:
     """Decode the RLE masks of several images (one list per image, all masks
@@ -68,63 +150,7 @@ def decode_rle_batch(images: Sequence[Sequence[dict]], device="cuda", *,
 
     ``validate`` reads back the decoder's error word and raises ValueError
     for a malformed RLE; without it a malformed mask decodes as all-zero."""
-    device = _native.require_gpu(device)
-    if sizes is not None and len(sizes) != len(images):
-        raise ValueError("sizes must give one (h, w) per image")
-    rows, chunks = [], []
-    data_bytes = units_total = columns = out_words = 0
-    max_width = 0
-    shapes = []
-    for i, masks in enumerate(images):
-        shape = None if sizes is None else (int(sizes[i][0]), int(sizes[i][1]))
-        for rle in masks:
-            h, w = _size(rle)
-            if shape is not None and (h, w) != shape:
-                raise ValueError("all RLE masks of one image must share its size")
-            shape = (h, w)
-        if shape is None:
-            raise ValueError("an image without masks needs its size")
-        h, w = shape
-        if h <= 0 or w <= 0:
-            raise ValueError("RLE masks need positive image dimensions")
-        words = mask_words(h * w)
-        shapes.append((len(masks), h, w, out_words))
-        for rle in masks:
-            form, payload, units = _payload(rle)
-            data_bytes = (data_bytes + 3) // 4 * 4  # int32 counts stay aligned
-            rows.append((data_bytes, units, out_words, columns, h, w, form, units_total))
-            chunks.append((data_bytes, payload))
-            data_bytes += len(payload)
-            units_total += units
-            columns += w
-            out_words += words
-            max_width = max(max_width, w)
-    N = len(rows)
-    bits = torch.zeros(max(out_words, 4), dtype=torch.int32, device=device)
-    if N:
-        # One host buffer, one copy: the table (N x 8 int64), then the data.
-        table_bytes = 64 * N
-        host = np.zeros(table_bytes + max(data_bytes, 1), dtype=np.uint8)
-        host[:table_bytes] = np.frombuffer(np.asarray(rows, dtype="<i8").tobytes(), dtype=np.uint8)
-        for offset, payload in chunks:
-            host[table_bytes + offset:table_bytes + offset + len(payload)] = np.frombuffer(payload, dtype=np.uint8)
-        upload = torch.from_numpy(host).to(device)
-        size = _native.library().pe_rle_workspace_bytes(N, units_total, columns)
-        workspace = torch.empty(max(size, 256), dtype=torch.uint8, device=device)
-        error = torch.zeros(1, dtype=torch.int32, device=device)
-        _native.call("pe_rle_decode", N, upload.data_ptr(), upload.data_ptr() + table_bytes, units_total, columns,
-                     max_width, bits.data_ptr(), error.data_ptr(), workspace.data_ptr(), workspace.numel(),
-                     _native.stream(device))
-        if validate:
-            code = int(error.item())
-            if code:
-                reasons = ", ".join(text for bit, text in _ERRORS if code & bit)
-                raise ValueError(f"malformed RLE mask: {reasons}")
-    out = []
-    for n, h, w, start in shapes:
-        words = mask_words(h * w)
-        out.append(PackedMasks(bits[start:start + n * words].view(n, words), h, w))
-    return out
+    return RleBatch(images, device, sizes=sizes).decode(validate=validate)
 
 
 def decode_rle(rles: Sequence[dict], device="cuda", *, size: tuple[int, int] | None = None,

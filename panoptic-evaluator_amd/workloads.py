@@ -230,3 +230,33 @@ def config3(device="cpu", seed=3, B=64, H=1024, W=2048):
 
 def isthing_vector(num_classes, num_things):
     return [c < num_things for c in range(num_classes)]
+
+
+def rle_encode(masks: torch.Tensor) -> list:
+    """COCO RLE dicts of N x H x W bool masks in pycocotools' compressed-string
+    form (synthetic inputs for the RLE configuration): column-major runs are
+    found on the masks' device, the strings are written on the host."""
+    N, H, W = masks.shape
+    flat = masks.transpose(1, 2).reshape(N, H * W)  # pixel (y, x) at x * H + y
+    previous = torch.cat([torch.zeros((N, 1), dtype=torch.bool, device=masks.device), flat[:, :-1]], dim=1)
+    rows, positions = (flat != previous).nonzero(as_tuple=True)
+    rows, positions = rows.cpu().tolist(), positions.cpu().tolist()
+    edges = [[0] for _ in range(N)]
+    for r, p in zip(rows, positions):
+        edges[r].append(p)
+    out = []
+    for n in range(N):
+        e = edges[n] + [H * W]
+        counts = [e[i + 1] - e[i] for i in range(len(e) - 1)]
+        chars = []
+        for i, count in enumerate(counts):
+            x = count - counts[i - 2] if i > 2 else count
+            while True:
+                c = x & 0x1F
+                x >>= 5
+                more = (x != -1) if (c & 0x10) else (x != 0)
+                chars.append(chr(48 + (c | 0x20 if more else c)))
+                if not more:
+                    break
+        out.append({"size": [H, W], "counts": "".join(chars).encode("ascii")})
+    return out

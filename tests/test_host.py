@@ -101,3 +101,19 @@ def test_evaluator_signatures_follow_the_reference():
     assert inspect.signature(pe.InstanceEvaluator).parameters["device"].default == "cpu"
     with pytest.raises(ValueError, match="AMD GPUs"):
         pe.InstanceEvaluator(3)
+
+
+def test_workload_rle_encoder_matches_oracle():
+    """The synthetic-input RLE encoder (workloads.rle_encode) writes exactly
+    the oracle's pycocotools strings."""
+    import numpy as np
+    import torch
+    import workloads
+    from oracle import rle as orle
+    rng = np.random.default_rng(3)
+    for n, h, w in ((6, 7, 9), (3, 1, 5), (4, 33, 17), (2, 16, 64)):
+        m = torch.from_numpy(rng.random((n, h, w)) < 0.4)
+        m[0] = True
+        m[1] = False
+        for got, mask in zip(workloads.rle_encode(m), m.numpy()):
+            assert got == orle.encode(mask)

@@ -182,3 +182,30 @@ def test_instance_evaluator_on_rle(pe):
     assert_exact(results[0]["recall"], expected["recall"], "recall")
     for key in SUMMARY:
         assert_close(results[0][key], expected[key], what=key)
+
+
+def test_rle_batch_decodes_in_a_step_graph(pe):
+    """RleBatch keeps its strings and decoded rows on the device: decode()
+    may run inside a whole-step graph (CapturedStep's prepare), and every
+    replay equals the eager step on bool masks."""
+    import workloads
+    from panoptic_evaluator.graphs import CapturedStep
+    pair, masks, classes, scores = workloads.config1(device="cuda", seed=12, B=2, H=120, W=176, detections=30)
+    isthing = workloads.isthing_vector(133, 80)
+    rle = pe.RleBatch([workloads.rle_encode(m) for m in masks], "cuda")
+    decoded = rle.decode(validate=True)
+    for packed, m in zip(decoded, masks):
+        assert_exact(packed.bits, pe.pack_masks(m).bits.cpu())
+    target = pe.PanopticBatch(pair.gt, pair.gc, pair.crowd)
+    prediction = pe.PanopticBatch(pair.pred, pair.pc)
+    eager = pe.SegmentationEvaluator(133, isthing=isthing, validate=False)
+    eager.update(prediction, pe.InstanceBatch(masks, classes, scores), target)
+    want = to_numpy(eager.compute())
+    ev = pe.SegmentationEvaluator(133, isthing=isthing, validate=False)
+    step = CapturedStep(ev, prediction, pe.InstanceBatch(rle.masks, classes, scores), target, prepare=rle.decode)
+    for _ in range(3):
+        rle.bits.fill_(-1)  # every replay decodes again from the strings
+        got = to_numpy(step.replay())
+        assert_exact(got["tp"], want["tp"], "tp")
+        assert_exact(got["instance"]["precision"], want["instance"]["precision"], "precision")
+        assert_exact(got["instance"]["recall"], want["instance"]["recall"], "recall")
