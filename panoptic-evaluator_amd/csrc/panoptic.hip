@@ -166,6 +166,45 @@ struct LdsHashIds {
   }
 };
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(3))) u32x4* LdsBuckets;
+constexpr uint32_t kEmptyKey32 = 0xFFFFFFFFu;
+constexpr int kProbeOn = INT_MIN;  // a full bucket without the key: probe the next
+
+// The table for maps whose pixels are 32-bit (int32 IDs, RGB): buckets of
+// four keys, so one lookup is one pair of 16-byte LDS reads and almost never
+// probes further.  Bucket h is two uint4 words: keys (kEmptyKey32 when free),
+// then what the binary search returns for each key.  Inserts fill a bucket's
+// entries in order and move to the next bucket only when all four are taken,
+// so a bucket whose last entry is free ends every probe.  IDs above INT32_MAX
+// are left out: no 32-bit pixel holds them.
+struct LdsBucketIds {
+  LdsBuckets bucket;
+  uint32_t mask;
+  int shift;
+  __device__ __forceinline__ uint32_t home(uint32_t key) const { return (key * 0x9E3779B1u) >> shift; }
+  // The slot of `key` in one bucket (selects, no branches), -1 when the
+  // bucket is not full and lacks it, kProbeOn when it is full and lacks it.
+  __device__ __forceinline__ static int match(u32x4 k, u32x4 s, uint32_t key) {
+    int r = k.w == kEmptyKey32 ? -1 : kProbeOn;
+    r = k.w == key ? static_cast<int>(s.w) : r;
+    r = k.z == key ? static_cast<int>(s.z) : r;
+    r = k.y == key ? static_cast<int>(s.y) : r;
+    return k.x == key ? static_cast<int>(s.x) : r;
+  }
+  __device__ int probe(uint32_t key, uint32_t h) const {
+    for (;; h = (h + 1) & mask) {
+      const int r = match(bucket[2 * h], bucket[2 * h + 1], key);
+      if (r != kProbeOn) return r;
+    }
+  }
+  __device__ __forceinline__ int find(i64 key) const {
+    if (key == 0) return 0;
+    if (key < 0 || key > INT_MAX) return -1;
+    return probe(static_cast<uint32_t>(key), home(static_cast<uint32_t>(key)));
+  }
+};
+
 template <typename T, bool LOOKUP, typename Table = SortedIds>
 struct SlotResolver {
   using V = typename MapValue<T>::type;
@@ -181,6 +220,44 @@ struct SlotResolver {
       last_slot = table.find(static_cast<i64>(v));
     }
     return last_slot;
+  }
+  // Four consecutive pixels.  Only a positive key that differs from the
+  // positive key before it is looked up (void or invalid specks inside a
+  // segment cost nothing), and the lookups run in rounds of one per lane:
+  // the wave repeats the lookup code once per round while any lane has one
+  // left, usually once per quad, rather than once per pixel position.
+  __device__ __forceinline__ void quad(const V (&v)[4], int (&out)[4]) {
+    if (!LOOKUP || !mapped) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) out[j] = compact_slot(v[j], S);
+      return;
+    }
+    V prev = last_key;
+    uint32_t pending = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      pending |= static_cast<uint32_t>(v[j] > 0 && v[j] != prev) << j;
+      prev = v[j] > 0 ? v[j] : prev;
+    }
+    const uint32_t looked = pending;
+    int r[4] = {0, 0, 0, 0};
+    while (pending) {
+      const int j = __builtin_ctz(pending);
+      const V key = j == 0 ? v[0] : (j == 1 ? v[1] : (j == 2 ? v[2] : v[3]));
+      const int slot = table.find(static_cast<i64>(key));
+#pragma unroll
+      for (int i = 0; i < 4; ++i) r[i] = j == i ? slot : r[i];
+      pending &= pending - 1;
+    }
+    int slot = last_slot;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int here = v[j] == 0 ? 0 : (v[j] < 0 ? -1 : (((looked >> j) & 1u) ? r[j] : slot));
+      out[j] = here;
+      slot = v[j] > 0 ? here : slot;
+    }
+    last_key = prev;
+    last_slot = slot;
   }
 };
 
@@ -198,18 +275,20 @@ struct KeyCache {
     if (GLOBAL) atomicAdd(pairs + key, static_cast<u64>(n));
     else atomicAdd(bins + (key >> 1), n << ((key & 1u) * 16u));
   }
-  __device__ __forceinline__ void add(uint32_t key, uint32_t* bins, u64* pairs) {
-    if (key == k0) {
-      ++c0;
-    } else if (key == k1) {
-      ++c1;
-    } else {
-      if (c1) flush(bins, pairs, k1, c1);
-      k1 = k0;
-      c1 = c0;
-      k0 = key;
-      c0 = 1;
-    }
+  // One pixel's pair (ignored unless `valid`): a hit counts in registers; a
+  // miss flushes the older entry and takes its place.  Selects only, apart
+  // from the flush's atomic.
+  __device__ __forceinline__ void add(uint32_t key, bool valid, uint32_t* bins, u64* pairs) {
+    const bool hit0 = key == k0, hit1 = key == k1 && !hit0;
+    const bool miss = valid && !hit0 && !hit1;
+    const uint32_t out_key = k1, out_count = miss ? c1 : 0u;
+    const uint32_t n0 = miss ? 1u : c0 + static_cast<uint32_t>(valid && hit0);
+    const uint32_t n1 = miss ? c0 : c1 + static_cast<uint32_t>(valid && hit1);
+    k1 = miss ? k0 : k1;
+    k0 = miss ? key : k0;
+    c0 = n0;
+    c1 = n1;
+    if (out_count) flush(bins, pairs, out_key, out_count);
   }
   __device__ __forceinline__ void drain(uint32_t* bins, u64* pairs) {
     if (c0) flush(bins, pairs, k0, c0);
@@ -217,21 +296,27 @@ struct KeyCache {
   }
 };
 
-// The image's ID tables: hashes in LDS (STAGE) or sorted tables in global memory.
-template <bool STAGE>
+// Whether both maps' pixels are 32-bit (the bucket tables apply).
+template <typename GT, typename PR>
+constexpr bool kKey32 = sizeof(typename MapValue<GT>::type) == 4 && sizeof(typename MapValue<PR>::type) == 4;
+
+// The image's ID tables: hashes in LDS (STAGE; buckets for 32-bit pixels) or
+// sorted tables in global memory.
+template <bool STAGE, bool KEY32>
 struct ImageTables {
-  using Table = typename std::conditional<STAGE, LdsHashIds, SortedIds>::type;
+  using Table = typename std::conditional<STAGE, typename std::conditional<KEY32, LdsBucketIds, LdsHashIds>::type,
+                                          SortedIds>::type;
   Table gt, pred;
 };
 
 template <typename GT, typename PR, bool LOOKUP, bool VEC, bool GLOBAL, bool STAGE>
 __device__ __forceinline__ void histogram_chunk(const HistArgs& a, int b, i64 begin, i64 end, uint32_t* bins,
-                                                const ImageTables<STAGE>& t) {
+                                                const ImageTables<STAGE, kKey32<GT, PR>>& t) {
   const GT* gt = static_cast<const GT*>(a.gt);
   const PR* pred = static_cast<const PR*>(a.pred);
   const int tid = threadIdx.x, nt = blockDim.x;  // 512, or 256 beside another stream's kernel
   u64* pairs = a.pairs + static_cast<i64>(b) * a.G * a.P;
-  using Table = typename ImageTables<STAGE>::Table;
+  using Table = typename ImageTables<STAGE, kKey32<GT, PR>>::Table;
   SlotResolver<GT, LOOKUP, Table> gslot{t.gt, a.G, LOOKUP && a.gt_ids != nullptr};
   SlotResolver<PR, LOOKUP, Table> pslot{t.pred, a.P, LOOKUP && a.pred_ids != nullptr};
   KeyCache<GLOBAL> cache;
@@ -239,14 +324,17 @@ __device__ __forceinline__ void histogram_chunk(const HistArgs& a, int b, i64 be
   const uint32_t P = static_cast<uint32_t>(a.P);
   using GV = typename MapValue<GT>::type;
   using PV = typename MapValue<PR>::type;
+  auto count = [&](int g, int p) {
+    const bool valid = (g | p) >= 0;
+    bad |= !valid;
+    // Slots are below 2^24 wherever the bins are in LDS (G * P < 2^17).
+    const uint32_t key = GLOBAL ? static_cast<uint32_t>(g) * P + static_cast<uint32_t>(p)
+                                : __umul24(static_cast<uint32_t>(g), P) + static_cast<uint32_t>(p);
+    cache.add(key, valid, bins, pairs);
+  };
   auto visit = [&](GV gv, PV pv) {
-    int g = gslot(gv);
-    int p = pslot(pv);
-    if ((g | p) < 0) {
-      bad = true;
-      return;
-    }
-    cache.add(static_cast<uint32_t>(g) * P + static_cast<uint32_t>(p), bins, pairs);
+    const int g = gslot(gv);
+    count(g, pslot(pv));
   };
   if (VEC) {
     // Quads of four pixels (load_quad); scalar head/tail of <= 3 pixels.
@@ -260,20 +348,25 @@ __device__ __forceinline__ void histogram_chunk(const HistArgs& a, int b, i64 be
       for (i64 base = (va >> 2) + tid; base < q_end; base += static_cast<i64>(nt) * U) {
         GV gv[U][4];
         PV pv[U][4];
+        // All U quads are loaded before any is used (indices past the end
+        // re-read the last quad and are not counted): a load under its own
+        // bounds test would be waited for inside that test whenever its
+        // pixels need unpacking (RGB), leaving one quad in flight per lane.
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          const i64 q = base + static_cast<i64>(u) * nt;
-          if (q < q_end) {
-            load_quad(gt, q, gv[u]);
-            load_quad(pred, q, pv[u]);
-          }
+          const i64 q = min(base + static_cast<i64>(u) * nt, q_end - 1);
+          load_quad(gt, q, gv[u]);
+          load_quad(pred, q, pv[u]);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const i64 q = base + static_cast<i64>(u) * nt;
           if (q < q_end) {
+            int g[4], p[4];
+            gslot.quad(gv[u], g);
+            pslot.quad(pv[u], p);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) visit(gv[u][j], pv[u][j]);
+            for (int j = 0; j < 4; ++j) count(g[j], p[j]);
           }
         }
       }
@@ -306,9 +399,53 @@ __device__ __forceinline__ void histogram_item(const HistArgs& a, int item, uint
   if (!GLOBAL) {
     for (int w = tid; w < a.words4; w += nt) lds4[w] = make_uint4(0, 0, 0, 0);
   }
-  ImageTables<STAGE> t{};
+  ImageTables<STAGE, kKey32<GT, PR>> t{};
   const i64 gt_row = static_cast<i64>(b) * a.G, pred_row = static_cast<i64>(b) * a.P;
-  if constexpr (STAGE) {
+  if constexpr (STAGE && kKey32<GT, PR>) {
+    // Bucket tables after the bins: 2^bits buckets per map, each four keys
+    // then four words (lowest sorted index while building, then the slot).
+    const int Tg = 1 << a.gt_hash_bits, Tp = 1 << a.pred_hash_bits;
+    auto* table = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(lds4) + a.table_offset);
+    for (int i = tid; i < (Tg + Tp) * 4; i += nt) {
+      const int w = (i >> 2) * 8 + (i & 3);
+      table[w] = kEmptyKey32;
+      table[w + 4] = INT_MAX;
+    }
+    __syncthreads();
+    for (int i = tid; i < a.G + a.P; i += nt) {
+      const bool g = i < a.G;
+      const i64* src = g ? a.gt_ids : a.pred_ids;
+      const int index = g ? i : i - a.G;
+      if (!src || index == 0) continue;  // index 0 is the void entry (ID 0)
+      const i64 key = src[g ? gt_row + index : pred_row + index];
+      if (key <= 0 || key > INT_MAX) continue;  // padding, or no 32-bit pixel holds it
+      uint32_t* own = g ? table : table + 8 * Tg;
+      const uint32_t mask = static_cast<uint32_t>((g ? Tg : Tp) - 1);
+      const uint32_t k32 = static_cast<uint32_t>(key);
+      bool placed = false;
+      for (uint32_t h = hash_id(key, 32 - (g ? a.gt_hash_bits : a.pred_hash_bits)); !placed; h = (h + 1) & mask) {
+        for (int e = 0; e < 4 && !placed; ++e) {
+          const uint32_t prev = atomicCAS(own + 8 * h + e, kEmptyKey32, k32);
+          if (prev == kEmptyKey32 || prev == k32) {
+            atomicMin(reinterpret_cast<int*>(own) + 8 * h + 4 + e, index);
+            placed = true;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    for (int i = tid; i < (Tg + Tp) * 4; i += nt) {
+      const int w = (i >> 2) * 8 + (i & 3);
+      if (table[w] == kEmptyKey32) continue;
+      const bool g = (i >> 2) < Tg;
+      const int S = g ? a.G : a.P;
+      const int lowest = static_cast<int>(table[w + 4]);
+      const int s = g ? a.gt_slots[gt_row + lowest] : a.pred_slots[pred_row + lowest];
+      table[w + 4] = static_cast<uint32_t>((s > 0 && s < S) ? s : -1);
+    }
+    t.gt = {(LdsBuckets)(table), static_cast<uint32_t>(Tg - 1), 32 - a.gt_hash_bits};
+    t.pred = {(LdsBuckets)(table + 8 * Tg), static_cast<uint32_t>(Tp - 1), 32 - a.pred_hash_bits};
+  } else if constexpr (STAGE) {
     // Hash the image's ID tables into LDS after the bins: Tg + Tp int64 keys,
     // then Tg + Tp int32 words (lowest sorted index while building, then the
     // resolved slot).
@@ -487,16 +624,20 @@ int launch_joint_histogram(HistArgs a, int gt_bytes, int pred_bytes, int chunk, 
   if (lookup && a.persist) return fail(kBadArgument, "original-ID maps cannot be packed");
   // ID hashes (12 bytes per entry, at least two entries per ID) after the
   // bins, if the total fits.
-  auto hash_bits = [](int n) {
-    int bits = 4;
-    while (bits < 30 && (static_cast<i64>(1) << bits) < 2 * static_cast<i64>(n)) ++bits;
+  // Maps of 32-bit pixels use buckets of four 32-bit keys (32 bytes per
+  // bucket, at least two entries per ID); int64 maps use 12-byte entries.
+  const bool key32 = gt_bytes <= 4 && pred_bytes <= 4;
+  auto hash_bits = [key32](int n) {
+    int bits = key32 ? 2 : 4;
+    const i64 per = key32 ? 4 : 1;
+    while (bits < 30 && per * (static_cast<i64>(1) << bits) < 2 * static_cast<i64>(n)) ++bits;
     return bits;
   };
   a.gt_hash_bits = hash_bits(a.G);
   a.pred_hash_bits = hash_bits(a.P);
-  const size_t table_bytes =
-      align_up((static_cast<size_t>(1) << a.gt_hash_bits) * 12 + (static_cast<size_t>(1) << a.pred_hash_bits) * 12,
-               16);
+  const size_t entry_bytes = key32 ? 32 : 12;
+  const size_t table_bytes = align_up(((static_cast<size_t>(1) << a.gt_hash_bits) +
+                                       (static_cast<size_t>(1) << a.pred_hash_bits)) * entry_bytes, 16);
   a.stage_tables = lookup && lds + table_bytes <= kMaxHistLds + 16 * 1024;
   if (lookup && !a.stage_tables && !global && table_bytes <= kMaxHistLds) {
     // The bins and the tables do not fit together: bins go to global memory.

@@ -31,7 +31,7 @@ struct HistArgs {
   size_t lds_bytes;
   bool stage_tables;     // original-ID tables hashed into LDS after the bins
   int table_offset;      // their byte offset in LDS
-  int gt_hash_bits;      // log2 of the hash sizes (>= 2 entries per ID)
+  int gt_hash_bits;      // log2 of the table sizes: buckets of 4 keys (32-bit pixels), else entries; >= 2 entries per ID
   int pred_hash_bits;
 };
 

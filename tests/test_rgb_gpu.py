@@ -138,3 +138,62 @@ def test_rgb_lookup_with_global_pair_bins(pe):
         ev = pe.PanopticEvaluator(len(CATS), isthing=[True, True, False, False], validate=validate)
         ev.update(prediction, target)
         compare_panoptic(to_numpy(ev.compute()), expected)
+
+
+def same_home_ids(count, bits, bucket):
+    """24-bit IDs whose home bucket in a table of 2**bits buckets is `bucket`
+    under the histogram's multiplicative hash (x * 0x9E3779B1 mod 2^32, top
+    bits), so that buckets overflow and lookups probe on."""
+    out, x = [], 1
+    while len(out) < count:
+        if ((x * 0x9E3779B1) & 0xFFFFFFFF) >> (32 - bits) == bucket:
+            out.append(x)
+        x += 1
+    return out
+
+
+def test_lookup_with_overflowing_buckets(pe):
+    """20 segments per image (21 slots: 16 buckets of four): twelve IDs share
+    one home bucket, so eight of them live in the buckets after it.  RGB, int32
+    and int64 maps of the same IDs give the oracle's result; an unknown ID
+    homed in the full bucket is refused by validation."""
+    H, W, S = 48, 64, 20
+    rng = np.random.default_rng(31)
+    crowded = same_home_ids(12, 4, 5)
+    maps, infos = [], []
+    yy, xx = np.mgrid[0:H, 0:W]
+    for b in range(2):
+        others = rng.choice(np.arange(1 << 16, 1 << 24), S - len(crowded), replace=False)
+        ids = np.concatenate([crowded, others])
+        rng.shuffle(ids)
+        cy, cx = rng.integers(0, H, S), rng.integers(0, W, S)
+        nearest = np.argmin((yy[..., None] - cy) ** 2 + (xx[..., None] - cx) ** 2, axis=-1)
+        image = ids[nearest]
+        image[rng.random((H, W)) < 0.05] = 0
+        maps.append(image)
+        infos.append(segments_of(image, rng))
+    gt = np.stack(maps).astype(np.int64)
+    pred = np.roll(gt, 3, axis=2)
+    pi = [[dict(e, iscrowd=0) for e in segments_of(p, rng, g)] for p, g in zip(pred, infos)]
+    assert 1 + max(map(len, infos)) <= 21 and sum(int(i) in crowded for i in np.unique(gt)) > 8
+    expected = expected_state(gt, pred, infos, pi)
+    forms = {
+        "rgb": (rgb_tensor(gt), rgb_tensor(pred)),
+        "int32": (torch.from_numpy(gt).int().cuda(), torch.from_numpy(pred).int().cuda()),
+        "int64": (torch.from_numpy(gt).cuda(), torch.from_numpy(pred).cuda()),
+    }
+    for name, (g, p) in forms.items():
+        target = pe.PanopticBatch.from_coco(g, infos, CATS, compact=False)
+        prediction = pe.PanopticBatch.from_coco(p, pi, CATS, compact=False)
+        for validate in (False, True):
+            ev = pe.PanopticEvaluator(len(CATS), isthing=[True, True, False, False], validate=validate)
+            ev.update(prediction, target)
+            compare_panoptic(to_numpy(ev.compute()), expected)
+    unknown_id = same_home_ids(13, 4, 5)[-1]
+    bad = gt.copy()
+    bad[1, 5, 7] = unknown_id
+    for g in (rgb_tensor(bad), torch.from_numpy(bad).int().cuda()):
+        unknown = pe.PanopticBatch.from_coco(g, infos, CATS, compact=False)
+        ev = pe.PanopticEvaluator(len(CATS), validate=True)
+        with pytest.raises(ValueError, match="unknown segment ID"):
+            ev.update(pe.PanopticBatch.from_coco(forms["int32"][1], pi, CATS, compact=False), unknown)
