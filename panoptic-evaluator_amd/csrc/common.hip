@@ -341,7 +341,12 @@ const char* const kBitsVariants[] = {
     "popcounts of ANDed words, 64 dets x 64 GT x 16384 px",
     "detection operands expanded in registers, GT in LDS, 128 dets x 64 GT x 4096 px, 2 detection tiles per wave",
     "detection operands expanded in registers, GT in LDS, 128 dets x 64 GT x 4096 px, 4 detection tiles per wave",
-    "detection operands expanded in registers, GT in LDS, 128 dets x 64 GT x 8192 px, 2 detection tiles per wave"};
+    "detection operands expanded in registers, GT in LDS, 128 dets x 64 GT x 8192 px, 2 detection tiles per wave",
+    "FP4 MFMA 16x16x128, no LDS, 64 dets x 64 GT x 4096 px, 2 blocks ahead",
+    "FP4 MFMA 16x16x128, no LDS, 64 dets x 64 GT, one round of waves, 2 blocks ahead",
+    "FP4 MFMA 32x32x64, no LDS, 64 dets x 64 GT, one round of waves, 4 blocks ahead",
+    "FP4 MFMA 16x16x128, no LDS, 64 dets x 32 GT, one round of waves, 4 blocks ahead",
+    "FP4 MFMA 32x32x64, no LDS, 64 dets x 64 GT, one round of waves (3 per SIMD), 1 block ahead"};
 const char* const kDetBitsVariants[] = {
     "2 words x 8 masks", "2 words x 16 masks", "1 word x 16 masks", "4 words x 4 masks", "4 words x 8 masks",
     "1 word x 8 masks", "2 words x 8 masks, wave-summed bins"};
@@ -351,7 +356,7 @@ const TuneSpec kSpecs[kTuneKernels] = {
     {"mask_overlap_mfma", 9, 0, kOverlapVariants},
     {"packed_grid", 3, 0, kGridVariants},
     {"detection_group", 4, 0, kGroupVariants},
-    {"mask_overlap_bits", 10, 0, kBitsVariants},
+    {"mask_overlap_bits", 15, 0, kBitsVariants},
     {"detection_histogram_bits", 7, 0, kDetBitsVariants},
 };
 std::atomic<int> g_selected[kTuneKernels] = {{-1}, {-1}, {-1}, {-1}, {-1}, {-1}, {-1}};

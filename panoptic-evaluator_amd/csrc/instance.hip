@@ -84,6 +84,7 @@ struct OverlapArgs {
   int dgroups, ggroups;   // work items: dgroups x ggroups x (images * chunks)
   int items;
   int xcd_span;           // > 0: work items per XCD (grid padded to kXcds * xcd_span)
+  i64 chunk_px;           // pixels per work item, for kernels whose chunk is set at launch
   int32_t* inter;
   int32_t* pa;            // may be null
   int32_t* ga;            // may be null
@@ -786,11 +787,199 @@ __global__ __launch_bounds__(64 * DT / DPW) void mask_overlap_bits_reg(OverlapAr
   }
 }
 
+// Bit-packed masks on the FP4 matrix cores: gfx950's scaled f8f6f4 MFMAs
+// (v_mfma_scale_f32_32x32x64_f8f6f4 for M = 32, ..._16x16x128_... for M = 16)
+// with both operands in e2m1 and scales 1.0.  A mask bit becomes the e2m1
+// value 1.0 (nibble 0b0010) or 0.  Lane l supplies 32 pixels of row l % M of
+// each operand: nibble j of operand dword i holds bit 4j + i of the lane's
+// mask word, so a word becomes its operand with one shift and one AND per
+// dword.  Detection and GT operands use the same pixel-to-k map, so a product
+// counts |det AND gt| over the lane group's pixels whatever order the hardware
+// gives the k slots.  A work item's sums are integers of at most Chunk, which
+// FP32 holds exactly.
+// No LDS and no barriers: every wave is one work item (M * DT detections x
+// M * GT GT masks over a pixel range of chunk_px, chosen at launch so that
+// one round of waves covers the batch: each wave streams a long range, and the
+// start-up latency of its first loads is paid once) and keeps DT x GT M x M
+// tiles in accumulation registers.  A block is 4 * (64 / M) words of every row: lane
+// group q = l / M loads words 4q .. 4q + 3 of its row as one 16-byte load, so
+// a row's block is one contiguous run, and the k-step s of the block takes
+// word s of every lane.  Ahead blocks are in flight.
+// Result layout: M = 32, lane l, register r: row 8 (r / 4) + 4 (l / 32) + r % 4,
+// column l % 32; M = 16: row 4 (l / 16) + r, column l % 16 (rows: detections,
+// columns: GT masks).
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ v8i fp4_operand(uint32_t w) {
+  return v8i{static_cast<int>((w << 1) & 0x22222222u), static_cast<int>(w & 0x22222222u),
+             static_cast<int>((w >> 1) & 0x22222222u), static_cast<int>((w >> 2) & 0x22222222u), 0, 0, 0, 0};
+}
+
+template <int M>
+struct Fp4Tile;
+template <>
+struct Fp4Tile<32> {
+  using Acc = v16f;
+  static constexpr int kRegs = 16;
+  __device__ static Acc mma(v8i a, v8i b, Acc c) {
+    return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 4, 4, 0, 127, 0, 127);
+  }
+  __device__ static int row(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+};
+template <>
+struct Fp4Tile<16> {
+  using Acc = v4f;
+  static constexpr int kRegs = 4;
+  __device__ static Acc mma(v8i a, v8i b, Acc c) {
+    return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a, b, c, 4, 4, 0, 127, 0, 127);
+  }
+  __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+template <int M, int DT, int GT, int Ahead>
+__global__ __launch_bounds__(64) void mask_overlap_fp4(OverlapArgs o) {
+  using Tile = Fp4Tile<M>;
+  constexpr int kGroups = kWave / M;        // lane groups per row
+  constexpr int kBlockWords = 4 * kGroups;  // words of a row per block
+  constexpr int kRows = DT + GT;
+  const i64 chunk_px = o.chunk_px;          // a multiple of 32 * kBlockWords (set at launch)
+  OverlapItem it;
+  if (!overlap_item(o, M * DT, M * GT, chunk_px, it)) return;
+  const int D = it.D, G = it.G;
+  if (D == 0 && G == 0) return;  // nothing to count
+  const int lane = threadIdx.x;
+  const int row = lane % M, q = lane / M;
+  const int d_base = it.dgroup * M * DT, g_base = it.ggroup * M * GT;
+  const i64 row_words = mask_words(it.HW);  // a multiple of 4
+  const i64 w_begin = static_cast<i64>(it.chunk) * (chunk_px / 32);
+  const i64 w_end = min(w_begin + chunk_px / 32, row_words);
+  const i64 w_full = w_begin + (w_end - w_begin) / kBlockWords * kBlockWords;  // end of the whole blocks
+  // Tiles holding at least one mask (uniform per wave; 0 on a side with no
+  // masks, whose item still counts the other side's areas).
+  const int dtiles = max(0, min(DT, (D - d_base + M - 1) / M));
+  const int gtiles = max(0, min(GT, (G - g_base + M - 1) / M));
+  const bool count_d = it.pa != nullptr && it.ggroup == 0;
+  const bool count_g = it.ga != nullptr && it.dgroup == 0;
+  // Rows 0 .. DT-1: detection tiles, DT .. DT+GT-1: GT tiles.  Every load is
+  // unconditional, so the compiler can keep Ahead blocks in flight: a row past
+  // its side's count reads the side's last row (or, for a side with no masks,
+  // the other side's first row), and the results of such rows are never
+  // stored (the flush checks det < D and g < G; areas check `valid`).
+  const __attribute__((address_space(1))) uint32_t* src[kRows];
+  bool valid[kRows];
+#pragma unroll
+  for (int j = 0; j < kRows; ++j) {
+    const bool det = j < DT;
+    const int idx = det ? d_base + M * j + row : g_base + M * (j - DT) + row;
+    const int count = det ? D : G;
+    valid[j] = idx < count;
+    const uint8_t* base = count > 0 ? (det ? it.pm : it.gm) : (det ? it.gm : it.pm);
+    const int safe = count > 0 ? min(idx, count - 1) : 0;
+    src[j] = as_global(reinterpret_cast<const uint32_t*>(base)) + static_cast<i64>(safe) * row_words;
+  }
+  // This lane's 4 words of the block at word wb: the piece index is clamped
+  // into the row, so blocks fetched ahead past the end stay in bounds (they
+  // are never used); `masked` zeroes pieces past w_end (the row's last,
+  // partial block).
+  auto load = [&](i64 wb, v4i* piece, bool masked) {
+    const i64 w = wb + 4 * q;
+    const bool inside = w < w_end;
+    const i64 at = min(w, row_words - 4);
+#pragma unroll
+    for (int j = 0; j < kRows; ++j) {
+      const v4i v = *reinterpret_cast<const __attribute__((address_space(1))) v4i*>(src[j] + at);
+      piece[j] = !masked || inside ? v : v4i{0, 0, 0, 0};
+    }
+  };
+  typename Tile::Acc acc[DT][GT];
+#pragma unroll
+  for (int d = 0; d < DT; ++d)
+#pragma unroll
+    for (int t = 0; t < GT; ++t) acc[d][t] = typename Tile::Acc{};
+  int area[kRows];
+#pragma unroll
+  for (int j = 0; j < kRows; ++j) area[j] = 0;
+  auto consume = [&](const v4i* piece) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      v8i op[kRows];
+#pragma unroll
+      for (int j = 0; j < kRows; ++j) {
+        const uint32_t w = static_cast<uint32_t>(piece[j][s]);
+        area[j] += __builtin_popcount(w);
+        op[j] = fp4_operand(w);
+      }
+#pragma unroll
+      for (int d = 0; d < DT; ++d) {
+        if (d >= dtiles) break;
+#pragma unroll
+        for (int t = 0; t < GT; ++t) {
+          if (t >= gtiles) break;
+          acc[d][t] = Tile::mma(op[d], op[DT + t], acc[d][t]);
+        }
+      }
+    }
+  };
+  // Every pass of the main loop consumes and refills exactly Ahead blocks (no
+  // early exit inside it), so the compiler's count of loads in flight stays
+  // exact and each block waits only for its own loads.  The blocks left after
+  // it were fetched by its last pass.
+  v4i buf[Ahead][kRows];
+#pragma unroll
+  for (int a = 0; a < Ahead; ++a) load(w_begin + a * kBlockWords, buf[a], false);
+  i64 w0 = w_begin;
+  for (; w0 + Ahead * kBlockWords <= w_full; w0 += Ahead * kBlockWords) {
+#pragma unroll
+    for (int a = 0; a < Ahead; ++a) {
+      consume(buf[a]);
+      load(w0 + (a + Ahead) * kBlockWords, buf[a], false);
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < Ahead - 1; ++a)
+    if (w0 + a * kBlockWords < w_full) consume(buf[a]);
+  if (w_full < w_end) {  // the row's last block, partial
+    v4i piece[kRows];
+    load(w_full, piece, true);
+    consume(piece);
+  }
+#pragma unroll
+  for (int d = 0; d < DT; ++d) {
+    if (d >= dtiles) break;
+#pragma unroll
+    for (int t = 0; t < GT; ++t) {
+      if (t >= gtiles) break;
+      const int g = g_base + M * t + row;
+      if (g >= G) continue;
+#pragma unroll
+      for (int r = 0; r < Tile::kRegs; ++r) {
+        const int det = d_base + M * d + Tile::row(lane, r);
+        const int v = static_cast<int>(acc[d][t][r]);
+        if (det < D && v) atomicAdd(it.inter + static_cast<i64>(det) * G + g, v);
+      }
+    }
+  }
+  // Areas: the kGroups lanes of a row hold disjoint words of it.
+#pragma unroll
+  for (int j = 0; j < kRows; ++j) {
+    const bool det = j < DT;
+    int sum = area[j];
+#pragma unroll
+    for (int m = M; m < kWave; m <<= 1) sum += __shfl_xor(sum, m, kWave);
+    const int idx = det ? d_base + M * j + row : g_base + M * (j - DT) + row;
+    if (q == 0 && valid[j] && sum && (det ? count_d : count_g)) atomicAdd(det ? it.pa + idx : it.ga + idx, sum);
+  }
+}
+
 struct BitsVariant {
   int dets, gts;
-  i64 chunk_px;
+  i64 chunk_px;  // 0: sized at launch (one round of waves_per_simd waves per SIMD)
   int threads;
   void (*kernel)(OverlapArgs);
+  int block_px = 0;        // chunk granularity of the launch-sized kernels
+  int waves_per_simd = 0;  // their occupancy
 };
 // Order matches the tuning registry's descriptions.
 const BitsVariant kOverlapBits[] = {
@@ -804,6 +993,11 @@ const BitsVariant kOverlapBits[] = {
     {128, 64, 4096, 256, mask_overlap_bits_reg<8, 4, 4096, 2>},
     {128, 64, 4096, 128, mask_overlap_bits_reg<8, 4, 4096, 4>},
     {128, 64, 8192, 256, mask_overlap_bits_reg<8, 4, 8192, 2>},
+    {64, 64, 4096, 64, mask_overlap_fp4<16, 4, 4, 2>, 512, 2},
+    {64, 64, 0, 64, mask_overlap_fp4<16, 4, 4, 2>, 512, 2},
+    {64, 64, 0, 64, mask_overlap_fp4<32, 2, 2, 4>, 256, 2},
+    {64, 32, 0, 64, mask_overlap_fp4<16, 4, 2, 4>, 512, 2},
+    {64, 64, 0, 64, mask_overlap_fp4<32, 2, 2, 1>, 256, 3},
 };
 
 int launch_overlap_bits(OverlapArgs o, int64_t B, int64_t max_d, int64_t max_g, int64_t max_hw, int64_t total_d,
@@ -814,9 +1008,18 @@ int launch_overlap_bits(OverlapArgs o, int64_t B, int64_t max_d, int64_t max_g, 
   if (o.ga && total_g && (rc = launch_zero(o.ga, 4 * static_cast<size_t>(total_g), stream))) return rc;
   if (B == 0 || max_hw == 0 || (max_d == 0 && max_g == 0)) return kOk;
   const BitsVariant& v = kOverlapBits[tuned_variant(kTuneMaskOverlapBits)];
-  const i64 chunks = (max_hw + v.chunk_px - 1) / v.chunk_px;
   const i64 dgroups = std::max<i64>(1, (max_d + v.dets - 1) / v.dets);
   const i64 ggroups = std::max<i64>(1, (max_g + v.gts - 1) / v.gts);
+  i64 chunk_px = v.chunk_px;
+  if (chunk_px == 0) {
+    // One round of waves: every SIMD holds waves_per_simd of them, each with
+    // an equal share of the batch's (image, group, pixel) work, at least four
+    // blocks long.
+    const i64 waves = static_cast<i64>(v.waves_per_simd) * 4 * compute_units();
+    const i64 share = (B * dgroups * ggroups * max_hw + waves - 1) / waves;
+    chunk_px = std::max<i64>(4 * v.block_px, (share + v.block_px - 1) / v.block_px * v.block_px);
+  }
+  const i64 chunks = (max_hw + chunk_px - 1) / chunk_px;
   const i64 items = dgroups * ggroups * B * chunks;
   if (items >= INT_MAX - kXcds) return fail(kUnsupportedShape, "too many masks or pixels for one overlap launch");
   o.chunks = static_cast<int>(chunks);
@@ -824,6 +1027,7 @@ int launch_overlap_bits(OverlapArgs o, int64_t B, int64_t max_d, int64_t max_g, 
   o.ggroups = static_cast<int>(ggroups);
   o.items = static_cast<int>(items);
   o.xcd_span = static_cast<int>((items + kXcds - 1) / kXcds);
+  o.chunk_px = chunk_px;
   ProfileScope timing("mask_overlap_bits", stream);
   v.kernel<<<static_cast<unsigned>(static_cast<i64>(o.xcd_span) * kXcds), v.threads, 0, stream>>>(o);
   PE_LAUNCH_CHECK();
