@@ -52,6 +52,7 @@ for _path in (ROOT, ROOT / "panoptic-evaluator_amd"):
 METRIC = "images/sec PQ+mIoU+mask AP, batched vs plain on 1 GPU; p50/p99 reward latency"
 HBM_PEAK_GBS = 8000.0
 MFMA_INT8_PEAK_TOPS = 5000.0  # dense int8, MI355X
+MFMA_FP4_PEAK_TFLOPS = 10100.0  # dense fp4 (e2m1), MI355X
 
 
 def log(*args):
@@ -603,13 +604,18 @@ class RleCombinedConfig(CombinedConfig):
 
 
 class PackedInstanceConfig(InstanceConfig):
-    """c1i with bit-packed detection and GT masks: intersections are popcounts
-    of ANDed words (mask_overlap_bits) instead of the int8 MFMA contraction.
-    Packing happens once in setup, outside the timed steps."""
+    """c1i with bit-packed detection and GT masks: the intersections run on the
+    FP4 matrix cores (mask bits expanded to e2m1 operands in registers) or, in
+    the other tuner variants, as int8 MFMA over bytes expanded in LDS or as
+    popcounts of ANDed words.  Packing happens once in setup, outside the timed
+    steps."""
 
     dominant = "mask_overlap_bits"
     tuned = ("mask_overlap_bits",)
-    dtype = "bit-packed masks (popcount, int32 accumulate, f64 ratios)"
+    dtype = "bit-packed masks (e2m1 MFMA operands, exact f32 accumulate of counts below 2^24, f64 ratios)"
+    mfma_peak = MFMA_FP4_PEAK_TFLOPS
+    mfma_unit = "TFLOP/s (e2m1)"
+    mfma_variant = "FP4"
 
     def setup(self, device):
         import panoptic_evaluator as pe
@@ -621,7 +627,6 @@ class PackedInstanceConfig(InstanceConfig):
         H, W = self.config["height"], self.config["width"]
         rows = sum(m.shape[0] for m in self.masks) + sum(m.shape[0] for m in self.gt_masks)
         self.bytes_per_launch = 4 * mask_words(H * W) * rows
-        self.mfma_ops = None
         self.config["mask_format"] = "bits"
 
 
@@ -972,11 +977,13 @@ def measure_steps(cfg, args, *, cpu_budget=None):
         "cpu_baseline": cpu,
         "tuning": tuning_report,
     }
-    if getattr(cfg, "mfma_ops", None) and launches:
+    variant = (tuning_report or {}).get(cfg.dominant, {}).get("describe", "")
+    if getattr(cfg, "mfma_ops", None) and launches and getattr(cfg, "mfma_variant", "") in variant:
         tops = cfg.mfma_ops / (avg_ms * 1e-3) / 1e12
+        peak = getattr(cfg, "mfma_peak", MFMA_INT8_PEAK_TOPS)
         line["mfma"] = {"kernel": cfg.dominant, "dense_equivalent_ops_per_launch": cfg.mfma_ops,
-                        "achieved": round(tops, 2), "peak": MFMA_INT8_PEAK_TOPS, "unit": "TOP/s",
-                        "frac": round(tops / MFMA_INT8_PEAK_TOPS, 4)}
+                        "achieved": round(tops, 2), "peak": peak, "unit": getattr(cfg, "mfma_unit", "TOP/s"),
+                        "frac": round(tops / peak, 4)}
     return line
 
 

@@ -30,6 +30,7 @@
 //    cumulative TP/FP scan runs forward and the precision envelope backward,
 //    matching the reference's searchsorted/cummax result exactly.
 
+#include <atomic>
 #include <cfloat>
 #include <climits>
 #include <type_traits>
@@ -975,11 +976,10 @@ __global__ __launch_bounds__(64) void mask_overlap_fp4(OverlapArgs o) {
 
 struct BitsVariant {
   int dets, gts;
-  i64 chunk_px;  // 0: sized at launch (one round of waves_per_simd waves per SIMD)
+  i64 chunk_px;  // 0: sized at launch (one round of as many waves as fit on the device)
   int threads;
   void (*kernel)(OverlapArgs);
-  int block_px = 0;        // chunk granularity of the launch-sized kernels
-  int waves_per_simd = 0;  // their occupancy
+  int block_px = 0;  // chunk granularity of the launch-sized kernels
 };
 // Order matches the tuning registry's descriptions.
 const BitsVariant kOverlapBits[] = {
@@ -993,12 +993,33 @@ const BitsVariant kOverlapBits[] = {
     {128, 64, 4096, 256, mask_overlap_bits_reg<8, 4, 4096, 2>},
     {128, 64, 4096, 128, mask_overlap_bits_reg<8, 4, 4096, 4>},
     {128, 64, 8192, 256, mask_overlap_bits_reg<8, 4, 8192, 2>},
-    {64, 64, 4096, 64, mask_overlap_fp4<16, 4, 4, 2>, 512, 2},
-    {64, 64, 0, 64, mask_overlap_fp4<16, 4, 4, 2>, 512, 2},
-    {64, 64, 0, 64, mask_overlap_fp4<32, 2, 2, 4>, 256, 2},
-    {64, 32, 0, 64, mask_overlap_fp4<16, 4, 2, 4>, 512, 2},
-    {64, 64, 0, 64, mask_overlap_fp4<32, 2, 2, 1>, 256, 3},
+    {64, 64, 4096, 64, mask_overlap_fp4<16, 4, 4, 2>, 512},
+    {64, 64, 0, 64, mask_overlap_fp4<16, 4, 4, 2>, 512},
+    {64, 64, 0, 64, mask_overlap_fp4<32, 2, 2, 4>, 256},
+    {64, 32, 0, 64, mask_overlap_fp4<16, 4, 2, 4>, 512},
+    {64, 64, 0, 64, mask_overlap_fp4<32, 2, 2, 1>, 256},
+    {128, 64, 0, 64, mask_overlap_fp4<16, 8, 4, 2>, 512},
+    {128, 64, 0, 64, mask_overlap_fp4<16, 8, 4, 1>, 512},
+    {128, 64, 0, 64, mask_overlap_fp4<32, 4, 2, 2>, 256},
 };
+constexpr int kOverlapBitsCount = sizeof(kOverlapBits) / sizeof(kOverlapBits[0]);
+
+// Workgroups of variant v resident per compute unit (HIP's occupancy query,
+// cached per device and variant).
+int resident_per_cu(int v) {
+  static std::atomic<int> cached[64][kOverlapBitsCount] = {};
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= 64) device = 0;
+  int n = cached[device][v].load(std::memory_order_relaxed);
+  if (n <= 0) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kOverlapBits[v].kernel, kOverlapBits[v].threads, 0) !=
+            hipSuccess ||
+        n <= 0)
+      n = 8;
+    cached[device][v].store(n, std::memory_order_relaxed);
+  }
+  return n;
+}
 
 int launch_overlap_bits(OverlapArgs o, int64_t B, int64_t max_d, int64_t max_g, int64_t max_hw, int64_t total_d,
                         int64_t total_g, int64_t total_inter, hipStream_t stream) {
@@ -1007,17 +1028,21 @@ int launch_overlap_bits(OverlapArgs o, int64_t B, int64_t max_d, int64_t max_g, 
   if (o.pa && total_d && (rc = launch_zero(o.pa, 4 * static_cast<size_t>(total_d), stream))) return rc;
   if (o.ga && total_g && (rc = launch_zero(o.ga, 4 * static_cast<size_t>(total_g), stream))) return rc;
   if (B == 0 || max_hw == 0 || (max_d == 0 && max_g == 0)) return kOk;
-  const BitsVariant& v = kOverlapBits[tuned_variant(kTuneMaskOverlapBits)];
+  const int variant = tuned_variant(kTuneMaskOverlapBits);
+  const BitsVariant& v = kOverlapBits[variant];
   const i64 dgroups = std::max<i64>(1, (max_d + v.dets - 1) / v.dets);
   const i64 ggroups = std::max<i64>(1, (max_g + v.gts - 1) / v.gts);
   i64 chunk_px = v.chunk_px;
   if (chunk_px == 0) {
-    // One round of waves: every SIMD holds waves_per_simd of them, each with
+    // One round of workgroups: as many as the device holds at once, each with
     // an equal share of the batch's (image, group, pixel) work, at least four
     // blocks long.
-    const i64 waves = static_cast<i64>(v.waves_per_simd) * 4 * compute_units();
+    const i64 waves = static_cast<i64>(resident_per_cu(variant)) * compute_units();
     const i64 share = (B * dgroups * ggroups * max_hw + waves - 1) / waves;
     chunk_px = std::max<i64>(4 * v.block_px, (share + v.block_px - 1) / v.block_px * v.block_px);
+    // FP32 accumulation is exact while every sum stays below 2^24: a work
+    // item covers at most 2^22 pixels.
+    chunk_px = std::min<i64>(chunk_px, i64{1} << 22);
   }
   const i64 chunks = (max_hw + chunk_px - 1) / chunk_px;
   const i64 items = dgroups * ggroups * B * chunks;

@@ -99,6 +99,51 @@ def test_overlap_bits_variants(pe, tuning, D, G, H, W):
         assert_exact(ga.long(), gflat.sum(1))
 
 
+def ellipse_masks(rng, n, H, W):
+    yy, xx = np.mgrid[:H, :W]
+    cy, cx = rng.uniform(0, H, n), rng.uniform(0, W, n)
+    ry, rx = rng.uniform(4, H / 3, n), rng.uniform(4, W / 3, n)
+    return ((yy[None] - cy[:, None, None]) / ry[:, None, None]) ** 2 + \
+        ((xx[None] - cx[:, None, None]) / rx[:, None, None]) ** 2 <= 1
+
+
+def test_overlap_bits_variants_full_size_uneven_batch(pe, tuning):
+    """Every launch variant on two uneven full-size images in one batch (the
+    FP4 forms size their pixel ranges from the batch; ranges end inside a
+    block, groups are partly filled) against an exact float32 product."""
+    from panoptic_evaluator import _native
+    rng = np.random.default_rng(11)
+    shapes = [(100, 48, 800, 1088), (37, 70, 333, 501)]
+    pms = [ellipse_masks(rng, d, h, w) for d, g, h, w in shapes]
+    gms = [ellipse_masks(rng, g, h, w) for d, g, h, w in shapes]
+    expected = [(p.reshape(len(p), -1).astype(np.float32) @ g.reshape(len(g), -1).astype(np.float32).T)
+                .astype(np.int64) for p, g in zip(pms, gms)]
+    pbits = [pe.pack_masks(cuda(p)) for p in pms]
+    gbits = [pe.pack_masks(cuda(g)) for g in gms]
+    dets = [d for d, _, _, _ in shapes]
+    gts = [g for _, g, _, _ in shapes]
+    det_off, gt_off = np.cumsum([0] + dets), np.cumsum([0] + gts)
+    inter_off = np.cumsum([0] + [d * g for d, g in zip(dets, gts)])
+    pixels = [h * w for _, _, h, w in shapes]
+    table = lambda xs: _native.device_table(list(map(int, xs)), "cuda")  # noqa: E731
+    for variant in range(len(tuning.variants("mask_overlap_bits"))):
+        tuning.select("mask_overlap_bits", variant)
+        inter = torch.empty(int(inter_off[-1]), dtype=torch.int32, device="cuda")
+        pa = torch.empty(int(det_off[-1]), dtype=torch.int32, device="cuda")
+        ga = torch.empty(int(gt_off[-1]), dtype=torch.int32, device="cuda")
+        _native.call("pe_mask_overlap_bits_batch", 2, table([b.bits.data_ptr() for b in pbits]).data_ptr(),
+                     table([b.bits.data_ptr() for b in gbits]).data_ptr(), table(det_off).data_ptr(),
+                     table(gt_off).data_ptr(), table(inter_off).data_ptr(), table(pixels).data_ptr(), 0,
+                     max(dets), max(gts), max(pixels), int(det_off[-1]), int(gt_off[-1]), int(inter_off[-1]),
+                     inter.data_ptr(), pa.data_ptr(), ga.data_ptr(), _native.stream())
+        got = inter.cpu().numpy().astype(np.int64)
+        for b in range(2):
+            block = got[inter_off[b]:inter_off[b + 1]].reshape(dets[b], gts[b])
+            assert_exact(block, expected[b], f"variant {variant}, image {b}")
+        assert_exact(pa.cpu().numpy(), np.concatenate([p.reshape(len(p), -1).sum(1) for p in pms]))
+        assert_exact(ga.cpu().numpy(), np.concatenate([g.reshape(len(g), -1).sum(1) for g in gms]))
+
+
 def uneven_instance_inputs(seed):
     rng = np.random.default_rng(seed)
     sizes = [(24, 37), (40, 64), (17, 29), (33, 50)]
