@@ -349,7 +349,9 @@ const char* const kBitsVariants[] = {
     "FP4 MFMA 32x32x64, no LDS, 64 dets x 64 GT, one round of waves, 1 block ahead",
     "FP4 MFMA 16x16x128, no LDS, 128 dets x 64 GT, one round of waves, 2 blocks ahead",
     "FP4 MFMA 16x16x128, no LDS, 128 dets x 64 GT, one round of waves, 1 block ahead",
-    "FP4 MFMA 32x32x64, no LDS, 128 dets x 64 GT, one round of waves, 2 blocks ahead"};
+    "FP4 MFMA 32x32x64, no LDS, 128 dets x 64 GT, one round of waves, 2 blocks ahead",
+    "FP4 MFMA 16x16x128, no LDS, 2 waves of 64 dets x 64 GT in step, one round, 2 blocks ahead",
+    "FP4 MFMA 16x16x128, no LDS, 2 waves of 64 dets x 64 GT in step, one round, 3 blocks ahead"};
 const char* const kDetBitsVariants[] = {
     "2 words x 8 masks", "2 words x 16 masks", "1 word x 16 masks", "4 words x 4 masks", "4 words x 8 masks",
     "1 word x 8 masks", "2 words x 8 masks, wave-summed bins"};
@@ -359,7 +361,7 @@ const TuneSpec kSpecs[kTuneKernels] = {
     {"mask_overlap_mfma", 9, 0, kOverlapVariants},
     {"packed_grid", 3, 0, kGridVariants},
     {"detection_group", 4, 0, kGroupVariants},
-    {"mask_overlap_bits", 18, 11, kBitsVariants},
+    {"mask_overlap_bits", 20, 18, kBitsVariants},
     {"detection_histogram_bits", 7, 0, kDetBitsVariants},
 };
 std::atomic<int> g_selected[kTuneKernels] = {{-1}, {-1}, {-1}, {-1}, {-1}, {-1}, {-1}};

@@ -839,20 +839,26 @@ struct Fp4Tile<16> {
   __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-template <int M, int DT, int GT, int Ahead>
-__global__ __launch_bounds__(64) void mask_overlap_fp4(OverlapArgs o) {
+template <int M, int DT, int GT, int Ahead, int WG = 1>
+__global__ __launch_bounds__(64 * WG) void mask_overlap_fp4(OverlapArgs o) {
   using Tile = Fp4Tile<M>;
   constexpr int kGroups = kWave / M;        // lane groups per row
   constexpr int kBlockWords = 4 * kGroups;  // words of a row per block
   constexpr int kRows = DT + GT;
   const i64 chunk_px = o.chunk_px;          // a multiple of 32 * kBlockWords (set at launch)
   OverlapItem it;
-  if (!overlap_item(o, M * DT, M * GT, chunk_px, it)) return;
+  if (!overlap_item(o, M * DT * WG, M * GT, chunk_px, it)) return;
   const int D = it.D, G = it.G;
   if (D == 0 && G == 0) return;  // nothing to count
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
   const int row = lane % M, q = lane / M;
-  const int d_base = it.dgroup * M * DT, g_base = it.ggroup * M * GT;
+  // WG > 1: the waves of a workgroup take consecutive detection groups of one
+  // pixel range and stay within Ahead blocks of each other (a barrier per
+  // pass of the main loop), so the GT rows they share are read from HBM once
+  // and found in the cache by the other waves.  A wave without detections
+  // leaves at once (the barrier then waits only for the waves still running).
+  const int d_base = (it.dgroup * WG + wave) * M * DT, g_base = it.ggroup * M * GT;
+  if (wave > 0 && d_base >= D) return;
   const i64 row_words = mask_words(it.HW);  // a multiple of 4
   const i64 w_begin = static_cast<i64>(it.chunk) * (chunk_px / 32);
   const i64 w_end = min(w_begin + chunk_px / 32, row_words);
@@ -862,7 +868,7 @@ __global__ __launch_bounds__(64) void mask_overlap_fp4(OverlapArgs o) {
   const int dtiles = max(0, min(DT, (D - d_base + M - 1) / M));
   const int gtiles = max(0, min(GT, (G - g_base + M - 1) / M));
   const bool count_d = it.pa != nullptr && it.ggroup == 0;
-  const bool count_g = it.ga != nullptr && it.dgroup == 0;
+  const bool count_g = it.ga != nullptr && it.dgroup == 0 && wave == 0;
   // Rows 0 .. DT-1: detection tiles, DT .. DT+GT-1: GT tiles.  Every load is
   // unconditional, so the compiler can keep Ahead blocks in flight: a row past
   // its side's count reads the side's last row (or, for a side with no masks,
@@ -932,6 +938,7 @@ __global__ __launch_bounds__(64) void mask_overlap_fp4(OverlapArgs o) {
   for (int a = 0; a < Ahead; ++a) load(w_begin + a * kBlockWords, buf[a], false);
   i64 w0 = w_begin;
   for (; w0 + Ahead * kBlockWords <= w_full; w0 += Ahead * kBlockWords) {
+    if constexpr (WG > 1) __builtin_amdgcn_s_barrier();  // progress only: no memory is shared
 #pragma unroll
     for (int a = 0; a < Ahead; ++a) {
       consume(buf[a]);
@@ -1001,6 +1008,8 @@ const BitsVariant kOverlapBits[] = {
     {128, 64, 0, 64, mask_overlap_fp4<16, 8, 4, 2>, 512},
     {128, 64, 0, 64, mask_overlap_fp4<16, 8, 4, 1>, 512},
     {128, 64, 0, 64, mask_overlap_fp4<32, 4, 2, 2>, 256},
+    {128, 64, 0, 128, mask_overlap_fp4<16, 4, 4, 2, 2>, 512},
+    {128, 64, 0, 128, mask_overlap_fp4<16, 4, 4, 3, 2>, 512},
 };
 constexpr int kOverlapBitsCount = sizeof(kOverlapBits) / sizeof(kOverlapBits[0]);
 
