@@ -371,6 +371,22 @@ int pe_score_order(int64_t B, int64_t N, int64_t max_dets, const int64_t* det_of
                    int64_t* sorted_classes, int32_t* sorted_classes32, void* workspace,
                    int64_t workspace_bytes, void* stream);
 
+/* The tables InstanceEvaluator.update hands to pe_coco_match, in one launch
+ * for at most 64 images: det_area[i] = pa[det_off[b] + order[i]] as f64 for
+ * the i-th detection in score order (pa in mask order, as pe_mask_overlap*
+ * writes it), and for the GT masks of every image concatenated (image b at
+ * gt_off[b]) their f64 pixel areas (from ga), classes as int64 and int32 and
+ * crowd flags.  gt_classes and gt_crowd are HOST arrays of B device pointers
+ * (image b's int32 or int64 classes, per class_bytes, and its bool flags),
+ * copied into the kernel's arguments.  Replaces the per-image pixel areas,
+ * their score-order permutation and the concatenation of classes, crowd flags
+ * and areas that the reference's InstanceEvaluator update performs
+ * (instance.py:121, :127-130, :143, :186-197). */
+int pe_instance_tables(int64_t B, const int64_t* det_off, const int64_t* order, const int32_t* pa, int64_t N,
+                       const int64_t* gt_off, const int32_t* ga, int64_t total_g, const void* const* gt_classes,
+                       int class_bytes, const uint8_t* const* gt_crowd, double* det_area, double* gt_area,
+                       int64_t* gt_cls, int32_t* gt_cls32, uint8_t* crowd, void* stream);
+
 /* pe_score_order for at most 64 images of at most 4096 detections each, read
  * in place: scores and classes are HOST arrays of B device pointers (image b's
  * f64 scores and its int32 or int64 classes, per class_bytes), copied into the

@@ -3540,6 +3540,77 @@ extern "C" int pe_score_order(int64_t B, int64_t N, int64_t max_dets, const int6
   return kOk;
 }
 
+// The per-batch tables the matcher reads, in one launch: detection pixel
+// areas in score order (f64), and the GT pixel areas (f64), classes (int64 and
+// int32) and crowd flags of every image concatenated.  Image b's GT classes and
+// crowd flags are read in place through the addresses in the arguments.
+struct GtArrays {
+  const void* classes[kOrderArgImages];
+  const uint8_t* crowd[kOrderArgImages];
+  int class_bytes;
+};
+
+// Image of row i of a batch laid out by offsets off[0..B] (off[B] > i).
+__device__ __forceinline__ int image_of(const i64* off, int B, i64 i) {
+  int lo = 0, hi = B - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void instance_tables(int B, const i64* det_off, const i64* order, const int32_t* pa,
+                                                       i64 N, const i64* gt_off, const int32_t* ga, i64 total_g,
+                                                       GtArrays g, double* det_area, double* gt_area, i64* gt_cls,
+                                                       int32_t* gt_cls32, uint8_t* crowd) {
+  const i64 stride = static_cast<i64>(gridDim.x) * blockDim.x;
+  for (i64 i = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; i < N + total_g; i += stride) {
+    if (i < N) {
+      const int b = image_of(det_off, B, i);
+      det_area[i] = static_cast<double>(pa[det_off[b] + order[i]]);
+    } else {
+      const i64 j = i - N;
+      const int b = image_of(gt_off, B, j);
+      const i64 k = j - gt_off[b];
+      const i64 c = g.class_bytes == 8 ? static_cast<const i64*>(g.classes[b])[k]
+                                       : static_cast<i64>(static_cast<const int32_t*>(g.classes[b])[k]);
+      gt_area[j] = static_cast<double>(ga[j]);
+      gt_cls[j] = c;
+      gt_cls32[j] = static_cast<int32_t>(c);
+      crowd[j] = g.crowd[b][k];
+    }
+  }
+}
+
+extern "C" int pe_instance_tables(int64_t B, const int64_t* det_off, const int64_t* order, const int32_t* pa,
+                                  int64_t N, const int64_t* gt_off, const int32_t* ga, int64_t total_g,
+                                  const void* const* gt_classes, int class_bytes, const uint8_t* const* gt_crowd,
+                                  double* det_area, double* gt_area, int64_t* gt_cls, int32_t* gt_cls32,
+                                  uint8_t* crowd, void* stream_ptr) {
+  PE_REQUIRE(B >= 0 && B <= kOrderArgImages, "at most 64 images per call");
+  PE_REQUIRE(N >= 0 && total_g >= 0, "invalid table sizes");
+  PE_REQUIRE(class_bytes == 4 || class_bytes == 8, "classes must be int32 or int64");
+  PE_REQUIRE(B == 0 || (det_off && gt_off && gt_classes && gt_crowd), "null batch tables");
+  PE_REQUIRE(N == 0 || (order && pa && det_area), "null detection tables");
+  PE_REQUIRE(total_g == 0 || (ga && gt_area && gt_cls && gt_cls32 && crowd), "null GT tables");
+  if (B == 0 || N + total_g == 0) return kOk;
+  GtArrays g{};
+  for (int64_t b = 0; b < B; ++b) {
+    g.classes[b] = gt_classes[b];
+    g.crowd[b] = gt_crowd[b];
+  }
+  g.class_bytes = class_bytes;
+  const i64 blocks = std::min<i64>((N + total_g + 255) / 256, 4 * compute_units());
+  hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+  instance_tables<<<static_cast<unsigned>(blocks), 256, 0, stream>>>(
+      static_cast<int>(B), reinterpret_cast<const i64*>(det_off), reinterpret_cast<const i64*>(order), pa, N,
+      reinterpret_cast<const i64*>(gt_off), ga, total_g, g, det_area, gt_area, reinterpret_cast<i64*>(gt_cls),
+      gt_cls32, crowd);
+  PE_LAUNCH_CHECK();
+  return kOk;
+}
+
 extern "C" int pe_score_order_images(int64_t B, int64_t max_dets, const int64_t* det_off,
                                      const double* const* scores, const void* const* classes, int class_bytes,
                                      int64_t* order, double* sorted_scores, int64_t* sorted_classes,

@@ -40,6 +40,8 @@ for _name, _sig in {
     "pe_score_order_workspace_bytes": (_i64, [_i64, _i64]),
     "pe_score_order": (_int, [_i64, _i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _ptr]),
     "pe_score_order_images": (_int, [_i64, _i64, _ptr, _ptr, _ptr, _int, _ptr, _ptr, _ptr, _ptr, _ptr]),
+    "pe_instance_tables": (_int, [_i64, _ptr, _ptr, _ptr, _i64, _ptr, _ptr, _i64, _ptr, _int, _ptr, _ptr, _ptr, _ptr,
+                                  _ptr, _ptr, _ptr]),
     "pe_ap_workspace_bytes": (_i64, [_i64, _i64]),
     "pe_ap_compute": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _ptr, _ptr, _ptr, _ptr, _i64, _ptr]),
     "pe_row_sums": (_int, [_ptr, _i64, _i64, _ptr, _ptr]),
@@ -216,6 +218,35 @@ def score_order(images, device) -> ScoreOrder:
     return ScoreOrder(counts, det_off, order, sorted_scores, sorted_classes, sorted_classes32)
 
 
+def match_tables(ranked, pa, ga, gt_off, targets, device, stream):
+    """Detection pixel areas in score order and the GT areas, classes and crowd
+    flags of every image concatenated (instance.py:121, :127-130, :143): one
+    launch (pe_instance_tables) reading each image's arrays in place."""
+    B, N, total_g = len(targets), pa.numel(), ga.numel()
+    class_dtype = targets[0].classes.dtype if targets else torch.int64
+    if B <= ORDER_ARG_IMAGES and all(g.classes.dtype == class_dtype and g.classes.is_contiguous()
+                                     and g.crowd.is_contiguous() for g in targets):
+        det_area = torch.empty(N, dtype=torch.float64, device=device)
+        gt_area = torch.empty(total_g, dtype=torch.float64, device=device)
+        gt_cls = torch.empty(total_g, dtype=torch.int64, device=device)
+        gt_cls32 = torch.empty(total_g, dtype=torch.int32, device=device)
+        crowd = torch.empty(total_g, dtype=torch.bool, device=device)
+        pointers = ctypes.c_void_p * B
+        _native.call("pe_instance_tables", B, ranked.det_off.data_ptr(), ranked.order.data_ptr(), pa.data_ptr(), N,
+                     gt_off.data_ptr(), ga.data_ptr(), total_g, pointers(*(g.classes.data_ptr() for g in targets)),
+                     8 if class_dtype == torch.int64 else 4, pointers(*(g.crowd.data_ptr() for g in targets)),
+                     det_area.data_ptr(), gt_area.data_ptr(), gt_cls.data_ptr(), gt_cls32.data_ptr(),
+                     crowd.data_ptr(), stream)
+        return det_area, gt_area, gt_cls, gt_cls32, crowd
+    first, base = 0, []
+    for count in ranked.counts:
+        base.extend([first] * count)
+        first += count
+    det_area = pa.double()[ranked.order + _native.device_table(base, device)]
+    gt_cls = _cat([g.classes for g in targets], torch.int64, device)
+    return det_area, ga.double(), gt_cls, gt_cls.int(), _cat([g.crowd for g in targets], torch.bool, device)
+
+
 def raise_instance_error(code: int, *, combined: bool = False):
     """Map the instance validation bitmask to the reference's messages."""
     if code & 1:
@@ -357,14 +388,7 @@ class InstanceEvaluator:
                          _native.device_table([p.masks.data_ptr() for p in predictions], device).data_ptr(),
                          _native.device_table([g.masks.data_ptr() for g in targets], device).data_ptr(),
                          *shape_args, aligned, stream)
-        # Detection pixel areas in score order.
-        first, base = 0, []
-        for count in ranked.counts:
-            base.extend([first] * count)
-            first += count
-        det_area = pa.double()[ranked.order + _native.device_table(base, device)]
-        gt_cls = _cat([g.classes for g in targets], torch.int64, device)
-        gt_area = ga.double()
+        det_area, gt_area, gt_cls, gt_cls32, crowd = match_tables(ranked, pa, ga, gt_off, targets, device, stream)
         # Annotation areas drive the area ranges; images without them use the
         # pixel areas (the matcher reads gt_area when none are given at all).
         ann = None
@@ -385,8 +409,6 @@ class InstanceEvaluator:
         det_row = ranked.order  # intersection rows are in mask order
         status = torch.empty((N, 40), dtype=torch.int8, device=device)
         ranks = torch.empty(N, dtype=torch.int32, device=device)
-        gt_cls32 = gt_cls.int()
-        crowd = _cat([g.crowd for g in targets], torch.bool, device)
         _native.call("pe_coco_match", len(predictions), C, det_off.data_ptr(), gt_off.data_ptr(),
                      inter_off.data_ptr(), det_cls32.data_ptr(), det_area.data_ptr(), det_row.data_ptr(),
                      gt_cls32.data_ptr(), gt_area.data_ptr(), crowd.data_ptr(), _native.ptr(ann), inter.data_ptr(), 4,

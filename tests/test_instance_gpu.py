@@ -476,6 +476,43 @@ def test_score_order_input_dtypes(pe):
         compare_instance(to_numpy(ev.compute()), expected)
 
 
+def test_match_tables_gt_dtypes_and_image_counts(pe):
+    """The matcher's GT tables (areas, classes, crowd flags) built in one launch
+    from each image's arrays in place, or by concatenation when the batch has
+    mixed GT class dtypes or more than 64 images: every form gives the oracle's
+    results."""
+    rng = np.random.default_rng(13)
+    B = 70
+    dets = rng.integers(0, 6, B)
+    gts = rng.integers(0, 5, B)
+    pm = [rng.random((d, 12, 15)) > 0.6 for d in dets]
+    gm = [rng.random((g, 12, 15)) > 0.55 for g in gts]
+    for p, g in zip(pm, gm):
+        if len(p) and len(g):
+            p[0] = g[0]
+    pcl = [rng.integers(0, 3, d) for d in dets]
+    gcl = [rng.integers(0, 3, g) for g in gts]
+    sc = [rng.random(d) for d in dets]
+    cr = [rng.random(g) < 0.2 for g in gts]
+    for images in (5, B):
+        state = oinst.InstanceState(3)
+        state.update([dict(masks=m, classes=c, scores=s) for m, c, s in zip(pm[:images], pcl[:images], sc[:images])],
+                     [dict(masks=m, classes=c, crowd=k) for m, c, k in zip(gm[:images], gcl[:images], cr[:images])])
+        expected = state.compute()
+        forms = {
+            "int64": [cuda(c) for c in gcl[:images]],
+            "int32": [cuda(c).int() for c in gcl[:images]],
+            "mixed": [cuda(c).int() if i % 2 else cuda(c) for i, c in enumerate(gcl[:images])],
+        }
+        for name, classes in forms.items():
+            ev = pe.InstanceEvaluator(3, device="cuda")
+            ev.update(pe.InstanceBatch([cuda(m) for m in pm[:images]], [cuda(c) for c in pcl[:images]],
+                                       [cuda(s) for s in sc[:images]]),
+                      pe.InstanceBatch([cuda(m) for m in gm[:images]], classes,
+                                       crowd=[cuda(k) for k in cr[:images]]))
+            compare_instance(to_numpy(ev.compute()), expected)
+
+
 def test_matcher_with_many_pairs_of_one_class(pe):
     """40 detections x 20 ground-truth masks of one class exceed the matcher's
     LDS staging (512 pairs), so its chains read global memory; a second image
