@@ -413,6 +413,17 @@ int pe_ap_compute(int64_t N, const double* scores, const int32_t* classes, const
                   double* recall, double* summary, void* workspace, int64_t workspace_bytes,
                   void* stream);
 
+/* pe_ap_compute with the 12 summaries folded by the AP launch's last class
+ * workgroup instead of a launch of their own (for at most 4096 records; larger
+ * sets fold them in a separate launch as pe_ap_compute does).  done is one
+ * device int32 word, zero before the first call; every call leaves it zero, and
+ * calls that share it must not overlap.  Replaces the same instance_compute
+ * binding (bindings.cpp:16-17, :54; instance_kernel.cu:176-209, :311-333). */
+int pe_ap_compute_done(int64_t N, const double* scores, const int32_t* classes, const int32_t* ranks,
+                       const int8_t* status, const int64_t* counts, int64_t C, double* precision,
+                       double* recall, double* summary, void* workspace, int64_t workspace_bytes,
+                       int32_t* done, void* stream);
+
 /* pe_compute and pe_ap_compute in one call (no reference counterpart; used by
  * the combined evaluator's compute()).  The first 17 arguments are
  * pe_compute's without its stream, the rest pe_ap_compute's.  For at most

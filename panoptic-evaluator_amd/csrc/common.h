@@ -109,6 +109,8 @@ __device__ __forceinline__ const __attribute__((address_space(1))) T* as_global(
 __global__ void zero_words(uint4* data, i64 count16, uint32_t* tail, i64 tail_words);
 
 int launch_zero(void* ptr, size_t bytes, hipStream_t stream);
+// Clears words [0, w_i) of each non-null p_i in one launch.
+int launch_zero3(uint32_t* p0, i64 w0, uint32_t* p1, i64 w1, uint32_t* p2, i64 w2, hipStream_t stream);
 
 // ---- work lists over packed batches of uneven images
 //

@@ -69,6 +69,35 @@ int launch_zero(void* ptr, size_t bytes, hipStream_t stream) {
   return kOk;
 }
 
+// Up to three word ranges cleared by one launch (coalesced 4-byte stores).
+struct ZeroRanges {
+  uint32_t* ptr[3];
+  i64 words[3];
+};
+
+__global__ void zero_ranges(ZeroRanges r) {
+  const i64 a = r.words[0], ab = a + r.words[1], total = ab + r.words[2];
+  const i64 stride = static_cast<i64>(gridDim.x) * blockDim.x;
+  for (i64 i = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) {
+    if (i < a)
+      r.ptr[0][i] = 0;
+    else if (i < ab)
+      r.ptr[1][i - a] = 0;
+    else
+      r.ptr[2][i - ab] = 0;
+  }
+}
+
+int launch_zero3(uint32_t* p0, i64 w0, uint32_t* p1, i64 w1, uint32_t* p2, i64 w2, hipStream_t stream) {
+  ZeroRanges r{{p0, p1, p2}, {p0 ? w0 : 0, p1 ? w1 : 0, p2 ? w2 : 0}};
+  const i64 total = r.words[0] + r.words[1] + r.words[2];
+  if (total == 0) return kOk;
+  const int blocks = static_cast<int>(std::min<i64>((total + 255) / 256, 4 * compute_units()));
+  zero_ranges<<<blocks, 256, 0, stream>>>(r);
+  PE_LAUNCH_CHECK();
+  return kOk;
+}
+
 // One workgroup: per-thread runs of images are summed, the run totals scanned
 // in LDS, then every image writes its tiles' image index.
 constexpr int kTileListThreads = 1024;

@@ -432,9 +432,9 @@ const OverlapVariant kOverlap[] = {
 int launch_overlap(OverlapArgs o, int64_t B, int64_t max_d, int64_t max_g, int64_t max_hw, int64_t total_d,
                    int64_t total_g, int64_t total_inter, hipStream_t stream) {
   int rc;
-  if (total_inter && (rc = launch_zero(o.inter, 4 * static_cast<size_t>(total_inter), stream))) return rc;
-  if (o.pa && total_d && (rc = launch_zero(o.pa, 4 * static_cast<size_t>(total_d), stream))) return rc;
-  if (o.ga && total_g && (rc = launch_zero(o.ga, 4 * static_cast<size_t>(total_g), stream))) return rc;
+  if ((rc = launch_zero3(reinterpret_cast<uint32_t*>(o.inter), total_inter, reinterpret_cast<uint32_t*>(o.pa),
+                         total_d, reinterpret_cast<uint32_t*>(o.ga), total_g, stream)))
+    return rc;
   if (B == 0 || max_hw == 0 || (max_d == 0 && max_g == 0)) return kOk;
   const OverlapVariant& v = kOverlap[tuned_variant(kTuneMaskOverlap)];
   const i64 chunks = (max_hw + v.chunk - 1) / v.chunk;
@@ -1033,9 +1033,9 @@ int resident_per_cu(int v) {
 int launch_overlap_bits(OverlapArgs o, int64_t B, int64_t max_d, int64_t max_g, int64_t max_hw, int64_t total_d,
                         int64_t total_g, int64_t total_inter, hipStream_t stream) {
   int rc;
-  if (total_inter && (rc = launch_zero(o.inter, 4 * static_cast<size_t>(total_inter), stream))) return rc;
-  if (o.pa && total_d && (rc = launch_zero(o.pa, 4 * static_cast<size_t>(total_d), stream))) return rc;
-  if (o.ga && total_g && (rc = launch_zero(o.ga, 4 * static_cast<size_t>(total_g), stream))) return rc;
+  if ((rc = launch_zero3(reinterpret_cast<uint32_t*>(o.inter), total_inter, reinterpret_cast<uint32_t*>(o.pa),
+                         total_d, reinterpret_cast<uint32_t*>(o.ga), total_g, stream)))
+    return rc;
   if (B == 0 || max_hw == 0 || (max_d == 0 && max_g == 0)) return kOk;
   const int variant = tuned_variant(kTuneMaskOverlapBits);
   const BitsVariant& v = kOverlapBits[variant];
@@ -2647,9 +2647,12 @@ __device__ __forceinline__ void ap_small_class(const ApSmallArgs& s, int c, uint
                             reinterpret_cast<int*>(codes + 101 * kApJobs), n <= kApSmallThreads);
 }
 
+__device__ __forceinline__ void summaries_if_last(const ApArgs& a);
+
 __global__ __launch_bounds__(kApSmallThreads) void ap_small(ApSmallArgs s) {
   extern __shared__ uint4 lds4[];
   ap_small_class(s, blockIdx.x, lds4);
+  if (s.ap.done) summaries_if_last(s.ap);
 }
 
 // The same, with the panoptic finaliser's rows as extra workgroups of the
@@ -3316,9 +3319,14 @@ static int ap_compute(int64_t N, const double* scores, const int32_t* classes, c
       if (match && (rc = launch_match_grouped(*match, stream))) return rc;
       PE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(ap_small), hipFuncAttributeMaxDynamicSharedMemorySize,
                                  static_cast<int>(lds)));
+      // With a completion counter the last class workgroup folds the
+      // summaries.
+      sa.ap.done = done;
+      sa.ap.summary = summary;
       ProfileScope timing("ap_accumulate", stream);
       ap_small<<<static_cast<unsigned>(C), kApSmallThreads, lds, stream>>>(sa);
       PE_LAUNCH_CHECK();
+      if (done) return kOk;
     }
     ap_summaries<<<1, 256, 0, stream>>>(static_cast<int>(C), w.partial, summary);
     PE_LAUNCH_CHECK();
@@ -3348,6 +3356,15 @@ static int ap_compute(int64_t N, const double* scores, const int32_t* classes, c
   ap_summaries<<<1, 256, 0, stream>>>(static_cast<int>(C), w.partial, summary);
   PE_LAUNCH_CHECK();
   return kOk;
+}
+
+extern "C" int pe_ap_compute_done(int64_t N, const double* scores, const int32_t* classes, const int32_t* ranks,
+                                  const int8_t* status, const int64_t* counts, int64_t C, double* precision,
+                                  double* recall, double* summary, void* workspace, int64_t workspace_bytes,
+                                  int32_t* done, void* stream_ptr) {
+  PE_REQUIRE(done != nullptr, "null completion counter");
+  return ap_compute(N, scores, classes, ranks, status, counts, C, precision, recall, summary, workspace,
+                    workspace_bytes, stream_ptr, nullptr, 0, nullptr, done);
 }
 
 extern "C" int pe_ap_compute(int64_t N, const double* scores, const int32_t* classes, const int32_t* ranks,

@@ -44,6 +44,8 @@ for _name, _sig in {
                                   _ptr, _ptr, _ptr]),
     "pe_ap_workspace_bytes": (_i64, [_i64, _i64]),
     "pe_ap_compute": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _ptr, _ptr, _ptr, _ptr, _i64, _ptr]),
+    "pe_ap_compute_done": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _ptr, _ptr, _ptr, _ptr, _i64, _ptr,
+                                  _ptr]),
     "pe_row_sums": (_int, [_ptr, _i64, _i64, _ptr, _ptr]),
     "pe_instance_checks": (_int, [_i64, _ptr, _ptr, _ptr, _i64, _ptr, _ptr, _ptr, _i64, _ptr, _i64, _ptr, _ptr,
                                   _ptr]),
@@ -287,6 +289,8 @@ class InstanceEvaluator:
         self.device = _native.require_gpu(device)
         self.validate = validate
         self._error = torch.zeros(1, dtype=torch.int32, device=self.device)
+        # Completion word of compute()'s AP launch (zero between launches).
+        self._done = torch.zeros(1, dtype=torch.int32, device=self.device)
         self.reset()
         self.device = self._counts.device  # e.g. "cuda" -> cuda:0, as instance.py:70
 
@@ -420,14 +424,18 @@ class InstanceEvaluator:
     @torch.no_grad()
     def compute(self):
         """COCO AP/AR summaries and the full precision/recall arrays."""
-        return accumulate(self._records, self._counts, self.num_classes, self.device)
+        return accumulate(self._records, self._counts, self.num_classes, self.device, done=self._done)
 
 
-def accumulate(records, counts, num_classes, device):
+def accumulate(records, counts, num_classes, device, *, done=None):
     """Sort all records by score and accumulate AP/AR on the device
-    (instance.py:205-248)."""
+    (instance.py:205-248).  With `done` (a zeroed int32 word that every call
+    leaves zero) the AP launch folds the summaries itself."""
     outputs, args, keep = ap_launch_args(records, counts, num_classes, device)
-    _native.call("pe_ap_compute", *args, _native.stream(device))
+    if done is None:
+        _native.call("pe_ap_compute", *args, _native.stream(device))
+    else:
+        _native.call("pe_ap_compute_done", *args, done.data_ptr(), _native.stream(device))
     return ap_result(*outputs)
 
 
