@@ -2386,61 +2386,57 @@ __device__ __forceinline__ void ap_class(const ApArgs& a, int c, OrderPtr order,
       }
   }
   __syncthreads();
-  // The class's precision values: (t, r, q) in memory order, q = ar * 3 + m
-  // the job within t.  Eight values per thread and step, every LDS read
-  // issued before the divisions.
+  // The class's precision values, and each job's sum over the thresholds it
+  // reaches.  Thread (t, part, q) -- 960 threads, q = ar * 3 + m the job within
+  // t -- takes thresholds r = part, part + 8, ... of job t * 12 + q: twelve
+  // neighbouring threads write one (t, r) row's twelve values, and each thread
+  // sums its own values, so the sums need no second pass over memory.  A job's
+  // eight partial sums are then added in part order: a fixed order.
+  static_assert(THREADS >= 960, "one thread per (threshold, part, job) triple");
+  constexpr int kParts = 8, kPerPart = (101 + kParts - 1) / kParts;
   double* base = a.precision + c * 12;
+  double* part_sums = reinterpret_cast<double*>(s_flag + 4096);  // kApJobs x kParts, beside sh below
   {
-    constexpr int kValues = 10 * 101 * 12, W = 8;
-    auto rr_of = [](int e) { return (min(e, kValues - 1) / 12) % 101; };
-    for (int e0 = tid; e0 < kValues; e0 += THREADS * W) {
-      Code counts[W];
-      int reached[W];
+    const bool writer = tid < 10 * kParts * 12;
+    const int q = tid % 12, part = (tid / 12) % kParts, tt = min(tid / (12 * kParts), 9);
+    const int j = tt * 12 + q;
+    const int reached = last[j];
+    Code cw[kPerPart];
 #pragma unroll
-      for (int w = 0; w < W; ++w) {
-        const int e = min(e0 + w * THREADS, kValues - 1);
-        const int q = e % 12, tr = e / 12, rr = tr % 101, tt = tr / 101;
-        const int j = tt * 12 + q;
-        reached[w] = last[j];
-        counts[w] = codes[s_kr[q / 3][rr] * kApJobs + j];
-      }
-      // Envelopes with no true positive are exactly 0 (0 / (fp + eps)); a
-      // wave whose values are all such skips the divisions.
-      bool divide = false;
+    for (int i = 0; i < kPerPart; ++i) cw[i] = codes[s_kr[q / 3][min(part + kParts * i, 100)] * kApJobs + j];
+    // Envelopes with no true positive are exactly 0 (0 / (fp + eps)); a wave
+    // whose values are all such skips the divisions.
+    bool divide = false;
 #pragma unroll
-      for (int w = 0; w < W; ++w) divide |= reached[w] >= -1 && rr_of(e0 + w * THREADS) <= reached[w] &&
-                                            (counts[w] >> kShift) != 0;
-      const bool any_divide = __ballot(divide) != 0;
+    for (int i = 0; i < kPerPart; ++i) {
+      const int rr = part + kParts * i;
+      divide |= writer && rr <= 100 && reached >= -1 && rr <= reached && (cw[i] >> kShift) != 0;
+    }
+    const bool any_divide = __ballot(divide) != 0;
+    double sum = 0.0;
+    if (writer) {
 #pragma unroll
-      for (int w = 0; w < W; ++w) {
-        const int e = e0 + w * THREADS;
-        if (e >= kValues) break;
-        const int q = e % 12, tr = e / 12, rr = tr % 101, tt = tr / 101;
-        const Code code = rr <= reached[w] ? counts[w] : Code(0);
-        double value = reached[w] < -1 ? -1.0 : 0.0;
-        if (any_divide && reached[w] >= -1) value = code_precision(code);
+      for (int i = 0; i < kPerPart; ++i) {
+        const int rr = part + kParts * i;
+        if (rr > 100) break;
+        const Code code = rr <= reached ? cw[i] : Code(0);
+        double value = reached < -1 ? -1.0 : 0.0;
+        if (any_divide && reached >= -1) value = code_precision(code);
         base[(static_cast<i64>(tt) * 101 + rr) * pstride + q] = value;
+        if (rr <= reached) sum += value;
       }
+      part_sums[j * kParts + part] = sum;
     }
   }
   __syncthreads();
-  // Each job's threshold sum (highest first, read back from the values just
-  // written), recall, and the class's share of the 12 summaries: 24 (sum,
-  // count) accumulators, each folded over the 120 jobs in job order.
+  // Recall and the class's share of the 12 summaries: 24 (sum, count)
+  // accumulators, each folded over the 120 jobs in job order.
   double part_sum = 0.0, part_n = 0.0, rv = -1.0;
   if (active) {
     if (positives) {
       double sum = 0.0;
-      const double* mine = base + static_cast<i64>(t) * 101 * pstride + ar * 3 + m;
-      constexpr int R = 8;  // loads in flight; the adds keep the order
-      for (int q0 = top; q0 >= 0; q0 -= R) {
-        double value[R];
 #pragma unroll
-        for (int u = 0; u < R; ++u) value[u] = mine[max(q0 - u, 0) * pstride];
-#pragma unroll
-        for (int u = 0; u < R; ++u)
-          if (q0 - u >= 0) sum += value[u];
-      }
+      for (int p = 0; p < kParts; ++p) sum += part_sums[job * kParts + p];
       rv = static_cast<double>(tp) / npos;
       part_sum = sum;
       part_n = 101.0;
