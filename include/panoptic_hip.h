@@ -284,11 +284,21 @@ int pe_detection_histogram(int64_t B, const uint8_t* const* masks, const int64_t
 
 /* pe_detection_histogram over bit-packed detection masks (rows as for
  * pe_pack_masks, 16-byte aligned): same arguments and outputs, no
- * `aligned`.  Replaces the same ``instance_histogram`` interface. */
+ * `aligned`.  extents (may be NULL): B device pointers, image b's to 2 int32
+ * per mask in mask order, a word range [first, end) outside which the mask's
+ * row is zero (pe_mask_extents); a workgroup then reads only the detections
+ * whose range meets its tile.  Replaces the same ``instance_histogram``
+ * interface. */
 int pe_detection_histogram_bits(int64_t B, const uint32_t* const* masks, const int64_t* det_off,
                                 const int64_t* order, const int32_t* target, const int64_t* pix_off,
                                 int64_t HW, int64_t S, int64_t max_dets, int64_t total_dets, int64_t* pairs,
-                                int pairs_zeroed, void* workspace, int64_t workspace_bytes, void* stream);
+                                int pairs_zeroed, const int32_t* const* extents, void* workspace,
+                                int64_t workspace_bytes, void* stream);
+
+/* Extents of N bit rows of row_words words (a multiple of 4, 16-byte aligned
+ * rows): extents[2n], extents[2n + 1] = the first nonzero word of row n and
+ * one past its last (0, 0 for an empty row).  No reference counterpart. */
+int pe_mask_extents(const uint32_t* bits, int64_t N, int64_t row_words, int32_t* extents, void* stream);
 
 /* COCO greedy matching of a batch of images.  Per image b, detections
  * det_off[b]..det_off[b+1] are in score order (det_cls int32, det_area f64

@@ -1085,6 +1085,9 @@ struct DetHistArgs {
   int group;                     // detections per group
   u64* pairs;                    // (sum D) x S
   int aligned;
+  // Optional, bit-packed masks only: per image, 2 int32 per mask (in mask
+  // order): a word range [first, end) outside which the mask row is zero.
+  const int32_t* const* extents;
 };
 
 // Pixel-by-pixel count of a chunk given as bits (bit j = pixel base + j),
@@ -1439,21 +1442,41 @@ __device__ __forceinline__ void detection_item_bits(const DetHistArgs& a, int it
   // first pixel is inside the image are inside the row.
   const i64 word0 = inside[0] ? lane_px >> 5 : 0;
   const auto* rows = as_global(reinterpret_cast<const uint32_t*>(a.masks[it.b]));
-  for (int r0 = 0; r0 < nd; r0 += U) {
+  // With mask extents, only the detections whose nonzero words can meet this
+  // tile are read: their indices are listed in LDS (in any order: the counts
+  // are integer adds) after the row offsets; list[group] is the count.
+  int* list = reinterpret_cast<int*>(it.row_off + a.group);
+  int m = nd;
+  if (a.extents) {
+    const int32_t* ext = a.extents[it.b];
+    const i64 tw0 = static_cast<i64>(it.tile) * (W * kDetThreads), tw1 = tw0 + W * kDetThreads;
+    if (threadIdx.x == 0) list[a.group] = 0;
+    __syncthreads();
+    for (int d = threadIdx.x; d < nd; d += kDetThreads) {
+      const i64 mask = a.order ? a.order[it.first + it.d0 + d] : it.d0 + d;
+      if (ext[2 * mask] < tw1 && ext[2 * mask + 1] > tw0) list[atomicAdd(list + a.group, 1)] = d;
+    }
+    __syncthreads();
+    m = list[a.group];
+  }
+  for (int r0 = 0; r0 < m; r0 += U) {
     Vec raw[U];
+    int det[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) det[u] = a.extents ? list[min(r0 + u, m - 1)] : min(r0 + u, nd - 1);
 #pragma unroll
     for (int u = 0; u < U; ++u)
       raw[u] = __builtin_nontemporal_load(reinterpret_cast<const __attribute__((address_space(1))) Vec*>(
-          rows + it.row_off[min(r0 + u, nd - 1)] + word0));
+          rows + it.row_off[det[u]] + word0));
 #pragma unroll
     for (int u = 0; u < U; ++u)
 #pragma unroll
       for (int w = 0; w < W; ++w) {
-        const uint32_t on = r0 + u < nd && inside[w] ? word_of<W>(raw[u], w) : 0u;
+        const uint32_t on = r0 + u < m && inside[w] ? word_of<W>(raw[u], w) : 0u;
         if constexpr (AGG) {
-          count_wave(on, groups[w], target, lane_px + 32 * w, HW, a.S, (r0 + u) * a.S, it.bins);
+          count_wave(on, groups[w], target, lane_px + 32 * w, HW, a.S, det[u] * a.S, it.bins);
         } else {
-          if (on) count_lane(on, groups[w], target, lane_px + 32 * w, HW, a.S, (r0 + u) * a.S, it.bins);
+          if (on) count_lane(on, groups[w], target, lane_px + 32 * w, HW, a.S, det[u] * a.S, it.bins);
         }
       }
   }
@@ -2846,7 +2869,8 @@ static int detection_histogram_launch(int64_t B, const uint8_t* const* masks, co
                                       const int64_t* order, const int32_t* target, const int64_t* pix_off,
                                       int64_t HW, int64_t S, int64_t max_dets, int64_t total_dets,
                                       int64_t* pairs, int pairs_zeroed, int aligned, bool bits, void* workspace,
-                                      int64_t workspace_bytes, void* stream_ptr) {
+                                      int64_t workspace_bytes, void* stream_ptr,
+                                      const int32_t* const* extents = nullptr) {
   PE_REQUIRE(B >= 0 && B < INT_MAX && HW > 0 && S > 0 && S < INT_MAX && max_dets >= 0 && total_dets >= 0,
              "invalid shapes");
   PE_REQUIRE(!pix_off || workspace_bytes >= pe_detection_histogram_workspace_bytes(B, HW),
@@ -2858,6 +2882,7 @@ static int detection_histogram_launch(int64_t B, const uint8_t* const* masks, co
   if (B == 0 || max_dets == 0) return kOk;
   DetHistArgs a{};
   a.masks = masks;
+  a.extents = bits ? extents : nullptr;
   a.det_off = reinterpret_cast<const i64*>(det_off);
   a.order = reinterpret_cast<const i64*>(order);
   a.target = target;
@@ -2904,7 +2929,10 @@ static int detection_histogram_launch(int64_t B, const uint8_t* const* masks, co
     blocks = B * t.tiles_per_image * a.dgroups;
   }
   PE_REQUIRE(blocks < INT_MAX, "too many workgroups");
-  const size_t lds = static_cast<size_t>(det_bins4(group, S)) * 16 + 8 * static_cast<size_t>(group);
+  // Bins, row offsets, and (bit-packed masks) the list of detections that
+  // meet the tile plus its count.
+  const size_t lds = static_cast<size_t>(det_bins4(group, S)) * 16 + 8 * static_cast<size_t>(group) +
+                     (bits ? 4 * static_cast<size_t>(group + 1) : 0);
   auto kernel = persist ? v.persistent : v.kernel;
   if (lds > 64 * 1024)
     PE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -2927,11 +2955,58 @@ extern "C" int pe_detection_histogram(int64_t B, const uint8_t* const* masks, co
 extern "C" int pe_detection_histogram_bits(int64_t B, const uint32_t* const* masks, const int64_t* det_off,
                                            const int64_t* order, const int32_t* target, const int64_t* pix_off,
                                            int64_t HW, int64_t S, int64_t max_dets, int64_t total_dets,
-                                           int64_t* pairs, int pairs_zeroed, void* workspace,
-                                           int64_t workspace_bytes, void* stream_ptr) {
+                                           int64_t* pairs, int pairs_zeroed, const int32_t* const* extents,
+                                           void* workspace, int64_t workspace_bytes, void* stream_ptr) {
   return detection_histogram_launch(B, reinterpret_cast<const uint8_t* const*>(masks), det_off, order, target,
                                     pix_off, HW, S, max_dets, total_dets, pairs, pairs_zeroed, 1, true, workspace,
-                                    workspace_bytes, stream_ptr);
+                                    workspace_bytes, stream_ptr, extents);
+}
+
+// [first, end) of the nonzero words of every bit row (first = end = 0 for an
+// empty row): one workgroup per row, 16-byte loads, a workgroup min / max.
+__global__ __launch_bounds__(256) void mask_extents(const uint32_t* bits, i64 row_words, int32_t* extents) {
+  const i64 n = blockIdx.x;
+  const auto* row = reinterpret_cast<const __attribute__((address_space(1))) v4i*>(as_global(bits + n * row_words));
+  int lo = INT_MAX, hi = -1;
+  for (i64 q = threadIdx.x; q < row_words / 4; q += 256) {
+    const v4i v = row[q];
+    const int w = static_cast<int>(4 * q);
+    if (v.x) { lo = min(lo, w); hi = max(hi, w); }
+    if (v.y) { lo = min(lo, w + 1); hi = max(hi, w + 1); }
+    if (v.z) { lo = min(lo, w + 2); hi = max(hi, w + 2); }
+    if (v.w) { lo = min(lo, w + 3); hi = max(hi, w + 3); }
+  }
+  for (int m = 1; m < kWave; m <<= 1) {
+    lo = min(lo, __shfl_xor(lo, m, kWave));
+    hi = max(hi, __shfl_xor(hi, m, kWave));
+  }
+  __shared__ int s_lo[4], s_hi[4];
+  const int wave = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    s_lo[wave] = lo;
+    s_hi[wave] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; ++k) {
+      lo = min(lo, s_lo[k]);
+      hi = max(hi, s_hi[k]);
+    }
+    extents[2 * n] = hi < 0 ? 0 : lo;
+    extents[2 * n + 1] = hi < 0 ? 0 : hi + 1;
+  }
+}
+
+extern "C" int pe_mask_extents(const uint32_t* bits, int64_t N, int64_t row_words, int32_t* extents,
+                               void* stream_ptr) {
+  PE_REQUIRE(N >= 0 && N < INT_MAX && row_words >= 0 && row_words % 4 == 0 && row_words < INT_MAX,
+             "invalid bit-row shapes");
+  PE_REQUIRE(N == 0 || (bits && extents), "null bit rows or extents");
+  PE_REQUIRE(reinterpret_cast<uintptr_t>(bits) % 16 == 0, "bit rows must be 16-byte aligned");
+  if (N == 0) return kOk;
+  mask_extents<<<static_cast<unsigned>(N), 256, 0, static_cast<hipStream_t>(stream_ptr)>>>(bits, row_words, extents);
+  PE_LAUNCH_CHECK();
+  return kOk;
 }
 
 extern "C" int pe_coco_match(int64_t B, int64_t C, const int64_t* det_off, const int64_t* gt_off,

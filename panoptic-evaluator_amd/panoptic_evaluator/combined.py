@@ -195,7 +195,12 @@ class SegmentationEvaluator:
                 ranked = order_and_group()
         if bits:
             # Bit rows (bool images are packed first): 1/8 of the mask bytes.
-            _native.call("pe_detection_histogram_bits", *common, *ws)
+            # With every image's extents, a tile reads only the masks that can
+            # meet it.
+            extents = None
+            if all(m.extents is not None for m in masks):
+                extents = _native.device_table([m.extents.data_ptr() for m in masks], device).data_ptr()
+            _native.call("pe_detection_histogram_bits", *common, extents, *ws)
         else:
             _native.call("pe_detection_histogram", *common, aligned, *ws)
         if ranked is None:

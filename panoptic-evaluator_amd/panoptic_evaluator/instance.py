@@ -26,7 +26,8 @@ for _name, _sig in {
     "pe_mask_overlap_bits_batch": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _i64, _i64,
                                           _i64, _i64, _ptr, _ptr, _ptr, _ptr]),
     "pe_detection_histogram_bits": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _i64, _ptr,
-                                           _int, _ptr, _i64, _ptr]),
+                                           _int, _ptr, _ptr, _i64, _ptr]),
+    "pe_mask_extents": (_int, [_ptr, _i64, _i64, _ptr, _ptr]),
     "pe_detection_histogram_workspace_bytes": (_i64, [_i64, _i64]),
     "pe_detection_histogram": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _i64, _ptr, _int,
                                       _int, _ptr, _i64, _ptr]),
@@ -69,11 +70,19 @@ class PackedMasks:
     contiguous int32 N x mask_words(H * W) device tensor in which bit p % 32
     of word p // 32 of row n is pixel p = y * W + x of mask n; bits past H * W
     are zero.  InstanceBatch accepts it wherever it accepts a bool mask
-    tensor, and the evaluators then read 1/8 of the bytes."""
+    tensor, and the evaluators then read 1/8 of the bytes.
+
+    ``extents`` (optional, from_bool computes it) is an N x 2 int32 device
+    tensor: for mask n, a word range [first, end) outside which its row is
+    zero.  The combined evaluator's detection histogram then reads only the
+    masks whose range meets each pixel tile.  A caller that builds the bits
+    itself may pass extents that bound the nonzero words (wider is allowed,
+    narrower gives wrong counts) or leave them out."""
 
     bits: torch.Tensor
     height: int
     width: int
+    extents: torch.Tensor | None = None
 
     def __post_init__(self):
         if self.height <= 0 or self.width <= 0:
@@ -84,6 +93,10 @@ class PackedMasks:
                 or bits.data_ptr() % 16):
             raise ValueError("packed mask bits must be a contiguous 16-byte aligned int32 N x mask_words(H * W) "
                              "tensor")
+        ext = self.extents
+        if ext is not None and (ext.shape != (bits.shape[0], 2) or ext.dtype != torch.int32
+                                or not ext.is_contiguous() or ext.device != bits.device):
+            raise ValueError("packed mask extents must be a contiguous int32 N x 2 tensor on the bits' device")
 
     @classmethod
     def from_bool(cls, masks: torch.Tensor) -> PackedMasks:
@@ -94,8 +107,9 @@ class PackedMasks:
         masks = masks.contiguous()
         N, H, W = masks.shape
         bits = torch.empty((N, mask_words(H * W)), dtype=torch.int32, device=device)
-        _native.call("pe_pack_masks", masks.data_ptr(), N, H * W, bits.data_ptr(), _native.stream(device))
-        return cls(bits, H, W)
+        stream = _native.stream(device)
+        _native.call("pe_pack_masks", masks.data_ptr(), N, H * W, bits.data_ptr(), stream)
+        return cls(bits, H, W, mask_extents(bits))
 
     def to_bool(self) -> torch.Tensor:
         """The N x H x W bool masks (torch ops; for inspection and tests)."""
@@ -125,6 +139,15 @@ class PackedMasks:
 
     def contiguous(self) -> PackedMasks:
         return self
+
+
+def mask_extents(bits: torch.Tensor) -> torch.Tensor:
+    """[first, end) of the nonzero words of every row of an N x words int32
+    bit tensor (0, 0 for an empty row), as an N x 2 int32 device tensor."""
+    extents = torch.empty((bits.shape[0], 2), dtype=torch.int32, device=bits.device)
+    _native.call("pe_mask_extents", bits.data_ptr(), bits.shape[0], bits.shape[1], extents.data_ptr(),
+                 _native.stream(bits.device))
+    return extents
 
 
 def pack_masks(masks: torch.Tensor) -> PackedMasks:

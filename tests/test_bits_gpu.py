@@ -190,16 +190,47 @@ def test_combined_packed_detections_all_variants(pe, tuning):
                  [dict(masks=m.numpy(), classes=c.numpy(), scores=s.numpy()) for m, c, s in zip(masks, classes, scores)],
                  pair.gt.numpy(), pair.gc.numpy(), pair.crowd.numpy())
     expected = state.compute()
-    dets = pe.InstanceBatch([pe.pack_masks(m.cuda()) for m in masks], [c.cuda() for c in classes],
-                            [s.cuda() for s in scores])
-    for variant in range(len(tuning.variants("detection_histogram_bits"))):
-        tuning.select("detection_histogram_bits", variant)
-        joint = pe.SegmentationEvaluator(133, isthing=isthing)
-        joint.update(pe.PanopticBatch(pair.pred.cuda(), pair.pc.cuda()), dets,
-                     pe.PanopticBatch(pair.gt.cuda(), pair.gc.cuda(), pair.crowd.cuda()))
-        result = to_numpy(joint.compute())
-        compare_panoptic(result, expected)
-        compare_instance(result["instance"], expected["instance"])
+    packed = [pe.pack_masks(m.cuda()) for m in masks]
+    # With the exact extents pack_masks computes, without extents, and with
+    # extents covering whole rows: the same counts.
+    forms = {
+        "extents": packed,
+        "none": [pe.PackedMasks(p.bits, p.height, p.width) for p in packed],
+        "whole rows": [pe.PackedMasks(p.bits, p.height, p.width,
+                                      torch.tensor([[0, p.bits.shape[1]]] * p.bits.shape[0], dtype=torch.int32,
+                                                   device="cuda")) for p in packed],
+    }
+    for name, form in forms.items():
+        dets = pe.InstanceBatch(form, [c.cuda() for c in classes], [s.cuda() for s in scores])
+        for variant in range(len(tuning.variants("detection_histogram_bits"))):
+            tuning.select("detection_histogram_bits", variant)
+            joint = pe.SegmentationEvaluator(133, isthing=isthing)
+            joint.update(pe.PanopticBatch(pair.pred.cuda(), pair.pc.cuda()), dets,
+                         pe.PanopticBatch(pair.gt.cuda(), pair.gc.cuda(), pair.crowd.cuda()))
+            result = to_numpy(joint.compute())
+            compare_panoptic(result, expected)
+            compare_instance(result["instance"], expected["instance"])
+
+
+@pytest.mark.parametrize("N,H,W", [(6, 37, 53), (5, 200, 333), (3, 1, 4096)])
+def test_mask_extents(pe, N, H, W):
+    """pack_masks' extents are the first nonzero word and one past the last
+    (0, 0 for an empty mask), against numpy on ellipse, empty, full and
+    one-pixel masks."""
+    rng = np.random.default_rng(N + H)
+    masks = ellipse_masks(rng, N, H, W) if H > 1 else rng.random((N, H, W)) < 0.01
+    masks[0] = False
+    masks[1] = True
+    masks[2] = False
+    masks[2, H - 1, W - 1] = True
+    packed = pe.pack_masks(cuda(masks))
+    words = packbits_words(masks)
+    expected = np.zeros((N, 2), np.int64)
+    for n in range(N):
+        nz = np.flatnonzero(words[n])
+        if nz.size:
+            expected[n] = nz[0], nz[-1] + 1
+    assert_exact(packed.extents.cpu().numpy(), expected)
 
 
 def test_combined_packed_uneven_batch(pe, tuning):
