@@ -251,14 +251,19 @@ int pe_rle_decode(int64_t N, const int64_t* table, const uint8_t* data, int64_t 
                   void* stream);
 
 /* pe_mask_overlap_batch over bit-packed masks: pm/gm are device arrays of B
- * pointers to bit rows (16-byte aligned); intersections are popcounts of
- * ANDed words.  Same arguments, outputs and replaced reference interface as
- * pe_mask_overlap_batch (no `aligned`: bit rows always are). */
+ * pointers to bit rows (16-byte aligned); the intersections are exact counts
+ * (FP4 matrix cores by default).  Same arguments, outputs and replaced
+ * reference interface as pe_mask_overlap_batch (no `aligned`: bit rows always
+ * are), plus optional mask extents pm_ext / gm_ext (device arrays of B
+ * pointers, image b's to 2 int32 per mask: a word range [first, end) outside
+ * which the row is zero, as pe_mask_extents writes; NULL for none): rows
+ * that are zero over a work item's pixel range are then skipped. */
 int pe_mask_overlap_bits_batch(int64_t B, const uint32_t* const* pm, const uint32_t* const* gm,
                                const int64_t* det_off, const int64_t* gt_off, const int64_t* inter_off,
                                const int64_t* pixels, int64_t HW, int64_t max_d, int64_t max_g, int64_t max_hw,
                                int64_t total_d, int64_t total_g, int64_t total_inter, int32_t* inter,
-                               int32_t* pa, int32_t* ga, void* stream);
+                               int32_t* pa, int32_t* ga, const int32_t* const* pm_ext,
+                               const int32_t* const* gm_ext, void* stream);
 
 /* Scratch bytes of pe_detection_histogram for a packed batch of
  * packed_pixels pixels (0 for a batch of one image size: no scratch). */

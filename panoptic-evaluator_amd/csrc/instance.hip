@@ -86,6 +86,10 @@ struct OverlapArgs {
   int items;
   int xcd_span;           // > 0: work items per XCD (grid padded to kXcds * xcd_span)
   i64 chunk_px;           // pixels per work item, for kernels whose chunk is set at launch
+  // Optional (batch form, bit-packed masks): per image, 2 int32 per mask, a
+  // word range [first, end) outside which the mask's row is zero.
+  const int32_t* const* pm_ext;
+  const int32_t* const* gm_ext;
   int32_t* inter;
   int32_t* pa;            // may be null
   int32_t* ga;            // may be null
@@ -101,6 +105,7 @@ constexpr int kXcds = 8;
 // (workgroup i takes item (i % kXcds) * xcd_span + i / kXcds) and share its
 // L2; otherwise the dispatcher spreads them over every XCD.
 struct OverlapItem {
+  int b;
   int dgroup, ggroup, chunk;
   int D, G;
   i64 HW;
@@ -121,6 +126,7 @@ __device__ __forceinline__ bool overlap_item(const OverlapArgs& o, int dets_per_
   it.ggroup = rest % o.ggroups;
   const int z = rest / o.ggroups;
   const int b = z / o.chunks;
+  it.b = b;
   it.chunk = z - b * o.chunks;
   const bool table = o.pm_table != nullptr;
   const i64 d_first = table ? o.det_off[b] : 0, g_first = table ? o.gt_off[b] : 0;
@@ -863,28 +869,54 @@ __global__ __launch_bounds__(64 * WG) void mask_overlap_fp4(OverlapArgs o) {
   const i64 w_begin = static_cast<i64>(it.chunk) * (chunk_px / 32);
   const i64 w_end = min(w_begin + chunk_px / 32, row_words);
   const i64 w_full = w_begin + (w_end - w_begin) / kBlockWords * kBlockWords;  // end of the whole blocks
-  // Tiles holding at least one mask (uniform per wave; 0 on a side with no
-  // masks, whose item still counts the other side's areas).
-  const int dtiles = max(0, min(DT, (D - d_base + M - 1) / M));
-  const int gtiles = max(0, min(GT, (G - g_base + M - 1) / M));
+  // The rows of this wave's groups that can have pixels in [w_begin, w_end):
+  // all of them, or with mask extents only those whose range meets it, listed
+  // in LDS in row order.  Tiles are built from the lists, so rows that are
+  // zero over the range are neither loaded nor multiplied.
+  __shared__ int s_rows[WG][M * DT + M * GT];
+  int* dlist = s_rows[wave];
+  int* glist = dlist + M * DT;
+  auto list_rows = [&](int* list, int first, int count, const int32_t* ext) {
+    int n = 0;
+    for (int i0 = 0; i0 < count; i0 += kWave) {
+      const int i = i0 + lane;
+      const bool keep =
+          i < count && (!ext || (ext[2 * (first + i)] < w_end && ext[2 * (first + i) + 1] > w_begin));
+      const u64 votes = __ballot(keep);
+      if (keep) list[n + __popcll(votes & ((1ULL << lane) - 1))] = first + i;
+      n += __popcll(votes);
+    }
+    return n;
+  };
+  const int nd = list_rows(dlist, d_base, max(0, min(M * DT, D - d_base)), o.pm_ext ? o.pm_ext[it.b] : nullptr);
+  const int ng = list_rows(glist, g_base, max(0, min(M * GT, G - g_base)), o.gm_ext ? o.gm_ext[it.b] : nullptr);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
   const bool count_d = it.pa != nullptr && it.ggroup == 0;
   const bool count_g = it.ga != nullptr && it.dgroup == 0 && wave == 0;
+  // Uniform: leave when the range holds no products and no areas to count.
+  if (!((nd > 0 && ng > 0) || (nd > 0 && count_d) || (ng > 0 && count_g))) return;
+  // Tiles holding at least one listed row (uniform per wave; 0 on a side
+  // with none, whose item still counts the other side's areas).
+  const int dtiles = (nd + M - 1) / M, gtiles = (ng + M - 1) / M;
   // Rows 0 .. DT-1: detection tiles, DT .. DT+GT-1: GT tiles.  Every load is
-  // unconditional, so the compiler can keep Ahead blocks in flight: a row past
-  // its side's count reads the side's last row (or, for a side with no masks,
-  // the other side's first row), and the results of such rows are never
-  // stored (the flush checks det < D and g < G; areas check `valid`).
+  // unconditional, so the compiler can keep Ahead blocks in flight: a tile
+  // slot past its list reads the list's last row (or, for an empty list, the
+  // other list's first row), and the results of such slots are never stored
+  // (the flush and the areas check `valid`).
   const __attribute__((address_space(1))) uint32_t* src[kRows];
   bool valid[kRows];
 #pragma unroll
   for (int j = 0; j < kRows; ++j) {
     const bool det = j < DT;
-    const int idx = det ? d_base + M * j + row : g_base + M * (j - DT) + row;
-    const int count = det ? D : G;
-    valid[j] = idx < count;
-    const uint8_t* base = count > 0 ? (det ? it.pm : it.gm) : (det ? it.gm : it.pm);
-    const int safe = count > 0 ? min(idx, count - 1) : 0;
-    src[j] = as_global(reinterpret_cast<const uint32_t*>(base)) + static_cast<i64>(safe) * row_words;
+    const int li = det ? M * j + row : M * (j - DT) + row;
+    const int n = det ? nd : ng;
+    valid[j] = li < n;
+    const int* own = det ? dlist : glist;
+    const int* other = det ? glist : dlist;
+    const int idx = n > 0 ? own[min(li, n - 1)] : other[0];
+    const uint8_t* base = n > 0 ? (det ? it.pm : it.gm) : (det ? it.gm : it.pm);
+    src[j] = as_global(reinterpret_cast<const uint32_t*>(base)) + static_cast<i64>(idx) * row_words;
   }
   // This lane's 4 words of the block at word wb: the piece index is clamped
   // into the row, so blocks fetched ahead past the end stay in bounds (they
@@ -959,25 +991,29 @@ __global__ __launch_bounds__(64 * WG) void mask_overlap_fp4(OverlapArgs o) {
 #pragma unroll
     for (int t = 0; t < GT; ++t) {
       if (t >= gtiles) break;
-      const int g = g_base + M * t + row;
-      if (g >= G) continue;
+      const int lg = M * t + row;
+      if (lg >= ng) continue;
+      const int g = glist[lg];
 #pragma unroll
       for (int r = 0; r < Tile::kRegs; ++r) {
-        const int det = d_base + M * d + Tile::row(lane, r);
+        const int ld = M * d + Tile::row(lane, r);
         const int v = static_cast<int>(acc[d][t][r]);
-        if (det < D && v) atomicAdd(it.inter + static_cast<i64>(det) * G + g, v);
+        if (ld < nd && v) atomicAdd(it.inter + static_cast<i64>(dlist[ld]) * G + g, v);
       }
     }
   }
-  // Areas: the kGroups lanes of a row hold disjoint words of it.
+  // Areas: the kGroups lanes of a row hold disjoint words of it (rows left
+  // out of the lists are zero over the range).
 #pragma unroll
   for (int j = 0; j < kRows; ++j) {
     const bool det = j < DT;
     int sum = area[j];
 #pragma unroll
     for (int m = M; m < kWave; m <<= 1) sum += __shfl_xor(sum, m, kWave);
-    const int idx = det ? d_base + M * j + row : g_base + M * (j - DT) + row;
-    if (q == 0 && valid[j] && sum && (det ? count_d : count_g)) atomicAdd(det ? it.pa + idx : it.ga + idx, sum);
+    if (q == 0 && valid[j] && sum && (det ? count_d : count_g)) {
+      const int idx = det ? dlist[M * j + row] : glist[M * (j - DT) + row];
+      atomicAdd(det ? it.pa + idx : it.ga + idx, sum);
+    }
   }
 }
 
@@ -2838,7 +2874,8 @@ extern "C" int pe_mask_overlap_bits_batch(int64_t B, const uint32_t* const* pm, 
                                           const int64_t* det_off, const int64_t* gt_off, const int64_t* inter_off,
                                           const int64_t* pixels, int64_t HW, int64_t max_d, int64_t max_g,
                                           int64_t max_hw, int64_t total_d, int64_t total_g, int64_t total_inter,
-                                          int32_t* inter, int32_t* pa, int32_t* ga, void* stream_ptr) {
+                                          int32_t* inter, int32_t* pa, int32_t* ga, const int32_t* const* pm_ext,
+                                          const int32_t* const* gm_ext, void* stream_ptr) {
   PE_REQUIRE(B >= 0 && B < INT_MAX && max_d >= 0 && max_d < INT_MAX && max_g >= 0 && max_g < INT_MAX &&
                  max_hw >= 0 && total_d >= 0 && total_g >= 0 && total_inter >= 0,
              "invalid mask batch shapes");
@@ -2855,6 +2892,8 @@ extern "C" int pe_mask_overlap_bits_batch(int64_t B, const uint32_t* const* pm, 
   o.inter = inter;
   o.pa = pa;
   o.ga = ga;
+  o.pm_ext = pm_ext;
+  o.gm_ext = gm_ext;
   return launch_overlap_bits(o, B, max_d, max_g, max_hw, total_d, total_g, total_inter,
                              static_cast<hipStream_t>(stream_ptr));
 }

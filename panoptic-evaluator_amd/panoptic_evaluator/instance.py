@@ -24,7 +24,7 @@ for _name, _sig in {
     "pe_mask_words": (_i64, [_i64]),
     "pe_pack_masks": (_int, [_ptr, _i64, _i64, _ptr, _ptr]),
     "pe_mask_overlap_bits_batch": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _i64, _i64,
-                                          _i64, _i64, _ptr, _ptr, _ptr, _ptr]),
+                                          _i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr]),
     "pe_detection_histogram_bits": (_int, [_i64, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _i64, _ptr,
                                            _int, _ptr, _ptr, _i64, _ptr]),
     "pe_mask_extents": (_int, [_ptr, _i64, _i64, _ptr, _ptr]),
@@ -400,14 +400,20 @@ class InstanceEvaluator:
                       max(pixels, default=0), N, total_g, total_inter, inter.data_ptr(), pa.data_ptr(),
                       ga.data_ptr())
         if any(isinstance(x.masks, PackedMasks) for x in predictions + targets):
-            # Bit rows on both sides (bool images are packed first): popcounts
-            # of ANDed words instead of the int8 MFMA contraction.
+            # Bit rows on both sides (bool images are packed first): the FP4
+            # matrix-core contraction instead of the int8 one over bytes.
             pred_bits = [as_bits(p.masks) for p in predictions]
             gt_bits = [as_bits(g.masks) for g in targets]
+            # Mask extents (when every mask set has them) let each work item
+            # skip the rows that are zero over its pixel range.
+            ext = [None, None]
+            for side, bits in enumerate((pred_bits, gt_bits)):
+                if all(m.extents is not None for m in bits):
+                    ext[side] = _native.device_table([m.extents.data_ptr() for m in bits], device).data_ptr()
             _native.call("pe_mask_overlap_bits_batch", len(predictions),
                          _native.device_table([m.data_ptr() for m in pred_bits], device).data_ptr(),
                          _native.device_table([m.data_ptr() for m in gt_bits], device).data_ptr(),
-                         *shape_args, stream)
+                         *shape_args, ext[0], ext[1], stream)
         else:
             aligned = int(all(n % 16 == 0 for n in pixels)
                           and all(x.masks.data_ptr() % 16 == 0 for x in predictions + targets))

@@ -93,7 +93,7 @@ def test_overlap_bits_variants(pe, tuning, D, G, H, W):
         _native.call("pe_mask_overlap_bits_batch", 1, table([pbits.data_ptr()]).data_ptr(),
                      table([gbits.data_ptr()]).data_ptr(), table([0, D]).data_ptr(), table([0, G]).data_ptr(),
                      table([0, D * G]).data_ptr(), None, H * W, D, G, H * W, D, G, D * G, inter.data_ptr(),
-                     pa.data_ptr(), ga.data_ptr(), _native.stream())
+                     pa.data_ptr(), ga.data_ptr(), None, None, _native.stream())
         assert_exact(inter.view(D, G).long(), pflat @ gflat.T, f"variant {variant}")
         assert_exact(pa.long(), pflat.sum(1))
         assert_exact(ga.long(), gflat.sum(1))
@@ -110,7 +110,9 @@ def ellipse_masks(rng, n, H, W):
 def test_overlap_bits_variants_full_size_uneven_batch(pe, tuning):
     """Every launch variant on two uneven full-size images in one batch (the
     FP4 forms size their pixel ranges from the batch; ranges end inside a
-    block, groups are partly filled) against an exact float32 product."""
+    block, groups are partly filled) against an exact float32 product, with
+    the masks' exact extents, and the FP4 forms also without extents and with
+    extents covering whole rows."""
     from panoptic_evaluator import _native
     rng = np.random.default_rng(11)
     shapes = [(100, 48, 800, 1088), (37, 70, 333, 501)]
@@ -126,8 +128,17 @@ def test_overlap_bits_variants_full_size_uneven_batch(pe, tuning):
     inter_off = np.cumsum([0] + [d * g for d, g in zip(dets, gts)])
     pixels = [h * w for _, _, h, w in shapes]
     table = lambda xs: _native.device_table(list(map(int, xs)), "cuda")  # noqa: E731
-    for variant in range(len(tuning.variants("mask_overlap_bits"))):
+    whole = lambda b: torch.tensor([[0, b.bits.shape[1]]] * b.bits.shape[0], dtype=torch.int32,  # noqa: E731
+                                   device="cuda")
+    extents = {"none": (None, None),
+               "exact": ([b.extents for b in pbits], [b.extents for b in gbits]),
+               "whole rows": ([whole(b) for b in pbits], [whole(b) for b in gbits])}
+    cases = [(v, "exact") for v in range(len(tuning.variants("mask_overlap_bits")))]
+    cases += [(v, e) for v in (0, 11, 18) for e in ("none", "whole rows")]
+    for variant, form in cases:
         tuning.select("mask_overlap_bits", variant)
+        pm_ext, gm_ext = (None if e is None else table([x.data_ptr() for x in e]).data_ptr()
+                          for e in extents[form])
         inter = torch.empty(int(inter_off[-1]), dtype=torch.int32, device="cuda")
         pa = torch.empty(int(det_off[-1]), dtype=torch.int32, device="cuda")
         ga = torch.empty(int(gt_off[-1]), dtype=torch.int32, device="cuda")
@@ -135,11 +146,11 @@ def test_overlap_bits_variants_full_size_uneven_batch(pe, tuning):
                      table([b.bits.data_ptr() for b in gbits]).data_ptr(), table(det_off).data_ptr(),
                      table(gt_off).data_ptr(), table(inter_off).data_ptr(), table(pixels).data_ptr(), 0,
                      max(dets), max(gts), max(pixels), int(det_off[-1]), int(gt_off[-1]), int(inter_off[-1]),
-                     inter.data_ptr(), pa.data_ptr(), ga.data_ptr(), _native.stream())
+                     inter.data_ptr(), pa.data_ptr(), ga.data_ptr(), pm_ext, gm_ext, _native.stream())
         got = inter.cpu().numpy().astype(np.int64)
         for b in range(2):
             block = got[inter_off[b]:inter_off[b + 1]].reshape(dets[b], gts[b])
-            assert_exact(block, expected[b], f"variant {variant}, image {b}")
+            assert_exact(block, expected[b], f"variant {variant}, extents {form}, image {b}")
         assert_exact(pa.cpu().numpy(), np.concatenate([p.reshape(len(p), -1).sum(1) for p in pms]))
         assert_exact(ga.cpu().numpy(), np.concatenate([g.reshape(len(g), -1).sum(1) for g in gms]))
 
