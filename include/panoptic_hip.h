@@ -245,10 +245,13 @@ int pe_pack_masks(const uint8_t* masks, int64_t N, int64_t HW, uint32_t* bits, v
  * total other than H * W) is left all-zero and ORs a bit into the device
  * word *error (1 character, 2 group length, 4 negative count, 8 total).
  * Workspace: pe_rle_workspace_bytes(N, units, columns). */
+/* extents (may be NULL): N x 2 int32, set to each mask's nonzero-word range
+ * [first, end) (INT32_MAX, 0 for an empty or malformed mask), as
+ * pe_mask_extents would compute it. */
 int64_t pe_rle_workspace_bytes(int64_t N, int64_t units, int64_t columns);
 int pe_rle_decode(int64_t N, const int64_t* table, const uint8_t* data, int64_t units, int64_t columns,
-                  int64_t max_width, uint32_t* bits, int32_t* error, void* workspace, int64_t workspace_bytes,
-                  void* stream);
+                  int64_t max_width, uint32_t* bits, int32_t* error, int32_t* extents, void* workspace,
+                  int64_t workspace_bytes, void* stream);
 
 /* pe_mask_overlap_batch over bit-packed masks: pm/gm are device arrays of B
  * pointers to bit rows (16-byte aligned); the intersections are exact counts

@@ -80,7 +80,7 @@ __device__ __forceinline__ void wave_sync_lds() {
 }
 
 __global__ __launch_bounds__(256) void rle_runs(i64 N, const RleMask* table, const uint8_t* data, int32_t* ends,
-                                                int32_t* col_run, int32_t* nruns, int32_t* error) {
+                                                int32_t* col_run, int32_t* nruns, int32_t* error, int32_t* extents) {
   __shared__ int32_t staging[4][3][kWave];  // per wave: prefix offsets, first columns, run indices
   const int lane = threadIdx.x & (kWave - 1);
   int32_t* off_lds = staging[threadIdx.x >> 6][0];
@@ -88,6 +88,12 @@ __global__ __launch_bounds__(256) void rle_runs(i64 N, const RleMask* table, con
   int32_t* idx_lds = staging[threadIdx.x >> 6][2];
   const i64 n = static_cast<i64>(blockIdx.x) * 4 + (threadIdx.x >> 6);
   if (n >= N) return;  // uniform per wave
+  // The mask's extents start empty; the raster kernel widens them to its
+  // nonzero words (a malformed mask stays empty).
+  if (extents && lane == 0) {
+    extents[2 * n] = INT_MAX;
+    extents[2 * n + 1] = 0;
+  }
   const RleMask m = table[n];
   const i64 H = m.height, W = m.width, total = H * W;
   const uint8_t* chars = data + m.data_off;
@@ -226,7 +232,8 @@ __global__ __launch_bounds__(256) void rle_runs(i64 N, const RleMask* table, con
 // produced 32 at a time: the column's word of the block from its runs, then
 // 32 ballots give 64-pixel row segments.
 __global__ __launch_bounds__(256) void rle_raster(i64 N, int groups, const RleMask* table, const int32_t* ends,
-                                                  const int32_t* col_run, const int32_t* nruns, uint32_t* bits) {
+                                                  const int32_t* col_run, const int32_t* nruns, uint32_t* bits,
+                                                  int32_t* extents) {
   const int lane = threadIdx.x & (kWave - 1);
   const i64 n = blockIdx.x / groups;
   const int band = (blockIdx.x - static_cast<int>(n) * groups) * 4 + (threadIdx.x >> 6);
@@ -247,6 +254,12 @@ __global__ __launch_bounds__(256) void rle_raster(i64 N, int groups, const RleMa
   i64 r_start = column && r > 0 ? run_end[r - 1] : 0;
   i64 r_next = column && r + 1 < runs ? run_end[r + 1] : 0;
   const bool words_aligned = (W & 31) == 0;
+  // The nonzero words this lane writes: their lowest and highest index.
+  int w_lo = INT_MAX, w_hi = -1;
+  auto note = [&](i64 w) {
+    w_lo = min(w_lo, static_cast<int>(w));
+    w_hi = max(w_hi, static_cast<int>(w));
+  };
   for (i64 y0 = 0; y0 < H; y0 += 32) {
     const int rows = static_cast<int>(min<i64>(32, H - y0));
     uint32_t word = 0;
@@ -284,16 +297,35 @@ __global__ __launch_bounds__(256) void rle_raster(i64 N, int groups, const RleMa
       const int shift = static_cast<int>(p & 31);
       if (words_aligned) {
         const uint32_t lo32 = static_cast<uint32_t>(segment), hi32 = static_cast<uint32_t>(segment >> 32);
-        if (lo32) out[w0] = lo32;
-        if (hi32) out[w0 + 1] = hi32;
+        if (lo32) {
+          out[w0] = lo32;
+          note(w0);
+        }
+        if (hi32) {
+          out[w0 + 1] = hi32;
+          note(w0 + 1);
+        }
       } else {
         const uint32_t parts[3] = {static_cast<uint32_t>(segment << shift),
                                    static_cast<uint32_t>(shift ? segment >> (32 - shift) : segment >> 32),
                                    static_cast<uint32_t>(shift ? segment >> (64 - shift) : 0)};
 #pragma unroll
         for (int k = 0; k < 3; ++k)
-          if (parts[k]) atomicOr(out + w0 + k, parts[k]);
+          if (parts[k]) {
+            atomicOr(out + w0 + k, parts[k]);
+            note(w0 + k);
+          }
       }
+    }
+  }
+  if (extents) {
+    for (int sh = 1; sh < kWave; sh <<= 1) {
+      w_lo = min(w_lo, __shfl_xor(w_lo, sh, kWave));
+      w_hi = max(w_hi, __shfl_xor(w_hi, sh, kWave));
+    }
+    if (lane == 0 && w_hi >= 0) {
+      atomicMin(extents + 2 * n, w_lo);
+      atomicMax(extents + 2 * n + 1, w_hi + 1);
     }
   }
 }
@@ -313,8 +345,8 @@ extern "C" int64_t pe_rle_workspace_bytes(int64_t N, int64_t units, int64_t colu
 }
 
 extern "C" int pe_rle_decode(int64_t N, const int64_t* table, const uint8_t* data, int64_t units, int64_t columns,
-                             int64_t max_width, uint32_t* bits, int32_t* error, void* workspace,
-                             int64_t workspace_bytes, void* stream_ptr) {
+                             int64_t max_width, uint32_t* bits, int32_t* error, int32_t* extents,
+                             void* workspace, int64_t workspace_bytes, void* stream_ptr) {
   PE_REQUIRE(N >= 0 && N < INT_MAX / 4 && units >= 0 && columns >= 0 && max_width >= 0 && max_width < INT_MAX,
              "invalid RLE shapes");
   PE_REQUIRE(workspace_bytes >= pe_rle_workspace_bytes(N, units, columns), "workspace too small");
@@ -328,14 +360,16 @@ extern "C" int pe_rle_decode(int64_t N, const int64_t* table, const uint8_t* dat
   const auto* rows = reinterpret_cast<const RleMask*>(table);
   {
     ProfileScope timing("rle_runs", stream);
-    rle_runs<<<static_cast<unsigned>((N + 3) / 4), 256, 0, stream>>>(N, rows, data, ends, col_run, nruns, error);
+    rle_runs<<<static_cast<unsigned>((N + 3) / 4), 256, 0, stream>>>(N, rows, data, ends, col_run, nruns, error,
+                                                                      extents);
     PE_LAUNCH_CHECK();
   }
   const int groups = static_cast<int>((max_width + 4 * kWave - 1) / (4 * kWave));
   if (groups == 0) return kOk;
   PE_REQUIRE(N * groups < INT_MAX, "too many RLE masks for one launch");
   ProfileScope timing("rle_raster", stream);
-  rle_raster<<<static_cast<unsigned>(N * groups), 256, 0, stream>>>(N, groups, rows, ends, col_run, nruns, bits);
+  rle_raster<<<static_cast<unsigned>(N * groups), 256, 0, stream>>>(N, groups, rows, ends, col_run, nruns, bits,
+                                                                     extents);
   PE_LAUNCH_CHECK();
   return kOk;
 }

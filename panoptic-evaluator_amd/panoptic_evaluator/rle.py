@@ -27,7 +27,7 @@ from .instance import PackedMasks, mask_words
 
 _i64, _int, _ptr = ctypes.c_int64, ctypes.c_int, ctypes.c_void_p
 _native.register("pe_rle_workspace_bytes", _i64, [_i64, _i64, _i64])
-_native.register("pe_rle_decode", _int, [_i64, _ptr, _ptr, _i64, _i64, _i64, _ptr, _ptr, _ptr, _i64, _ptr])
+_native.register("pe_rle_decode", _int, [_i64, _ptr, _ptr, _i64, _i64, _i64, _ptr, _ptr, _ptr, _ptr, _i64, _ptr])
 
 _ERRORS = ((1, "a character outside the RLE alphabet"), (2, "a count of more than seven 5-bit groups or an "
            "unterminated count"), (4, "a negative run length"), (8, "run lengths that do not sum to h * w"))
@@ -117,8 +117,13 @@ class RleBatch:
         self.upload = torch.from_numpy(host).to(device)
         size = _native.library().pe_rle_workspace_bytes(self.count, units_total, columns)
         self.workspace = torch.empty(max(size, 256), dtype=torch.uint8, device=device)
-        self.masks = [PackedMasks(self.bits[start:start + n * mask_words(h * w)].view(n, mask_words(h * w)), h, w)
-                      for n, h, w, start in shapes]
+        # Each mask's nonzero-word range, written by the decode itself.
+        self.extents = torch.zeros((max(self.count, 1), 2), dtype=torch.int32, device=device)
+        self.masks, first = [], 0
+        for n, h, w, start in shapes:
+            self.masks.append(PackedMasks(self.bits[start:start + n * mask_words(h * w)].view(n, mask_words(h * w)),
+                                          h, w, self.extents[first:first + n]))
+            first += n
 
     @torch.no_grad()
     def decode(self, *, validate: bool = False) -> list[PackedMasks]:
@@ -131,7 +136,7 @@ class RleBatch:
             self.error.zero_()
             _native.call("pe_rle_decode", self.count, self.upload.data_ptr(),
                          self.upload.data_ptr() + self.table_bytes, self.units, self.columns, self.max_width,
-                         self.bits.data_ptr(), self.error.data_ptr(), self.workspace.data_ptr(),
+                         self.bits.data_ptr(), self.error.data_ptr(), self.extents.data_ptr(), self.workspace.data_ptr(),
                          self.workspace.numel(), _native.stream(self.device))
             if validate:
                 code = int(self.error.item())

@@ -76,8 +76,19 @@ def test_decode_matches_oracle(pe, N, H, W, form):
     expected = np.stack([orle.decode(c) for c in codes])
     assert_exact(expected, masks)
     assert_exact(packed.to_bool(), expected)
-    # The same bits as packing the bool masks on the device.
-    assert_exact(packed.bits, pe.pack_masks(cuda(masks)).bits.cpu())
+    # The same bits as packing the bool masks on the device, and the same
+    # nonzero-word extents (an empty mask's range is empty in both).
+    bool_packed = pe.pack_masks(cuda(masks))
+    assert_exact(packed.bits, bool_packed.bits.cpu())
+    assert_exact(normal_extents(packed.extents), normal_extents(bool_packed.extents))
+
+
+def normal_extents(ext):
+    """Extents with every empty range as (0, 0)."""
+    ext = ext.cpu().numpy().astype(np.int64)
+    empty = ext[:, 1] <= ext[:, 0]
+    ext[empty] = 0
+    return ext
 
 
 def test_decode_batch_of_uneven_images(pe):
@@ -98,6 +109,8 @@ def test_decode_batch_of_uneven_images(pe):
     for packed, masks in zip(decoded, originals):
         assert packed.shape == masks.shape
         assert_exact(packed.to_bool(), masks)
+        if len(masks):
+            assert_exact(normal_extents(packed.extents), normal_extents(pe.pack_masks(cuda(masks)).extents))
 
 
 def test_decode_large_ellipses(pe):
