@@ -239,8 +239,9 @@ int pe_pack_masks(const uint8_t* masks, int64_t N, int64_t HW, uint32_t* bits, v
  * offset of its bit row in `bits`, offset of its W entries in the column
  * table, H, W, format (0: string, 1: int32 counts; counts 4-byte aligned)
  * and offset of its run table (`units` entries).  `units` and `columns` are
- * the totals of the units and W entries, `max_width` the largest W.  `bits`
- * must be zero-filled (only set words are written).  A malformed mask (bad
+ * the totals of the units and W entries, `max_width` the largest W.  Each
+ * mask's row (pe_mask_words(H * W) words at its offset in `bits`) is
+ * rewritten whole, so `bits` need not be cleared.  A malformed mask (bad
  * character, a count of more than seven 5-bit groups, a negative count or a
  * total other than H * W) is left all-zero and ORs a bit into the device
  * word *error (1 character, 2 group length, 4 negative count, 8 total).

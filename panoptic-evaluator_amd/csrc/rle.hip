@@ -31,12 +31,14 @@
 //    input (a character outside the alphabet, a count of more than seven
 //    groups, a negative count, a total other than H * W) marks the mask and
 //    sets an error bit; the raster kernel leaves such masks all-zero.
+//    The same launch clears every mask's bit row: one workgroup per mask
+//    after the parsing workgroups, running beside them.
 //  * rle_raster: four waves per (mask, 256 columns), lane = column.  Each
 //    lane walks its column's runs down the image 32 rows at a time and forms
 //    the column's 32-bit word of the block; a block with no set pixel in the
 //    wave's 64 columns is skipped after one ballot, otherwise 32 ballots turn
 //    the 64 column words into 32 rows of 64 pixels and lane t writes row t's
-//    segment.  Only nonzero words are written into the zero-filled rows:
+//    segment.  Only nonzero words are written into the cleared rows:
 //    with W a multiple of 32 every word has one writer (plain stores), else
 //    segments straddle words and are merged with atomicOr.
 #include <climits>
@@ -80,8 +82,24 @@ __device__ __forceinline__ void wave_sync_lds() {
 }
 
 __global__ __launch_bounds__(256) void rle_runs(i64 N, const RleMask* table, const uint8_t* data, int32_t* ends,
-                                                int32_t* col_run, int32_t* nruns, int32_t* error, int32_t* extents) {
+                                                int32_t* col_run, int32_t* nruns, int32_t* error, int32_t* extents,
+                                                uint32_t* rows_out) {
   __shared__ int32_t staging[4][3][kWave];  // per wave: prefix offsets, first columns, run indices
+  const unsigned parse_groups = static_cast<unsigned>((N + 3) / 4);
+  if (blockIdx.x >= parse_groups) {
+    // The workgroups after the parsing ones clear one bit row each; the
+    // parse is latency-bound and leaves most of the bandwidth to them.
+    const RleMask m = table[blockIdx.x - parse_groups];
+    uint32_t* out = rows_out + m.out_off;
+    const i64 words = m.height > 0 && m.width > 0 ? mask_words(m.height * m.width) : 0;
+    if ((reinterpret_cast<uintptr_t>(out) & 15) == 0) {
+      uint4* out4 = reinterpret_cast<uint4*>(out);
+      for (i64 i = threadIdx.x; i < words / 4; i += blockDim.x) out4[i] = make_uint4(0, 0, 0, 0);
+    } else {
+      for (i64 i = threadIdx.x; i < words; i += blockDim.x) out[i] = 0;
+    }
+    return;
+  }
   const int lane = threadIdx.x & (kWave - 1);
   int32_t* off_lds = staging[threadIdx.x >> 6][0];
   int32_t* x0_lds = staging[threadIdx.x >> 6][1];
@@ -187,8 +205,14 @@ __global__ __launch_bounds__(256) void rle_runs(i64 N, const RleMask* table, con
         if (end > total) bad |= kRleErrTotal;
         if (idx < m.units) run_end[idx] = static_cast<int32_t>(min<i64>(end, INT_MAX));
         if (clamped > 0 && end <= total) {
-          x0 = (lo + H - 1) / H;
-          ncols = static_cast<int>(max<i64>(0, (end - 1) / H - x0 + 1));
+          if (total <= INT_MAX) {  // uniform per mask: 32-bit division
+            const uint32_t h = static_cast<uint32_t>(H);
+            x0 = static_cast<uint32_t>(lo + H - 1) / h;
+            ncols = static_cast<int>(max<i64>(0, static_cast<i64>(static_cast<uint32_t>(end - 1) / h) - x0 + 1));
+          } else {
+            x0 = (lo + H - 1) / H;
+            ncols = static_cast<int>(max<i64>(0, (end - 1) / H - x0 + 1));
+          }
         }
       }
       // The step's column entries are written by the whole wave (a run of 0s
@@ -360,8 +384,8 @@ extern "C" int pe_rle_decode(int64_t N, const int64_t* table, const uint8_t* dat
   const auto* rows = reinterpret_cast<const RleMask*>(table);
   {
     ProfileScope timing("rle_runs", stream);
-    rle_runs<<<static_cast<unsigned>((N + 3) / 4), 256, 0, stream>>>(N, rows, data, ends, col_run, nruns, error,
-                                                                      extents);
+    rle_runs<<<static_cast<unsigned>((N + 3) / 4 + N), 256, 0, stream>>>(N, rows, data, ends, col_run, nruns,
+                                                                          error, extents, bits);
     PE_LAUNCH_CHECK();
   }
   const int groups = static_cast<int>((max_width + 4 * kWave - 1) / (4 * kWave));

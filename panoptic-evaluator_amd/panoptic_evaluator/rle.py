@@ -127,11 +127,10 @@ class RleBatch:
 
     @torch.no_grad()
     def decode(self, *, validate: bool = False) -> list[PackedMasks]:
-        """Decode every mask into ``masks`` (three launches: a fill of the bit
-        rows, then pe_rle_decode's two kernels).  ``validate`` reads back the
+        """Decode every mask into ``masks`` (pe_rle_decode's two launches,
+        which rewrite every bit row whole).  ``validate`` reads back the
         decoder's error word (a host synchronisation) and raises ValueError for
         a malformed RLE; without it a malformed mask decodes as all-zero."""
-        self.bits.zero_()
         if self.count:
             self.error.zero_()
             _native.call("pe_rle_decode", self.count, self.upload.data_ptr(),
