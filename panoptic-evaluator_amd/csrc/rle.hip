@@ -66,13 +66,53 @@ constexpr int kRleErrLongCount = 2;    // a count of more than seven groups, or 
 constexpr int kRleErrNegative = 4;     // a negative count after delta decoding
 constexpr int kRleErrTotal = 8;        // counts not summing to H * W
 
-__device__ __forceinline__ i64 wave_inclusive_scan(i64 v, int lane) {
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const i64 t = __shfl_up(v, d, kWave);
-    if (lane >= d) v += t;
-  }
+// Inclusive scans over the wave with DPP moves (a few cycles each, where a
+// cross-lane shuffle through LDS takes about a hundred): shifts by 1, 2, 4
+// and 8 lanes within each row of 16, then row 0's total broadcast into row 1
+// and row 2's into row 3 (lane 15 / 47), then the total of rows 0-1 into
+// rows 2 and 3 (lane 31).  Lanes whose source is outside their row, or whose
+// row is masked off, add 0.
+template <int Ctrl, int RowMask, bool Bound>
+__device__ __forceinline__ int dpp_shift(int v, int old = 0) {
+  return __builtin_amdgcn_update_dpp(old, v, Ctrl, RowMask, 0xF, Bound);
+}
+template <int Ctrl, int RowMask, bool Bound>
+__device__ __forceinline__ i64 dpp_shift(i64 v) {
+  const u64 u = static_cast<u64>(v);
+  const uint32_t lo = static_cast<uint32_t>(dpp_shift<Ctrl, RowMask, Bound>(static_cast<int>(u)));
+  const uint32_t hi = static_cast<uint32_t>(dpp_shift<Ctrl, RowMask, Bound>(static_cast<int>(u >> 32)));
+  return static_cast<i64>((static_cast<u64>(hi) << 32) | lo);
+}
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_scan(T v) {
+  v += dpp_shift<0x111, 0xF, true>(v);   // row_shr:1
+  v += dpp_shift<0x112, 0xF, true>(v);   // row_shr:2
+  v += dpp_shift<0x114, 0xF, true>(v);   // row_shr:4
+  v += dpp_shift<0x118, 0xF, true>(v);   // row_shr:8
+  v += dpp_shift<0x142, 0xA, false>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_shift<0x143, 0xC, false>(v);  // row_bcast:31 into rows 2 and 3
   return v;
+}
+
+// The last lane's value, read into every lane.
+__device__ __forceinline__ int last_lane(int v) { return __builtin_amdgcn_readlane(v, kWave - 1); }
+
+// The wave's minimum: the same scan with min, lanes without a source (or
+// masked rows) taking INT_MAX instead of 0.
+__device__ __forceinline__ int wave_min(int v) {
+  v = min(v, dpp_shift<0x111, 0xF, false>(v, INT_MAX));
+  v = min(v, dpp_shift<0x112, 0xF, false>(v, INT_MAX));
+  v = min(v, dpp_shift<0x114, 0xF, false>(v, INT_MAX));
+  v = min(v, dpp_shift<0x118, 0xF, false>(v, INT_MAX));
+  v = min(v, dpp_shift<0x142, 0xA, false>(v, INT_MAX));
+  v = min(v, dpp_shift<0x143, 0xC, false>(v, INT_MAX));
+  return last_lane(v);
+}
+__device__ __forceinline__ i64 last_lane(i64 v) {
+  const u64 u = static_cast<u64>(v);
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(u), kWave - 1));
+  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(u >> 32), kWave - 1));
+  return static_cast<i64>((static_cast<u64>(hi) << 32) | lo);
 }
 
 // Orders one wave's LDS writes and its later reads by other lanes.
@@ -85,6 +125,11 @@ __global__ __launch_bounds__(256) void rle_runs(i64 N, const RleMask* table, con
                                                 int32_t* col_run, int32_t* nruns, int32_t* error, int32_t* extents,
                                                 uint32_t* rows_out) {
   __shared__ int32_t staging[4][3][kWave];  // per wave: prefix offsets, first columns, run indices
+  // Characters (or counts) of 16 steps are loaded together, indices clamped
+  // into the mask's units, before the first is used: the steps are serial,
+  // the loads need not be, and most strings fit one such block.
+  constexpr int kSteps = 16;
+  __shared__ int32_t block_units[4][kSteps * kWave];
   const unsigned parse_groups = static_cast<unsigned>((N + 3) / 4);
   if (blockIdx.x >= parse_groups) {
     // The workgroups after the parsing ones clear one bit row each; the
@@ -124,18 +169,19 @@ __global__ __launch_bounds__(256) void rle_runs(i64 N, const RleMask* table, con
   int pend_k = 0;                      // how many groups it has
   i64 chain_even = 0, chain_odd = 0;   // running sums of the stored values (delta decoding)
   i64 position = 0;                    // end position of the last completed run
-  // Four steps' characters (or counts) are loaded together, indices clamped
-  // into the mask's units, before the first is used: the steps are serial,
-  // the loads need not be.
-  constexpr int kSteps = 4;
+  // Each lane keeps its units of the block in its own LDS slots (no other
+  // lane reads them), so the steps need not be unrolled.
+  int32_t* unit = block_units[threadIdx.x >> 6] + lane;
   for (i64 block = 0; block < m.units; block += kSteps * kWave) {
-    int unit[kSteps];
+    int loaded[kSteps];
 #pragma unroll
     for (int s = 0; s < kSteps; ++s) {
       const i64 u = min<i64>(block + s * kWave + lane, m.units - 1);
-      unit[s] = m.format == 0 ? static_cast<int>(chars[u]) : given[u];
+      loaded[s] = m.format == 0 ? static_cast<int>(chars[u]) : given[u];
     }
 #pragma unroll
+    for (int s = 0; s < kSteps; ++s) unit[s * kWave] = loaded[s];
+#pragma unroll 1
     for (int s = 0; s < kSteps; ++s) {
       const i64 base = block + s * kWave;
       if (base >= m.units) break;  // uniform
@@ -145,7 +191,7 @@ __global__ __launch_bounds__(256) void rle_runs(i64 N, const RleMask* table, con
       bool ends_here;
       u64 emask;
       if (m.format == 0) {
-        const int v = in ? unit[s] - 48 : 0;
+        const int v = in ? unit[s * kWave] - 48 : 0;
         if (in && (v < 0 || v > 63)) bad |= kRleErrBadChar;
         const int group = v & 0x1F;
         ends_here = in && !(v & 0x20);
@@ -178,7 +224,7 @@ __global__ __launch_bounds__(256) void rle_runs(i64 N, const RleMask* table, con
       } else {
         ends_here = in;
         emask = __ballot(in);
-        value = in ? static_cast<i64>(unit[s]) : 0;
+        value = in ? static_cast<i64>(unit[s * kWave]) : 0;
       }
       // Count index of each ending lane, and the delta decoding of the string
       // form: count i > 2 adds count i - 2, i.e. the stored values of i, i - 2,
@@ -187,16 +233,16 @@ __global__ __launch_bounds__(256) void rle_runs(i64 N, const RleMask* table, con
       i64 count = value;
       if (m.format == 0) {
         const bool chained = ends_here && idx >= 1;
-        const i64 even = wave_inclusive_scan(chained && !(idx & 1) ? value : 0, lane);
-        const i64 odd = wave_inclusive_scan(chained && (idx & 1) ? value : 0, lane);
+        const i64 even = wave_inclusive_scan<i64>(chained && !(idx & 1) ? value : 0);
+        const i64 odd = wave_inclusive_scan<i64>(chained && (idx & 1) ? value : 0);
         if (chained) count = (idx & 1) ? chain_odd + odd : chain_even + even;
-        chain_even += __shfl(even, kWave - 1, kWave);
-        chain_odd += __shfl(odd, kWave - 1, kWave);
+        chain_even += last_lane(even);
+        chain_odd += last_lane(odd);
       }
       if (ends_here && count < 0) bad |= kRleErrNegative;
       const i64 clamped = ends_here && count > 0 ? count : 0;
-      const i64 end = position + wave_inclusive_scan(clamped, lane);
-      position = __shfl(end, kWave - 1, kWave);
+      const i64 end = position + wave_inclusive_scan<i64>(clamped);
+      position = last_lane(end);
       // Columns whose first pixel lies in this lane's run: [x0, x0 + ncols).
       int ncols = 0;
       i64 x0 = 0;
@@ -218,8 +264,8 @@ __global__ __launch_bounds__(256) void rle_runs(i64 N, const RleMask* table, con
       // The step's column entries are written by the whole wave (a run of 0s
       // can span hundreds of columns): entry q belongs to the last lane whose
       // exclusive prefix of column counts is <= q.
-      const int incl = static_cast<int>(wave_inclusive_scan(ncols, lane));
-      const int entries = __shfl(incl, kWave - 1, kWave);
+      const int incl = wave_inclusive_scan<int>(ncols);
+      const int entries = last_lane(incl);
       if (entries) {
         wave_sync_lds();
         off_lds[lane] = incl - ncols;
@@ -284,7 +330,22 @@ __global__ __launch_bounds__(256) void rle_raster(i64 N, int groups, const RleMa
     w_lo = min(w_lo, static_cast<int>(w));
     w_hi = max(w_hi, static_cast<int>(w));
   };
+  const bool skippable = H <= INT_MAX;  // uniform per mask
   for (i64 y0 = 0; y0 < H; y0 += 32) {
+    if (skippable) {
+      // Jump to the first block in which some column of the wave can hold a
+      // set pixel: a column inside a run of 0s has none before that run ends
+      // (later, if the next runs are empty).  The run state stays valid: each
+      // lane's run still ends at or after the new block's first pixel.
+      i64 next = H;
+      if (column) {
+        if (r & 1) next = y0;
+        else if (r + 1 < runs) next = min<i64>(H, r_end - x * H);
+      }
+      const i64 target = static_cast<i64>(wave_min(static_cast<int>(next))) & ~static_cast<i64>(31);
+      if (target >= H) break;
+      y0 = max(y0, target);
+    }
     const int rows = static_cast<int>(min<i64>(32, H - y0));
     uint32_t word = 0;
     if (column) {
