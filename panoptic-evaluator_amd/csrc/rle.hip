@@ -97,6 +97,25 @@ __device__ __forceinline__ T wave_inclusive_scan(T v) {
 // The last lane's value, read into every lane.
 __device__ __forceinline__ int last_lane(int v) { return __builtin_amdgcn_readlane(v, kWave - 1); }
 
+// Lane `src`'s value (src uniform), read into every lane.
+__device__ __forceinline__ i64 read_lane(i64 v, int src) {
+  const u64 u = static_cast<u64>(v);
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(u), src));
+  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(u >> 32), src));
+  return static_cast<i64>((static_cast<u64>(hi) << 32) | lo);
+}
+
+// Inclusive running maximum over the wave (lanes without a source take -1).
+__device__ __forceinline__ int wave_inclusive_max(int v) {
+  v = max(v, dpp_shift<0x111, 0xF, false>(v, -1));
+  v = max(v, dpp_shift<0x112, 0xF, false>(v, -1));
+  v = max(v, dpp_shift<0x114, 0xF, false>(v, -1));
+  v = max(v, dpp_shift<0x118, 0xF, false>(v, -1));
+  v = max(v, dpp_shift<0x142, 0xA, false>(v, -1));
+  v = max(v, dpp_shift<0x143, 0xC, false>(v, -1));
+  return v;
+}
+
 // The wave's minimum: the same scan with min, lanes without a source (or
 // masked rows) taking INT_MAX instead of 0.
 __device__ __forceinline__ int wave_min(int v) {
@@ -217,8 +236,8 @@ __global__ __launch_bounds__(256) void rle_runs(i64 N, const RleMask* table, con
         // carries on): its groups so far.
         const int last = static_cast<int>(min<i64>(kWave, m.units - base)) - 1;
         const bool open = !((emask >> last) & 1ULL);
-        const i64 open_sum = __shfl(sum, last, kWave);
-        const int open_k = __shfl(k, last, kWave) + 1;
+        const i64 open_sum = read_lane(sum, last);
+        const int open_k = __builtin_amdgcn_readlane(k, last) + 1;
         pend_val = open ? open_sum : 0;
         pend_k = open ? open_k : 0;
       } else {
@@ -267,19 +286,23 @@ __global__ __launch_bounds__(256) void rle_runs(i64 N, const RleMask* table, con
       const int incl = wave_inclusive_scan<int>(ncols);
       const int entries = last_lane(incl);
       if (entries) {
+        // 64 entries at a time: the owner's lane is marked at its first
+        // entry, and a running maximum over the marks (carried from the
+        // previous 64) gives every entry's owner.
+        const int off = incl - ncols;
         wave_sync_lds();
-        off_lds[lane] = incl - ncols;
-        x0_lds[lane] = static_cast<int32_t>(x0);
+        x0_lds[lane] = static_cast<int32_t>(x0) - off;  // entry q is column x0_lds[owner] + q
         idx_lds[lane] = static_cast<int32_t>(idx);
-        wave_sync_lds();
-        for (int q = lane; q < entries; q += kWave) {
-          int lo = 0, hi = kWave - 1;
-          while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (off_lds[mid] <= q) lo = mid;
-            else hi = mid - 1;
-          }
-          cols[x0_lds[lo] + (q - off_lds[lo])] = idx_lds[lo];
+        int owner = 0;
+        for (int qb = 0; qb < entries; qb += kWave) {
+          off_lds[lane] = -1;
+          wave_sync_lds();
+          if (ncols > 0 && off >= qb && off < qb + kWave) off_lds[off - qb] = lane;
+          wave_sync_lds();
+          const int o = max(wave_inclusive_max(off_lds[lane]), owner);
+          owner = last_lane(o);
+          if (qb + lane < entries) cols[x0_lds[o] + qb + lane] = idx_lds[o];
+          wave_sync_lds();
         }
       }
       count_base += __popcll(emask);

@@ -113,6 +113,54 @@ def test_decode_batch_of_uneven_images(pe):
             assert_exact(normal_extents(packed.extents), normal_extents(pe.pack_masks(cuda(masks)).extents))
 
 
+def block_aligned_masks(H, W):
+    """Runs that start and end on the decoder's 32-row blocks, or one pixel
+    either side of them, in every column of a 64-column band, and bands with
+    no set pixel between bands with some."""
+    masks = np.zeros((6, H, W), bool)
+    for x in range(W):
+        band = x // 64
+        if band % 2:
+            continue  # an empty band
+        for k, (y0, y1) in enumerate(((32, 64), (31, 65), (33, 63), (0, 32), (H - 32, H), (64, 96))):
+            masks[k, max(y0, 0):min(y1, H), x] = (x + k) % 3 != 0
+    return masks
+
+
+def with_empty_runs(counts, rng):
+    """The same mask with pairs of zero-length runs inserted (a 0-run and a
+    1-run, or the other way round, of length 0 leave every later run's kind
+    unchanged), at the start, between runs and at the end."""
+    out = list(counts)
+    for _ in range(1 + len(out) // 4):
+        at = int(rng.integers(0, len(out) + 1))
+        out[at:at] = [0, 0]
+    return out + [0, 0]
+
+
+@pytest.mark.parametrize("form", ["string", "counts"])
+def test_decode_runs_on_block_edges_and_empty_runs(pe, form):
+    """The rasteriser skips the 32-row blocks that runs of 0s cover: runs that
+    end on a block edge, one row before or after it, empty bands and
+    zero-length runs anywhere in the list must all decode exactly."""
+    rng = np.random.default_rng(41)
+    for H, W in ((160, 200), (97, 256), (129, 70)):
+        masks = np.concatenate([block_aligned_masks(H, W), random_masks(rng, 7, H, W)])
+        codes = []
+        for m in masks:
+            counts = with_empty_runs(orle.counts_from_string(orle.encode(m)["counts"]), rng)
+            code = {"size": [H, W], "counts": counts}
+            if form == "string":
+                code = {"size": [H, W], "counts": orle.counts_to_string(counts)}
+            assert_exact(orle.decode(code), m)
+            codes.append(code)
+        packed = pe.decode_rle(codes, "cuda")
+        assert_exact(packed.to_bool(), masks)
+        bool_packed = pe.pack_masks(cuda(masks))
+        assert_exact(packed.bits, bool_packed.bits.cpu())
+        assert_exact(normal_extents(packed.extents), normal_extents(bool_packed.extents))
+
+
 def test_decode_large_ellipses(pe):
     """c1-sized images: 100 ellipse masks of 800 x 1088, and the same at an
     unaligned width."""
