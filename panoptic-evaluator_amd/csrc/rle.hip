@@ -36,11 +36,12 @@
 //  * rle_raster: four waves per (mask, 256 columns), lane = column.  Each
 //    lane walks its column's runs down the image 32 rows at a time and forms
 //    the column's 32-bit word of the block; a block with no set pixel in the
-//    wave's 64 columns is skipped after one ballot, otherwise 32 ballots turn
-//    the 64 column words into 32 rows of 64 pixels and lane t writes row t's
-//    segment.  Only nonzero words are written into the cleared rows:
-//    with W a multiple of 32 every word has one writer (plain stores), else
-//    segments straddle words and are merged with atomicOr.
+//    wave's 64 columns is skipped after one ballot, otherwise five butterfly
+//    stages transpose each half-wave's 32 column words into 32 row words and
+//    lane 32h + t writes row t's word of the half.  Only nonzero words are
+//    written into the cleared rows: with W a multiple of 32 every word has
+//    one writer (plain stores), else words straddle two and are merged with
+//    atomicOr.
 #include <climits>
 
 #include "common.h"
@@ -96,6 +97,19 @@ __device__ __forceinline__ T wave_inclusive_scan(T v) {
 
 // The last lane's value, read into every lane.
 __device__ __forceinline__ int last_lane(int v) { return __builtin_amdgcn_readlane(v, kWave - 1); }
+
+// One butterfly stage of a 32 x 32 bit transpose within each half-wave
+// (lane = column, bit = row on entry): the lane K apart (ds_swizzle's xor
+// mode) trades the off-diagonal K x K blocks, the lane with bit K clear
+// taking its partner's rows below K into its rows from K up, the other lane
+// the partner's rows from K up into its rows below K.
+template <int K>
+__device__ __forceinline__ uint32_t transpose_stage(uint32_t w, int lane) {
+  constexpr uint32_t low = K == 16 ? 0x0000FFFFu : K == 8 ? 0x00FF00FFu : K == 4 ? 0x0F0F0F0Fu
+                         : K == 2 ? 0x33333333u : 0x55555555u;
+  const uint32_t partner = static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(static_cast<int>(w), 0x1F | (K << 10)));
+  return (lane & K) ? (w & ~low) | ((partner >> K) & low) : (w & low) | ((partner & low) << K);
+}
 
 // Lane `src`'s value (src uniform), read into every lane.
 __device__ __forceinline__ i64 read_lane(i64 v, int src) {
@@ -323,7 +337,7 @@ __global__ __launch_bounds__(256) void rle_runs(i64 N, const RleMask* table, con
 
 // Four waves per (mask, 256-column group); lane = column x.  Rows are
 // produced 32 at a time: the column's word of the block from its runs, then
-// 32 ballots give 64-pixel row segments.
+// a bit transpose gives 32-pixel row words.
 __global__ __launch_bounds__(256) void rle_raster(i64 N, int groups, const RleMask* table, const int32_t* ends,
                                                   const int32_t* col_run, const int32_t* nruns, uint32_t* bits,
                                                   int32_t* extents) {
@@ -392,37 +406,33 @@ __global__ __launch_bounds__(256) void rle_raster(i64 N, int groups, const RleMa
     }
     // Most blocks of a band hold no pixel of the mask: one ballot skips them.
     if (__ballot(word != 0) == 0) continue;
-    // Row t of the block: bit j = column x0 + j.
-    u64 segment = 0;
-#pragma unroll
-    for (int t = 0; t < 32; ++t) {
-      const u64 row_bits = __ballot((word >> t) & 1u);
-      if (lane == t) segment = row_bits;
-    }
-    if (lane < rows && segment) {
-      const i64 p = (y0 + lane) * W + x0;  // first pixel of the segment
+    // Transpose each half-wave's 32 x 32 bit block (lane = column, bit =
+    // row): afterwards lane 32h + j holds row j's pixels of columns
+    // x0 + 32h ... x0 + 32h + 31 (bit c = column x0 + 32h + c).
+    uint32_t segment = word;
+    segment = transpose_stage<16>(segment, lane);
+    segment = transpose_stage<8>(segment, lane);
+    segment = transpose_stage<4>(segment, lane);
+    segment = transpose_stage<2>(segment, lane);
+    segment = transpose_stage<1>(segment, lane);
+    const int j = lane & 31;
+    if (j < rows && segment) {
+      const i64 p = (y0 + j) * W + x0 + (lane & 32);  // first pixel of the segment
       const i64 w0 = p >> 5;
       const int shift = static_cast<int>(p & 31);
       if (words_aligned) {
-        const uint32_t lo32 = static_cast<uint32_t>(segment), hi32 = static_cast<uint32_t>(segment >> 32);
-        if (lo32) {
-          out[w0] = lo32;
+        out[w0] = segment;
+        note(w0);
+      } else {
+        const uint32_t lo_part = segment << shift, hi_part = shift ? segment >> (32 - shift) : 0;
+        if (lo_part) {
+          atomicOr(out + w0, lo_part);
           note(w0);
         }
-        if (hi32) {
-          out[w0 + 1] = hi32;
+        if (hi_part) {
+          atomicOr(out + w0 + 1, hi_part);
           note(w0 + 1);
         }
-      } else {
-        const uint32_t parts[3] = {static_cast<uint32_t>(segment << shift),
-                                   static_cast<uint32_t>(shift ? segment >> (32 - shift) : segment >> 32),
-                                   static_cast<uint32_t>(shift ? segment >> (64 - shift) : 0)};
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-          if (parts[k]) {
-            atomicOr(out + w0 + k, parts[k]);
-            note(w0 + k);
-          }
       }
     }
   }
