@@ -926,7 +926,10 @@ def measure_steps(cfg, args, *, cpu_budget=None):
         from panoptic_evaluator.graphs import CapturedStep, CapturedUpdate
         log(f"[bench] {cfg.name}: capturing the batched update, and the whole step, as HIP graphs")
         captured = CapturedUpdate(cfg.evaluator, *cfg.update_args())
-        stepped = CapturedStep(cfg.evaluator, *cfg.update_args(), prepare=prepare)
+        # The RLE decode writes only the detection bit rows and extents: the
+        # step graph runs it beside the panoptic branch.
+        stepped = CapturedStep(cfg.evaluator, *cfg.update_args(), prepare=prepare,
+                               prepare_masks_only=prepare is not None)
 
         def graph_step():
             cfg.evaluator.reset()

@@ -60,6 +60,9 @@ class SegmentationEvaluator:
         # the update's grouped matching runs inside compute()'s launch instead
         # of at the end of the update.
         self._capturing_step = False
+        # Set while a step graph records its mask-making work on a branch of
+        # its own: makes the current stream wait for that branch.
+        self._masks_ready = None
         self._pending = None
         self._pairs = None
         self._pairs_clean = False
@@ -89,6 +92,11 @@ class SegmentationEvaluator:
         if self._side is None:
             self._side = _native.SideStream(self.device)
         return self._side
+
+    def _wait_for_masks(self):
+        ready, self._masks_ready = self._masks_ready, None
+        if ready is not None:
+            ready()
 
     def _flush(self):
         """Run the deferred matching of the last update, if any."""
@@ -164,6 +172,8 @@ class SegmentationEvaluator:
 
         ranked = None
         bits = any(isinstance(image.masks, PackedMasks) for image in prepared)
+        if not all(isinstance(image.masks, PackedMasks) for image in prepared):
+            self._wait_for_masks()  # bool masks are read (packed) below
         masks = [as_bits(image.masks) for image in prepared] if bits else [image.masks for image in prepared]
         pointers = _native.device_table([m.data_ptr() for m in masks], device)
         aligned = int(all((h * w) % 16 == 0 for h, w in sizes) and all(m.data_ptr() % 16 == 0 for m in masks))
@@ -193,6 +203,7 @@ class SegmentationEvaluator:
             areas = self.panoptic._update(prediction, target, return_areas=True, side=side)
             with torch.cuda.stream(side.stream):
                 ranked = order_and_group()
+        self._wait_for_masks()
         if bits:
             # Bit rows (bool images are packed first): 1/8 of the mask bytes.
             # With every image's extents, a tile reads only the masks that can

@@ -156,7 +156,7 @@ class CapturedStep:
     update's state, as after the eager calls (a later eager update() adds to
     it, compute() recomputes it).  The batch layout is fixed at capture."""
 
-    def __init__(self, evaluator, *args, prepare=None, **kwargs):
+    def __init__(self, evaluator, *args, prepare=None, prepare_masks_only=False, **kwargs):
         panoptic, instance = _parts(evaluator)
         if any(part is not None and part.validate for part in (panoptic, instance)):
             raise ValueError("capture needs validate=False: validation reads its result on the host")
@@ -164,7 +164,12 @@ class CapturedStep:
         self._instance = instance
         self._flush = getattr(evaluator, "_flush", None)
         # prepare(): device work that makes the update's inputs (such as
-        # RleBatch.decode), recorded between reset and update.
+        # RleBatch.decode), recorded between reset and update.  With
+        # prepare_masks_only (prepare writes only PackedMasks bit rows and
+        # extents, as RleBatch.decode does) a combined evaluator's step runs it
+        # on a branch of its own: the panoptic branch, score order and grouping
+        # do not wait for it, only the detection histogram does.
+        early = prepare is not None and prepare_masks_only and hasattr(evaluator, "_masks_ready")
         prepare = prepare or (lambda: None)
         device = (panoptic or instance).device
         # Warm-up step on a side stream (default priority): scratch,
@@ -184,17 +189,30 @@ class CapturedStep:
         # histogram and matches inside compute()'s launch when the three are
         # in one graph.
         stepwise = hasattr(evaluator, "_capturing_step") and not evaluator.validate
+        branch = torch.cuda.Stream(device) if early and stepwise else None
         if stepwise:
             evaluator._capturing_step = True
         try:
             with _native.pinned_tables() as tables, torch.cuda.graph(self.graph):
                 evaluator.reset()
-                prepare()
+                if branch is not None:
+                    main = torch.cuda.current_stream(device)
+                    branch.wait_stream(main)
+                    with torch.cuda.stream(branch):
+                        prepare()
+                    evaluator._masks_ready = lambda: main.wait_stream(branch)
+                else:
+                    prepare()
                 evaluator.update(*args, **kwargs)
+                join = getattr(evaluator, "_masks_ready", None)
+                if join is not None:  # the branch joins before compute() in any case
+                    evaluator._masks_ready = None
+                    join()
                 self._result = evaluator.compute()
         finally:
             if stepwise:
                 evaluator._capturing_step = False
+                evaluator._masks_ready = None
         self._tables = tables
         self._scratch = _scratch_of(evaluator)
         self._records = list(instance._records) if instance is not None else None

@@ -245,10 +245,12 @@ def test_instance_evaluator_on_rle(pe):
         assert_close(results[0][key], expected[key], what=key)
 
 
-def test_rle_batch_decodes_in_a_step_graph(pe):
+@pytest.mark.parametrize("masks_only", [False, True])
+def test_rle_batch_decodes_in_a_step_graph(pe, masks_only):
     """RleBatch keeps its strings and decoded rows on the device: decode()
-    may run inside a whole-step graph (CapturedStep's prepare), and every
-    replay equals the eager step on bool masks."""
+    may run inside a whole-step graph (CapturedStep's prepare, in line or on
+    a branch of its own beside the panoptic branch), and every replay equals
+    the eager step on bool masks."""
     import workloads
     from panoptic_evaluator.graphs import CapturedStep
     pair, masks, classes, scores = workloads.config1(device="cuda", seed=12, B=2, H=120, W=176, detections=30)
@@ -263,7 +265,8 @@ def test_rle_batch_decodes_in_a_step_graph(pe):
     eager.update(prediction, pe.InstanceBatch(masks, classes, scores), target)
     want = to_numpy(eager.compute())
     ev = pe.SegmentationEvaluator(133, isthing=isthing, validate=False)
-    step = CapturedStep(ev, prediction, pe.InstanceBatch(rle.masks, classes, scores), target, prepare=rle.decode)
+    step = CapturedStep(ev, prediction, pe.InstanceBatch(rle.masks, classes, scores), target, prepare=rle.decode,
+                        prepare_masks_only=masks_only)
     for _ in range(3):
         rle.bits.fill_(-1)  # every replay decodes again from the strings
         got = to_numpy(step.replay())
