@@ -88,10 +88,13 @@ class SegmentationEvaluator:
             self._pairs_clean = True
         return self._pairs[:N]
 
-    def _side_stream(self):
+    def _side_stream(self, which=0):
+        # Created on first use: the second only for bit-packed detections.
         if self._side is None:
-            self._side = _native.SideStream(self.device)
-        return self._side
+            self._side = [None, None]
+        if self._side[which] is None:
+            self._side[which] = _native.SideStream(self.device)
+        return self._side[which]
 
     def _wait_for_masks(self):
         ready, self._masks_ready = self._masks_ready, None
@@ -199,9 +202,16 @@ class SegmentationEvaluator:
             # shape and kept).  They are launched first, so that their
             # workgroups are placed before the histogram's fill the device,
             # and in narrow workgroups, which fit where the histogram's retire.
+            # With bit rows the histogram is shorter than the panoptic branch,
+            # so the score order and the grouping (which need nothing from
+            # that branch) take a third stream; beside the byte-mask histogram
+            # a third queue only slows the step.
             side = self._side_stream()
             areas = self.panoptic._update(prediction, target, return_areas=True, side=side)
-            with torch.cuda.stream(side.stream):
+            order_side = self._side_stream(1) if bits else side
+            if bits:
+                order_side.fork(stream)
+            with torch.cuda.stream(order_side.stream):
                 ranked = order_and_group()
         self._wait_for_masks()
         if bits:
@@ -234,6 +244,8 @@ class SegmentationEvaluator:
         # update's int64 area table and keeps only thing-class slots.
         if fork:
             side.join(stream)
+            if order_side is not side:
+                order_side.join(stream)
         else:
             areas = self.panoptic._update(prediction, target, return_areas=True)
         record = (scores, det_cls32, ranks, status)

@@ -194,14 +194,16 @@ class CapturedStep:
             evaluator._capturing_step = True
         try:
             with _native.pinned_tables() as tables, torch.cuda.graph(self.graph):
-                evaluator.reset()
                 if branch is not None:
+                    # Recorded first, so that the graph starts it first: it
+                    # leads to the detection histogram, the longest path.
                     main = torch.cuda.current_stream(device)
                     branch.wait_stream(main)
                     with torch.cuda.stream(branch):
                         prepare()
                     evaluator._masks_ready = lambda: main.wait_stream(branch)
-                else:
+                evaluator.reset()
+                if branch is None:
                     prepare()
                 evaluator.update(*args, **kwargs)
                 join = getattr(evaluator, "_masks_ready", None)
