@@ -2044,8 +2044,9 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// One wave matches class c of image b from the lists of match_groups:
-// round 1 the class's header, round 2 its lists, round 3 GT metadata,
+// One wave matches class c of image b from the lists of match_groups, in
+// three dependent load trips:
+// trip 1 the class's header, trip 2 its lists, trip 3 GT metadata,
 // intersections and detection row sums, then the chains; last it zeroes the
 // rows of the class's detections.  `lds` is this wave's match_wave_lds(S).
 __device__ void match_grouped_wave(const MatchArgs& a, int b, int c, char* lds) {
@@ -2060,7 +2061,7 @@ __device__ void match_grouped_wave(const MatchArgs& a, int b, int c, char* lds) 
   int* gts = reinterpret_cast<int*>(gt_an + S);                // S
   uint8_t* gt_cr = reinterpret_cast<uint8_t*>(gts + S);        // S
   uint32_t* used = reinterpret_cast<uint32_t*>(gt_cr + ((S + 15) & ~15));  // 40 x words
-  // Round 1: the class's header (and the no-class bucket's for class 0).
+  // Load trip 1: the class's header (and the no-class bucket's for class 0).
   const int4* hdr = a.group_hdr + static_cast<i64>(b) * (C + 1);
   const int4 h = hdr[c];
   const int4 hx = hdr[c == 0 ? C : c];
@@ -2068,7 +2069,7 @@ __device__ void match_grouped_wave(const MatchArgs& a, int b, int c, char* lds) 
   const int nd_all = h.y, ng = h.w, extra = c == 0 ? hx.y : 0;
   if (nd_all == 0 && ng == 0 && extra == 0) return;  // uniform
   const int nd = min(nd_all, 100);
-  // Round 2: the lists.
+  // Load trip 2: the lists.
   for (int r = lane; r < nd; r += kWave) {
     const int2 v = a.group_det[d0 + h.x + r];
     dets[r] = v.x;
@@ -2076,7 +2077,7 @@ __device__ void match_grouped_wave(const MatchArgs& a, int b, int c, char* lds) 
   }
   for (int q = lane; q < ng; q += kWave) gts[q] = a.group_gt[g0 + h.z + q];
   wave_sync();
-  // Round 3: GT metadata (areas, crowd flags) and positives per area range.
+  // Load trip 3: GT metadata (areas, crowd flags) and positives per area range.
   int positives[4] = {0, 0, 0, 0};
   for (int q0 = 0; q0 < ng; q0 += kWave) {
     const int q = q0 + lane;
